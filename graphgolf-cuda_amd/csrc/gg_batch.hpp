@@ -1,0 +1,42 @@
+// gg_batch.hpp -- the batch evaluator object (host side).
+#pragma once
+#include "gg_internal.hpp"
+
+struct gg_batch {
+    int N = 0, degree = 0, max_k = 0, device = 0, W = 0;
+    bool narrow = true;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    gg::DenseCtx dc;
+    void *d_tab = nullptr, *h_tab = nullptr;  // transposed offset table (dense kernels)
+    size_t tab_bytes = 0;
+    int *d_adjR = nullptr;                    // row-major neighbours (sparse kernel)
+    int *d_src = nullptr;
+    ColPart *d_parts = nullptr;               // [(max_k + 1) * W]; slot max_k = base
+    ResultRec *d_res = nullptr, *h_res = nullptr;  // [max_k + 1]
+    unsigned *d_done = nullptr;               // [max_k + 1]
+    gg_swap *d_swaps = nullptr, *h_swaps = nullptr;  // [max_k + 1]
+    int *d_ecc = nullptr;                     // [N] base per-source eccentricity
+    long long *d_base_reach = nullptr;        // [W][LEVEL_LIMIT+1]
+    // incremental (sparse) evaluation state
+    int snap_cap = 0;                         // base levels whose words are stored
+    unsigned long long *d_snap = nullptr;     // [snap_cap][W][N]
+    int cap = 0;                              // Delta list capacity per task
+    int *d_dl_key = nullptr;
+    unsigned long long *d_dl_val = nullptr;
+    int *d_dl_cnt = nullptr, *d_task_state = nullptr, *d_task_end = nullptr,
+        *d_task_flags = nullptr;
+    long long *d_dcnt = nullptr;
+    int2 *d_queue = nullptr;
+    int *d_qctl = nullptr;                    // [2] queue count, queue next
+    ColPart *d_dense_parts = nullptr;         // [max_k * W]
+    int mode_opt = 0;                         // 0 auto, 1 dense only, 2 sparse when possible
+    int sparse_ok = 0;                        // base allows the sparse path
+    int last_sparse = 0;                      // last gg_batch_eval used the sparse path
+    int use_graph = 1;
+    ResultRec base_res{};
+    int have_base = 0, have_timing = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int graph_k = 0;
+};

@@ -1,0 +1,434 @@
+// gg_dense.hpp -- dense column BFS kernels (bit-parallel, 64 sources / column).
+//
+// Algorithm (semantics of cudacode/cudaASPLconv.cu:197-266, re-designed):
+// every 64 consecutive sources form an independent "column"; one workgroup
+// runs the whole level-synchronous BFS of one column.  The column's visited
+// bitset (one uint64 per vertex) lives in LDS, so every level is an LDS
+// neighbour-OR gather and the level barrier is a workgroup barrier -- no host
+// round trip per level (the reference synchronises and copies to the host
+// after every level, cudaASPLconv.cu:232-240).
+//
+//   new[v]  = (OR_{u in adj(v)} vis[u]) & ~vis[v]     (pull over visited)
+//   vis[v] |= new[v];  total += level * popc(new[v])
+//
+// Pulling over the full visited set is exact: a bit of vis[u] that was set
+// before the previous level reached v one level earlier, so it is already in
+// vis[v].  A vertex whose word is full is skipped, and a gather stops as soon
+// as (acc | vis[v]) is full.  Bits of sources >= M are padding and start set
+// (cudaASPLconv.cu:38-44).  Per-source eccentricity = last level in which the
+// source's bit appeared anywhere in the column.
+//
+// Each (candidate, column) workgroup writes a ColPart; the last workgroup of a
+// candidate (agent-scope counter, reset by that workgroup) reduces the parts
+// into the candidate's ResultRec, so one launch yields the final answer.
+#pragma once
+#include "gg_common.hpp"
+
+// ---- topologies ---------------------------------------------------------------
+// Offset form, 1-D ring: neighbour i of v is v + tab[i*R + v % R] (mod N) with
+// offsets stored normalised to [0, N).  R = M for 'part' (cppcode/part.hpp),
+// R = N for an explicit adjacency (offset = (u - v) mod N).
+template <class IdxT>
+struct TopoRing {
+    const IdxT *tab;
+    unsigned N, R;
+    struct Ctx { unsigned v, row; };
+    __device__ __forceinline__ Ctx ctx(unsigned v) const {
+        return Ctx{v, v < R ? v : v % R};
+    }
+    __device__ __forceinline__ unsigned nbr(const Ctx &c, int i) const {
+        unsigned u = c.v + (unsigned)tab[(size_t)i * R + c.row];
+        return u >= N ? u - N : u;
+    }
+};
+
+// 2-D torus 'piece' (cppcode/piece.hpp, cudacode/cudaASPLpiece.cu:56-75):
+// v = x*Ny + y; offsets (dx,dy) normalised, packed dx | dy << 16.
+struct TopoPiece {
+    const uint32_t *tab;
+    unsigned Nx, Ny, Mx, My, R;
+    struct Ctx { unsigned x, y, row; };
+    __device__ __forceinline__ Ctx ctx(unsigned v) const {
+        unsigned x = v / Ny, y = v - x * Ny;
+        return Ctx{x, y, (x % Mx) * My + (y % My)};
+    }
+    __device__ __forceinline__ unsigned nbr(const Ctx &c, int i) const {
+        uint32_t o = tab[(size_t)i * R + c.row];
+        unsigned tx = c.x + (o & 0xFFFFu); tx = tx >= Nx ? tx - Nx : tx;
+        unsigned ty = c.y + (o >> 16);     ty = ty >= Ny ? ty - Ny : ty;
+        return tx * Ny + ty;
+    }
+};
+
+// 3-D torus 'box' (cppcode/box.hpp:29-36, cudacode/cudaASPLbox.cu:38-61):
+// v = (x*Ny + y)*Nz + z; offsets packed dx | dy << 21 | dz << 42.
+struct TopoBox {
+    const uint64_t *tab;
+    unsigned Nx, Ny, Nz, Mx, My, Mz, R;
+    struct Ctx { unsigned x, y, z, row; };
+    __device__ __forceinline__ Ctx ctx(unsigned v) const {
+        unsigned yz = Ny * Nz;
+        unsigned x = v / yz, r = v - x * yz;
+        unsigned y = r / Nz, z = r - y * Nz;
+        return Ctx{x, y, z, ((x % Mx) * My + (y % My)) * Mz + (z % Mz)};
+    }
+    __device__ __forceinline__ unsigned nbr(const Ctx &c, int i) const {
+        uint64_t o = tab[(size_t)i * R + c.row];
+        unsigned tx = c.x + (unsigned)(o & 0x1FFFFFu);         tx = tx >= Nx ? tx - Nx : tx;
+        unsigned ty = c.y + (unsigned)((o >> 21) & 0x1FFFFFu); ty = ty >= Ny ? ty - Ny : ty;
+        unsigned tz = c.z + (unsigned)((o >> 42) & 0x1FFFFFu); tz = tz >= Nz ? tz - Nz : tz;
+        return (tx * Ny + ty) * Nz + tz;
+    }
+};
+
+// ---- kernel arguments ---------------------------------------------------------
+struct DenseArgs {
+    int N, M, D;            // vertices, sources, slots per row
+    int W;                  // 64-source columns per candidate
+    int K;                  // candidates in this launch
+    int lmax;               // highest level evaluated (GG_LEVEL_LIMIT)
+    const int *src_vertex;  // [M] vertex of source s
+    const gg_swap *swaps;   // [K] slot replacements, or nullptr
+    ColPart *parts;         // [K*W]
+    ResultRec *res;         // [K]
+    unsigned *done;         // [K] finished-column counters (self-resetting)
+    int *ecc_out;           // [M] per-source eccentricity, or nullptr
+    long long *reach_col;   // [W*(lmax+1)] per-column level counts, or nullptr
+    long long *reach_out;   // [lmax+1] summed level counts, or nullptr
+    int want_wvc;           // finalize computes WVC from ecc_out
+    unsigned long long *gscratch;  // global variant: 2*N words per workgroup
+    // base snapshots: after level l (1 <= l <= snap_levels) the column's words
+    // are stored at snap[((l-1)*W + w)*N + v]
+    unsigned long long *snap;
+    int snap_levels;
+    // queue mode: tasks (k, w) = queue[i] for i < *queue_count, claimed via
+    // *queue_next; parts are written but no candidate is finalised
+    const int2 *queue;
+    const int *queue_count;
+    int *queue_next;
+};
+
+struct SwapRegs {
+    int v[4], s[4], n[4];
+};
+
+__device__ __forceinline__ void load_swap(const gg_swap *sw, int k, SwapRegs &r, bool &has) {
+    has = false;
+    if (!sw) { for (int j = 0; j < 4; j++) { r.v[j] = -1; r.s[j] = 0; r.n[j] = 0; } return; }
+    const gg_swap &s = sw[k];
+#pragma unroll
+    for (int j = 0; j < 4; j++) { r.v[j] = s.v[j]; r.s[j] = s.slot[j]; r.n[j] = s.nbr[j]; }
+    has = true;
+}
+
+// Neighbour-OR gather for vertex v with early exit once (acc | own) is full.
+template <class Topo, class Vis>
+__device__ __forceinline__ unsigned long long gather_or(const Topo &topo, Vis vis, unsigned v,
+                                                        unsigned long long own, int D,
+                                                        const SwapRegs &sw, bool has_sw) {
+    unsigned long long acc = 0;
+    typename Topo::Ctx c = topo.ctx(v);
+    bool hit = has_sw && ((int)v == sw.v[0] || (int)v == sw.v[1] || (int)v == sw.v[2] ||
+                          (int)v == sw.v[3]);
+    if (hit) {
+        for (int i = 0; i < D; i++) {
+            unsigned u = topo.nbr(c, i);
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                if (sw.v[j] == (int)v && sw.s[j] == i) u = (unsigned)sw.n[j];
+            acc |= vis[u];
+        }
+        return acc;
+    }
+    int i = 0;
+    for (; i + 4 <= D; i += 4) {
+        unsigned u0 = topo.nbr(c, i), u1 = topo.nbr(c, i + 1);
+        unsigned u2 = topo.nbr(c, i + 2), u3 = topo.nbr(c, i + 3);
+        acc |= vis[u0] | vis[u1] | vis[u2] | vis[u3];
+        if ((acc | own) == ~0ull) return acc;
+    }
+    for (; i < D; i++) acc |= vis[topo.nbr(c, i)];
+    return acc;
+}
+
+// Padding bits of column w: sources 64w+j >= M are treated as already reached.
+__device__ __forceinline__ unsigned long long pad_mask(int w, int M) {
+    int base = w * 64;
+    if (base + 64 <= M) return 0ull;
+    if (base >= M) return ~0ull;
+    return ~0ull << (M - base);
+}
+
+// Last workgroup of candidate k reduces the column parts.  Called by all
+// threads of the block after thread 0 holds the finished part in `p`.
+template <int BS>
+__device__ void finish_column(const DenseArgs &a, int k, int w, const ColPart &p) {
+    __shared__ int s_last;
+    __shared__ long long s_tot[BS / 64], s_rch[BS / 64];
+    __shared__ int s_diam[BS / 64], s_lev[BS / 64], s_fl[BS / 64];
+    __shared__ int s_cls[32];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    if (tid == 0) a.parts[(size_t)k * a.W + w] = p;
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) {
+        unsigned prev = atomicAdd(&a.done[k], 1u);
+        s_last = (prev == (unsigned)a.W - 1);
+    }
+    if (tid < 32) s_cls[tid] = -1;
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    long long tot = 0, rch = 0;
+    int diam = 0, lev = 0, fl = 0;
+    for (int i = tid; i < a.W; i += BS) {
+        const ColPart *q = a.parts + (size_t)k * a.W + i;
+        tot += ld_agent(&q->total);
+        rch += ld_agent(&q->reached);
+        int e = ld_agent(&q->ecc), l = ld_agent(&q->levels);
+        diam = e > diam ? e : diam;
+        lev = l > lev ? l : lev;
+        fl |= ld_agent(&q->flags);
+    }
+    tot = wave_sum64(tot);
+    rch = wave_sum64(rch);
+    diam = wave_max32(diam);
+    lev = wave_max32(lev);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) fl |= __shfl_xor(fl, o, 64);
+    if (lane == 0) { s_tot[wid] = tot; s_rch[wid] = rch; s_diam[wid] = diam; s_lev[wid] = lev; s_fl[wid] = fl; }
+    // WVC: classes j in [0,32) merge sources j, j+32, ... (cudaASPLconv.cu:258-285)
+    if (a.want_wvc && a.ecc_out) {
+        for (int s = tid; s < a.M; s += BS) atomicMax(&s_cls[s & 31], ld_agent(&a.ecc_out[s]));
+    }
+    __syncthreads();
+    if (tid == 0) {
+        for (int i = 1; i < BS / 64; i++) {
+            s_tot[0] += s_tot[i]; s_rch[0] += s_rch[i];
+            s_diam[0] = s_diam[i] > s_diam[0] ? s_diam[i] : s_diam[0];
+            s_lev[0] = s_lev[i] > s_lev[0] ? s_lev[i] : s_lev[0];
+            s_fl[0] |= s_fl[i];
+        }
+        ResultRec r;
+        r.total = s_tot[0];
+        r.reached = s_rch[0];
+        r.diameter = s_diam[0];
+        r.levels = s_lev[0];
+        r.flags = s_fl[0];
+        int wvc = 0;
+        if (a.want_wvc && a.ecc_out)
+            for (int j = 0; j < 32; j++) wvc += (s_cls[j] == r.diameter) ? 1 : 0;
+        r.wvc = wvc;
+        a.res[k] = r;
+        a.done[k] = 0u;
+    }
+    if (a.reach_out && a.reach_col) {
+        int L = a.lmax + 1;
+        for (int l = tid; l < L; l += BS) {
+            long long s = 0;
+            for (int i = 0; i < a.W; i++) {
+                const ColPart *q = a.parts + (size_t)k * a.W + i;
+                if (l <= ld_agent(&q->levels)) s += ld_agent(&a.reach_col[(size_t)i * L + l]);
+            }
+            a.reach_out[l] = s;
+        }
+    }
+}
+
+// ---- LDS-resident column BFS ---------------------------------------------------
+// One workgroup runs one (candidate, column) task with vis[N] in dynamic LDS.
+// Thread t owns vertices t, t+BS, ..., so new words stay in registers across
+// the level barrier (no second LDS buffer).  Returns the column part in
+// thread 0.
+template <class Topo, int BS, int VPT>
+__device__ __forceinline__ ColPart dense_column(const Topo &topo, const DenseArgs &a, int k, int w,
+                                                unsigned long long *vis) {
+    __shared__ unsigned long long s_or[2];
+    __shared__ unsigned s_cnt[2];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const unsigned N = (unsigned)a.N;
+    const int D = a.D;
+    SwapRegs sw;
+    bool has_sw;
+    load_swap(a.swaps, k, sw, has_sw);
+
+    const unsigned long long pad = pad_mask(w, a.M);
+    for (unsigned v = tid; v < N; v += BS) vis[v] = pad;
+    if (tid < 2) { s_or[tid] = 0; s_cnt[tid] = 0; }
+    __syncthreads();
+    const int nvalid = (a.M - w * 64) < 64 ? (a.M - w * 64) : 64;
+    if (tid < nvalid) {
+        unsigned sv = (unsigned)a.src_vertex[w * 64 + tid];
+        atomicOr(&vis[sv], 1ull << tid);
+    }
+    __syncthreads();
+
+    long long total = 0, reached = nvalid;
+    int my_ecc = 0, last_new = 0, level = 0, flags = 0;
+    long long *reach = a.reach_col ? a.reach_col + (size_t)w * (a.lmax + 1) : nullptr;
+    if (reach && tid == 0) reach[0] = nvalid;
+    for (level = 1;; level++) {
+        unsigned long long nw[VPT];
+#pragma unroll
+        for (int j = 0; j < VPT; j++) {
+            unsigned v = tid + j * BS;
+            nw[j] = 0;
+            if (v < N) {
+                unsigned long long own = vis[v];
+                if (~own) nw[j] = gather_or(topo, vis, v, own, D, sw, has_sw) & ~own;
+            }
+        }
+        __syncthreads();
+        unsigned long long lor = 0;
+        unsigned cnt = 0;
+        unsigned long long *snap =
+            (a.snap && level <= a.snap_levels)
+                ? a.snap + ((size_t)(level - 1) * a.W + w) * N : nullptr;
+#pragma unroll
+        for (int j = 0; j < VPT; j++) {
+            unsigned v = tid + j * BS;
+            if (nw[j]) {
+                vis[v] |= nw[j];  // only this thread writes vertex v
+                lor |= nw[j];
+                cnt += __popcll(nw[j]);
+            }
+            if (snap && v < N) snap[v] = vis[v];
+        }
+        lor = wave_or64(lor);
+        cnt = wave_sum32(cnt);
+        if (lane == 0 && cnt) {
+            atomicOr(&s_or[level & 1], lor);
+            atomicAdd(&s_cnt[level & 1], cnt);
+        }
+        __syncthreads();
+        const unsigned long long blk = s_or[level & 1];
+        if (tid == 0) {
+            unsigned c = s_cnt[level & 1];
+            total += (long long)level * c;
+            reached += c;
+            if (reach) reach[level] = c;
+            s_or[(level + 1) & 1] = 0;
+            s_cnt[(level + 1) & 1] = 0;
+        }
+        if (tid < 64 && ((blk >> tid) & 1ull)) my_ecc = level;
+        if (blk == 0) break;
+        last_new = level;
+        if (level >= a.lmax) { flags |= GG_FLAG_LIMIT; break; }
+    }
+    if (a.ecc_out && tid < nvalid) a.ecc_out[w * 64 + tid] = my_ecc;
+    ColPart p;
+    p.total = total;
+    p.reached = reached;
+    p.ecc = last_new;
+    p.levels = level;
+    p.flags = flags;
+    p.pad = 0;
+    return p;
+}
+
+// One workgroup per (candidate, column); the last one of a candidate finalises it.
+template <class Topo, int BS, int VPT>
+__global__ void __launch_bounds__(BS) k_dense_lds(Topo topo, DenseArgs a) {
+    extern __shared__ unsigned long long vis[];
+    const int t = blockIdx.x;
+    const int k = t / a.W, w = t - k * a.W;
+    ColPart p = dense_column<Topo, BS, VPT>(topo, a, k, w, vis);
+    finish_column<BS>(a, k, w, p);
+}
+
+// Persistent workgroups draining a device-side task queue; parts only.
+template <class Topo, int BS, int VPT>
+__global__ void __launch_bounds__(BS) k_dense_lds_queue(Topo topo, DenseArgs a) {
+    extern __shared__ unsigned long long vis[];
+    __shared__ int s_task;
+    for (;;) {
+        if (threadIdx.x == 0) s_task = atomicAdd(a.queue_next, 1);
+        __syncthreads();
+        const int t = s_task;
+        if (t >= *a.queue_count) break;
+        const int2 kw = a.queue[t];
+        ColPart p = dense_column<Topo, BS, VPT>(topo, a, kw.x, kw.y, vis);
+        if (threadIdx.x == 0) a.parts[(size_t)kw.x * a.W + kw.y] = p;
+        __syncthreads();
+    }
+}
+
+// ---- global-memory column BFS (N too large for LDS) -------------------------------
+// Persistent grid; workgroup b owns scratch cur/nxt of N words each.
+template <class Topo>
+__global__ void __launch_bounds__(1024) k_dense_global(Topo topo, DenseArgs a, int tasks) {
+    constexpr int BS = 1024;
+    // three buffers: with one barrier per level, the buffer reset at the end
+    // of level l is the one used at level l+2 (read by everyone before the
+    // barrier of level l), never the one of level l+1.
+    __shared__ unsigned long long s_or[3];
+    __shared__ unsigned s_cnt[3];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const unsigned N = (unsigned)a.N;
+    const int D = a.D;
+    unsigned long long *bufA = a.gscratch + (size_t)blockIdx.x * 2 * N;
+    unsigned long long *bufB = bufA + N;
+    for (int t = blockIdx.x; t < tasks; t += gridDim.x) {
+        const int k = t / a.W, w = t - k * a.W;
+        SwapRegs sw;
+        bool has_sw;
+        load_swap(a.swaps, k, sw, has_sw);
+        unsigned long long *cur = bufA, *nxt = bufB;
+        const unsigned long long pad = pad_mask(w, a.M);
+        for (unsigned v = tid; v < N; v += BS) cur[v] = pad;
+        if (tid < 3) { s_or[tid] = 0; s_cnt[tid] = 0; }
+        __syncthreads();
+        const int nvalid = (a.M - w * 64) < 64 ? (a.M - w * 64) : 64;
+        if (tid < nvalid) {
+            unsigned sv = (unsigned)a.src_vertex[w * 64 + tid];
+            cur[sv] |= 1ull << tid;
+        }
+        __syncthreads();
+        long long total = 0, reached = nvalid;
+        int my_ecc = 0, last_new = 0, level = 0, flags = 0;
+        long long *reach = a.reach_col ? a.reach_col + (size_t)w * (a.lmax + 1) : nullptr;
+        if (reach && tid == 0) reach[0] = nvalid;
+        for (level = 1;; level++) {
+            unsigned long long lor = 0;
+            unsigned cnt = 0;
+            for (unsigned v = tid; v < N; v += BS) {
+                unsigned long long own = cur[v], n = 0;
+                if (~own) n = gather_or(topo, cur, v, own, D, sw, has_sw) & ~own;
+                nxt[v] = own | n;
+                lor |= n;
+                cnt += __popcll(n);
+            }
+            lor = wave_or64(lor);
+            cnt = wave_sum32(cnt);
+            if (lane == 0 && cnt) {
+                atomicOr(&s_or[level % 3], lor);
+                atomicAdd(&s_cnt[level % 3], cnt);
+            }
+            __syncthreads();
+            const unsigned long long blk = s_or[level % 3];
+            if (tid == 0) {
+                unsigned c = s_cnt[level % 3];
+                total += (long long)level * c;
+                reached += c;
+                if (reach) reach[level] = c;
+                s_or[(level + 2) % 3] = 0;
+                s_cnt[(level + 2) % 3] = 0;
+            }
+            if (tid < 64 && ((blk >> tid) & 1ull)) my_ecc = level;
+            unsigned long long *tmp = cur; cur = nxt; nxt = tmp;
+            if (blk == 0) break;
+            last_new = level;
+            if (level >= a.lmax) { flags |= GG_FLAG_LIMIT; break; }
+        }
+        if (a.ecc_out && tid < nvalid) a.ecc_out[w * 64 + tid] = my_ecc;
+        ColPart p;
+        p.total = total;
+        p.reached = reached;
+        p.ecc = last_new;
+        p.levels = level;
+        p.flags = flags;
+        p.pad = 0;
+        finish_column<BS>(a, k, w, p);
+        __syncthreads();
+    }
+}

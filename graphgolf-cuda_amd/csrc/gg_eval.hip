@@ -1,0 +1,660 @@
+// gg_eval.hip -- single-graph exact evaluators behind the C ABI.
+//
+// Replaces cudaASPLconv / cudaASPLpiece / cudaASPLbox / cudaASPLgraph
+// (cudacode/*.cu).  One call = pack the table on the host, one async H2D, one
+// kernel launch (all levels, all columns, final reduction), one 48-byte D2H.
+#include <cstring>
+#include <cstdlib>
+#include <mutex>
+#include <string>
+#include <vector>
+#include "gg_common.hpp"
+#include "gg_dense.hpp"
+#include "gg_internal.hpp"
+
+// ---- error bookkeeping -----------------------------------------------------------
+static thread_local std::string g_last_hip_error;
+
+void gg_set_last_hip_error(hipError_t e, const char *what, const char *file, int line) {
+    char buf[512];
+    snprintf(buf, sizeof buf, "HIP error %d (%s) in %s at %s:%d", (int)e, hipGetErrorString(e),
+             what, file, line);
+    g_last_hip_error = buf;
+}
+
+extern "C" const char *gg_strerror(int status) {
+    switch (status) {
+        case GG_OK: return "ok";
+        case GG_UNCONNECTED: return "graph is unconnected";
+        case GG_DIAMETER_LIMIT: return "diameter reaches the level limit (99)";
+        case GG_BAD_ARG: return "bad argument";
+        case GG_HIP_ERROR: return g_last_hip_error.empty() ? "HIP error" : g_last_hip_error.c_str();
+        case GG_OOM: return "out of device memory";
+        case GG_NO_DEVICE: return "no HIP device";
+        case GG_UNSUPPORTED: return "unsupported shape";
+        default: return "unknown status";
+    }
+}
+
+extern "C" int gg_device_count(int *count) {
+    if (!count) return GG_BAD_ARG;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) { *count = 0; return GG_NO_DEVICE; }
+    *count = n;
+    return GG_OK;
+}
+
+extern "C" double gg_aspl(const gg_result *r, int N) {
+    if (!r || N < 2) return 0.0;
+    return double(r->total_scaled) / (int64_t(N) * (N - 1));
+}
+
+// ---- kernel dispatch ------------------------------------------------------------------
+namespace gg {
+
+template <class Topo, int BS, int VPT>
+static hipError_t launch_lds_inst(const Topo &topo, const DenseArgs &a, int grid, size_t lds,
+                                  hipStream_t st) {
+    static int configured_bytes = -1;
+    auto fn = &k_dense_lds<Topo, BS, VPT>;
+    if ((int)lds > configured_bytes) {
+        hipError_t e = hipFuncSetAttribute((const void *)fn,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        configured_bytes = (int)lds;
+    }
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(BS), lds, st, topo, a);
+    return hipGetLastError();
+}
+
+template <class Topo>
+static hipError_t launch_lds(const Topo &topo, const DenseArgs &a, int grid, size_t lds, int bs,
+                             int vpt, hipStream_t st) {
+#define GG_CASE(B, V) \
+    if (bs == B && vpt == V) return launch_lds_inst<Topo, B, V>(topo, a, grid, lds, st);
+    GG_CASE(256, 1) GG_CASE(256, 2) GG_CASE(256, 4) GG_CASE(256, 8) GG_CASE(256, 16)
+    GG_CASE(1024, 1) GG_CASE(1024, 2) GG_CASE(1024, 4) GG_CASE(1024, 8) GG_CASE(1024, 16)
+    GG_CASE(1024, 20)
+#undef GG_CASE
+    return hipErrorInvalidValue;
+}
+
+template <class Topo, int BS, int VPT>
+static hipError_t launch_lds_queue_inst(const Topo &topo, const DenseArgs &a, int grid, size_t lds,
+                                        hipStream_t st) {
+    static int configured_bytes = -1;
+    auto fn = &k_dense_lds_queue<Topo, BS, VPT>;
+    if ((int)lds > configured_bytes) {
+        hipError_t e = hipFuncSetAttribute((const void *)fn,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        configured_bytes = (int)lds;
+    }
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(BS), lds, st, topo, a);
+    return hipGetLastError();
+}
+
+template <class Topo>
+static hipError_t launch_lds_queue(const Topo &topo, const DenseArgs &a, int grid, size_t lds,
+                                   int vpt, hipStream_t st) {
+#define GG_CASE(V) \
+    if (vpt == V) return launch_lds_queue_inst<Topo, 1024, V>(topo, a, grid, lds, st);
+    GG_CASE(1) GG_CASE(2) GG_CASE(4) GG_CASE(8) GG_CASE(16) GG_CASE(20)
+#undef GG_CASE
+    return hipErrorInvalidValue;
+}
+
+template <class Topo>
+static hipError_t launch_global(const Topo &topo, const DenseArgs &a, int grid, int tasks,
+                                hipStream_t st) {
+    hipLaunchKernelGGL(k_dense_global<Topo>, dim3(grid), dim3(1024), 0, st, topo, a, tasks);
+    return hipGetLastError();
+}
+
+// Smallest supported VPT >= need for block size bs (0 if none).
+int pick_vpt(int bs, int need) {
+    static const int v256[] = {1, 2, 4, 8, 16};
+    static const int v1024[] = {1, 2, 4, 8, 16, 20};
+    const int *v = bs == 256 ? v256 : v1024;
+    int n = bs == 256 ? 5 : 6;
+    for (int i = 0; i < n; i++)
+        if (v[i] >= need) return v[i];
+    return 0;
+}
+
+int device_lds_limit(int device) {
+    int lds = 0;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess)
+        lds = 65536;
+    return lds;
+}
+
+int device_cu_count(int device) {
+    int cu = 0;
+    if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+        cu = 256;
+    return cu;
+}
+
+// Static LDS used by the dense kernels besides the bitset (reduction scratch).
+static const int kStaticLdsReserve = 2048;
+
+// Chooses the kernel variant and launches one dense evaluation of K candidates.
+int run_dense(DenseCtx &c, const DenseArgs &a, hipStream_t st) {
+    const int tasks = a.K * a.W;
+    const size_t lds = (size_t)a.N * 8;
+    int lds_cap = c.lds_limit_opt > 0 ? c.lds_limit_opt : c.lds_device - kStaticLdsReserve;
+    int bs = c.block_opt;
+    if (bs != 256 && bs != 1024) bs = (a.N <= 4096 && tasks >= 4 * c.cu_count) ? 256 : 1024;
+    int vpt = pick_vpt(bs, (a.N + bs - 1) / bs);
+    if (vpt == 0 && bs == 256) { bs = 1024; vpt = pick_vpt(bs, (a.N + bs - 1) / bs); }
+    hipError_t e;
+    if ((int)lds <= lds_cap && vpt > 0) {
+        c.last_variant = bs == 256 ? 1 : 2;
+        switch (c.topo_kind) {
+            case TOPO_RING16: e = launch_lds(c.ring16, a, tasks, lds, bs, vpt, st); break;
+            case TOPO_RING32: e = launch_lds(c.ring32, a, tasks, lds, bs, vpt, st); break;
+            case TOPO_PIECE: e = launch_lds(c.piece, a, tasks, lds, bs, vpt, st); break;
+            default: e = launch_lds(c.box, a, tasks, lds, bs, vpt, st); break;
+        }
+    } else {
+        // global-memory variant: persistent grid, 2*N words of scratch per workgroup
+        int grid = tasks < 2 * c.cu_count ? tasks : 2 * c.cu_count;
+        size_t need = (size_t)grid * 2 * a.N * sizeof(unsigned long long);
+        if (need > c.gscratch_bytes) {
+            if (c.gscratch) { (void)hipFree(c.gscratch); c.gscratch = nullptr; c.gscratch_bytes = 0; }
+            if (hipMalloc((void **)&c.gscratch, need) != hipSuccess) return GG_OOM;
+            c.gscratch_bytes = need;
+        }
+        DenseArgs g = a;
+        g.gscratch = c.gscratch;
+        c.last_variant = 3;
+        switch (c.topo_kind) {
+            case TOPO_RING16: e = launch_global(c.ring16, g, grid, tasks, st); break;
+            case TOPO_RING32: e = launch_global(c.ring32, g, grid, tasks, st); break;
+            case TOPO_PIECE: e = launch_global(c.piece, g, grid, tasks, st); break;
+            default: e = launch_global(c.box, g, grid, tasks, st); break;
+        }
+    }
+    if (e != hipSuccess) {
+        gg_set_last_hip_error(e, "dense BFS launch", __FILE__, __LINE__);
+        return GG_HIP_ERROR;
+    }
+    return GG_OK;
+}
+
+// Dense column BFS of the tasks in a device-side queue (ring topologies only;
+// 1024-thread workgroups, bitset in LDS).  Returns GG_UNSUPPORTED when the
+// column does not fit in LDS.
+int run_dense_queue(DenseCtx &c, const DenseArgs &a, int grid, hipStream_t st) {
+    const size_t lds = (size_t)a.N * 8;
+    int lds_cap = c.lds_limit_opt > 0 ? c.lds_limit_opt : c.lds_device - kStaticLdsReserve;
+    int vpt = pick_vpt(1024, (a.N + 1023) / 1024);
+    if ((int)lds > lds_cap || vpt == 0) return GG_UNSUPPORTED;
+    hipError_t e;
+    switch (c.topo_kind) {
+        case TOPO_RING16: e = launch_lds_queue(c.ring16, a, grid, lds, vpt, st); break;
+        case TOPO_RING32: e = launch_lds_queue(c.ring32, a, grid, lds, vpt, st); break;
+        default: return GG_UNSUPPORTED;
+    }
+    if (e != hipSuccess) {
+        gg_set_last_hip_error(e, "dense queue launch", __FILE__, __LINE__);
+        return GG_HIP_ERROR;
+    }
+    return GG_OK;
+}
+
+int dense_fits_lds(const DenseCtx &c, int N) {
+    int lds_cap = c.lds_limit_opt > 0 ? c.lds_limit_opt : c.lds_device - kStaticLdsReserve;
+    return (size_t)N * 8 <= (size_t)lds_cap && pick_vpt(1024, (N + 1023) / 1024) > 0;
+}
+
+}  // namespace gg
+
+using namespace gg;
+
+// ---- evaluator object ------------------------------------------------------------------
+struct gg_eval {
+    EvalKind kind;
+    int N, M, degree;
+    int Nx, Ny, Nz, Mx, My, Mz;
+    int device;
+    int W;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    DenseCtx dc;
+    void *d_tab = nullptr;
+    size_t tab_bytes = 0;
+    void *h_tab = nullptr;  // pinned staging
+    int *d_src = nullptr;
+    ColPart *d_parts = nullptr;
+    ResultRec *d_res = nullptr;
+    ResultRec *h_res = nullptr;  // pinned
+    unsigned *d_done = nullptr;
+    int *d_ecc = nullptr;
+    long long *d_reach_col = nullptr;
+    long long *d_reach = nullptr;
+    gg_swap *d_swap = nullptr;
+    gg_swap *h_swap = nullptr;  // pinned
+    int have_base = 0;  // adjacency evaluators: a table is resident
+    int have_result = 0;
+    int last_levels = 0;
+};
+
+static int eval_alloc(gg_eval *ev, size_t tab_elem_bytes) {
+    GG_HIP(hipSetDevice(ev->device));
+    GG_HIP(hipStreamCreateWithFlags(&ev->stream, hipStreamNonBlocking));
+    GG_HIP(hipEventCreate(&ev->ev0));
+    GG_HIP(hipEventCreate(&ev->ev1));
+    int R = ev->kind == EVAL_ADJ ? ev->N : ev->M;
+    ev->tab_bytes = (size_t)R * ev->degree * tab_elem_bytes;
+    size_t tb = ev->tab_bytes ? ev->tab_bytes : 8;
+    if (hipMalloc(&ev->d_tab, tb) != hipSuccess) return GG_OOM;
+    if (hipHostMalloc(&ev->h_tab, tb, hipHostMallocDefault) != hipSuccess) return GG_OOM;
+    ev->W = (ev->M + 63) / 64;
+    const int L = GG_LEVEL_LIMIT + 1;
+    if (hipMalloc((void **)&ev->d_src, sizeof(int) * ev->M) != hipSuccess) return GG_OOM;
+    if (hipMalloc((void **)&ev->d_parts, sizeof(ColPart) * ev->W) != hipSuccess) return GG_OOM;
+    if (hipMalloc((void **)&ev->d_res, sizeof(ResultRec)) != hipSuccess) return GG_OOM;
+    if (hipHostMalloc((void **)&ev->h_res, sizeof(ResultRec), hipHostMallocDefault) != hipSuccess)
+        return GG_OOM;
+    if (hipMalloc((void **)&ev->d_done, sizeof(unsigned)) != hipSuccess) return GG_OOM;
+    if (hipMalloc((void **)&ev->d_ecc, sizeof(int) * ev->M) != hipSuccess) return GG_OOM;
+    if (hipMalloc((void **)&ev->d_reach_col, sizeof(long long) * (size_t)ev->W * L) != hipSuccess)
+        return GG_OOM;
+    if (hipMalloc((void **)&ev->d_reach, sizeof(long long) * L) != hipSuccess) return GG_OOM;
+    if (hipMalloc((void **)&ev->d_swap, sizeof(gg_swap)) != hipSuccess) return GG_OOM;
+    if (hipHostMalloc((void **)&ev->h_swap, sizeof(gg_swap), hipHostMallocDefault) != hipSuccess)
+        return GG_OOM;
+    GG_HIP(hipMemsetAsync(ev->d_done, 0, sizeof(unsigned), ev->stream));
+    // source -> vertex map
+    std::vector<int> src(ev->M);
+    for (int s = 0; s < ev->M; s++) {
+        switch (ev->kind) {
+            case EVAL_PIECE: {
+                int x = s / ev->My, y = s % ev->My;
+                src[s] = x * ev->Ny + y;
+                break;
+            }
+            case EVAL_BOX: {
+                int x = s / (ev->My * ev->Mz), r = s % (ev->My * ev->Mz);
+                int y = r / ev->Mz, z = r % ev->Mz;
+                src[s] = (x * ev->Ny + y) * ev->Nz + z;
+                break;
+            }
+            default: src[s] = s;
+        }
+    }
+    GG_HIP(hipMemcpyAsync(ev->d_src, src.data(), sizeof(int) * ev->M, hipMemcpyHostToDevice,
+                          ev->stream));
+    GG_HIP(hipStreamSynchronize(ev->stream));
+    ev->dc.lds_device = device_lds_limit(ev->device);
+    ev->dc.cu_count = device_cu_count(ev->device);
+    return GG_OK;
+}
+
+static int check_device(int device) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return GG_NO_DEVICE;
+    if (device < 0 || device >= n) return GG_BAD_ARG;
+    return GG_OK;
+}
+
+static int finish_create(gg_eval *ev, size_t elem, gg_eval **out) {
+    int st = eval_alloc(ev, elem);
+    if (st != GG_OK) { gg_destroy(ev); return st; }
+    *out = ev;
+    return GG_OK;
+}
+
+extern "C" int gg_create_part(int N, int M, int degree, int device, gg_eval **out) {
+    if (!out) return GG_BAD_ARG;
+    *out = nullptr;
+    if (N < 1 || M < 1 || M > N || degree < 0) return GG_BAD_ARG;
+    int st = check_device(device);
+    if (st != GG_OK) return st;
+    gg_eval *ev = new gg_eval();
+    ev->kind = EVAL_PART;
+    ev->N = N; ev->M = M; ev->degree = degree; ev->device = device;
+    bool narrow = N <= 65535;
+    ev->dc.topo_kind = narrow ? TOPO_RING16 : TOPO_RING32;
+    return finish_create(ev, narrow ? 2 : 4, out);
+}
+
+extern "C" int gg_create_adj(int N, int degree_max, int device, gg_eval **out) {
+    if (!out) return GG_BAD_ARG;
+    *out = nullptr;
+    if (N < 1 || degree_max < 0) return GG_BAD_ARG;
+    int st = check_device(device);
+    if (st != GG_OK) return st;
+    gg_eval *ev = new gg_eval();
+    ev->kind = EVAL_ADJ;
+    ev->N = N; ev->M = N; ev->degree = degree_max; ev->device = device;
+    bool narrow = N <= 65535;
+    ev->dc.topo_kind = narrow ? TOPO_RING16 : TOPO_RING32;
+    return finish_create(ev, narrow ? 2 : 4, out);
+}
+
+extern "C" int gg_create_piece(int Nx, int Ny, int Mx, int My, int degree, int device,
+                               gg_eval **out) {
+    if (!out) return GG_BAD_ARG;
+    *out = nullptr;
+    if (Nx < 1 || Ny < 1 || Mx < 1 || My < 1 || Mx > Nx || My > Ny || degree < 0) return GG_BAD_ARG;
+    if (Nx > 65535 || Ny > 65535 || (long long)Nx * Ny > 0x7FFFFFFF) return GG_UNSUPPORTED;
+    int st = check_device(device);
+    if (st != GG_OK) return st;
+    gg_eval *ev = new gg_eval();
+    ev->kind = EVAL_PIECE;
+    ev->Nx = Nx; ev->Ny = Ny; ev->Mx = Mx; ev->My = My;
+    ev->N = Nx * Ny; ev->M = Mx * My; ev->degree = degree; ev->device = device;
+    ev->dc.topo_kind = TOPO_PIECE;
+    return finish_create(ev, 4, out);
+}
+
+extern "C" int gg_create_box(int Nx, int Ny, int Nz, int Mx, int My, int Mz, int degree,
+                             int device, gg_eval **out) {
+    if (!out) return GG_BAD_ARG;
+    *out = nullptr;
+    if (Nx < 1 || Ny < 1 || Nz < 1 || Mx < 1 || My < 1 || Mz < 1 || Mx > Nx || My > Ny ||
+        Mz > Nz || degree < 0)
+        return GG_BAD_ARG;
+    if (Nx >= (1 << 21) || Ny >= (1 << 21) || Nz >= (1 << 21) ||
+        (long long)Nx * Ny * Nz > 0x7FFFFFFF)
+        return GG_UNSUPPORTED;
+    int st = check_device(device);
+    if (st != GG_OK) return st;
+    gg_eval *ev = new gg_eval();
+    ev->kind = EVAL_BOX;
+    ev->Nx = Nx; ev->Ny = Ny; ev->Nz = Nz; ev->Mx = Mx; ev->My = My; ev->Mz = Mz;
+    ev->N = Nx * Ny * Nz; ev->M = Mx * My * Mz; ev->degree = degree; ev->device = device;
+    ev->dc.topo_kind = TOPO_BOX;
+    return finish_create(ev, 8, out);
+}
+
+extern "C" void gg_destroy(gg_eval *ev) {
+    if (!ev) return;
+    (void)hipSetDevice(ev->device);
+    if (ev->stream) (void)hipStreamSynchronize(ev->stream);
+    (void)hipFree(ev->d_tab);
+    (void)hipHostFree(ev->h_tab);
+    (void)hipFree(ev->d_src);
+    (void)hipFree(ev->d_parts);
+    (void)hipFree(ev->d_res);
+    (void)hipHostFree(ev->h_res);
+    (void)hipFree(ev->d_done);
+    (void)hipFree(ev->d_ecc);
+    (void)hipFree(ev->d_reach_col);
+    (void)hipFree(ev->d_reach);
+    (void)hipFree(ev->d_swap);
+    (void)hipHostFree(ev->h_swap);
+    if (ev->dc.gscratch) (void)hipFree(ev->dc.gscratch);
+    if (ev->ev0) (void)hipEventDestroy(ev->ev0);
+    if (ev->ev1) (void)hipEventDestroy(ev->ev1);
+    if (ev->stream) (void)hipStreamDestroy(ev->stream);
+    delete ev;
+}
+
+// ---- packing ---------------------------------------------------------------------------
+static inline long long norm_mod(long long x, long long n) {
+    long long r = x % n;
+    return r < 0 ? r + n : r;
+}
+
+// The reference adds N before % (cudaASPLconv.cu:66); for every offset where
+// that is well defined the value equals the mathematical residue used here.
+static int pack_ring(gg_eval *ev, const int32_t *table, int R, bool adj) {
+    const int N = ev->N, D = ev->degree;
+    if (ev->dc.topo_kind == TOPO_RING16) {
+        uint16_t *h = (uint16_t *)ev->h_tab;
+        for (int r = 0; r < R; r++)
+            for (int i = 0; i < D; i++) {
+                long long x = table[(size_t)r * D + i];
+                long long off;
+                if (adj) {
+                    if (x < 0) off = 0;  // padding = self-loop (cudaASPLgraph.cu:167-170)
+                    else if (x >= N) return GG_BAD_ARG;
+                    else off = norm_mod(x - r, N);
+                } else {
+                    off = norm_mod(x, N);
+                }
+                h[(size_t)i * R + r] = (uint16_t)off;
+            }
+    } else {
+        uint32_t *h = (uint32_t *)ev->h_tab;
+        for (int r = 0; r < R; r++)
+            for (int i = 0; i < D; i++) {
+                long long x = table[(size_t)r * D + i];
+                long long off;
+                if (adj) {
+                    if (x < 0) off = 0;
+                    else if (x >= N) return GG_BAD_ARG;
+                    else off = norm_mod(x - r, N);
+                } else {
+                    off = norm_mod(x, N);
+                }
+                h[(size_t)i * R + r] = (uint32_t)off;
+            }
+    }
+    return GG_OK;
+}
+
+static int pack_piece(gg_eval *ev, const int32_t *table) {
+    const int R = ev->M, D = ev->degree;
+    uint32_t *h = (uint32_t *)ev->h_tab;
+    for (int r = 0; r < R; r++)
+        for (int i = 0; i < D; i++) {
+            long long dx = norm_mod(table[((size_t)r * D + i) * 2], ev->Nx);
+            long long dy = norm_mod(table[((size_t)r * D + i) * 2 + 1], ev->Ny);
+            h[(size_t)i * R + r] = (uint32_t)dx | ((uint32_t)dy << 16);
+        }
+    return GG_OK;
+}
+
+static int pack_box(gg_eval *ev, const int32_t *table) {
+    const int R = ev->M, D = ev->degree;
+    uint64_t *h = (uint64_t *)ev->h_tab;
+    for (int r = 0; r < R; r++)
+        for (int i = 0; i < D; i++) {
+            const int32_t *o = table + ((size_t)r * D + i) * 3;
+            uint64_t dx = (uint64_t)norm_mod(o[0], ev->Nx);
+            uint64_t dy = (uint64_t)norm_mod(o[1], ev->Ny);
+            uint64_t dz = (uint64_t)norm_mod(o[2], ev->Nz);
+            h[(size_t)i * R + r] = dx | (dy << 21) | (dz << 42);
+        }
+    return GG_OK;
+}
+
+static void bind_topology(gg_eval *ev) {
+    DenseCtx &c = ev->dc;
+    unsigned R = ev->kind == EVAL_ADJ ? ev->N : ev->M;
+    c.ring16 = TopoRing<uint16_t>{(const uint16_t *)ev->d_tab, (unsigned)ev->N, R};
+    c.ring32 = TopoRing<uint32_t>{(const uint32_t *)ev->d_tab, (unsigned)ev->N, R};
+    c.piece = TopoPiece{(const uint32_t *)ev->d_tab, (unsigned)ev->Nx, (unsigned)ev->Ny,
+                        (unsigned)ev->Mx, (unsigned)ev->My, (unsigned)ev->M};
+    c.box = TopoBox{(const uint64_t *)ev->d_tab, (unsigned)ev->Nx, (unsigned)ev->Ny,
+                    (unsigned)ev->Nz, (unsigned)ev->Mx, (unsigned)ev->My, (unsigned)ev->Mz,
+                    (unsigned)ev->M};
+}
+
+static void fill_result(const gg_eval *ev, const ResultRec &r, gg_result *out) {
+    out->total_reps = r.total;
+    out->total_scaled = r.total * (long long)(ev->N / ev->M);
+    out->diameter = r.diameter;
+    out->wvc = r.wvc;
+    out->levels = r.levels;
+    const long long pairs = (long long)ev->N * ev->M;
+    if (r.flags & GG_FLAG_LIMIT) out->status = GG_DIAMETER_LIMIT;
+    else if (r.reached != pairs) out->status = GG_UNCONNECTED;
+    else out->status = GG_OK;
+}
+
+static int eval_common(gg_eval *ev, gg_result *out, bool upload = true,
+                       const gg_swap *d_swap = nullptr) {
+    if (upload)
+        GG_HIP(hipMemcpyAsync(ev->d_tab, ev->h_tab, ev->tab_bytes, hipMemcpyHostToDevice,
+                              ev->stream));
+    bind_topology(ev);
+    DenseArgs a{};
+    a.N = ev->N; a.M = ev->M; a.D = ev->degree; a.W = ev->W; a.K = 1;
+    a.lmax = GG_LEVEL_LIMIT;
+    a.src_vertex = ev->d_src;
+    a.swaps = d_swap;
+    a.parts = ev->d_parts;
+    a.res = ev->d_res;
+    a.done = ev->d_done;
+    a.ecc_out = ev->d_ecc;
+    a.reach_col = ev->d_reach_col;
+    a.reach_out = ev->d_reach;
+    a.want_wvc = ev->kind == EVAL_PART ? 1 : 0;
+    GG_HIP(hipEventRecord(ev->ev0, ev->stream));
+    int st = run_dense(ev->dc, a, ev->stream);
+    if (st != GG_OK) return st;
+    GG_HIP(hipEventRecord(ev->ev1, ev->stream));
+    GG_HIP(hipMemcpyAsync(ev->h_res, ev->d_res, sizeof(ResultRec), hipMemcpyDeviceToHost,
+                          ev->stream));
+    GG_HIP(hipStreamSynchronize(ev->stream));
+    fill_result(ev, *ev->h_res, out);
+    ev->have_result = 1;
+    ev->last_levels = ev->h_res->levels;
+    return GG_OK;
+}
+
+extern "C" int gg_eval_part(gg_eval *ev, const int32_t *table, gg_result *out) {
+    if (!ev || !out || (!table && ev->degree > 0) || ev->kind != EVAL_PART) return GG_BAD_ARG;
+    GG_HIP(hipSetDevice(ev->device));
+    int st = pack_ring(ev, table, ev->M, false);
+    if (st != GG_OK) return st;
+    return eval_common(ev, out);
+}
+
+extern "C" int gg_eval_adj(gg_eval *ev, const int32_t *adj, gg_result *out) {
+    if (!ev || !out || (!adj && ev->degree > 0) || ev->kind != EVAL_ADJ) return GG_BAD_ARG;
+    GG_HIP(hipSetDevice(ev->device));
+    int st = pack_ring(ev, adj, ev->N, true);
+    if (st != GG_OK) return st;
+    st = eval_common(ev, out);
+    if (st == GG_OK) ev->have_base = 1;
+    return st;
+}
+
+// Validates a swap against the evaluator's shape (rows, slots, neighbours).
+static int check_swap(const gg_eval *ev, const gg_swap *sw) {
+    for (int j = 0; j < 4; j++) {
+        if (sw->v[j] < 0) continue;
+        if (sw->v[j] >= ev->N || sw->slot[j] < 0 || sw->slot[j] >= ev->degree ||
+            sw->nbr[j] < 0 || sw->nbr[j] >= ev->N)
+            return GG_BAD_ARG;
+    }
+    return GG_OK;
+}
+
+extern "C" int gg_eval_adj_swap(gg_eval *ev, const gg_swap *sw, gg_result *out) {
+    if (!ev || !sw || !out || ev->kind != EVAL_ADJ || !ev->have_base) return GG_BAD_ARG;
+    int st = check_swap(ev, sw);
+    if (st != GG_OK) return st;
+    GG_HIP(hipSetDevice(ev->device));
+    *ev->h_swap = *sw;
+    GG_HIP(hipMemcpyAsync(ev->d_swap, ev->h_swap, sizeof(gg_swap), hipMemcpyHostToDevice,
+                          ev->stream));
+    return eval_common(ev, out, false, ev->d_swap);
+}
+
+template <class IdxT>
+__global__ void k_commit_swap(IdxT *tab, int N, const gg_swap *sw) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (int j = 0; j < 4; j++) {
+        int v = sw->v[j];
+        if (v < 0) continue;
+        int off = sw->nbr[j] - v;
+        if (off < 0) off += N;
+        tab[(size_t)sw->slot[j] * N + v] = (IdxT)off;
+    }
+}
+
+extern "C" int gg_eval_adj_commit(gg_eval *ev, const gg_swap *sw) {
+    if (!ev || !sw || ev->kind != EVAL_ADJ || !ev->have_base) return GG_BAD_ARG;
+    int st = check_swap(ev, sw);
+    if (st != GG_OK) return st;
+    GG_HIP(hipSetDevice(ev->device));
+    *ev->h_swap = *sw;
+    GG_HIP(hipMemcpyAsync(ev->d_swap, ev->h_swap, sizeof(gg_swap), hipMemcpyHostToDevice,
+                          ev->stream));
+    if (ev->dc.topo_kind == TOPO_RING16)
+        hipLaunchKernelGGL(k_commit_swap<uint16_t>, dim3(1), dim3(64), 0, ev->stream,
+                           (uint16_t *)ev->d_tab, ev->N, (const gg_swap *)ev->d_swap);
+    else
+        hipLaunchKernelGGL(k_commit_swap<uint32_t>, dim3(1), dim3(64), 0, ev->stream,
+                           (uint32_t *)ev->d_tab, ev->N, (const gg_swap *)ev->d_swap);
+    GG_HIP(hipGetLastError());
+    GG_HIP(hipStreamSynchronize(ev->stream));
+    return GG_OK;
+}
+
+extern "C" int gg_eval_piece(gg_eval *ev, const int32_t *table, gg_result *out) {
+    if (!ev || !out || (!table && ev->degree > 0) || ev->kind != EVAL_PIECE) return GG_BAD_ARG;
+    GG_HIP(hipSetDevice(ev->device));
+    int st = pack_piece(ev, table);
+    if (st != GG_OK) return st;
+    return eval_common(ev, out);
+}
+
+extern "C" int gg_eval_box(gg_eval *ev, const int32_t *table, gg_result *out) {
+    if (!ev || !out || (!table && ev->degree > 0) || ev->kind != EVAL_BOX) return GG_BAD_ARG;
+    GG_HIP(hipSetDevice(ev->device));
+    int st = pack_box(ev, table);
+    if (st != GG_OK) return st;
+    return eval_common(ev, out);
+}
+
+extern "C" int gg_eval_levels(gg_eval *ev, int32_t *ecc_out, int64_t *reach_out, int reach_cap) {
+    if (!ev || reach_cap < 0) return GG_BAD_ARG;
+    if (!ev->have_result) return GG_BAD_ARG;
+    GG_HIP(hipSetDevice(ev->device));
+    if (ecc_out)
+        GG_HIP(hipMemcpyAsync(ecc_out, ev->d_ecc, sizeof(int) * ev->M, hipMemcpyDeviceToHost,
+                              ev->stream));
+    if (reach_out && reach_cap > 0) {
+        int L = GG_LEVEL_LIMIT + 1;
+        int n = reach_cap < L ? reach_cap : L;
+        std::vector<long long> tmp(L, 0);
+        GG_HIP(hipMemcpyAsync(tmp.data(), ev->d_reach, sizeof(long long) * L,
+                              hipMemcpyDeviceToHost, ev->stream));
+        GG_HIP(hipStreamSynchronize(ev->stream));
+        for (int l = 0; l < reach_cap; l++)
+            reach_out[l] = (l < n && l <= ev->last_levels) ? tmp[l] : 0;
+    }
+    GG_HIP(hipStreamSynchronize(ev->stream));
+    return GG_OK;
+}
+
+extern "C" int gg_eval_last_kernel_ms(gg_eval *ev, float *ms) {
+    if (!ev || !ms || !ev->have_result) return GG_BAD_ARG;
+    GG_HIP(hipEventElapsedTime(ms, ev->ev0, ev->ev1));
+    return GG_OK;
+}
+
+extern "C" int gg_eval_set_option(gg_eval *ev, int option, int value) {
+    if (!ev) return GG_BAD_ARG;
+    switch (option) {
+        case GG_OPT_BLOCK:
+            if (value != 0 && value != 256 && value != 1024) return GG_BAD_ARG;
+            ev->dc.block_opt = value;
+            return GG_OK;
+        case GG_OPT_LDS_LIMIT:
+            if (value < 0) return GG_BAD_ARG;
+            ev->dc.lds_limit_opt = value;
+            return GG_OK;
+        default: return GG_BAD_ARG;
+    }
+}
+
+extern "C" int gg_eval_get_option(gg_eval *ev, int option, int *value) {
+    if (!ev || !value) return GG_BAD_ARG;
+    switch (option) {
+        case GG_OPT_BLOCK: *value = ev->dc.block_opt; return GG_OK;
+        case GG_OPT_LDS_LIMIT: *value = ev->dc.lds_limit_opt; return GG_OK;
+        case 100: *value = ev->dc.last_variant; return GG_OK;  // kernel variant of the last call
+        default: return GG_BAD_ARG;
+    }
+}

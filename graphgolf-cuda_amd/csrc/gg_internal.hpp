@@ -1,0 +1,34 @@
+// gg_internal.hpp -- host-side types shared by the evaluator translation units.
+#pragma once
+#include "gg_common.hpp"
+#include "gg_dense.hpp"
+
+namespace gg {
+
+enum EvalKind { EVAL_PART = 0, EVAL_PIECE = 1, EVAL_BOX = 2, EVAL_ADJ = 3 };
+enum TopoKind { TOPO_RING16 = 0, TOPO_RING32 = 1, TOPO_PIECE = 2, TOPO_BOX = 3 };
+
+// Kernel-selection state of one evaluator (or batch evaluator).
+struct DenseCtx {
+    TopoKind topo_kind = TOPO_RING16;
+    TopoRing<uint16_t> ring16{};
+    TopoRing<uint32_t> ring32{};
+    TopoPiece piece{};
+    TopoBox box{};
+    int lds_device = 65536;   // max LDS bytes per workgroup on the device
+    int cu_count = 256;
+    int block_opt = 0;        // 0 = auto, else 256 / 1024
+    int lds_limit_opt = 0;    // 0 = device limit
+    int last_variant = 0;     // 1 = LDS/256, 2 = LDS/1024, 3 = global
+    unsigned long long *gscratch = nullptr;
+    size_t gscratch_bytes = 0;
+};
+
+int run_dense(DenseCtx &c, const DenseArgs &a, hipStream_t st);
+int run_dense_queue(DenseCtx &c, const DenseArgs &a, int grid, hipStream_t st);
+int dense_fits_lds(const DenseCtx &c, int N);
+int pick_vpt(int bs, int need);
+int device_lds_limit(int device);
+int device_cu_count(int device);
+
+}  // namespace gg

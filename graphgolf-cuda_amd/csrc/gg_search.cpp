@@ -1,0 +1,318 @@
+// gg_search.cpp -- search drivers (host C++): simulated annealing and hill
+// climbing over 2-opt edge swaps of a regular graph, on the exact evaluator.
+//
+// Reference drivers: cppcode/annealing.cpp:396-443 (acceptance + schedule),
+// cppcode/HillClimb.cpp:222-249 (hill climbing), createNeighbour
+// annealing.cpp:112-197 (2-opt).  Differences, fixed here on purpose:
+//   * proposals follow the 2-opt of the configs: pick edges (a,b),(c,d)
+//     uniformly (vertex then slot), rewire to (a,c),(b,d) or (a,d),(b,c) by a
+//     coin, and redraw when the result is not simple (the reference always
+//     takes (a,d),(b,c) and never rejects loops or multi-edges);
+//   * every draw goes through gg::Rng (mt19937_64 + explicit bounded/real
+//     mappings), so a GPU run and a CPU run of this driver draw the same moves;
+//   * the temperature is geometric per step, T(i) = T0 (T1/T0)^(i/steps), on
+//     the ASPL scale; the reference's exp((fx-fy) N(N-1)/temp) takes
+//     temp = T(i) N(N-1).
+//
+// Evaluation modes:
+//   PLAIN    one candidate per step, one exact launch (gg_eval_adj_swap)
+//   BATCHED  K candidates per step in one launch (gg_batch_eval); the best by
+//            (diameter, total) goes through the acceptance rule
+//   LEARNED  P proposals scored by the MLP; the top K by score are evaluated
+//            exactly as in BATCHED (see gg_learn.cpp)
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+#include "graphgolf_mi355x.h"
+#include "gg_rng.hpp"
+#include "graphgolf_search.h"
+#include "gg_search_impl.hpp"
+
+namespace gg {
+
+// ---- evaluator backends ------------------------------------------------------------
+
+struct PlainBackend : Backend {
+    gg_eval *ev = nullptr;
+    PlainBackend(int N, int d, int device, int *st) { *st = gg_create_adj(N, d, device, &ev); }
+    ~PlainBackend() override { gg_destroy(ev); }
+    int set_base(const int32_t *adj, gg_result *out) override { return gg_eval_adj(ev, adj, out); }
+    int eval(const gg_swap *sw, int K, gg_result *out) override {
+        for (int k = 0; k < K; k++) {
+            int st = gg_eval_adj_swap(ev, &sw[k], &out[k]);
+            if (st != GG_OK) return st;
+        }
+        return GG_OK;
+    }
+    int commit(const gg_swap *sw, gg_result *out) override {
+        int st = gg_eval_adj_commit(ev, sw);
+        if (st != GG_OK) return st;
+        gg_swap none;
+        for (int j = 0; j < 4; j++) { none.v[j] = -1; none.slot[j] = 0; none.nbr[j] = 0; }
+        return gg_eval_adj_swap(ev, &none, out);
+    }
+};
+
+struct BatchBackend : Backend {
+    gg_batch *b = nullptr;
+    BatchBackend(int N, int d, int K, int device, int *st) {
+        *st = gg_batch_create(N, d, K, device, &b);
+    }
+    ~BatchBackend() override { gg_batch_destroy(b); }
+    int set_base(const int32_t *adj, gg_result *out) override {
+        return gg_batch_set_base(b, adj, out);
+    }
+    int eval(const gg_swap *sw, int K, gg_result *out) override {
+        return gg_batch_eval(b, sw, K, out);
+    }
+    int commit(const gg_swap *sw, gg_result *out) override { return gg_batch_commit(b, sw, out); }
+};
+
+struct CallbackBackend : Backend {
+    gg_eval_callbacks cb;
+    explicit CallbackBackend(const gg_eval_callbacks &c) : cb(c) {}
+    int set_base(const int32_t *adj, gg_result *out) override {
+        return cb.set_base(cb.ctx, adj, out);
+    }
+    int eval(const gg_swap *sw, int K, gg_result *out) override {
+        return cb.eval(cb.ctx, sw, K, out);
+    }
+    int commit(const gg_swap *sw, gg_result *out) override { return cb.commit(cb.ctx, sw, out); }
+};
+
+// ---- moves ----------------------------------------------------------------------------
+
+static bool row_has(const std::vector<int32_t> &adj, int d, int v, int x) {
+    const int32_t *r = &adj[(size_t)v * d];
+    for (int i = 0; i < d; i++)
+        if (r[i] == x) return true;
+    return false;
+}
+
+static int row_slot(const std::vector<int32_t> &adj, int d, int v, int x) {
+    const int32_t *r = &adj[(size_t)v * d];
+    for (int i = 0; i < d; i++)
+        if (r[i] == x) return i;
+    return -1;
+}
+
+// One proposal attempt: draws a, ia, c, ic, kind (5 draws).  Returns false
+// when the rewired graph would not be simple.
+bool propose_swap(const std::vector<int32_t> &adj, int N, int d, Rng &rng, gg_swap *sw,
+                  MoveRec *mv) {
+    int a = (int)rng.below(N), ia = (int)rng.below(d);
+    int c = (int)rng.below(N), ic = (int)rng.below(d);
+    int kind = (int)rng.below(2);
+    int b = adj[(size_t)a * d + ia], dd = adj[(size_t)c * d + ic];
+    if (a == b || a == c || a == dd || b == c || b == dd || c == dd) return false;
+    // kind 1: (a,b),(c,dd) -> (a,dd),(b,c);  kind 0: -> (a,c),(b,dd)
+    int xa = kind ? dd : c;   // new neighbour of a
+    int xb = kind ? c : dd;   // new neighbour of b
+    int xc = kind ? b : a;    // new neighbour of c
+    int xd = kind ? a : b;    // new neighbour of dd
+    if (row_has(adj, d, a, xa) || row_has(adj, d, b, xb)) return false;
+    int ib = row_slot(adj, d, b, a), id = row_slot(adj, d, dd, c);
+    if (ib < 0 || id < 0) return false;
+    sw->v[0] = a;  sw->slot[0] = ia; sw->nbr[0] = xa;
+    sw->v[1] = b;  sw->slot[1] = ib; sw->nbr[1] = xb;
+    sw->v[2] = c;  sw->slot[2] = ic; sw->nbr[2] = xc;
+    sw->v[3] = dd; sw->slot[3] = id; sw->nbr[3] = xd;
+    if (mv) { mv->a = a; mv->b = b; mv->c = c; mv->d = dd; mv->kind = kind; }
+    return true;
+}
+
+void apply_swap(std::vector<int32_t> &adj, int d, const gg_swap &sw) {
+    for (int j = 0; j < 4; j++)
+        if (sw.v[j] >= 0) adj[(size_t)sw.v[j] * d + sw.slot[j]] = sw.nbr[j];
+}
+
+// Objective key: unconnected / level-limit candidates rank last.
+static inline bool better(const gg_result &x, const gg_result &y) {
+    bool xo = x.status == GG_OK, yo = y.status == GG_OK;
+    if (xo != yo) return xo;
+    if (x.diameter != y.diameter) return x.diameter < y.diameter;
+    return x.total_scaled < y.total_scaled;
+}
+
+}  // namespace gg
+
+using namespace gg;
+
+// ---- C ABI ------------------------------------------------------------------------------
+
+static int search_init(gg_search *s, int N, int d, const int32_t *adj, const gg_search_cfg *cfg) {
+    s->N = N; s->d = d; s->cfg = *cfg;
+    s->adj.assign(adj, adj + (size_t)N * d);
+    s->rng.reset(new Rng(cfg->seed));
+    gg_result base;
+    int st = s->be->set_base(s->adj.data(), &base);
+    if (st != GG_OK) return st;
+    s->cur = base;
+    s->best = base;
+    s->best_adj = s->adj;
+    s->sw.resize(std::max(1, std::max(cfg->K, cfg->P)));
+    s->mv.resize(s->sw.size());
+    s->res.resize(s->sw.size());
+    return GG_OK;
+}
+
+static int validate_cfg(int N, int d, const int32_t *adj, const gg_search_cfg *cfg) {
+    if (!cfg || !adj || N < 5 || d < 2) return GG_BAD_ARG;
+    if (cfg->mode < GG_MODE_PLAIN || cfg->mode > GG_MODE_LEARNED) return GG_BAD_ARG;
+    if (cfg->rule != GG_RULE_ANNEAL && cfg->rule != GG_RULE_HILL) return GG_BAD_ARG;
+    if (cfg->mode != GG_MODE_PLAIN && cfg->K < 1) return GG_BAD_ARG;
+    if (cfg->mode == GG_MODE_LEARNED && cfg->P < cfg->K) return GG_BAD_ARG;
+    if (cfg->rule == GG_RULE_ANNEAL && (!(cfg->T0 > 0) || !(cfg->T1 > 0))) return GG_BAD_ARG;
+    if (cfg->total_steps < 1) return GG_BAD_ARG;
+    return GG_OK;
+}
+
+extern "C" int gg_search_create(int N, int d, const int32_t *adj, const gg_search_cfg *cfg,
+                                int device, gg_search **out) {
+    if (!out) return GG_BAD_ARG;
+    *out = nullptr;
+    int st = validate_cfg(N, d, adj, cfg);
+    if (st != GG_OK) return st;
+    gg_search *s = new gg_search();
+    if (cfg->mode == GG_MODE_PLAIN) s->be.reset(new PlainBackend(N, d, device, &st));
+    else s->be.reset(new BatchBackend(N, d, std::max(cfg->K, 1), device, &st));
+    if (st == GG_OK && cfg->mode == GG_MODE_LEARNED) st = learn_create(s, device);
+    if (st == GG_OK) st = search_init(s, N, d, adj, cfg);
+    if (st != GG_OK) { gg_search_destroy(s); return st; }
+    *out = s;
+    return GG_OK;
+}
+
+extern "C" int gg_search_create_cb(int N, int d, const int32_t *adj, const gg_search_cfg *cfg,
+                                   const gg_eval_callbacks *cb, gg_search **out) {
+    if (!out || !cb || !cb->set_base || !cb->eval || !cb->commit) return GG_BAD_ARG;
+    *out = nullptr;
+    int st = validate_cfg(N, d, adj, cfg);
+    if (st != GG_OK) return st;
+    if (cfg->mode == GG_MODE_LEARNED) return GG_UNSUPPORTED;
+    gg_search *s = new gg_search();
+    s->be.reset(new CallbackBackend(*cb));
+    st = search_init(s, N, d, adj, cfg);
+    if (st != GG_OK) { gg_search_destroy(s); return st; }
+    *out = s;
+    return GG_OK;
+}
+
+extern "C" void gg_search_destroy(gg_search *s) {
+    if (!s) return;
+    learn_destroy(s);
+    delete s;
+}
+
+// One step: propose, evaluate exactly, accept or reject.
+static int search_step(gg_search *s) {
+    const gg_search_cfg &cfg = s->cfg;
+    const int N = s->N, d = s->d;
+    Rng &rng = *s->rng;
+    int K = cfg.mode == GG_MODE_PLAIN ? 1 : cfg.K;
+    int st;
+    int nprop = cfg.mode == GG_MODE_LEARNED ? cfg.P : K;
+    for (int k = 0; k < nprop; k++) {
+        int tries = 0;
+        while (!propose_swap(s->adj, N, d, rng, &s->sw[k], &s->mv[k])) {
+            if (++tries > 1000000) return GG_BAD_ARG;  // no simple swap exists
+        }
+    }
+    s->stats.proposals += nprop;
+    if (cfg.mode == GG_MODE_LEARNED) {
+        st = learn_select(s, nprop, K);  // reorders sw/mv so the first K are the chosen ones
+        if (st != GG_OK) return st;
+    }
+    st = s->be->eval(s->sw.data(), K, s->res.data());
+    if (st != GG_OK) return st;
+    s->stats.exact_evals += K;
+    if (cfg.mode == GG_MODE_LEARNED) {
+        st = learn_observe(s, K);
+        if (st != GG_OK) return st;
+    }
+    int bi = 0;
+    for (int k = 1; k < K; k++)
+        if (better(s->res[k], s->res[bi])) bi = k;
+    const gg_result &y = s->res[bi];
+    s->step++;
+    // acceptance (annealing.cpp:406-412 / HillClimb.cpp:239)
+    const double NN1 = (double)N * (N - 1);
+    const double fx = (double)s->cur.total_scaled / NN1;
+    const int dx = s->cur.diameter;
+    const bool ok = y.status == GG_OK;
+    const double fy = ok ? (double)y.total_scaled / NN1 : 1e8;
+    const int dy = ok ? y.diameter : 100000000;
+    bool accept = false;
+    if (cfg.rule == GG_RULE_HILL) {
+        accept = ok && (dy < dx || (dy == dx && fy < fx));
+    } else {
+        double T = cfg.T0 * std::pow(cfg.T1 / cfg.T0, (double)s->step / cfg.total_steps);
+        if (fy < fx || dy < dx) accept = true;
+        else if (rng.uniform() < std::exp((fx - fy) * NN1 / (T * NN1))) accept = true;
+        if (dy > dx || !ok) accept = false;
+    }
+    if (s->log_cap > 0 && (int64_t)s->log.size() < s->log_cap * 10) {
+        const MoveRec &m = s->mv[bi];
+        int64_t rec[10] = {s->step, m.a, m.b, m.c, m.d, m.kind, y.total_scaled, y.diameter,
+                           y.status, accept ? 1 : 0};
+        s->log.insert(s->log.end(), rec, rec + 10);
+    }
+    if (accept) {
+        gg_result base;
+        st = s->be->commit(&s->sw[bi], &base);
+        if (st != GG_OK) return st;
+        apply_swap(s->adj, d, s->sw[bi]);
+        s->cur = base;
+        s->stats.accepted++;
+        if (better(s->cur, s->best)) {
+            s->best = s->cur;
+            s->best_adj = s->adj;
+        }
+    }
+    return GG_OK;
+}
+
+extern "C" int gg_search_run(gg_search *s, int steps, gg_search_stats *out) {
+    if (!s || steps < 0) return GG_BAD_ARG;
+    auto t0 = std::chrono::steady_clock::now();
+    for (int i = 0; i < steps; i++) {
+        int st = search_step(s);
+        if (st != GG_OK) return st;
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    s->stats.seconds += std::chrono::duration<double>(t1 - t0).count();
+    s->stats.steps = s->step;
+    s->stats.total_cur = s->cur.total_scaled;
+    s->stats.diam_cur = s->cur.diameter;
+    s->stats.total_best = s->best.total_scaled;
+    s->stats.diam_best = s->best.diameter;
+    learn_fill_stats(s, &s->stats);
+    if (out) *out = s->stats;
+    return GG_OK;
+}
+
+extern "C" int gg_search_graph(gg_search *s, int32_t *adj_out, int best) {
+    if (!s || !adj_out) return GG_BAD_ARG;
+    const std::vector<int32_t> &a = best ? s->best_adj : s->adj;
+    memcpy(adj_out, a.data(), sizeof(int32_t) * a.size());
+    return GG_OK;
+}
+
+extern "C" int gg_search_set_log(gg_search *s, int capacity) {
+    if (!s || capacity < 0) return GG_BAD_ARG;
+    s->log_cap = capacity;
+    s->log.clear();
+    s->log.reserve((size_t)capacity * 10);
+    return GG_OK;
+}
+
+extern "C" int gg_search_get_log(gg_search *s, int64_t *buf, int capacity, int *count) {
+    if (!s || !count) return GG_BAD_ARG;
+    int n = (int)(s->log.size() / 10);
+    *count = n;
+    if (buf) memcpy(buf, s->log.data(), sizeof(int64_t) * 10 * std::min(n, capacity));
+    return GG_OK;
+}

@@ -1,0 +1,53 @@
+// gg_search_impl.hpp -- internal state of a search (host side).
+#pragma once
+#include <cstdint>
+#include <memory>
+#include <vector>
+#include "graphgolf_search.h"
+#include "gg_rng.hpp"
+
+namespace gg {
+
+struct Backend {
+    virtual ~Backend() {}
+    virtual int set_base(const int32_t *adj, gg_result *out) = 0;
+    virtual int eval(const gg_swap *sw, int K, gg_result *out) = 0;
+    virtual int commit(const gg_swap *sw, gg_result *out) = 0;
+};
+
+struct MoveRec {
+    int a, b, c, d, kind;
+};
+
+struct LearnState;  // gg_learn.cpp
+
+bool propose_swap(const std::vector<int32_t> &adj, int N, int d, Rng &rng, gg_swap *sw,
+                  MoveRec *mv);
+void apply_swap(std::vector<int32_t> &adj, int d, const gg_swap &sw);
+
+}  // namespace gg
+
+struct gg_search {
+    int N = 0, d = 0;
+    gg_search_cfg cfg{};
+    std::unique_ptr<gg::Backend> be;
+    std::unique_ptr<gg::Rng> rng;
+    std::vector<int32_t> adj, best_adj;
+    gg_result cur{}, best{};
+    std::vector<gg_swap> sw;
+    std::vector<gg::MoveRec> mv;
+    std::vector<gg_result> res;
+    int64_t step = 0;
+    gg_search_stats stats{};
+    int log_cap = 0;
+    std::vector<int64_t> log;
+    gg::LearnState *learn = nullptr;
+};
+
+namespace gg {
+int learn_create(gg_search *s, int device);
+void learn_destroy(gg_search *s);
+int learn_select(gg_search *s, int P, int K);
+int learn_observe(gg_search *s, int K);
+void learn_fill_stats(gg_search *s, gg_search_stats *st);
+}  // namespace gg

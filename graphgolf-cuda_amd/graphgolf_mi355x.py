@@ -1,0 +1,496 @@
+"""ctypes binding of libgg_mi355x.so (the C ABI in include/graphgolf_mi355x.h).
+
+The evaluator classes mirror the reference's evaluator objects
+(cudacode/cudaASPLconv.hpp:6-23, cudaASPLpiece.hpp, cudaASPLbox.hpp,
+cudaASPLgraph.hpp): a constructor with the graph dimensions and a device id,
+and one call that returns the exact integer distance total and the diameter.
+Every call runs on the GPU; there is no CPU fallback, and a missing library or
+device raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgg_mi355x.so")
+
+GG_OK = 0
+GG_UNCONNECTED = 1
+GG_DIAMETER_LIMIT = 2
+GG_BAD_ARG = -1
+GG_HIP_ERROR = -2
+GG_OOM = -3
+GG_NO_DEVICE = -4
+GG_UNSUPPORTED = -5
+GG_LEVEL_LIMIT = 99
+
+GG_OPT_BLOCK = 1
+GG_OPT_LDS_LIMIT = 2
+
+GG_BATCH_OPT_PATH = 1
+GG_BATCH_OPT_GRAPH = 2
+GG_BATCH_INFO_LAST_SPARSE = 100
+GG_BATCH_INFO_SPARSE_OK = 101
+PATH_AUTO, PATH_DENSE, PATH_SPARSE = 0, 1, 2
+
+
+class GGError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{msg} (status {code})")
+        self.code = code
+
+
+class Result(C.Structure):
+    _fields_ = [
+        ("total_reps", C.c_int64),
+        ("total_scaled", C.c_int64),
+        ("diameter", C.c_int32),
+        ("wvc", C.c_int32),
+        ("levels", C.c_int32),
+        ("status", C.c_int32),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Swap(C.Structure):
+    _fields_ = [("v", C.c_int32 * 4), ("slot", C.c_int32 * 4), ("nbr", C.c_int32 * 4)]
+
+
+class SearchCfg(C.Structure):
+    _fields_ = [
+        ("mode", C.c_int32), ("rule", C.c_int32), ("K", C.c_int32), ("P", C.c_int32),
+        ("total_steps", C.c_int32), ("warmup_pairs", C.c_int32), ("train_every", C.c_int32),
+        ("train_batch", C.c_int32), ("T0", C.c_double), ("T1", C.c_double), ("lr", C.c_double),
+        ("seed", C.c_uint64),
+    ]
+
+
+class SearchStats(C.Structure):
+    _fields_ = [
+        ("steps", C.c_int64), ("exact_evals", C.c_int64), ("proposals", C.c_int64),
+        ("accepted", C.c_int64), ("total_cur", C.c_int64), ("total_best", C.c_int64),
+        ("diam_cur", C.c_int32), ("diam_best", C.c_int32), ("seconds", C.c_double),
+        ("train_pairs", C.c_int64), ("train_steps", C.c_int64), ("train_loss", C.c_double),
+        ("spearman", C.c_double), ("miss_rate", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+SET_BASE_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_int32), C.POINTER(Result))
+EVAL_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(Swap), C.c_int, C.POINTER(Result))
+COMMIT_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(Swap), C.POINTER(Result))
+
+
+class EvalCallbacks(C.Structure):
+    _fields_ = [("ctx", C.c_void_p), ("set_base", SET_BASE_CB), ("eval", EVAL_CB),
+                ("commit", COMMIT_CB)]
+
+
+MODE_PLAIN, MODE_BATCHED, MODE_LEARNED = 0, 1, 2
+RULE_ANNEAL, RULE_HILL = 0, 1
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise GGError(GG_UNSUPPORTED, f"{LIB_PATH} not built: run __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    i32p, i64p = P(C.c_int32), P(C.c_int64)
+    vp = C.c_void_p
+    sig = {
+        "gg_strerror": (C.c_char_p, [C.c_int]),
+        "gg_device_count": (C.c_int, [P(C.c_int)]),
+        "gg_aspl": (C.c_double, [P(Result), C.c_int]),
+        "gg_create_part": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, P(vp)]),
+        "gg_create_piece": (C.c_int, [C.c_int] * 6 + [P(vp)]),
+        "gg_create_box": (C.c_int, [C.c_int] * 8 + [P(vp)]),
+        "gg_create_adj": (C.c_int, [C.c_int, C.c_int, C.c_int, P(vp)]),
+        "gg_destroy": (None, [vp]),
+        "gg_eval_part": (C.c_int, [vp, i32p, P(Result)]),
+        "gg_eval_piece": (C.c_int, [vp, i32p, P(Result)]),
+        "gg_eval_box": (C.c_int, [vp, i32p, P(Result)]),
+        "gg_eval_adj": (C.c_int, [vp, i32p, P(Result)]),
+        "gg_eval_levels": (C.c_int, [vp, i32p, i64p, C.c_int]),
+        "gg_eval_last_kernel_ms": (C.c_int, [vp, P(C.c_float)]),
+        "gg_eval_set_option": (C.c_int, [vp, C.c_int, C.c_int]),
+        "gg_eval_get_option": (C.c_int, [vp, C.c_int, P(C.c_int)]),
+        "gg_eval_adj_swap": (C.c_int, [vp, P(Swap), P(Result)]),
+        "gg_eval_adj_commit": (C.c_int, [vp, P(Swap)]),
+        "gg_batch_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, P(vp)]),
+        "gg_batch_destroy": (None, [vp]),
+        "gg_batch_set_base": (C.c_int, [vp, i32p, P(Result)]),
+        "gg_batch_eval": (C.c_int, [vp, P(Swap), C.c_int, P(Result)]),
+        "gg_batch_commit": (C.c_int, [vp, P(Swap), P(Result)]),
+        "gg_batch_last_kernel_ms": (C.c_int, [vp, P(C.c_float)]),
+        "gg_batch_base_ecc": (C.c_int, [vp, i32p]),
+        "gg_batch_set_option": (C.c_int, [vp, C.c_int, C.c_int]),
+        "gg_batch_get_option": (C.c_int, [vp, C.c_int, P(C.c_int)]),
+        "gg_search_create": (C.c_int, [C.c_int, C.c_int, i32p, P(SearchCfg), C.c_int, P(vp)]),
+        "gg_search_create_cb": (C.c_int, [C.c_int, C.c_int, i32p, P(SearchCfg),
+                                          P(EvalCallbacks), P(vp)]),
+        "gg_search_destroy": (None, [vp]),
+        "gg_search_run": (C.c_int, [vp, C.c_int, P(SearchStats)]),
+        "gg_search_graph": (C.c_int, [vp, i32p, C.c_int]),
+        "gg_search_set_log": (C.c_int, [vp, C.c_int]),
+        "gg_search_get_log": (C.c_int, [vp, i64p, C.c_int, P(C.c_int)]),
+        "gg_gen_random_regular": (C.c_int, [C.c_int, C.c_int, C.c_uint64, i32p]),
+        "gg_gen_orbit_part": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_uint64, i32p]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def strerror(code: int) -> str:
+    return lib().gg_strerror(code).decode()
+
+
+def check(code: int, what: str = "call"):
+    if code < 0:
+        raise GGError(code, f"{what}: {strerror(code)}")
+    return code
+
+
+def _i32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _ptr(a: np.ndarray, ct=C.c_int32):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    code = lib().gg_device_count(C.byref(n))
+    return n.value if code == GG_OK else 0
+
+
+# ---- generators (host code inside the library) ----------------------------------
+
+def random_regular(n: int, d: int, seed: int) -> np.ndarray:
+    """Seeded simple d-regular graph as an int32 [n, d] adjacency with sorted rows."""
+    out = np.empty((n, d), dtype=np.int32)
+    check(lib().gg_gen_random_regular(n, d, seed, _ptr(out)), "gg_gen_random_regular")
+    return out
+
+
+def orbit_part(g: int, m: int, d: int, seed: int) -> np.ndarray:
+    """Seeded orbit graph (N = g*m, M = m) as an int32 [m, d] 'part' offset table."""
+    out = np.empty((m, d), dtype=np.int32)
+    check(lib().gg_gen_orbit_part(g, m, d, seed, _ptr(out)), "gg_gen_orbit_part")
+    return out
+
+
+# ---- evaluators ----------------------------------------------------------------------
+
+class _Evaluator:
+    kind = ""
+
+    def __init__(self):
+        self._h = C.c_void_p()
+
+    def close(self):
+        if self._h:
+            lib().gg_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _run(self, fn, table: np.ndarray) -> Result:
+        r = Result()
+        check(fn(self._h, _ptr(table), C.byref(r)), fn.__name__)
+        return r
+
+    def levels(self, reach_cap: int = GG_LEVEL_LIMIT + 1):
+        ecc = np.empty(self.M, dtype=np.int32)
+        reach = np.zeros(reach_cap, dtype=np.int64)
+        check(lib().gg_eval_levels(self._h, _ptr(ecc), _ptr(reach, C.c_int64), reach_cap),
+              "gg_eval_levels")
+        return ecc, reach
+
+    def last_kernel_ms(self) -> float:
+        ms = C.c_float(0)
+        check(lib().gg_eval_last_kernel_ms(self._h, C.byref(ms)), "gg_eval_last_kernel_ms")
+        return ms.value
+
+    def set_option(self, opt: int, value: int):
+        check(lib().gg_eval_set_option(self._h, opt, value), "gg_eval_set_option")
+
+    def get_option(self, opt: int) -> int:
+        v = C.c_int(0)
+        check(lib().gg_eval_get_option(self._h, opt, C.byref(v)), "gg_eval_get_option")
+        return v.value
+
+
+class PartEvaluator(_Evaluator):
+    """1-D symmetric graph (cudaASPLconv): table = [M, degree] offsets."""
+
+    def __init__(self, N: int, M: int, degree: int, device: int = 0):
+        super().__init__()
+        self.N, self.M, self.degree = N, M, degree
+        check(lib().gg_create_part(N, M, degree, device, C.byref(self._h)), "gg_create_part")
+
+    def eval(self, table) -> Result:
+        t = _i32(table).reshape(self.M, self.degree)
+        return self._run(lib().gg_eval_part, t)
+
+
+class AdjEvaluator(_Evaluator):
+    """Explicit adjacency (cudaASPLgraph): adj = [N, degree_max], -1 = padding."""
+
+    def __init__(self, N: int, degree_max: int, device: int = 0):
+        super().__init__()
+        self.N, self.M, self.degree = N, N, degree_max
+        check(lib().gg_create_adj(N, degree_max, device, C.byref(self._h)), "gg_create_adj")
+
+    def eval(self, adj) -> Result:
+        a = _i32(adj).reshape(self.N, self.degree)
+        return self._run(lib().gg_eval_adj, a)
+
+
+class SwapEvaluator(AdjEvaluator):
+    """Plain single-move path: resident base adjacency, one swap per launch."""
+
+    def set_base(self, adj) -> Result:
+        return self.eval(adj)
+
+    def eval_swap(self, sw: "Swap") -> Result:
+        r = Result()
+        check(lib().gg_eval_adj_swap(self._h, C.byref(sw), C.byref(r)), "gg_eval_adj_swap")
+        return r
+
+    def commit(self, sw: "Swap"):
+        check(lib().gg_eval_adj_commit(self._h, C.byref(sw)), "gg_eval_adj_commit")
+
+
+class BatchEvaluator:
+    """K swap candidates against a resident base in one launch."""
+
+    def __init__(self, N: int, degree: int, max_k: int, device: int = 0):
+        self.N, self.degree, self.max_k = N, degree, max_k
+        self._h = C.c_void_p()
+        check(lib().gg_batch_create(N, degree, max_k, device, C.byref(self._h)),
+              "gg_batch_create")
+
+    def close(self):
+        if self._h:
+            lib().gg_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_base(self, adj) -> Result:
+        a = _i32(adj).reshape(self.N, self.degree)
+        r = Result()
+        check(lib().gg_batch_set_base(self._h, _ptr(a), C.byref(r)), "gg_batch_set_base")
+        return r
+
+    def eval(self, swaps) -> list:
+        K = len(swaps)
+        arr = (Swap * K)(*swaps)
+        out = (Result * K)()
+        check(lib().gg_batch_eval(self._h, arr, K, out), "gg_batch_eval")
+        return list(out)
+
+    def commit(self, sw: "Swap") -> Result:
+        r = Result()
+        check(lib().gg_batch_commit(self._h, C.byref(sw), C.byref(r)), "gg_batch_commit")
+        return r
+
+    def last_kernel_ms(self) -> float:
+        ms = C.c_float(0)
+        check(lib().gg_batch_last_kernel_ms(self._h, C.byref(ms)), "gg_batch_last_kernel_ms")
+        return ms.value
+
+    def set_option(self, opt: int, value: int):
+        check(lib().gg_batch_set_option(self._h, opt, value), "gg_batch_set_option")
+
+    def get_option(self, opt: int) -> int:
+        v = C.c_int(0)
+        check(lib().gg_batch_get_option(self._h, opt, C.byref(v)), "gg_batch_get_option")
+        return v.value
+
+
+def make_swap(adj, a, ia, c, ic, kind=1):
+    """2-opt swap of edges (a, adj[a][ia]) and (c, adj[c][ic]) as four slot
+    replacements: kind 1 -> (a,d),(b,c); kind 0 -> (a,c),(b,d).  Returns
+    (Swap, new_adjacency) or (None, None) when the result is not simple."""
+    adj = np.asarray(adj)
+    b = int(adj[a, ia])
+    d = int(adj[c, ic])
+    if len({int(a), b, int(c), d}) < 4:
+        return None, None
+    x, y = (d, c) if kind == 1 else (c, d)  # a gets x, b gets y
+    if x in adj[a] or y in adj[b]:
+        return None, None
+    ib = int(np.nonzero(adj[b] == a)[0][0])
+    idd = int(np.nonzero(adj[d] == c)[0][0])
+    # a: b->x ; b: a->y ; c: d->(b if kind==1 else a) ; d: c->(a if kind==1 else b)
+    cx = b if kind == 1 else a
+    dx = a if kind == 1 else b
+    sw = Swap()
+    for j, (v, s, n) in enumerate([(a, ia, x), (b, ib, y), (c, ic, cx), (d, idd, dx)]):
+        sw.v[j], sw.slot[j], sw.nbr[j] = int(v), int(s), int(n)
+    new = adj.copy()
+    for j in range(4):
+        new[sw.v[j], sw.slot[j]] = sw.nbr[j]
+    return sw, new
+
+
+class PieceEvaluator(_Evaluator):
+    """2-D symmetric graph (cudaASPLpiece): table = [Mx*My, degree, 2]."""
+
+    def __init__(self, Nx, Ny, Mx, My, degree, device=0):
+        super().__init__()
+        self.N, self.M, self.degree = Nx * Ny, Mx * My, degree
+        check(lib().gg_create_piece(Nx, Ny, Mx, My, degree, device, C.byref(self._h)),
+              "gg_create_piece")
+
+    def eval(self, table) -> Result:
+        t = _i32(table).reshape(self.M, self.degree, 2)
+        return self._run(lib().gg_eval_piece, t)
+
+
+class BoxEvaluator(_Evaluator):
+    """3-D symmetric graph (cudaASPLbox): table = [Mx*My*Mz, degree, 3]."""
+
+    def __init__(self, Nx, Ny, Nz, Mx, My, Mz, degree, device=0):
+        super().__init__()
+        self.N, self.M, self.degree = Nx * Ny * Nz, Mx * My * Mz, degree
+        check(lib().gg_create_box(Nx, Ny, Nz, Mx, My, Mz, degree, device, C.byref(self._h)),
+              "gg_create_box")
+
+    def eval(self, table) -> Result:
+        t = _i32(table).reshape(self.M, self.degree, 3)
+        return self._run(lib().gg_eval_box, t)
+
+
+def aspl(result: Result, N: int) -> float:
+    """ASPL as the reference derives it (cudacode/cudaASPLconv.cu:265)."""
+    return float(result.total_scaled) / (N * (N - 1))
+
+
+# ---- search drivers ---------------------------------------------------------------------
+
+def search_cfg(mode=MODE_PLAIN, rule=RULE_ANNEAL, K=1, P=1, total_steps=1000, T0=1e-3,
+               T1=1e-6, seed=12345, warmup_pairs=2048, train_every=1, train_batch=256,
+               lr=1e-3) -> SearchCfg:
+    return SearchCfg(mode=mode, rule=rule, K=K, P=P, total_steps=total_steps,
+                     warmup_pairs=warmup_pairs, train_every=train_every, train_batch=train_batch,
+                     T0=T0, T1=T1, lr=lr, seed=seed)
+
+
+class Search:
+    """Annealing / hill-climbing driver over 2-opt swaps on the exact evaluator.
+    With `callbacks`, the exact evaluator is the caller's (as the reference
+    swaps cudaASPLconv for cpuASPLqueue, cppcode/annealing_cpu.cpp:292)."""
+
+    def __init__(self, adj, cfg: SearchCfg, device: int = 0, callbacks=None, log: int = 0):
+        a = _i32(adj)
+        self.N, self.d = a.shape
+        self._h = C.c_void_p()
+        self._cb = callbacks  # keep the ctypes callbacks alive
+        if callbacks is None:
+            check(lib().gg_search_create(self.N, self.d, _ptr(a), C.byref(cfg), device,
+                                         C.byref(self._h)), "gg_search_create")
+        else:
+            check(lib().gg_search_create_cb(self.N, self.d, _ptr(a), C.byref(cfg),
+                                            C.byref(callbacks.struct), C.byref(self._h)),
+                  "gg_search_create_cb")
+        if log:
+            check(lib().gg_search_set_log(self._h, log), "gg_search_set_log")
+
+    def run(self, steps: int) -> SearchStats:
+        st = SearchStats()
+        check(lib().gg_search_run(self._h, steps, C.byref(st)), "gg_search_run")
+        return st
+
+    def graph(self, best=False) -> np.ndarray:
+        out = np.empty((self.N, self.d), dtype=np.int32)
+        check(lib().gg_search_graph(self._h, _ptr(out), 1 if best else 0), "gg_search_graph")
+        return out
+
+    def log(self) -> np.ndarray:
+        n = C.c_int(0)
+        check(lib().gg_search_get_log(self._h, None, 0, C.byref(n)), "gg_search_get_log")
+        buf = np.zeros((n.value, 10), dtype=np.int64)
+        check(lib().gg_search_get_log(self._h, _ptr(buf, C.c_int64), n.value, C.byref(n)),
+              "gg_search_get_log")
+        return buf
+
+    def close(self):
+        if self._h:
+            lib().gg_search_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class PyEvaluatorCallbacks:
+    """Wraps a Python evaluator object exposing set_base(adj) -> (total, diam,
+    status), eval(list of Swap) -> list of (total, diam, status) and
+    commit(Swap) -> (total, diam, status) as gg_eval_callbacks."""
+
+    def __init__(self, impl, N, d):
+        self.impl, self.N, self.d = impl, N, d
+
+        def _fill(r, t):
+            r.total_reps, r.total_scaled, r.diameter, r.wvc, r.levels, r.status = (
+                t[0], t[0], t[1], 0, t[1] + 1, t[2])
+
+        def set_base(ctx, adj, out):
+            a = np.ctypeslib.as_array(adj, shape=(N * d,)).reshape(N, d).copy()
+            _fill(out.contents, impl.set_base(a))
+            return 0
+
+        def ev(ctx, swaps, K, out):
+            for k, t in enumerate(impl.eval([swaps[k] for k in range(K)])):
+                _fill(out[k], t)
+            return 0
+
+        def commit(ctx, sw, out):
+            _fill(out.contents, impl.commit(sw.contents))
+            return 0
+
+        self._fns = (SET_BASE_CB(set_base), EVAL_CB(ev), COMMIT_CB(commit))
+        self.struct = EvalCallbacks(None, *self._fns)

@@ -1,0 +1,164 @@
+/*
+ * graphgolf_mi355x.h -- C ABI of the MI355X exact ASPL / diameter evaluator.
+ *
+ * Plain pointers and sizes only.  Every call is synchronous: results are valid
+ * on return and caller memory is read only during the call.  One handle is
+ * used by one host thread at a time (handles own their device buffers and
+ * stream; distinct handles may be used concurrently).
+ *
+ * Each entry point names the reference interface it replaces
+ * (confused-uec/graphgolf-cuda, file:line).
+ */
+#ifndef GRAPHGOLF_MI355X_H
+#define GRAPHGOLF_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------------
+ * The reference checks no device status and reports graph conditions through
+ * sentinels (cudacode/cudaASPLconv.cu:228-229, :252, :257; cppcode/
+ * cpuASPLqueue.cpp:92).  Here graph conditions are positive codes carried in
+ * gg_result.status; errors are negative return values. */
+#define GG_OK               0
+#define GG_UNCONNECTED      1   /* some source does not reach every vertex          */
+#define GG_DIAMETER_LIMIT   2   /* level 99 still found new vertices (cudaASPLconv.cu:231) */
+#define GG_BAD_ARG        (-1)
+#define GG_HIP_ERROR      (-2)
+#define GG_OOM            (-3)
+#define GG_NO_DEVICE      (-4)
+#define GG_UNSUPPORTED    (-5)
+
+/* Highest BFS level evaluated: the reference loops step = 1..99
+ * (cudacode/cudaASPLconv.cu:231) and reports a sentinel when step 99 still
+ * finds new vertices. */
+#define GG_LEVEL_LIMIT 99
+
+typedef struct gg_eval gg_eval;
+typedef struct gg_batch gg_batch;
+
+/* Result of one exact evaluation. */
+typedef struct gg_result {
+    int64_t total_reps;   /* sum over the M sources s of sum_v d(s,v) (pairs reached) */
+    int64_t total_scaled; /* total_reps * (N / M), integer N/M as cudaASPLconv.cu:264 */
+    int32_t diameter;     /* max eccentricity over the sources (reached part)         */
+    int32_t wvc;          /* WVC as cudaASPLconv.cu:258-308 computes it (part only)   */
+    int32_t levels;       /* BFS levels executed (diameter + 1 when connected)        */
+    int32_t status;       /* GG_OK, GG_UNCONNECTED or GG_DIAMETER_LIMIT                */
+} gg_result;
+
+/* One 2-opt edge swap as four adjacency-slot replacements on a shared base:
+ * row v[j], slot slot[j] becomes nbr[j].  (a,b),(c,d) -> (a,d),(b,c) is
+ *   {a, idx(a->b), d}, {b, idx(b->a), c}, {c, idx(c->d), b}, {d, idx(d->c), a}.
+ * Rows may repeat (entries are applied in order j = 0..3).  Unused entries have
+ * v = -1.  No reference counterpart: the reference re-uploads the whole table
+ * per move (cudacode/cudaASPLconv.cu:222-225). */
+typedef struct gg_swap {
+    int32_t v[4];
+    int32_t slot[4];
+    int32_t nbr[4];
+} gg_swap;
+
+/* ---- errors / device ----------------------------------------------------- */
+const char *gg_strerror(int status);
+int gg_device_count(int *count);
+/* ASPL exactly as the reference derives it: double(total)/(int64(N)*(N-1))
+ * (cudacode/cudaASPLconv.cu:265; cppcode/cpuASPLqueue.cpp:94). */
+double gg_aspl(const gg_result *r, int N);
+
+/* ---- single-graph evaluators ---------------------------------------------
+ * Replace the constructors cudaASPLconv(N,M,degree,device)
+ * (cudacode/cudaASPLconv.hpp:18, .cu:145-160), cudaASPLpiece(Nx,Ny,Mx,My,
+ * degree,device) (cudaASPLpiece.hpp:18, .cu:149-166), cudaASPLbox(Nx,Ny,Nz,Mx,
+ * My,Mz,degree,device) (cudaASPLbox.hpp:18) and cudaASPLgraph(N,degree_max,
+ * device) (cudaASPLgraph.hpp:16, .cu:133-146).  N need not be a multiple of
+ * M: the scaled total uses integer N/M as the reference does
+ * (cudaASPLconv.cu:264). */
+int gg_create_part(int N, int M, int degree, int device, gg_eval **out);
+int gg_create_piece(int Nx, int Ny, int Mx, int My, int degree, int device, gg_eval **out);
+int gg_create_box(int Nx, int Ny, int Nz, int Mx, int My, int Mz, int degree,
+                  int device, gg_eval **out);
+int gg_create_adj(int N, int degree_max, int device, gg_eval **out);
+/* Replaces the destructors (cudacode/cudaASPLconv.cu:161-173 and siblings). */
+void gg_destroy(gg_eval *ev);
+
+/* Replaces cudaASPLconv::diameterASPL / calc / WVCdiameterASPL
+ * (cudacode/cudaASPLconv.cu:174-331).  table is the reference's packed h_edges:
+ * M rows of `degree` int32 offsets, short rows padded with 0
+ * (cudaASPLconv.cu:222-224).  Vertex v's neighbours are (v + off + N) % N for
+ * the offsets of row v % M. */
+int gg_eval_part(gg_eval *ev, const int32_t *table, gg_result *out);
+/* Replaces cudaASPLpiece::diameterASPL (cudacode/cudaASPLpiece.cu:180-237);
+ * table = h_edges[((x*My+y)*degree+e)*2 + {0,1}] (cudaASPLpiece.cu:184-195). */
+int gg_eval_piece(gg_eval *ev, const int32_t *table, gg_result *out);
+/* Replaces cudaASPLbox::diameterASPL (cudacode/cudaASPLbox.cu:166-226);
+ * table = h_edges[((x*My*Mz+y*Mz+z)*degree+e)*3 + {0,1,2}]. */
+int gg_eval_box(gg_eval *ev, const int32_t *table, gg_result *out);
+/* Replaces cudaASPLgraph::calc (cudacode/cudaASPLgraph.cu:160-218): explicit
+ * adjacency N x degree_max, entries < 0 are padding (the reference pads with
+ * self-loops, cudaASPLgraph.cu:163-172).  All N vertices are sources. */
+int gg_eval_adj(gg_eval *ev, const int32_t *adj, gg_result *out);
+
+/* Plain single-move path on a resident adjacency (the table of the last
+ * gg_eval_adj call stays on the device): evaluate base + one swap in one
+ * launch without changing the base, and commit an accepted swap to the base.
+ * Replaces the per-move cudaASPLgraph/cudaASPLconv call pattern of the drivers
+ * (cppcode/annealing.cpp:399-403), which re-uploads the whole table. */
+int gg_eval_adj_swap(gg_eval *ev, const gg_swap *sw, gg_result *out);
+int gg_eval_adj_commit(gg_eval *ev, const gg_swap *sw);
+
+/* Per-source eccentricities (M entries, source order: part/adj = vertex id;
+ * piece = x*My+y; box = (x*My+y)*Mz+z) and per-level reach counts (pairs at
+ * distance exactly l, l < reach_cap) of the most recent evaluation.  Either
+ * pointer may be NULL.  No reference counterpart (the reference never exposes
+ * them); used by the proposal features. */
+int gg_eval_levels(gg_eval *ev, int32_t *ecc_out, int64_t *reach_out, int reach_cap);
+
+/* Device time (ms, HIP events on the evaluator's stream) of the BFS kernel of
+ * the most recent evaluation. */
+int gg_eval_last_kernel_ms(gg_eval *ev, float *ms);
+
+/* Tuning knobs (0 = automatic choice). */
+#define GG_OPT_BLOCK      1   /* threads per workgroup: 256 or 1024                  */
+#define GG_OPT_LDS_LIMIT  2   /* max LDS bytes for the column bitset; a smaller value
+                                 forces the global-memory kernel (used by tests)      */
+int gg_eval_set_option(gg_eval *ev, int option, int value);
+int gg_eval_get_option(gg_eval *ev, int option, int *value);
+
+/* ---- batched evaluation ----------------------------------------------------
+ * K swap candidates against one resident base adjacency (N x degree, explicit
+ * neighbours, all N vertices are sources), exact per-candidate totals and
+ * diameters from one launch.  No reference counterpart (the reference
+ * evaluates one move per host round trip, cppcode/annealing.cpp:397-403). */
+int gg_batch_create(int N, int degree, int max_k, int device, gg_batch **out);
+void gg_batch_destroy(gg_batch *b);
+/* Upload and evaluate the base graph. */
+int gg_batch_set_base(gg_batch *b, const int32_t *adj, gg_result *base_out);
+/* Evaluate base + swaps[k] for k < K (K <= max_k); the base is unchanged. */
+int gg_batch_eval(gg_batch *b, const gg_swap *swaps, int K, gg_result *out);
+/* Apply one swap to the base and re-evaluate it. */
+int gg_batch_commit(gg_batch *b, const gg_swap *sw, gg_result *base_out);
+/* Device time (ms) of the most recent gg_batch_eval launch. */
+int gg_batch_last_kernel_ms(gg_batch *b, float *ms);
+/* Per-vertex eccentricity of the current base (N entries). */
+int gg_batch_base_ecc(gg_batch *b, int32_t *ecc_out);
+
+/* Evaluation path: 0 = automatic (incremental when the base allows it),
+ * 1 = dense re-evaluation of every candidate, 2 = incremental when possible. */
+#define GG_BATCH_OPT_PATH          1
+/* 1 = replay the incremental step as one captured HIP graph (default). */
+#define GG_BATCH_OPT_GRAPH         2
+/* read-only: 1 if the last gg_batch_eval used the incremental path */
+#define GG_BATCH_INFO_LAST_SPARSE  100
+/* read-only: 1 if the current base allows the incremental path */
+#define GG_BATCH_INFO_SPARSE_OK    101
+int gg_batch_set_option(gg_batch *b, int option, int value);
+int gg_batch_get_option(gg_batch *b, int option, int *value);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

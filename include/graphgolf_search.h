@@ -1,0 +1,80 @@
+/*
+ * graphgolf_search.h -- C ABI of the search drivers built on the exact
+ * evaluator: simulated annealing / hill climbing over 2-opt swaps of a
+ * regular graph, with plain, batched and learned-proposal evaluation.
+ *
+ * Replaces the driver loops of cppcode/annealing.cpp:396-443 (acceptance and
+ * schedule), cppcode/HillClimb.cpp:222-249 and createNeighbour
+ * (annealing.cpp:112-197), which the reference compiles into each CLI.
+ */
+#ifndef GRAPHGOLF_SEARCH_H
+#define GRAPHGOLF_SEARCH_H
+
+#include <stdint.h>
+#include "graphgolf_mi355x.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gg_search gg_search;
+
+#define GG_MODE_PLAIN   0  /* one exact evaluation per step                  */
+#define GG_MODE_BATCHED 1  /* K candidates per step, one launch               */
+#define GG_MODE_LEARNED 2  /* P proposals scored by the MLP, top K exact     */
+
+#define GG_RULE_ANNEAL  0  /* cppcode/annealing.cpp:406-412                   */
+#define GG_RULE_HILL    1  /* cppcode/HillClimb.cpp:239                        */
+
+typedef struct gg_search_cfg {
+    int32_t mode;
+    int32_t rule;
+    int32_t K;            /* exact candidates per step (batched, learned)    */
+    int32_t P;            /* proposals scored per step (learned)             */
+    int32_t total_steps;  /* length of the temperature schedule              */
+    int32_t warmup_pairs; /* learned: uniform selection until this many exact pairs */
+    int32_t train_every;  /* learned: optimiser steps after this many steps  */
+    int32_t train_batch;  /* learned: minibatch size                         */
+    double T0, T1;        /* temperature on the ASPL scale, geometric T0 -> T1 */
+    double lr;            /* learned: Adam learning rate                     */
+    uint64_t seed;
+} gg_search_cfg;
+
+typedef struct gg_search_stats {
+    int64_t steps, exact_evals, proposals, accepted;
+    int64_t total_cur, total_best;   /* scaled distance totals              */
+    int32_t diam_cur, diam_best;
+    double seconds;                  /* wall time inside gg_search_run       */
+    /* learned mode */
+    int64_t train_pairs, train_steps;
+    double train_loss;               /* last minibatch MSE (scaled target)   */
+    double spearman;                 /* rank correlation score vs exact delta (last audit) */
+    double miss_rate;                /* fraction of the exact top-K the scorer missed (last audit) */
+} gg_search_stats;
+
+/* Exact evaluator supplied by the caller (the reference's compile-time
+ * evaluator substitution, annealing.cpp:296 vs annealing_cpu.cpp:292). */
+typedef struct gg_eval_callbacks {
+    void *ctx;
+    int (*set_base)(void *ctx, const int32_t *adj, gg_result *out);
+    int (*eval)(void *ctx, const gg_swap *swaps, int K, gg_result *out);
+    int (*commit)(void *ctx, const gg_swap *sw, gg_result *out);
+} gg_eval_callbacks;
+
+int gg_search_create(int N, int d, const int32_t *adj, const gg_search_cfg *cfg, int device,
+                     gg_search **out);
+int gg_search_create_cb(int N, int d, const int32_t *adj, const gg_search_cfg *cfg,
+                        const gg_eval_callbacks *cb, gg_search **out);
+void gg_search_destroy(gg_search *s);
+int gg_search_run(gg_search *s, int steps, gg_search_stats *out);
+/* Current (best = 0) or best (best = 1) graph as N x d adjacency. */
+int gg_search_graph(gg_search *s, int32_t *adj_out, int best);
+/* Per-step log: 10 int64 per step {step, a, b, c, d, kind, total, diameter,
+ * status, accepted} of the candidate that went through the acceptance rule. */
+int gg_search_set_log(gg_search *s, int capacity);
+int gg_search_get_log(gg_search *s, int64_t *buf, int capacity, int *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

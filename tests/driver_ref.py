@@ -1,0 +1,127 @@
+"""Pure-Python restatement of the search driver, used to pin the C++ driver
+(graphgolf-cuda_amd/csrc/gg_search.cpp) move for move.  Test infrastructure.
+
+RNG: std::mt19937_64 (the C++ standard's parameters: w=64, n=312, m=156,
+r=31, a=0xB5026F5AA96619E9, u=29, d=0x5555555555555555, s=17,
+b=0x71D67FFFEDA60000, t=37, c=0xFFF7EEE000000000, l=43, f=6364136223846793005),
+Lemire's bounded integers, 53-bit doubles.  Acceptance: cppcode/annealing.cpp:
+406-412 and cppcode/HillClimb.cpp:239.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+MASK = (1 << 64) - 1
+
+
+class MT64:
+    def __init__(self, seed):
+        self.mt = [0] * 312
+        self.mt[0] = seed & MASK
+        for i in range(1, 312):
+            self.mt[i] = (6364136223846793005 * (self.mt[i - 1] ^ (self.mt[i - 1] >> 62)) + i) & MASK
+        self.idx = 312
+
+    def _twist(self):
+        mt = self.mt
+        for i in range(312):
+            x = (mt[i] & 0xFFFFFFFF80000000) | (mt[(i + 1) % 312] & 0x7FFFFFFF)
+            xa = x >> 1
+            if x & 1:
+                xa ^= 0xB5026F5AA96619E9
+            mt[i] = mt[(i + 156) % 312] ^ xa
+        self.idx = 0
+
+    def next(self):
+        if self.idx >= 312:
+            self._twist()
+        y = self.mt[self.idx]
+        self.idx += 1
+        y ^= (y >> 29) & 0x5555555555555555
+        y ^= (y << 17) & 0x71D67FFFEDA60000
+        y ^= (y << 37) & 0xFFF7EEE000000000
+        y ^= y >> 43
+        return y & MASK
+
+    def below(self, rng):
+        x = self.next()
+        m = x * rng
+        lo = m & MASK
+        if lo < rng:
+            t = ((1 << 64) - rng) % rng
+            while lo < t:
+                x = self.next()
+                m = x * rng
+                lo = m & MASK
+        return m >> 64
+
+    def uniform(self):
+        return (self.next() >> 11) * (1.0 / 9007199254740992.0)
+
+
+def propose(adj, rng):
+    n, d = adj.shape
+    a = rng.below(n); ia = rng.below(d)
+    c = rng.below(n); ic = rng.below(d)
+    kind = rng.below(2)
+    b = int(adj[a, ia]); dd = int(adj[c, ic])
+    if len({a, b, c, dd}) < 4:
+        return None
+    xa, xb = (dd, c) if kind else (c, dd)
+    xc, xd = (b, a) if kind else (a, b)
+    if xa in adj[a] or xb in adj[b]:
+        return None
+    ib = int(np.nonzero(adj[b] == a)[0][0]); idd = int(np.nonzero(adj[dd] == c)[0][0])
+    return (a, b, c, dd, kind), [(a, ia, xa), (b, ib, xb), (c, ic, xc), (dd, idd, xd)]
+
+
+def run(adj, evaluate, steps, mode_K=1, rule="anneal", T0=1e-3, T1=1e-6, total_steps=None,
+        seed=12345):
+    """evaluate(adj) -> (total_scaled, diameter, ok).  Returns the per-step log
+    rows {step, a, b, c, d, kind, total, diameter, status, accepted}."""
+    adj = np.array(adj, dtype=np.int32)
+    n, d = adj.shape
+    total_steps = total_steps or steps
+    rng = MT64(seed)
+    tx, dx, _ = evaluate(adj)
+    log = []
+    NN1 = float(n) * (n - 1)
+    for step in range(1, steps + 1):
+        cands = []
+        for _ in range(mode_K):
+            while True:
+                p = propose(adj, rng)
+                if p is not None:
+                    break
+            cands.append(p)
+        evals = []
+        for mv, ents in cands:
+            new = adj.copy()
+            for v, s, x in ents:
+                new[v, s] = x
+            evals.append(evaluate(new) + (new,))
+        # best by (ok, diameter, total)
+        bi = min(range(len(evals)), key=lambda k: (not evals[k][2], evals[k][1], evals[k][0]))
+        ty, dy, ok, new = evals[bi]
+        fx = tx / NN1
+        fy = ty / NN1 if ok else 1e8
+        dyv = dy if ok else 100000000
+        if rule == "hill":
+            acc = ok and (dyv < dx or (dyv == dx and fy < fx))
+        else:
+            T = T0 * math.pow(T1 / T0, step / total_steps)
+            acc = False
+            if fy < fx or dyv < dx:
+                acc = True
+            elif rng.uniform() < math.exp((fx - fy) * NN1 / (T * NN1)):
+                acc = True
+            if dyv > dx or not ok:
+                acc = False
+        mv = cands[bi][0]
+        log.append([step, *mv, ty, dy, 0 if ok else 1, int(acc)])
+        if acc:
+            adj = new
+            tx, dx = ty, dy
+    return np.array(log, dtype=np.int64), adj

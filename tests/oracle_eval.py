@@ -1,0 +1,45 @@
+"""Exact evaluator on the CPU oracle with the search-callback interface
+(set_base / eval / commit).  Test infrastructure only."""
+import ctypes as C
+import os
+
+import numpy as np
+
+import oracle_lib as ol
+
+
+def evaluate(adj, threads=None):
+    """(total, diameter, status) of an explicit adjacency via the oracle BFS."""
+    a = np.ascontiguousarray(adj, dtype=np.int32)
+    n, d = a.shape
+    srcs = np.arange(n, dtype=np.int32)
+    tot = C.c_int64()
+    dia = C.c_int32()
+    th = threads or min(16, os.cpu_count() or 1)
+    rc = ol.oracle().oracle_bfs_omp(n, d, ol._p(a), n, ol._p(srcs), th, C.byref(tot),
+                                    C.byref(dia))
+    assert rc >= 0
+    return tot.value, (dia.value if rc == 0 else 0), (0 if rc == 0 else 1)
+
+
+class OracleSwapEvaluator:
+    def __init__(self):
+        self.adj = None
+
+    def set_base(self, adj):
+        self.adj = np.array(adj, dtype=np.int32)
+        return evaluate(self.adj)
+
+    def _apply(self, sw, adj):
+        new = adj.copy()
+        for j in range(4):
+            if sw.v[j] >= 0:
+                new[sw.v[j], sw.slot[j]] = sw.nbr[j]
+        return new
+
+    def eval(self, swaps):
+        return [evaluate(self._apply(sw, self.adj)) for sw in swaps]
+
+    def commit(self, sw):
+        self.adj = self._apply(sw, self.adj)
+        return evaluate(self.adj)

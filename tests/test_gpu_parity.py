@@ -1,0 +1,214 @@
+"""GPU parity: the HIP evaluators against the CPU oracle, bit-exact on the
+integer distance total, the diameter, per-source eccentricities and the
+reference WVC, on the seeded config shapes and on edge cases."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import graphs
+import oracle_lib as ol
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "golden.json")
+
+
+def _check_adj(gg, adj, r, o):
+    assert r.total_reps == o["total"], (r.as_dict(), o["total"])
+    if o["unconnected"]:
+        assert r.status == gg.GG_UNCONNECTED
+    else:
+        assert r.status == gg.GG_OK, r.as_dict()
+        assert r.diameter == o["diameter"]
+        assert r.levels == o["diameter"] + 1
+
+
+@pytest.mark.parametrize("name,adj", [
+    ("C5cycle", graphs.cycle(5)), ("C64", graphs.cycle(64)), ("C65", graphs.cycle(65)),
+    ("K7", graphs.complete(7)), ("Q6", graphs.hypercube(6)), ("Q10", graphs.hypercube(10)),
+    ("petersen", graphs.petersen()), ("K33", graphs.kdd(3)),
+    ("circ", graphs.circulant(300, [1, 7, 31])), ("two_cycles", graphs.disjoint_cycles(30, 40)),
+])
+def test_closed_form_graphs(gpu, name, adj):
+    gg = gpu
+    n, d = adj.shape
+    o = ol.adj_eval(adj)
+    with gg.AdjEvaluator(n, d) as ev:
+        r = ev.eval(adj)
+    _check_adj(gg, adj, r, o)
+    # the same graph as a 'part' with M = N
+    with gg.PartEvaluator(n, n, d) as ev:
+        r2 = ev.eval(graphs.part_of_adj(adj))
+    assert r2.total_reps == o["total"] and r2.status == r.status
+
+
+def test_golden_configs(gpu):
+    """Every config graph against the values of the reference's own CPU oracle."""
+    gg = gpu
+    gold = json.load(open(GOLDEN))
+    for case in gold["cases"]:
+        N = case["N"]
+        print("golden case", case["name"], N, flush=True)
+        if case["kind"] == "regular":
+            adj = gg.random_regular(N, case["d"], case["seed"])
+            with gg.AdjEvaluator(N, case["d"]) as ev:
+                r = ev.eval(adj)
+            assert r.total_scaled == case["total_scaled"]
+        else:
+            tab = gg.orbit_part(case["g"], case["m"], case["d"], case["seed"])
+            with gg.PartEvaluator(N, case["m"], case["d"]) as ev:
+                r = ev.eval(tab)
+            assert r.total_scaled == case["total_scaled"]
+        assert r.diameter == case["diameter"] and r.status == gg.GG_OK
+        assert gg.aspl(r, N) == case["aspl"]
+
+
+@pytest.mark.parametrize("n,d,seed", [(256, 4, 1), (256, 4, 2), (1000, 6, 3), (4096, 12, 2),
+                                      (8192, 16, 4)])
+def test_random_regular_parity_with_levels(gpu, n, d, seed):
+    gg = gpu
+    adj = gg.random_regular(n, d, seed)
+    o = ol.adj_eval(adj)
+    with gg.AdjEvaluator(n, d) as ev:
+        r = ev.eval(adj)
+        _check_adj(gg, adj, r, o)
+        ecc, reach = ev.levels()
+    assert np.array_equal(ecc, o["ecc"])
+    assert np.array_equal(reach[:len(o["reach"])], o["reach"][:len(reach)])
+
+
+@pytest.mark.parametrize("block", [256, 1024])
+@pytest.mark.parametrize("lds_limit", [0, 1024])
+def test_kernel_variants_agree(gpu, block, lds_limit):
+    gg = gpu
+    adj = gg.random_regular(2048, 8, 17)
+    o = ol.adj_eval(adj)
+    with gg.AdjEvaluator(2048, 8) as ev:
+        ev.set_option(gg.GG_OPT_BLOCK, block)
+        ev.set_option(gg.GG_OPT_LDS_LIMIT, lds_limit)
+        r = ev.eval(adj)
+        variant = ev.get_option(100)
+        ecc, _ = ev.levels()
+    assert variant == (3 if lds_limit else (1 if block == 256 else 2))
+    _check_adj(gg, adj, r, o)
+    assert np.array_equal(ecc, o["ecc"])
+
+
+@pytest.mark.parametrize("g,m,d,seed", [(32, 128, 16, 3), (6, 50, 4, 9), (4, 20, 3, 1),
+                                        (16, 70, 6, 2), (2, 300, 5, 8)])
+def test_symmetric_part_parity(gpu, g, m, d, seed):
+    gg = gpu
+    if (m * d) % 2:
+        d += 1
+    tab = gg.orbit_part(g, m, d, seed)
+    N = g * m
+    o = ol.part_eval(N, m, tab)
+    with gg.PartEvaluator(N, m, d) as ev:
+        r = ev.eval(tab)
+        ecc, _ = ev.levels()
+    assert r.total_reps == o["total"] and r.total_scaled == o["total_scaled"]
+    if not o["unconnected"]:
+        assert r.diameter == o["diameter"] and r.status == gg.GG_OK
+        assert r.wvc == o["wvc"]
+        assert np.array_equal(ecc, o["ecc"])
+    else:
+        assert r.status == gg.GG_UNCONNECTED
+
+
+def test_part_with_irregular_offsets(gpu):
+    """Random offset tables (not symmetric graphs; edges are directed as the
+    reference's kernel reads them), offsets beyond +-N included."""
+    gg = gpu
+    rng = np.random.default_rng(3)
+    for trial in range(6):
+        M = int(rng.integers(1, 200))
+        g = int(rng.integers(1, 6))
+        N = M * g
+        D = int(rng.integers(1, 7))
+        tab = rng.integers(-N, 3 * N, size=(M, D)).astype(np.int32)
+        o = ol.part_eval(N, M, tab)
+        with gg.PartEvaluator(N, M, D) as ev:
+            r = ev.eval(tab)
+        assert r.total_reps == o["total"], trial
+        if not o["unconnected"]:
+            assert r.diameter == o["diameter"] and r.wvc == o["wvc"]
+
+
+def test_piece_and_box_parity(gpu):
+    gg = gpu
+    rng = np.random.default_rng(7)
+    # C3 as a piece: Nx = 32, Mx = 1, Ny = My = 128, offset (s, j - i)
+    tab = gg.orbit_part(32, 128, 16, 3)
+    s = np.floor_divide(tab + np.arange(128)[:, None], 128)
+    piece = np.stack([s, tab - 128 * s], axis=2)
+    o = ol.piece_eval(32, 128, 1, 128, piece)
+    with gg.PieceEvaluator(32, 128, 1, 128, 16) as ev:
+        r = ev.eval(piece)
+    assert r.total_scaled == o["total_scaled"] == ol.part_eval(4096, 128, tab)["total_scaled"]
+    assert r.diameter == o["diameter"]
+    for _ in range(5):
+        Nx, Ny, Mx, My, D = [int(x) for x in (rng.integers(2, 12), rng.integers(2, 12), 0, 0, 0)][:5]
+        Mx = int(rng.integers(1, Nx + 1)); My = int(rng.integers(1, Ny + 1)); D = int(rng.integers(2, 6))
+        t = rng.integers(-2 * Nx, 2 * Ny, size=(Mx * My, D, 2))
+        o = ol.piece_eval(Nx, Ny, Mx, My, t)
+        with gg.PieceEvaluator(Nx, Ny, Mx, My, D) as ev:
+            r = ev.eval(t)
+        assert r.total_scaled == o["total_scaled"]
+        if not o["unconnected"]:
+            assert r.diameter == o["diameter"]
+        dims = [int(x) for x in rng.integers(2, 7, size=3)]
+        ms = [int(rng.integers(1, x + 1)) for x in dims]
+        D = int(rng.integers(2, 6))
+        t = rng.integers(-8, 8, size=(ms[0] * ms[1] * ms[2], D, 3))
+        o = ol.box_eval(*dims, *ms, t)
+        with gg.BoxEvaluator(*dims, *ms, D) as ev:
+            r = ev.eval(t)
+        assert r.total_scaled == o["total_scaled"]
+        if not o["unconnected"]:
+            assert r.diameter == o["diameter"]
+
+
+def test_diameter_limit_and_unconnected(gpu):
+    gg = gpu
+    # diameter 98 is evaluated; 99 and 100 hit the reference's level limit
+    for n, want in [(196, gg.GG_OK), (198, gg.GG_DIAMETER_LIMIT), (200, gg.GG_DIAMETER_LIMIT)]:
+        with gg.AdjEvaluator(n, 2) as ev:
+            r = ev.eval(graphs.cycle(n))
+        assert r.status == want, (n, r.as_dict())
+        if want == gg.GG_OK:
+            assert r.diameter == 98 and r.total_reps == graphs.cycle_total(n)
+    adj = graphs.disjoint_cycles(100, 101)
+    o = ol.adj_eval(adj)
+    with gg.AdjEvaluator(201, 2) as ev:
+        r = ev.eval(adj)
+    assert r.status == gg.GG_UNCONNECTED and r.total_reps == o["total"]
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_fuzz_small_irregular(gpu, seed):
+    gg = gpu
+    rng = np.random.default_rng(100 + seed)
+    n = int(rng.integers(2, 300))
+    p = float(rng.uniform(0.005, 0.1))
+    m = np.triu(rng.random((n, n)) < p, 1)
+    m = m | m.T
+    deg = max(1, int(m.sum(1).max()))
+    adj = -np.ones((n, deg), dtype=np.int32)
+    for v in range(n):
+        nb = np.nonzero(m[v])[0]
+        adj[v, :len(nb)] = nb
+    o = ol.adj_eval(adj)
+    with gg.AdjEvaluator(n, deg) as ev:
+        r = ev.eval(adj)
+    _check_adj(gg, adj, r, o)
+
+
+def test_repeated_calls_reset_state(gpu):
+    gg = gpu
+    adjs = [gg.random_regular(512, 6, s) for s in range(4)]
+    with gg.AdjEvaluator(512, 6) as ev:
+        for _ in range(3):
+            for a in adjs:
+                r = ev.eval(a)
+                assert r.total_reps == ol.adj_eval(a)["total"]
