@@ -151,6 +151,35 @@ __device__ __forceinline__ unsigned long long gather_or(const Topo &topo, Vis vi
     return acc;
 }
 
+// Fixed-degree gather: all DT neighbour indices are loaded up front (one
+// memory latency per vertex instead of one per group), then OR-ed from LDS in
+// groups of four with the early exit.
+template <class Topo, int DT>
+struct FixedRow {
+    unsigned u[DT];
+    __device__ __forceinline__ void load(const Topo &topo, unsigned v) {
+        typename Topo::Ctx c = topo.ctx(v);
+#pragma unroll
+        for (int i = 0; i < DT; i++) u[i] = topo.nbr(c, i);
+    }
+};
+
+template <int DT, class Vis>
+__device__ __forceinline__ unsigned long long gather_fixed(const unsigned *u, Vis vis,
+                                                           unsigned long long own) {
+    unsigned long long acc = 0;
+#pragma unroll
+    for (int i = 0; i < DT; i += 4) {
+        unsigned long long g = vis[u[i]];
+        if (i + 1 < DT) g |= vis[u[i + 1]];
+        if (i + 2 < DT) g |= vis[u[i + 2]];
+        if (i + 3 < DT) g |= vis[u[i + 3]];
+        acc |= g;
+        if (i + 4 < DT && (acc | own) == ~0ull) break;
+    }
+    return acc;
+}
+
 // Padding bits of column w: sources 64w+j >= M are treated as already reached.
 __device__ __forceinline__ unsigned long long pad_mask(int w, int M) {
     int base = w * 64;
@@ -240,7 +269,7 @@ __device__ void finish_column(const DenseArgs &a, int k, int w, const ColPart &p
 // Thread t owns vertices t, t+BS, ..., so new words stay in registers across
 // the level barrier (no second LDS buffer).  Returns the column part in
 // thread 0.
-template <class Topo, int BS, int VPT>
+template <class Topo, int BS, int VPT, int DT>
 __device__ __forceinline__ ColPart dense_column(const Topo &topo, const DenseArgs &a, int k, int w,
                                                 unsigned long long *vis) {
     __shared__ unsigned long long s_or[2];
@@ -269,13 +298,29 @@ __device__ __forceinline__ ColPart dense_column(const Topo &topo, const DenseArg
     if (reach && tid == 0) reach[0] = nvalid;
     for (level = 1;; level++) {
         unsigned long long nw[VPT];
+        if (DT > 0 && !has_sw) {
+            // fixed degree: the next vertex's indices load while this one gathers
+            FixedRow<Topo, (DT > 0 ? DT : 1)> row[2];
+            if (tid < N) row[0].load(topo, tid);
 #pragma unroll
-        for (int j = 0; j < VPT; j++) {
-            unsigned v = tid + j * BS;
-            nw[j] = 0;
-            if (v < N) {
-                unsigned long long own = vis[v];
-                if (~own) nw[j] = gather_or(topo, vis, v, own, D, sw, has_sw) & ~own;
+            for (int j = 0; j < VPT; j++) {
+                unsigned v = tid + j * BS;
+                if (j + 1 < VPT && v + BS < N) row[(j + 1) & 1].load(topo, v + BS);
+                nw[j] = 0;
+                if (v < N) {
+                    unsigned long long own = vis[v];
+                    if (~own) nw[j] = gather_fixed<(DT > 0 ? DT : 1)>(row[j & 1].u, vis, own) & ~own;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < VPT; j++) {
+                unsigned v = tid + j * BS;
+                nw[j] = 0;
+                if (v < N) {
+                    unsigned long long own = vis[v];
+                    if (~own) nw[j] = gather_or(topo, vis, v, own, D, sw, has_sw) & ~own;
+                }
             }
         }
         __syncthreads();
@@ -327,17 +372,17 @@ __device__ __forceinline__ ColPart dense_column(const Topo &topo, const DenseArg
 }
 
 // One workgroup per (candidate, column); the last one of a candidate finalises it.
-template <class Topo, int BS, int VPT>
+template <class Topo, int BS, int VPT, int DT>
 __global__ void __launch_bounds__(BS) k_dense_lds(Topo topo, DenseArgs a) {
     extern __shared__ unsigned long long vis[];
     const int t = blockIdx.x;
     const int k = t / a.W, w = t - k * a.W;
-    ColPart p = dense_column<Topo, BS, VPT>(topo, a, k, w, vis);
+    ColPart p = dense_column<Topo, BS, VPT, DT>(topo, a, k, w, vis);
     finish_column<BS>(a, k, w, p);
 }
 
 // Persistent workgroups draining a device-side task queue; parts only.
-template <class Topo, int BS, int VPT>
+template <class Topo, int BS, int VPT, int DT>
 __global__ void __launch_bounds__(BS) k_dense_lds_queue(Topo topo, DenseArgs a) {
     extern __shared__ unsigned long long vis[];
     __shared__ int s_task;
@@ -347,7 +392,7 @@ __global__ void __launch_bounds__(BS) k_dense_lds_queue(Topo topo, DenseArgs a) 
         const int t = s_task;
         if (t >= *a.queue_count) break;
         const int2 kw = a.queue[t];
-        ColPart p = dense_column<Topo, BS, VPT>(topo, a, kw.x, kw.y, vis);
+        ColPart p = dense_column<Topo, BS, VPT, DT>(topo, a, kw.x, kw.y, vis);
         if (threadIdx.x == 0) a.parts[(size_t)kw.x * a.W + kw.y] = p;
         __syncthreads();
     }

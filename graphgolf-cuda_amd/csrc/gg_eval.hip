@@ -11,6 +11,7 @@
 #include "gg_common.hpp"
 #include "gg_dense.hpp"
 #include "gg_internal.hpp"
+#include "gg_fixed_impl.hpp"
 
 // ---- error bookkeeping -----------------------------------------------------------
 static thread_local std::string g_last_hip_error;
@@ -53,11 +54,11 @@ extern "C" double gg_aspl(const gg_result *r, int N) {
 // ---- kernel dispatch ------------------------------------------------------------------
 namespace gg {
 
-template <class Topo, int BS, int VPT>
+template <class Topo, int BS, int VPT, int DT>
 static hipError_t launch_lds_inst(const Topo &topo, const DenseArgs &a, int grid, size_t lds,
                                   hipStream_t st) {
     static int configured_bytes = -1;
-    auto fn = &k_dense_lds<Topo, BS, VPT>;
+    auto fn = &k_dense_lds<Topo, BS, VPT, DT>;
     if ((int)lds > configured_bytes) {
         hipError_t e = hipFuncSetAttribute((const void *)fn,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -72,7 +73,7 @@ template <class Topo>
 static hipError_t launch_lds(const Topo &topo, const DenseArgs &a, int grid, size_t lds, int bs,
                              int vpt, hipStream_t st) {
 #define GG_CASE(B, V) \
-    if (bs == B && vpt == V) return launch_lds_inst<Topo, B, V>(topo, a, grid, lds, st);
+    if (bs == B && vpt == V) return launch_lds_inst<Topo, B, V, 0>(topo, a, grid, lds, st);
     GG_CASE(256, 1) GG_CASE(256, 2) GG_CASE(256, 4) GG_CASE(256, 8) GG_CASE(256, 16)
     GG_CASE(1024, 1) GG_CASE(1024, 2) GG_CASE(1024, 4) GG_CASE(1024, 8) GG_CASE(1024, 16)
     GG_CASE(1024, 20)
@@ -80,11 +81,26 @@ static hipError_t launch_lds(const Topo &topo, const DenseArgs &a, int grid, siz
     return hipErrorInvalidValue;
 }
 
-template <class Topo, int BS, int VPT>
+// Fixed-degree kernels exist for the 16-bit ring topology, 1024-thread
+// workgroups and these degrees (gg_fixed_d*.hip).
+static hipError_t launch_fixed(const TopoRing<uint16_t> &topo, const DenseArgs &a, int grid,
+                               size_t lds, int vpt, bool queue, hipStream_t st, bool *done) {
+    *done = true;
+    switch (a.D) {
+        case 4: return launch_fixed_d4(topo, a, grid, lds, vpt, queue, st);
+        case 8: return launch_fixed_d8(topo, a, grid, lds, vpt, queue, st);
+        case 12: return launch_fixed_d12(topo, a, grid, lds, vpt, queue, st);
+        case 16: return launch_fixed_d16(topo, a, grid, lds, vpt, queue, st);
+        case 24: return launch_fixed_d24(topo, a, grid, lds, vpt, queue, st);
+        default: *done = false; return hipSuccess;
+    }
+}
+
+template <class Topo, int BS, int VPT, int DT>
 static hipError_t launch_lds_queue_inst(const Topo &topo, const DenseArgs &a, int grid, size_t lds,
                                         hipStream_t st) {
     static int configured_bytes = -1;
-    auto fn = &k_dense_lds_queue<Topo, BS, VPT>;
+    auto fn = &k_dense_lds_queue<Topo, BS, VPT, DT>;
     if ((int)lds > configured_bytes) {
         hipError_t e = hipFuncSetAttribute((const void *)fn,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -99,7 +115,7 @@ template <class Topo>
 static hipError_t launch_lds_queue(const Topo &topo, const DenseArgs &a, int grid, size_t lds,
                                    int vpt, hipStream_t st) {
 #define GG_CASE(V) \
-    if (vpt == V) return launch_lds_queue_inst<Topo, 1024, V>(topo, a, grid, lds, st);
+    if (vpt == V) return launch_lds_queue_inst<Topo, 1024, V, 0>(topo, a, grid, lds, st);
     GG_CASE(1) GG_CASE(2) GG_CASE(4) GG_CASE(8) GG_CASE(16) GG_CASE(20)
 #undef GG_CASE
     return hipErrorInvalidValue;
@@ -153,7 +169,14 @@ int run_dense(DenseCtx &c, const DenseArgs &a, hipStream_t st) {
     if ((int)lds <= lds_cap && vpt > 0) {
         c.last_variant = bs == 256 ? 1 : 2;
         switch (c.topo_kind) {
-            case TOPO_RING16: e = launch_lds(c.ring16, a, tasks, lds, bs, vpt, st); break;
+            case TOPO_RING16: {
+                bool done = false;
+                e = hipSuccess;
+                if (bs == 1024 && !c.generic_opt)
+                    e = launch_fixed(c.ring16, a, tasks, lds, vpt, false, st, &done);
+                if (!done) e = launch_lds(c.ring16, a, tasks, lds, bs, vpt, st);
+                break;
+            }
             case TOPO_RING32: e = launch_lds(c.ring32, a, tasks, lds, bs, vpt, st); break;
             case TOPO_PIECE: e = launch_lds(c.piece, a, tasks, lds, bs, vpt, st); break;
             default: e = launch_lds(c.box, a, tasks, lds, bs, vpt, st); break;
@@ -194,7 +217,13 @@ int run_dense_queue(DenseCtx &c, const DenseArgs &a, int grid, hipStream_t st) {
     if ((int)lds > lds_cap || vpt == 0) return GG_UNSUPPORTED;
     hipError_t e;
     switch (c.topo_kind) {
-        case TOPO_RING16: e = launch_lds_queue(c.ring16, a, grid, lds, vpt, st); break;
+        case TOPO_RING16: {
+            bool done = false;
+            e = hipSuccess;
+            if (!c.generic_opt) e = launch_fixed(c.ring16, a, grid, lds, vpt, true, st, &done);
+            if (!done) e = launch_lds_queue(c.ring16, a, grid, lds, vpt, st);
+            break;
+        }
         case TOPO_RING32: e = launch_lds_queue(c.ring32, a, grid, lds, vpt, st); break;
         default: return GG_UNSUPPORTED;
     }
@@ -645,6 +674,9 @@ extern "C" int gg_eval_set_option(gg_eval *ev, int option, int value) {
             if (value < 0) return GG_BAD_ARG;
             ev->dc.lds_limit_opt = value;
             return GG_OK;
+        case GG_OPT_GENERIC:
+            ev->dc.generic_opt = value ? 1 : 0;
+            return GG_OK;
         default: return GG_BAD_ARG;
     }
 }
@@ -654,6 +686,7 @@ extern "C" int gg_eval_get_option(gg_eval *ev, int option, int *value) {
     switch (option) {
         case GG_OPT_BLOCK: *value = ev->dc.block_opt; return GG_OK;
         case GG_OPT_LDS_LIMIT: *value = ev->dc.lds_limit_opt; return GG_OK;
+        case GG_OPT_GENERIC: *value = ev->dc.generic_opt; return GG_OK;
         case 100: *value = ev->dc.last_variant; return GG_OK;  // kernel variant of the last call
         default: return GG_BAD_ARG;
     }

@@ -19,6 +19,7 @@ struct DenseCtx {
     int cu_count = 256;
     int block_opt = 0;        // 0 = auto, else 256 / 1024
     int lds_limit_opt = 0;    // 0 = device limit
+    int generic_opt = 0;      // 1 = runtime-degree kernels even when a specialisation exists
     int last_variant = 0;     // 1 = LDS/256, 2 = LDS/1024, 3 = global
     unsigned long long *gscratch = nullptr;
     size_t gscratch_bytes = 0;

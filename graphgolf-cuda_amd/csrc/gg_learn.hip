@@ -1,0 +1,460 @@
+// gg_learn.hip -- learned proposal scorer: feature kernel and a 32-128-1 MLP
+// (BF16 weights, FP32 accumulation) with hand-written forward, backward and
+// Adam kernels.  No reference counterpart: the reference evaluates every
+// proposal exactly (cppcode/annealing.cpp:399-403).
+//
+// Features of a swap (a,b),(c,d) -> (p1,q1),(p2,q2) come from the base BFS
+// level words the batch evaluator stores: bit s of B_l[w][x] (s = 64w + j) is
+// [d(s, x) <= l], so the N-bit row R_l(x) = (B_l[w][x])_w is the radius-l ball
+// of x, and L_l(x) = R_l(x) & ~R_{l-1}(x) its l-th layer.
+#include <hip/hip_bf16.h>
+#include <cmath>
+#include <cstring>
+#include <vector>
+#include "gg_common.hpp"
+#include "gg_internal.hpp"
+#include "gg_batch.hpp"
+#include "gg_learn.hpp"
+
+using namespace gg;
+
+// ---- features --------------------------------------------------------------------------
+struct FeatArgs {
+    int N, W, S, diam, d;
+    const unsigned long long *snap;  // [S][W][N]
+    const ColPart *base_parts;       // [W]
+    const int *ecc;                  // [N] base eccentricity per vertex
+    const int *moves;                // [P][5]: a, b, c, d, kind
+    int P;
+    float *X;                        // [P][GG_FEATS]
+};
+
+// B_l[w][x] with levels >= the column's eccentricity full and level 0 = {x}.
+__device__ __forceinline__ unsigned long long ball_word(const FeatArgs &f, int l, int w, int x,
+                                                       int ecc_w) {
+    if (l <= 0) return ((x >> 6) == w) ? (1ull << (x & 63)) : 0ull;
+    if (l >= ecc_w) return ~0ull;
+    return f.snap[((size_t)(l - 1) * f.W + w) * f.N + x];
+}
+
+// One wave per proposal.  Counts are wave-reduced popcounts over the W words.
+__global__ void __launch_bounds__(256) k_features(FeatArgs f) {
+    const int lane = threadIdx.x & 63;
+    const int p = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (p >= f.P) return;
+    const int *m = f.moves + (size_t)p * 5;
+    const int a = m[0], b = m[1], c = m[2], dd = m[3], kind = m[4];
+    // added edges (p1,q1),(p2,q2); removed (a,b),(c,dd)
+    const int p1 = a, q1 = kind ? dd : c, p2 = b, q2 = kind ? c : dd;
+    const int ends[4] = {a, b, c, dd};
+    // per-lane partial counts
+    float cnt[24];
+#pragma unroll
+    for (int i = 0; i < 24; i++) cnt[i] = 0.f;
+    for (int w = lane; w < f.W; w += 64) {
+        const int ew = f.base_parts[w].ecc;
+        unsigned long long R[4][4];  // R[endpoint][level 0..3]
+#pragma unroll
+        for (int e = 0; e < 4; e++)
+#pragma unroll
+            for (int l = 0; l < 4; l++) R[e][l] = ball_word(f, l, w, ends[e], ew);
+        auto L = [&](int e, int l) { return R[e][l] & ~R[e][l - 1]; };
+        // index of p1,q1,p2,q2 in ends: a=0,b=1,c=2,dd=3
+        const int ip1 = 0, iq1 = kind ? 3 : 2, ip2 = 1, iq2 = kind ? 2 : 3;
+        const int pairs[2][2] = {{ip1, iq1}, {ip2, iq2}};
+#pragma unroll
+        for (int t = 0; t < 2; t++) {
+            int x = pairs[t][0], y = pairs[t][1];
+            cnt[t * 6 + 0] += __popcll(L(x, 1) & L(y, 1));
+            cnt[t * 6 + 1] += __popcll(L(x, 1) & L(y, 2)) + __popcll(L(x, 2) & L(y, 1));
+            cnt[t * 6 + 2] += __popcll(L(x, 2) & L(y, 2));
+            cnt[t * 6 + 3] += __popcll(R[x][2] | R[y][2]);
+            cnt[t * 6 + 4] += __popcll(L(x, 3) & L(y, 3));
+            cnt[t * 6 + 5] += __popcll(~R[x][3] & ~R[y][3]);
+        }
+        const int rem[2][2] = {{0, 1}, {2, 3}};
+#pragma unroll
+        for (int t = 0; t < 2; t++) {
+            int x = rem[t][0], y = rem[t][1];
+            cnt[12 + t * 4 + 0] += __popcll(L(x, 1) & L(y, 1));
+            cnt[12 + t * 4 + 1] += __popcll(L(x, 1) & L(y, 2)) + __popcll(L(x, 2) & L(y, 1));
+            cnt[12 + t * 4 + 2] += __popcll(L(x, 2) & L(y, 2));
+            cnt[12 + t * 4 + 3] += __popcll(R[x][2] & R[y][2]);
+        }
+#pragma unroll
+        for (int e = 0; e < 4; e++) cnt[20 + e] += __popcll(L(e, 2));
+    }
+#pragma unroll
+    for (int i = 0; i < 24; i++) {
+        float v = cnt[i];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        cnt[i] = v;
+    }
+    if (lane != 0) return;
+    // distance of each added pair before the swap
+    auto dist = [&](int x, int y) {
+        const int w = y >> 6, ew = f.base_parts[w].ecc;
+        for (int l = 1; l < 64; l++)
+            if ((ball_word(f, l, w, x, ew) >> (y & 63)) & 1ull) return l;
+        return 64;
+    };
+    float *X = f.X + (size_t)p * GG_FEATS;
+    const float d1 = (float)f.d, d2 = d1 * d1, n = (float)f.N, inv_diam = 1.f / (float)f.diam;
+    const int qs[2] = {q1, q2}, ps[2] = {p1, p2};
+    for (int t = 0; t < 2; t++) {
+        X[t * 8 + 0] = dist(ps[t], qs[t]) * inv_diam;
+        X[t * 8 + 1] = cnt[t * 6 + 0] / d1;
+        X[t * 8 + 2] = cnt[t * 6 + 1] / d2;
+        X[t * 8 + 3] = cnt[t * 6 + 2] / d2;
+        X[t * 8 + 4] = cnt[t * 6 + 3] / (2.f * d2);
+        X[t * 8 + 5] = cnt[t * 6 + 4] / n;
+        X[t * 8 + 6] = cnt[t * 6 + 5] / n;
+        X[t * 8 + 7] = (float)(f.ecc[ps[t]] + f.ecc[qs[t]] - 2 * f.diam) * 0.5f;
+    }
+    for (int t = 0; t < 2; t++) {
+        X[16 + t * 6 + 0] = cnt[12 + t * 4 + 0] / d1;
+        X[16 + t * 6 + 1] = cnt[12 + t * 4 + 1] / d2;
+        X[16 + t * 6 + 2] = cnt[12 + t * 4 + 2] / d2;
+        X[16 + t * 6 + 3] = cnt[12 + t * 4 + 3] / d2;
+        X[16 + t * 6 + 4] = (float)(f.ecc[ends[2 * t]] - f.diam);
+        X[16 + t * 6 + 5] = (float)(f.ecc[ends[2 * t + 1]] - f.diam);
+    }
+    for (int e = 0; e < 4; e++) X[28 + e] = cnt[20 + e] / d2;
+}
+
+// ---- MLP ---------------------------------------------------------------------------------
+// Parameter layout (fp32 masters and bf16 copies): W1[H][F], b1[H], W2[H], b2.
+#define OFF_W1 0
+#define OFF_B1 (GG_HIDDEN * GG_FEATS)
+#define OFF_W2 (OFF_B1 + GG_HIDDEN)
+#define OFF_B2 (OFF_W2 + GG_HIDDEN)
+
+__device__ __forceinline__ float bf(const __hip_bfloat16 v) { return __bfloat162float(v); }
+
+// Forward, one thread per sample; bf16 weights staged in LDS, fp32 accumulation.
+__global__ void __launch_bounds__(256) k_mlp_forward(const float *X, int P,
+                                                     const __hip_bfloat16 *wq, float *out) {
+    __shared__ float sw[GG_NPARAMS];
+    for (int i = threadIdx.x; i < GG_NPARAMS; i += 256) sw[i] = bf(wq[i]);
+    __syncthreads();
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= P) return;
+    float x[GG_FEATS];
+#pragma unroll
+    for (int i = 0; i < GG_FEATS; i++) x[i] = X[(size_t)p * GG_FEATS + i];
+    float o = sw[OFF_B2];
+    for (int h = 0; h < GG_HIDDEN; h++) {
+        float acc = sw[OFF_B1 + h];
+#pragma unroll
+        for (int i = 0; i < GG_FEATS; i++) acc = fmaf(sw[OFF_W1 + h * GG_FEATS + i], x[i], acc);
+        o = fmaf(sw[OFF_W2 + h], acc > 0.f ? acc : 0.f, o);
+    }
+    out[p] = o;
+}
+
+// Gradient of the mean squared error over a minibatch of B samples (rows
+// idx[0..B) of X/Y), forward with the bf16 weights, fp32 accumulation.
+// One workgroup.  pre[] is scratch [B][H].
+__global__ void __launch_bounds__(256) k_mlp_grad(const float *X, const float *Y, const int *idx,
+                                                  int B, const __hip_bfloat16 *wq, float *pre,
+                                                  float *grad, float *loss) {
+    __shared__ float sw[GG_NPARAMS];
+    __shared__ float g[GG_MAX_BATCH];
+    __shared__ float sl[4];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < GG_NPARAMS; i += 256) sw[i] = bf(wq[i]);
+    __syncthreads();
+    float lsum = 0.f;
+    for (int b = tid; b < B; b += 256) {
+        const int r = idx ? idx[b] : b;
+        float x[GG_FEATS];
+#pragma unroll
+        for (int i = 0; i < GG_FEATS; i++) x[i] = X[(size_t)r * GG_FEATS + i];
+        float o = sw[OFF_B2];
+        for (int h = 0; h < GG_HIDDEN; h++) {
+            float acc = sw[OFF_B1 + h];
+#pragma unroll
+            for (int i = 0; i < GG_FEATS; i++) acc = fmaf(sw[OFF_W1 + h * GG_FEATS + i], x[i], acc);
+            pre[(size_t)b * GG_HIDDEN + h] = acc;
+            o = fmaf(sw[OFF_W2 + h], acc > 0.f ? acc : 0.f, o);
+        }
+        const float e = o - Y[r];
+        lsum += e * e;
+        g[b] = 2.f * e / (float)B;
+    }
+    for (int o = 32; o > 0; o >>= 1) lsum += __shfl_xor(lsum, o, 64);
+    if ((tid & 63) == 0) sl[tid >> 6] = lsum;
+    __syncthreads();
+    if (tid == 0 && loss) *loss = (sl[0] + sl[1] + sl[2] + sl[3]) / (float)B;
+    // per hidden unit: threads h and h+128 split the feature range
+    const int h = tid & (GG_HIDDEN - 1), half = tid >> 7;
+    const float w2 = sw[OFF_W2 + h];
+    float gw1[GG_FEATS / 2];
+#pragma unroll
+    for (int i = 0; i < GG_FEATS / 2; i++) gw1[i] = 0.f;
+    float gb1 = 0.f, gw2 = 0.f;
+    for (int b = 0; b < B; b++) {
+        const int r = idx ? idx[b] : b;
+        const float a = pre[(size_t)b * GG_HIDDEN + h];
+        if (a <= 0.f) continue;
+        const float gb = g[b];
+        gw2 += gb * a;
+        const float gh = gb * w2;
+        gb1 += gh;
+#pragma unroll
+        for (int i = 0; i < GG_FEATS / 2; i++)
+            gw1[i] = fmaf(gh, X[(size_t)r * GG_FEATS + half * (GG_FEATS / 2) + i], gw1[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < GG_FEATS / 2; i++)
+        grad[OFF_W1 + h * GG_FEATS + half * (GG_FEATS / 2) + i] = gw1[i];
+    if (half == 0) {
+        grad[OFF_B1 + h] = gb1;
+        grad[OFF_W2 + h] = gw2;
+    }
+    if (tid == 0) {
+        float gb2 = 0.f;
+        for (int b = 0; b < B; b++) gb2 += g[b];
+        grad[OFF_B2] = gb2;
+    }
+}
+
+// Adam step on the fp32 masters, then refresh the bf16 copies.
+__global__ void __launch_bounds__(256) k_mlp_adam(float *w, float *m, float *v,
+                                                  const float *grad, __hip_bfloat16 *wq, float lr,
+                                                  float b1t, float b2t) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= GG_NPARAMS) return;
+    const float beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
+    const float gi = grad[i];
+    const float mi = beta1 * m[i] + (1.f - beta1) * gi;
+    const float vi = beta2 * v[i] + (1.f - beta2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float mh = mi / (1.f - b1t), vh = vi / (1.f - b2t);
+    const float wi = w[i] - lr * mh / (sqrtf(vh) + eps);
+    w[i] = wi;
+    wq[i] = __float2bfloat16(wi);
+}
+
+__global__ void k_to_bf16(const float *w, __hip_bfloat16 *wq, int n) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) wq[i] = __float2bfloat16(w[i]);
+}
+
+// ---- host objects ------------------------------------------------------------------------
+static int mlp_alloc(gg_mlp *m) {
+    GG_HIP(hipSetDevice(m->device));
+    GG_HIP(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+    const size_t P = GG_NPARAMS;
+    if (hipMalloc((void **)&m->d_w, 4 * P) != hipSuccess) return GG_OOM;
+    if (hipMalloc((void **)&m->d_m, 4 * P) != hipSuccess) return GG_OOM;
+    if (hipMalloc((void **)&m->d_v, 4 * P) != hipSuccess) return GG_OOM;
+    if (hipMalloc((void **)&m->d_g, 4 * P) != hipSuccess) return GG_OOM;
+    if (hipMalloc((void **)&m->d_wq, 2 * P) != hipSuccess) return GG_OOM;
+    if (hipMalloc((void **)&m->d_pre, 4 * (size_t)GG_MAX_BATCH * GG_HIDDEN) != hipSuccess)
+        return GG_OOM;
+    if (hipMalloc((void **)&m->d_loss, 4) != hipSuccess) return GG_OOM;
+    m->xcap = 0;
+    return GG_OK;
+}
+
+static int ensure_io(gg_mlp *m, int n) {
+    if (n <= m->xcap) return GG_OK;
+    if (m->d_x) (void)hipFree(m->d_x);
+    if (m->d_y) (void)hipFree(m->d_y);
+    if (m->d_out) (void)hipFree(m->d_out);
+    m->d_x = nullptr; m->d_y = nullptr; m->d_out = nullptr;
+    if (hipMalloc((void **)&m->d_x, 4 * (size_t)n * GG_FEATS) != hipSuccess) return GG_OOM;
+    if (hipMalloc((void **)&m->d_y, 4 * (size_t)n) != hipSuccess) return GG_OOM;
+    if (hipMalloc((void **)&m->d_out, 4 * (size_t)n) != hipSuccess) return GG_OOM;
+    m->xcap = n;
+    return GG_OK;
+}
+
+// He-style init from the shared explicit RNG (deterministic per seed).
+int gg::mlp_init_params(std::vector<float> &w, uint64_t seed) {
+    gg::Rng rng(seed);
+    w.assign(GG_NPARAMS, 0.f);
+    const double s1 = std::sqrt(2.0 / GG_FEATS), s2 = std::sqrt(1.0 / GG_HIDDEN);
+    auto normal = [&]() {  // Box-Muller on two uniforms
+        double u1 = rng.uniform(), u2 = rng.uniform();
+        if (u1 < 1e-300) u1 = 1e-300;
+        return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+    };
+    for (int i = 0; i < GG_HIDDEN * GG_FEATS; i++) w[OFF_W1 + i] = (float)(normal() * s1);
+    for (int i = 0; i < GG_HIDDEN; i++) w[OFF_W2 + i] = (float)(normal() * s2);
+    return GG_OK;
+}
+
+extern "C" int gg_mlp_create(int device, uint64_t seed, gg_mlp **out) {
+    if (!out) return GG_BAD_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return GG_NO_DEVICE;
+    if (device < 0 || device >= n) return GG_BAD_ARG;
+    gg_mlp *m = new gg_mlp();
+    m->device = device;
+    int st = mlp_alloc(m);
+    if (st == GG_OK) {
+        std::vector<float> w;
+        gg::mlp_init_params(w, seed);
+        st = gg_mlp_set_params(m, w.data());
+    }
+    if (st != GG_OK) { gg_mlp_destroy(m); return st; }
+    *out = m;
+    return GG_OK;
+}
+
+extern "C" void gg_mlp_destroy(gg_mlp *m) {
+    if (!m) return;
+    (void)hipSetDevice(m->device);
+    if (m->stream) (void)hipStreamSynchronize(m->stream);
+    void *bufs[] = {m->d_w, m->d_m, m->d_v, m->d_g, m->d_wq, m->d_pre, m->d_loss, m->d_x,
+                    m->d_y, m->d_out};
+    for (void *p : bufs)
+        if (p) (void)hipFree(p);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
+    delete m;
+}
+
+extern "C" int gg_mlp_set_params(gg_mlp *m, const float *params) {
+    if (!m || !params) return GG_BAD_ARG;
+    GG_HIP(hipSetDevice(m->device));
+    GG_HIP(hipMemcpyAsync(m->d_w, params, 4 * GG_NPARAMS, hipMemcpyHostToDevice, m->stream));
+    hipLaunchKernelGGL(k_to_bf16, dim3((GG_NPARAMS + 255) / 256), dim3(256), 0, m->stream, m->d_w,
+                       m->d_wq, GG_NPARAMS);
+    GG_HIP(hipGetLastError());
+    std::vector<float> z(GG_NPARAMS, 0.f);
+    GG_HIP(hipMemcpyAsync(m->d_m, z.data(), 4 * GG_NPARAMS, hipMemcpyHostToDevice, m->stream));
+    GG_HIP(hipMemcpyAsync(m->d_v, z.data(), 4 * GG_NPARAMS, hipMemcpyHostToDevice, m->stream));
+    GG_HIP(hipStreamSynchronize(m->stream));
+    m->t = 0;
+    return GG_OK;
+}
+
+extern "C" int gg_mlp_get_params(gg_mlp *m, float *params) {
+    if (!m || !params) return GG_BAD_ARG;
+    GG_HIP(hipSetDevice(m->device));
+    GG_HIP(hipMemcpyAsync(params, m->d_w, 4 * GG_NPARAMS, hipMemcpyDeviceToHost, m->stream));
+    GG_HIP(hipStreamSynchronize(m->stream));
+    return GG_OK;
+}
+
+int gg::mlp_forward_dev(gg_mlp *m, const float *d_x, int P, float *d_out, hipStream_t st) {
+    hipLaunchKernelGGL(k_mlp_forward, dim3((P + 255) / 256), dim3(256), 0, st, d_x, P, m->d_wq,
+                       d_out);
+    GG_HIP(hipGetLastError());
+    return GG_OK;
+}
+
+int gg::mlp_grad_dev(gg_mlp *m, const float *d_x, const float *d_y, const int *d_idx, int B,
+                     hipStream_t st) {
+    if (B < 1 || B > GG_MAX_BATCH) return GG_BAD_ARG;
+    hipLaunchKernelGGL(k_mlp_grad, dim3(1), dim3(256), 0, st, d_x, d_y, d_idx, B, m->d_wq,
+                       m->d_pre, m->d_g, m->d_loss);
+    GG_HIP(hipGetLastError());
+    return GG_OK;
+}
+
+int gg::mlp_adam_dev(gg_mlp *m, float lr, hipStream_t st) {
+    m->t++;
+    const float b1t = std::pow(0.9f, (float)m->t), b2t = std::pow(0.999f, (float)m->t);
+    hipLaunchKernelGGL(k_mlp_adam, dim3((GG_NPARAMS + 255) / 256), dim3(256), 0, st, m->d_w,
+                       m->d_m, m->d_v, m->d_g, m->d_wq, lr, b1t, b2t);
+    GG_HIP(hipGetLastError());
+    return GG_OK;
+}
+
+extern "C" int gg_mlp_forward(gg_mlp *m, const float *X, int P, float *out) {
+    if (!m || !X || !out || P < 1) return GG_BAD_ARG;
+    GG_HIP(hipSetDevice(m->device));
+    int st = ensure_io(m, P);
+    if (st != GG_OK) return st;
+    GG_HIP(hipMemcpyAsync(m->d_x, X, 4 * (size_t)P * GG_FEATS, hipMemcpyHostToDevice, m->stream));
+    st = mlp_forward_dev(m, m->d_x, P, m->d_out, m->stream);
+    if (st != GG_OK) return st;
+    GG_HIP(hipMemcpyAsync(out, m->d_out, 4 * (size_t)P, hipMemcpyDeviceToHost, m->stream));
+    GG_HIP(hipStreamSynchronize(m->stream));
+    return GG_OK;
+}
+
+extern "C" int gg_mlp_grad(gg_mlp *m, const float *X, const float *Y, int B, float *loss,
+                           float *grad) {
+    if (!m || !X || !Y || !grad || B < 1 || B > GG_MAX_BATCH) return GG_BAD_ARG;
+    GG_HIP(hipSetDevice(m->device));
+    int st = ensure_io(m, B);
+    if (st != GG_OK) return st;
+    GG_HIP(hipMemcpyAsync(m->d_x, X, 4 * (size_t)B * GG_FEATS, hipMemcpyHostToDevice, m->stream));
+    GG_HIP(hipMemcpyAsync(m->d_y, Y, 4 * (size_t)B, hipMemcpyHostToDevice, m->stream));
+    st = mlp_grad_dev(m, m->d_x, m->d_y, nullptr, B, m->stream);
+    if (st != GG_OK) return st;
+    GG_HIP(hipMemcpyAsync(grad, m->d_g, 4 * GG_NPARAMS, hipMemcpyDeviceToHost, m->stream));
+    if (loss) GG_HIP(hipMemcpyAsync(loss, m->d_loss, 4, hipMemcpyDeviceToHost, m->stream));
+    GG_HIP(hipStreamSynchronize(m->stream));
+    return GG_OK;
+}
+
+extern "C" int gg_mlp_train(gg_mlp *m, const float *X, const float *Y, int B, float lr,
+                            float *loss) {
+    if (!m || !X || !Y || B < 1 || B > GG_MAX_BATCH || !(lr > 0)) return GG_BAD_ARG;
+    GG_HIP(hipSetDevice(m->device));
+    int st = ensure_io(m, B);
+    if (st != GG_OK) return st;
+    GG_HIP(hipMemcpyAsync(m->d_x, X, 4 * (size_t)B * GG_FEATS, hipMemcpyHostToDevice, m->stream));
+    GG_HIP(hipMemcpyAsync(m->d_y, Y, 4 * (size_t)B, hipMemcpyHostToDevice, m->stream));
+    st = mlp_grad_dev(m, m->d_x, m->d_y, nullptr, B, m->stream);
+    if (st != GG_OK) return st;
+    st = mlp_adam_dev(m, lr, m->stream);
+    if (st != GG_OK) return st;
+    if (loss) GG_HIP(hipMemcpyAsync(loss, m->d_loss, 4, hipMemcpyDeviceToHost, m->stream));
+    GG_HIP(hipStreamSynchronize(m->stream));
+    return GG_OK;
+}
+
+// Features of P moves (a, b, c, d, kind) on the batch evaluator's base.
+int gg::features_dev(gg_batch *b, const int *d_moves, int P, float *d_x, hipStream_t st) {
+    if (!b->sparse_ok) return GG_UNSUPPORTED;
+    FeatArgs f{};
+    f.N = b->N; f.W = b->W; f.S = b->snap_cap;
+    f.diam = b->base_res.diameter > 0 ? b->base_res.diameter : 1;
+    f.d = b->degree;
+    f.snap = b->d_snap;
+    f.base_parts = b->d_parts + (size_t)b->max_k * b->W;
+    f.ecc = b->d_ecc;
+    f.moves = d_moves;
+    f.P = P;
+    f.X = d_x;
+    hipLaunchKernelGGL(k_features, dim3((P + 3) / 4), dim3(256), 0, st, f);
+    GG_HIP(hipGetLastError());
+    return GG_OK;
+}
+
+extern "C" int gg_batch_features(gg_batch *b, const int32_t *moves, int P, float *X) {
+    if (!b || !moves || !X || P < 1 || !b->have_base) return GG_BAD_ARG;
+    for (int p = 0; p < P; p++)
+        for (int j = 0; j < 4; j++)
+            if (moves[p * 5 + j] < 0 || moves[p * 5 + j] >= b->N) return GG_BAD_ARG;
+    GG_HIP(hipSetDevice(b->device));
+    int *d_m = nullptr;
+    float *d_x = nullptr;
+    if (hipMalloc((void **)&d_m, 4 * (size_t)P * 5) != hipSuccess) return GG_OOM;
+    if (hipMalloc((void **)&d_x, 4 * (size_t)P * GG_FEATS) != hipSuccess) {
+        (void)hipFree(d_m);
+        return GG_OOM;
+    }
+    int st = GG_OK;
+    if (hipMemcpyAsync(d_m, moves, 4 * (size_t)P * 5, hipMemcpyHostToDevice, b->stream) !=
+        hipSuccess)
+        st = GG_HIP_ERROR;
+    if (st == GG_OK) st = features_dev(b, d_m, P, d_x, b->stream);
+    if (st == GG_OK &&
+        hipMemcpyAsync(X, d_x, 4 * (size_t)P * GG_FEATS, hipMemcpyDeviceToHost, b->stream) !=
+            hipSuccess)
+        st = GG_HIP_ERROR;
+    if (hipStreamSynchronize(b->stream) != hipSuccess) st = GG_HIP_ERROR;
+    (void)hipFree(d_m);
+    (void)hipFree(d_x);
+    return st;
+}

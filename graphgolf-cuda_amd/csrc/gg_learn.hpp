@@ -1,0 +1,37 @@
+// gg_learn.hpp -- learned proposal scorer (host side declarations).
+#pragma once
+#include <vector>
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+#include "gg_rng.hpp"
+#include "graphgolf_learn.h"
+
+#define GG_FEATS 32
+#define GG_HIDDEN 128
+#define GG_NPARAMS (GG_HIDDEN * GG_FEATS + GG_HIDDEN + GG_HIDDEN + 1)
+#define GG_MAX_BATCH 512
+
+struct gg_mlp {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    float *d_w = nullptr, *d_m = nullptr, *d_v = nullptr, *d_g = nullptr;  // fp32 masters, Adam, grad
+    __hip_bfloat16 *d_wq = nullptr;  // bf16 weights used by every forward
+    float *d_pre = nullptr, *d_loss = nullptr;
+    float *d_x = nullptr, *d_y = nullptr, *d_out = nullptr;
+    int xcap = 0;
+    int t = 0;  // Adam step
+};
+
+struct gg_batch;
+
+namespace gg {
+int mlp_init_params(std::vector<float> &w, uint64_t seed);
+int mlp_forward_dev(gg_mlp *m, const float *d_x, int P, float *d_out, hipStream_t st);
+int mlp_grad_dev(gg_mlp *m, const float *d_x, const float *d_y, const int *d_idx, int B,
+                 hipStream_t st);
+int mlp_adam_dev(gg_mlp *m, float lr, hipStream_t st);
+int features_dev(gg_batch *b, const int *d_moves, int P, float *d_x, hipStream_t st);
+int gather_rows_dev(const float *src, const int *idx, int n, float *dst, int dst_cap, int head,
+                    hipStream_t st);
+hipStream_t batch_stream(gg_batch *b);
+}  // namespace gg

@@ -62,6 +62,7 @@ struct BatchBackend : Backend {
         *st = gg_batch_create(N, d, K, device, &b);
     }
     ~BatchBackend() override { gg_batch_destroy(b); }
+    gg_batch *batch() override { return b; }
     int set_base(const int32_t *adj, gg_result *out) override {
         return gg_batch_set_base(b, adj, out);
     }
@@ -178,9 +179,12 @@ extern "C" int gg_search_create(int N, int d, const int32_t *adj, const gg_searc
     if (st != GG_OK) return st;
     gg_search *s = new gg_search();
     if (cfg->mode == GG_MODE_PLAIN) s->be.reset(new PlainBackend(N, d, device, &st));
-    else s->be.reset(new BatchBackend(N, d, std::max(cfg->K, 1), device, &st));
-    if (st == GG_OK && cfg->mode == GG_MODE_LEARNED) st = learn_create(s, device);
+    else if (cfg->mode == GG_MODE_BATCHED)
+        s->be.reset(new BatchBackend(N, d, std::max(cfg->K, 1), device, &st));
+    else  // learned: room for audits that evaluate all P proposals
+        s->be.reset(new BatchBackend(N, d, std::max(cfg->K, cfg->P), device, &st));
     if (st == GG_OK) st = search_init(s, N, d, adj, cfg);
+    if (st == GG_OK && cfg->mode == GG_MODE_LEARNED) st = learn_create(s, device);
     if (st != GG_OK) { gg_search_destroy(s); return st; }
     *out = s;
     return GG_OK;

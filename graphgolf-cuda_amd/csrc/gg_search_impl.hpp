@@ -6,10 +6,13 @@
 #include "graphgolf_search.h"
 #include "gg_rng.hpp"
 
+struct gg_batch;
+
 namespace gg {
 
 struct Backend {
     virtual ~Backend() {}
+    virtual gg_batch *batch() { return nullptr; }
     virtual int set_base(const int32_t *adj, gg_result *out) = 0;
     virtual int eval(const gg_swap *sw, int K, gg_result *out) = 0;
     virtual int commit(const gg_swap *sw, gg_result *out) = 0;

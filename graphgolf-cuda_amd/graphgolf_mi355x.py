@@ -29,6 +29,7 @@ GG_LEVEL_LIMIT = 99
 
 GG_OPT_BLOCK = 1
 GG_OPT_LDS_LIMIT = 2
+GG_OPT_GENERIC = 3
 
 GG_BATCH_OPT_PATH = 1
 GG_BATCH_OPT_GRAPH = 2
@@ -94,6 +95,9 @@ class EvalCallbacks(C.Structure):
 
 
 MODE_PLAIN, MODE_BATCHED, MODE_LEARNED = 0, 1, 2
+MLP_FEATURES, MLP_HIDDEN = 32, 128
+MLP_PARAMS = MLP_HIDDEN * MLP_FEATURES + MLP_HIDDEN + MLP_HIDDEN + 1
+MLP_MAX_BATCH = 512
 RULE_ANNEAL, RULE_HILL = 0, 1
 
 _lib = None
@@ -146,6 +150,18 @@ def lib() -> C.CDLL:
         "gg_search_graph": (C.c_int, [vp, i32p, C.c_int]),
         "gg_search_set_log": (C.c_int, [vp, C.c_int]),
         "gg_search_get_log": (C.c_int, [vp, i64p, C.c_int, P(C.c_int)]),
+        "gg_search_audit": (C.c_int, [vp, C.c_uint64, P(C.c_double), P(C.c_double),
+                                      P(C.c_float), P(C.c_double)]),
+        "gg_mlp_create": (C.c_int, [C.c_int, C.c_uint64, P(vp)]),
+        "gg_mlp_destroy": (None, [vp]),
+        "gg_mlp_set_params": (C.c_int, [vp, P(C.c_float)]),
+        "gg_mlp_get_params": (C.c_int, [vp, P(C.c_float)]),
+        "gg_mlp_forward": (C.c_int, [vp, P(C.c_float), C.c_int, P(C.c_float)]),
+        "gg_mlp_grad": (C.c_int, [vp, P(C.c_float), P(C.c_float), C.c_int, P(C.c_float),
+                                  P(C.c_float)]),
+        "gg_mlp_train": (C.c_int, [vp, P(C.c_float), P(C.c_float), C.c_int, C.c_float,
+                                   P(C.c_float)]),
+        "gg_batch_features": (C.c_int, [vp, i32p, C.c_int, P(C.c_float)]),
         "gg_gen_random_regular": (C.c_int, [C.c_int, C.c_int, C.c_uint64, i32p]),
         "gg_gen_orbit_part": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_uint64, i32p]),
     }
@@ -338,6 +354,14 @@ class BatchEvaluator:
         check(lib().gg_batch_last_kernel_ms(self._h, C.byref(ms)), "gg_batch_last_kernel_ms")
         return ms.value
 
+    def features(self, moves) -> np.ndarray:
+        """32 features per move row (a, b, c, d, kind) on the current base."""
+        m = _i32(moves).reshape(-1, 5)
+        X = np.empty((m.shape[0], MLP_FEATURES), dtype=np.float32)
+        check(lib().gg_batch_features(self._h, _ptr(m), m.shape[0], _ptr(X, C.c_float)),
+              "gg_batch_features")
+        return X
+
     def set_option(self, opt: int, value: int):
         check(lib().gg_batch_set_option(self._h, opt, value), "gg_batch_set_option")
 
@@ -424,6 +448,7 @@ class Search:
     def __init__(self, adj, cfg: SearchCfg, device: int = 0, callbacks=None, log: int = 0):
         a = _i32(adj)
         self.N, self.d = a.shape
+        self.P = cfg.P
         self._h = C.c_void_p()
         self._cb = callbacks  # keep the ctypes callbacks alive
         if callbacks is None:
@@ -453,6 +478,16 @@ class Search:
         check(lib().gg_search_get_log(self._h, _ptr(buf, C.c_int64), n.value, C.byref(n)),
               "gg_search_get_log")
         return buf
+
+    def audit(self, seed: int):
+        """Learned mode: (spearman, miss_rate, scores[P], exact_deltas[P])."""
+        P = self.P
+        sc = np.empty(P, dtype=np.float32)
+        dl = np.empty(P, dtype=np.float64)
+        rho, miss = C.c_double(), C.c_double()
+        check(lib().gg_search_audit(self._h, seed, C.byref(rho), C.byref(miss),
+                                    _ptr(sc, C.c_float), _ptr(dl, C.c_double)), "gg_search_audit")
+        return rho.value, miss.value, sc, dl
 
     def close(self):
         if self._h:
@@ -494,3 +529,56 @@ class PyEvaluatorCallbacks:
 
         self._fns = (SET_BASE_CB(set_base), EVAL_CB(ev), COMMIT_CB(commit))
         self.struct = EvalCallbacks(None, *self._fns)
+
+
+class MLP:
+    """32-128-1 proposal scorer (bf16 weights, fp32 accumulation) on the GPU."""
+
+    def __init__(self, seed: int = 1, device: int = 0):
+        self._h = C.c_void_p()
+        check(lib().gg_mlp_create(device, seed, C.byref(self._h)), "gg_mlp_create")
+
+    def close(self):
+        if self._h:
+            lib().gg_mlp_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self) -> np.ndarray:
+        w = np.empty(MLP_PARAMS, dtype=np.float32)
+        check(lib().gg_mlp_get_params(self._h, _ptr(w, C.c_float)), "gg_mlp_get_params")
+        return w
+
+    def set_params(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        assert w.size == MLP_PARAMS
+        check(lib().gg_mlp_set_params(self._h, _ptr(w, C.c_float)), "gg_mlp_set_params")
+
+    def forward(self, X) -> np.ndarray:
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, MLP_FEATURES)
+        out = np.empty(X.shape[0], dtype=np.float32)
+        check(lib().gg_mlp_forward(self._h, _ptr(X, C.c_float), X.shape[0], _ptr(out, C.c_float)),
+              "gg_mlp_forward")
+        return out
+
+    def grad(self, X, Y):
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, MLP_FEATURES)
+        Y = np.ascontiguousarray(Y, dtype=np.float32)
+        g = np.empty(MLP_PARAMS, dtype=np.float32)
+        loss = C.c_float()
+        check(lib().gg_mlp_grad(self._h, _ptr(X, C.c_float), _ptr(Y, C.c_float), X.shape[0],
+                                C.byref(loss), _ptr(g, C.c_float)), "gg_mlp_grad")
+        return loss.value, g
+
+    def train(self, X, Y, lr=1e-3) -> float:
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, MLP_FEATURES)
+        Y = np.ascontiguousarray(Y, dtype=np.float32)
+        loss = C.c_float()
+        check(lib().gg_mlp_train(self._h, _ptr(X, C.c_float), _ptr(Y, C.c_float), X.shape[0], lr,
+                                 C.byref(loss)), "gg_mlp_train")
+        return loss.value

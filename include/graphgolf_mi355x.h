@@ -112,7 +112,10 @@ int gg_eval_adj_commit(gg_eval *ev, const gg_swap *sw);
 
 /* Per-source eccentricities (M entries, source order: part/adj = vertex id;
  * piece = x*My+y; box = (x*My+y)*Mz+z) and per-level reach counts (pairs at
- * distance exactly l, l < reach_cap) of the most recent evaluation.  Either
+ * distance exactly l, l < reach_cap) of the most recent evaluation.  The BFS
+ * pulls from each vertex's listed neighbours (as cudaASPLconv.cu:65-67 does),
+ * so on a table that describes a directed graph these are in-eccentricities;
+ * on undirected graphs (all the reference's drivers produce) in = out.  Either
  * pointer may be NULL.  No reference counterpart (the reference never exposes
  * them); used by the proposal features. */
 int gg_eval_levels(gg_eval *ev, int32_t *ecc_out, int64_t *reach_out, int reach_cap);
@@ -125,6 +128,8 @@ int gg_eval_last_kernel_ms(gg_eval *ev, float *ms);
 #define GG_OPT_BLOCK      1   /* threads per workgroup: 256 or 1024                  */
 #define GG_OPT_LDS_LIMIT  2   /* max LDS bytes for the column bitset; a smaller value
                                  forces the global-memory kernel (used by tests)      */
+#define GG_OPT_GENERIC    3   /* 1 = runtime-degree kernel even when the degree has a
+                                 compile-time specialisation (used by tests)          */
 int gg_eval_set_option(gg_eval *ev, int option, int value);
 int gg_eval_get_option(gg_eval *ev, int option, int *value);
 

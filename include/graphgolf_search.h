@@ -74,6 +74,14 @@ int gg_search_graph(gg_search *s, int32_t *adj_out, int best);
 int gg_search_set_log(gg_search *s, int capacity);
 int gg_search_get_log(gg_search *s, int64_t *buf, int capacity, int *count);
 
+/* Learned mode only: draw P fresh proposals with an RNG seeded by `seed`
+ * (the search state is unchanged), score them, evaluate all P exactly, and
+ * report the scorer's Spearman rank correlation with the exact change of the
+ * objective and the fraction of the exact top-K it misses.  scores_out /
+ * delta_out (P entries each) may be NULL. */
+int gg_search_audit(gg_search *s, uint64_t seed, double *spearman, double *miss_rate,
+                    float *scores_out, double *delta_out);
+
 #ifdef __cplusplus
 }
 #endif
