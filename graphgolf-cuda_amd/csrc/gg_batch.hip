@@ -18,19 +18,15 @@
 
 using namespace gg;
 
-static const int kSparseWaves = 8;
+static const int kSparseBlock = 1024;
 static const int kSnapCapDefault = 8;
 static const size_t kSnapBytesMax = (size_t)8 << 30;
+static const size_t kOverrideBytesMax = (size_t)4 << 30;
+static const int kDeltaCap = 2048;
 
-// LDS hash slots per wave for the sparse kernel, or 0 if it does not fit.
-static int sparse_hash_slots(const gg_batch *b) {
-    const size_t bits = (size_t)b->N * 8;
-    const size_t avail = (size_t)b->dc.lds_device - 1024;
-    for (int hs : {256, 128}) {
-        size_t need = bits + (size_t)kSparseWaves * 2 * hs * 12;
-        if (need <= avail) return hs;
-    }
-    return 0;
+// The sparse kernel keeps one column (N words) in LDS plus small bookkeeping.
+static bool sparse_fits(const gg_batch *b) {
+    return (size_t)b->N * 8 + 4096 <= (size_t)b->dc.lds_device;
 }
 
 static int batch_alloc(gg_batch *b) {
@@ -43,7 +39,7 @@ static int batch_alloc(gg_batch *b) {
     b->W = (N + 63) / 64;
     const int W = b->W;
     b->narrow = N <= 65535;
-    size_t tb = (size_t)N * D * (b->narrow ? 2 : 4);
+    size_t tb = b->narrow ? (size_t)quads(D) * N * 8 : (size_t)N * D * 4;
     if (hipMalloc(&b->d_tab, tb) != hipSuccess) return GG_OOM;
     if (hipHostMalloc(&b->h_tab, tb, hipHostMallocDefault) != hipSuccess) return GG_OOM;
     b->tab_bytes = tb;
@@ -69,24 +65,29 @@ static int batch_alloc(gg_batch *b) {
     GG_HIP(hipMemcpyAsync(b->d_src, src.data(), sizeof(int) * N, hipMemcpyHostToDevice, b->stream));
     b->dc.lds_device = device_lds_limit(b->device);
     b->dc.cu_count = device_cu_count(b->device);
-    b->dc.topo_kind = b->narrow ? TOPO_RING16 : TOPO_RING32;
-    b->dc.ring16 = TopoRing<uint16_t>{(const uint16_t *)b->d_tab, (unsigned)N, (unsigned)N};
+    b->dc.topo_kind = b->narrow ? TOPO_RINGQ : TOPO_RING32;
+    b->dc.ringq = TopoRingQ{(const uint64_t *)b->d_tab, (unsigned)N, (unsigned)N};
     b->dc.ring32 = TopoRing<uint32_t>{(const uint32_t *)b->d_tab, (unsigned)N, (unsigned)N};
 
-    // sparse-path buffers (only when the column bitset and hashes fit in LDS)
-    const int hs = sparse_hash_slots(b);
-    if (hs > 0 && dense_fits_lds(b->dc, N)) {
+    // sparse-path buffers (only when the column bitset fits in LDS)
+    if (sparse_fits(b) && dense_fits_lds(b->dc, N)) {
         size_t per_level = (size_t)W * N * 8;
         int cap = kSnapCapDefault;
         while (cap > 1 && per_level * cap > kSnapBytesMax) cap--;
         b->snap_cap = cap;
-        b->cap = hs / 2;
-        const size_t T = (size_t)K * W;
+        b->cap = kDeltaCap < N ? kDeltaCap : N;
+        const int NB = (N + 31) / 32;
+        const size_t per_task = 2 * ((size_t)N * 8 + (size_t)NB * 4 + (size_t)b->cap * 4);
+        int chunk = K < GG_SPARSE_MAX_K ? K : GG_SPARSE_MAX_K;
+        while (chunk > 1 && per_task * W * chunk > kOverrideBytesMax) chunk--;
+        b->chunk = chunk;
+        const size_t T = (size_t)K * W, TC = (size_t)chunk * W;
         if (hipMalloc((void **)&b->d_snap, per_level * cap) != hipSuccess) return GG_OOM;
-        if (hipMalloc((void **)&b->d_dl_key, sizeof(int) * T * 2 * b->cap) != hipSuccess)
+        if (hipMalloc((void **)&b->d_dl_key, sizeof(int) * TC * 2 * b->cap) != hipSuccess)
             return GG_OOM;
-        if (hipMalloc((void **)&b->d_dl_val, 8 * T * 2 * b->cap) != hipSuccess) return GG_OOM;
-        if (hipMalloc((void **)&b->d_dl_cnt, sizeof(int) * T * 2) != hipSuccess) return GG_OOM;
+        if (hipMalloc((void **)&b->d_dl_bits, 4 * TC * 2 * NB) != hipSuccess) return GG_OOM;
+        if (hipMalloc((void **)&b->d_dl_ovr, 8 * TC * 2 * (size_t)N) != hipSuccess) return GG_OOM;
+        if (hipMalloc((void **)&b->d_dl_cnt, sizeof(int) * TC * 2) != hipSuccess) return GG_OOM;
         if (hipMalloc((void **)&b->d_task_state, sizeof(int) * T) != hipSuccess) return GG_OOM;
         if (hipMalloc((void **)&b->d_task_end, sizeof(int) * T) != hipSuccess) return GG_OOM;
         if (hipMalloc((void **)&b->d_task_flags, sizeof(int) * T) != hipSuccess) return GG_OOM;
@@ -96,6 +97,7 @@ static int batch_alloc(gg_batch *b) {
         if (hipMalloc((void **)&b->d_dense_parts, sizeof(ColPart) * T) != hipSuccess)
             return GG_OOM;
         GG_HIP(hipMemsetAsync(b->d_qctl, 0, sizeof(int) * 2, b->stream));
+        GG_HIP(hipMemsetAsync(b->d_dl_bits, 0, 4 * TC * 2 * NB, b->stream));
     }
     GG_HIP(hipStreamSynchronize(b->stream));
     return GG_OK;
@@ -130,7 +132,8 @@ extern "C" void gg_batch_destroy(gg_batch *b) {
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     drop_graph(b);
     void *bufs[] = {b->d_tab, b->d_adjR, b->d_src, b->d_parts, b->d_res, b->d_done, b->d_swaps,
-                    b->d_ecc, b->d_base_reach, b->d_snap, b->d_dl_key, b->d_dl_val, b->d_dl_cnt,
+                    b->d_ecc, b->d_base_reach, b->d_snap, b->d_dl_key, b->d_dl_bits, b->d_dl_ovr,
+                    b->d_dl_cnt,
                     b->d_task_state, b->d_task_end, b->d_task_flags, b->d_dcnt, b->d_queue,
                     b->d_qctl, b->d_dense_parts};
     for (void *p : bufs)
@@ -202,11 +205,12 @@ extern "C" int gg_batch_set_base(gg_batch *b, const int32_t *adj, gg_result *out
     const int N = b->N, D = b->degree;
     for (size_t i = 0; i < (size_t)N * D; i++)
         if (adj[i] < 0 || adj[i] >= N) return GG_BAD_ARG;
+    if (b->narrow) memset(b->h_tab, 0, b->tab_bytes);
     for (int v = 0; v < N; v++)
         for (int i = 0; i < D; i++) {
             long long u = adj[(size_t)v * D + i];
             long long off = ((u - v) % N + N) % N;
-            if (b->narrow) ((uint16_t *)b->h_tab)[(size_t)i * N + v] = (uint16_t)off;
+            if (b->narrow) ((uint16_t *)b->h_tab)[quad_index(i, N, v)] = (uint16_t)off;
             else ((uint32_t *)b->h_tab)[(size_t)i * N + v] = (uint32_t)off;
         }
     GG_HIP(hipMemcpyAsync(b->d_tab, b->h_tab, b->tab_bytes, hipMemcpyHostToDevice, b->stream));
@@ -227,18 +231,29 @@ static int check_swaps(const gg_batch *b, const gg_swap *sw, int K) {
     return GG_OK;
 }
 
-template <int NW, int HS>
+template <int DT>
 static hipError_t launch_sparse_inst(const SparseArgs &s, size_t lds, hipStream_t st) {
     static int configured = -1;
-    auto fn = &k_sparse<NW, HS>;
+    auto fn = &k_sparse<kSparseBlock, DT>;
     if ((int)lds > configured) {
         hipError_t e = hipFuncSetAttribute((const void *)fn,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
         configured = (int)lds;
     }
-    hipLaunchKernelGGL(fn, dim3(s.W), dim3(NW * 64), lds, st, s);
+    hipLaunchKernelGGL(fn, dim3(s.W), dim3(kSparseBlock), lds, st, s);
     return hipGetLastError();
+}
+
+static hipError_t launch_sparse(const SparseArgs &s, size_t lds, hipStream_t st) {
+    switch (s.D) {
+        case 4: return launch_sparse_inst<4>(s, lds, st);
+        case 8: return launch_sparse_inst<8>(s, lds, st);
+        case 12: return launch_sparse_inst<12>(s, lds, st);
+        case 16: return launch_sparse_inst<16>(s, lds, st);
+        case 24: return launch_sparse_inst<24>(s, lds, st);
+        default: return launch_sparse_inst<0>(s, lds, st);
+    }
 }
 
 // Enqueues the sparse pipeline for K candidates whose swaps are in d_swaps.
@@ -254,8 +269,9 @@ static int enqueue_sparse(gg_batch *b, int K) {
     s.base_reach = b->d_base_reach;
     s.cap = b->cap;
     s.dl_key = b->d_dl_key;
-    s.dl_val = b->d_dl_val;
     s.dl_cnt = b->d_dl_cnt;
+    s.dl_bits = b->d_dl_bits;
+    s.dl_ovr = b->d_dl_ovr;
     s.dcnt = b->d_dcnt;
     s.task_state = b->d_task_state;
     s.task_end = b->d_task_end;
@@ -266,9 +282,16 @@ static int enqueue_sparse(gg_batch *b, int K) {
     s.dense_parts = b->d_dense_parts;
     s.res = b->d_res;
     const size_t lds = (size_t)N * 8;
-    hipError_t e = b->cap * 2 == 256 ? launch_sparse_inst<kSparseWaves, 256>(s, lds, b->stream)
-                                     : launch_sparse_inst<kSparseWaves, 128>(s, lds, b->stream);
-    if (e != hipSuccess) { gg_set_last_hip_error(e, "k_sparse", __FILE__, __LINE__); return GG_HIP_ERROR; }
+    for (int k0 = 0; k0 < K; k0 += b->chunk) {
+        SparseArgs c = s;
+        c.k0 = k0;
+        c.K = K - k0 < b->chunk ? K - k0 : b->chunk;
+        hipError_t e = launch_sparse(c, lds, b->stream);
+        if (e != hipSuccess) {
+            gg_set_last_hip_error(e, "k_sparse", __FILE__, __LINE__);
+            return GG_HIP_ERROR;
+        }
+    }
     // dense fallback for queued tasks
     DenseArgs a = dense_args(b, K, b->d_swaps, b->d_dense_parts, nullptr, nullptr, nullptr);
     a.queue = b->d_queue;
@@ -346,7 +369,7 @@ __global__ void k_batch_commit(uint16_t *tab16, uint32_t *tab32, int *adjR, int 
         if (v < 0) continue;
         int off = sw->nbr[j] - v;
         if (off < 0) off += N;
-        if (tab16) tab16[(size_t)sw->slot[j] * N + v] = (uint16_t)off;
+        if (tab16) tab16[quad_index(sw->slot[j], N, v)] = (uint16_t)off;
         else tab32[(size_t)sw->slot[j] * N + v] = (uint32_t)off;
         adjR[(size_t)v * D + sw->slot[j]] = sw->nbr[j];
     }

@@ -22,8 +22,10 @@ struct gg_batch {
     int snap_cap = 0;                         // base levels whose words are stored
     unsigned long long *d_snap = nullptr;     // [snap_cap][W][N]
     int cap = 0;                              // Delta list capacity per task
+    int chunk = 0;                            // candidates per sparse launch
     int *d_dl_key = nullptr;
-    unsigned long long *d_dl_val = nullptr;
+    unsigned *d_dl_bits = nullptr;            // [chunk*W][2][ceil(N/32)]
+    unsigned long long *d_dl_ovr = nullptr;   // [chunk*W][2][N]
     int *d_dl_cnt = nullptr, *d_task_state = nullptr, *d_task_end = nullptr,
         *d_task_flags = nullptr;
     long long *d_dcnt = nullptr;

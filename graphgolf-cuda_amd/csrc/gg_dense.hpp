@@ -22,6 +22,7 @@
 // candidate (agent-scope counter, reset by that workgroup) reduces the parts
 // into the candidate's ResultRec, so one launch yields the final answer.
 #pragma once
+#include <type_traits>
 #include "gg_common.hpp"
 
 // ---- topologies ---------------------------------------------------------------
@@ -38,6 +39,24 @@ struct TopoRing {
     }
     __device__ __forceinline__ unsigned nbr(const Ctx &c, int i) const {
         unsigned u = c.v + (unsigned)tab[(size_t)i * R + c.row];
+        return u >= N ? u - N : u;
+    }
+};
+
+// Offset form with 16-bit offsets packed four per 64-bit word (N <= 65535):
+// word (q, r) holds the offsets of slots 4q..4q+3 of row r, slot 4q+j in bits
+// 16j..16j+15.  A vertex's D neighbour indices are ceil(D/4) coalesced 8-byte
+// loads per wave.
+struct TopoRingQ {
+    const uint64_t *tab;  // [ceil(D/4)][R]
+    unsigned N, R;
+    struct Ctx { unsigned v, row; };
+    __device__ __forceinline__ Ctx ctx(unsigned v) const {
+        return Ctx{v, v < R ? v : v % R};
+    }
+    __device__ __forceinline__ unsigned nbr(const Ctx &c, int i) const {
+        uint64_t q = tab[(size_t)(i >> 2) * R + c.row];
+        unsigned u = c.v + ((unsigned)(q >> (16 * (i & 3))) & 0xFFFFu);
         return u >= N ? u - N : u;
     }
 };
@@ -123,23 +142,10 @@ __device__ __forceinline__ void load_swap(const gg_swap *sw, int k, SwapRegs &r,
 
 // Neighbour-OR gather for vertex v with early exit once (acc | own) is full.
 template <class Topo, class Vis>
-__device__ __forceinline__ unsigned long long gather_or(const Topo &topo, Vis vis, unsigned v,
-                                                        unsigned long long own, int D,
-                                                        const SwapRegs &sw, bool has_sw) {
+__device__ __forceinline__ unsigned long long gather_plain(const Topo &topo, Vis vis, unsigned v,
+                                                           unsigned long long own, int D) {
     unsigned long long acc = 0;
     typename Topo::Ctx c = topo.ctx(v);
-    bool hit = has_sw && ((int)v == sw.v[0] || (int)v == sw.v[1] || (int)v == sw.v[2] ||
-                          (int)v == sw.v[3]);
-    if (hit) {
-        for (int i = 0; i < D; i++) {
-            unsigned u = topo.nbr(c, i);
-#pragma unroll
-            for (int j = 0; j < 4; j++)
-                if (sw.v[j] == (int)v && sw.s[j] == i) u = (unsigned)sw.n[j];
-            acc |= vis[u];
-        }
-        return acc;
-    }
     int i = 0;
     for (; i + 4 <= D; i += 4) {
         unsigned u0 = topo.nbr(c, i), u1 = topo.nbr(c, i + 1);
@@ -151,31 +157,68 @@ __device__ __forceinline__ unsigned long long gather_or(const Topo &topo, Vis vi
     return acc;
 }
 
-// Fixed-degree gather: all DT neighbour indices are loaded up front (one
-// memory latency per vertex instead of one per group), then OR-ed from LDS in
-// groups of four with the early exit.
-template <class Topo, int DT>
-struct FixedRow {
-    unsigned u[DT];
-    __device__ __forceinline__ void load(const Topo &topo, unsigned v) {
-        typename Topo::Ctx c = topo.ctx(v);
+// Gather of a vertex whose row a candidate swap patches (slot by slot).
+template <class Topo, class Vis>
+__device__ __forceinline__ unsigned long long gather_patched(const Topo &topo, Vis vis, unsigned v,
+                                                             int D, const SwapRegs &sw) {
+    unsigned long long acc = 0;
+    typename Topo::Ctx c = topo.ctx(v);
+    for (int i = 0; i < D; i++) {
+        unsigned u = topo.nbr(c, i);
 #pragma unroll
-        for (int i = 0; i < DT; i++) u[i] = topo.nbr(c, i);
+        for (int j = 0; j < 4; j++)
+            if (sw.v[j] == (int)v && sw.s[j] == i) u = (unsigned)sw.n[j];
+        acc |= vis[u];
+    }
+    return acc;
+}
+
+// Generic entry used by the global-memory kernel.
+template <class Topo, class Vis>
+__device__ __forceinline__ unsigned long long gather_or(const Topo &topo, Vis vis, unsigned v,
+                                                        unsigned long long own, int D,
+                                                        const SwapRegs &sw, bool has_sw) {
+    bool hit = has_sw && ((int)v == sw.v[0] || (int)v == sw.v[1] || (int)v == sw.v[2] ||
+                          (int)v == sw.v[3]);
+    return hit ? gather_patched(topo, vis, v, D, sw) : gather_plain(topo, vis, v, own, D);
+}
+
+// Fixed-degree gather on the quad-packed ring topology: the ceil(DT/4) words
+// of a vertex are loaded up front (one memory latency per vertex), then the
+// neighbours are OR-ed from LDS in groups of four with the early exit.
+template <int DT>
+struct FixedRow {
+    static constexpr int Q = (DT + 3) / 4;
+    unsigned long long q[Q];
+    unsigned v, n;
+    __device__ __forceinline__ void load(const TopoRingQ &topo, unsigned vv) {
+        v = vv;
+        n = topo.N;
+        const unsigned row = vv < topo.R ? vv : vv % topo.R;
+#pragma unroll
+        for (int j = 0; j < Q; j++) q[j] = topo.tab[(size_t)j * topo.R + row];
+    }
+    __device__ __forceinline__ unsigned u(int i) const {
+        unsigned x = v + ((unsigned)(q[i >> 2] >> (16 * (i & 3))) & 0xFFFFu);
+        return x >= n ? x - n : x;
     }
 };
 
 template <int DT, class Vis>
-__device__ __forceinline__ unsigned long long gather_fixed(const unsigned *u, Vis vis,
+__device__ __forceinline__ unsigned long long gather_fixed(const FixedRow<DT> &row, Vis vis,
                                                            unsigned long long own) {
     unsigned long long acc = 0;
 #pragma unroll
     for (int i = 0; i < DT; i += 4) {
-        unsigned long long g = vis[u[i]];
-        if (i + 1 < DT) g |= vis[u[i + 1]];
-        if (i + 2 < DT) g |= vis[u[i + 2]];
-        if (i + 3 < DT) g |= vis[u[i + 3]];
+        unsigned long long g = vis[row.u(i)];
+        if (i + 1 < DT) g |= vis[row.u(i + 1)];
+        if (i + 2 < DT) g |= vis[row.u(i + 2)];
+        if (i + 3 < DT) g |= vis[row.u(i + 3)];
         acc |= g;
         if (i + 4 < DT && (acc | own) == ~0ull) break;
+        // compiler-only barrier: keeps the next group's LDS loads below the
+        // early exit (hoisting all DT loads costs registers and spills)
+        __asm__ volatile("" ::: "memory");
     }
     return acc;
 }
@@ -266,8 +309,8 @@ __device__ void finish_column(const DenseArgs &a, int k, int w, const ColPart &p
 
 // ---- LDS-resident column BFS ---------------------------------------------------
 // One workgroup runs one (candidate, column) task with vis[N] in dynamic LDS.
-// Thread t owns vertices t, t+BS, ..., so new words stay in registers across
-// the level barrier (no second LDS buffer).  Returns the column part in
+// Each thread owns VPT vertices, so new words stay in registers across the
+// level barrier (no second LDS buffer).  Returns the column part in
 // thread 0.
 template <class Topo, int BS, int VPT, int DT>
 __device__ __forceinline__ ColPart dense_column(const Topo &topo, const DenseArgs &a, int k, int w,
@@ -296,30 +339,53 @@ __device__ __forceinline__ ColPart dense_column(const Topo &topo, const DenseArg
     int my_ecc = 0, last_new = 0, level = 0, flags = 0;
     long long *reach = a.reach_col ? a.reach_col + (size_t)w * (a.lmax + 1) : nullptr;
     if (reach && tid == 0) reach[0] = nvalid;
+    // wave-blocked ownership: wave q owns vertices [64*VPT*q, 64*VPT*(q+1)),
+    // slot j of lane l is vb + 64*j (coalesced per slot, small LDS offsets)
+    const unsigned vb = (unsigned)(tid >> 6) * 64u * VPT + (unsigned)lane;
+    // slots of this lane whose row the candidate's swap patches (one VGPR)
+    unsigned hitmask = 0;
+    if (has_sw) {
+#pragma unroll
+        for (int j = 0; j < VPT; j++) {
+            const int v = (int)(vb + j * 64);
+            if (v == sw.v[0] || v == sw.v[1] || v == sw.v[2] || v == sw.v[3]) hitmask |= 1u << j;
+        }
+    }
     for (level = 1;; level++) {
         unsigned long long nw[VPT];
-        if (DT > 0 && !has_sw) {
-            // fixed degree: the next vertex's indices load while this one gathers
-            FixedRow<Topo, (DT > 0 ? DT : 1)> row[2];
-            if (tid < N) row[0].load(topo, tid);
+        if constexpr (DT > 0 && std::is_same<Topo, TopoRingQ>::value) {
+            // fixed degree: the next vertex's words load while this one gathers
+            FixedRow<DT> row[2];
+            if (vb < N) row[0].load(topo, vb);
 #pragma unroll
             for (int j = 0; j < VPT; j++) {
-                unsigned v = tid + j * BS;
-                if (j + 1 < VPT && v + BS < N) row[(j + 1) & 1].load(topo, v + BS);
+                unsigned v = vb + j * 64;
+                if (j + 1 < VPT && v + 64 < N) row[(j + 1) & 1].load(topo, v + 64);
                 nw[j] = 0;
                 if (v < N) {
                     unsigned long long own = vis[v];
-                    if (~own) nw[j] = gather_fixed<(DT > 0 ? DT : 1)>(row[j & 1].u, vis, own) & ~own;
+                    if (~own) {
+                        unsigned long long acc = ((hitmask >> j) & 1u)
+                                                     ? gather_patched(topo, vis, v, D, sw)
+                                                     : gather_fixed<DT>(row[j & 1], vis, own);
+                        nw[j] = acc & ~own;
+                    }
                 }
+                __asm__ volatile("" ::: "memory");  // one row in flight ahead, no more
             }
         } else {
 #pragma unroll
             for (int j = 0; j < VPT; j++) {
-                unsigned v = tid + j * BS;
+                unsigned v = vb + j * 64;
                 nw[j] = 0;
                 if (v < N) {
                     unsigned long long own = vis[v];
-                    if (~own) nw[j] = gather_or(topo, vis, v, own, D, sw, has_sw) & ~own;
+                    if (~own) {
+                        unsigned long long acc = ((hitmask >> j) & 1u)
+                                                     ? gather_patched(topo, vis, v, D, sw)
+                                                     : gather_plain(topo, vis, v, own, D);
+                        nw[j] = acc & ~own;
+                    }
                 }
             }
         }
@@ -331,7 +397,7 @@ __device__ __forceinline__ ColPart dense_column(const Topo &topo, const DenseArg
                 ? a.snap + ((size_t)(level - 1) * a.W + w) * N : nullptr;
 #pragma unroll
         for (int j = 0; j < VPT; j++) {
-            unsigned v = tid + j * BS;
+            unsigned v = vb + j * 64;
             if (nw[j]) {
                 vis[v] |= nw[j];  // only this thread writes vertex v
                 lor |= nw[j];

@@ -83,7 +83,7 @@ static hipError_t launch_lds(const Topo &topo, const DenseArgs &a, int grid, siz
 
 // Fixed-degree kernels exist for the 16-bit ring topology, 1024-thread
 // workgroups and these degrees (gg_fixed_d*.hip).
-static hipError_t launch_fixed(const TopoRing<uint16_t> &topo, const DenseArgs &a, int grid,
+static hipError_t launch_fixed(const TopoRingQ &topo, const DenseArgs &a, int grid,
                                size_t lds, int vpt, bool queue, hipStream_t st, bool *done) {
     *done = true;
     switch (a.D) {
@@ -169,12 +169,12 @@ int run_dense(DenseCtx &c, const DenseArgs &a, hipStream_t st) {
     if ((int)lds <= lds_cap && vpt > 0) {
         c.last_variant = bs == 256 ? 1 : 2;
         switch (c.topo_kind) {
-            case TOPO_RING16: {
+            case TOPO_RINGQ: {
                 bool done = false;
                 e = hipSuccess;
                 if (bs == 1024 && !c.generic_opt)
-                    e = launch_fixed(c.ring16, a, tasks, lds, vpt, false, st, &done);
-                if (!done) e = launch_lds(c.ring16, a, tasks, lds, bs, vpt, st);
+                    e = launch_fixed(c.ringq, a, tasks, lds, vpt, false, st, &done);
+                if (!done) e = launch_lds(c.ringq, a, tasks, lds, bs, vpt, st);
                 break;
             }
             case TOPO_RING32: e = launch_lds(c.ring32, a, tasks, lds, bs, vpt, st); break;
@@ -194,7 +194,7 @@ int run_dense(DenseCtx &c, const DenseArgs &a, hipStream_t st) {
         g.gscratch = c.gscratch;
         c.last_variant = 3;
         switch (c.topo_kind) {
-            case TOPO_RING16: e = launch_global(c.ring16, g, grid, tasks, st); break;
+            case TOPO_RINGQ: e = launch_global(c.ringq, g, grid, tasks, st); break;
             case TOPO_RING32: e = launch_global(c.ring32, g, grid, tasks, st); break;
             case TOPO_PIECE: e = launch_global(c.piece, g, grid, tasks, st); break;
             default: e = launch_global(c.box, g, grid, tasks, st); break;
@@ -217,11 +217,11 @@ int run_dense_queue(DenseCtx &c, const DenseArgs &a, int grid, hipStream_t st) {
     if ((int)lds > lds_cap || vpt == 0) return GG_UNSUPPORTED;
     hipError_t e;
     switch (c.topo_kind) {
-        case TOPO_RING16: {
+        case TOPO_RINGQ: {
             bool done = false;
             e = hipSuccess;
-            if (!c.generic_opt) e = launch_fixed(c.ring16, a, grid, lds, vpt, true, st, &done);
-            if (!done) e = launch_lds_queue(c.ring16, a, grid, lds, vpt, st);
+            if (!c.generic_opt) e = launch_fixed(c.ringq, a, grid, lds, vpt, true, st, &done);
+            if (!done) e = launch_lds_queue(c.ringq, a, grid, lds, vpt, st);
             break;
         }
         case TOPO_RING32: e = launch_lds_queue(c.ring32, a, grid, lds, vpt, st); break;
@@ -267,6 +267,7 @@ struct gg_eval {
     gg_swap *d_swap = nullptr;
     gg_swap *h_swap = nullptr;  // pinned
     int have_base = 0;  // adjacency evaluators: a table is resident
+    int lmax = GG_LEVEL_LIMIT;
     int have_result = 0;
     int last_levels = 0;
 };
@@ -277,12 +278,13 @@ static int eval_alloc(gg_eval *ev, size_t tab_elem_bytes) {
     GG_HIP(hipEventCreate(&ev->ev0));
     GG_HIP(hipEventCreate(&ev->ev1));
     int R = ev->kind == EVAL_ADJ ? ev->N : ev->M;
-    ev->tab_bytes = (size_t)R * ev->degree * tab_elem_bytes;
+    ev->tab_bytes = ev->dc.topo_kind == TOPO_RINGQ ? (size_t)quads(ev->degree) * R * 8
+                                                   : (size_t)R * ev->degree * tab_elem_bytes;
     size_t tb = ev->tab_bytes ? ev->tab_bytes : 8;
     if (hipMalloc(&ev->d_tab, tb) != hipSuccess) return GG_OOM;
     if (hipHostMalloc(&ev->h_tab, tb, hipHostMallocDefault) != hipSuccess) return GG_OOM;
     ev->W = (ev->M + 63) / 64;
-    const int L = GG_LEVEL_LIMIT + 1;
+    const int L = GG_LEVEL_LIMIT_MAX + 1;
     if (hipMalloc((void **)&ev->d_src, sizeof(int) * ev->M) != hipSuccess) return GG_OOM;
     if (hipMalloc((void **)&ev->d_parts, sizeof(ColPart) * ev->W) != hipSuccess) return GG_OOM;
     if (hipMalloc((void **)&ev->d_res, sizeof(ResultRec)) != hipSuccess) return GG_OOM;
@@ -347,7 +349,7 @@ extern "C" int gg_create_part(int N, int M, int degree, int device, gg_eval **ou
     ev->kind = EVAL_PART;
     ev->N = N; ev->M = M; ev->degree = degree; ev->device = device;
     bool narrow = N <= 65535;
-    ev->dc.topo_kind = narrow ? TOPO_RING16 : TOPO_RING32;
+    ev->dc.topo_kind = narrow ? TOPO_RINGQ : TOPO_RING32;
     return finish_create(ev, narrow ? 2 : 4, out);
 }
 
@@ -361,7 +363,7 @@ extern "C" int gg_create_adj(int N, int degree_max, int device, gg_eval **out) {
     ev->kind = EVAL_ADJ;
     ev->N = N; ev->M = N; ev->degree = degree_max; ev->device = device;
     bool narrow = N <= 65535;
-    ev->dc.topo_kind = narrow ? TOPO_RING16 : TOPO_RING32;
+    ev->dc.topo_kind = narrow ? TOPO_RINGQ : TOPO_RING32;
     return finish_create(ev, narrow ? 2 : 4, out);
 }
 
@@ -434,8 +436,9 @@ static inline long long norm_mod(long long x, long long n) {
 // that is well defined the value equals the mathematical residue used here.
 static int pack_ring(gg_eval *ev, const int32_t *table, int R, bool adj) {
     const int N = ev->N, D = ev->degree;
-    if (ev->dc.topo_kind == TOPO_RING16) {
+    if (ev->dc.topo_kind == TOPO_RINGQ) {
         uint16_t *h = (uint16_t *)ev->h_tab;
+        memset(h, 0, ev->tab_bytes);  // padding slots: offset 0 (self-loop)
         for (int r = 0; r < R; r++)
             for (int i = 0; i < D; i++) {
                 long long x = table[(size_t)r * D + i];
@@ -447,7 +450,7 @@ static int pack_ring(gg_eval *ev, const int32_t *table, int R, bool adj) {
                 } else {
                     off = norm_mod(x, N);
                 }
-                h[(size_t)i * R + r] = (uint16_t)off;
+                h[quad_index(i, R, r)] = (uint16_t)off;
             }
     } else {
         uint32_t *h = (uint32_t *)ev->h_tab;
@@ -497,7 +500,7 @@ static int pack_box(gg_eval *ev, const int32_t *table) {
 static void bind_topology(gg_eval *ev) {
     DenseCtx &c = ev->dc;
     unsigned R = ev->kind == EVAL_ADJ ? ev->N : ev->M;
-    c.ring16 = TopoRing<uint16_t>{(const uint16_t *)ev->d_tab, (unsigned)ev->N, R};
+    c.ringq = TopoRingQ{(const uint64_t *)ev->d_tab, (unsigned)ev->N, R};
     c.ring32 = TopoRing<uint32_t>{(const uint32_t *)ev->d_tab, (unsigned)ev->N, R};
     c.piece = TopoPiece{(const uint32_t *)ev->d_tab, (unsigned)ev->Nx, (unsigned)ev->Ny,
                         (unsigned)ev->Mx, (unsigned)ev->My, (unsigned)ev->M};
@@ -526,7 +529,7 @@ static int eval_common(gg_eval *ev, gg_result *out, bool upload = true,
     bind_topology(ev);
     DenseArgs a{};
     a.N = ev->N; a.M = ev->M; a.D = ev->degree; a.W = ev->W; a.K = 1;
-    a.lmax = GG_LEVEL_LIMIT;
+    a.lmax = ev->lmax;
     a.src_vertex = ev->d_src;
     a.swaps = d_swap;
     a.parts = ev->d_parts;
@@ -589,15 +592,16 @@ extern "C" int gg_eval_adj_swap(gg_eval *ev, const gg_swap *sw, gg_result *out) 
     return eval_common(ev, out, false, ev->d_swap);
 }
 
-template <class IdxT>
-__global__ void k_commit_swap(IdxT *tab, int N, const gg_swap *sw) {
+// Applies a swap to the resident table (ring topologies, R = N).
+__global__ void k_commit_swap(uint16_t *tabq, uint32_t *tab32, int N, const gg_swap *sw) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     for (int j = 0; j < 4; j++) {
         int v = sw->v[j];
         if (v < 0) continue;
         int off = sw->nbr[j] - v;
         if (off < 0) off += N;
-        tab[(size_t)sw->slot[j] * N + v] = (IdxT)off;
+        if (tabq) tabq[quad_index(sw->slot[j], N, v)] = (uint16_t)off;
+        else tab32[(size_t)sw->slot[j] * N + v] = (uint32_t)off;
     }
 }
 
@@ -609,12 +613,10 @@ extern "C" int gg_eval_adj_commit(gg_eval *ev, const gg_swap *sw) {
     *ev->h_swap = *sw;
     GG_HIP(hipMemcpyAsync(ev->d_swap, ev->h_swap, sizeof(gg_swap), hipMemcpyHostToDevice,
                           ev->stream));
-    if (ev->dc.topo_kind == TOPO_RING16)
-        hipLaunchKernelGGL(k_commit_swap<uint16_t>, dim3(1), dim3(64), 0, ev->stream,
-                           (uint16_t *)ev->d_tab, ev->N, (const gg_swap *)ev->d_swap);
-    else
-        hipLaunchKernelGGL(k_commit_swap<uint32_t>, dim3(1), dim3(64), 0, ev->stream,
-                           (uint32_t *)ev->d_tab, ev->N, (const gg_swap *)ev->d_swap);
+    const bool q = ev->dc.topo_kind == TOPO_RINGQ;
+    hipLaunchKernelGGL(k_commit_swap, dim3(1), dim3(64), 0, ev->stream,
+                       q ? (uint16_t *)ev->d_tab : nullptr, q ? nullptr : (uint32_t *)ev->d_tab,
+                       ev->N, (const gg_swap *)ev->d_swap);
     GG_HIP(hipGetLastError());
     GG_HIP(hipStreamSynchronize(ev->stream));
     return GG_OK;
@@ -644,7 +646,7 @@ extern "C" int gg_eval_levels(gg_eval *ev, int32_t *ecc_out, int64_t *reach_out,
         GG_HIP(hipMemcpyAsync(ecc_out, ev->d_ecc, sizeof(int) * ev->M, hipMemcpyDeviceToHost,
                               ev->stream));
     if (reach_out && reach_cap > 0) {
-        int L = GG_LEVEL_LIMIT + 1;
+        int L = ev->lmax + 1;
         int n = reach_cap < L ? reach_cap : L;
         std::vector<long long> tmp(L, 0);
         GG_HIP(hipMemcpyAsync(tmp.data(), ev->d_reach, sizeof(long long) * L,
@@ -677,6 +679,10 @@ extern "C" int gg_eval_set_option(gg_eval *ev, int option, int value) {
         case GG_OPT_GENERIC:
             ev->dc.generic_opt = value ? 1 : 0;
             return GG_OK;
+        case GG_OPT_LEVEL_LIMIT:
+            if (value < 1 || value > GG_LEVEL_LIMIT_MAX) return GG_BAD_ARG;
+            ev->lmax = value;
+            return GG_OK;
         default: return GG_BAD_ARG;
     }
 }
@@ -687,6 +693,7 @@ extern "C" int gg_eval_get_option(gg_eval *ev, int option, int *value) {
         case GG_OPT_BLOCK: *value = ev->dc.block_opt; return GG_OK;
         case GG_OPT_LDS_LIMIT: *value = ev->dc.lds_limit_opt; return GG_OK;
         case GG_OPT_GENERIC: *value = ev->dc.generic_opt; return GG_OK;
+        case GG_OPT_LEVEL_LIMIT: *value = ev->lmax; return GG_OK;
         case 100: *value = ev->dc.last_variant; return GG_OK;  // kernel variant of the last call
         default: return GG_BAD_ARG;
     }

@@ -1,7 +1,7 @@
-// Fixed-degree (24) dense kernels for the 16-bit ring topology.
+// Fixed-degree (24) dense kernels for the quad-packed 16-bit ring topology.
 #include "gg_fixed_impl.hpp"
 namespace gg {
-hipError_t launch_fixed_d24(const TopoRing<uint16_t> &topo, const DenseArgs &a, int grid,
+hipError_t launch_fixed_d24(const TopoRingQ &topo, const DenseArgs &a, int grid,
                             size_t lds, int vpt, bool queue, hipStream_t st) {
     return launch_fixed_deg<24>(topo, a, grid, lds, vpt, queue, st);
 }

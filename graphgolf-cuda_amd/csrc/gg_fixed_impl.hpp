@@ -1,5 +1,5 @@
 // gg_fixed_impl.hpp -- launchers of the fixed-degree dense kernels for the
-// 16-bit ring topology (part / adjacency graphs with N <= 65535).  Each degree
+// quad-packed 16-bit ring topology (part / adjacency graphs with N <= 65535).  Each degree
 // is instantiated in its own translation unit (gg_fixed_d*.hip) so the
 // unrolled kernels compile in parallel.
 #pragma once
@@ -19,16 +19,16 @@ static hipError_t set_lds(K fn, size_t lds, int &configured) {
 }
 
 template <int VPT, int DT>
-static hipError_t launch_fixed_inst(const TopoRing<uint16_t> &topo, const DenseArgs &a, int grid,
+static hipError_t launch_fixed_inst(const TopoRingQ &topo, const DenseArgs &a, int grid,
                                     size_t lds, bool queue, hipStream_t st) {
     static int cfg_lds = -1, cfg_queue = -1;
     hipError_t e;
     if (queue) {
-        auto fn = &k_dense_lds_queue<TopoRing<uint16_t>, 1024, VPT, DT>;
+        auto fn = &k_dense_lds_queue<TopoRingQ, 1024, VPT, DT>;
         if ((e = set_lds(fn, lds, cfg_queue)) != hipSuccess) return e;
         hipLaunchKernelGGL(fn, dim3(grid), dim3(1024), lds, st, topo, a);
     } else {
-        auto fn = &k_dense_lds<TopoRing<uint16_t>, 1024, VPT, DT>;
+        auto fn = &k_dense_lds<TopoRingQ, 1024, VPT, DT>;
         if ((e = set_lds(fn, lds, cfg_lds)) != hipSuccess) return e;
         hipLaunchKernelGGL(fn, dim3(grid), dim3(1024), lds, st, topo, a);
     }
@@ -36,7 +36,7 @@ static hipError_t launch_fixed_inst(const TopoRing<uint16_t> &topo, const DenseA
 }
 
 template <int DT>
-static hipError_t launch_fixed_deg(const TopoRing<uint16_t> &topo, const DenseArgs &a, int grid,
+static hipError_t launch_fixed_deg(const TopoRingQ &topo, const DenseArgs &a, int grid,
                                    size_t lds, int vpt, bool queue, hipStream_t st) {
     switch (vpt) {
         case 1: return launch_fixed_inst<1, DT>(topo, a, grid, lds, queue, st);
@@ -50,15 +50,15 @@ static hipError_t launch_fixed_deg(const TopoRing<uint16_t> &topo, const DenseAr
 }
 
 // Defined in gg_fixed_d*.hip.
-hipError_t launch_fixed_d4(const TopoRing<uint16_t> &, const DenseArgs &, int, size_t, int, bool,
+hipError_t launch_fixed_d4(const TopoRingQ &, const DenseArgs &, int, size_t, int, bool,
                            hipStream_t);
-hipError_t launch_fixed_d8(const TopoRing<uint16_t> &, const DenseArgs &, int, size_t, int, bool,
+hipError_t launch_fixed_d8(const TopoRingQ &, const DenseArgs &, int, size_t, int, bool,
                            hipStream_t);
-hipError_t launch_fixed_d12(const TopoRing<uint16_t> &, const DenseArgs &, int, size_t, int, bool,
+hipError_t launch_fixed_d12(const TopoRingQ &, const DenseArgs &, int, size_t, int, bool,
                             hipStream_t);
-hipError_t launch_fixed_d16(const TopoRing<uint16_t> &, const DenseArgs &, int, size_t, int, bool,
+hipError_t launch_fixed_d16(const TopoRingQ &, const DenseArgs &, int, size_t, int, bool,
                             hipStream_t);
-hipError_t launch_fixed_d24(const TopoRing<uint16_t> &, const DenseArgs &, int, size_t, int, bool,
+hipError_t launch_fixed_d24(const TopoRingQ &, const DenseArgs &, int, size_t, int, bool,
                             hipStream_t);
 
 }  // namespace gg

@@ -6,12 +6,12 @@
 namespace gg {
 
 enum EvalKind { EVAL_PART = 0, EVAL_PIECE = 1, EVAL_BOX = 2, EVAL_ADJ = 3 };
-enum TopoKind { TOPO_RING16 = 0, TOPO_RING32 = 1, TOPO_PIECE = 2, TOPO_BOX = 3 };
+enum TopoKind { TOPO_RINGQ = 0, TOPO_RING32 = 1, TOPO_PIECE = 2, TOPO_BOX = 3 };
 
 // Kernel-selection state of one evaluator (or batch evaluator).
 struct DenseCtx {
-    TopoKind topo_kind = TOPO_RING16;
-    TopoRing<uint16_t> ring16{};
+    TopoKind topo_kind = TOPO_RINGQ;
+    TopoRingQ ringq{};
     TopoRing<uint32_t> ring32{};
     TopoPiece piece{};
     TopoBox box{};
@@ -31,5 +31,11 @@ int dense_fits_lds(const DenseCtx &c, int N);
 int pick_vpt(int bs, int need);
 int device_lds_limit(int device);
 int device_cu_count(int device);
+
+// Quad-packed table size / index helpers (TopoRingQ).
+inline int quads(int D) { return (D + 3) / 4; }
+__host__ __device__ inline size_t quad_index(int slot, int R, int row) {  // uint16 index into the table
+    return ((size_t)(slot >> 2) * R + row) * 4 + (slot & 3);
+}
 
 }  // namespace gg

@@ -14,72 +14,69 @@
 // stops at a fixed point (level >= base column eccentricity with Delta_l empty,
 // or level > base eccentricity with Delta_l == Delta_{l-1}).
 //
-// One workgroup per column keeps B_{l-1}[w][*] in LDS and walks the levels;
-// each wave handles some candidates of the column, holding Delta_{l-1} in a
-// small LDS hash (Delta lists persist across levels in global scratch).
-// Tasks whose Delta outgrows the capacity, and the columns that contain a swap
-// endpoint as a source, are queued for the dense kernel (k_dense_lds_queue).
+// One workgroup per column keeps B_{l-1}[w][*] in LDS and walks the levels
+// for all candidates of the launch.  All its waves draw the level's items
+// (C_l of every active candidate, flattened) from one pool, so a candidate
+// with a large Delta does not idle the others.  Delta_{l-1} membership is a
+// per-task bitmap and its words a per-task dense override array in HBM
+// (written only at Delta positions), so the D neighbour lookups of an item
+// are independent loads.  Tasks whose Delta outgrows the list capacity, and
+// the columns that contain a swap endpoint as a source, are queued for the
+// dense kernel (k_dense_lds_queue).
 #pragma once
 #include "gg_dense.hpp"
 
 #define GG_TASK_ACTIVE 0
 #define GG_TASK_DONE 1
 #define GG_TASK_DENSE 2
+#define GG_SPARSE_MAX_K 64  // candidates per sparse launch (LDS bookkeeping)
 
 struct SparseArgs {
     int N, D, W, K, lmax;
+    int k0;                          // first candidate of this launch (swaps/results offset)
     const int *adjR;                 // [N][D] base neighbours, row-major
-    const gg_swap *swaps;            // [K]
+    const gg_swap *swaps;            // [k0 + K]
     const unsigned long long *snap;  // [S][W][N] base words after levels 1..S
     int S;
     const ColPart *base_parts;       // [W] base column parts (ecc, levels)
     const long long *base_reach;     // [W][lmax+1] base per-level new pairs
     int cap;                         // Delta list capacity per task
-    int *dl_key;                     // [K*W][2][cap]
-    unsigned long long *dl_val;      // [K*W][2][cap]
-    int *dl_cnt;                     // [K*W][2]
-    long long *dcnt;                 // [K*W][lmax+1] count deltas per level
-    int *task_state;                 // [K*W]
-    int *task_end;                   // [K*W] level of the fixed point
-    int *task_flags;                 // [K*W]
-    int2 *queue;                     // [K*W] dense fallback tasks
+    // per task t = k*W + w (k local to the launch), parity p = level & 1
+    int *dl_key;                     // [T][2][cap] Delta vertices
+    int *dl_cnt;                     // [T][2]
+    unsigned *dl_bits;               // [T][2][ceil(N/32)] Delta membership
+    unsigned long long *dl_ovr;      // [T][2][N] Delta words (valid where the bit is set)
+    long long *dcnt;                 // [k0+K][W][lmax+1] count deltas per level
+    int *task_state;                 // [k0+K][W]
+    int *task_end;                   // [k0+K][W] level of the fixed point
+    int *task_flags;                 // [k0+K][W]
+    int2 *queue;                     // dense fallback tasks (global k, w)
     int *queue_count;
     int *queue_next;
-    ColPart *dense_parts;            // [K*W] written by the dense queue kernel
-    ResultRec *res;                  // [K]
+    ColPart *dense_parts;            // [k0+K][W] written by the dense queue kernel
+    ResultRec *res;                  // [k0+K]
 };
 
-template <int HS>
-__device__ __forceinline__ bool hash_find(const int *hk, const unsigned long long *hv, int key,
-                                          unsigned long long &val) {
-    unsigned h = ((unsigned)key * 2654435761u) & (HS - 1);
-    for (int p = 0; p < HS; p++) {
-        int k = hk[h];
-        if (k == key) { val = hv[h]; return true; }
-        if (k < 0) return false;
-        h = (h + 1) & (HS - 1);
+// Row of candidate vertex v with the swap's replacements applied: DT > 0
+// loads the whole row as 16-byte vectors (one memory latency), DT = 0 loads
+// slot by slot.
+template <int DT>
+struct CandRow {
+    int u[DT > 0 ? DT : 1];
+    __device__ __forceinline__ void load(const int *adjR, int v, const SwapRegs &sw) {
+        const int4 *r = reinterpret_cast<const int4 *>(adjR + (size_t)v * DT);
+#pragma unroll
+        for (int q = 0; q < DT / 4; q++) {
+            int4 x = r[q];
+            u[4 * q] = x.x; u[4 * q + 1] = x.y; u[4 * q + 2] = x.z; u[4 * q + 3] = x.w;
+        }
+#pragma unroll
+        for (int i = 0; i < DT; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                if (sw.v[j] == v && sw.s[j] == i) u[i] = sw.n[j];
     }
-    return false;
-}
-
-// Inserts key if absent.  Returns 1 if inserted, 0 if present, -1 if full.
-template <int HS>
-__device__ __forceinline__ int hash_insert(int *hk, unsigned long long *hv, int key,
-                                           unsigned long long val) {
-    unsigned h = ((unsigned)key * 2654435761u) & (HS - 1);
-    for (int p = 0; p < HS; p++) {
-        int prev = atomicCAS(&hk[h], -1, key);
-        if (prev == -1) { hv[h] = val; return 1; }
-        if (prev == key) return 0;
-        h = (h + 1) & (HS - 1);
-    }
-    return -1;
-}
-
-__device__ __forceinline__ void wave_sync_lds() {
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-}
+};
 
 // Candidate neighbour i of v (base row with the swap's slot replacements).
 __device__ __forceinline__ int cand_nbr(const int *adjR, int D, int v, int i, const SwapRegs &sw) {
@@ -90,147 +87,210 @@ __device__ __forceinline__ int cand_nbr(const int *adjR, int D, int v, int i, co
     return u;
 }
 
-template <int NW, int HS>
-__global__ void __launch_bounds__(NW * 64) k_sparse(SparseArgs a) {
+// Word of x at the previous level: the task's override where Delta_{l-1}
+// holds x, else the base word in LDS.
+__device__ __forceinline__ unsigned long long prev_word(const unsigned *bits,
+                                                        const unsigned long long *ovr,
+                                                        const unsigned long long *Bprev, int x) {
+    return ((bits[x >> 5] >> (x & 31)) & 1u) ? ovr[x] : Bprev[x];
+}
+
+template <int BS, int DT>
+__global__ void __launch_bounds__(BS) k_sparse(SparseArgs a) {
     extern __shared__ unsigned long long Bprev[];
-    __shared__ int hk[NW][2][HS];
-    __shared__ unsigned long long hv[NW][2][HS];
-    __shared__ int s_ncur[NW];
-    __shared__ int s_active[2];  // parity-indexed: reset two levels ahead of its readers
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    __shared__ int s_items[GG_SPARSE_MAX_K + 1];  // prefix sums of items per candidate
+    __shared__ int s_nprev[GG_SPARSE_MAX_K];
+    __shared__ int s_ncur[GG_SPARSE_MAX_K];
+    __shared__ int s_active[GG_SPARSE_MAX_K];     // 1 if the task runs this level
+    __shared__ int s_over[GG_SPARSE_MAX_K];
+    __shared__ int s_diff[GG_SPARSE_MAX_K];
+    __shared__ unsigned long long s_dc[GG_SPARSE_MAX_K];
+    __shared__ int s_ended[GG_SPARSE_MAX_K];      // task ended last level: clear its bitmaps
+    __shared__ int s_next;
+    __shared__ int s_any;
+    const int tid = threadIdx.x;
     const int w = blockIdx.x;
     const int N = a.N, D = a.D, W = a.W, K = a.K;
+    const int NB = (N + 31) >> 5;
     const int ecc_w = a.base_parts[w].ecc;
     const unsigned long long pad = pad_mask(w, N);
+    auto gt = [&](int k) { return (size_t)(a.k0 + k) * W + w; };     // global (cand, col)
+    auto lt = [&](int k) { return (size_t)k * W + w; };              // task slot of this launch
 
     // task init: columns holding a swap endpoint as a source go dense at once
-    for (int k = tid; k < K; k += NW * 64) {
-        const size_t t = (size_t)k * W + w;
-        const gg_swap &s = a.swaps[k];
+    for (int k = tid; k < K; k += BS) {
+        const gg_swap &s = a.swaps[a.k0 + k];
         bool dense = false;
         for (int j = 0; j < 4; j++)
             if (s.v[j] >= 0 && (s.v[j] >> 6) == w) dense = true;
-        a.dl_cnt[t * 2] = 0;
-        a.task_flags[t] = 0;
-        a.task_end[t] = 0;
+        a.dl_cnt[lt(k) * 2] = 0;
+        a.dl_cnt[lt(k) * 2 + 1] = 0;
+        a.task_flags[gt(k)] = 0;
+        a.task_end[gt(k)] = 0;
         if (dense) {
-            a.task_state[t] = GG_TASK_DENSE;
+            a.task_state[gt(k)] = GG_TASK_DENSE;
             int q = atomicAdd(a.queue_count, 1);
-            a.queue[q] = make_int2(k, w);
+            a.queue[q] = make_int2(a.k0 + k, w);
         } else {
-            a.task_state[t] = GG_TASK_ACTIVE;
+            a.task_state[gt(k)] = GG_TASK_ACTIVE;
         }
+        s_ended[k] = 0;
     }
     __threadfence_block();
     __syncthreads();
+    // Bitmaps are all zero between launches: a task clears both of its
+    // bitmaps at the level after it ends (and after the last level).
+    auto clear_ended = [&]() {
+        for (int k = 0; k < K; k++) {
+            if (!s_ended[k]) continue;
+            unsigned *b0 = a.dl_bits + lt(k) * 2 * NB;
+            for (int i = tid; i < 2 * NB; i += BS) b0[i] = 0u;
+        }
+    };
 
     for (int l = 1; l <= a.lmax; l++) {
+        const int par = (l - 1) & 1, cur = l & 1;
         // B_{l-1} of this column into LDS
         const int lp = l - 1;
-        for (int v = tid; v < N; v += NW * 64) {
+        for (int v = tid; v < N; v += BS) {
             unsigned long long b;
             if (lp == 0) b = pad | (((v >> 6) == w) ? (1ull << (v & 63)) : 0ull);
             else if (lp >= ecc_w) b = ~0ull;
             else b = a.snap[((size_t)(lp - 1) * W + w) * N + v];
             Bprev[v] = b;
         }
-        if (tid == 0) s_active[l & 1] = 0;
+        // per-candidate bookkeeping; clear the stale bits of the cur parity
+        // (Delta_{l-2}) so Delta_l starts empty
+        for (int k = tid; k < K; k += BS) {
+            const bool act = a.task_state[gt(k)] == GG_TASK_ACTIVE;
+            s_active[k] = act;
+            s_nprev[k] = act ? a.dl_cnt[lt(k) * 2 + par] : 0;
+            s_ncur[k] = 0;
+            s_over[k] = 0;
+            s_diff[k] = 0;
+            s_dc[k] = 0;
+        }
+        if (tid == 0) { s_next = 0; s_any = 0; }
+        clear_ended();
         __syncthreads();
-        int *hkp = hk[wave][0], *hkc = hk[wave][1];
-        unsigned long long *hvp = hv[wave][0], *hvc = hv[wave][1];
-        for (int k = wave; k < K; k += NW) {
-            const size_t t = (size_t)k * W + w;
-            if (a.task_state[t] != GG_TASK_ACTIVE) continue;
+        for (int k = tid; k < K; k += BS) s_ended[k] = 0;
+        for (int k = 0; k < K; k++) {
+            if (!s_active[k]) continue;
+            const int nold = a.dl_cnt[lt(k) * 2 + cur];
+            const int *okey = a.dl_key + (lt(k) * 2 + cur) * a.cap;
+            unsigned *cbits = a.dl_bits + (lt(k) * 2 + cur) * NB;
+            for (int e = tid; e < nold; e += BS) {
+                const int x = okey[e];
+                atomicAnd(&cbits[x >> 5], ~(1u << (x & 31)));
+            }
+        }
+        if (tid == 0) {
+            int acc = 0;
+            for (int k = 0; k < K; k++) {
+                s_items[k] = acc;
+                if (s_active[k]) acc += 4 + s_nprev[k] * (D + 1);
+            }
+            s_items[K] = acc;
+        }
+        __threadfence_block();
+        __syncthreads();
+        const int total = s_items[K];
+        // shared pool of items over all active candidates
+        for (;;) {
+            int base_it = 0;
+            if ((tid & 63) == 0) base_it = atomicAdd(&s_next, 64);
+            base_it = __shfl(base_it, 0, 64);
+            if (base_it >= total) break;
+            const int it = base_it + (tid & 63);
+            if (it >= total) continue;
+            int k = 0;
+            while (s_items[k + 1] <= it) k++;
+            const int loc = it - s_items[k];
+            const size_t tl = lt(k);
             SwapRegs sw;
             bool has_sw;
-            load_swap(a.swaps, k, sw, has_sw);
-            const int par = (l - 1) & 1, cur = l & 1;
-            const int n_prev = a.dl_cnt[t * 2 + par];
-            const int *pkey = a.dl_key + (t * 2 + par) * a.cap;
-            const unsigned long long *pval = a.dl_val + (t * 2 + par) * a.cap;
-            int *ckey = a.dl_key + (t * 2 + cur) * a.cap;
-            unsigned long long *cval = a.dl_val + (t * 2 + cur) * a.cap;
-            for (int i = lane; i < HS; i += 64) { hkp[i] = -1; hkc[i] = -1; }
-            if (lane == 0) s_ncur[wave] = 0;
-            wave_sync_lds();
-            for (int e = lane; e < n_prev; e += 64) hash_insert<HS>(hkp, hvp, pkey[e], pval[e]);
-            wave_sync_lds();
-            const int items = 4 + n_prev * (D + 1);
-            long long dc = 0;
-            int overflow = 0, differs = 0;
-            for (int it = lane; it < items; it += 64) {
-                int v;
-                if (it < 4) {
-                    v = it == 0 ? sw.v[0] : it == 1 ? sw.v[1] : it == 2 ? sw.v[2] : sw.v[3];
-                    if (v < 0) continue;
+            load_swap(a.swaps, a.k0 + k, sw, has_sw);
+            const unsigned *pbits = a.dl_bits + (tl * 2 + par) * NB;
+            const unsigned long long *povr = a.dl_ovr + (tl * 2 + par) * (size_t)N;
+            int v;
+            if (loc < 4) {
+                v = loc == 0 ? sw.v[0] : loc == 1 ? sw.v[1] : loc == 2 ? sw.v[2] : sw.v[3];
+                if (v < 0) continue;
+            } else {
+                const int e = (loc - 4) / (D + 1), i = (loc - 4) - e * (D + 1);
+                const int u = a.dl_key[(tl * 2 + par) * a.cap + e];
+                v = i == 0 ? u : cand_nbr(a.adjR, D, u, i - 1, sw);
+            }
+            unsigned long long basew = ~0ull;  // issued before the gather
+            if (l < ecc_w) basew = a.snap[((size_t)(l - 1) * W + w) * N + v];
+            unsigned long long acc;
+            const bool none = s_nprev[k] == 0;
+            if constexpr (DT > 0) {
+                CandRow<DT> row;
+                row.load(a.adjR, v, sw);
+                if (none) {
+                    acc = Bprev[v];
+#pragma unroll
+                    for (int i = 0; i < DT; i++) acc |= Bprev[row.u[i]];
                 } else {
-                    int e = (it - 4) / (D + 1), i = (it - 4) - e * (D + 1);
-                    int u = pkey[e];
-                    v = i == 0 ? u : cand_nbr(a.adjR, D, u, i - 1, sw);
+                    acc = prev_word(pbits, povr, Bprev, v);
+#pragma unroll
+                    for (int i = 0; i < DT; i++) acc |= prev_word(pbits, povr, Bprev, row.u[i]);
                 }
-                unsigned long long own, acc;
-                if (n_prev == 0) {
+            } else {
+                if (none) {
                     acc = Bprev[v];
                     for (int i = 0; i < D; i++) acc |= Bprev[cand_nbr(a.adjR, D, v, i, sw)];
                 } else {
-                    if (!hash_find<HS>(hkp, hvp, v, own)) own = Bprev[v];
-                    acc = own;
-                    for (int i = 0; i < D; i++) {
-                        int x = cand_nbr(a.adjR, D, v, i, sw);
-                        unsigned long long xv;
-                        if (!hash_find<HS>(hkp, hvp, x, xv)) xv = Bprev[x];
-                        acc |= xv;
-                    }
-                }
-                unsigned long long base;
-                if (l >= ecc_w) base = ~0ull;
-                else base = a.snap[((size_t)(l - 1) * W + w) * N + v];
-                if (acc != base) {
-                    int ins = hash_insert<HS>(hkc, hvc, v, acc);
-                    if (ins < 0) { overflow = 1; continue; }
-                    if (ins == 1) {
-                        int idx = atomicAdd(&s_ncur[wave], 1);
-                        if (idx >= a.cap) { overflow = 1; continue; }
-                        ckey[idx] = v;
-                        cval[idx] = acc;
-                        dc += (long long)__popcll(acc) - (long long)__popcll(base);
-                        unsigned long long pv;
-                        if (!hash_find<HS>(hkp, hvp, v, pv) || pv != acc) differs = 1;
-                    }
+                    acc = prev_word(pbits, povr, Bprev, v);
+                    for (int i = 0; i < D; i++)
+                        acc |= prev_word(pbits, povr, Bprev, cand_nbr(a.adjR, D, v, i, sw));
                 }
             }
-            wave_sync_lds();
-            dc = wave_sum64(dc);
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) {
-                overflow |= __shfl_xor(overflow, o, 64);
-                differs |= __shfl_xor(differs, o, 64);
-            }
-            const int n_cur = s_ncur[wave];
-            if (lane == 0) {
-                if (overflow) {
-                    a.task_state[t] = GG_TASK_DENSE;
-                    int q = atomicAdd(a.queue_count, 1);
-                    a.queue[q] = make_int2(k, w);
-                } else {
-                    a.dcnt[t * (a.lmax + 1) + l] = dc;
-                    a.dl_cnt[t * 2 + cur] = n_cur;
-                    bool fixed = (l >= ecc_w && n_cur == 0) ||
-                                 (l > ecc_w && !differs && n_cur == n_prev);
-                    if (fixed || l >= a.lmax) {
-                        a.task_state[t] = GG_TASK_DONE;
-                        a.task_end[t] = l;
-                        if (!fixed) a.task_flags[t] = GG_FLAG_LIMIT;
-                    } else {
-                        atomicAdd(&s_active[l & 1], 1);
-                    }
-                }
-            }
-            wave_sync_lds();
+            if (acc == basew) continue;
+            unsigned *cbits = a.dl_bits + (tl * 2 + cur) * NB;
+            const unsigned m = 1u << (v & 31);
+            const unsigned old = atomicOr(&cbits[v >> 5], m);
+            if (old & m) continue;  // another item of this task recorded v
+            const int idx = atomicAdd(&s_ncur[k], 1);
+            if (idx >= a.cap) { s_over[k] = 1; continue; }
+            a.dl_key[(tl * 2 + cur) * a.cap + idx] = v;
+            a.dl_ovr[(tl * 2 + cur) * (size_t)N + v] = acc;
+            atomicAdd(&s_dc[k], (unsigned long long)((long long)__popcll(acc) -
+                                                      (long long)__popcll(basew)));
+            if (none || !((pbits[v >> 5] >> (v & 31)) & 1u) || povr[v] != acc) s_diff[k] = 1;
         }
+        __threadfence_block();
         __syncthreads();
-        if (s_active[l & 1] == 0) break;
+        for (int k = tid; k < K; k += BS) {
+            if (!s_active[k]) continue;
+            const size_t g = gt(k);
+            if (s_over[k]) {
+                a.task_state[g] = GG_TASK_DENSE;
+                int q = atomicAdd(a.queue_count, 1);
+                a.queue[q] = make_int2(a.k0 + k, w);
+                s_ended[k] = 1;
+                continue;
+            }
+            const int n_cur = s_ncur[k];
+            a.dcnt[g * (a.lmax + 1) + l] = (long long)s_dc[k];
+            a.dl_cnt[lt(k) * 2 + cur] = n_cur;
+            const bool fixed = (l >= ecc_w && n_cur == 0) ||
+                               (l > ecc_w && !s_diff[k] && n_cur == s_nprev[k]);
+            if (fixed || l >= a.lmax) {
+                a.task_state[g] = GG_TASK_DONE;
+                a.task_end[g] = l;
+                if (!fixed) a.task_flags[g] = GG_FLAG_LIMIT;
+                s_ended[k] = 1;
+            } else {
+                atomicOr(&s_any, 1);
+            }
+        }
+        __threadfence_block();
+        __syncthreads();
+        if (!s_any) break;
     }
+    clear_ended();
 }
 
 // One workgroup per candidate: merges sparse column results (base counts +

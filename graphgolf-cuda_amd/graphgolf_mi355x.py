@@ -30,6 +30,8 @@ GG_LEVEL_LIMIT = 99
 GG_OPT_BLOCK = 1
 GG_OPT_LDS_LIMIT = 2
 GG_OPT_GENERIC = 3
+GG_OPT_LEVEL_LIMIT = 4
+GG_LEVEL_LIMIT_MAX = 199
 
 GG_BATCH_OPT_PATH = 1
 GG_BATCH_OPT_GRAPH = 2

@@ -32,10 +32,13 @@ extern "C" {
 #define GG_NO_DEVICE      (-4)
 #define GG_UNSUPPORTED    (-5)
 
-/* Highest BFS level evaluated: the reference loops step = 1..99
+/* Highest BFS level evaluated by default: the reference loops step = 1..99
  * (cudacode/cudaASPLconv.cu:231) and reports a sentinel when step 99 still
- * finds new vertices. */
+ * finds new vertices.  WVCdiameterASPL loops to 100 (cudaASPLconv.cu:287) and
+ * cudaASPLgraph::calc to 199 (cudaASPLgraph.cu:182); GG_OPT_LEVEL_LIMIT sets
+ * the limit per evaluator, up to GG_LEVEL_LIMIT_MAX. */
 #define GG_LEVEL_LIMIT 99
+#define GG_LEVEL_LIMIT_MAX 199
 
 typedef struct gg_eval gg_eval;
 typedef struct gg_batch gg_batch;
@@ -130,6 +133,7 @@ int gg_eval_last_kernel_ms(gg_eval *ev, float *ms);
                                  forces the global-memory kernel (used by tests)      */
 #define GG_OPT_GENERIC    3   /* 1 = runtime-degree kernel even when the degree has a
                                  compile-time specialisation (used by tests)          */
+#define GG_OPT_LEVEL_LIMIT 4  /* highest level evaluated (1..GG_LEVEL_LIMIT_MAX)      */
 int gg_eval_set_option(gg_eval *ev, int option, int value);
 int gg_eval_get_option(gg_eval *ev, int option, int *value);
 
