@@ -216,7 +216,10 @@ extern "C" int gg_batch_set_base(gg_batch *b, const int32_t *adj, gg_result *out
     GG_HIP(hipMemcpyAsync(b->d_tab, b->h_tab, b->tab_bytes, hipMemcpyHostToDevice, b->stream));
     GG_HIP(hipMemcpyAsync(b->d_adjR, adj, sizeof(int) * (size_t)N * D, hipMemcpyHostToDevice,
                           b->stream));
+    b->h_adj.assign(adj, adj + (size_t)N * D);
+    b->base_sym = adj_symmetric(adj, N, D);
     b->have_base = 1;
+    b->dc.push_ok = b->base_sym;
     return eval_base(b, out);
 }
 
@@ -333,6 +336,10 @@ extern "C" int gg_batch_eval(gg_batch *b, const gg_swap *swaps, int K, gg_result
     if (st != GG_OK) return st;
     GG_HIP(hipSetDevice(b->device));
     memcpy(b->h_swaps, swaps, sizeof(gg_swap) * K);
+    bool sym = b->base_sym;
+    for (int k = 0; k < K && sym; k++) sym = swap_keeps_symmetry(b->h_adj.data(), b->degree, swaps[k]);
+    if ((b->dc.push_ok != 0) != sym) drop_graph(b);  // the captured launches bake the push choice
+    b->dc.push_ok = sym;
     const bool sparse = b->sparse_ok && b->mode_opt != 1;
     GG_HIP(hipEventRecord(b->ev0, b->stream));
     if (sparse && b->use_graph) {
@@ -381,6 +388,10 @@ extern "C" int gg_batch_commit(gg_batch *b, const gg_swap *sw, gg_result *out) {
     if (st != GG_OK) return st;
     GG_HIP(hipSetDevice(b->device));
     const int K = b->max_k;
+    b->base_sym = b->base_sym && swap_keeps_symmetry(b->h_adj.data(), b->degree, *sw);
+    for (int j = 0; j < 4; j++)
+        if (sw->v[j] >= 0) b->h_adj[(size_t)sw->v[j] * b->degree + sw->slot[j]] = sw->nbr[j];
+    b->dc.push_ok = b->base_sym;
     b->h_swaps[K] = *sw;
     GG_HIP(hipMemcpyAsync(b->d_swaps + K, b->h_swaps + K, sizeof(gg_swap), hipMemcpyHostToDevice,
                           b->stream));

@@ -1,5 +1,6 @@
 // gg_batch.hpp -- the batch evaluator object (host side).
 #pragma once
+#include <vector>
 #include "gg_internal.hpp"
 
 struct gg_batch {
@@ -37,6 +38,8 @@ struct gg_batch {
     int last_sparse = 0;                      // last gg_batch_eval used the sparse path
     int use_graph = 1;
     ResultRec base_res{};
+    std::vector<int32_t> h_adj;               // host copy of the base adjacency
+    bool base_sym = false;                    // the base is undirected
     int have_base = 0, have_timing = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;

@@ -125,6 +125,9 @@ struct DenseArgs {
     const int2 *queue;
     const int *queue_count;
     int *queue_next;
+    // push levels: capacity of the level-1 frontier list that follows vis[N]
+    // in dynamic LDS (0 = pull from level 1)
+    int push_cap;
 };
 
 struct SwapRegs {
@@ -223,6 +226,19 @@ __device__ __forceinline__ unsigned long long gather_fixed(const FixedRow<DT> &r
     return acc;
 }
 
+// Neighbour i of u in the candidate graph (the swap's slot replacements applied).
+template <class Topo>
+__device__ __forceinline__ unsigned nbr_patched(const Topo &topo, unsigned u, int i,
+                                                const SwapRegs &sw, bool has_sw) {
+    unsigned x = topo.nbr(topo.ctx(u), i);
+    if (has_sw) {
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            if (sw.v[j] == (int)u && sw.s[j] == i) x = (unsigned)sw.n[j];
+    }
+    return x;
+}
+
 // Padding bits of column w: sources 64w+j >= M are treated as already reached.
 __device__ __forceinline__ unsigned long long pad_mask(int w, int M) {
     int base = w * 64;
@@ -317,6 +333,7 @@ __device__ __forceinline__ ColPart dense_column(const Topo &topo, const DenseArg
                                                 unsigned long long *vis) {
     __shared__ unsigned long long s_or[2];
     __shared__ unsigned s_cnt[2];
+    __shared__ int s_fn;
     const int tid = threadIdx.x, lane = tid & 63;
     const unsigned N = (unsigned)a.N;
     const int D = a.D;
@@ -351,7 +368,90 @@ __device__ __forceinline__ ColPart dense_column(const Topo &topo, const DenseArg
             if (v == sw.v[0] || v == sw.v[1] || v == sw.v[2] || v == sw.v[3]) hitmask |= 1u << j;
         }
     }
-    for (level = 1;; level++) {
+    auto snap_of = [&](int l) -> unsigned long long * {
+        return (a.snap && l <= a.snap_levels) ? a.snap + ((size_t)(l - 1) * a.W + w) * N
+                                              : nullptr;
+    };
+    // level bookkeeping shared by push and pull levels: folds this thread's
+    // new-pair count and new-bit mask into the block, returns true when the
+    // BFS of the column is finished
+    auto end_level = [&](int l, unsigned cnt, unsigned long long lor) -> bool {
+        lor = wave_or64(lor);
+        cnt = wave_sum32(cnt);
+        if (lane == 0 && cnt) {
+            atomicOr(&s_or[l & 1], lor);
+            atomicAdd(&s_cnt[l & 1], cnt);
+        }
+        __syncthreads();
+        const unsigned long long blk = s_or[l & 1];
+        if (tid == 0) {
+            unsigned c = s_cnt[l & 1];
+            total += (long long)l * c;
+            reached += c;
+            if (reach) reach[l] = c;
+            s_or[(l + 1) & 1] = 0;
+            s_cnt[(l + 1) & 1] = 0;
+        }
+        if (tid < 64 && ((blk >> tid) & 1ull)) my_ecc = l;
+        if (blk == 0) return true;
+        last_new = l;
+        if (l >= a.lmax) { flags |= GG_FLAG_LIMIT; return true; }
+        return false;
+    };
+    bool done = false;
+    level = 1;
+    // push follows edges from the frontier, pull reads each vertex's listed
+    // neighbours: the same BFS only on undirected graphs, which the host
+    // guarantees before it sets push_cap
+    if (a.push_cap > 0) {
+        // levels 1 and 2 push from the (small) frontier with LDS atomics;
+        // the old word tells whether a bit is new, so counts stay exact
+        unsigned *flist = reinterpret_cast<unsigned *>(vis + N);
+        if (tid == 0) s_fn = 0;
+        __syncthreads();
+        unsigned cnt = 0;
+        unsigned long long lor = 0;
+        for (int it = tid; it < nvalid * D; it += BS) {
+            const int j = it / D, i = it - j * D;
+            const unsigned sv = (unsigned)a.src_vertex[w * 64 + j];
+            const unsigned u = nbr_patched(topo, sv, i, sw, has_sw);
+            const unsigned long long bit = 1ull << j;
+            const unsigned long long old = atomicOr(&vis[u], bit);
+            if (!(old & bit)) {
+                cnt++;
+                lor |= bit;
+                const int idx = atomicAdd(&s_fn, 1);
+                if (idx < a.push_cap) flist[idx] = u | ((unsigned)j << 17);
+            }
+        }
+        __syncthreads();
+        if (unsigned long long *sn = snap_of(1))
+            for (unsigned v = tid; v < N; v += BS) sn[v] = vis[v];
+        done = end_level(1, cnt, lor);
+        const int nf = s_fn;
+        if (!done && nf <= a.push_cap) {
+            level = 2;
+            cnt = 0;
+            lor = 0;
+            for (int it = tid; it < nf * D; it += BS) {
+                const int e = it / D, i = it - e * D;
+                const unsigned f = flist[e];
+                const unsigned u = f & 0x1FFFFu, j = f >> 17;
+                const unsigned x = nbr_patched(topo, u, i, sw, has_sw);
+                const unsigned long long bit = 1ull << j;
+                const unsigned long long old = atomicOr(&vis[x], bit);
+                if (!(old & bit)) { cnt++; lor |= bit; }
+            }
+            __syncthreads();
+            if (unsigned long long *sn = snap_of(2))
+                for (unsigned v = tid; v < N; v += BS) sn[v] = vis[v];
+            done = end_level(2, cnt, lor);
+        }
+    } else {
+        level = 0;
+    }
+    // `level` is the last level executed; pull continues from the next one
+    if (!done) for (level = level + 1;; level++) {
         unsigned long long nw[VPT];
         if constexpr (DT > 0 && std::is_same<Topo, TopoRingQ>::value) {
             // fixed degree: the next vertex's words load while this one gathers
@@ -392,9 +492,7 @@ __device__ __forceinline__ ColPart dense_column(const Topo &topo, const DenseArg
         __syncthreads();
         unsigned long long lor = 0;
         unsigned cnt = 0;
-        unsigned long long *snap =
-            (a.snap && level <= a.snap_levels)
-                ? a.snap + ((size_t)(level - 1) * a.W + w) * N : nullptr;
+        unsigned long long *snap = snap_of(level);
 #pragma unroll
         for (int j = 0; j < VPT; j++) {
             unsigned v = vb + j * 64;
@@ -405,26 +503,7 @@ __device__ __forceinline__ ColPart dense_column(const Topo &topo, const DenseArg
             }
             if (snap && v < N) snap[v] = vis[v];
         }
-        lor = wave_or64(lor);
-        cnt = wave_sum32(cnt);
-        if (lane == 0 && cnt) {
-            atomicOr(&s_or[level & 1], lor);
-            atomicAdd(&s_cnt[level & 1], cnt);
-        }
-        __syncthreads();
-        const unsigned long long blk = s_or[level & 1];
-        if (tid == 0) {
-            unsigned c = s_cnt[level & 1];
-            total += (long long)level * c;
-            reached += c;
-            if (reach) reach[level] = c;
-            s_or[(level + 1) & 1] = 0;
-            s_cnt[(level + 1) & 1] = 0;
-        }
-        if (tid < 64 && ((blk >> tid) & 1ull)) my_ecc = level;
-        if (blk == 0) break;
-        last_new = level;
-        if (level >= a.lmax) { flags |= GG_FLAG_LIMIT; break; }
+        if (end_level(level, cnt, lor)) break;
     }
     if (a.ecc_out && tid < nvalid) a.ecc_out[w * 64 + tid] = my_ecc;
     ColPart p;

@@ -3,9 +3,12 @@
 // Replaces cudaASPLconv / cudaASPLpiece / cudaASPLbox / cudaASPLgraph
 // (cudacode/*.cu).  One call = pack the table on the host, one async H2D, one
 // kernel launch (all levels, all columns, final reduction), one 48-byte D2H.
+#include <algorithm>
 #include <cstring>
 #include <cstdlib>
+#include <fstream>
 #include <mutex>
+#include <sstream>
 #include <string>
 #include <vector>
 #include "gg_common.hpp"
@@ -157,16 +160,28 @@ int device_cu_count(int device) {
 static const int kStaticLdsReserve = 2048;
 
 // Chooses the kernel variant and launches one dense evaluation of K candidates.
-int run_dense(DenseCtx &c, const DenseArgs &a, hipStream_t st) {
+// Level-1 frontier list capacity for the push levels (0 = pull only): the
+// list follows the column bitset in LDS and packs (vertex, bit) in 17+6 bits.
+static int push_capacity(const DenseCtx &c, int N, int D, int lds_cap) {
+    if (c.push_off || !c.push_ok || N > (1 << 17)) return 0;
+    long long room = ((long long)lds_cap - (long long)N * 8) / 4;
+    long long cap = 64LL * D;
+    if (room < cap) cap = room;
+    return cap >= 64 ? (int)cap : 0;
+}
+
+int run_dense(DenseCtx &c, const DenseArgs &a_in, hipStream_t st) {
+    DenseArgs a = a_in;
     const int tasks = a.K * a.W;
-    const size_t lds = (size_t)a.N * 8;
     int lds_cap = c.lds_limit_opt > 0 ? c.lds_limit_opt : c.lds_device - kStaticLdsReserve;
+    a.push_cap = push_capacity(c, a.N, a.D, lds_cap);
+    const size_t lds = (size_t)a.N * 8 + (size_t)a.push_cap * 4;
     int bs = c.block_opt;
     if (bs != 256 && bs != 1024) bs = (a.N <= 4096 && tasks >= 4 * c.cu_count) ? 256 : 1024;
     int vpt = pick_vpt(bs, (a.N + bs - 1) / bs);
     if (vpt == 0 && bs == 256) { bs = 1024; vpt = pick_vpt(bs, (a.N + bs - 1) / bs); }
     hipError_t e;
-    if ((int)lds <= lds_cap && vpt > 0) {
+    if ((size_t)a.N * 8 <= (size_t)lds_cap && vpt > 0) {
         c.last_variant = bs == 256 ? 1 : 2;
         switch (c.topo_kind) {
             case TOPO_RINGQ: {
@@ -192,6 +207,7 @@ int run_dense(DenseCtx &c, const DenseArgs &a, hipStream_t st) {
         }
         DenseArgs g = a;
         g.gscratch = c.gscratch;
+        g.push_cap = 0;
         c.last_variant = 3;
         switch (c.topo_kind) {
             case TOPO_RINGQ: e = launch_global(c.ringq, g, grid, tasks, st); break;
@@ -210,11 +226,13 @@ int run_dense(DenseCtx &c, const DenseArgs &a, hipStream_t st) {
 // Dense column BFS of the tasks in a device-side queue (ring topologies only;
 // 1024-thread workgroups, bitset in LDS).  Returns GG_UNSUPPORTED when the
 // column does not fit in LDS.
-int run_dense_queue(DenseCtx &c, const DenseArgs &a, int grid, hipStream_t st) {
-    const size_t lds = (size_t)a.N * 8;
+int run_dense_queue(DenseCtx &c, const DenseArgs &a_in, int grid, hipStream_t st) {
+    DenseArgs a = a_in;
     int lds_cap = c.lds_limit_opt > 0 ? c.lds_limit_opt : c.lds_device - kStaticLdsReserve;
+    a.push_cap = push_capacity(c, a.N, a.D, lds_cap);
+    const size_t lds = (size_t)a.N * 8 + (size_t)a.push_cap * 4;
     int vpt = pick_vpt(1024, (a.N + 1023) / 1024);
-    if ((int)lds > lds_cap || vpt == 0) return GG_UNSUPPORTED;
+    if ((size_t)a.N * 8 > (size_t)lds_cap || vpt == 0) return GG_UNSUPPORTED;
     hipError_t e;
     switch (c.topo_kind) {
         case TOPO_RINGQ: {
@@ -237,6 +255,48 @@ int run_dense_queue(DenseCtx &c, const DenseArgs &a, int grid, hipStream_t st) {
 int dense_fits_lds(const DenseCtx &c, int N) {
     int lds_cap = c.lds_limit_opt > 0 ? c.lds_limit_opt : c.lds_device - kStaticLdsReserve;
     return (size_t)N * 8 <= (size_t)lds_cap && pick_vpt(1024, (N + 1023) / 1024) > 0;
+}
+
+// Adjacency (entries < 0 = self-loop padding) is undirected: the directed
+// edge multiset equals its reverse.
+bool adj_symmetric(const int32_t *adj, int N, int D) {
+    std::vector<unsigned long long> fwd((size_t)N * D), rev((size_t)N * D);
+    for (int v = 0; v < N; v++)
+        for (int i = 0; i < D; i++) {
+            long long u = adj[(size_t)v * D + i];
+            if (u < 0) u = v;
+            if (u >= N) return false;
+            fwd[(size_t)v * D + i] = ((unsigned long long)v << 32) | (unsigned long long)u;
+            rev[(size_t)v * D + i] = ((unsigned long long)u << 32) | (unsigned long long)v;
+        }
+    std::sort(fwd.begin(), fwd.end());
+    std::sort(rev.begin(), rev.end());
+    return fwd == rev;
+}
+
+// Sufficient condition for a swap to keep an undirected graph undirected:
+// distinct (row, slot) entries whose removed and added directed edges are
+// each closed under reversal.
+bool swap_keeps_symmetry(const int32_t *adj, int D, const gg_swap &sw) {
+    std::vector<std::pair<int, int>> rem, add;
+    for (int j = 0; j < 4; j++) {
+        if (sw.v[j] < 0) continue;
+        for (int i = 0; i < j; i++)
+            if (sw.v[i] == sw.v[j] && sw.slot[i] == sw.slot[j]) return false;
+        int old = adj[(size_t)sw.v[j] * D + sw.slot[j]];
+        if (old < 0) old = sw.v[j];
+        rem.emplace_back(sw.v[j], old);
+        add.emplace_back(sw.v[j], sw.nbr[j]);
+    }
+    for (auto *e : {&rem, &add}) {
+        std::vector<std::pair<int, int>> r;
+        for (auto &p : *e) r.emplace_back(p.second, p.first);
+        std::vector<std::pair<int, int>> f = *e;
+        std::sort(f.begin(), f.end());
+        std::sort(r.begin(), r.end());
+        if (f != r) return false;
+    }
+    return true;
 }
 
 }  // namespace gg
@@ -266,6 +326,8 @@ struct gg_eval {
     long long *d_reach = nullptr;
     gg_swap *d_swap = nullptr;
     gg_swap *h_swap = nullptr;  // pinned
+    std::vector<int32_t> h_adj;  // adjacency evaluators: host copy of the resident base
+    bool base_sym = false;       // the resident base is undirected
     int have_base = 0;  // adjacency evaluators: a table is resident
     int lmax = GG_LEVEL_LIMIT;
     int have_result = 0;
@@ -332,9 +394,12 @@ static int check_device(int device) {
     return GG_OK;
 }
 
+static void tune_lookup(gg_eval *ev);
+
 static int finish_create(gg_eval *ev, size_t elem, gg_eval **out) {
     int st = eval_alloc(ev, elem);
     if (st != GG_OK) { gg_destroy(ev); return st; }
+    tune_lookup(ev);
     *out = ev;
     return GG_OK;
 }
@@ -565,6 +630,9 @@ extern "C" int gg_eval_adj(gg_eval *ev, const int32_t *adj, gg_result *out) {
     GG_HIP(hipSetDevice(ev->device));
     int st = pack_ring(ev, adj, ev->N, true);
     if (st != GG_OK) return st;
+    ev->h_adj.assign(adj, adj + (size_t)ev->N * ev->degree);
+    ev->base_sym = adj_symmetric(adj, ev->N, ev->degree);
+    ev->dc.push_ok = ev->base_sym;
     st = eval_common(ev, out);
     if (st == GG_OK) ev->have_base = 1;
     return st;
@@ -589,6 +657,7 @@ extern "C" int gg_eval_adj_swap(gg_eval *ev, const gg_swap *sw, gg_result *out) 
     *ev->h_swap = *sw;
     GG_HIP(hipMemcpyAsync(ev->d_swap, ev->h_swap, sizeof(gg_swap), hipMemcpyHostToDevice,
                           ev->stream));
+    ev->dc.push_ok = ev->base_sym && swap_keeps_symmetry(ev->h_adj.data(), ev->degree, *sw);
     return eval_common(ev, out, false, ev->d_swap);
 }
 
@@ -609,6 +678,9 @@ extern "C" int gg_eval_adj_commit(gg_eval *ev, const gg_swap *sw) {
     if (!ev || !sw || ev->kind != EVAL_ADJ || !ev->have_base) return GG_BAD_ARG;
     int st = check_swap(ev, sw);
     if (st != GG_OK) return st;
+    ev->base_sym = ev->base_sym && swap_keeps_symmetry(ev->h_adj.data(), ev->degree, *sw);
+    for (int j = 0; j < 4; j++)
+        if (sw->v[j] >= 0) ev->h_adj[(size_t)sw->v[j] * ev->degree + sw->slot[j]] = sw->nbr[j];
     GG_HIP(hipSetDevice(ev->device));
     *ev->h_swap = *sw;
     GG_HIP(hipMemcpyAsync(ev->d_swap, ev->h_swap, sizeof(gg_swap), hipMemcpyHostToDevice,
@@ -683,6 +755,9 @@ extern "C" int gg_eval_set_option(gg_eval *ev, int option, int value) {
             if (value < 1 || value > GG_LEVEL_LIMIT_MAX) return GG_BAD_ARG;
             ev->lmax = value;
             return GG_OK;
+        case GG_OPT_PUSH:
+            ev->dc.push_off = value ? 0 : 1;
+            return GG_OK;
         default: return GG_BAD_ARG;
     }
 }
@@ -694,7 +769,124 @@ extern "C" int gg_eval_get_option(gg_eval *ev, int option, int *value) {
         case GG_OPT_LDS_LIMIT: *value = ev->dc.lds_limit_opt; return GG_OK;
         case GG_OPT_GENERIC: *value = ev->dc.generic_opt; return GG_OK;
         case GG_OPT_LEVEL_LIMIT: *value = ev->lmax; return GG_OK;
+        case GG_OPT_PUSH: *value = ev->dc.push_off ? 0 : 1; return GG_OK;
         case 100: *value = ev->dc.last_variant; return GG_OK;  // kernel variant of the last call
         default: return GG_BAD_ARG;
     }
+}
+
+// ---- autotuner -----------------------------------------------------------------------------
+// Times the dense kernel variants (workgroup size x fixed/generic gather) on
+// the caller's graph, applies the fastest and records it in a cache file
+// (GG_TUNE_CACHE, default $HOME/.cache/gg_mi355x_tune.txt), keyed by device
+// architecture, evaluator kind, N, M and degree; evaluators created later with
+// the same key start from the cached choice.
+static std::mutex g_tune_mu;
+
+static std::string tune_cache_path() {
+    const char *e = getenv("GG_TUNE_CACHE");
+    if (e && *e) return e;
+    const char *h = getenv("HOME");
+    return std::string(h ? h : "/tmp") + "/.cache/gg_mi355x_tune.txt";
+}
+
+static std::string tune_key(const gg_eval *ev) {
+    hipDeviceProp_t prop;
+    std::string arch = "unknown";
+    if (hipGetDeviceProperties(&prop, ev->device) == hipSuccess) arch = prop.gcnArchName;
+    std::ostringstream k;
+    k << arch << ' ' << (int)ev->kind << ' ' << ev->N << ' ' << ev->M << ' ' << ev->degree;
+    return k.str();
+}
+
+static void tune_lookup(gg_eval *ev) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    std::ifstream f(tune_cache_path());
+    if (!f) return;
+    const std::string key = tune_key(ev);
+    std::string line;
+    while (std::getline(f, line)) {
+        if (line.compare(0, key.size(), key) != 0 || line.size() <= key.size() ||
+            line[key.size()] != ' ')
+            continue;
+        std::istringstream in(line.substr(key.size()));
+        int block = 0, generic = 0;
+        if (in >> block >> generic) {
+            ev->dc.block_opt = block;
+            ev->dc.generic_opt = generic;
+        }
+    }
+}
+
+static void tune_store(gg_eval *ev, int block, int generic, float ms) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    const std::string path = tune_cache_path(), key = tune_key(ev);
+    std::vector<std::string> keep;
+    {
+        std::ifstream f(path);
+        std::string line;
+        while (f && std::getline(f, line))
+            if (line.compare(0, key.size() + 1, key + " ") != 0) keep.push_back(line);
+    }
+    const size_t slash = path.rfind('/');
+    if (slash != std::string::npos) {
+        std::string dir = path.substr(0, slash);
+        std::string cmd = "mkdir -p '" + dir + "'";
+        if (system(cmd.c_str()) != 0) return;
+    }
+    std::ofstream f(path, std::ios::trunc);
+    for (auto &l : keep) f << l << '\n';
+    f << key << ' ' << block << ' ' << generic << ' ' << ms << '\n';
+}
+
+extern "C" int gg_eval_autotune(gg_eval *ev, const int32_t *table, int reps,
+                                gg_tune_report *report) {
+    if (!ev || !table || reps < 1) return GG_BAD_ARG;
+    GG_HIP(hipSetDevice(ev->device));
+    int st;
+    switch (ev->kind) {
+        case EVAL_PART: st = pack_ring(ev, table, ev->M, false); break;
+        case EVAL_ADJ: st = pack_ring(ev, table, ev->N, true); break;
+        case EVAL_PIECE: st = pack_piece(ev, table); break;
+        default: st = pack_box(ev, table); break;
+    }
+    if (st != GG_OK) return st;
+    const int saved_block = ev->dc.block_opt, saved_generic = ev->dc.generic_opt;
+    const int blocks[2] = {256, 1024};
+    float best_ms = 1e30f;
+    int best_block = 1024, best_generic = 0, n = 0;
+    gg_tune_report rep{};
+    for (int bi = 0; bi < 2; bi++)
+        for (int generic = 0; generic < 2; generic++) {
+            ev->dc.block_opt = blocks[bi];
+            ev->dc.generic_opt = generic;
+            std::vector<float> t;
+            gg_result r;
+            for (int i = 0; i < reps + 1; i++) {
+                st = eval_common(ev, &r, i == 0);
+                if (st != GG_OK) { ev->dc.block_opt = saved_block; ev->dc.generic_opt = saved_generic; return st; }
+                float ms = 0;
+                GG_HIP(hipEventElapsedTime(&ms, ev->ev0, ev->ev1));
+                if (i > 0) t.push_back(ms);
+            }
+            std::sort(t.begin(), t.end());
+            const float med = t[t.size() / 2];
+            if (n < 4) {
+                rep.block[n] = blocks[bi];
+                rep.generic[n] = generic;
+                rep.variant[n] = ev->dc.last_variant;
+                rep.ms[n] = med;
+                n++;
+            }
+            if (med < best_ms) { best_ms = med; best_block = blocks[bi]; best_generic = generic; }
+        }
+    ev->dc.block_opt = best_block;
+    ev->dc.generic_opt = best_generic;
+    rep.count = n;
+    rep.best_block = best_block;
+    rep.best_generic = best_generic;
+    rep.best_ms = best_ms;
+    tune_store(ev, best_block, best_generic, best_ms);
+    if (report) *report = rep;
+    return GG_OK;
 }

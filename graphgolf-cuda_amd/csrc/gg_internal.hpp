@@ -20,6 +20,8 @@ struct DenseCtx {
     int block_opt = 0;        // 0 = auto, else 256 / 1024
     int lds_limit_opt = 0;    // 0 = device limit
     int generic_opt = 0;      // 1 = runtime-degree kernels even when a specialisation exists
+    int push_off = 0;         // 1 = pull from level 1 (no push levels)
+    int push_ok = 0;          // the graph of this launch is known to be undirected
     int last_variant = 0;     // 1 = LDS/256, 2 = LDS/1024, 3 = global
     unsigned long long *gscratch = nullptr;
     size_t gscratch_bytes = 0;
@@ -31,6 +33,10 @@ int dense_fits_lds(const DenseCtx &c, int N);
 int pick_vpt(int bs, int need);
 int device_lds_limit(int device);
 int device_cu_count(int device);
+
+// Undirected-graph checks that gate the push levels (host side).
+bool adj_symmetric(const int32_t *adj, int N, int D);
+bool swap_keeps_symmetry(const int32_t *adj, int D, const gg_swap &sw);
 
 // Quad-packed table size / index helpers (TopoRingQ).
 inline int quads(int D) { return (D + 3) / 4; }

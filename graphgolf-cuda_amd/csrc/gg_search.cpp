@@ -260,8 +260,10 @@ static int search_step(gg_search *s) {
     }
     if (s->log_cap > 0 && (int64_t)s->log.size() < s->log_cap * 10) {
         const MoveRec &m = s->mv[bi];
-        int64_t rec[10] = {s->step, m.a, m.b, m.c, m.d, m.kind, y.total_scaled, y.diameter,
-                           y.status, accept ? 1 : 0};
+        // failed candidates log total = diameter = -1 (their partial values
+        // depend on the evaluator)
+        int64_t rec[10] = {s->step, m.a, m.b, m.c, m.d, m.kind, ok ? y.total_scaled : -1,
+                           ok ? y.diameter : -1, y.status, accept ? 1 : 0};
         s->log.insert(s->log.end(), rec, rec + 10);
     }
     if (accept) {

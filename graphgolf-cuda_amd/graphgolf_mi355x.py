@@ -31,6 +31,7 @@ GG_OPT_BLOCK = 1
 GG_OPT_LDS_LIMIT = 2
 GG_OPT_GENERIC = 3
 GG_OPT_LEVEL_LIMIT = 4
+GG_OPT_PUSH = 5
 GG_LEVEL_LIMIT_MAX = 199
 
 GG_BATCH_OPT_PATH = 1
@@ -58,6 +59,12 @@ class Result(C.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class TuneReport(C.Structure):
+    _fields_ = [("count", C.c_int32), ("block", C.c_int32 * 4), ("generic", C.c_int32 * 4),
+                ("variant", C.c_int32 * 4), ("ms", C.c_float * 4), ("best_block", C.c_int32),
+                ("best_generic", C.c_int32), ("best_ms", C.c_float)]
 
 
 class Swap(C.Structure):
@@ -134,6 +141,7 @@ def lib() -> C.CDLL:
         "gg_eval_set_option": (C.c_int, [vp, C.c_int, C.c_int]),
         "gg_eval_get_option": (C.c_int, [vp, C.c_int, P(C.c_int)]),
         "gg_eval_adj_swap": (C.c_int, [vp, P(Swap), P(Result)]),
+        "gg_eval_autotune": (C.c_int, [vp, i32p, C.c_int, P(TuneReport)]),
         "gg_eval_adj_commit": (C.c_int, [vp, P(Swap)]),
         "gg_batch_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, P(vp)]),
         "gg_batch_destroy": (None, [vp]),
@@ -259,6 +267,12 @@ class _Evaluator:
 
     def set_option(self, opt: int, value: int):
         check(lib().gg_eval_set_option(self._h, opt, value), "gg_eval_set_option")
+
+    def autotune(self, table, reps: int = 5) -> TuneReport:
+        t = _i32(table)
+        rep = TuneReport()
+        check(lib().gg_eval_autotune(self._h, _ptr(t), reps, C.byref(rep)), "gg_eval_autotune")
+        return rep
 
     def get_option(self, opt: int) -> int:
         v = C.c_int(0)

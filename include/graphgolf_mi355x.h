@@ -134,8 +134,25 @@ int gg_eval_last_kernel_ms(gg_eval *ev, float *ms);
 #define GG_OPT_GENERIC    3   /* 1 = runtime-degree kernel even when the degree has a
                                  compile-time specialisation (used by tests)          */
 #define GG_OPT_LEVEL_LIMIT 4  /* highest level evaluated (1..GG_LEVEL_LIMIT_MAX)      */
+#define GG_OPT_PUSH       5   /* 1 (default) = levels 1-2 push from the frontier; 0 = pull */
 int gg_eval_set_option(gg_eval *ev, int option, int value);
 int gg_eval_get_option(gg_eval *ev, int option, int *value);
+
+/* Autotuner: times the kernel variants (workgroup size 256/1024 x
+ * fixed-degree / runtime-degree gather) on `table` (the layout of the
+ * evaluator's eval call), `reps` launches each, applies the fastest and
+ * records it in the cache file named by GG_TUNE_CACHE (default
+ * $HOME/.cache/gg_mi355x_tune.txt), keyed by device architecture, evaluator
+ * kind, N, M and degree.  Evaluators created later with the same key start
+ * from the cached choice.  No reference counterpart. */
+typedef struct gg_tune_report {
+    int32_t count;            /* variants timed                         */
+    int32_t block[4], generic[4], variant[4];
+    float ms[4];              /* median kernel time of each variant     */
+    int32_t best_block, best_generic;
+    float best_ms;
+} gg_tune_report;
+int gg_eval_autotune(gg_eval *ev, const int32_t *table, int reps, gg_tune_report *report);
 
 /* ---- batched evaluation ----------------------------------------------------
  * K swap candidates against one resident base adjacency (N x degree, explicit

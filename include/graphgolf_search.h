@@ -70,7 +70,8 @@ int gg_search_run(gg_search *s, int steps, gg_search_stats *out);
 /* Current (best = 0) or best (best = 1) graph as N x d adjacency. */
 int gg_search_graph(gg_search *s, int32_t *adj_out, int best);
 /* Per-step log: 10 int64 per step {step, a, b, c, d, kind, total, diameter,
- * status, accepted} of the candidate that went through the acceptance rule. */
+ * status, accepted} of the candidate that went through the acceptance rule
+ * (total = diameter = -1 when its status is not GG_OK). */
 int gg_search_set_log(gg_search *s, int capacity);
 int gg_search_get_log(gg_search *s, int64_t *buf, int capacity, int *count);
 

@@ -120,7 +120,7 @@ def run(adj, evaluate, steps, mode_K=1, rule="anneal", T0=1e-3, T1=1e-6, total_s
             if dyv > dx or not ok:
                 acc = False
         mv = cands[bi][0]
-        log.append([step, *mv, ty, dy, 0 if ok else 1, int(acc)])
+        log.append([step, *mv, ty if ok else -1, dy if ok else -1, 0 if ok else 1, int(acc)])
         if acc:
             adj = new
             tx, dx = ty, dy

@@ -81,9 +81,10 @@ def test_random_regular_parity_with_levels(gpu, n, d, seed):
 @pytest.mark.parametrize("block", [256, 1024])
 @pytest.mark.parametrize("lds_limit", [0, 1024])
 @pytest.mark.parametrize("generic", [0, 1])
-def test_kernel_variants_agree(gpu, block, lds_limit, generic):
-    """256/1024-thread LDS kernels, the global-memory kernel, and the
-    fixed-degree (prefetching) vs runtime-degree gathers."""
+@pytest.mark.parametrize("push", [1, 0])
+def test_kernel_variants_agree(gpu, block, lds_limit, generic, push):
+    """256/1024-thread LDS kernels, the global-memory kernel, fixed-degree
+    (prefetching) vs runtime-degree gathers, push vs pull first levels."""
     gg = gpu
     adj = gg.random_regular(2048, 8, 17)
     o = ol.adj_eval(adj)
@@ -91,6 +92,7 @@ def test_kernel_variants_agree(gpu, block, lds_limit, generic):
         ev.set_option(gg.GG_OPT_BLOCK, block)
         ev.set_option(gg.GG_OPT_LDS_LIMIT, lds_limit)
         ev.set_option(gg.GG_OPT_GENERIC, generic)
+        ev.set_option(gg.GG_OPT_PUSH, push)
         r = ev.eval(adj)
         variant = ev.get_option(100)
         ecc, _ = ev.levels()
@@ -288,3 +290,24 @@ def test_repeated_calls_reset_state(gpu):
             for a in adjs:
                 r = ev.eval(a)
                 assert r.total_reps == ol.adj_eval(a)["total"]
+
+
+def test_autotuner_picks_and_caches(gpu, tmp_path, monkeypatch):
+    gg = gpu
+    monkeypatch.setenv("GG_TUNE_CACHE", str(tmp_path / "tune.txt"))
+    adj = gg.random_regular(4096, 12, 2)
+    o = ol.adj_eval(adj)
+    with gg.AdjEvaluator(4096, 12) as ev:
+        rep = ev.autotune(adj, reps=3)
+        assert rep.count == 4 and rep.best_ms == min(rep.ms[:4])
+        assert (ev.get_option(gg.GG_OPT_BLOCK), ev.get_option(gg.GG_OPT_GENERIC)) == \
+            (rep.best_block, rep.best_generic)
+        r = ev.eval(adj)
+        assert r.total_reps == o["total"]
+        print("autotune n=4096 d=12:", [(rep.block[i], rep.generic[i], round(rep.ms[i], 4))
+                                        for i in range(4)])
+    text = (tmp_path / "tune.txt").read_text()
+    assert " 4096 4096 12 " in text
+    with gg.AdjEvaluator(4096, 12) as ev2:  # starts from the cached choice
+        assert ev2.get_option(gg.GG_OPT_BLOCK) == rep.best_block
+        assert ev2.get_option(gg.GG_OPT_GENERIC) == rep.best_generic
