@@ -62,6 +62,10 @@ def plain_kernel_roofline(gg, adj, reps=30):
     rng = np.random.default_rng(7)
     ev = gg.SwapEvaluator(n, d)
     ev.set_base(adj)
+    info = {k: ev.get_option(v) for k, v in (("variant", gg.GG_INFO_VARIANT),
+                                               ("vpt", gg.GG_INFO_VPT),
+                                               ("fixed", gg.GG_INFO_FIXED),
+                                               ("push_cap", gg.GG_INFO_PUSH_CAP))}
     kms, lev = [], []
     while len(kms) < reps:
         a, c = (int(x) for x in rng.integers(0, n, 2))
@@ -78,10 +82,46 @@ def plain_kernel_roofline(gg, adj, reps=30):
     alg = 8.0 * (d + 2) * n * W * L
     t = float(np.mean(kms[3:])) * 1e-3
     ach = alg / t / 1e9
+    bs = 256 if info["variant"] == 1 else 1024
+    name = (f"k_dense_lds<TopoRingQ, {bs}, {info['vpt']}, {info['fixed']}>"
+            if info["variant"] in (1, 2) else "k_dense_global<TopoRingQ>")
     return {"bound": "lds", "achieved": ach, "peak": LDS_PEAK_GBS, "unit": "GB/s",
-            "frac": ach / LDS_PEAK_GBS, "traffic": None, "kernel": "k_dense_lds<TopoRing<u16>,1024,16>",
+            "frac": ach / LDS_PEAK_GBS, "traffic": hbm_traffic(n, d), "kernel": name,
             "kernel_ms": t * 1e3, "levels": L, "alg_bytes_per_launch": alg,
-            "hbm_frac": ach / HBM_PEAK_GBS}
+            "push_cap": info["push_cap"], "hbm_frac": ach / HBM_PEAK_GBS}
+
+
+def hbm_traffic(n, d):
+    """HBM bytes per launch of the dense kernel from the committed rocprofv3
+    --pmc summary for this shape (FETCH_SIZE + WRITE_SIZE, scaled by the
+    copy-kernel calibration recorded beside them), or None."""
+    path = os.path.join(REPO, "profiles", f"pmc_k_dense_lds_n{n}_d{d}.txt")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    with open(path) as f:
+        for line in f:
+            if ":" in line and not line.startswith("#"):
+                k, v = line.split(":", 1)
+                vals[k.strip()] = v.strip()
+    try:
+        return float(vals["hbm_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
+
+
+def copy_calibration(nbytes=1 << 30, reps=5):
+    """Device-to-device copies of a known size (torch's elementwise copy
+    kernel), the reference point that scales the FETCH_SIZE / WRITE_SIZE
+    counters to bytes."""
+    import torch
+    x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "reps": reps}),
+          flush=True)
 
 
 def run_search(gg, adj, cfg, warmup, steps):
@@ -106,6 +146,9 @@ def main():
     ap.add_argument("--seed", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--only", choices=["roofline", "copy"], default=None,
+                    help="run only the dense-kernel roofline launches, or only a device copy "
+                         "of known size (the two legs of the rocprofv3 --pmc traffic runs)")
     ap.add_argument("--equal-time", type=float, default=0.0,
                     help="also run plain and learned for this many seconds each and report "
                          "the final ASPL of both")
@@ -116,7 +159,13 @@ def main():
     import graphgolf_mi355x as gg
     assert gg.device_count() >= 1, "no HIP device"
     n, d = args.n, args.d
+    if args.only == "copy":
+        copy_calibration()
+        return
     adj = gg.random_regular(n, d, args.seed)
+    if args.only == "roofline":
+        print(json.dumps({"roofline": plain_kernel_roofline(gg, adj)}), flush=True)
+        return
     T0, T1 = 1e-3, 1e-6
     total = args.warmup + args.steps
 

@@ -159,7 +159,6 @@ int device_cu_count(int device) {
 // Static LDS used by the dense kernels besides the bitset (reduction scratch).
 static const int kStaticLdsReserve = 2048;
 
-// Chooses the kernel variant and launches one dense evaluation of K candidates.
 // Level-1 frontier list capacity for the push levels (0 = pull only): the
 // list follows the column bitset in LDS and packs (vertex, bit) in 17+6 bits.
 static int push_capacity(const DenseCtx &c, int N, int D, int lds_cap) {
@@ -170,6 +169,7 @@ static int push_capacity(const DenseCtx &c, int N, int D, int lds_cap) {
     return cap >= 64 ? (int)cap : 0;
 }
 
+// Chooses the kernel variant and launches one dense evaluation of K candidates.
 int run_dense(DenseCtx &c, const DenseArgs &a_in, hipStream_t st) {
     DenseArgs a = a_in;
     const int tasks = a.K * a.W;
@@ -183,13 +183,17 @@ int run_dense(DenseCtx &c, const DenseArgs &a_in, hipStream_t st) {
     hipError_t e;
     if ((size_t)a.N * 8 <= (size_t)lds_cap && vpt > 0) {
         c.last_variant = bs == 256 ? 1 : 2;
+        c.last_vpt = vpt;
+        c.last_fixed = 0;
+        c.last_push_cap = a.push_cap;
         switch (c.topo_kind) {
             case TOPO_RINGQ: {
                 bool done = false;
                 e = hipSuccess;
                 if (bs == 1024 && !c.generic_opt)
                     e = launch_fixed(c.ringq, a, tasks, lds, vpt, false, st, &done);
-                if (!done) e = launch_lds(c.ringq, a, tasks, lds, bs, vpt, st);
+                if (done) c.last_fixed = a.D;
+                else e = launch_lds(c.ringq, a, tasks, lds, bs, vpt, st);
                 break;
             }
             case TOPO_RING32: e = launch_lds(c.ring32, a, tasks, lds, bs, vpt, st); break;
@@ -209,6 +213,9 @@ int run_dense(DenseCtx &c, const DenseArgs &a_in, hipStream_t st) {
         g.gscratch = c.gscratch;
         g.push_cap = 0;
         c.last_variant = 3;
+        c.last_vpt = 0;
+        c.last_fixed = 0;
+        c.last_push_cap = 0;
         switch (c.topo_kind) {
             case TOPO_RINGQ: e = launch_global(c.ringq, g, grid, tasks, st); break;
             case TOPO_RING32: e = launch_global(c.ring32, g, grid, tasks, st); break;
@@ -771,6 +778,9 @@ extern "C" int gg_eval_get_option(gg_eval *ev, int option, int *value) {
         case GG_OPT_LEVEL_LIMIT: *value = ev->lmax; return GG_OK;
         case GG_OPT_PUSH: *value = ev->dc.push_off ? 0 : 1; return GG_OK;
         case 100: *value = ev->dc.last_variant; return GG_OK;  // kernel variant of the last call
+        case 101: *value = ev->dc.last_vpt; return GG_OK;      // its vertices per thread
+        case 102: *value = ev->dc.last_fixed; return GG_OK;    // its fixed degree (0 = generic)
+        case 103: *value = ev->dc.last_push_cap; return GG_OK; // its frontier list capacity
         default: return GG_BAD_ARG;
     }
 }

@@ -23,6 +23,9 @@ struct DenseCtx {
     int push_off = 0;         // 1 = pull from level 1 (no push levels)
     int push_ok = 0;          // the graph of this launch is known to be undirected
     int last_variant = 0;     // 1 = LDS/256, 2 = LDS/1024, 3 = global
+    int last_vpt = 0;         // vertices per thread of the last LDS launch
+    int last_fixed = 0;       // fixed-degree instance of the last launch (0 = generic)
+    int last_push_cap = 0;    // frontier list capacity of the last launch (0 = pull only)
     unsigned long long *gscratch = nullptr;
     size_t gscratch_bytes = 0;
 };

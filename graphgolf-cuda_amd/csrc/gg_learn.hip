@@ -155,69 +155,102 @@ __global__ void __launch_bounds__(256) k_mlp_forward(const float *X, int P,
 
 // Gradient of the mean squared error over a minibatch of B samples (rows
 // idx[0..B) of X/Y), forward with the bf16 weights, fp32 accumulation.
-// One workgroup.  pre[] is scratch [B][H].
-__global__ void __launch_bounds__(256) k_mlp_grad(const float *X, const float *Y, const int *idx,
-                                                  int B, const __hip_bfloat16 *wq, float *pre,
-                                                  float *grad, float *loss) {
+// Workgroup g takes samples [16g, 16g+16) and writes its partial gradient
+// (and partial squared error at index GG_NPARAMS) to part[g][]; k_mlp_reduce
+// sums the partials in workgroup order, so the result is deterministic.
+#define GG_GRAD_SAMPLES 16
+#define GG_PART_STRIDE (GG_NPARAMS + 1)
+__global__ void __launch_bounds__(256) k_mlp_grad_part(const float *X, const float *Y,
+                                                       const int *idx, int B,
+                                                       const __hip_bfloat16 *wq, float *part) {
     __shared__ float sw[GG_NPARAMS];
-    __shared__ float g[GG_MAX_BATCH];
-    __shared__ float sl[4];
-    const int tid = threadIdx.x;
+    __shared__ float sx[GG_GRAD_SAMPLES][GG_FEATS];
+    __shared__ float sa[GG_GRAD_SAMPLES][GG_HIDDEN];  // hidden pre-activations
+    __shared__ float sg[GG_GRAD_SAMPLES];             // dLoss/dOutput (0 for padding)
+    __shared__ float se[GG_GRAD_SAMPLES];             // squared error
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int s0 = blockIdx.x * GG_GRAD_SAMPLES;
+    const int ns = min(GG_GRAD_SAMPLES, B - s0);
     for (int i = tid; i < GG_NPARAMS; i += 256) sw[i] = bf(wq[i]);
-    __syncthreads();
-    float lsum = 0.f;
-    for (int b = tid; b < B; b += 256) {
-        const int r = idx ? idx[b] : b;
-        float x[GG_FEATS];
-#pragma unroll
-        for (int i = 0; i < GG_FEATS; i++) x[i] = X[(size_t)r * GG_FEATS + i];
-        float o = sw[OFF_B2];
-        for (int h = 0; h < GG_HIDDEN; h++) {
-            float acc = sw[OFF_B1 + h];
-#pragma unroll
-            for (int i = 0; i < GG_FEATS; i++) acc = fmaf(sw[OFF_W1 + h * GG_FEATS + i], x[i], acc);
-            pre[(size_t)b * GG_HIDDEN + h] = acc;
-            o = fmaf(sw[OFF_W2 + h], acc > 0.f ? acc : 0.f, o);
-        }
-        const float e = o - Y[r];
-        lsum += e * e;
-        g[b] = 2.f * e / (float)B;
+    for (int i = tid; i < GG_GRAD_SAMPLES * GG_FEATS; i += 256) {
+        const int s = i / GG_FEATS, f = i - s * GG_FEATS;
+        float v = 0.f;
+        if (s < ns) v = X[(size_t)(idx ? idx[s0 + s] : s0 + s) * GG_FEATS + f];
+        sx[s][f] = v;
     }
-    for (int o = 32; o > 0; o >>= 1) lsum += __shfl_xor(lsum, o, 64);
-    if ((tid & 63) == 0) sl[tid >> 6] = lsum;
     __syncthreads();
-    if (tid == 0 && loss) *loss = (sl[0] + sl[1] + sl[2] + sl[3]) / (float)B;
-    // per hidden unit: threads h and h+128 split the feature range
+    // threads h and h+128 each take half of the samples for hidden unit h
     const int h = tid & (GG_HIDDEN - 1), half = tid >> 7;
+    {
+        float w1[GG_FEATS];
+#pragma unroll
+        for (int i = 0; i < GG_FEATS; i++) w1[i] = sw[OFF_W1 + h * GG_FEATS + i];
+        const float b1 = sw[OFF_B1 + h];
+        for (int q = 0; q < GG_GRAD_SAMPLES / 2; q++) {
+            const int s = half * (GG_GRAD_SAMPLES / 2) + q;
+            float acc = b1;
+#pragma unroll
+            for (int i = 0; i < GG_FEATS; i++) acc = fmaf(w1[i], sx[s][i], acc);
+            sa[s][h] = acc;
+        }
+    }
+    __syncthreads();
+    for (int s = wv; s < GG_GRAD_SAMPLES; s += 4) {
+        float p = 0.f;
+        for (int k = lane; k < GG_HIDDEN; k += 64) {
+            const float a = sa[s][k];
+            p = fmaf(sw[OFF_W2 + k], a > 0.f ? a : 0.f, p);
+        }
+        for (int o = 32; o > 0; o >>= 1) p += __shfl_xor(p, o, 64);
+        if (lane == 0) {
+            float e = 0.f;
+            if (s < ns) e = (p + sw[OFF_B2]) - Y[idx ? idx[s0 + s] : s0 + s];
+            sg[s] = 2.f * e / (float)B;
+            se[s] = e * e;
+        }
+    }
+    __syncthreads();
     const float w2 = sw[OFF_W2 + h];
     float gw1[GG_FEATS / 2];
 #pragma unroll
     for (int i = 0; i < GG_FEATS / 2; i++) gw1[i] = 0.f;
     float gb1 = 0.f, gw2 = 0.f;
-    for (int b = 0; b < B; b++) {
-        const int r = idx ? idx[b] : b;
-        const float a = pre[(size_t)b * GG_HIDDEN + h];
+    for (int s = 0; s < GG_GRAD_SAMPLES; s++) {
+        const float a = sa[s][h];
         if (a <= 0.f) continue;
-        const float gb = g[b];
-        gw2 += gb * a;
-        const float gh = gb * w2;
+        const float g = sg[s];
+        gw2 = fmaf(g, a, gw2);
+        const float gh = g * w2;
         gb1 += gh;
 #pragma unroll
         for (int i = 0; i < GG_FEATS / 2; i++)
-            gw1[i] = fmaf(gh, X[(size_t)r * GG_FEATS + half * (GG_FEATS / 2) + i], gw1[i]);
+            gw1[i] = fmaf(gh, sx[s][half * (GG_FEATS / 2) + i], gw1[i]);
     }
+    float *out = part + (size_t)blockIdx.x * GG_PART_STRIDE;
 #pragma unroll
     for (int i = 0; i < GG_FEATS / 2; i++)
-        grad[OFF_W1 + h * GG_FEATS + half * (GG_FEATS / 2) + i] = gw1[i];
+        out[OFF_W1 + h * GG_FEATS + half * (GG_FEATS / 2) + i] = gw1[i];
     if (half == 0) {
-        grad[OFF_B1 + h] = gb1;
-        grad[OFF_W2 + h] = gw2;
+        out[OFF_B1 + h] = gb1;
+        out[OFF_W2 + h] = gw2;
     }
     if (tid == 0) {
-        float gb2 = 0.f;
-        for (int b = 0; b < B; b++) gb2 += g[b];
-        grad[OFF_B2] = gb2;
+        float gb2 = 0.f, l = 0.f;
+        for (int s = 0; s < GG_GRAD_SAMPLES; s++) { gb2 += sg[s]; l += se[s]; }
+        out[OFF_B2] = gb2;
+        out[GG_NPARAMS] = l;
     }
+}
+
+// Sums the G partial gradients (fixed order) into grad[] and the mean loss.
+__global__ void __launch_bounds__(256) k_mlp_reduce(const float *part, int G, int B, float *grad,
+                                                    float *loss) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i > GG_NPARAMS) return;
+    float acc = 0.f;
+    for (int g = 0; g < G; g++) acc += part[(size_t)g * GG_PART_STRIDE + i];
+    if (i < GG_NPARAMS) grad[i] = acc;
+    else if (loss) *loss = acc / (float)B;
 }
 
 // Adam step on the fp32 masters, then refresh the bf16 copies.
@@ -253,7 +286,8 @@ static int mlp_alloc(gg_mlp *m) {
     if (hipMalloc((void **)&m->d_v, 4 * P) != hipSuccess) return GG_OOM;
     if (hipMalloc((void **)&m->d_g, 4 * P) != hipSuccess) return GG_OOM;
     if (hipMalloc((void **)&m->d_wq, 2 * P) != hipSuccess) return GG_OOM;
-    if (hipMalloc((void **)&m->d_pre, 4 * (size_t)GG_MAX_BATCH * GG_HIDDEN) != hipSuccess)
+    if (hipMalloc((void **)&m->d_pre,
+                  4 * (size_t)(GG_MAX_BATCH / GG_GRAD_SAMPLES) * GG_PART_STRIDE) != hipSuccess)
         return GG_OOM;
     if (hipMalloc((void **)&m->d_loss, 4) != hipSuccess) return GG_OOM;
     m->xcap = 0;
@@ -352,8 +386,11 @@ int gg::mlp_forward_dev(gg_mlp *m, const float *d_x, int P, float *d_out, hipStr
 int gg::mlp_grad_dev(gg_mlp *m, const float *d_x, const float *d_y, const int *d_idx, int B,
                      hipStream_t st) {
     if (B < 1 || B > GG_MAX_BATCH) return GG_BAD_ARG;
-    hipLaunchKernelGGL(k_mlp_grad, dim3(1), dim3(256), 0, st, d_x, d_y, d_idx, B, m->d_wq,
-                       m->d_pre, m->d_g, m->d_loss);
+    const int G = (B + GG_GRAD_SAMPLES - 1) / GG_GRAD_SAMPLES;
+    hipLaunchKernelGGL(k_mlp_grad_part, dim3(G), dim3(256), 0, st, d_x, d_y, d_idx, B, m->d_wq,
+                       m->d_pre);
+    hipLaunchKernelGGL(k_mlp_reduce, dim3(GG_NPARAMS / 256 + 1), dim3(256), 0, st, m->d_pre, G,
+                       B, m->d_g, m->d_loss);
     GG_HIP(hipGetLastError());
     return GG_OK;
 }

@@ -16,7 +16,7 @@ struct gg_mlp {
     hipStream_t stream = nullptr;
     float *d_w = nullptr, *d_m = nullptr, *d_v = nullptr, *d_g = nullptr;  // fp32 masters, Adam, grad
     __hip_bfloat16 *d_wq = nullptr;  // bf16 weights used by every forward
-    float *d_pre = nullptr, *d_loss = nullptr;
+    float *d_pre = nullptr, *d_loss = nullptr;  // d_pre: per-workgroup partial gradients
     float *d_x = nullptr, *d_y = nullptr, *d_out = nullptr;
     int xcap = 0;
     int t = 0;  // Adam step

@@ -134,7 +134,14 @@ int gg_eval_last_kernel_ms(gg_eval *ev, float *ms);
 #define GG_OPT_GENERIC    3   /* 1 = runtime-degree kernel even when the degree has a
                                  compile-time specialisation (used by tests)          */
 #define GG_OPT_LEVEL_LIMIT 4  /* highest level evaluated (1..GG_LEVEL_LIMIT_MAX)      */
-#define GG_OPT_PUSH       5   /* 1 (default) = levels 1-2 push from the frontier; 0 = pull */
+#define GG_OPT_PUSH       5   /* 1 (default) = levels 1-2 push from the frontier when
+                                 the graph is undirected (adjacency evaluators verify
+                                 it on upload and per swap); 0 = pull only           */
+/* Read-only facts about the most recent launch (gg_eval_get_option). */
+#define GG_INFO_VARIANT   100 /* 1 = LDS bitset / 256 threads, 2 = LDS / 1024, 3 = HBM */
+#define GG_INFO_VPT       101 /* vertices per thread of the LDS kernel             */
+#define GG_INFO_FIXED     102 /* compile-time degree of the kernel (0 = runtime)     */
+#define GG_INFO_PUSH_CAP  103 /* frontier list capacity (0 = pull-only launch)      */
 int gg_eval_set_option(gg_eval *ev, int option, int value);
 int gg_eval_get_option(gg_eval *ev, int option, int *value);
 

@@ -94,11 +94,47 @@ def test_kernel_variants_agree(gpu, block, lds_limit, generic, push):
         ev.set_option(gg.GG_OPT_GENERIC, generic)
         ev.set_option(gg.GG_OPT_PUSH, push)
         r = ev.eval(adj)
-        variant = ev.get_option(100)
+        variant = ev.get_option(gg.GG_INFO_VARIANT)
+        fixed = ev.get_option(gg.GG_INFO_FIXED)
+        push_cap = ev.get_option(gg.GG_INFO_PUSH_CAP)
         ecc, _ = ev.levels()
     assert variant == (3 if lds_limit else (1 if block == 256 else 2))
+    assert fixed == (8 if (variant == 2 and not generic) else 0)
+    assert (push_cap > 0) == bool(push and not lds_limit)
     _check_adj(gg, adj, r, o)
     assert np.array_equal(ecc, o["ecc"])
+
+
+def test_push_only_on_undirected(gpu):
+    """The push levels run only where they equal the pull BFS: on an
+    undirected adjacency, for swaps that keep it undirected; directed
+    adjacencies and reference offset tables stay pull-only."""
+    gg = gpu
+    n, d = 1024, 6
+    adj = gg.random_regular(n, d, 3)
+    with gg.SwapEvaluator(n, d) as ev:
+        ev.set_base(adj)
+        assert ev.get_option(gg.GG_INFO_PUSH_CAP) > 0
+        c = 700
+        sw, new = gg.make_swap(adj, 5, 1, c, 2, 0)
+        while sw is None:
+            c += 1
+            sw, new = gg.make_swap(adj, 5, 1, c, 2, 0)
+        r = ev.eval_swap(sw)
+        assert ev.get_option(gg.GG_INFO_PUSH_CAP) > 0
+        o = ol.adj_eval(new)
+        assert r.total_reps == o["total"] and r.diameter == o["diameter"]
+    directed = adj.copy()
+    directed[0, 0] = next(v for v in range(1, n) if v not in adj[0])
+    o = ol.adj_eval(directed)
+    with gg.AdjEvaluator(n, d) as ev:
+        r = ev.eval(directed)
+        assert ev.get_option(gg.GG_INFO_PUSH_CAP) == 0
+    assert r.total_reps == o["total"]
+    tab = gg.orbit_part(8, 128, 6, 4)
+    with gg.PartEvaluator(8 * 128, 128, 6) as ev:
+        ev.eval(tab)
+        assert ev.get_option(gg.GG_INFO_PUSH_CAP) == 0
 
 
 @pytest.mark.parametrize("g,m,d,seed", [(32, 128, 16, 3), (6, 50, 4, 9), (4, 20, 3, 1),
