@@ -111,17 +111,54 @@ def hbm_traffic(n, d):
 
 
 def copy_calibration(nbytes=1 << 30, reps=5):
-    """Device-to-device copies of a known size (torch's elementwise copy
+    """Device-to-device hipMemcpy of a known size (the runtime's blit copy
     kernel), the reference point that scales the FETCH_SIZE / WRITE_SIZE
     counters to bytes."""
-    import torch
-    x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
-    y = torch.empty_like(x)
+    hip = C.CDLL("libamdhip64.so")
+    src, dst = C.c_void_p(), C.c_void_p()
+    for p in (src, dst):
+        assert hip.hipMalloc(C.byref(p), C.c_size_t(nbytes)) == 0, "hipMalloc"
+    assert hip.hipMemset(src, 1, C.c_size_t(nbytes)) == 0, "hipMemset"
     for _ in range(reps):
-        y.copy_(x)
-    torch.cuda.synchronize()
+        assert hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3) == 0, "hipMemcpy"  # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    for p in (src, dst):
+        hip.hipFree(p)
     print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "reps": reps}),
           flush=True)
+
+
+def batched_vs_sequential(gg, n=8192, d=16, K=256, steps=10, seed=4):
+    """K candidate 2-opt swaps on a random d-regular graph, one batched launch
+    per step (incremental path, HIP graph) against K sequential single-move
+    launches of the plain evaluator on the same candidates."""
+    rng = np.random.default_rng(seed)
+    adj = gg.random_regular(n, d, seed)
+    cands = []
+    while len(cands) < K:
+        a, c = (int(x) for x in rng.integers(0, n, 2))
+        ia, ic = (int(x) for x in rng.integers(0, d, 2))
+        sw, _ = gg.make_swap(adj, a, ia, c, ic, int(rng.integers(0, 2)))
+        if sw is not None:
+            cands.append(sw)
+    with gg.BatchEvaluator(n, d, max_k=K) as be:
+        be.set_base(adj)
+        rb = be.eval(cands)  # warm-up (captures the graph)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            rb = be.eval(cands)
+        tb = (time.perf_counter() - t0) / steps
+    ev = gg.SwapEvaluator(n, d)
+    ev.set_base(adj)
+    ev.eval_swap(cands[0])
+    t0 = time.perf_counter()
+    rs = [ev.eval_swap(sw) for sw in cands]
+    ts = time.perf_counter() - t0
+    ev.close()
+    assert all(a.total_reps == b.total_reps and a.diameter == b.diameter for a, b in zip(rb, rs))
+    return {"workload": f"K={K} 2-opt candidates, random {d}-regular graph n={n}",
+            "batched_ms_per_step": tb * 1e3, "sequential_ms": ts * 1e3,
+            "speedup": ts / tb, "batched_evals_per_s": K / tb, "sequential_evals_per_s": K / ts}
 
 
 def run_search(gg, adj, cfg, warmup, steps):
@@ -199,6 +236,7 @@ def main():
     }
     if not args.no_roofline:
         out["roofline"] = plain_kernel_roofline(gg, adj)
+    out["batched_vs_sequential"] = batched_vs_sequential(gg)
     if args.equal_time > 0:
         res = {}
         for name, cfg in (("plain", plain_cfg), ("learned", learn_cfg)):

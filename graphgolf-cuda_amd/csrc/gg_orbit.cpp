@@ -1,0 +1,284 @@
+// gg_orbit.cpp -- search driver over cyclically symmetric ("orbit") graphs in
+// the reference's `part` form: M representative rows of offsets, N = g*M,
+// neighbour i of vertex v = (v + table[v mod M][i]) mod N (cppcode/part.hpp:6-44).
+// Each candidate is a whole table, evaluated exactly from the M
+// representatives with the total scaled by g (cudaASPLconv::diameterASPL,
+// cudacode/cudaASPLconv.cu:197-243, through gg_eval_part).
+//
+// Moves follow createNeighbour for symmetric parts (cppcode/annealing.cpp:112-197):
+//   coin 0 (only when g > 1): edge-length change -- edge orbit (from, slot)
+//     with offset diff, partner row to = (from + diff) mod M and the slot
+//     holding -diff; with from <= to the pair becomes
+//     newdiff = (to - from) + M*k, -newdiff, k uniform in [0, g)
+//     (annealing.cpp:114-139);
+//   coin 1 (or g == 1): orbit 2-opt -- edge orbits (a->b) and (c->d) become
+//     (a,d) and (b,c), each with a fresh lift k (annealing.cpp:140-197).
+// Differences, fixed here on purpose (the same ones as the 2-opt driver):
+// every draw goes through gg::Rng, so a GPU run and a CPU run of this driver
+// draw the same moves; partner slots match offsets modulo N; and a move whose
+// table is not a simple graph (a zero offset or a repeated offset in a row,
+// i.e. a self-loop or a multi-edge in every copy) is redrawn, where the
+// reference keeps it.  Acceptance and schedule are those of the 2-opt driver
+// (annealing.cpp:406-412, HillClimb.cpp:239).
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+#include "graphgolf_mi355x.h"
+#include "graphgolf_search.h"
+#include "gg_rng.hpp"
+
+struct gg_orbit_search {
+    int N = 0, M = 0, d = 0, g = 0;
+    gg_search_cfg cfg{};
+    gg_eval *ev = nullptr;
+    gg_table_callbacks cb{};
+    bool use_cb = false;
+    std::unique_ptr<gg::Rng> rng;
+    std::vector<int32_t> tab, cand, best_tab;
+    gg_result cur{}, best{};
+    int64_t step = 0;
+    gg_search_stats stats{};
+    int log_cap = 0;
+    std::vector<int64_t> log;
+};
+
+namespace {
+
+struct OrbitMove {
+    int kind, a, ia, c, ic, k1, k2;
+};
+
+inline int64_t mod_n(int64_t x, int64_t N) {
+    x %= N;
+    return x < 0 ? x + N : x;
+}
+
+// Slot of row `row` holding the reverse of offset `diff` seen from row
+// `from`/slot `slot` (the same slot is skipped, as annealing.cpp:121 skips it
+// for diff == 0); -1 when there is none.
+int partner_slot(const std::vector<int32_t> &t, int N, int d, int row, int from, int slot,
+                 int32_t diff) {
+    for (int i = 0; i < d; i++) {
+        if (row == from && i == slot) continue;
+        if (mod_n((int64_t)t[(size_t)row * d + i] + diff, N) == 0) return i;
+    }
+    return -1;
+}
+
+// Row `r` describes simple neighbourhoods: offsets nonzero and distinct mod N.
+bool row_simple(const std::vector<int32_t> &t, int N, int d, int r) {
+    for (int i = 0; i < d; i++) {
+        int64_t x = mod_n(t[(size_t)r * d + i], N);
+        if (x == 0) return false;
+        for (int j = 0; j < i; j++)
+            if (mod_n(t[(size_t)r * d + j], N) == x) return false;
+    }
+    return true;
+}
+
+// One proposal attempt on `cur`, written to `out` (a copy of cur).  Returns
+// false when the move is degenerate or its table is not simple.
+bool propose_orbit(const std::vector<int32_t> &cur, int N, int M, int d, gg::Rng &rng,
+                   std::vector<int32_t> &out, OrbitMove *mv) {
+    const int g = N / M;
+    const int coin = (int)rng.below(2);
+    out = cur;
+    if (g > 1 && coin == 0) {
+        int from = (int)rng.below(M), ifrom = (int)rng.below(d);
+        const int32_t diff = cur[(size_t)from * d + ifrom];
+        int to = (int)(mod_n((int64_t)from + diff, N) % M);
+        int ito = partner_slot(cur, N, d, to, from, ifrom, diff);
+        if (ito < 0) return false;
+        if (from > to) { std::swap(from, to); std::swap(ifrom, ito); }
+        const int k = (int)rng.below(g);
+        const int32_t nd = (to - from) + M * k;
+        out[(size_t)from * d + ifrom] = nd;
+        out[(size_t)to * d + ito] = -nd;
+        *mv = OrbitMove{0, from, ifrom, to, ito, k, 0};
+        return row_simple(out, N, d, from) && row_simple(out, N, d, to);
+    }
+    int a = (int)rng.below(M), ia = (int)rng.below(d);
+    int c = (int)rng.below(M), ic = (int)rng.below(d);
+    const int32_t dab = cur[(size_t)a * d + ia], dcd = cur[(size_t)c * d + ic];
+    int b = (int)(mod_n((int64_t)a + dab, N) % M);
+    int dd = (int)(mod_n((int64_t)c + dcd, N) % M);
+    int ib = partner_slot(cur, N, d, b, a, ia, dab);
+    int id = partner_slot(cur, N, d, dd, c, ic, dcd);
+    if (ib < 0 || id < 0) return false;
+    // the same edge orbit drawn twice (annealing.cpp:168-172)
+    if ((c == a && ic == ia) || (c == b && ic == ib)) return false;
+    const int ka = (int)rng.below(g), kb = (int)rng.below(g);
+    *mv = OrbitMove{1, a, ia, c, ic, ka, kb};
+    if (a > dd) { std::swap(a, dd); std::swap(ia, id); }
+    const int32_t dad = (dd - a) + M * ka;
+    out[(size_t)a * d + ia] = dad;
+    out[(size_t)dd * d + id] = -dad;
+    if (b > c) { std::swap(b, c); std::swap(ib, ic); }
+    const int32_t dbc = (c - b) + M * kb;
+    out[(size_t)b * d + ib] = dbc;
+    out[(size_t)c * d + ic] = -dbc;
+    return row_simple(out, N, d, a) && row_simple(out, N, d, dd) && row_simple(out, N, d, b) &&
+           row_simple(out, N, d, c);
+}
+
+bool better(const gg_result &x, const gg_result &y) {
+    bool xo = x.status == GG_OK, yo = y.status == GG_OK;
+    if (xo != yo) return xo;
+    if (x.diameter != y.diameter) return x.diameter < y.diameter;
+    return x.total_scaled < y.total_scaled;
+}
+
+int eval_table(gg_orbit_search *s, const int32_t *t, gg_result *out) {
+    if (s->use_cb) return s->cb.eval(s->cb.ctx, t, out);
+    return gg_eval_part(s->ev, t, out);
+}
+
+int validate(int N, int M, int d, const int32_t *table, const gg_search_cfg *cfg) {
+    if (!cfg || !table || M < 1 || N < 5 || d < 2 || N % M != 0) return GG_BAD_ARG;
+    if (cfg->mode != GG_MODE_PLAIN) return GG_UNSUPPORTED;
+    if (cfg->rule != GG_RULE_ANNEAL && cfg->rule != GG_RULE_HILL) return GG_BAD_ARG;
+    if (cfg->rule == GG_RULE_ANNEAL && (!(cfg->T0 > 0) || !(cfg->T1 > 0))) return GG_BAD_ARG;
+    if (cfg->total_steps < 1) return GG_BAD_ARG;
+    return GG_OK;
+}
+
+int init(gg_orbit_search *s, int N, int M, int d, const int32_t *table, const gg_search_cfg *cfg) {
+    s->N = N; s->M = M; s->d = d; s->g = N / M; s->cfg = *cfg;
+    s->tab.assign(table, table + (size_t)M * d);
+    s->rng.reset(new gg::Rng(cfg->seed));
+    gg_result r;
+    int st = eval_table(s, s->tab.data(), &r);
+    if (st != GG_OK) return st;
+    s->cur = s->best = r;
+    s->best_tab = s->tab;
+    return GG_OK;
+}
+
+int orbit_step(gg_orbit_search *s) {
+    const gg_search_cfg &cfg = s->cfg;
+    gg::Rng &rng = *s->rng;
+    OrbitMove mv{};
+    int tries = 0;
+    while (!propose_orbit(s->tab, s->N, s->M, s->d, rng, s->cand, &mv))
+        if (++tries > 1000000) return GG_BAD_ARG;  // no simple move exists
+    s->stats.proposals++;
+    gg_result y;
+    int st = eval_table(s, s->cand.data(), &y);
+    if (st != GG_OK) return st;
+    s->stats.exact_evals++;
+    s->step++;
+    const double NN1 = (double)s->N * (s->N - 1);
+    const double fx = (double)s->cur.total_scaled / NN1;
+    const int dx = s->cur.diameter;
+    const bool ok = y.status == GG_OK;
+    const double fy = ok ? (double)y.total_scaled / NN1 : 1e8;
+    const int dy = ok ? y.diameter : 100000000;
+    bool accept = false;
+    if (cfg.rule == GG_RULE_HILL) {
+        accept = ok && (dy < dx || (dy == dx && fy < fx));
+    } else {
+        double T = cfg.T0 * std::pow(cfg.T1 / cfg.T0, (double)s->step / cfg.total_steps);
+        if (fy < fx || dy < dx) accept = true;
+        else if (rng.uniform() < std::exp((fx - fy) * NN1 / (T * NN1))) accept = true;
+        if (dy > dx || !ok) accept = false;
+    }
+    if (s->log_cap > 0 && (int64_t)s->log.size() < (int64_t)s->log_cap * 10) {
+        int64_t rec[10] = {s->step, mv.a * (int64_t)s->d + mv.ia, mv.c * (int64_t)s->d + mv.ic,
+                           mv.k1, mv.k2, mv.kind, ok ? y.total_scaled : -1, ok ? y.diameter : -1,
+                           y.status, accept ? 1 : 0};
+        s->log.insert(s->log.end(), rec, rec + 10);
+    }
+    if (accept) {
+        s->tab.swap(s->cand);
+        s->cur = y;
+        s->stats.accepted++;
+        if (better(s->cur, s->best)) {
+            s->best = s->cur;
+            s->best_tab = s->tab;
+        }
+    }
+    return GG_OK;
+}
+
+}  // namespace
+
+extern "C" int gg_orbit_search_create(int N, int M, int d, const int32_t *table,
+                                      const gg_search_cfg *cfg, int device,
+                                      gg_orbit_search **out) {
+    if (!out) return GG_BAD_ARG;
+    *out = nullptr;
+    int st = validate(N, M, d, table, cfg);
+    if (st != GG_OK) return st;
+    gg_orbit_search *s = new gg_orbit_search();
+    st = gg_create_part(N, M, d, device, &s->ev);
+    if (st == GG_OK) st = init(s, N, M, d, table, cfg);
+    if (st != GG_OK) { gg_orbit_search_destroy(s); return st; }
+    *out = s;
+    return GG_OK;
+}
+
+extern "C" int gg_orbit_search_create_cb(int N, int M, int d, const int32_t *table,
+                                         const gg_search_cfg *cfg, const gg_table_callbacks *cb,
+                                         gg_orbit_search **out) {
+    if (!out || !cb || !cb->eval) return GG_BAD_ARG;
+    *out = nullptr;
+    int st = validate(N, M, d, table, cfg);
+    if (st != GG_OK) return st;
+    gg_orbit_search *s = new gg_orbit_search();
+    s->cb = *cb;
+    s->use_cb = true;
+    st = init(s, N, M, d, table, cfg);
+    if (st != GG_OK) { gg_orbit_search_destroy(s); return st; }
+    *out = s;
+    return GG_OK;
+}
+
+extern "C" void gg_orbit_search_destroy(gg_orbit_search *s) {
+    if (!s) return;
+    gg_destroy(s->ev);
+    delete s;
+}
+
+extern "C" int gg_orbit_search_run(gg_orbit_search *s, int steps, gg_search_stats *out) {
+    if (!s || steps < 0) return GG_BAD_ARG;
+    auto t0 = std::chrono::steady_clock::now();
+    for (int i = 0; i < steps; i++) {
+        int st = orbit_step(s);
+        if (st != GG_OK) return st;
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    s->stats.seconds += std::chrono::duration<double>(t1 - t0).count();
+    s->stats.steps = s->step;
+    s->stats.total_cur = s->cur.total_scaled;
+    s->stats.diam_cur = s->cur.diameter;
+    s->stats.total_best = s->best.total_scaled;
+    s->stats.diam_best = s->best.diameter;
+    if (out) *out = s->stats;
+    return GG_OK;
+}
+
+extern "C" int gg_orbit_search_table(gg_orbit_search *s, int32_t *table_out, int best) {
+    if (!s || !table_out) return GG_BAD_ARG;
+    const std::vector<int32_t> &t = best ? s->best_tab : s->tab;
+    memcpy(table_out, t.data(), sizeof(int32_t) * t.size());
+    return GG_OK;
+}
+
+extern "C" int gg_orbit_search_set_log(gg_orbit_search *s, int capacity) {
+    if (!s || capacity < 0) return GG_BAD_ARG;
+    s->log_cap = capacity;
+    s->log.clear();
+    s->log.reserve((size_t)capacity * 10);
+    return GG_OK;
+}
+
+extern "C" int gg_orbit_search_get_log(gg_orbit_search *s, int64_t *buf, int capacity,
+                                       int *count) {
+    if (!s || !count) return GG_BAD_ARG;
+    int n = (int)(s->log.size() / 10);
+    *count = n;
+    if (buf) memcpy(buf, s->log.data(), sizeof(int64_t) * 10 * (size_t)std::min(n, capacity));
+    return GG_OK;
+}

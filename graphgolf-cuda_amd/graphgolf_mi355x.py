@@ -107,6 +107,13 @@ class EvalCallbacks(C.Structure):
                 ("commit", COMMIT_CB)]
 
 
+TABLE_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_int32), C.POINTER(Result))
+
+
+class TableCallbacks(C.Structure):
+    _fields_ = [("ctx", C.c_void_p), ("eval", TABLE_CB)]
+
+
 MODE_PLAIN, MODE_BATCHED, MODE_LEARNED = 0, 1, 2
 MLP_FEATURES, MLP_HIDDEN = 32, 128
 MLP_PARAMS = MLP_HIDDEN * MLP_FEATURES + MLP_HIDDEN + MLP_HIDDEN + 1
@@ -166,6 +173,15 @@ def lib() -> C.CDLL:
         "gg_search_get_log": (C.c_int, [vp, i64p, C.c_int, P(C.c_int)]),
         "gg_search_audit": (C.c_int, [vp, C.c_uint64, P(C.c_double), P(C.c_double),
                                       P(C.c_float), P(C.c_double)]),
+        "gg_orbit_search_create": (C.c_int, [C.c_int, C.c_int, C.c_int, i32p, P(SearchCfg),
+                                             C.c_int, P(vp)]),
+        "gg_orbit_search_create_cb": (C.c_int, [C.c_int, C.c_int, C.c_int, i32p, P(SearchCfg),
+                                                P(TableCallbacks), P(vp)]),
+        "gg_orbit_search_destroy": (None, [vp]),
+        "gg_orbit_search_run": (C.c_int, [vp, C.c_int, P(SearchStats)]),
+        "gg_orbit_search_table": (C.c_int, [vp, i32p, C.c_int]),
+        "gg_orbit_search_set_log": (C.c_int, [vp, C.c_int]),
+        "gg_orbit_search_get_log": (C.c_int, [vp, i64p, C.c_int, P(C.c_int)]),
         "gg_mlp_create": (C.c_int, [C.c_int, C.c_uint64, P(vp)]),
         "gg_mlp_destroy": (None, [vp]),
         "gg_mlp_set_params": (C.c_int, [vp, P(C.c_float)]),
@@ -513,6 +529,79 @@ class Search:
         if self._h:
             lib().gg_search_destroy(self._h)
             self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class OrbitSearch:
+    """Annealing / hill climbing over cyclically symmetric graphs in the
+    reference's `part` form (M rows of offsets, N = g*M): createNeighbour's
+    edge-length change and orbit 2-opt (cppcode/annealing.cpp:112-197), each
+    candidate table evaluated exactly from the M representatives.  With
+    `table_eval` (a Python function table[M][d] -> (total_scaled, diameter,
+    status)), the exact evaluator is the caller's."""
+
+    def __init__(self, N, table, cfg: SearchCfg, device: int = 0, table_eval=None, log: int = 0):
+        t = _i32(table)
+        self.N = N
+        self.M, self.d = t.shape
+        self._h = C.c_void_p()
+        if table_eval is None:
+            self._cb = None
+            check(lib().gg_orbit_search_create(N, self.M, self.d, _ptr(t), C.byref(cfg), device,
+                                               C.byref(self._h)), "gg_orbit_search_create")
+        else:
+            M, d = self.M, self.d
+
+            def ev(ctx, tab, out):
+                a = np.ctypeslib.as_array(tab, shape=(M * d,)).reshape(M, d).copy()
+                total, diam, status = table_eval(a)
+                r = out.contents
+                r.total_reps, r.total_scaled, r.diameter, r.wvc, r.levels, r.status = (
+                    total // (N // M), total, diam, 0, diam + 1, status)
+                return 0
+
+            self._fn = TABLE_CB(ev)
+            self._cb = TableCallbacks(None, self._fn)
+            check(lib().gg_orbit_search_create_cb(N, self.M, self.d, _ptr(t), C.byref(cfg),
+                                                  C.byref(self._cb), C.byref(self._h)),
+                  "gg_orbit_search_create_cb")
+        if log:
+            check(lib().gg_orbit_search_set_log(self._h, log), "gg_orbit_search_set_log")
+
+    def run(self, steps: int) -> SearchStats:
+        st = SearchStats()
+        check(lib().gg_orbit_search_run(self._h, steps, C.byref(st)), "gg_orbit_search_run")
+        return st
+
+    def table(self, best=False) -> np.ndarray:
+        out = np.empty((self.M, self.d), dtype=np.int32)
+        check(lib().gg_orbit_search_table(self._h, _ptr(out), 1 if best else 0),
+              "gg_orbit_search_table")
+        return out
+
+    def log(self) -> np.ndarray:
+        n = C.c_int(0)
+        check(lib().gg_orbit_search_get_log(self._h, None, 0, C.byref(n)), "get_log")
+        buf = np.zeros((n.value, 10), dtype=np.int64)
+        check(lib().gg_orbit_search_get_log(self._h, _ptr(buf, C.c_int64), n.value, C.byref(n)),
+              "get_log")
+        return buf
+
+    def close(self):
+        if self._h:
+            lib().gg_orbit_search_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
     def __del__(self):
         try:

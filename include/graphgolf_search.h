@@ -83,6 +83,39 @@ int gg_search_get_log(gg_search *s, int64_t *buf, int capacity, int *count);
 int gg_search_audit(gg_search *s, uint64_t seed, double *spearman, double *miss_rate,
                     float *scores_out, double *delta_out);
 
+/* ---- search over cyclically symmetric graphs ------------------------------
+ * The reference's `part` form (cppcode/part.hpp:6-44): M representative rows
+ * of `degree` offsets, N = g*M, neighbour i of vertex v is
+ * (v + table[v mod M][i]) mod N.  Moves are createNeighbour's for symmetric
+ * parts (cppcode/annealing.cpp:112-197: edge-length change and orbit 2-opt,
+ * with fresh lifts), redrawn when the table is not simple; each candidate
+ * table is evaluated exactly from the M representatives (gg_eval_part), and
+ * acceptance is the rule of gg_search (annealing.cpp:406-412 /
+ * HillClimb.cpp:239; pieceannealing.cpp:252-260 for the piece form of the
+ * same graphs).  Mode must be GG_MODE_PLAIN. */
+typedef struct gg_orbit_search gg_orbit_search;
+
+/* Exact table evaluator supplied by the caller (CPU twin of the driver). */
+typedef struct gg_table_callbacks {
+    void *ctx;
+    int (*eval)(void *ctx, const int32_t *table, gg_result *out);
+} gg_table_callbacks;
+
+int gg_orbit_search_create(int N, int M, int degree, const int32_t *table,
+                           const gg_search_cfg *cfg, int device, gg_orbit_search **out);
+int gg_orbit_search_create_cb(int N, int M, int degree, const int32_t *table,
+                              const gg_search_cfg *cfg, const gg_table_callbacks *cb,
+                              gg_orbit_search **out);
+void gg_orbit_search_destroy(gg_orbit_search *s);
+int gg_orbit_search_run(gg_orbit_search *s, int steps, gg_search_stats *out);
+/* Current (best = 0) or best (best = 1) table, M x degree offsets. */
+int gg_orbit_search_table(gg_orbit_search *s, int32_t *table_out, int best);
+/* Per-step log: 10 int64 per step {step, a*degree+slot_a, c*degree+slot_c,
+ * k1, k2, kind (0 = length change, 1 = 2-opt), total, diameter, status,
+ * accepted} (total = diameter = -1 when the status is not GG_OK). */
+int gg_orbit_search_set_log(gg_orbit_search *s, int capacity);
+int gg_orbit_search_get_log(gg_orbit_search *s, int64_t *buf, int capacity, int *count);
+
 #ifdef __cplusplus
 }
 #endif

@@ -125,3 +125,104 @@ def run(adj, evaluate, steps, mode_K=1, rule="anneal", T0=1e-3, T1=1e-6, total_s
             adj = new
             tx, dx = ty, dy
     return np.array(log, dtype=np.int64), adj
+
+
+# ---- symmetric ("orbit") part graphs: cppcode/annealing.cpp:112-197 ----------------------
+
+def _partner_slot(t, N, row, frm, slot, diff):
+    for i in range(t.shape[1]):
+        if row == frm and i == slot:
+            continue
+        if (int(t[row, i]) + diff) % N == 0:
+            return i
+    return -1
+
+
+def _row_simple(t, N, r):
+    xs = [int(x) % N for x in t[r]]
+    return 0 not in xs and len(set(xs)) == len(xs)
+
+
+def propose_orbit(t, N, rng):
+    """One attempt; returns (move record, new table) or None (redraw)."""
+    M, d = t.shape
+    g = N // M
+    coin = rng.below(2)
+    out = t.copy()
+    if g > 1 and coin == 0:
+        frm, ifrm = rng.below(M), rng.below(d)
+        diff = int(t[frm, ifrm])
+        to = ((frm + diff) % N) % M
+        ito = _partner_slot(t, N, to, frm, ifrm, diff)
+        if ito < 0:
+            return None
+        if frm > to:
+            frm, to, ifrm, ito = to, frm, ito, ifrm
+        k = rng.below(g)
+        nd = (to - frm) + M * k
+        out[frm, ifrm] = nd
+        out[to, ito] = -nd
+        if not (_row_simple(out, N, frm) and _row_simple(out, N, to)):
+            return None
+        return (frm * d + ifrm, to * d + ito, k, 0, 0), out
+    a, ia = rng.below(M), rng.below(d)
+    c, ic = rng.below(M), rng.below(d)
+    dab, dcd = int(t[a, ia]), int(t[c, ic])
+    b = ((a + dab) % N) % M
+    dd = ((c + dcd) % N) % M
+    ib = _partner_slot(t, N, b, a, ia, dab)
+    idd = _partner_slot(t, N, dd, c, ic, dcd)
+    if ib < 0 or idd < 0:
+        return None
+    if (c == a and ic == ia) or (c == b and ic == ib):
+        return None
+    ka, kb = rng.below(g), rng.below(g)
+    rec = (a * d + ia, c * d + ic, ka, kb, 1)
+    if a > dd:
+        a, dd, ia, idd = dd, a, idd, ia
+    out[a, ia] = (dd - a) + M * ka
+    out[dd, idd] = -((dd - a) + M * ka)
+    if b > c:
+        b, c, ib, ic = c, b, ic, ib
+    out[b, ib] = (c - b) + M * kb
+    out[c, ic] = -((c - b) + M * kb)
+    if not all(_row_simple(out, N, r) for r in (a, dd, b, c)):
+        return None
+    return rec, out
+
+
+def run_orbit(N, table, evaluate, steps, rule="hill", T0=1e-3, T1=1e-6, total_steps=None,
+              seed=1):
+    """evaluate(table) -> (total_scaled, diameter, ok).  Returns the per-step
+    log rows {step, a*d+slot_a, c*d+slot_c, k1, k2, kind, total, diameter,
+    status, accepted} and the final table."""
+    t = np.array(table, dtype=np.int32)
+    total_steps = total_steps or steps
+    rng = MT64(seed)
+    tx, dx, _ = evaluate(t)
+    NN1 = float(N) * (N - 1)
+    log = []
+    for step in range(1, steps + 1):
+        while True:
+            p = propose_orbit(t, N, rng)
+            if p is not None:
+                break
+        rec, new = p
+        ty, dy, ok = evaluate(new)
+        fx = tx / NN1
+        fy = ty / NN1 if ok else 1e8
+        dyv = dy if ok else 100000000
+        if rule == "hill":
+            acc = ok and (dyv < dx or (dyv == dx and fy < fx))
+        else:
+            T = T0 * math.pow(T1 / T0, step / total_steps)
+            acc = fy < fx or dyv < dx
+            if not acc and rng.uniform() < math.exp((fx - fy) * NN1 / (T * NN1)):
+                acc = True
+            if dyv > dx or not ok:
+                acc = False
+        log.append([step, *rec, ty if ok else -1, dy if ok else -1, 0 if ok else 1, int(acc)])
+        if acc:
+            t = new
+            tx, dx = ty, dy
+    return np.array(log, dtype=np.int64), t

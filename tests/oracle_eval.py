@@ -43,3 +43,20 @@ class OracleSwapEvaluator:
     def commit(self, sw):
         self.adj = self._apply(sw, self.adj)
         return evaluate(self.adj)
+
+
+def evaluate_part(N, table, threads=None):
+    """(total scaled by N/M, diameter, status) of a symmetric part table via
+    the oracle BFS from the M representatives (cpuASPLqueue.cpp:54-96)."""
+    t = np.ascontiguousarray(table, dtype=np.int32)
+    M = t.shape[0]
+    adj = ol.expand_part(N, M, t)
+    d = adj.shape[1]
+    srcs = np.arange(M, dtype=np.int32)
+    tot = C.c_int64()
+    dia = C.c_int32()
+    th = threads or min(16, os.cpu_count() or 1)
+    rc = ol.oracle().oracle_bfs_omp(N, d, ol._p(adj), M, ol._p(srcs), th, C.byref(tot),
+                                    C.byref(dia))
+    assert rc >= 0
+    return tot.value * (N // M), (dia.value if rc == 0 else 0), (0 if rc == 0 else 1)

@@ -21,7 +21,7 @@ def _eval_both(gg, be, swaps):
 
 
 @pytest.mark.parametrize("n,d,k", [(256, 4, 32), (1024, 6, 32), (4096, 12, 32), (8192, 16, 64),
-                                   (16384, 24, 32)])
+                                   (8192, 16, 256), (16384, 24, 32)])
 def test_sparse_matches_dense_and_oracle(gpu, n, d, k):
     gg = gpu
     print("sparse case", n, d, k, flush=True)

@@ -1,0 +1,76 @@
+"""The C++ orbit-graph search driver (symmetric `part` tables) against a
+pure-Python restatement of its semantics -- createNeighbour's edge-length
+change and orbit 2-opt (cppcode/annealing.cpp:112-197), RNG draws, acceptance
+-- both on the CPU oracle: identical per-step logs and final tables.  Every
+accepted table stays a symmetric simple regular graph."""
+import numpy as np
+import pytest
+
+import driver_ref
+import oracle_eval
+import oracle_lib as ol
+
+
+def _tab_eval(N):
+    def f(t):
+        tot, dia, st = oracle_eval.evaluate_part(N, t)
+        return tot, dia, st
+    return f
+
+
+def _check_symmetric_simple(N, t):
+    M, d = t.shape
+    for r in range(M):
+        xs = [int(x) % N for x in t[r]]
+        assert 0 not in xs and len(set(xs)) == d, (r, xs)
+        for x in xs:
+            partner = (r + x) % N % M
+            assert (-x) % N in [int(y) % N for y in t[partner]], (r, x)
+    adj = ol.expand_part(N, M, t)
+    # undirected: the expanded edge multiset equals its reverse
+    e = {(v, int(u)) for v in range(N) for u in adj[v]}
+    assert all((u, v) in e for v, u in e)
+
+
+@pytest.mark.parametrize("g,m,d,rule", [(8, 32, 6, "hill"), (8, 32, 6, "anneal"),
+                                        (1, 64, 4, "hill"), (5, 20, 4, "anneal")])
+def test_orbit_driver_matches_python_restatement(gg, g, m, d, rule):
+    N = g * m
+    tab = gg.orbit_part(g, m, d, 21) if g > 1 else ol_table_plain(gg, m, d)
+    steps = 80
+    cfg = gg.search_cfg(mode=gg.MODE_PLAIN,
+                        rule=gg.RULE_HILL if rule == "hill" else gg.RULE_ANNEAL,
+                        total_steps=steps, T0=2e-2, T1=1e-4, seed=99)
+    ev = _tab_eval(N)
+    with gg.OrbitSearch(N, tab, cfg, table_eval=ev, log=steps) as s:
+        st = s.run(steps)
+        log = s.log()
+        final = s.table()
+    ref_log, ref_tab = driver_ref.run_orbit(
+        N, tab, lambda t: (lambda r: (r[0], r[1], r[2] == 0))(ev(t)), steps, rule=rule,
+        T0=2e-2, T1=1e-4, seed=99)
+    assert np.array_equal(log, ref_log)
+    assert np.array_equal(final, ref_tab)
+    assert st.steps == steps and st.exact_evals == steps
+    assert st.accepted == int(ref_log[:, 9].sum())
+    if g > 1:
+        assert set(ref_log[:, 5].tolist()) == {0, 1}  # both move kinds drawn
+    _check_symmetric_simple(N, final)
+    tot, dia, status = ev(final)
+    assert status == 0 and st.total_cur == tot and st.diam_cur == dia
+    if rule == "hill":
+        t0, d0, _ = ev(tab)
+        assert (st.diam_cur, st.total_cur) <= (d0, t0)
+
+
+def ol_table_plain(gg, m, d):
+    """A plain graph (g = 1) as a part table: offsets v -> u as (u - v)."""
+    adj = gg.random_regular(m, d, 4)
+    return (adj - np.arange(m)[:, None]).astype(np.int32)
+
+
+def test_orbit_search_rejects_other_modes(gg):
+    tab = gg.orbit_part(4, 16, 4, 2)
+    cfg = gg.search_cfg(mode=gg.MODE_BATCHED, K=4, total_steps=10)
+    with pytest.raises(gg.GGError):
+        gg.OrbitSearch(64, tab, cfg, table_eval=_tab_eval(64))
