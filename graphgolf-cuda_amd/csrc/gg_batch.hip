@@ -51,7 +51,6 @@ static int batch_alloc(gg_batch *b) {
     if (hipHostMalloc((void **)&b->h_res, sizeof(ResultRec) * (K + 1), hipHostMallocDefault) !=
         hipSuccess)
         return GG_OOM;
-    if (hipMalloc((void **)&b->d_done, sizeof(unsigned) * (K + 1)) != hipSuccess) return GG_OOM;
     if (hipMalloc((void **)&b->d_swaps, sizeof(gg_swap) * (K + 1)) != hipSuccess) return GG_OOM;
     if (hipHostMalloc((void **)&b->h_swaps, sizeof(gg_swap) * (K + 1), hipHostMallocDefault) !=
         hipSuccess)
@@ -59,7 +58,6 @@ static int batch_alloc(gg_batch *b) {
     if (hipMalloc((void **)&b->d_ecc, sizeof(int) * N) != hipSuccess) return GG_OOM;
     if (hipMalloc((void **)&b->d_base_reach, sizeof(long long) * (size_t)W * L) != hipSuccess)
         return GG_OOM;
-    GG_HIP(hipMemsetAsync(b->d_done, 0, sizeof(unsigned) * (K + 1), b->stream));
     std::vector<int> src(N);
     for (int s = 0; s < N; s++) src[s] = s;
     GG_HIP(hipMemcpyAsync(b->d_src, src.data(), sizeof(int) * N, hipMemcpyHostToDevice, b->stream));
@@ -131,7 +129,7 @@ extern "C" void gg_batch_destroy(gg_batch *b) {
     (void)hipSetDevice(b->device);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     drop_graph(b);
-    void *bufs[] = {b->d_tab, b->d_adjR, b->d_src, b->d_parts, b->d_res, b->d_done, b->d_swaps,
+    void *bufs[] = {b->d_tab, b->d_adjR, b->d_src, b->d_parts, b->d_res, b->d_swaps,
                     b->d_ecc, b->d_base_reach, b->d_snap, b->d_dl_key, b->d_dl_bits, b->d_dl_ovr,
                     b->d_dl_cnt,
                     b->d_task_state, b->d_task_end, b->d_task_flags, b->d_dcnt, b->d_queue,
@@ -161,7 +159,7 @@ static void fill(const gg_batch *b, const ResultRec &r, gg_result *out) {
 }
 
 static DenseArgs dense_args(gg_batch *b, int K, const gg_swap *d_swaps, ColPart *parts,
-                            ResultRec *res, unsigned *done, int *ecc) {
+                            ResultRec *res, int *ecc) {
     DenseArgs a{};
     a.N = b->N; a.M = b->N; a.D = b->degree; a.W = b->W; a.K = K;
     a.lmax = GG_LEVEL_LIMIT;
@@ -169,7 +167,6 @@ static DenseArgs dense_args(gg_batch *b, int K, const gg_swap *d_swaps, ColPart 
     a.swaps = d_swaps;
     a.parts = parts;
     a.res = res;
-    a.done = done;
     a.ecc_out = ecc;
     return a;
 }
@@ -179,7 +176,7 @@ static DenseArgs dense_args(gg_batch *b, int K, const gg_swap *d_swaps, ColPart 
 static int eval_base(gg_batch *b, gg_result *out) {
     const int K = b->max_k;
     DenseArgs a = dense_args(b, 1, nullptr, b->d_parts + (size_t)K * b->W, b->d_res + K,
-                             b->d_done + K, b->d_ecc);
+                             b->d_ecc);
     a.reach_col = b->d_base_reach;
     a.snap = b->d_snap;
     a.snap_levels = b->d_snap ? b->snap_cap : 0;
@@ -210,7 +207,7 @@ extern "C" int gg_batch_set_base(gg_batch *b, const int32_t *adj, gg_result *out
         for (int i = 0; i < D; i++) {
             long long u = adj[(size_t)v * D + i];
             long long off = ((u - v) % N + N) % N;
-            if (b->narrow) ((uint16_t *)b->h_tab)[quad_index(i, N, v)] = (uint16_t)off;
+            if (b->narrow) ((uint16_t *)b->h_tab)[quad_index(i, N, v)] = (uint16_t)u;
             else ((uint32_t *)b->h_tab)[(size_t)i * N + v] = (uint32_t)off;
         }
     GG_HIP(hipMemcpyAsync(b->d_tab, b->h_tab, b->tab_bytes, hipMemcpyHostToDevice, b->stream));
@@ -296,7 +293,7 @@ static int enqueue_sparse(gg_batch *b, int K) {
         }
     }
     // dense fallback for queued tasks
-    DenseArgs a = dense_args(b, K, b->d_swaps, b->d_dense_parts, nullptr, nullptr, nullptr);
+    DenseArgs a = dense_args(b, K, b->d_swaps, b->d_dense_parts, nullptr, nullptr);
     a.queue = b->d_queue;
     a.queue_count = b->d_qctl;
     a.queue_next = b->d_qctl + 1;
@@ -321,7 +318,7 @@ static int enqueue_step(gg_batch *b, int K, bool sparse) {
     if (sparse) {
         st = enqueue_sparse(b, K);
     } else {
-        DenseArgs a = dense_args(b, K, b->d_swaps, b->d_parts, b->d_res, b->d_done, nullptr);
+        DenseArgs a = dense_args(b, K, b->d_swaps, b->d_parts, b->d_res, nullptr);
         st = run_dense(b->dc, a, b->stream);
     }
     if (st != GG_OK) return st;
@@ -376,7 +373,7 @@ __global__ void k_batch_commit(uint16_t *tab16, uint32_t *tab32, int *adjR, int 
         if (v < 0) continue;
         int off = sw->nbr[j] - v;
         if (off < 0) off += N;
-        if (tab16) tab16[quad_index(sw->slot[j], N, v)] = (uint16_t)off;
+        if (tab16) tab16[quad_index(sw->slot[j], N, v)] = (uint16_t)sw->nbr[j];
         else tab32[(size_t)sw->slot[j] * N + v] = (uint32_t)off;
         adjR[(size_t)v * D + sw->slot[j]] = sw->nbr[j];
     }

@@ -15,7 +15,6 @@ struct gg_batch {
     int *d_src = nullptr;
     ColPart *d_parts = nullptr;               // [(max_k + 1) * W]; slot max_k = base
     ResultRec *d_res = nullptr, *h_res = nullptr;  // [max_k + 1]
-    unsigned *d_done = nullptr;               // [max_k + 1]
     gg_swap *d_swaps = nullptr, *h_swaps = nullptr;  // [max_k + 1]
     int *d_ecc = nullptr;                     // [N] base per-source eccentricity
     long long *d_base_reach = nullptr;        // [W][LEVEL_LIMIT+1]

@@ -43,10 +43,11 @@ struct TopoRing {
     }
 };
 
-// Offset form with 16-bit offsets packed four per 64-bit word (N <= 65535):
-// word (q, r) holds the offsets of slots 4q..4q+3 of row r, slot 4q+j in bits
-// 16j..16j+15.  A vertex's D neighbour indices are ceil(D/4) coalesced 8-byte
-// loads per wave.
+// 16-bit entries packed four per 64-bit word (N <= 65535): word (q, r) holds
+// slots 4q..4q+3 of row r, slot 4q+j in bits 16j..16j+15.  A vertex's D
+// neighbours are ceil(D/4) coalesced 8-byte loads per wave.  When every
+// vertex is its own row (R == N) the entries are the neighbour ids
+// themselves; otherwise (symmetric tables, R < N) they are offsets mod N.
 struct TopoRingQ {
     const uint64_t *tab;  // [ceil(D/4)][R]
     unsigned N, R;
@@ -56,7 +57,9 @@ struct TopoRingQ {
     }
     __device__ __forceinline__ unsigned nbr(const Ctx &c, int i) const {
         uint64_t q = tab[(size_t)(i >> 2) * R + c.row];
-        unsigned u = c.v + ((unsigned)(q >> (16 * (i & 3))) & 0xFFFFu);
+        unsigned e = (unsigned)(q >> (16 * (i & 3))) & 0xFFFFu;
+        if (R == N) return e;
+        unsigned u = c.v + e;
         return u >= N ? u - N : u;
     }
 };
@@ -110,7 +113,6 @@ struct DenseArgs {
     const gg_swap *swaps;   // [K] slot replacements, or nullptr
     ColPart *parts;         // [K*W]
     ResultRec *res;         // [K]
-    unsigned *done;         // [K] finished-column counters (self-resetting)
     int *ecc_out;           // [M] per-source eccentricity, or nullptr
     long long *reach_col;   // [W*(lmax+1)] per-column level counts, or nullptr
     long long *reach_out;   // [lmax+1] summed level counts, or nullptr
@@ -186,39 +188,50 @@ __device__ __forceinline__ unsigned long long gather_or(const Topo &topo, Vis vi
     return hit ? gather_patched(topo, vis, v, D, sw) : gather_plain(topo, vis, v, own, D);
 }
 
-// Fixed-degree gather on the quad-packed ring topology: the ceil(DT/4) words
-// of a vertex are loaded up front (one memory latency per vertex), then the
-// neighbours are OR-ed from LDS in groups of four with the early exit.
+// Fixed-degree gather on the quad-packed ring topology with R == N (entries
+// are neighbour ids): the ceil(DT/4) words of a vertex are loaded up front (one
+// memory latency per vertex), then the neighbours are OR-ed from LDS in
+// groups of four with the early exit.
 template <int DT>
 struct FixedRow {
     static constexpr int Q = (DT + 3) / 4;
     unsigned long long q[Q];
-    unsigned v, n;
-    __device__ __forceinline__ void load(const TopoRingQ &topo, unsigned vv) {
-        v = vv;
-        n = topo.N;
-        const unsigned row = vv < topo.R ? vv : vv % topo.R;
+    // voff = 8 * vertex: 32-bit byte offsets from the table base, so each
+    // word's address is one add of a uniform stride
+    __device__ __forceinline__ void load(const char *base, unsigned stride8, unsigned voff) {
 #pragma unroll
-        for (int j = 0; j < Q; j++) q[j] = topo.tab[(size_t)j * topo.R + row];
+        for (int j = 0; j < Q; j++)
+            q[j] = *reinterpret_cast<const unsigned long long *>(base + (j * stride8 + voff));
+    }
+    // a patched row (the candidate's swap applied) staged in LDS
+    __device__ __forceinline__ void load_lds(const unsigned long long *p) {
+#pragma unroll
+        for (int j = 0; j < Q; j++) q[j] = p[j];
     }
     __device__ __forceinline__ unsigned u(int i) const {
-        unsigned x = v + ((unsigned)(q[i >> 2] >> (16 * (i & 3))) & 0xFFFFu);
-        return x >= n ? x - n : x;
+        return (unsigned)(q[i >> 2] >> (16 * (i & 3))) & 0xFFFFu;
     }
 };
 
+// Reads go out in groups of GG_GATHER_GROUP (all in flight together); the
+// word is tested after each group when `exits` (a wave-uniform flag: false on
+// levels where few words can fill up, which then skip the tests' round trips).
+#ifndef GG_GATHER_GROUP
+#define GG_GATHER_GROUP 8
+#endif
 template <int DT, class Vis>
 __device__ __forceinline__ unsigned long long gather_fixed(const FixedRow<DT> &row, Vis vis,
-                                                           unsigned long long own) {
+                                                           unsigned long long own, bool exits) {
+    constexpr int G = GG_GATHER_GROUP;
     unsigned long long acc = 0;
 #pragma unroll
-    for (int i = 0; i < DT; i += 4) {
+    for (int i = 0; i < DT; i += G) {
         unsigned long long g = vis[row.u(i)];
-        if (i + 1 < DT) g |= vis[row.u(i + 1)];
-        if (i + 2 < DT) g |= vis[row.u(i + 2)];
-        if (i + 3 < DT) g |= vis[row.u(i + 3)];
+#pragma unroll
+        for (int t = 1; t < G; t++)
+            if (i + t < DT) g |= vis[row.u(i + t)];
         acc |= g;
-        if (i + 4 < DT && (acc | own) == ~0ull) break;
+        if (i + G < DT && exits && (acc | own) == ~0ull) break;
         // compiler-only barrier: keeps the next group's LDS loads below the
         // early exit (hoisting all DT loads costs registers and spills)
         __asm__ volatile("" ::: "memory");
@@ -247,36 +260,30 @@ __device__ __forceinline__ unsigned long long pad_mask(int w, int M) {
     return ~0ull << (M - base);
 }
 
-// Last workgroup of candidate k reduces the column parts.  Called by all
-// threads of the block after thread 0 holds the finished part in `p`.
+// Reduces the column parts of candidate k = blockIdx.x into its ResultRec
+// (and the WVC and per-level reach of a single-candidate launch).  A kernel
+// of its own: the launch boundary orders the parts' stores before these
+// loads, where an in-kernel last-workgroup reduction needs an agent-scope
+// release per workgroup (an L2 writeback per XCD each time).
 template <int BS>
-__device__ void finish_column(const DenseArgs &a, int k, int w, const ColPart &p) {
-    __shared__ int s_last;
+__global__ void __launch_bounds__(BS) k_dense_finalize(DenseArgs a) {
     __shared__ long long s_tot[BS / 64], s_rch[BS / 64];
     __shared__ int s_diam[BS / 64], s_lev[BS / 64], s_fl[BS / 64];
     __shared__ int s_cls[32];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    if (tid == 0) a.parts[(size_t)k * a.W + w] = p;
-    __threadfence();
-    __syncthreads();
-    if (tid == 0) {
-        unsigned prev = atomicAdd(&a.done[k], 1u);
-        s_last = (prev == (unsigned)a.W - 1);
-    }
+    const int k = blockIdx.x;
     if (tid < 32) s_cls[tid] = -1;
     __syncthreads();
-    if (!s_last) return;
-    __threadfence();
     long long tot = 0, rch = 0;
     int diam = 0, lev = 0, fl = 0;
     for (int i = tid; i < a.W; i += BS) {
         const ColPart *q = a.parts + (size_t)k * a.W + i;
-        tot += ld_agent(&q->total);
-        rch += ld_agent(&q->reached);
-        int e = ld_agent(&q->ecc), l = ld_agent(&q->levels);
+        tot += q->total;
+        rch += q->reached;
+        int e = q->ecc, l = q->levels;
         diam = e > diam ? e : diam;
         lev = l > lev ? l : lev;
-        fl |= ld_agent(&q->flags);
+        fl |= q->flags;
     }
     tot = wave_sum64(tot);
     rch = wave_sum64(rch);
@@ -287,7 +294,7 @@ __device__ void finish_column(const DenseArgs &a, int k, int w, const ColPart &p
     if (lane == 0) { s_tot[wid] = tot; s_rch[wid] = rch; s_diam[wid] = diam; s_lev[wid] = lev; s_fl[wid] = fl; }
     // WVC: classes j in [0,32) merge sources j, j+32, ... (cudaASPLconv.cu:258-285)
     if (a.want_wvc && a.ecc_out) {
-        for (int s = tid; s < a.M; s += BS) atomicMax(&s_cls[s & 31], ld_agent(&a.ecc_out[s]));
+        for (int s = tid; s < a.M; s += BS) atomicMax(&s_cls[s & 31], a.ecc_out[s]);
     }
     __syncthreads();
     if (tid == 0) {
@@ -308,17 +315,30 @@ __device__ void finish_column(const DenseArgs &a, int k, int w, const ColPart &p
             for (int j = 0; j < 32; j++) wvc += (s_cls[j] == r.diameter) ? 1 : 0;
         r.wvc = wvc;
         a.res[k] = r;
-        a.done[k] = 0u;
     }
     if (a.reach_out && a.reach_col) {
-        int L = a.lmax + 1;
-        for (int l = tid; l < L; l += BS) {
+        // per-level histogram: one block-wide pass over the columns per level
+        // up to the deepest level executed, zeros above it
+        __shared__ long long s_red[BS / 64];
+        __syncthreads();  // s_lev[0] is the deepest level
+        const int L = a.lmax + 1;
+        const int top = s_lev[0] < L - 1 ? s_lev[0] : L - 1;
+        for (int l = top + 1 + tid; l < L; l += BS) a.reach_out[l] = 0;
+        for (int l = 0; l <= top; l++) {
             long long s = 0;
-            for (int i = 0; i < a.W; i++) {
+            for (int i = tid; i < a.W; i += BS) {
                 const ColPart *q = a.parts + (size_t)k * a.W + i;
-                if (l <= ld_agent(&q->levels)) s += ld_agent(&a.reach_col[(size_t)i * L + l]);
+                if (l <= q->levels) s += a.reach_col[(size_t)i * L + l];
             }
-            a.reach_out[l] = s;
+            s = wave_sum64(s);
+            if (lane == 0) s_red[wid] = s;
+            __syncthreads();
+            if (tid == 0) {
+                long long t = 0;
+                for (int x = 0; x < BS / 64; x++) t += s_red[x];
+                a.reach_out[l] = t;
+            }
+            __syncthreads();
         }
     }
 }
@@ -344,6 +364,29 @@ __device__ __forceinline__ ColPart dense_column(const Topo &topo, const DenseArg
     const unsigned long long pad = pad_mask(w, a.M);
     for (unsigned v = tid; v < N; v += BS) vis[v] = pad;
     if (tid < 2) { s_or[tid] = 0; s_cnt[tid] = 0; }
+    // fixed-degree kernels: the rows of the swapped vertices, patched, so every
+    // slot runs the same gather (row t of swap entry t, all entries on it applied)
+    constexpr bool kFixed = DT > 0 && std::is_same<Topo, TopoRingQ>::value;
+    constexpr int kQ = (DT + 3) / 4;
+    __shared__ unsigned long long s_prow[4 * (kQ > 0 ? kQ : 1)];
+    if constexpr (kFixed) {
+        if (has_sw && tid < 4 * kQ) {
+            const int t = tid / kQ, qq = tid - t * kQ;
+            const int vt = t == 0 ? sw.v[0] : t == 1 ? sw.v[1] : t == 2 ? sw.v[2] : sw.v[3];
+            if (vt >= 0) {
+                unsigned long long word = topo.tab[(size_t)qq * topo.R + (unsigned)vt];
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    if (sw.v[e] == vt && (sw.s[e] >> 2) == qq) {
+                        const int sh = 16 * (sw.s[e] & 3);
+                        word = (word & ~(0xFFFFull << sh)) |
+                               ((unsigned long long)(unsigned)sw.n[e] << sh);
+                    }
+                }
+                s_prow[t * kQ + qq] = word;
+            }
+        }
+    }
     __syncthreads();
     const int nvalid = (a.M - w * 64) < 64 ? (a.M - w * 64) : 64;
     if (tid < nvalid) {
@@ -359,13 +402,15 @@ __device__ __forceinline__ ColPart dense_column(const Topo &topo, const DenseArg
     // wave-blocked ownership: wave q owns vertices [64*VPT*q, 64*VPT*(q+1)),
     // slot j of lane l is vb + 64*j (coalesced per slot, small LDS offsets)
     const unsigned vb = (unsigned)(tid >> 6) * 64u * VPT + (unsigned)lane;
-    // slots of this lane whose row the candidate's swap patches (one VGPR)
-    unsigned hitmask = 0;
+    // slots of this lane whose row the candidate's swap patches (one VGPR),
+    // and for each the first swap entry on that vertex (2 bits per slot)
+    unsigned hitmask = 0, hitsel = 0;
     if (has_sw) {
 #pragma unroll
         for (int j = 0; j < VPT; j++) {
             const int v = (int)(vb + j * 64);
-            if (v == sw.v[0] || v == sw.v[1] || v == sw.v[2] || v == sw.v[3]) hitmask |= 1u << j;
+            const int t = v == sw.v[0] ? 0 : v == sw.v[1] ? 1 : v == sw.v[2] ? 2 : v == sw.v[3] ? 3 : -1;
+            if (t >= 0) { hitmask |= 1u << j; hitsel |= (unsigned)t << (2 * j); }
         }
     }
     auto snap_of = [&](int l) -> unsigned long long * {
@@ -375,6 +420,7 @@ __device__ __forceinline__ ColPart dense_column(const Topo &topo, const DenseArg
     // level bookkeeping shared by push and pull levels: folds this thread's
     // new-pair count and new-bit mask into the block, returns true when the
     // BFS of the column is finished
+    unsigned long long cover = (unsigned)nvalid;
     auto end_level = [&](int l, unsigned cnt, unsigned long long lor) -> bool {
         lor = wave_or64(lor);
         cnt = wave_sum32(cnt);
@@ -384,6 +430,7 @@ __device__ __forceinline__ ColPart dense_column(const Topo &topo, const DenseArg
         }
         __syncthreads();
         const unsigned long long blk = s_or[l & 1];
+        cover += s_cnt[l & 1];  // every thread: pairs reached so far
         if (tid == 0) {
             unsigned c = s_cnt[l & 1];
             total += (long long)l * c;
@@ -433,14 +480,41 @@ __device__ __forceinline__ ColPart dense_column(const Topo &topo, const DenseArg
             level = 2;
             cnt = 0;
             lor = 0;
-            for (int it = tid; it < nf * D; it += BS) {
-                const int e = it / D, i = it - e * D;
-                const unsigned f = flist[e];
-                const unsigned u = f & 0x1FFFFu, j = f >> 17;
-                const unsigned x = nbr_patched(topo, u, i, sw, has_sw);
-                const unsigned long long bit = 1ull << j;
-                const unsigned long long old = atomicOr(&vis[x], bit);
-                if (!(old & bit)) { cnt++; lor |= bit; }
+            if constexpr (kFixed) {
+                // one item per (frontier entry, table word): four neighbours
+                // from one 8-byte load; swapped rows come from s_prow
+                for (int it = tid; it < nf * kQ; it += BS) {
+                    const int e = it / kQ, q = it - e * kQ;
+                    const unsigned f = flist[e];
+                    const unsigned u = f & 0x1FFFFu, j = f >> 17;
+                    unsigned long long word;
+                    if (has_sw && ((int)u == sw.v[0] || (int)u == sw.v[1] || (int)u == sw.v[2] ||
+                                   (int)u == sw.v[3])) {
+                        const int t = (int)u == sw.v[0] ? 0 : (int)u == sw.v[1] ? 1
+                                    : (int)u == sw.v[2] ? 2 : 3;
+                        word = s_prow[t * kQ + q];
+                    } else {
+                        word = topo.tab[(size_t)q * topo.R + u];
+                    }
+                    const unsigned long long bit = 1ull << j;
+#pragma unroll
+                    for (int t = 0; t < 4; t++) {
+                        if (4 * q + t >= DT) break;
+                        const unsigned x = (unsigned)(word >> (16 * t)) & 0xFFFFu;
+                        const unsigned long long old = atomicOr(&vis[x], bit);
+                        if (!(old & bit)) { cnt++; lor |= bit; }
+                    }
+                }
+            } else {
+                for (int it = tid; it < nf * D; it += BS) {
+                    const int e = it / D, i = it - e * D;
+                    const unsigned f = flist[e];
+                    const unsigned u = f & 0x1FFFFu, j = f >> 17;
+                    const unsigned x = nbr_patched(topo, u, i, sw, has_sw);
+                    const unsigned long long bit = 1ull << j;
+                    const unsigned long long old = atomicOr(&vis[x], bit);
+                    if (!(old & bit)) { cnt++; lor |= bit; }
+                }
             }
             __syncthreads();
             if (unsigned long long *sn = snap_of(2))
@@ -455,21 +529,37 @@ __device__ __forceinline__ ColPart dense_column(const Topo &topo, const DenseArg
         unsigned long long nw[VPT];
         if constexpr (DT > 0 && std::is_same<Topo, TopoRingQ>::value) {
             // fixed degree: the next vertex's words load while this one gathers
+            const char *tbase = reinterpret_cast<const char *>(topo.tab);
+            const unsigned stride8 = topo.R * 8u;
+            // opaque per level: the row addresses are recomputed inside the
+            // level loop rather than kept live across it for every slot
+            unsigned voff = vb * 8u;
+            __asm__ volatile("" : "+v"(voff));
             FixedRow<DT> row[2];
-            if (vb < N) row[0].load(topo, vb);
+            auto load_row = [&](FixedRow<DT> &r, int j) {
+                if ((hitmask >> j) & 1u) r.load_lds(s_prow + ((hitsel >> (2 * j)) & 3u) * kQ);
+                else r.load(tbase, stride8, voff + (unsigned)j * 512u);
+            };
+            // under half of the column's pairs reached: words rarely fill up
+            // during this level, so skip the early-exit tests
+            const bool exits = 2 * cover >= (unsigned long long)N * (unsigned)nvalid;
+            // slots whose word is not full yet: only their rows are loaded
+            // (late levels touch few rows; the table reads are L2 traffic)
+            unsigned need = 0;
+#pragma unroll
+            for (int j = 0; j < VPT; j++) {
+                const unsigned v = vb + j * 64;
+                if (v < N && ~vis[v]) need |= 1u << j;
+            }
+            if (need & 1u) load_row(row[0], 0);
 #pragma unroll
             for (int j = 0; j < VPT; j++) {
                 unsigned v = vb + j * 64;
-                if (j + 1 < VPT && v + 64 < N) row[(j + 1) & 1].load(topo, v + 64);
+                if (j + 1 < VPT && ((need >> (j + 1)) & 1u)) load_row(row[(j + 1) & 1], j + 1);
                 nw[j] = 0;
-                if (v < N) {
+                if ((need >> j) & 1u) {
                     unsigned long long own = vis[v];
-                    if (~own) {
-                        unsigned long long acc = ((hitmask >> j) & 1u)
-                                                     ? gather_patched(topo, vis, v, D, sw)
-                                                     : gather_fixed<DT>(row[j & 1], vis, own);
-                        nw[j] = acc & ~own;
-                    }
+                    nw[j] = gather_fixed<DT>(row[j & 1], vis, own, exits) & ~own;
                 }
                 __asm__ volatile("" ::: "memory");  // one row in flight ahead, no more
             }
@@ -516,14 +606,14 @@ __device__ __forceinline__ ColPart dense_column(const Topo &topo, const DenseArg
     return p;
 }
 
-// One workgroup per (candidate, column); the last one of a candidate finalises it.
+// One workgroup per (candidate, column); k_dense_finalize then reduces the parts.
 template <class Topo, int BS, int VPT, int DT>
 __global__ void __launch_bounds__(BS) k_dense_lds(Topo topo, DenseArgs a) {
     extern __shared__ unsigned long long vis[];
     const int t = blockIdx.x;
     const int k = t / a.W, w = t - k * a.W;
     ColPart p = dense_column<Topo, BS, VPT, DT>(topo, a, k, w, vis);
-    finish_column<BS>(a, k, w, p);
+    if (threadIdx.x == 0) a.parts[(size_t)k * a.W + w] = p;
 }
 
 // Persistent workgroups draining a device-side task queue; parts only.
@@ -618,7 +708,7 @@ __global__ void __launch_bounds__(1024) k_dense_global(Topo topo, DenseArgs a, i
         p.levels = level;
         p.flags = flags;
         p.pad = 0;
-        finish_column<BS>(a, k, w, p);
+        if (tid == 0) a.parts[(size_t)k * a.W + w] = p;
         __syncthreads();
     }
 }

@@ -190,7 +190,7 @@ int run_dense(DenseCtx &c, const DenseArgs &a_in, hipStream_t st) {
             case TOPO_RINGQ: {
                 bool done = false;
                 e = hipSuccess;
-                if (bs == 1024 && !c.generic_opt)
+                if (bs == 1024 && !c.generic_opt && c.ringq.R == c.ringq.N)
                     e = launch_fixed(c.ringq, a, tasks, lds, vpt, false, st, &done);
                 if (done) c.last_fixed = a.D;
                 else e = launch_lds(c.ringq, a, tasks, lds, bs, vpt, st);
@@ -227,6 +227,12 @@ int run_dense(DenseCtx &c, const DenseArgs &a_in, hipStream_t st) {
         gg_set_last_hip_error(e, "dense BFS launch", __FILE__, __LINE__);
         return GG_HIP_ERROR;
     }
+    hipLaunchKernelGGL(k_dense_finalize<1024>, dim3(a.K), dim3(1024), 0, st, a);
+    e = hipGetLastError();
+    if (e != hipSuccess) {
+        gg_set_last_hip_error(e, "dense finalize launch", __FILE__, __LINE__);
+        return GG_HIP_ERROR;
+    }
     return GG_OK;
 }
 
@@ -245,7 +251,8 @@ int run_dense_queue(DenseCtx &c, const DenseArgs &a_in, int grid, hipStream_t st
         case TOPO_RINGQ: {
             bool done = false;
             e = hipSuccess;
-            if (!c.generic_opt) e = launch_fixed(c.ringq, a, grid, lds, vpt, true, st, &done);
+            if (!c.generic_opt && c.ringq.R == c.ringq.N)
+                e = launch_fixed(c.ringq, a, grid, lds, vpt, true, st, &done);
             if (!done) e = launch_lds_queue(c.ringq, a, grid, lds, vpt, st);
             break;
         }
@@ -327,7 +334,6 @@ struct gg_eval {
     ColPart *d_parts = nullptr;
     ResultRec *d_res = nullptr;
     ResultRec *h_res = nullptr;  // pinned
-    unsigned *d_done = nullptr;
     int *d_ecc = nullptr;
     long long *d_reach_col = nullptr;
     long long *d_reach = nullptr;
@@ -359,7 +365,6 @@ static int eval_alloc(gg_eval *ev, size_t tab_elem_bytes) {
     if (hipMalloc((void **)&ev->d_res, sizeof(ResultRec)) != hipSuccess) return GG_OOM;
     if (hipHostMalloc((void **)&ev->h_res, sizeof(ResultRec), hipHostMallocDefault) != hipSuccess)
         return GG_OOM;
-    if (hipMalloc((void **)&ev->d_done, sizeof(unsigned)) != hipSuccess) return GG_OOM;
     if (hipMalloc((void **)&ev->d_ecc, sizeof(int) * ev->M) != hipSuccess) return GG_OOM;
     if (hipMalloc((void **)&ev->d_reach_col, sizeof(long long) * (size_t)ev->W * L) != hipSuccess)
         return GG_OOM;
@@ -367,7 +372,6 @@ static int eval_alloc(gg_eval *ev, size_t tab_elem_bytes) {
     if (hipMalloc((void **)&ev->d_swap, sizeof(gg_swap)) != hipSuccess) return GG_OOM;
     if (hipHostMalloc((void **)&ev->h_swap, sizeof(gg_swap), hipHostMallocDefault) != hipSuccess)
         return GG_OOM;
-    GG_HIP(hipMemsetAsync(ev->d_done, 0, sizeof(unsigned), ev->stream));
     // source -> vertex map
     std::vector<int> src(ev->M);
     for (int s = 0; s < ev->M; s++) {
@@ -485,7 +489,6 @@ extern "C" void gg_destroy(gg_eval *ev) {
     (void)hipFree(ev->d_parts);
     (void)hipFree(ev->d_res);
     (void)hipHostFree(ev->h_res);
-    (void)hipFree(ev->d_done);
     (void)hipFree(ev->d_ecc);
     (void)hipFree(ev->d_reach_col);
     (void)hipFree(ev->d_reach);
@@ -522,6 +525,8 @@ static int pack_ring(gg_eval *ev, const int32_t *table, int R, bool adj) {
                 } else {
                     off = norm_mod(x, N);
                 }
+                // every vertex its own row: store the neighbour id itself
+                if (R == N) off = norm_mod(r + off, N);
                 h[quad_index(i, R, r)] = (uint16_t)off;
             }
     } else {
@@ -606,7 +611,6 @@ static int eval_common(gg_eval *ev, gg_result *out, bool upload = true,
     a.swaps = d_swap;
     a.parts = ev->d_parts;
     a.res = ev->d_res;
-    a.done = ev->d_done;
     a.ecc_out = ev->d_ecc;
     a.reach_col = ev->d_reach_col;
     a.reach_out = ev->d_reach;
@@ -668,7 +672,8 @@ extern "C" int gg_eval_adj_swap(gg_eval *ev, const gg_swap *sw, gg_result *out) 
     return eval_common(ev, out, false, ev->d_swap);
 }
 
-// Applies a swap to the resident table (ring topologies, R = N).
+// Applies a swap to the resident table (ring topologies, R = N: the 16-bit
+// table holds neighbour ids, the 32-bit one offsets).
 __global__ void k_commit_swap(uint16_t *tabq, uint32_t *tab32, int N, const gg_swap *sw) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     for (int j = 0; j < 4; j++) {
@@ -676,7 +681,7 @@ __global__ void k_commit_swap(uint16_t *tabq, uint32_t *tab32, int N, const gg_s
         if (v < 0) continue;
         int off = sw->nbr[j] - v;
         if (off < 0) off += N;
-        if (tabq) tabq[quad_index(sw->slot[j], N, v)] = (uint16_t)off;
+        if (tabq) tabq[quad_index(sw->slot[j], N, v)] = (uint16_t)sw->nbr[j];
         else tab32[(size_t)sw->slot[j] * N + v] = (uint32_t)off;
     }
 }
