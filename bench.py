@@ -183,9 +183,10 @@ def main():
     ap.add_argument("--seed", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
-    ap.add_argument("--only", choices=["roofline", "copy"], default=None,
-                    help="run only the dense-kernel roofline launches, or only a device copy "
-                         "of known size (the two legs of the rocprofv3 --pmc traffic runs)")
+    ap.add_argument("--only", choices=["roofline", "copy", "learned"], default=None,
+                    help="run only the dense-kernel roofline launches, only a device copy of "
+                         "known size (the two legs of the rocprofv3 --pmc traffic runs), or "
+                         "only the learned search (for kernel profiles of its step)")
     ap.add_argument("--equal-time", type=float, default=0.0,
                     help="also run plain and learned for this many seconds each and report "
                          "the final ASPL of both")
@@ -206,15 +207,21 @@ def main():
     T0, T1 = 1e-3, 1e-6
     total = args.warmup + args.steps
 
+    learn_cfg = gg.search_cfg(mode=gg.MODE_LEARNED, K=args.K, P=args.P, total_steps=total,
+                              T0=T0, T1=T1, seed=12345, warmup_pairs=2048, train_every=1,
+                              train_batch=256, lr=1e-3)
+    if args.only == "learned":
+        sl, stl, el, tl = run_search(gg, adj, learn_cfg, args.warmup, args.steps)
+        print(json.dumps({"learned_evals_per_s": el / tl, "ms_per_step": tl / args.steps * 1e3}),
+              flush=True)
+        return
+
     plain_cfg = gg.search_cfg(mode=gg.MODE_PLAIN, total_steps=total * args.K, T0=T0, T1=T1,
                               seed=12345)
     sp, stp, ep, tp = run_search(gg, adj, plain_cfg, args.warmup, args.steps)
     plain_rate = ep / tp
     sp.close()
 
-    learn_cfg = gg.search_cfg(mode=gg.MODE_LEARNED, K=args.K, P=args.P, total_steps=total,
-                              T0=T0, T1=T1, seed=12345, warmup_pairs=2048, train_every=1,
-                              train_batch=256, lr=1e-3)
     sl, stl, el, tl = run_search(gg, adj, learn_cfg, args.warmup, args.steps)
     learn_rate = el / tl
     rho, miss, _, _ = sl.audit(777)

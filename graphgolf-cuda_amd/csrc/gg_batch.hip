@@ -25,8 +25,20 @@ static const size_t kOverrideBytesMax = (size_t)4 << 30;
 static const int kDeltaCap = 2048;
 
 // The sparse kernel keeps one column (N words) in LDS plus small bookkeeping.
+// Static LDS of k_sparse (bookkeeping and the launch's swaps), rounded up.
+static const int kSparseStaticLds = 8192;
+static const int kStagedKeysMax = 8192;
+
 static bool sparse_fits(const gg_batch *b) {
-    return (size_t)b->N * 8 + 4096 <= (size_t)b->dc.lds_device;
+    return sparse_lds_bytes(b->N, 0) + kSparseStaticLds <= (size_t)b->dc.lds_device;
+}
+
+// Delta keys the sparse kernel can stage in the LDS left over.
+static int sparse_staged_keys(const gg_batch *b) {
+    long long room = ((long long)b->dc.lds_device - kSparseStaticLds -
+                      (long long)sparse_lds_bytes(b->N, 0)) / 4;
+    if (room < 0) room = 0;
+    return room < kStagedKeysMax ? (int)room : kStagedKeysMax;
 }
 
 static int batch_alloc(gg_batch *b) {
@@ -74,6 +86,7 @@ static int batch_alloc(gg_batch *b) {
         while (cap > 1 && per_level * cap > kSnapBytesMax) cap--;
         b->snap_cap = cap;
         b->cap = kDeltaCap < N ? kDeltaCap : N;
+        b->skcap = sparse_staged_keys(b);
         const int NB = (N + 31) / 32;
         const size_t per_task = 2 * ((size_t)N * 8 + (size_t)NB * 4 + (size_t)b->cap * 4);
         int chunk = K < GG_SPARSE_MAX_K ? K : GG_SPARSE_MAX_K;
@@ -268,6 +281,7 @@ static int enqueue_sparse(gg_batch *b, int K) {
     s.base_parts = b->d_parts + (size_t)b->max_k * W;
     s.base_reach = b->d_base_reach;
     s.cap = b->cap;
+    s.skcap = b->skcap;
     s.dl_key = b->d_dl_key;
     s.dl_cnt = b->d_dl_cnt;
     s.dl_bits = b->d_dl_bits;
@@ -281,7 +295,7 @@ static int enqueue_sparse(gg_batch *b, int K) {
     s.queue_next = b->d_qctl + 1;
     s.dense_parts = b->d_dense_parts;
     s.res = b->d_res;
-    const size_t lds = (size_t)N * 8;
+    const size_t lds = sparse_lds_bytes(N, b->skcap);
     for (int k0 = 0; k0 < K; k0 += b->chunk) {
         SparseArgs c = s;
         c.k0 = k0;
@@ -365,18 +379,33 @@ extern "C" int gg_batch_eval(gg_batch *b, const gg_swap *swaps, int K, gg_result
     return GG_OK;
 }
 
+// The swap travels as a kernel argument: no host buffer outlives the launch.
 __global__ void k_batch_commit(uint16_t *tab16, uint32_t *tab32, int *adjR, int N, int D,
-                               const gg_swap *sw) {
+                               gg_swap sw) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     for (int j = 0; j < 4; j++) {
-        int v = sw->v[j];
+        int v = sw.v[j];
         if (v < 0) continue;
-        int off = sw->nbr[j] - v;
+        int off = sw.nbr[j] - v;
         if (off < 0) off += N;
-        if (tab16) tab16[quad_index(sw->slot[j], N, v)] = (uint16_t)sw->nbr[j];
-        else tab32[(size_t)sw->slot[j] * N + v] = (uint32_t)off;
-        adjR[(size_t)v * D + sw->slot[j]] = sw->nbr[j];
+        if (tab16) tab16[quad_index(sw.slot[j], N, v)] = (uint16_t)sw.nbr[j];
+        else tab32[(size_t)sw.slot[j] * N + v] = (uint32_t)off;
+        adjR[(size_t)v * D + sw.slot[j]] = sw.nbr[j];
     }
+}
+
+// Applies the swap to the resident tables (host mirror and device).
+static int apply_commit(gg_batch *b, const gg_swap *sw) {
+    b->base_sym = b->base_sym && swap_keeps_symmetry(b->h_adj.data(), b->degree, *sw);
+    for (int j = 0; j < 4; j++)
+        if (sw->v[j] >= 0) b->h_adj[(size_t)sw->v[j] * b->degree + sw->slot[j]] = sw->nbr[j];
+    b->dc.push_ok = b->base_sym;
+    hipLaunchKernelGGL(k_batch_commit, dim3(1), dim3(64), 0, b->stream,
+                       b->narrow ? (uint16_t *)b->d_tab : nullptr,
+                       b->narrow ? nullptr : (uint32_t *)b->d_tab, b->d_adjR, b->N, b->degree,
+                       *sw);
+    GG_HIP(hipGetLastError());
+    return GG_OK;
 }
 
 extern "C" int gg_batch_commit(gg_batch *b, const gg_swap *sw, gg_result *out) {
@@ -384,20 +413,41 @@ extern "C" int gg_batch_commit(gg_batch *b, const gg_swap *sw, gg_result *out) {
     int st = check_swaps(b, sw, 1);
     if (st != GG_OK) return st;
     GG_HIP(hipSetDevice(b->device));
-    const int K = b->max_k;
-    b->base_sym = b->base_sym && swap_keeps_symmetry(b->h_adj.data(), b->degree, *sw);
-    for (int j = 0; j < 4; j++)
-        if (sw->v[j] >= 0) b->h_adj[(size_t)sw->v[j] * b->degree + sw->slot[j]] = sw->nbr[j];
-    b->dc.push_ok = b->base_sym;
-    b->h_swaps[K] = *sw;
-    GG_HIP(hipMemcpyAsync(b->d_swaps + K, b->h_swaps + K, sizeof(gg_swap), hipMemcpyHostToDevice,
-                          b->stream));
-    hipLaunchKernelGGL(k_batch_commit, dim3(1), dim3(64), 0, b->stream,
-                       b->narrow ? (uint16_t *)b->d_tab : nullptr,
-                       b->narrow ? nullptr : (uint32_t *)b->d_tab, b->d_adjR, b->N, b->degree,
-                       (const gg_swap *)(b->d_swaps + K));
-    GG_HIP(hipGetLastError());
+    st = apply_commit(b, sw);
+    if (st != GG_OK) return st;
     return eval_base(b, out);
+}
+
+// Commit of a candidate whose exact result the caller already holds (one of
+// the last batch's): the base re-evaluation that refreshes the stored levels
+// is queued on the batch stream, which orders every later launch after it,
+// and not waited for.  Errors of that launch surface at the next call.
+extern "C" int gg_batch_commit_known(gg_batch *b, const gg_swap *sw, const gg_result *known) {
+    if (!b || !sw || !known || !b->have_base) return GG_BAD_ARG;
+    int st = check_swaps(b, sw, 1);
+    if (st != GG_OK) return st;
+    GG_HIP(hipSetDevice(b->device));
+    st = apply_commit(b, sw);
+    if (st != GG_OK) return st;
+    const int K = b->max_k;
+    DenseArgs a = dense_args(b, 1, nullptr, b->d_parts + (size_t)K * b->W, b->d_res + K,
+                             b->d_ecc);
+    a.reach_col = b->d_base_reach;
+    a.snap = b->d_snap;
+    a.snap_levels = b->d_snap ? b->snap_cap : 0;
+    st = run_dense(b->dc, a, b->stream);
+    if (st != GG_OK) return st;
+    ResultRec r{};
+    r.total = known->total_reps;
+    r.reached = known->status == GG_OK ? (long long)b->N * b->N : 0;
+    r.diameter = known->diameter;
+    r.levels = known->levels;
+    r.flags = known->status == GG_DIAMETER_LIMIT ? GG_FLAG_LIMIT : 0;
+    r.wvc = known->wvc;
+    b->base_res = r;
+    b->sparse_ok = b->d_snap && known->status == GG_OK && known->diameter - 1 <= b->snap_cap &&
+                   b->dc.last_variant != 3;
+    return GG_OK;
 }
 
 extern "C" int gg_batch_last_kernel_ms(gg_batch *b, float *ms) {

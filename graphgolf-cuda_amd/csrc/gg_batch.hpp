@@ -22,6 +22,7 @@ struct gg_batch {
     int snap_cap = 0;                         // base levels whose words are stored
     unsigned long long *d_snap = nullptr;     // [snap_cap][W][N]
     int cap = 0;                              // Delta list capacity per task
+    int skcap = 0;                            // Delta keys staged in LDS per column and level
     int chunk = 0;                            // candidates per sparse launch
     int *d_dl_key = nullptr;
     unsigned *d_dl_bits = nullptr;            // [chunk*W][2][ceil(N/32)]

@@ -54,6 +54,7 @@ struct LearnState {
     bool norm_frozen = false;
     int64_t train_steps = 0;
     double last_loss = 0;
+    bool loss_pending = false;                   // a training step ran since the loss was read
     double spearman = 0, miss = 0;
     std::vector<int> order;
     std::unique_ptr<Rng> rng;                    // minibatch sampling / exploration
@@ -183,10 +184,7 @@ static int train_steps(gg_search *s, int steps) {
         if (st != GG_OK) return st;
         L->train_steps++;
     }
-    float loss = 0;
-    GG_HIP(hipMemcpyAsync(&loss, L->mlp->d_loss, 4, hipMemcpyDeviceToHost, L->st));
-    GG_HIP(hipStreamSynchronize(L->st));
-    L->last_loss = loss;
+    L->loss_pending = true;  // read when the stats are (no wait per step)
     return GG_OK;
 }
 
@@ -252,6 +250,13 @@ void learn_fill_stats(gg_search *s, gg_search_stats *st) {
     LearnState *L = s->learn;
     if (!L) return;
     st->train_steps = L->train_steps;
+    if (L->loss_pending) {
+        float loss = 0;
+        if (hipMemcpyAsync(&loss, L->mlp->d_loss, 4, hipMemcpyDeviceToHost, L->st) == hipSuccess &&
+            hipStreamSynchronize(L->st) == hipSuccess)
+            L->last_loss = loss;
+        L->loss_pending = false;
+    }
     st->train_loss = L->last_loss;
     st->spearman = L->spearman;
     st->miss_rate = L->miss;

@@ -37,19 +37,36 @@ namespace gg {
 
 struct PlainBackend : Backend {
     gg_eval *ev = nullptr;
+    std::vector<gg_swap> last_sw;     // candidates of the last eval and their results:
+    std::vector<gg_result> last_res;  // committing one of them needs no new evaluation
     PlainBackend(int N, int d, int device, int *st) { *st = gg_create_adj(N, d, device, &ev); }
     ~PlainBackend() override { gg_destroy(ev); }
-    int set_base(const int32_t *adj, gg_result *out) override { return gg_eval_adj(ev, adj, out); }
+    int set_base(const int32_t *adj, gg_result *out) override {
+        last_sw.clear();
+        return gg_eval_adj(ev, adj, out);
+    }
     int eval(const gg_swap *sw, int K, gg_result *out) override {
+        last_sw.assign(sw, sw + K);
+        last_res.resize(K);
         for (int k = 0; k < K; k++) {
             int st = gg_eval_adj_swap(ev, &sw[k], &out[k]);
-            if (st != GG_OK) return st;
+            if (st != GG_OK) { last_sw.clear(); return st; }
+            last_res[k] = out[k];
         }
         return GG_OK;
     }
     int commit(const gg_swap *sw, gg_result *out) override {
         int st = gg_eval_adj_commit(ev, sw);
         if (st != GG_OK) return st;
+        // the accepted graph is one just evaluated (annealing.cpp:426-428 keeps its
+        // result rather than evaluating again)
+        for (size_t k = 0; k < last_sw.size(); k++)
+            if (memcmp(&last_sw[k], sw, sizeof(gg_swap)) == 0) {
+                *out = last_res[k];
+                last_sw.clear();
+                return GG_OK;
+            }
+        last_sw.clear();
         gg_swap none;
         for (int j = 0; j < 4; j++) { none.v[j] = -1; none.slot[j] = 0; none.nbr[j] = 0; }
         return gg_eval_adj_swap(ev, &none, out);
@@ -66,10 +83,27 @@ struct BatchBackend : Backend {
     int set_base(const int32_t *adj, gg_result *out) override {
         return gg_batch_set_base(b, adj, out);
     }
+    std::vector<gg_swap> last_sw;     // candidates of the last eval and their results
+    std::vector<gg_result> last_res;
     int eval(const gg_swap *sw, int K, gg_result *out) override {
-        return gg_batch_eval(b, sw, K, out);
+        int st = gg_batch_eval(b, sw, K, out);
+        if (st == GG_OK) { last_sw.assign(sw, sw + K); last_res.assign(out, out + K); }
+        else last_sw.clear();
+        return st;
     }
-    int commit(const gg_swap *sw, gg_result *out) override { return gg_batch_commit(b, sw, out); }
+    int commit(const gg_swap *sw, gg_result *out) override {
+        // an evaluated candidate: its result is the new base's, and the
+        // re-evaluation that refreshes the stored levels runs unwaited
+        for (size_t k = 0; k < last_sw.size(); k++)
+            if (memcmp(&last_sw[k], sw, sizeof(gg_swap)) == 0) {
+                int st = gg_batch_commit_known(b, sw, &last_res[k]);
+                if (st == GG_OK) *out = last_res[k];
+                last_sw.clear();
+                return st;
+            }
+        last_sw.clear();
+        return gg_batch_commit(b, sw, out);
+    }
 };
 
 struct CallbackBackend : Backend {

@@ -8,6 +8,10 @@
 
 struct gg_batch;
 
+// Internal to the library (gg_batch.hip): commit of a candidate whose exact
+// result the caller holds; the base re-evaluation is queued, not waited for.
+extern "C" int gg_batch_commit_known(gg_batch *b, const gg_swap *sw, const gg_result *known);
+
 namespace gg {
 
 struct Backend {

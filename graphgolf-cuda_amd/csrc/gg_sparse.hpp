@@ -41,6 +41,7 @@ struct SparseArgs {
     const ColPart *base_parts;       // [W] base column parts (ecc, levels)
     const long long *base_reach;     // [W][lmax+1] base per-level new pairs
     int cap;                         // Delta list capacity per task
+    int skcap;                       // Delta keys of the previous level staged in LDS (max)
     // per task t = k*W + w (k local to the launch), parity p = level & 1
     int *dl_key;                     // [T][2][cap] Delta vertices
     int *dl_cnt;                     // [T][2]
@@ -88,16 +89,31 @@ __device__ __forceinline__ int cand_nbr(const int *adjR, int D, int v, int i, co
 }
 
 // Word of x at the previous level: the task's override where Delta_{l-1}
-// holds x, else the base word in LDS.
-__device__ __forceinline__ unsigned long long prev_word(const unsigned *bits,
+// holds x, else the base word in LDS.  `ubits` (LDS) is the union of
+// Delta_{l-1} over the column's tasks: outside it the base word is the
+// answer without touching the task's bitmap in global memory.
+__device__ __forceinline__ unsigned long long prev_word(const unsigned *ubits, const unsigned *bits,
                                                         const unsigned long long *ovr,
                                                         const unsigned long long *Bprev, int x) {
+    if (!((ubits[x >> 5] >> (x & 31)) & 1u)) return Bprev[x];
     return ((bits[x >> 5] >> (x & 31)) & 1u) ? ovr[x] : Bprev[x];
+}
+
+// Dynamic LDS of k_sparse: B_{l-1}[N] words, the Delta union bitmap, the
+// staged Delta keys.
+__host__ __device__ inline size_t sparse_lds_bytes(int N, int skcap) {
+    const size_t nb = ((size_t)(N + 31) / 32 + 1) & ~(size_t)1;  // even: keeps 8-byte alignment
+    return (size_t)N * 8 + nb * 4 + (size_t)skcap * 4;
 }
 
 template <int BS, int DT>
 __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a) {
     extern __shared__ unsigned long long Bprev[];
+    __shared__ gg_swap s_sw[GG_SPARSE_MAX_K];     // the launch's swaps
+    __shared__ int s_koff[GG_SPARSE_MAX_K + 1];   // staged Delta keys: offset per candidate
+    __shared__ int s_ooff[GG_SPARSE_MAX_K + 1];   // stale Delta_{l-2} entries: offset per candidate
+    __shared__ int s_nold[GG_SPARSE_MAX_K];
+    __shared__ int s_staged;
     __shared__ int s_items[GG_SPARSE_MAX_K + 1];  // prefix sums of items per candidate
     __shared__ int s_nprev[GG_SPARSE_MAX_K];
     __shared__ int s_ncur[GG_SPARSE_MAX_K];
@@ -114,6 +130,10 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a) {
     const int NB = (N + 31) >> 5;
     const int ecc_w = a.base_parts[w].ecc;
     const unsigned long long pad = pad_mask(w, N);
+    const int NBE = (NB + 1) & ~1;
+    unsigned *s_ubits = reinterpret_cast<unsigned *>(Bprev + N);  // [NBE]
+    int *s_keys = reinterpret_cast<int *>(s_ubits + NBE);          // [skcap]
+    for (int k = tid; k < K; k += BS) s_sw[k] = a.swaps[a.k0 + k];
     auto gt = [&](int k) { return (size_t)(a.k0 + k) * W + w; };     // global (cand, col)
     auto lt = [&](int k) { return (size_t)k * W + w; };              // task slot of this launch
 
@@ -150,21 +170,31 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a) {
 
     for (int l = 1; l <= a.lmax; l++) {
         const int par = (l - 1) & 1, cur = l & 1;
-        // B_{l-1} of this column into LDS
+        // B_{l-1} of this column into LDS (stored levels: 16-byte loads,
+        // eight per thread in flight before the LDS stores)
         const int lp = l - 1;
-        for (int v = tid; v < N; v += BS) {
-            unsigned long long b;
-            if (lp == 0) b = pad | (((v >> 6) == w) ? (1ull << (v & 63)) : 0ull);
-            else if (lp >= ecc_w) b = ~0ull;
-            else b = a.snap[((size_t)(lp - 1) * W + w) * N + v];
-            Bprev[v] = b;
+        if (lp >= 1 && lp < ecc_w && (N & 1) == 0) {
+            const uint4 *src = reinterpret_cast<const uint4 *>(a.snap + ((size_t)(lp - 1) * W + w) * N);
+            uint4 *dst = reinterpret_cast<uint4 *>(Bprev);
+            const int n2 = N >> 1;
+#pragma unroll 8
+            for (int i = tid; i < n2; i += BS) dst[i] = src[i];
+        } else {
+            for (int v = tid; v < N; v += BS) {
+                unsigned long long b;
+                if (lp == 0) b = pad | (((v >> 6) == w) ? (1ull << (v & 63)) : 0ull);
+                else if (lp >= ecc_w) b = ~0ull;
+                else b = a.snap[((size_t)(lp - 1) * W + w) * N + v];
+                Bprev[v] = b;
+            }
         }
-        // per-candidate bookkeeping; clear the stale bits of the cur parity
-        // (Delta_{l-2}) so Delta_l starts empty
+        for (int i = tid; i < NBE; i += BS) s_ubits[i] = 0u;
+        // per-candidate bookkeeping
         for (int k = tid; k < K; k += BS) {
             const bool act = a.task_state[gt(k)] == GG_TASK_ACTIVE;
             s_active[k] = act;
             s_nprev[k] = act ? a.dl_cnt[lt(k) * 2 + par] : 0;
+            s_nold[k] = act ? a.dl_cnt[lt(k) * 2 + cur] : 0;
             s_ncur[k] = 0;
             s_over[k] = 0;
             s_diff[k] = 0;
@@ -174,27 +204,56 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a) {
         clear_ended();
         __syncthreads();
         for (int k = tid; k < K; k += BS) s_ended[k] = 0;
-        for (int k = 0; k < K; k++) {
-            if (!s_active[k]) continue;
-            const int nold = a.dl_cnt[lt(k) * 2 + cur];
-            const int *okey = a.dl_key + (lt(k) * 2 + cur) * a.cap;
-            unsigned *cbits = a.dl_bits + (lt(k) * 2 + cur) * NB;
-            for (int e = tid; e < nold; e += BS) {
-                const int x = okey[e];
-                atomicAnd(&cbits[x >> 5], ~(1u << (x & 31)));
-            }
-        }
         if (tid == 0) {
-            int acc = 0;
+            int acc = 0, kacc = 0, oacc = 0;
             for (int k = 0; k < K; k++) {
                 s_items[k] = acc;
-                if (s_active[k]) acc += 4 + s_nprev[k] * (D + 1);
+                s_koff[k] = kacc;
+                s_ooff[k] = oacc;
+                if (s_active[k]) {
+                    acc += 4 + s_nprev[k] * (D + 1);
+                    kacc += s_nprev[k];
+                    oacc += s_nold[k];
+                }
             }
             s_items[K] = acc;
+            s_koff[K] = kacc;
+            s_ooff[K] = oacc;
+            s_staged = kacc <= a.skcap;
+        }
+        __syncthreads();
+        // last k with off[k] <= i (empty candidates share their successor's start)
+        auto owner = [&](const int *off, int i) {
+            int k = 0, hi = K;
+            while (hi - k > 1) {
+                const int mid = (k + hi) >> 1;
+                if (off[mid] <= i) k = mid;
+                else hi = mid;
+            }
+            return k;
+        };
+        // flat over (candidate, entry): clear the stale bits of the cur parity
+        // (Delta_{l-2}) so Delta_l starts empty; build the union of the
+        // Delta_{l-1} (LDS bitmap) and stage their keys
+        {
+            const int nold_all = s_ooff[K], nprev_all = s_koff[K];
+            const bool stg = s_staged;
+            for (int i = tid; i < nold_all; i += BS) {
+                const int k = owner(s_ooff, i);
+                const int x = a.dl_key[(lt(k) * 2 + cur) * a.cap + (i - s_ooff[k])];
+                atomicAnd(&a.dl_bits[(lt(k) * 2 + cur) * NB + (x >> 5)], ~(1u << (x & 31)));
+            }
+            for (int i = tid; i < nprev_all; i += BS) {
+                const int k = owner(s_koff, i);
+                const int x = a.dl_key[(lt(k) * 2 + par) * a.cap + (i - s_koff[k])];
+                atomicOr(&s_ubits[x >> 5], 1u << (x & 31));
+                if (stg) s_keys[i] = x;
+            }
         }
         __threadfence_block();
         __syncthreads();
         const int total = s_items[K];
+        const bool staged = s_staged;
         // shared pool of items over all active candidates
         for (;;) {
             int base_it = 0;
@@ -203,13 +262,12 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a) {
             if (base_it >= total) break;
             const int it = base_it + (tid & 63);
             if (it >= total) continue;
-            int k = 0;
-            while (s_items[k + 1] <= it) k++;
+            const int k = owner(s_items, it);
             const int loc = it - s_items[k];
             const size_t tl = lt(k);
             SwapRegs sw;
             bool has_sw;
-            load_swap(a.swaps, a.k0 + k, sw, has_sw);
+            load_swap(s_sw, k, sw, has_sw);
             const unsigned *pbits = a.dl_bits + (tl * 2 + par) * NB;
             const unsigned long long *povr = a.dl_ovr + (tl * 2 + par) * (size_t)N;
             int v;
@@ -218,7 +276,7 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a) {
                 if (v < 0) continue;
             } else {
                 const int e = (loc - 4) / (D + 1), i = (loc - 4) - e * (D + 1);
-                const int u = a.dl_key[(tl * 2 + par) * a.cap + e];
+                const int u = staged ? s_keys[s_koff[k] + e] : a.dl_key[(tl * 2 + par) * a.cap + e];
                 v = i == 0 ? u : cand_nbr(a.adjR, D, u, i - 1, sw);
             }
             unsigned long long basew = ~0ull;  // issued before the gather
@@ -233,18 +291,19 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a) {
 #pragma unroll
                     for (int i = 0; i < DT; i++) acc |= Bprev[row.u[i]];
                 } else {
-                    acc = prev_word(pbits, povr, Bprev, v);
+                    acc = prev_word(s_ubits, pbits, povr, Bprev, v);
 #pragma unroll
-                    for (int i = 0; i < DT; i++) acc |= prev_word(pbits, povr, Bprev, row.u[i]);
+                    for (int i = 0; i < DT; i++)
+                        acc |= prev_word(s_ubits, pbits, povr, Bprev, row.u[i]);
                 }
             } else {
                 if (none) {
                     acc = Bprev[v];
                     for (int i = 0; i < D; i++) acc |= Bprev[cand_nbr(a.adjR, D, v, i, sw)];
                 } else {
-                    acc = prev_word(pbits, povr, Bprev, v);
+                    acc = prev_word(s_ubits, pbits, povr, Bprev, v);
                     for (int i = 0; i < D; i++)
-                        acc |= prev_word(pbits, povr, Bprev, cand_nbr(a.adjR, D, v, i, sw));
+                        acc |= prev_word(s_ubits, pbits, povr, Bprev, cand_nbr(a.adjR, D, v, i, sw));
                 }
             }
             if (acc == basew) continue;
