@@ -98,7 +98,6 @@ static int batch_alloc(gg_batch *b) {
             return GG_OOM;
         if (hipMalloc((void **)&b->d_dl_bits, 4 * TC * 2 * NB) != hipSuccess) return GG_OOM;
         if (hipMalloc((void **)&b->d_dl_ovr, 8 * TC * 2 * (size_t)N) != hipSuccess) return GG_OOM;
-        if (hipMalloc((void **)&b->d_dl_cnt, sizeof(int) * TC * 2) != hipSuccess) return GG_OOM;
         if (hipMalloc((void **)&b->d_task_state, sizeof(int) * T) != hipSuccess) return GG_OOM;
         if (hipMalloc((void **)&b->d_task_end, sizeof(int) * T) != hipSuccess) return GG_OOM;
         if (hipMalloc((void **)&b->d_task_flags, sizeof(int) * T) != hipSuccess) return GG_OOM;
@@ -144,7 +143,6 @@ extern "C" void gg_batch_destroy(gg_batch *b) {
     drop_graph(b);
     void *bufs[] = {b->d_tab, b->d_adjR, b->d_src, b->d_parts, b->d_res, b->d_swaps,
                     b->d_ecc, b->d_base_reach, b->d_snap, b->d_dl_key, b->d_dl_bits, b->d_dl_ovr,
-                    b->d_dl_cnt,
                     b->d_task_state, b->d_task_end, b->d_task_flags, b->d_dcnt, b->d_queue,
                     b->d_qctl, b->d_dense_parts};
     for (void *p : bufs)
@@ -283,7 +281,6 @@ static int enqueue_sparse(gg_batch *b, int K) {
     s.cap = b->cap;
     s.skcap = b->skcap;
     s.dl_key = b->d_dl_key;
-    s.dl_cnt = b->d_dl_cnt;
     s.dl_bits = b->d_dl_bits;
     s.dl_ovr = b->d_dl_ovr;
     s.dcnt = b->d_dcnt;

@@ -27,7 +27,7 @@ struct gg_batch {
     int *d_dl_key = nullptr;
     unsigned *d_dl_bits = nullptr;            // [chunk*W][2][ceil(N/32)]
     unsigned long long *d_dl_ovr = nullptr;   // [chunk*W][2][N]
-    int *d_dl_cnt = nullptr, *d_task_state = nullptr, *d_task_end = nullptr,
+    int *d_task_state = nullptr, *d_task_end = nullptr,
         *d_task_flags = nullptr;
     long long *d_dcnt = nullptr;
     int2 *d_queue = nullptr;

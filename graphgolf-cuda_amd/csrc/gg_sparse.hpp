@@ -44,7 +44,6 @@ struct SparseArgs {
     int skcap;                       // Delta keys of the previous level staged in LDS (max)
     // per task t = k*W + w (k local to the launch), parity p = level & 1
     int *dl_key;                     // [T][2][cap] Delta vertices
-    int *dl_cnt;                     // [T][2]
     unsigned *dl_bits;               // [T][2][ceil(N/32)] Delta membership
     unsigned long long *dl_ovr;      // [T][2][N] Delta words (valid where the bit is set)
     long long *dcnt;                 // [k0+K][W][lmax+1] count deltas per level
@@ -110,6 +109,8 @@ template <int BS, int DT>
 __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a) {
     extern __shared__ unsigned long long Bprev[];
     __shared__ gg_swap s_sw[GG_SPARSE_MAX_K];     // the launch's swaps
+    __shared__ int s_state[GG_SPARSE_MAX_K];      // task state of this column
+    __shared__ int s_cnt[2][GG_SPARSE_MAX_K];     // |Delta| per parity
     __shared__ int s_koff[GG_SPARSE_MAX_K + 1];   // staged Delta keys: offset per candidate
     __shared__ int s_ooff[GG_SPARSE_MAX_K + 1];   // stale Delta_{l-2} entries: offset per candidate
     __shared__ int s_nold[GG_SPARSE_MAX_K];
@@ -124,7 +125,7 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a) {
     __shared__ int s_ended[GG_SPARSE_MAX_K];      // task ended last level: clear its bitmaps
     __shared__ int s_next;
     __shared__ int s_any;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63;
     const int w = blockIdx.x;
     const int N = a.N, D = a.D, W = a.W, K = a.K;
     const int NB = (N + 31) >> 5;
@@ -143,16 +144,15 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a) {
         bool dense = false;
         for (int j = 0; j < 4; j++)
             if (s.v[j] >= 0 && (s.v[j] >> 6) == w) dense = true;
-        a.dl_cnt[lt(k) * 2] = 0;
-        a.dl_cnt[lt(k) * 2 + 1] = 0;
+        s_cnt[0][k] = 0;
+        s_cnt[1][k] = 0;
         a.task_flags[gt(k)] = 0;
         a.task_end[gt(k)] = 0;
+        s_state[k] = dense ? GG_TASK_DENSE : GG_TASK_ACTIVE;
+        a.task_state[gt(k)] = s_state[k];
         if (dense) {
-            a.task_state[gt(k)] = GG_TASK_DENSE;
             int q = atomicAdd(a.queue_count, 1);
             a.queue[q] = make_int2(a.k0 + k, w);
-        } else {
-            a.task_state[gt(k)] = GG_TASK_ACTIVE;
         }
         s_ended[k] = 0;
     }
@@ -191,10 +191,10 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a) {
         for (int i = tid; i < NBE; i += BS) s_ubits[i] = 0u;
         // per-candidate bookkeeping
         for (int k = tid; k < K; k += BS) {
-            const bool act = a.task_state[gt(k)] == GG_TASK_ACTIVE;
+            const bool act = s_state[k] == GG_TASK_ACTIVE;
             s_active[k] = act;
-            s_nprev[k] = act ? a.dl_cnt[lt(k) * 2 + par] : 0;
-            s_nold[k] = act ? a.dl_cnt[lt(k) * 2 + cur] : 0;
+            s_nprev[k] = act ? s_cnt[par][k] : 0;
+            s_nold[k] = act ? s_cnt[cur][k] : 0;
             s_ncur[k] = 0;
             s_over[k] = 0;
             s_diff[k] = 0;
@@ -204,22 +204,25 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a) {
         clear_ended();
         __syncthreads();
         for (int k = tid; k < K; k += BS) s_ended[k] = 0;
-        if (tid == 0) {
-            int acc = 0, kacc = 0, oacc = 0;
-            for (int k = 0; k < K; k++) {
-                s_items[k] = acc;
-                s_koff[k] = kacc;
-                s_ooff[k] = oacc;
-                if (s_active[k]) {
-                    acc += 4 + s_nprev[k] * (D + 1);
-                    kacc += s_nprev[k];
-                    oacc += s_nold[k];
-                }
+        if (tid < 64) {  // prefix sums over the candidates (K <= 64), one wave
+            const bool act = tid < K && s_active[tid];
+            int it = act ? 4 + s_nprev[tid] * (D + 1) : 0;
+            int kk = act ? s_nprev[tid] : 0;
+            int oo = act ? s_nold[tid] : 0;
+            const int it0 = it, kk0 = kk, oo0 = oo;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int t1 = __shfl_up(it, o, 64), t2 = __shfl_up(kk, o, 64),
+                          t3 = __shfl_up(oo, o, 64);
+                if (lane >= o) { it += t1; kk += t2; oo += t3; }
             }
-            s_items[K] = acc;
-            s_koff[K] = kacc;
-            s_ooff[K] = oacc;
-            s_staged = kacc <= a.skcap;
+            if (tid < K) { s_items[tid] = it - it0; s_koff[tid] = kk - kk0; s_ooff[tid] = oo - oo0; }
+            if (tid == 63) {
+                s_items[K] = it;
+                s_koff[K] = kk;
+                s_ooff[K] = oo;
+                s_staged = kk <= a.skcap;
+            }
         }
         __syncthreads();
         // last k with off[k] <= i (empty candidates share their successor's start)
@@ -257,10 +260,10 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a) {
         // shared pool of items over all active candidates
         for (;;) {
             int base_it = 0;
-            if ((tid & 63) == 0) base_it = atomicAdd(&s_next, 64);
+            if (lane == 0) base_it = atomicAdd(&s_next, 64);
             base_it = __shfl(base_it, 0, 64);
             if (base_it >= total) break;
-            const int it = base_it + (tid & 63);
+            const int it = base_it + lane;
             if (it >= total) continue;
             const int k = owner(s_items, it);
             const int loc = it - s_items[k];
@@ -325,6 +328,7 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a) {
             if (!s_active[k]) continue;
             const size_t g = gt(k);
             if (s_over[k]) {
+                s_state[k] = GG_TASK_DENSE;
                 a.task_state[g] = GG_TASK_DENSE;
                 int q = atomicAdd(a.queue_count, 1);
                 a.queue[q] = make_int2(a.k0 + k, w);
@@ -333,10 +337,11 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a) {
             }
             const int n_cur = s_ncur[k];
             a.dcnt[g * (a.lmax + 1) + l] = (long long)s_dc[k];
-            a.dl_cnt[lt(k) * 2 + cur] = n_cur;
+            s_cnt[cur][k] = n_cur;
             const bool fixed = (l >= ecc_w && n_cur == 0) ||
                                (l > ecc_w && !s_diff[k] && n_cur == s_nprev[k]);
             if (fixed || l >= a.lmax) {
+                s_state[k] = GG_TASK_DONE;
                 a.task_state[g] = GG_TASK_DONE;
                 a.task_end[g] = l;
                 if (!fixed) a.task_flags[g] = GG_FLAG_LIMIT;
