@@ -103,10 +103,11 @@ static int batch_alloc(gg_batch *b) {
         if (hipMalloc((void **)&b->d_task_flags, sizeof(int) * T) != hipSuccess) return GG_OOM;
         if (hipMalloc((void **)&b->d_dcnt, sizeof(long long) * T * L) != hipSuccess) return GG_OOM;
         if (hipMalloc((void **)&b->d_queue, sizeof(int2) * T) != hipSuccess) return GG_OOM;
-        if (hipMalloc((void **)&b->d_qctl, sizeof(int) * 2) != hipSuccess) return GG_OOM;
+        b->nqctl = 2 + (K + chunk - 1) / chunk;  // queue count, queue next, endpoint counters
+        if (hipMalloc((void **)&b->d_qctl, sizeof(int) * b->nqctl) != hipSuccess) return GG_OOM;
         if (hipMalloc((void **)&b->d_dense_parts, sizeof(ColPart) * T) != hipSuccess)
             return GG_OOM;
-        GG_HIP(hipMemsetAsync(b->d_qctl, 0, sizeof(int) * 2, b->stream));
+        GG_HIP(hipMemsetAsync(b->d_qctl, 0, sizeof(int) * b->nqctl, b->stream));
         GG_HIP(hipMemsetAsync(b->d_dl_bits, 0, 4 * TC * 2 * NB, b->stream));
     }
     GG_HIP(hipStreamSynchronize(b->stream));
@@ -242,28 +243,17 @@ static int check_swaps(const gg_batch *b, const gg_swap *sw, int K) {
     return GG_OK;
 }
 
-template <int DT>
-static hipError_t launch_sparse_inst(const SparseArgs &s, size_t lds, hipStream_t st) {
-    static int configured = -1;
-    auto fn = &k_sparse<kSparseBlock, DT>;
-    if ((int)lds > configured) {
-        hipError_t e = hipFuncSetAttribute((const void *)fn,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return e;
-        configured = (int)lds;
-    }
-    hipLaunchKernelGGL(fn, dim3(s.W), dim3(kSparseBlock), lds, st, s);
-    return hipGetLastError();
-}
-
-static hipError_t launch_sparse(const SparseArgs &s, size_t lds, hipStream_t st) {
+// Sparse kernel for the degree (fixed-degree instances with a dense tail
+// when vpt > 0, the runtime-degree one without).
+static hipError_t launch_sparse(const SparseArgs &s, const TopoRingQ &topo, const DenseArgs &qa,
+                                size_t lds, int vpt, hipStream_t st) {
     switch (s.D) {
-        case 4: return launch_sparse_inst<4>(s, lds, st);
-        case 8: return launch_sparse_inst<8>(s, lds, st);
-        case 12: return launch_sparse_inst<12>(s, lds, st);
-        case 16: return launch_sparse_inst<16>(s, lds, st);
-        case 24: return launch_sparse_inst<24>(s, lds, st);
-        default: return launch_sparse_inst<0>(s, lds, st);
+        case 4: return launch_sparse_d4(s, topo, qa, lds, vpt, st);
+        case 8: return launch_sparse_d8(s, topo, qa, lds, vpt, st);
+        case 12: return launch_sparse_d12(s, topo, qa, lds, vpt, st);
+        case 16: return launch_sparse_d16(s, topo, qa, lds, vpt, st);
+        case 24: return launch_sparse_d24(s, topo, qa, lds, vpt, st);
+        default: return launch_sparse_deg<0>(s, topo, qa, lds, 0, st);
     }
 }
 
@@ -293,11 +283,22 @@ static int enqueue_sparse(gg_batch *b, int K) {
     s.dense_parts = b->d_dense_parts;
     s.res = b->d_res;
     const size_t lds = sparse_lds_bytes(N, b->skcap);
-    for (int k0 = 0; k0 < K; k0 += b->chunk) {
+    // dense tail of the sparse kernel (endpoint columns): fixed-degree ring
+    // table, its vertices-per-thread instance, a push list within the LDS
+    // the sparse layout leaves after B_{l-1}
+    int vpt = 0;
+    if (b->narrow && b->dc.ringq.R == b->dc.ringq.N && !b->dc.generic_opt) {
+        const int v = pick_vpt(1024, (N + 1023) / 1024);
+        if (v == 4 || v == 8 || v == 16) vpt = v;
+    }
+    DenseArgs qa = dense_args(b, K, b->d_swaps, b->d_dense_parts, nullptr, nullptr);
+    qa.push_cap = push_capacity(b->dc, N, b->degree, (int)lds);
+    for (int k0 = 0, c_i = 0; k0 < K; k0 += b->chunk, c_i++) {
         SparseArgs c = s;
         c.k0 = k0;
         c.K = K - k0 < b->chunk ? K - k0 : b->chunk;
-        hipError_t e = launch_sparse(c, lds, b->stream);
+        c.ep_next = b->d_qctl + 2 + c_i;
+        hipError_t e = launch_sparse(c, b->dc.ringq, qa, lds, vpt, b->stream);
         if (e != hipSuccess) {
             gg_set_last_hip_error(e, "k_sparse", __FILE__, __LINE__);
             return GG_HIP_ERROR;
@@ -317,7 +318,7 @@ static int enqueue_sparse(gg_batch *b, int K) {
     if (st != GG_OK) return st;
     hipLaunchKernelGGL(k_sparse_finalize<256>, dim3(K), dim3(256), 0, b->stream, s,
                        N - 64 * (W - 1));
-    hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, b->stream, b->d_qctl, b->d_qctl + 1);
+    hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, b->stream, b->d_qctl, b->nqctl);
     GG_HIP(hipGetLastError());
     return GG_OK;
 }

@@ -23,6 +23,7 @@ struct gg_batch {
     unsigned long long *d_snap = nullptr;     // [snap_cap][W][N]
     int cap = 0;                              // Delta list capacity per task
     int skcap = 0;                            // Delta keys staged in LDS per column and level
+    int nqctl = 0;                            // d_qctl words: queue count/next + endpoint counters
     int chunk = 0;                            // candidates per sparse launch
     int *d_dl_key = nullptr;
     unsigned *d_dl_bits = nullptr;            // [chunk*W][2][ceil(N/32)]

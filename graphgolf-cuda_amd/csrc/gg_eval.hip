@@ -161,7 +161,7 @@ static const int kStaticLdsReserve = 2048;
 
 // Level-1 frontier list capacity for the push levels (0 = pull only): the
 // list follows the column bitset in LDS and packs (vertex, bit) in 17+6 bits.
-static int push_capacity(const DenseCtx &c, int N, int D, int lds_cap) {
+int push_capacity(const DenseCtx &c, int N, int D, int lds_cap) {
     if (c.push_off || !c.push_ok || N > (1 << 17)) return 0;
     long long room = ((long long)lds_cap - (long long)N * 8) / 4;
     long long cap = 64LL * D;

@@ -37,6 +37,10 @@ int pick_vpt(int bs, int need);
 int device_lds_limit(int device);
 int device_cu_count(int device);
 
+// Frontier list capacity of the dense kernels' push levels (0 = pull only)
+// for a column bitset of N words within lds_cap bytes of LDS.
+int push_capacity(const DenseCtx &c, int N, int D, int lds_cap);
+
 // Undirected-graph checks that gate the push levels (host side).
 bool adj_symmetric(const int32_t *adj, int N, int D);
 bool swap_keeps_symmetry(const int32_t *adj, int D, const gg_swap &sw);

@@ -53,6 +53,7 @@ struct SparseArgs {
     int2 *queue;                     // dense fallback tasks (global k, w)
     int *queue_count;
     int *queue_next;
+    int *ep_next;                    // endpoint tasks claimed (kernels with a dense tail)
     ColPart *dense_parts;            // [k0+K][W] written by the dense queue kernel
     ResultRec *res;                  // [k0+K]
 };
@@ -105,8 +106,13 @@ __host__ __device__ inline size_t sparse_lds_bytes(int N, int skcap) {
     return (size_t)N * 8 + nb * 4 + (size_t)skcap * 4;
 }
 
-template <int BS, int DT>
-__global__ void __launch_bounds__(BS) k_sparse(SparseArgs a) {
+// VPT > 0: after its column, each workgroup runs dense BFS tasks of the
+// columns that hold a swap endpoint as a source (candidate k, endpoint j ->
+// task 4k+j, claimed from a.ep_next; the list follows from the swaps alone),
+// with the fixed-degree dense column of `qa` in the same LDS.  VPT = 0 queues
+// those columns for k_dense_lds_queue instead.
+template <int BS, int DT, int VPT>
+__global__ void __launch_bounds__(BS) k_sparse(SparseArgs a, TopoRingQ topo, DenseArgs qa) {
     extern __shared__ unsigned long long Bprev[];
     __shared__ gg_swap s_sw[GG_SPARSE_MAX_K];     // the launch's swaps
     __shared__ int s_state[GG_SPARSE_MAX_K];      // task state of this column
@@ -150,7 +156,7 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a) {
         a.task_end[gt(k)] = 0;
         s_state[k] = dense ? GG_TASK_DENSE : GG_TASK_ACTIVE;
         a.task_state[gt(k)] = s_state[k];
-        if (dense) {
+        if (dense && VPT == 0) {
             int q = atomicAdd(a.queue_count, 1);
             a.queue[q] = make_int2(a.k0 + k, w);
         }
@@ -355,6 +361,35 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a) {
         if (!s_any) break;
     }
     clear_ended();
+    if constexpr (VPT > 0) {
+        // dense tail: endpoint columns, each (candidate, column) once
+        __shared__ int s_task;
+        const int ntask = 4 * K;
+        for (;;) {
+            __syncthreads();  // the LDS is free again
+            if (tid == 0) {
+                int t;
+                for (;;) {
+                    t = atomicAdd(a.ep_next, 1);
+                    if (t >= ntask) break;
+                    const gg_swap &sk = s_sw[t >> 2];
+                    const int j = t & 3, v = sk.v[j];
+                    if (v < 0) continue;
+                    bool dup = false;
+                    for (int jj = 0; jj < j; jj++)
+                        if (sk.v[jj] >= 0 && (sk.v[jj] >> 6) == (v >> 6)) dup = true;
+                    if (!dup) break;
+                }
+                s_task = t;
+            }
+            __syncthreads();
+            const int t = s_task;
+            if (t >= ntask) break;
+            const int k = t >> 2, wv = s_sw[k].v[t & 3] >> 6;
+            const ColPart p = dense_column<TopoRingQ, BS, VPT, DT>(topo, qa, a.k0 + k, wv, Bprev);
+            if (tid == 0) qa.parts[(size_t)(a.k0 + k) * qa.W + wv] = p;
+        }
+    }
 }
 
 // One workgroup per candidate: merges sparse column results (base counts +
@@ -424,7 +459,49 @@ __global__ void __launch_bounds__(BS) k_sparse_finalize(SparseArgs a, int nvalid
     }
 }
 
-// Resets the dense queue counters (after the dense kernel and the finalize).
-__global__ void k_queue_reset(int *queue_count, int *queue_next) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) { *queue_count = 0; *queue_next = 0; }
+// Resets the dense queue counters and the endpoint-task counters (after the
+// dense kernel and the finalize).
+static __global__ void k_queue_reset(int *ctl, int n) {
+    if (blockIdx.x == 0)
+        for (int i = threadIdx.x; i < n; i += blockDim.x) ctl[i] = 0;
 }
+
+// Launch of the sparse kernel of degree DT (fixed-degree instances live in
+// gg_sparse_d*.hip); vpt picks the dense tail (0 = none).
+template <int DT>
+hipError_t launch_sparse_deg(const SparseArgs &s, const TopoRingQ &topo, const DenseArgs &qa,
+                             size_t lds, int vpt, hipStream_t st) {
+    auto go = [&](auto fn, int &configured) -> hipError_t {
+        if ((int)lds > configured) {
+            hipError_t e = hipFuncSetAttribute((const void *)fn,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            if (e != hipSuccess) return e;
+            configured = (int)lds;
+        }
+        hipLaunchKernelGGL(fn, dim3(s.W), dim3(1024), lds, st, s, topo, qa);
+        return hipGetLastError();
+    };
+    static int c0 = -1, c4 = -1, c8 = -1, c16 = -1;
+    if constexpr (DT > 0) {
+        switch (vpt) {
+            case 4: return go(&k_sparse<1024, DT, 4>, c4);
+            case 8: return go(&k_sparse<1024, DT, 8>, c8);
+            case 16: return go(&k_sparse<1024, DT, 16>, c16);
+            default: break;
+        }
+    }
+    return go(&k_sparse<1024, DT, 0>, c0);
+}
+
+namespace gg {
+hipError_t launch_sparse_d4(const SparseArgs &, const TopoRingQ &, const DenseArgs &, size_t, int,
+                            hipStream_t);
+hipError_t launch_sparse_d8(const SparseArgs &, const TopoRingQ &, const DenseArgs &, size_t, int,
+                            hipStream_t);
+hipError_t launch_sparse_d12(const SparseArgs &, const TopoRingQ &, const DenseArgs &, size_t, int,
+                             hipStream_t);
+hipError_t launch_sparse_d16(const SparseArgs &, const TopoRingQ &, const DenseArgs &, size_t, int,
+                             hipStream_t);
+hipError_t launch_sparse_d24(const SparseArgs &, const TopoRingQ &, const DenseArgs &, size_t, int,
+                             hipStream_t);
+}  // namespace gg
