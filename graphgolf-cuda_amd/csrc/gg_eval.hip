@@ -227,6 +227,13 @@ int run_dense(DenseCtx &c, const DenseArgs &a_in, hipStream_t st) {
         gg_set_last_hip_error(e, "dense BFS launch", __FILE__, __LINE__);
         return GG_HIP_ERROR;
     }
+    if (c.bfs_done) {
+        e = hipEventRecord(c.bfs_done, st);
+        if (e != hipSuccess) {
+            gg_set_last_hip_error(e, "event record", __FILE__, __LINE__);
+            return GG_HIP_ERROR;
+        }
+    }
     hipLaunchKernelGGL(k_dense_finalize<1024>, dim3(a.K), dim3(1024), 0, st, a);
     e = hipGetLastError();
     if (e != hipSuccess) {
@@ -352,6 +359,7 @@ static int eval_alloc(gg_eval *ev, size_t tab_elem_bytes) {
     GG_HIP(hipStreamCreateWithFlags(&ev->stream, hipStreamNonBlocking));
     GG_HIP(hipEventCreate(&ev->ev0));
     GG_HIP(hipEventCreate(&ev->ev1));
+    ev->dc.bfs_done = ev->ev1;  // the timed span ends with the BFS kernel
     int R = ev->kind == EVAL_ADJ ? ev->N : ev->M;
     ev->tab_bytes = ev->dc.topo_kind == TOPO_RINGQ ? (size_t)quads(ev->degree) * R * 8
                                                    : (size_t)R * ev->degree * tab_elem_bytes;
@@ -616,9 +624,8 @@ static int eval_common(gg_eval *ev, gg_result *out, bool upload = true,
     a.reach_out = ev->d_reach;
     a.want_wvc = ev->kind == EVAL_PART ? 1 : 0;
     GG_HIP(hipEventRecord(ev->ev0, ev->stream));
-    int st = run_dense(ev->dc, a, ev->stream);
+    int st = run_dense(ev->dc, a, ev->stream);  // records ev1 after the BFS kernel
     if (st != GG_OK) return st;
-    GG_HIP(hipEventRecord(ev->ev1, ev->stream));
     GG_HIP(hipMemcpyAsync(ev->h_res, ev->d_res, sizeof(ResultRec), hipMemcpyDeviceToHost,
                           ev->stream));
     GG_HIP(hipStreamSynchronize(ev->stream));
