@@ -132,25 +132,46 @@ __global__ void __launch_bounds__(256) k_features(FeatArgs f) {
 
 __device__ __forceinline__ float bf(const __hip_bfloat16 v) { return __bfloat162float(v); }
 
-// Forward, one thread per sample; bf16 weights staged in LDS, fp32 accumulation.
+// Forward of P samples: workgroup g takes samples [16g, 16g+16), threads h and
+// h+128 compute hidden unit h for half of them each, one wave per four
+// samples sums the output layer.  bf16 weights staged in LDS, fp32
+// accumulation.
+#define GG_FWD_SAMPLES 16
 __global__ void __launch_bounds__(256) k_mlp_forward(const float *X, int P,
                                                      const __hip_bfloat16 *wq, float *out) {
     __shared__ float sw[GG_NPARAMS];
-    for (int i = threadIdx.x; i < GG_NPARAMS; i += 256) sw[i] = bf(wq[i]);
-    __syncthreads();
-    const int p = blockIdx.x * 256 + threadIdx.x;
-    if (p >= P) return;
-    float x[GG_FEATS];
-#pragma unroll
-    for (int i = 0; i < GG_FEATS; i++) x[i] = X[(size_t)p * GG_FEATS + i];
-    float o = sw[OFF_B2];
-    for (int h = 0; h < GG_HIDDEN; h++) {
-        float acc = sw[OFF_B1 + h];
-#pragma unroll
-        for (int i = 0; i < GG_FEATS; i++) acc = fmaf(sw[OFF_W1 + h * GG_FEATS + i], x[i], acc);
-        o = fmaf(sw[OFF_W2 + h], acc > 0.f ? acc : 0.f, o);
+    __shared__ float sx[GG_FWD_SAMPLES][GG_FEATS];
+    __shared__ float sr[GG_FWD_SAMPLES][GG_HIDDEN];  // w2[h] * relu(hidden h)
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int s0 = blockIdx.x * GG_FWD_SAMPLES;
+    const int ns = min(GG_FWD_SAMPLES, P - s0);
+    for (int i = tid; i < GG_NPARAMS; i += 256) sw[i] = bf(wq[i]);
+    for (int i = tid; i < GG_FWD_SAMPLES * GG_FEATS; i += 256) {
+        const int s = i / GG_FEATS, f = i - s * GG_FEATS;
+        sx[s][f] = s < ns ? X[(size_t)(s0 + s) * GG_FEATS + f] : 0.f;
     }
-    out[p] = o;
+    __syncthreads();
+    const int h = tid & (GG_HIDDEN - 1), half = tid >> 7;
+    {
+        float w1[GG_FEATS];
+#pragma unroll
+        for (int i = 0; i < GG_FEATS; i++) w1[i] = sw[OFF_W1 + h * GG_FEATS + i];
+        const float b1 = sw[OFF_B1 + h], w2 = sw[OFF_W2 + h];
+        for (int q = 0; q < GG_FWD_SAMPLES / 2; q++) {
+            const int s = half * (GG_FWD_SAMPLES / 2) + q;
+            float acc = b1;
+#pragma unroll
+            for (int i = 0; i < GG_FEATS; i++) acc = fmaf(w1[i], sx[s][i], acc);
+            sr[s][h] = w2 * (acc > 0.f ? acc : 0.f);
+        }
+    }
+    __syncthreads();
+    for (int s = wv; s < GG_FWD_SAMPLES; s += 4) {
+        float o = 0.f;
+        for (int k = lane; k < GG_HIDDEN; k += 64) o += sr[s][k];
+        for (int d = 32; d > 0; d >>= 1) o += __shfl_xor(o, d, 64);
+        if (lane == 0 && s < ns) out[s0 + s] = o + sw[OFF_B2];
+    }
 }
 
 // Gradient of the mean squared error over a minibatch of B samples (rows
@@ -377,8 +398,8 @@ extern "C" int gg_mlp_get_params(gg_mlp *m, float *params) {
 }
 
 int gg::mlp_forward_dev(gg_mlp *m, const float *d_x, int P, float *d_out, hipStream_t st) {
-    hipLaunchKernelGGL(k_mlp_forward, dim3((P + 255) / 256), dim3(256), 0, st, d_x, P, m->d_wq,
-                       d_out);
+    hipLaunchKernelGGL(k_mlp_forward, dim3((P + GG_FWD_SAMPLES - 1) / GG_FWD_SAMPLES), dim3(256),
+                       0, st, d_x, P, m->d_wq, d_out);
     GG_HIP(hipGetLastError());
     return GG_OK;
 }

@@ -41,7 +41,7 @@ hipStream_t batch_stream(gg_batch *b) { return b->stream; }
 struct LearnState {
     gg_batch *batch = nullptr;
     gg_mlp *mlp = nullptr;
-    hipStream_t st = nullptr;
+    hipStream_t st = nullptr;                    // the batch evaluator's stream (all work)
     int P = 0, K = 0;
     int *d_moves = nullptr, *h_moves = nullptr;  // [P][5]
     float *d_x = nullptr;                        // [P][F]
@@ -60,7 +60,7 @@ struct LearnState {
     std::unique_ptr<Rng> rng;                    // minibatch sampling / exploration
     std::vector<int> last_sel;                   // original indices of the chosen K
     std::vector<double> raw;                     // raw deltas of the replay buffer (host mirror)
-    std::vector<float> ynorm;                    // normalised targets (host mirror)
+    float *h_ynorm = nullptr;                    // normalised targets (pinned host mirror)
 };
 
 static int alloc_learn(LearnState *L) {
@@ -79,6 +79,8 @@ static int alloc_learn(LearnState *L) {
     if (hipMalloc((void **)&L->d_by, 4 * (size_t)L->cap) != hipSuccess) return GG_OOM;
     if (hipMalloc((void **)&L->d_idx, 4 * GG_MAX_BATCH) != hipSuccess) return GG_OOM;
     if (hipHostMalloc((void **)&L->h_idx, 4 * GG_MAX_BATCH, hipHostMallocDefault) != hipSuccess)
+        return GG_OOM;
+    if (hipHostMalloc((void **)&L->h_ynorm, 4 * (size_t)L->cap, hipHostMallocDefault) != hipSuccess)
         return GG_OOM;
     return GG_OK;
 }
@@ -102,11 +104,12 @@ int learn_create(gg_search *s, int device) {
 void learn_destroy(gg_search *s) {
     LearnState *L = s->learn;
     if (!L) return;
+    if (L->st) (void)hipStreamSynchronize(L->st);
     gg_mlp_destroy(L->mlp);
     void *dev[] = {L->d_moves, L->d_x, L->d_score, L->d_sel, L->d_bx, L->d_by, L->d_idx};
     for (void *p : dev)
         if (p) (void)hipFree(p);
-    void *host[] = {L->h_moves, L->h_score, L->h_sel, L->h_idx};
+    void *host[] = {L->h_moves, L->h_score, L->h_sel, L->h_idx, L->h_ynorm};
     for (void *p : host)
         if (p) (void)hipHostFree(p);
     delete L;
@@ -176,6 +179,7 @@ static int train_steps(gg_search *s, int steps) {
     const int B = std::min(std::max(s->cfg.train_batch, 1), GG_MAX_BATCH);
     if (L->count < B) return GG_OK;
     for (int t = 0; t < steps; t++) {
+        if (t > 0) GG_HIP(hipStreamSynchronize(L->st));  // h_idx is reused
         for (int i = 0; i < B; i++) L->h_idx[i] = (int)L->rng->below(L->count);
         GG_HIP(hipMemcpyAsync(L->d_idx, L->h_idx, 4 * B, hipMemcpyHostToDevice, L->st));
         int st = mlp_grad_dev(L->mlp, L->d_bx, L->d_by, L->d_idx, B, L->st);
@@ -193,19 +197,19 @@ static int upload_targets(LearnState *L, int pos, int n) {
     int first = std::min(n, L->cap - pos);
     for (int i = 0; i < n; i++) {
         double y = (L->raw[(pos + i) % L->cap] - L->mu) / L->sigma;
-        L->ynorm[(pos + i) % L->cap] = (float)std::max(-8.0, std::min(8.0, y));
+        L->h_ynorm[(pos + i) % L->cap] = (float)std::max(-8.0, std::min(8.0, y));
     }
-    GG_HIP(hipMemcpyAsync(L->d_by + pos, L->ynorm.data() + pos, 4 * (size_t)first,
+    GG_HIP(hipMemcpyAsync(L->d_by + pos, L->h_ynorm + pos, 4 * (size_t)first,
                           hipMemcpyHostToDevice, L->st));
     if (n > first)
-        GG_HIP(hipMemcpyAsync(L->d_by, L->ynorm.data(), 4 * (size_t)(n - first),
+        GG_HIP(hipMemcpyAsync(L->d_by, L->h_ynorm, 4 * (size_t)(n - first),
                               hipMemcpyHostToDevice, L->st));
     return GG_OK;
 }
 
 int learn_observe(gg_search *s, int K) {
     LearnState *L = s->learn;
-    if (L->raw.empty()) { L->raw.assign(L->cap, 0.0); L->ynorm.assign(L->cap, 0.f); }
+    if (L->raw.empty()) L->raw.assign(L->cap, 0.0);
     const int pos = L->head;
     for (int k = 0; k < K; k++) {
         double dlt = raw_delta(s, s->res[k]);
