@@ -1,0 +1,269 @@
+// gg-anneal -- simulated annealing / hill climbing driver on the MI355X
+// evaluator with the command line, input/output text format and log lines
+// of the reference's drivers (cppcode/annealing.cpp:14-105 options and
+// set-up, :293-330 initial evaluation and log header, :396-452 loop, log and
+// result; cppcode/HillClimb.cpp for --hill).
+//
+//   gg-anneal [-c COUNT] [-s SEED] [-i IN] [-o OUT] [-l LOG] [-v VERBOSE] [-D DEVICE]
+//             [-t TEMP] [--hill] [--mode plain|batched|learned] [-K K] [-P P]
+//             [--schedule step|geometric] [--temp-end T]
+//
+// Input and output are `part` tables (part.hpp:13-43).  M = N: an explicit
+// graph, 2-opt swaps (gg_search); M < N: a cyclically symmetric graph, orbit
+// moves (gg_orbit_search).  TEMP is the reference's initial temperature in
+// total-distance units (default 46.1793, annealing.cpp:314); the default
+// schedule is the reference's step schedule (annealing.cpp:442).  The moves
+// are this library's (gg_search.cpp / gg_orbit.cpp), so a run does not
+// reproduce the reference's random trajectory; the log and console lines do
+// have the reference's layout.  The printed solution is the library's best by
+// (diameter, ASPL); the reference keeps the accepted solution of least ASPL
+// (annealing.cpp:429-436), which differs only when a later solution of
+// smaller diameter has a larger ASPL.  A best-solution dump in the verbose
+// file is written once per 1000 iterations (for the last improvement among
+// them) rather than at every improvement.
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <iostream>
+#include <random>
+#include <string>
+#include <vector>
+#include "gg_formats.hpp"
+#include "graphgolf_mi355x.h"
+#include "graphgolf_search.h"
+
+namespace {
+
+int usage(int code) {
+    std::cout << "gg-anneal [-c COUNT] [-s SEED] [-i IN] [-o OUT] [-l LOG] [-v VERBOSE] [-D DEVICE]\n"
+                 "          [-t TEMP] [--hill] [--mode plain|batched|learned] [-K K] [-P P]\n"
+                 "          [--schedule step|geometric] [--temp-end T]\n";
+    return code;
+}
+
+const char *kClear = "\x1b[F\x1b[E\x1b[K";  // the reference's line rewrite (annealing.cpp:418)
+
+}  // namespace
+
+int main(int argc, char **argv) {
+    auto t_init = std::chrono::steady_clock::now();
+    int count = 1000, device = 0, K = 32, P = 1024;
+    bool have_seed = false, hill = false;
+    unsigned int seed = 0;
+    double inittemp = 46.1793, temp_end = -1;
+    std::string in_path, out_path, log_path, verbose_path, mode = "plain", schedule = "step";
+    for (int i = 1; i < argc; i++) {
+        std::string a = argv[i];
+        auto next = [&]() -> std::string {
+            if (i + 1 >= argc) { std::cerr << "missing value for " << a << "\n"; std::exit(1); }
+            return argv[++i];
+        };
+        if (a == "-h" || a == "--help") return usage(0);
+        else if (a == "-c" || a == "--count") count = std::stoi(next());
+        else if (a == "-s" || a == "--seed") { seed = (unsigned)std::stoul(next()); have_seed = true; }
+        else if (a == "-i" || a == "--input") in_path = next();
+        else if (a == "-o" || a == "--output") out_path = next();
+        else if (a == "-l" || a == "--log") log_path = next();
+        else if (a == "-v" || a == "--verbose") verbose_path = next();
+        else if (a == "-D" || a == "--device") device = std::stoi(next());
+        else if (a == "-t" || a == "--temp") inittemp = std::stod(next());
+        else if (a == "--hill") hill = true;
+        else if (a == "--mode") mode = next();
+        else if (a == "-K") K = std::stoi(next());
+        else if (a == "-P") P = std::stoi(next());
+        else if (a == "--schedule") schedule = next();
+        else if (a == "--temp-end") temp_end = std::stod(next());
+        else { std::cout << "unknown option " << a << "\n"; return usage(1); }
+    }
+    std::ifstream inputfs;
+    std::ofstream outputfs, logfs, verbosefs;
+    if (!in_path.empty()) {
+        inputfs.open(in_path, std::ios::binary | std::ios::in);
+        if (!inputfs.is_open()) { std::cerr << "error: " << in_path << " is not open\n"; return 1; }
+    }
+    if (!out_path.empty()) {
+        outputfs.open(out_path, std::ios::binary | std::ios::out);
+        if (!outputfs.is_open()) { std::cerr << "error: " << out_path << " is not open\n"; return 1; }
+    }
+    const bool logging = !log_path.empty(), verbose = !verbose_path.empty();
+    if (logging) {
+        logfs.open(log_path, std::ios::binary | std::ios::out);
+        if (!logfs.is_open()) { std::cerr << "error: " << log_path << " is not open\n"; return 1; }
+        logfs.precision(10);
+    }
+    if (verbose) {
+        verbosefs.open(verbose_path, std::ios::binary | std::ios::out);
+        if (!verbosefs.is_open()) {
+            std::cerr << "error: " << verbose_path << " is not open\n";
+            return 1;
+        }
+        verbosefs.precision(10);
+    }
+    std::istream &ist = inputfs.is_open() ? inputfs : std::cin;
+    std::ostream &ost = outputfs.is_open() ? outputfs : std::cout;
+    for (std::ostream *f : {static_cast<std::ostream *>(&verbosefs), static_cast<std::ostream *>(&logfs)}) {
+        if (f == &verbosefs && !verbose) continue;
+        if (f == &logfs && !logging) continue;
+        if (!in_path.empty()) *f << "#input file: " << in_path << std::endl;
+        if (!out_path.empty()) *f << "#output file: " << out_path << std::endl;
+    }
+    graphgolf::part init;
+    init.load(ist);
+    if (!init.regular) {
+        std::cout << "non-regular graph" << std::endl;
+        return 1;
+    }
+    if (verbose) verbosefs << "#N: " << init.N << " M: " << init.M << std::endl;
+    if (!have_seed) seed = std::random_device{}();
+    if (verbose) verbosefs << "#seed: " << seed << std::endl;
+    const int N = init.N, M = init.M, d = init.degree;
+    const double NN1 = (double)N * (N - 1);
+    std::vector<int32_t> tab((size_t)M * d);
+    for (int r = 0; r < M; r++)
+        for (int j = 0; j < d; j++) tab[(size_t)r * d + j] = init.edges[r][j];
+
+    gg_search_cfg cfg{};
+    cfg.mode = mode == "learned" ? GG_MODE_LEARNED : mode == "batched" ? GG_MODE_BATCHED : GG_MODE_PLAIN;
+    cfg.rule = hill ? GG_RULE_HILL : GG_RULE_ANNEAL;
+    cfg.K = K;
+    cfg.P = P;
+    cfg.total_steps = count > 0 ? count : 1;
+    cfg.warmup_pairs = 2048;
+    cfg.train_every = 1;
+    cfg.train_batch = 256;
+    cfg.T0 = inittemp / NN1;
+    cfg.T1 = temp_end > 0 ? temp_end / NN1 : cfg.T0 * 1e-3;
+    cfg.lr = 1e-3;
+    cfg.seed = seed;
+    cfg.schedule = schedule == "geometric" ? GG_SCHED_GEOMETRIC : GG_SCHED_STEP;
+
+    // evaluator of the initial solution (annealing.cpp:296-312)
+    gg_eval *ev = nullptr;
+    int st = gg_create_part(N, M, d, device, &ev);
+    if (st != GG_OK) { std::cerr << "error: " << gg_strerror(st) << "\n"; return 1; }
+    gg_result r0;
+    st = gg_eval_part(ev, tab.data(), &r0);
+    gg_destroy(ev);
+    if (st != GG_OK) { std::cerr << "error: " << gg_strerror(st) << "\n"; return 1; }
+    if (r0.status != GG_OK) {
+        if (logging) logfs << "#initial solution is unconnected" << std::endl;
+        std::cout << "initial solution is unconnected" << std::endl;
+        return 0;
+    }
+    std::cout.precision(10);
+    const double init_aspl = gg_aspl(&r0, N);
+    std::cout << "ASPL(init_x): " << init_aspl << std::endl;
+    if (verbose) verbosefs << "#ASPL(init_x): " << init_aspl << std::endl;
+    if (logging) {
+        logfs << "#iteration ASPL(x_best) ASPL(x) ASPL(y) temp" << std::endl;
+        logfs << 0 << ' ' << init_aspl << ' ' << init_aspl << ' ' << init_aspl << ' ' << inittemp
+              << std::endl;
+    }
+
+    // the search: explicit graph (2-opt) or symmetric table (orbit moves)
+    gg_search *s = nullptr;
+    gg_orbit_search *os = nullptr;
+    std::vector<int32_t> adj;
+    if (M == N) {
+        adj.resize((size_t)N * d);
+        for (int v = 0; v < N; v++)
+            for (int j = 0; j < d; j++) {
+                long long u = ((long long)v + tab[(size_t)v * d + j]) % N;
+                adj[(size_t)v * d + j] = (int32_t)(u < 0 ? u + N : u);
+            }
+        st = gg_search_create(N, d, adj.data(), &cfg, device, &s);
+    } else {
+        cfg.mode = GG_MODE_PLAIN;
+        st = gg_orbit_search_create(N, M, d, tab.data(), &cfg, device, &os);
+    }
+    if (st != GG_OK) { std::cerr << "error: " << gg_strerror(st) << "\n"; return 1; }
+
+    double fx = init_aspl, fx_best = init_aspl;
+    int dx = r0.diameter, dx_best = r0.diameter;
+    const int chunk = 1000;
+    std::vector<int64_t> log((size_t)chunk * 10);
+    for (int done = 0; done < count;) {
+        const int n = count - done < chunk ? count - done : chunk;
+        auto t0 = std::chrono::steady_clock::now();
+        gg_search_stats stats;
+        int got = 0;
+        if (s) {
+            st = gg_search_set_log(s, n);
+            if (st == GG_OK) st = gg_search_run(s, n, &stats);
+            if (st == GG_OK) st = gg_search_get_log(s, log.data(), n, &got);
+        } else {
+            st = gg_orbit_search_set_log(os, n);
+            if (st == GG_OK) st = gg_orbit_search_run(os, n, &stats);
+            if (st == GG_OK) st = gg_orbit_search_get_log(os, log.data(), n, &got);
+        }
+        if (st != GG_OK) { std::cerr << "error: " << gg_strerror(st) << "\n"; return 1; }
+        auto t1 = std::chrono::steady_clock::now();
+        const double ms = std::chrono::duration<double, std::milli>(t1 - t0).count() / (n ? n : 1);
+        bool improved = false;
+        for (int e = 0; e < got; e++) {
+            const int64_t *rec = &log[(size_t)e * 10];
+            const int64_t i = rec[0];
+            const bool ok = rec[8] == GG_OK, accept = rec[9] != 0;
+            const double fy = ok ? (double)rec[6] / NN1 : 1e8;
+            const int dy = ok ? (int)rec[7] : 100000000;
+            const double temp = (cfg.schedule == GG_SCHED_STEP
+                                     ? cfg.T0 * std::pow(0.995, (double)((i - 1) / 10000))
+                                     : cfg.T0 * std::pow(cfg.T1 / cfg.T0, (double)i / cfg.total_steps)) *
+                                NN1;
+            if (accept) {
+                if (fx != fy) {
+                    std::cout << kClear << "iteration: " << i << " fx_best: " << fx_best
+                              << " fx: " << fx << " time: " << ms << "ms" << std::flush;
+                    if (verbose)
+                        verbosefs << "iteration: " << i << " fx_best: " << fx_best << " fx: " << fx
+                                  << " temp: " << temp << std::endl;
+                    if (logging) logfs << i << ' ' << fx_best << ' ' << fx << ' ' << temp << std::endl;
+                }
+                const bool best = fy < fx_best && dy <= dx_best;
+                fx = fy;
+                dx = dy;
+                if (best) {
+                    std::cout << std::endl;
+                    fx_best = fy;
+                    dx_best = dy;
+                    improved = true;
+                }
+            }
+            if (i % 1000 == 0)
+                std::cout << kClear << "iteration: " << i << " fx_best: " << fx_best << " fx: " << fx
+                          << " time: " << ms << "ms" << std::flush;
+        }
+        done += n;
+        if (improved && verbose) {
+            graphgolf::part cur = init;
+            std::vector<int32_t> out(M == N ? (size_t)N * d : (size_t)M * d);
+            if (s) gg_search_graph(s, out.data(), 1);
+            else gg_orbit_search_table(os, out.data(), 1);
+            for (int r = 0; r < M; r++)
+                for (int j = 0; j < d; j++)
+                    cur.edges[r][j] = M == N ? out[(size_t)r * d + j] - r : out[(size_t)r * d + j];
+            verbosefs << "#update. new solution:" << std::endl;
+            cur.print(verbosefs);
+        }
+    }
+    const double elapsed =
+        std::chrono::duration_cast<std::chrono::seconds>(std::chrono::steady_clock::now() - t_init)
+            .count();
+    std::cout << std::endl << "total elapsed time: " << elapsed << "s" << std::endl;
+    if (verbose) verbosefs << "#total elapsed time: " << elapsed << "s" << std::endl;
+    // best solution in the input's format (offsets v -> u as u - v for M = N)
+    graphgolf::part best = init;
+    std::vector<int32_t> out(M == N ? (size_t)N * d : (size_t)M * d);
+    if (s) gg_search_graph(s, out.data(), 1);
+    else gg_orbit_search_table(os, out.data(), 1);
+    for (int r = 0; r < M; r++)
+        for (int j = 0; j < d; j++)
+            best.edges[r][j] = M == N ? out[(size_t)r * d + j] - r : out[(size_t)r * d + j];
+    best.print(ost);
+    if (s) gg_search_destroy(s);
+    if (os) gg_orbit_search_destroy(os);
+    (void)dx;
+    return 0;
+}

@@ -28,6 +28,7 @@
 #include "graphgolf_mi355x.h"
 #include "graphgolf_search.h"
 #include "gg_rng.hpp"
+#include "gg_search_impl.hpp"
 
 struct gg_orbit_search {
     int N = 0, M = 0, d = 0, g = 0;
@@ -46,9 +47,7 @@ struct gg_orbit_search {
 
 namespace {
 
-struct OrbitMove {
-    int kind, a, ia, c, ic, k1, k2;
-};
+using gg::OrbitMove;
 
 inline int64_t mod_n(int64_t x, int64_t N) {
     x %= N;
@@ -78,10 +77,12 @@ bool row_simple(const std::vector<int32_t> &t, int N, int d, int r) {
     return true;
 }
 
+}  // namespace
+
 // One proposal attempt on `cur`, written to `out` (a copy of cur).  Returns
 // false when the move is degenerate or its table is not simple.
-bool propose_orbit(const std::vector<int32_t> &cur, int N, int M, int d, gg::Rng &rng,
-                   std::vector<int32_t> &out, OrbitMove *mv) {
+bool gg::propose_orbit(const std::vector<int32_t> &cur, int N, int M, int d, gg::Rng &rng,
+                       std::vector<int32_t> &out, OrbitMove *mv) {
     const int g = N / M;
     const int coin = (int)rng.below(2);
     out = cur;
@@ -123,6 +124,8 @@ bool propose_orbit(const std::vector<int32_t> &cur, int N, int M, int d, gg::Rng
            row_simple(out, N, d, c);
 }
 
+namespace {
+
 bool better(const gg_result &x, const gg_result &y) {
     bool xo = x.status == GG_OK, yo = y.status == GG_OK;
     if (xo != yo) return xo;
@@ -140,6 +143,7 @@ int validate(int N, int M, int d, const int32_t *table, const gg_search_cfg *cfg
     if (cfg->mode != GG_MODE_PLAIN) return GG_UNSUPPORTED;
     if (cfg->rule != GG_RULE_ANNEAL && cfg->rule != GG_RULE_HILL) return GG_BAD_ARG;
     if (cfg->rule == GG_RULE_ANNEAL && (!(cfg->T0 > 0) || !(cfg->T1 > 0))) return GG_BAD_ARG;
+    if (cfg->schedule != GG_SCHED_GEOMETRIC && cfg->schedule != GG_SCHED_STEP) return GG_BAD_ARG;
     if (cfg->total_steps < 1) return GG_BAD_ARG;
     return GG_OK;
 }
@@ -161,7 +165,7 @@ int orbit_step(gg_orbit_search *s) {
     gg::Rng &rng = *s->rng;
     OrbitMove mv{};
     int tries = 0;
-    while (!propose_orbit(s->tab, s->N, s->M, s->d, rng, s->cand, &mv))
+    while (!gg::propose_orbit(s->tab, s->N, s->M, s->d, rng, s->cand, &mv))
         if (++tries > 1000000) return GG_BAD_ARG;  // no simple move exists
     s->stats.proposals++;
     gg_result y;
@@ -179,7 +183,7 @@ int orbit_step(gg_orbit_search *s) {
     if (cfg.rule == GG_RULE_HILL) {
         accept = ok && (dy < dx || (dy == dx && fy < fx));
     } else {
-        double T = cfg.T0 * std::pow(cfg.T1 / cfg.T0, (double)s->step / cfg.total_steps);
+        const double T = gg::temperature(cfg, s->step);
         if (fy < fx || dy < dx) accept = true;
         else if (rng.uniform() < std::exp((fx - fy) * NN1 / (T * NN1))) accept = true;
         if (dy > dx || !ok) accept = false;

@@ -201,6 +201,7 @@ static int validate_cfg(int N, int d, const int32_t *adj, const gg_search_cfg *c
     if (cfg->mode != GG_MODE_PLAIN && cfg->K < 1) return GG_BAD_ARG;
     if (cfg->mode == GG_MODE_LEARNED && cfg->P < cfg->K) return GG_BAD_ARG;
     if (cfg->rule == GG_RULE_ANNEAL && (!(cfg->T0 > 0) || !(cfg->T1 > 0))) return GG_BAD_ARG;
+    if (cfg->schedule != GG_SCHED_GEOMETRIC && cfg->schedule != GG_SCHED_STEP) return GG_BAD_ARG;
     if (cfg->total_steps < 1) return GG_BAD_ARG;
     return GG_OK;
 }
@@ -287,7 +288,7 @@ static int search_step(gg_search *s) {
     if (cfg.rule == GG_RULE_HILL) {
         accept = ok && (dy < dx || (dy == dx && fy < fx));
     } else {
-        double T = cfg.T0 * std::pow(cfg.T1 / cfg.T0, (double)s->step / cfg.total_steps);
+        const double T = temperature(cfg, s->step);
         if (fy < fx || dy < dx) accept = true;
         else if (rng.uniform() < std::exp((fx - fy) * NN1 / (T * NN1))) accept = true;
         if (dy > dx || !ok) accept = false;

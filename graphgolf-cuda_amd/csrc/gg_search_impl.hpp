@@ -1,5 +1,6 @@
 // gg_search_impl.hpp -- internal state of a search (host side).
 #pragma once
+#include <cmath>
 #include <cstdint>
 #include <memory>
 #include <vector>
@@ -30,6 +31,21 @@ struct LearnState;  // gg_learn.cpp
 
 bool propose_swap(const std::vector<int32_t> &adj, int N, int d, Rng &rng, gg_swap *sw,
                   MoveRec *mv);
+
+// Temperature of step i (1-based) on the ASPL scale.
+inline double temperature(const gg_search_cfg &cfg, int64_t i) {
+    if (cfg.schedule == GG_SCHED_STEP)
+        return cfg.T0 * std::pow(0.995, (double)((i - 1) / 10000));
+    return cfg.T0 * std::pow(cfg.T1 / cfg.T0, (double)i / cfg.total_steps);
+}
+
+// Symmetric `part` tables (gg_orbit.cpp): one move attempt (edge-length change
+// or orbit 2-opt) written to `out`; false when it must be redrawn.
+struct OrbitMove {
+    int kind, a, ia, c, ic, k1, k2;
+};
+bool propose_orbit(const std::vector<int32_t> &cur, int N, int M, int d, Rng &rng,
+                   std::vector<int32_t> &out, OrbitMove *mv);
 void apply_swap(std::vector<int32_t> &adj, int d, const gg_swap &sw);
 
 }  // namespace gg

@@ -80,7 +80,7 @@ class SearchCfg(C.Structure):
         ("mode", C.c_int32), ("rule", C.c_int32), ("K", C.c_int32), ("P", C.c_int32),
         ("total_steps", C.c_int32), ("warmup_pairs", C.c_int32), ("train_every", C.c_int32),
         ("train_batch", C.c_int32), ("T0", C.c_double), ("T1", C.c_double), ("lr", C.c_double),
-        ("seed", C.c_uint64),
+        ("seed", C.c_uint64), ("schedule", C.c_int32),
     ]
 
 
@@ -110,6 +110,11 @@ class EvalCallbacks(C.Structure):
 TABLE_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_int32), C.POINTER(Result))
 
 
+class InitTempResult(C.Structure):
+    _fields_ = [("count", C.c_int64), ("mean_increase_total", C.c_double),
+                ("temp_total", C.c_double), ("temp_aspl", C.c_double)]
+
+
 class TableCallbacks(C.Structure):
     _fields_ = [("ctx", C.c_void_p), ("eval", TABLE_CB)]
 
@@ -119,6 +124,7 @@ MLP_FEATURES, MLP_HIDDEN = 32, 128
 MLP_PARAMS = MLP_HIDDEN * MLP_FEATURES + MLP_HIDDEN + MLP_HIDDEN + 1
 MLP_MAX_BATCH = 512
 RULE_ANNEAL, RULE_HILL = 0, 1
+SCHED_GEOMETRIC, SCHED_STEP = 0, 1
 
 _lib = None
 
@@ -182,6 +188,10 @@ def lib() -> C.CDLL:
         "gg_orbit_search_table": (C.c_int, [vp, i32p, C.c_int]),
         "gg_orbit_search_set_log": (C.c_int, [vp, C.c_int]),
         "gg_orbit_search_get_log": (C.c_int, [vp, i64p, C.c_int, P(C.c_int)]),
+        "gg_init_temp": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_int,
+                                   P(InitTempResult)]),
+        "gg_init_temp_part": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64,
+                                        C.c_int, P(InitTempResult)]),
         "gg_mlp_create": (C.c_int, [C.c_int, C.c_uint64, P(vp)]),
         "gg_mlp_destroy": (None, [vp]),
         "gg_mlp_set_params": (C.c_int, [vp, P(C.c_float)]),
@@ -470,10 +480,10 @@ def aspl(result: Result, N: int) -> float:
 
 def search_cfg(mode=MODE_PLAIN, rule=RULE_ANNEAL, K=1, P=1, total_steps=1000, T0=1e-3,
                T1=1e-6, seed=12345, warmup_pairs=2048, train_every=1, train_batch=256,
-               lr=1e-3) -> SearchCfg:
+               lr=1e-3, schedule=0) -> SearchCfg:
     return SearchCfg(mode=mode, rule=rule, K=K, P=P, total_steps=total_steps,
                      warmup_pairs=warmup_pairs, train_every=train_every, train_batch=train_batch,
-                     T0=T0, T1=T1, lr=lr, seed=seed)
+                     T0=T0, T1=T1, lr=lr, seed=seed, schedule=schedule)
 
 
 class Search:
@@ -535,6 +545,19 @@ class Search:
             self.close()
         except Exception:
             pass
+
+
+def init_temp(N, degree, graphs=100, neighbours=100, seed=1, device=0, M=None) -> InitTempResult:
+    """Initial temperature as cppcode/initTemp.cpp:399-430 computes it: explicit
+    random regular graphs (M None or N), or symmetric part tables with M rows."""
+    r = InitTempResult()
+    if M is None or M == N:
+        check(lib().gg_init_temp(N, degree, graphs, neighbours, seed, device, C.byref(r)),
+              "gg_init_temp")
+    else:
+        check(lib().gg_init_temp_part(N, M, degree, graphs, neighbours, seed, device, C.byref(r)),
+              "gg_init_temp_part")
+    return r
 
 
 class OrbitSearch:

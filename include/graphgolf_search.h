@@ -26,6 +26,10 @@ typedef struct gg_search gg_search;
 #define GG_RULE_ANNEAL  0  /* cppcode/annealing.cpp:406-412                   */
 #define GG_RULE_HILL    1  /* cppcode/HillClimb.cpp:239                        */
 
+#define GG_SCHED_GEOMETRIC 0  /* T(i) = T0 (T1/T0)^(i/total_steps)              */
+#define GG_SCHED_STEP      1  /* T(i) = T0 0.995^floor((i-1)/10^4), updated after
+                                 every 10^4 iterations (annealing.cpp:442)     */
+
 typedef struct gg_search_cfg {
     int32_t mode;
     int32_t rule;
@@ -38,6 +42,7 @@ typedef struct gg_search_cfg {
     double T0, T1;        /* temperature on the ASPL scale, geometric T0 -> T1 */
     double lr;            /* learned: Adam learning rate                     */
     uint64_t seed;
+    int32_t schedule;     /* GG_SCHED_*                                       */
 } gg_search_cfg;
 
 typedef struct gg_search_stats {
@@ -115,6 +120,40 @@ int gg_orbit_search_table(gg_orbit_search *s, int32_t *table_out, int best);
  * accepted} (total = diameter = -1 when the status is not GG_OK). */
 int gg_orbit_search_set_log(gg_orbit_search *s, int capacity);
 int gg_orbit_search_get_log(gg_orbit_search *s, int64_t *buf, int capacity, int *count);
+
+/* ---- seeded instances ----------------------------------------------------
+ * gg_gen_random_regular: simple d-regular graph on n vertices as an n x d
+ * adjacency with sorted rows; configuration-model pairing, resampled when
+ * d <= 6, repaired by double-edge swaps otherwise (the reference's pairing
+ * of initTemp.cpp:399-408 and RNDpiecegen.cpp:115-127 never rejects loops or
+ * multi-edges).  gg_gen_orbit_part: cyclically symmetric d-regular graph on
+ * g*m vertices as an m x d `part` table of orbit offsets (j - i) + m*s,
+ * orbits drawn by stub pairing and lift s uniform in [0, g), resampled until
+ * there are no self-loops, half orbits or repeated orbits.  Both draw from
+ * mt19937_64(seed) with explicit bounded-integer mappings. */
+int gg_gen_random_regular(int n, int d, uint64_t seed, int32_t *adj_out);
+int gg_gen_orbit_part(int g, int m, int d, uint64_t seed, int32_t *table_out);
+
+/* ---- initial temperature ------------------------------------------------
+ * cppcode/initTemp.cpp:399-430 (pieceinitTemp.cpp:228-260): over `graphs`
+ * random graphs, the mean increase of the distance total over the
+ * `neighbours` moves of each that make it worse (unconnected results
+ * skipped), and T = -mean / ln 0.4.  gg_init_temp: random regular graphs and
+ * 2-opt swaps, one batched evaluation per graph; gg_init_temp_part: random
+ * orbit graphs (N = g*M, g >= 2) and orbit moves.  Graph i comes from seed
+ * seed + 1 + i, the moves from one generator seeded with `seed`.
+ * temp_total is the reference's inittemp (annealing.cpp -t, total-distance
+ * units); temp_aspl = temp_total / (N(N-1)) is gg_search_cfg.T0's scale. */
+typedef struct gg_init_temp_result {
+    int64_t count;               /* worsening moves seen                        */
+    double mean_increase_total;  /* their mean increase of the distance total   */
+    double temp_total;
+    double temp_aspl;
+} gg_init_temp_result;
+int gg_init_temp(int N, int degree, int graphs, int neighbours, uint64_t seed, int device,
+                 gg_init_temp_result *out);
+int gg_init_temp_part(int N, int M, int degree, int graphs, int neighbours, uint64_t seed,
+                      int device, gg_init_temp_result *out);
 
 #ifdef __cplusplus
 }

@@ -77,8 +77,17 @@ def propose(adj, rng):
     return (a, b, c, dd, kind), [(a, ia, xa), (b, ib, xb), (c, ic, xc), (dd, idd, xd)]
 
 
+def temperature(step, T0, T1, total_steps, schedule=0):
+    """Temperature of step `step` (1-based) on the ASPL scale: geometric
+    T0 -> T1, or the reference's T0 * 0.995^floor((step-1)/10^4)
+    (cppcode/annealing.cpp:442, updated after every 10^4 iterations)."""
+    if schedule == 1:
+        return T0 * math.pow(0.995, (step - 1) // 10000)
+    return T0 * math.pow(T1 / T0, step / total_steps)
+
+
 def run(adj, evaluate, steps, mode_K=1, rule="anneal", T0=1e-3, T1=1e-6, total_steps=None,
-        seed=12345):
+        seed=12345, schedule=0):
     """evaluate(adj) -> (total_scaled, diameter, ok).  Returns the per-step log
     rows {step, a, b, c, d, kind, total, diameter, status, accepted}."""
     adj = np.array(adj, dtype=np.int32)
@@ -111,7 +120,7 @@ def run(adj, evaluate, steps, mode_K=1, rule="anneal", T0=1e-3, T1=1e-6, total_s
         if rule == "hill":
             acc = ok and (dyv < dx or (dyv == dx and fy < fx))
         else:
-            T = T0 * math.pow(T1 / T0, step / total_steps)
+            T = temperature(step, T0, T1, total_steps, schedule)
             acc = False
             if fy < fx or dyv < dx:
                 acc = True

@@ -39,3 +39,20 @@ def test_driver_matches_python_restatement(gg, rule, K):
     assert st.steps == steps and st.exact_evals == steps * K
     assert st.accepted == int(ref_log[:, 9].sum())
     s.close()
+
+
+def test_step_schedule_matches_python_restatement(gg):
+    """The reference's step schedule (annealing.cpp:442) across the first
+    10^4-iteration boundary."""
+    adj = gg.random_regular(32, 3, 2)
+    steps = 10030
+    cfg = gg.search_cfg(mode=gg.MODE_PLAIN, rule=gg.RULE_ANNEAL, total_steps=steps, T0=3e-2,
+                        T1=1e-6, seed=5, schedule=gg.SCHED_STEP)
+    cb = gg.PyEvaluatorCallbacks(oracle_eval.OracleSwapEvaluator(), *adj.shape)
+    s = gg.Search(adj, cfg, callbacks=cb, log=steps)
+    s.run(steps)
+    log = s.log()
+    ref_log, _ = driver_ref.run(adj, _oracle_eval_ok, steps, rule="anneal", T0=3e-2, T1=1e-6,
+                                seed=5, schedule=1)
+    assert np.array_equal(log, ref_log)
+    s.close()
