@@ -159,6 +159,68 @@ bool propose_swap(const std::vector<int32_t> &adj, int N, int d, Rng &rng, gg_sw
     return true;
 }
 
+// `count` proposals, identical to `count` calls of the loop
+//   while (!propose_swap(...)) {}
+// An attempt's five draws do not depend on the graph, so the attempts of a
+// round (as many as proposals still missing, never more than the loop would
+// make) are drawn first; then their rows are prefetched and the attempts are
+// checked in draw order.  Returns false after max_tries attempts without
+// filling all proposals.
+bool propose_swaps(const std::vector<int32_t> &adj, int N, int d, Rng &rng, int count,
+                   gg_swap *sw, MoveRec *mv, int64_t max_tries) {
+    struct Att {
+        int a, ia, c, ic, kind;
+    };
+    std::vector<Att> att((size_t)count);
+    const int32_t *A = adj.data();
+    int filled = 0;
+    int64_t tries = 0;
+    while (filled < count) {
+        const int need = count - filled;
+        for (int j = 0; j < need; j++) {
+            Att &t = att[j];
+            t.a = (int)rng.below(N);
+            t.ia = (int)rng.below(d);
+            t.c = (int)rng.below(N);
+            t.ic = (int)rng.below(d);
+            t.kind = (int)rng.below(2);
+            __builtin_prefetch(A + (size_t)t.a * d + t.ia);
+            __builtin_prefetch(A + (size_t)t.c * d + t.ic);
+        }
+        for (int j = 0; j < need; j++) {  // rows of b and dd
+            const Att &t = att[j];
+            const int32_t *rb = A + (size_t)A[(size_t)t.a * d + t.ia] * d;
+            const int32_t *rd = A + (size_t)A[(size_t)t.c * d + t.ic] * d;
+            __builtin_prefetch(rb);
+            __builtin_prefetch(rb + d - 1);
+            __builtin_prefetch(rd);
+            __builtin_prefetch(rd + d - 1);
+            __builtin_prefetch(A + (size_t)t.a * d);
+            __builtin_prefetch(A + (size_t)t.a * d + d - 1);
+        }
+        for (int j = 0; j < need; j++) {
+            const Att &t = att[j];
+            tries++;
+            const int a = t.a, c = t.c, kind = t.kind;
+            const int b = A[(size_t)a * d + t.ia], dd = A[(size_t)c * d + t.ic];
+            if (a == b || a == c || a == dd || b == c || b == dd || c == dd) continue;
+            const int xa = kind ? dd : c, xb = kind ? c : dd, xc = kind ? b : a, xd = kind ? a : b;
+            if (row_has(adj, d, a, xa) || row_has(adj, d, b, xb)) continue;
+            const int ib = row_slot(adj, d, b, a), id = row_slot(adj, d, dd, c);
+            if (ib < 0 || id < 0) continue;
+            gg_swap &o = sw[filled];
+            o.v[0] = a;  o.slot[0] = t.ia; o.nbr[0] = xa;
+            o.v[1] = b;  o.slot[1] = ib;   o.nbr[1] = xb;
+            o.v[2] = c;  o.slot[2] = t.ic; o.nbr[2] = xc;
+            o.v[3] = dd; o.slot[3] = id;   o.nbr[3] = xd;
+            if (mv) { mv[filled].a = a; mv[filled].b = b; mv[filled].c = c; mv[filled].d = dd; mv[filled].kind = kind; }
+            filled++;
+        }
+        if (filled < count && tries > max_tries) return false;
+    }
+    return true;
+}
+
 void apply_swap(std::vector<int32_t> &adj, int d, const gg_swap &sw) {
     for (int j = 0; j < 4; j++)
         if (sw.v[j] >= 0) adj[(size_t)sw.v[j] * d + sw.slot[j]] = sw.nbr[j];
@@ -187,7 +249,10 @@ static int search_init(gg_search *s, int N, int d, const int32_t *adj, const gg_
     if (st != GG_OK) return st;
     s->cur = base;
     s->best = base;
-    s->best_adj = s->adj;
+    s->best_base = s->adj;
+    s->trail.clear();
+    s->best_len = 0;
+    s->trail_live = true;
     s->sw.resize(std::max(1, std::max(cfg->K, cfg->P)));
     s->mv.resize(s->sw.size());
     s->res.resize(s->sw.size());
@@ -246,6 +311,36 @@ extern "C" void gg_search_destroy(gg_search *s) {
     delete s;
 }
 
+// Best-graph bookkeeping after an accepted swap: O(1) per step instead of a
+// copy of the graph per improvement.  The trail is folded into best_base when
+// it outgrows kTrailCap; when most of it lies past the best it is dropped, and
+// the next improvement copies the graph.
+static void track_best(gg_search *s, const gg_swap &sw) {
+    constexpr size_t kTrailCap = 1 << 16;
+    if (s->trail_live) s->trail.push_back(sw);
+    if (better(s->cur, s->best)) {
+        s->best = s->cur;
+        if (s->trail_live) {
+            s->best_len = s->trail.size();
+        } else {
+            s->best_base = s->adj;
+            s->trail.clear();
+            s->best_len = 0;
+            s->trail_live = true;
+        }
+    }
+    if (s->trail.size() > kTrailCap) {
+        for (size_t i = 0; i < s->best_len; i++) apply_swap(s->best_base, s->d, s->trail[i]);
+        if (s->trail.size() - s->best_len > kTrailCap / 2) {
+            s->trail.clear();
+            s->trail_live = false;
+        } else {
+            s->trail.erase(s->trail.begin(), s->trail.begin() + (ptrdiff_t)s->best_len);
+        }
+        s->best_len = 0;
+    }
+}
+
 // One step: propose, evaluate exactly, accept or reject.
 static int search_step(gg_search *s) {
     const gg_search_cfg &cfg = s->cfg;
@@ -254,12 +349,9 @@ static int search_step(gg_search *s) {
     int K = cfg.mode == GG_MODE_PLAIN ? 1 : cfg.K;
     int st;
     int nprop = cfg.mode == GG_MODE_LEARNED ? cfg.P : K;
-    for (int k = 0; k < nprop; k++) {
-        int tries = 0;
-        while (!propose_swap(s->adj, N, d, rng, &s->sw[k], &s->mv[k])) {
-            if (++tries > 1000000) return GG_BAD_ARG;  // no simple swap exists
-        }
-    }
+    if (!propose_swaps(s->adj, N, d, rng, nprop, s->sw.data(), s->mv.data(),
+                       1000000 + 4 * (int64_t)nprop))
+        return GG_BAD_ARG;  // no simple swap exists
     s->stats.proposals += nprop;
     if (cfg.mode == GG_MODE_LEARNED) {
         st = learn_select(s, nprop, K);  // reorders sw/mv so the first K are the chosen ones
@@ -308,10 +400,7 @@ static int search_step(gg_search *s) {
         apply_swap(s->adj, d, s->sw[bi]);
         s->cur = base;
         s->stats.accepted++;
-        if (better(s->cur, s->best)) {
-            s->best = s->cur;
-            s->best_adj = s->adj;
-        }
+        track_best(s, s->sw[bi]);
     }
     return GG_OK;
 }
@@ -337,8 +426,16 @@ extern "C" int gg_search_run(gg_search *s, int steps, gg_search_stats *out) {
 
 extern "C" int gg_search_graph(gg_search *s, int32_t *adj_out, int best) {
     if (!s || !adj_out) return GG_BAD_ARG;
-    const std::vector<int32_t> &a = best ? s->best_adj : s->adj;
-    memcpy(adj_out, a.data(), sizeof(int32_t) * a.size());
+    if (!best) {
+        memcpy(adj_out, s->adj.data(), sizeof(int32_t) * s->adj.size());
+        return GG_OK;
+    }
+    memcpy(adj_out, s->best_base.data(), sizeof(int32_t) * s->best_base.size());
+    for (size_t i = 0; i < s->best_len; i++) {
+        const gg_swap &sw = s->trail[i];
+        for (int j = 0; j < 4; j++)
+            if (sw.v[j] >= 0) adj_out[(size_t)sw.v[j] * s->d + sw.slot[j]] = sw.nbr[j];
+    }
     return GG_OK;
 }
 

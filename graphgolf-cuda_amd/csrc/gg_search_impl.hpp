@@ -31,6 +31,10 @@ struct LearnState;  // gg_learn.cpp
 
 bool propose_swap(const std::vector<int32_t> &adj, int N, int d, Rng &rng, gg_swap *sw,
                   MoveRec *mv);
+// `count` proposals as `count` redraw loops over propose_swap (same draws,
+// same results); false after max_tries attempts.  mv may be null.
+bool propose_swaps(const std::vector<int32_t> &adj, int N, int d, Rng &rng, int count,
+                   gg_swap *sw, MoveRec *mv, int64_t max_tries);
 
 // Temperature of step i (1-based) on the ASPL scale.
 inline double temperature(const gg_search_cfg &cfg, int64_t i) {
@@ -55,7 +59,14 @@ struct gg_search {
     gg_search_cfg cfg{};
     std::unique_ptr<gg::Backend> be;
     std::unique_ptr<gg::Rng> rng;
-    std::vector<int32_t> adj, best_adj;
+    std::vector<int32_t> adj;
+    // best graph = best_base with trail[0, best_len) applied; while trail_live
+    // the trail holds every accepted swap since best_base (so the current
+    // graph is best_base with the whole trail applied)
+    std::vector<int32_t> best_base;
+    std::vector<gg_swap> trail;
+    size_t best_len = 0;
+    bool trail_live = true;
     gg_result cur{}, best{};
     std::vector<gg_swap> sw;
     std::vector<gg::MoveRec> mv;

@@ -295,6 +295,24 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a, TopoRingQ topo, Den
             if constexpr (DT > 0) {
                 CandRow<DT> row;
                 row.load(a.adjR, v, sw);
+                if (l >= ecc_w && !none) {
+                    // B_l is full here, so the item only has to show V_l[v]
+                    // full: words of neighbours outside the Delta union are
+                    // B_{l-1} (LDS), a lower bound of the OR; groups of 8 until
+                    // it is full.  Otherwise the exact OR below.
+                    auto outside = [&](int x) {
+                        return ((s_ubits[x >> 5] >> (x & 31)) & 1u) ? 0ull : Bprev[x];
+                    };
+                    acc = outside(v);
+                    bool full = false;
+#pragma unroll
+                    for (int g = 0; g < DT; g += 8) {
+#pragma unroll
+                        for (int i = g; i < (g + 8 < DT ? g + 8 : DT); i++) acc |= outside(row.u[i]);
+                        if (acc == ~0ull) { full = true; break; }
+                    }
+                    if (full) continue;
+                }
                 if (none) {
                     acc = Bprev[v];
 #pragma unroll
