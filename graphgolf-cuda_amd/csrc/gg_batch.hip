@@ -15,14 +15,16 @@
 #include "gg_sparse.hpp"
 #include "gg_internal.hpp"
 #include "gg_batch.hpp"
+#include "gg_learn.hpp"
 
 using namespace gg;
 
 static const int kSparseBlock = 1024;
 static const int kSnapCapDefault = 8;
 static const size_t kSnapBytesMax = (size_t)8 << 30;
-static const size_t kOverrideBytesMax = (size_t)4 << 30;
+static const size_t kOverrideBytesMax = (size_t)8 << 30;
 static const int kDeltaCap = 2048;
+static const int kRowLevels = 3;  // levels kept row-major (the features use 1..3)
 
 // The sparse kernel keeps one column (N words) in LDS plus small bookkeeping.
 // Static LDS of k_sparse (bookkeeping and the launch's swaps), rounded up.
@@ -88,12 +90,18 @@ static int batch_alloc(gg_batch *b) {
         b->cap = kDeltaCap < N ? kDeltaCap : N;
         b->skcap = sparse_staged_keys(b);
         const int NB = (N + 31) / 32;
-        const size_t per_task = 2 * ((size_t)N * 8 + (size_t)NB * 4 + (size_t)b->cap * 4);
+        // per task: Delta state of two parities and the commit logs of `cap`
+        // levels; per candidate: four endpoint columns' words per level
+        const size_t per_task = 2 * ((size_t)N * 8 + (size_t)NB * 4 + (size_t)b->cap * 4) +
+                                (size_t)cap * ((size_t)b->cap * 12 + 4);
+        const size_t per_cand = per_task * W + 4 * ((size_t)cap * N * 8 + (size_t)L * 8);
         int chunk = K < GG_SPARSE_MAX_K ? K : GG_SPARSE_MAX_K;
-        while (chunk > 1 && per_task * W * chunk > kOverrideBytesMax) chunk--;
+        while (chunk > 1 && per_cand * chunk > kOverrideBytesMax) chunk--;
         b->chunk = chunk;
         const size_t T = (size_t)K * W, TC = (size_t)chunk * W;
         if (hipMalloc((void **)&b->d_snap, per_level * cap) != hipSuccess) return GG_OOM;
+        b->rows_levels = cap < kRowLevels ? cap : kRowLevels;
+        if (hipMalloc((void **)&b->d_rows, per_level * b->rows_levels) != hipSuccess) return GG_OOM;
         if (hipMalloc((void **)&b->d_dl_key, sizeof(int) * TC * 2 * b->cap) != hipSuccess)
             return GG_OOM;
         if (hipMalloc((void **)&b->d_dl_bits, 4 * TC * 2 * NB) != hipSuccess) return GG_OOM;
@@ -101,12 +109,27 @@ static int batch_alloc(gg_batch *b) {
         if (hipMalloc((void **)&b->d_task_state, sizeof(int) * T) != hipSuccess) return GG_OOM;
         if (hipMalloc((void **)&b->d_task_end, sizeof(int) * T) != hipSuccess) return GG_OOM;
         if (hipMalloc((void **)&b->d_task_flags, sizeof(int) * T) != hipSuccess) return GG_OOM;
+        if (hipMalloc((void **)&b->d_task_ecc, sizeof(int) * T) != hipSuccess) return GG_OOM;
         if (hipMalloc((void **)&b->d_dcnt, sizeof(long long) * T * L) != hipSuccess) return GG_OOM;
         if (hipMalloc((void **)&b->d_queue, sizeof(int2) * T) != hipSuccess) return GG_OOM;
         b->nqctl = 2 + (K + chunk - 1) / chunk;  // queue count, queue next, endpoint counters
         if (hipMalloc((void **)&b->d_qctl, sizeof(int) * b->nqctl) != hipSuccess) return GG_OOM;
         if (hipMalloc((void **)&b->d_dense_parts, sizeof(ColPart) * T) != hipSuccess)
             return GG_OOM;
+        // commit logs: optional (without them every commit runs a new BFS)
+        const size_t lg = TC * cap * (size_t)b->cap;
+        if (hipMalloc((void **)&b->d_lg_key, 4 * lg) != hipSuccess ||
+            hipMalloc((void **)&b->d_lg_word, 8 * lg) != hipSuccess ||
+            hipMalloc((void **)&b->d_lg_n, 4 * TC * cap) != hipSuccess ||
+            hipMalloc((void **)&b->d_ep_snap, 8 * (size_t)4 * chunk * cap * N) != hipSuccess ||
+            hipMalloc((void **)&b->d_ep_reach, 8 * (size_t)4 * chunk * L) != hipSuccess) {
+            (void)hipGetLastError();
+            void *lgb[] = {b->d_lg_key, b->d_lg_word, b->d_lg_n, b->d_ep_snap, b->d_ep_reach};
+            for (void *p : lgb)
+                if (p) (void)hipFree(p);
+            b->d_lg_key = nullptr; b->d_lg_word = nullptr; b->d_lg_n = nullptr;
+            b->d_ep_snap = nullptr; b->d_ep_reach = nullptr;
+        }
         GG_HIP(hipMemsetAsync(b->d_qctl, 0, sizeof(int) * b->nqctl, b->stream));
         GG_HIP(hipMemsetAsync(b->d_dl_bits, 0, 4 * TC * 2 * NB, b->stream));
     }
@@ -145,7 +168,8 @@ extern "C" void gg_batch_destroy(gg_batch *b) {
     void *bufs[] = {b->d_tab, b->d_adjR, b->d_src, b->d_parts, b->d_res, b->d_swaps,
                     b->d_ecc, b->d_base_reach, b->d_snap, b->d_dl_key, b->d_dl_bits, b->d_dl_ovr,
                     b->d_task_state, b->d_task_end, b->d_task_flags, b->d_dcnt, b->d_queue,
-                    b->d_qctl, b->d_dense_parts};
+                    b->d_qctl, b->d_dense_parts, b->d_lg_key, b->d_lg_word, b->d_lg_n,
+                    b->d_ep_snap, b->d_ep_reach, b->d_rows, b->d_task_ecc};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     if (b->h_tab) (void)hipHostFree(b->h_tab);
@@ -183,6 +207,37 @@ static DenseArgs dense_args(gg_batch *b, int K, const gg_swap *d_swaps, ColPart 
     return a;
 }
 
+// Row-major copy of levels 1..L of the stored words: tile (64 vertices x 64
+// columns) through LDS, levels at or above a column's eccentricity written as
+// all ones.  Grid (ceil(N/64), ceil(W/64), L), 256 threads.
+__global__ void __launch_bounds__(256) k_rows_transpose(const unsigned long long *snap,
+                                                        const ColPart *parts,
+                                                        unsigned long long *rows, int N, int W) {
+    __shared__ unsigned long long t[64][65];
+    const int v0 = blockIdx.x * 64, w0 = blockIdx.y * 64, l = blockIdx.z + 1;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int i = ty; i < 64; i += 4) {  // column w0+i, vertices v0+tx (coalesced)
+        const int w = w0 + i, v = v0 + tx;
+        unsigned long long x = ~0ull;
+        if (w < W && v < N && l < parts[w].ecc) x = snap[((size_t)(l - 1) * W + w) * N + v];
+        t[i][tx] = x;
+    }
+    __syncthreads();
+    for (int j = ty; j < 64; j += 4) {  // vertex v0+j, columns w0+tx (coalesced)
+        const int v = v0 + j, w = w0 + tx;
+        if (v < N && w < W) rows[((size_t)(l - 1) * N + v) * W + w] = t[tx][j];
+    }
+}
+
+static int enqueue_rows(gg_batch *b) {
+    if (!b->d_rows || b->rows_levels < 1) return GG_OK;
+    hipLaunchKernelGGL(k_rows_transpose, dim3((b->N + 63) / 64, (b->W + 63) / 64, b->rows_levels),
+                       dim3(256), 0, b->stream, b->d_snap, b->d_parts + (size_t)b->max_k * b->W,
+                       b->d_rows, b->N, b->W);
+    GG_HIP(hipGetLastError());
+    return GG_OK;
+}
+
 // Evaluates the resident base (slot max_k), storing the per-level snapshots
 // and per-column level counts the sparse path starts from.
 static int eval_base(gg_batch *b, gg_result *out) {
@@ -194,6 +249,7 @@ static int eval_base(gg_batch *b, gg_result *out) {
     a.snap_levels = b->d_snap ? b->snap_cap : 0;
     int st = run_dense(b->dc, a, b->stream);
     if (st != GG_OK) return st;
+    if ((st = enqueue_rows(b)) != GG_OK) return st;
     GG_HIP(hipMemcpyAsync(b->h_res + K, b->d_res + K, sizeof(ResultRec), hipMemcpyDeviceToHost,
                           b->stream));
     GG_HIP(hipStreamSynchronize(b->stream));
@@ -201,6 +257,8 @@ static int eval_base(gg_batch *b, gg_result *out) {
     fill(b, b->h_res[K], &r);
     if (out) *out = r;
     b->base_res = b->h_res[K];
+    b->ecc_stale = 0;
+    b->log_valid = 0;
     // the sparse path needs a connected base whose levels below the diameter
     // are all stored, and the LDS-resident kernels
     b->sparse_ok = b->d_snap && r.status == GG_OK && r.diameter - 1 <= b->snap_cap &&
@@ -258,7 +316,7 @@ static hipError_t launch_sparse(const SparseArgs &s, const TopoRingQ &topo, cons
 }
 
 // Enqueues the sparse pipeline for K candidates whose swaps are in d_swaps.
-static int enqueue_sparse(gg_batch *b, int K) {
+static int enqueue_sparse(gg_batch *b, int K, bool reset = true) {
     const int W = b->W, N = b->N;
     SparseArgs s{};
     s.N = N; s.D = b->degree; s.W = W; s.K = K; s.lmax = GG_LEVEL_LIMIT;
@@ -277,11 +335,17 @@ static int enqueue_sparse(gg_batch *b, int K) {
     s.task_state = b->d_task_state;
     s.task_end = b->d_task_end;
     s.task_flags = b->d_task_flags;
+    s.task_ecc = b->d_task_ecc;
     s.queue = b->d_queue;
     s.queue_count = b->d_qctl;
     s.queue_next = b->d_qctl + 1;
     s.dense_parts = b->d_dense_parts;
     s.res = b->d_res;
+    s.lg_key = b->d_lg_key;
+    s.lg_word = b->d_lg_word;
+    s.lg_n = b->d_lg_n;
+    s.ep_snap = b->d_ep_snap;
+    s.ep_reach = b->d_ep_reach;
     const size_t lds = sparse_lds_bytes(N, b->skcap);
     // dense tail of the sparse kernel (endpoint columns): fixed-degree ring
     // table, its vertices-per-thread instance, a push list within the LDS
@@ -293,6 +357,9 @@ static int enqueue_sparse(gg_batch *b, int K) {
     }
     DenseArgs qa = dense_args(b, K, b->d_swaps, b->d_dense_parts, nullptr, nullptr);
     qa.push_cap = push_capacity(b->dc, N, b->degree, (int)lds);
+    // the logs of the last launch survive: they cover its candidates
+    b->log_k0 = ((K - 1) / b->chunk) * b->chunk;
+    b->log_K = K;
     for (int k0 = 0, c_i = 0; k0 < K; k0 += b->chunk, c_i++) {
         SparseArgs c = s;
         c.k0 = k0;
@@ -309,6 +376,10 @@ static int enqueue_sparse(gg_batch *b, int K) {
     a.queue = b->d_queue;
     a.queue_count = b->d_qctl;
     a.queue_next = b->d_qctl + 1;
+    a.ep_snap = b->d_ep_snap;
+    a.ep_reach = b->d_ep_reach;
+    a.ep_k0 = b->log_k0;
+    a.ep_S = b->snap_cap;
     int per_cu = (int)((size_t)b->dc.lds_device / (lds + 4096));
     if (per_cu < 1) per_cu = 1;
     if (per_cu > 2) per_cu = 2;
@@ -318,7 +389,8 @@ static int enqueue_sparse(gg_batch *b, int K) {
     if (st != GG_OK) return st;
     hipLaunchKernelGGL(k_sparse_finalize<256>, dim3(K), dim3(256), 0, b->stream, s,
                        N - 64 * (W - 1));
-    hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, b->stream, b->d_qctl, b->nqctl);
+    if (reset)
+        hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, b->stream, b->d_qctl, b->nqctl);
     GG_HIP(hipGetLastError());
     return GG_OK;
 }
@@ -373,9 +445,59 @@ extern "C" int gg_batch_eval(gg_batch *b, const gg_swap *swaps, int K, gg_result
     GG_HIP(hipStreamSynchronize(b->stream));
     for (int k = 0; k < K; k++) fill(b, b->h_res[k], &out[k]);
     b->last_sparse = sparse ? 1 : 0;
+    b->log_valid = sparse && b->d_lg_key != nullptr;
     b->have_timing = 1;
     return GG_OK;
 }
+
+// ---- steps whose swaps a kernel writes ------------------------------------------------
+// The learned search draws and selects its candidates on the GPU: the chosen
+// swaps are already in d_swaps when the incremental step runs.  These two
+// halves wrap that step for a caller that captures its own graph.
+namespace gg {
+// Enqueues (on the batch stream, typically under capture) the incremental
+// evaluation of the K candidates in d_swaps and the copies of their results
+// and swaps to the host buffers.
+// The caller reads the results and swaps back (batch_out) and resets the
+// queue counters after the step.
+int batch_enqueue_dev(gg_batch *b, int K) {
+    if (!b->have_base || !b->sparse_ok || b->mode_opt == 1 || K < 1 || K > b->max_k)
+        return GG_UNSUPPORTED;
+    // swaps drawn as 2-opt rewirings of an undirected base keep it undirected
+    b->dc.push_ok = b->base_sym;
+    return enqueue_sparse(b, K, false);
+}
+
+BatchOut batch_out(gg_batch *b) {
+    BatchOut o;
+    o.d_res = b->d_res;
+    o.h_res = b->h_res;
+    o.h_swaps = b->h_swaps;
+    o.qctl = b->d_qctl;
+    o.nqctl = b->nqctl;
+    return o;
+}
+
+// After the enqueued step has completed: results, and the bookkeeping a
+// later commit of one of the candidates uses.
+int batch_finish_dev(gg_batch *b, int K, gg_result *out) {
+    for (int k = 0; k < K; k++) fill(b, b->h_res[k], &out[k]);
+    b->last_sparse = 1;
+    b->log_valid = b->d_lg_key != nullptr;
+    return GG_OK;
+}
+
+int batch_push_state(gg_batch *b) { return b->base_sym ? 1 : 0; }
+const int *b_adjR(gg_batch *b) { return b->d_adjR; }
+int batch_shape(gg_batch *b, int *N, int *d) {
+    if (!b->have_base) return GG_BAD_ARG;
+    *N = b->N;
+    *d = b->degree;
+    return GG_OK;
+}
+gg_swap *b_swaps(gg_batch *b) { return b->d_swaps; }
+const gg_swap *b_hswaps(gg_batch *b) { return b->h_swaps; }
+}  // namespace gg
 
 // The swap travels as a kernel argument: no host buffer outlives the launch.
 __global__ void k_batch_commit(uint16_t *tab16, uint32_t *tab32, int *adjR, int N, int D,
@@ -392,6 +514,158 @@ __global__ void k_batch_commit(uint16_t *tab16, uint32_t *tab32, int *adjR, int 
     }
 }
 
+// ---- in-place commit ------------------------------------------------------------------
+// Committing candidate k of the last sparse launch turns its logged levels into
+// the new base's: B_l := V_l on Delta_l for every incremental column, the
+// endpoint columns' words copied whole from the dense tail's copy, then the
+// per-column level counts and parts rebuilt from the candidate's count
+// deltas.  Levels at or above a column's (new) eccentricity are never read
+// (every reader takes them as full), so they are left as they are.
+struct CommitArgs {
+    int N, W, S, L1, cap, D;
+    int k, gk;                       // candidate: index in the logs' launch, index in the batch
+    gg_swap sw;
+    unsigned long long *snap;        // [S][W][N]
+    unsigned long long *rows;        // [RL][N][W] row-major levels 1..RL (null: none)
+    int RL;
+    ColPart *base_parts;             // [W]
+    long long *base_reach;           // [W][L1]
+    const int *task_state, *task_end, *task_ecc;
+    const long long *dcnt;           // [K][W][L1]
+    const int *lg_key;
+    const unsigned long long *lg_word;
+    const int *lg_n;
+    const unsigned long long *ep_snap;
+    const long long *ep_reach;
+    const ColPart *dense_parts;      // [K][W]
+    uint16_t *tab16;                 // resident tables: the swap's slots are rewritten
+    uint32_t *tab32;
+    int *adjR;
+};
+
+// Endpoint task (4k + j) of the dense tail that ran column w: the first
+// endpoint of the swap in that column.
+__device__ __forceinline__ int commit_ep_slot(const CommitArgs &c, int w) {
+    for (int j = 0; j < 4; j++)
+        if (c.sw.v[j] >= 0 && (c.sw.v[j] >> 6) == w) return 4 * c.k + j;
+    return -1;
+}
+
+// New eccentricity of the candidate's column w (recorded by the finalize).
+__device__ __forceinline__ int commit_col_ecc(const CommitArgs &c, int w) {
+    return c.task_ecc[(size_t)c.gk * c.W + w];
+}
+
+// Grid (W + 4 * chunks, S) with chunks = ceil(N / 1024): workgroup (w, l-1),
+// w < W, rewrites level l of column w (and of the row-major copy for l <= RL
+// when the column is incremental); workgroups w >= W write the row-major copy
+// of the endpoint columns, 1024 vertices each (endpoint j, vertex chunk).
+__global__ void __launch_bounds__(1024) k_commit_levels(CommitArgs c) {
+    const int l = blockIdx.y + 1, tid = threadIdx.x;
+    const int N = c.N;
+    if ((int)blockIdx.x >= c.W) {
+        if (!c.rows || l > c.RL) return;
+        const int chunks = (N + 1023) >> 10;
+        const int e = ((int)blockIdx.x - c.W) / chunks, ch = ((int)blockIdx.x - c.W) - e * chunks;
+        const int v0 = c.sw.v[e];
+        if (v0 < 0) return;
+        const int w = v0 >> 6;
+        if (commit_ep_slot(c, w) != 4 * c.k + e) return;  // column written by a lower endpoint
+        const int ecc_new = commit_col_ecc(c, w);
+        const unsigned long long *src = c.ep_snap + ((size_t)(4 * c.k + e) * c.S + (l - 1)) * N;
+        const int v = ch * 1024 + tid;
+        if (v < N) c.rows[((size_t)(l - 1) * N + v) * c.W + w] = l < ecc_new ? src[v] : ~0ull;
+        return;
+    }
+    const int w = blockIdx.x;
+    const size_t t = (size_t)c.gk * c.W + w;
+    const int ecc_new = commit_col_ecc(c, w);
+    unsigned long long *__restrict__ dst = c.snap + ((size_t)(l - 1) * c.W + w) * N;
+    if (c.task_state[t] == GG_TASK_DENSE) {
+        if (l >= ecc_new) return;
+        const int slot = commit_ep_slot(c, w);
+        const unsigned long long *__restrict__ src =
+            c.ep_snap + ((size_t)slot * c.S + (l - 1)) * N;
+        if ((N & 1) == 0) {  // 16-byte words, eight per thread in flight
+            const uint4 *__restrict__ s4 = reinterpret_cast<const uint4 *>(src);
+            uint4 *__restrict__ d4 = reinterpret_cast<uint4 *>(dst);
+#pragma unroll 8
+            for (int i = tid; i < (N >> 1); i += 1024) d4[i] = s4[i];
+        } else {
+            for (int v = tid; v < N; v += 1024) dst[v] = src[v];
+        }
+        return;
+    }
+    const int end = c.task_end[t];
+    const size_t lt = ((size_t)c.k * c.W + w) * c.S + (l - 1);
+    const int *key = c.lg_key + lt * c.cap;
+    const unsigned long long *word = c.lg_word + lt * c.cap;
+    // the row-major copy stores full words explicitly: the Delta entries are
+    // all that change (no fill)
+    if (c.rows && l <= c.RL && l <= end) {
+        unsigned long long *rdst = c.rows + (size_t)(l - 1) * N * c.W + w;
+        const int n = c.lg_n[lt];
+        for (int i = tid; i < n; i += 1024) rdst[(size_t)key[i] * c.W] = word[i];
+    }
+    if (l >= ecc_new) return;
+    if (l >= c.base_parts[w].ecc) {
+        // the base column was full at this level, so nothing was stored
+        for (int v = tid; v < N; v += 1024) dst[v] = ~0ull;
+        __syncthreads();
+    }
+    const int n = c.lg_n[lt];
+    for (int i = tid; i < n; i += 1024) dst[key[i]] = word[i];
+}
+
+// Grid W, one wave: column w's new part and per-level counts; workgroup 0
+// also rewrites the swap's table slots.  Runs after k_commit_levels, which
+// reads the old parts.
+__global__ void __launch_bounds__(64) k_commit_parts(CommitArgs c) {
+    const int w = blockIdx.x, lane = threadIdx.x;
+    const size_t t = (size_t)c.gk * c.W + w;
+    long long *br = c.base_reach + (size_t)w * c.L1;
+    if (w == 0 && lane < 4) {
+        const int v = c.sw.v[lane];
+        if (v >= 0) {
+            const int slot = c.sw.slot[lane], u = c.sw.nbr[lane];
+            int off = u - v;
+            if (off < 0) off += c.N;
+            if (c.tab16) c.tab16[quad_index(slot, c.N, v)] = (uint16_t)u;
+            else c.tab32[(size_t)slot * c.N + v] = (uint32_t)off;
+            c.adjR[(size_t)v * c.D + slot] = u;
+        }
+    }
+    if (c.task_state[t] == GG_TASK_DENSE) {
+        const ColPart p = c.dense_parts[t];
+        const long long *er = c.ep_reach + (size_t)commit_ep_slot(c, w) * c.L1;
+        for (int l = lane; l <= p.levels && l < c.L1; l += 64) br[l] = er[l];
+        if (lane == 0) c.base_parts[w] = p;
+        return;
+    }
+    if (lane != 0) return;
+    const int ecc = commit_col_ecc(c, w);
+    // one pass in place: level l's old count is read before it is replaced
+    const int blev = c.base_parts[w].levels;
+    const long long *dd = c.dcnt + t * c.L1;
+    long long cb = br[0], cprev = cb, tot = 0;
+    for (int l = 1; l <= ecc; l++) {
+        if (l <= blev) cb += br[l];
+        const long long cur = cb + dd[l];
+        br[l] = cur - cprev;
+        tot += (long long)l * (cur - cprev);
+        cprev = cur;
+    }
+    if (ecc + 1 < c.L1) br[ecc + 1] = 0;
+    ColPart p;
+    p.total = tot;
+    p.reached = cprev;
+    p.ecc = ecc;
+    p.levels = ecc + 1;
+    p.flags = 0;
+    p.pad = 0;
+    c.base_parts[w] = p;
+}
+
 // Applies the swap to the resident tables (host mirror and device).
 static int apply_commit(gg_batch *b, const gg_swap *sw) {
     b->base_sym = b->base_sym && swap_keeps_symmetry(b->h_adj.data(), b->degree, *sw);
@@ -406,35 +680,99 @@ static int apply_commit(gg_batch *b, const gg_swap *sw) {
     return GG_OK;
 }
 
+extern "C" int gg_batch_commit_known(gg_batch *b, const gg_swap *sw, const gg_result *known);
+
 extern "C" int gg_batch_commit(gg_batch *b, const gg_swap *sw, gg_result *out) {
     if (!b || !sw || !b->have_base) return GG_BAD_ARG;
     int st = check_swaps(b, sw, 1);
     if (st != GG_OK) return st;
     GG_HIP(hipSetDevice(b->device));
+    // one of the last incremental launch's candidates: its result is known
+    if (b->log_valid && b->commit_mode == 0)
+        for (int k = b->log_k0; k < b->log_K; k++)
+            if (memcmp(&b->h_swaps[k], sw, sizeof(gg_swap)) == 0) {
+                gg_result r;
+                fill(b, b->h_res[k], &r);
+                if (r.status != GG_OK) break;
+                st = gg_batch_commit_known(b, sw, &r);
+                if (st == GG_OK && out) *out = r;
+                return st;
+            }
     st = apply_commit(b, sw);
     if (st != GG_OK) return st;
     return eval_base(b, out);
 }
 
 // Commit of a candidate whose exact result the caller already holds (one of
-// the last batch's): the base re-evaluation that refreshes the stored levels
-// is queued on the batch stream, which orders every later launch after it,
-// and not waited for.  Errors of that launch surface at the next call.
+// the last batch's).  When the last launch kept its logs for that candidate
+// the stored base levels are rewritten in place (two small launches);
+// otherwise the base re-evaluation that refreshes them is queued.  Either way
+// the work runs on the batch stream, which orders every later launch after
+// it, and is not waited for: errors surface at the next call.
 extern "C" int gg_batch_commit_known(gg_batch *b, const gg_swap *sw, const gg_result *known) {
     if (!b || !sw || !known || !b->have_base) return GG_BAD_ARG;
     int st = check_swaps(b, sw, 1);
     if (st != GG_OK) return st;
     GG_HIP(hipSetDevice(b->device));
-    st = apply_commit(b, sw);
-    if (st != GG_OK) return st;
     const int K = b->max_k;
-    DenseArgs a = dense_args(b, 1, nullptr, b->d_parts + (size_t)K * b->W, b->d_res + K,
-                             b->d_ecc);
-    a.reach_col = b->d_base_reach;
-    a.snap = b->d_snap;
-    a.snap_levels = b->d_snap ? b->snap_cap : 0;
-    st = run_dense(b->dc, a, b->stream);
-    if (st != GG_OK) return st;
+    int kc = -1;
+    if (b->log_valid && b->commit_mode == 0)
+        for (int k = b->log_k0; k < b->log_K; k++)
+            if (memcmp(&b->h_swaps[k], sw, sizeof(gg_swap)) == 0) { kc = k; break; }
+    const bool inplace = kc >= 0 && known->status == GG_OK &&
+                         known->diameter - 1 <= b->snap_cap &&
+                         !(b->h_res[kc].flags & (GG_FLAG_QUEUED | GG_FLAG_LIMIT)) &&
+                         b->h_res[kc].total == known->total_reps;
+    b->log_valid = 0;
+    b->last_commit_inplace = inplace ? 1 : 0;
+    if (inplace) {
+        b->base_sym = b->base_sym && swap_keeps_symmetry(b->h_adj.data(), b->degree, *sw);
+        for (int j = 0; j < 4; j++)
+            if (sw->v[j] >= 0) b->h_adj[(size_t)sw->v[j] * b->degree + sw->slot[j]] = sw->nbr[j];
+        b->dc.push_ok = b->base_sym;
+        CommitArgs c{};
+        c.N = b->N; c.W = b->W; c.S = b->snap_cap; c.L1 = GG_LEVEL_LIMIT + 1; c.cap = b->cap;
+        c.D = b->degree;
+        c.k = kc - b->log_k0;
+        c.gk = kc;
+        c.sw = *sw;
+        c.snap = b->d_snap;
+        c.rows = b->d_rows;
+        c.RL = b->rows_levels;
+        c.base_parts = b->d_parts + (size_t)K * b->W;
+        c.base_reach = b->d_base_reach;
+        c.task_state = b->d_task_state;
+        c.task_end = b->d_task_end;
+        c.task_ecc = b->d_task_ecc;
+        c.dcnt = b->d_dcnt;
+        c.lg_key = b->d_lg_key;
+        c.lg_word = b->d_lg_word;
+        c.lg_n = b->d_lg_n;
+        c.ep_snap = b->d_ep_snap;
+        c.ep_reach = b->d_ep_reach;
+        c.dense_parts = b->d_dense_parts;
+        c.tab16 = b->narrow ? (uint16_t *)b->d_tab : nullptr;
+        c.tab32 = b->narrow ? nullptr : (uint32_t *)b->d_tab;
+        c.adjR = b->d_adjR;
+        const int chunks = (b->N + 1023) / 1024;
+        hipLaunchKernelGGL(k_commit_levels, dim3(b->W + 4 * chunks, b->snap_cap), dim3(1024), 0,
+                           b->stream, c);
+        hipLaunchKernelGGL(k_commit_parts, dim3(b->W), dim3(64), 0, b->stream, c);
+        GG_HIP(hipGetLastError());
+        b->ecc_stale = 1;
+    } else {
+        st = apply_commit(b, sw);
+        if (st != GG_OK) return st;
+        DenseArgs a = dense_args(b, 1, nullptr, b->d_parts + (size_t)K * b->W, b->d_res + K,
+                                 b->d_ecc);
+        a.reach_col = b->d_base_reach;
+        a.snap = b->d_snap;
+        a.snap_levels = b->d_snap ? b->snap_cap : 0;
+        st = run_dense(b->dc, a, b->stream);
+        if (st != GG_OK) return st;
+        if ((st = enqueue_rows(b)) != GG_OK) return st;
+        b->ecc_stale = 0;
+    }
     ResultRec r{};
     r.total = known->total_reps;
     r.reached = known->status == GG_OK ? (long long)b->N * b->N : 0;
@@ -457,9 +795,85 @@ extern "C" int gg_batch_last_kernel_ms(gg_batch *b, float *ms) {
 extern "C" int gg_batch_base_ecc(gg_batch *b, int32_t *ecc_out) {
     if (!b || !ecc_out || !b->have_base) return GG_BAD_ARG;
     GG_HIP(hipSetDevice(b->device));
+    if (b->ecc_stale) {  // an in-place commit kept the levels, not the eccentricities
+        int st = eval_base(b, nullptr);
+        if (st != GG_OK) return st;
+    }
     GG_HIP(hipMemcpyAsync(ecc_out, b->d_ecc, sizeof(int) * b->N, hipMemcpyDeviceToHost, b->stream));
     GG_HIP(hipStreamSynchronize(b->stream));
     return GG_OK;
+}
+
+// ---- digest of the stored base state ----------------------------------------------------
+__device__ __forceinline__ unsigned long long mix64(unsigned long long x) {
+    // splitmix64 finaliser
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+// Grid (W, S + 1 + RL): y = 0 hashes column w's part and counts, y = l in
+// 1..S its words after level l (all ones at or above the column's
+// eccentricity), y = S + l its row-major copy of level l <= RL.
+__global__ void __launch_bounds__(256) k_levels_digest(const unsigned long long *snap,
+                                                       const ColPart *parts,
+                                                       const long long *reach, int N, int W,
+                                                       int L1, unsigned long long *out,
+                                                       const unsigned long long *rows, int S) {
+    __shared__ unsigned long long s_h[4];
+    const int w = blockIdx.x, l = blockIdx.y, tid = threadIdx.x;
+    const ColPart p = parts[w];
+    unsigned long long h = 0;
+    if (l == 0) {
+        if (tid == 0) {
+            h = mix64((unsigned long long)p.total ^ 0x1111) + mix64((unsigned long long)p.reached ^ 0x2222) +
+                mix64(((unsigned long long)p.ecc << 32 | (unsigned)p.levels) ^ 0x3333);
+            h = mix64(h + (unsigned long long)w);
+        }
+        if (tid >= 1 && tid <= p.levels && tid < L1)
+            h = mix64((unsigned long long)reach[(size_t)w * L1 + tid] + mix64((unsigned long long)tid * 977 + (unsigned long long)w * 7919));
+    } else if (l > S) {
+        const int lr = l - S;  // compared with the stored words of level lr
+        const unsigned long long salt = mix64(((unsigned long long)lr << 40) ^ ((unsigned long long)w << 20));
+        for (int v = tid; v < N; v += 256)
+            h += mix64(rows[((size_t)(lr - 1) * N + v) * W + w] ^ mix64(salt + (unsigned long long)v));
+    } else {
+        const bool full = l >= p.ecc;
+        const unsigned long long *src = snap + ((size_t)(l - 1) * W + w) * N;
+        const unsigned long long salt = mix64(((unsigned long long)l << 40) ^ ((unsigned long long)w << 20));
+        for (int v = tid; v < N; v += 256)
+            h += mix64((full ? ~0ull : src[v]) ^ mix64(salt + (unsigned long long)v));
+    }
+    for (int o = 32; o > 0; o >>= 1) h += __shfl_xor(h, o, 64);
+    if ((tid & 63) == 0) s_h[tid >> 6] = h;
+    __syncthreads();
+    if (tid == 0) atomicAdd(out, s_h[0] + s_h[1] + s_h[2] + s_h[3]);
+}
+
+extern "C" int gg_batch_levels_digest(gg_batch *b, uint64_t *digest) {
+    if (!b || !digest || !b->have_base || !b->d_snap) return GG_BAD_ARG;
+    GG_HIP(hipSetDevice(b->device));
+    unsigned long long *d = nullptr;
+    if (hipMalloc((void **)&d, 8) != hipSuccess) return GG_OOM;
+    int st = GG_OK;
+    if (hipMemsetAsync(d, 0, 8, b->stream) != hipSuccess) st = GG_HIP_ERROR;
+    if (st == GG_OK) {
+        // the row-major copy hashes like the words it copies: equal digests
+        // need both to hold the same levels
+        hipLaunchKernelGGL(k_levels_digest, dim3(b->W, b->snap_cap + 1 + b->rows_levels), dim3(256),
+                           0, b->stream, b->d_snap, b->d_parts + (size_t)b->max_k * b->W,
+                           b->d_base_reach, b->N, b->W, GG_LEVEL_LIMIT + 1, d, b->d_rows,
+                           b->snap_cap);
+        if (hipGetLastError() != hipSuccess) st = GG_HIP_ERROR;
+    }
+    unsigned long long h = 0;
+    if (st == GG_OK && hipMemcpyAsync(&h, d, 8, hipMemcpyDeviceToHost, b->stream) != hipSuccess)
+        st = GG_HIP_ERROR;
+    if (hipStreamSynchronize(b->stream) != hipSuccess) st = GG_HIP_ERROR;
+    (void)hipFree(d);
+    *digest = h;
+    return st;
 }
 
 extern "C" int gg_batch_set_option(gg_batch *b, int option, int value) {
@@ -473,6 +887,10 @@ extern "C" int gg_batch_set_option(gg_batch *b, int option, int value) {
             b->use_graph = value ? 1 : 0;
             if (!value) drop_graph(b);
             return GG_OK;
+        case GG_BATCH_OPT_COMMIT:
+            if (value < 0 || value > 1) return GG_BAD_ARG;
+            b->commit_mode = value;
+            return GG_OK;
         default: return GG_BAD_ARG;
     }
 }
@@ -484,6 +902,8 @@ extern "C" int gg_batch_get_option(gg_batch *b, int option, int *value) {
         case GG_BATCH_OPT_GRAPH: *value = b->use_graph; return GG_OK;
         case GG_BATCH_INFO_LAST_SPARSE: *value = b->last_sparse; return GG_OK;
         case GG_BATCH_INFO_SPARSE_OK: *value = b->sparse_ok; return GG_OK;
+        case GG_BATCH_OPT_COMMIT: *value = b->commit_mode; return GG_OK;
+        case GG_BATCH_INFO_LAST_COMMIT_INPLACE: *value = b->last_commit_inplace; return GG_OK;
         default: return GG_BAD_ARG;
     }
 }

@@ -21,6 +21,10 @@ struct gg_batch {
     // incremental (sparse) evaluation state
     int snap_cap = 0;                         // base levels whose words are stored
     unsigned long long *d_snap = nullptr;     // [snap_cap][W][N]
+    // the same words row-major for the first rows_levels levels, full levels
+    // stored as all ones: rows[l-1][x][w] (features read a vertex's ball row)
+    unsigned long long *d_rows = nullptr;     // [rows_levels][N][W]
+    int rows_levels = 0;
     int cap = 0;                              // Delta list capacity per task
     int skcap = 0;                            // Delta keys staged in LDS per column and level
     int nqctl = 0;                            // d_qctl words: queue count/next + endpoint counters
@@ -29,11 +33,22 @@ struct gg_batch {
     unsigned *d_dl_bits = nullptr;            // [chunk*W][2][ceil(N/32)]
     unsigned long long *d_dl_ovr = nullptr;   // [chunk*W][2][N]
     int *d_task_state = nullptr, *d_task_end = nullptr,
-        *d_task_flags = nullptr;
+        *d_task_flags = nullptr, *d_task_ecc = nullptr;
     long long *d_dcnt = nullptr;
     int2 *d_queue = nullptr;
     int *d_qctl = nullptr;                    // [2] queue count, queue next
     ColPart *d_dense_parts = nullptr;         // [max_k * W]
+    // kept by the sparse launch for an in-place commit (null: not kept)
+    int *d_lg_key = nullptr;                  // [chunk*W][snap_cap][cap]
+    unsigned long long *d_lg_word = nullptr;  // [chunk*W][snap_cap][cap]
+    int *d_lg_n = nullptr;                    // [chunk*W][snap_cap]
+    unsigned long long *d_ep_snap = nullptr;  // [4*chunk][snap_cap][N]
+    long long *d_ep_reach = nullptr;          // [4*chunk][LEVEL_LIMIT+1]
+    int log_valid = 0;                        // the last eval was sparse with logs kept
+    int log_k0 = 0, log_K = 0;                // candidates the logs cover: [log_k0, log_K)
+    int ecc_stale = 0;                        // d_ecc predates an in-place commit
+    int commit_mode = 0;                      // 0 in place when possible, 1 always a new BFS
+    int last_commit_inplace = 0;
     int mode_opt = 0;                         // 0 auto, 1 dense only, 2 sparse when possible
     int sparse_ok = 0;                        // base allows the sparse path
     int last_sparse = 0;                      // last gg_batch_eval used the sparse path

@@ -29,6 +29,7 @@ struct ResultRec {
 };
 
 #define GG_FLAG_LIMIT 1
+#define GG_FLAG_QUEUED 2  // a column ran in the dense fallback queue (no stored levels)
 
 // Per (candidate, column) partial written by the workgroup that ran it.
 struct ColPart {

@@ -130,6 +130,12 @@ struct DenseArgs {
     // push levels: capacity of the level-1 frontier list that follows vis[N]
     // in dynamic LDS (0 = pull from level 1)
     int push_cap;
+    // queue mode: the words after each level of an endpoint task (candidate
+    // k >= ep_k0, column holding endpoint j of its swap) go to slot
+    // 4(k - ep_k0) + j of ep_snap [.][ep_S][N] / ep_reach [.][lmax+1]
+    unsigned long long *ep_snap;
+    long long *ep_reach;
+    int ep_k0, ep_S;
 };
 
 struct SwapRegs {
@@ -343,6 +349,26 @@ __global__ void __launch_bounds__(BS) k_dense_finalize(DenseArgs a) {
     }
 }
 
+// Where a column BFS stores its per-level words and counts: the words after
+// level l (1 <= l <= levels) at snap + (l-1)*lstride, the new pairs of level l
+// at reach[l] (either pointer may be null).
+struct ColOut {
+    unsigned long long *snap;
+    size_t lstride;
+    int levels;
+    long long *reach;
+};
+
+// The evaluator's own layout: snap[(l-1)][w][v], reach_col[w][l].
+__device__ __forceinline__ ColOut col_out(const DenseArgs &a, int w) {
+    ColOut o;
+    o.snap = a.snap ? a.snap + (size_t)w * a.N : nullptr;
+    o.lstride = (size_t)a.W * a.N;
+    o.levels = a.snap_levels;
+    o.reach = a.reach_col ? a.reach_col + (size_t)w * (a.lmax + 1) : nullptr;
+    return o;
+}
+
 // ---- LDS-resident column BFS ---------------------------------------------------
 // One workgroup runs one (candidate, column) task with vis[N] in dynamic LDS.
 // Each thread owns VPT vertices, so new words stay in registers across the
@@ -350,7 +376,7 @@ __global__ void __launch_bounds__(BS) k_dense_finalize(DenseArgs a) {
 // thread 0.
 template <class Topo, int BS, int VPT, int DT>
 __device__ __forceinline__ ColPart dense_column(const Topo &topo, const DenseArgs &a, int k, int w,
-                                                unsigned long long *vis) {
+                                                unsigned long long *vis, const ColOut &out) {
     __shared__ unsigned long long s_or[2];
     __shared__ unsigned s_cnt[2];
     __shared__ int s_fn;
@@ -397,7 +423,7 @@ __device__ __forceinline__ ColPart dense_column(const Topo &topo, const DenseArg
 
     long long total = 0, reached = nvalid;
     int my_ecc = 0, last_new = 0, level = 0, flags = 0;
-    long long *reach = a.reach_col ? a.reach_col + (size_t)w * (a.lmax + 1) : nullptr;
+    long long *reach = out.reach;
     if (reach && tid == 0) reach[0] = nvalid;
     // wave-blocked ownership: wave q owns vertices [64*VPT*q, 64*VPT*(q+1)),
     // slot j of lane l is vb + 64*j (coalesced per slot, small LDS offsets)
@@ -414,8 +440,7 @@ __device__ __forceinline__ ColPart dense_column(const Topo &topo, const DenseArg
         }
     }
     auto snap_of = [&](int l) -> unsigned long long * {
-        return (a.snap && l <= a.snap_levels) ? a.snap + ((size_t)(l - 1) * a.W + w) * N
-                                              : nullptr;
+        return (out.snap && l <= out.levels) ? out.snap + (size_t)(l - 1) * out.lstride : nullptr;
     };
     // level bookkeeping shared by push and pull levels: folds this thread's
     // new-pair count and new-bit mask into the block, returns true when the
@@ -612,7 +637,7 @@ __global__ void __launch_bounds__(BS) k_dense_lds(Topo topo, DenseArgs a) {
     extern __shared__ unsigned long long vis[];
     const int t = blockIdx.x;
     const int k = t / a.W, w = t - k * a.W;
-    ColPart p = dense_column<Topo, BS, VPT, DT>(topo, a, k, w, vis);
+    ColPart p = dense_column<Topo, BS, VPT, DT>(topo, a, k, w, vis, col_out(a, w));
     if (threadIdx.x == 0) a.parts[(size_t)k * a.W + w] = p;
 }
 
@@ -627,7 +652,21 @@ __global__ void __launch_bounds__(BS) k_dense_lds_queue(Topo topo, DenseArgs a) 
         const int t = s_task;
         if (t >= *a.queue_count) break;
         const int2 kw = a.queue[t];
-        ColPart p = dense_column<Topo, BS, VPT, DT>(topo, a, kw.x, kw.y, vis);
+        ColOut o = col_out(a, kw.y);
+        if (a.ep_snap && a.swaps && kw.x >= a.ep_k0) {
+            const gg_swap &sw = a.swaps[kw.x];
+            int j = -1;
+            for (int e = 3; e >= 0; e--)
+                if (sw.v[e] >= 0 && (sw.v[e] >> 6) == kw.y) j = e;
+            if (j >= 0) {
+                const size_t slot = 4 * (size_t)(kw.x - a.ep_k0) + j;
+                o.snap = a.ep_snap + slot * a.ep_S * a.N;
+                o.lstride = (size_t)a.N;
+                o.levels = a.ep_S;
+                o.reach = a.ep_reach + slot * (a.lmax + 1);
+            }
+        }
+        ColPart p = dense_column<Topo, BS, VPT, DT>(topo, a, kw.x, kw.y, vis, o);
         if (threadIdx.x == 0) a.parts[(size_t)kw.x * a.W + kw.y] = p;
         __syncthreads();
     }

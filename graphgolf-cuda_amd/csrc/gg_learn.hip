@@ -21,9 +21,11 @@ using namespace gg;
 // ---- features --------------------------------------------------------------------------
 struct FeatArgs {
     int N, W, S, diam, d;
+    const int *diam_dev;             // diameter on the device (overrides diam)
     const unsigned long long *snap;  // [S][W][N]
+    const unsigned long long *rows;  // [RL][N][W] levels 1..RL row-major (full words stored)
+    int RL;
     const ColPart *base_parts;       // [W]
-    const int *ecc;                  // [N] base eccentricity per vertex
     const int *moves;                // [P][5]: a, b, c, d, kind
     int P;
     float *X;                        // [P][GG_FEATS]
@@ -33,6 +35,7 @@ struct FeatArgs {
 __device__ __forceinline__ unsigned long long ball_word(const FeatArgs &f, int l, int w, int x,
                                                        int ecc_w) {
     if (l <= 0) return ((x >> 6) == w) ? (1ull << (x & 63)) : 0ull;
+    if (l <= f.RL) return f.rows[((size_t)(l - 1) * f.N + x) * f.W + w];  // lanes over w: coalesced
     if (l >= ecc_w) return ~0ull;
     return f.snap[((size_t)(l - 1) * f.W + w) * f.N + x];
 }
@@ -51,6 +54,9 @@ __global__ void __launch_bounds__(256) k_features(FeatArgs f) {
     float cnt[24];
 #pragma unroll
     for (int i = 0; i < 24; i++) cnt[i] = 0.f;
+    // eccentricity of each endpoint x: the first level whose ball row R_l(x)
+    // is full in every column (per lane over its columns, then the wave max)
+    int eccl[4] = {0, 0, 0, 0};
     for (int w = lane; w < f.W; w += 64) {
         const int ew = f.base_parts[w].ecc;
         unsigned long long R[4][4];  // R[endpoint][level 0..3]
@@ -58,6 +64,15 @@ __global__ void __launch_bounds__(256) k_features(FeatArgs f) {
         for (int e = 0; e < 4; e++)
 #pragma unroll
             for (int l = 0; l < 4; l++) R[e][l] = ball_word(f, l, w, ends[e], ew);
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            int fl = 0;
+            for (int l = 0; l <= ew; l++) {
+                const unsigned long long x = l < 4 ? R[e][l] : ball_word(f, l, w, ends[e], ew);
+                if (x == ~0ull) { fl = l; break; }
+            }
+            eccl[e] = fl > eccl[e] ? fl : eccl[e];
+        }
         auto L = [&](int e, int l) { return R[e][l] & ~R[e][l - 1]; };
         // index of p1,q1,p2,q2 in ends: a=0,b=1,c=2,dd=3
         const int ip1 = 0, iq1 = kind ? 3 : 2, ip2 = 1, iq2 = kind ? 2 : 3;
@@ -91,7 +106,10 @@ __global__ void __launch_bounds__(256) k_features(FeatArgs f) {
         for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
         cnt[i] = v;
     }
+#pragma unroll
+    for (int e = 0; e < 4; e++) eccl[e] = wave_max32(eccl[e]);
     if (lane != 0) return;
+    const int diam = f.diam_dev ? (*f.diam_dev > 0 ? *f.diam_dev : 1) : f.diam;
     // distance of each added pair before the swap
     auto dist = [&](int x, int y) {
         const int w = y >> 6, ew = f.base_parts[w].ecc;
@@ -100,8 +118,11 @@ __global__ void __launch_bounds__(256) k_features(FeatArgs f) {
         return 64;
     };
     float *X = f.X + (size_t)p * GG_FEATS;
-    const float d1 = (float)f.d, d2 = d1 * d1, n = (float)f.N, inv_diam = 1.f / (float)f.diam;
+    const float d1 = (float)f.d, d2 = d1 * d1, n = (float)f.N, inv_diam = 1.f / (float)diam;
     const int qs[2] = {q1, q2}, ps[2] = {p1, p2};
+    // eccentricity by endpoint: p1 = a, p2 = b; q1, q2 are d, c or c, d
+    const int eq[2] = {kind ? eccl[3] : eccl[2], kind ? eccl[2] : eccl[3]};
+    const int ep[2] = {eccl[0], eccl[1]};
     for (int t = 0; t < 2; t++) {
         X[t * 8 + 0] = dist(ps[t], qs[t]) * inv_diam;
         X[t * 8 + 1] = cnt[t * 6 + 0] / d1;
@@ -110,15 +131,15 @@ __global__ void __launch_bounds__(256) k_features(FeatArgs f) {
         X[t * 8 + 4] = cnt[t * 6 + 3] / (2.f * d2);
         X[t * 8 + 5] = cnt[t * 6 + 4] / n;
         X[t * 8 + 6] = cnt[t * 6 + 5] / n;
-        X[t * 8 + 7] = (float)(f.ecc[ps[t]] + f.ecc[qs[t]] - 2 * f.diam) * 0.5f;
+        X[t * 8 + 7] = (float)(ep[t] + eq[t] - 2 * diam) * 0.5f;
     }
     for (int t = 0; t < 2; t++) {
         X[16 + t * 6 + 0] = cnt[12 + t * 4 + 0] / d1;
         X[16 + t * 6 + 1] = cnt[12 + t * 4 + 1] / d2;
         X[16 + t * 6 + 2] = cnt[12 + t * 4 + 2] / d2;
         X[16 + t * 6 + 3] = cnt[12 + t * 4 + 3] / d2;
-        X[16 + t * 6 + 4] = (float)(f.ecc[ends[2 * t]] - f.diam);
-        X[16 + t * 6 + 5] = (float)(f.ecc[ends[2 * t + 1]] - f.diam);
+        X[16 + t * 6 + 4] = (float)(eccl[2 * t] - diam);
+        X[16 + t * 6 + 5] = (float)(eccl[2 * t + 1] - diam);
     }
     for (int e = 0; e < 4; e++) X[28 + e] = cnt[20 + e] / d2;
 }
@@ -189,14 +210,17 @@ __global__ void __launch_bounds__(256) k_mlp_grad_part(const float *X, const flo
     __shared__ float sa[GG_GRAD_SAMPLES][GG_HIDDEN];  // hidden pre-activations
     __shared__ float sg[GG_GRAD_SAMPLES];             // dLoss/dOutput (0 for padding)
     __shared__ float se[GG_GRAD_SAMPLES];             // squared error
+    __shared__ int si[GG_GRAD_SAMPLES];               // sample rows (idx may be host memory)
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int s0 = blockIdx.x * GG_GRAD_SAMPLES;
     const int ns = min(GG_GRAD_SAMPLES, B - s0);
+    if (tid < ns) si[tid] = idx ? idx[s0 + tid] : s0 + tid;
     for (int i = tid; i < GG_NPARAMS; i += 256) sw[i] = bf(wq[i]);
+    __syncthreads();
     for (int i = tid; i < GG_GRAD_SAMPLES * GG_FEATS; i += 256) {
         const int s = i / GG_FEATS, f = i - s * GG_FEATS;
         float v = 0.f;
-        if (s < ns) v = X[(size_t)(idx ? idx[s0 + s] : s0 + s) * GG_FEATS + f];
+        if (s < ns) v = X[(size_t)si[s] * GG_FEATS + f];
         sx[s][f] = v;
     }
     __syncthreads();
@@ -225,7 +249,7 @@ __global__ void __launch_bounds__(256) k_mlp_grad_part(const float *X, const flo
         for (int o = 32; o > 0; o >>= 1) p += __shfl_xor(p, o, 64);
         if (lane == 0) {
             float e = 0.f;
-            if (s < ns) e = (p + sw[OFF_B2]) - Y[idx ? idx[s0 + s] : s0 + s];
+            if (s < ns) e = (p + sw[OFF_B2]) - Y[si[s]];
             sg[s] = 2.f * e / (float)B;
             se[s] = e * e;
         }
@@ -472,15 +496,18 @@ extern "C" int gg_mlp_train(gg_mlp *m, const float *X, const float *Y, int B, fl
 }
 
 // Features of P moves (a, b, c, d, kind) on the batch evaluator's base.
-int gg::features_dev(gg_batch *b, const int *d_moves, int P, float *d_x, hipStream_t st) {
+int gg::features_dev(gg_batch *b, const int *d_moves, int P, float *d_x, hipStream_t st,
+                     const int *diam_dev) {
     if (!b->sparse_ok) return GG_UNSUPPORTED;
     FeatArgs f{};
+    f.diam_dev = diam_dev;
     f.N = b->N; f.W = b->W; f.S = b->snap_cap;
     f.diam = b->base_res.diameter > 0 ? b->base_res.diameter : 1;
     f.d = b->degree;
     f.snap = b->d_snap;
+    f.rows = b->d_rows;
+    f.RL = b->d_rows ? b->rows_levels : 0;
     f.base_parts = b->d_parts + (size_t)b->max_k * b->W;
-    f.ecc = b->d_ecc;
     f.moves = d_moves;
     f.P = P;
     f.X = d_x;

@@ -5,6 +5,7 @@
 #include <hip/hip_bf16.h>
 #include "gg_rng.hpp"
 #include "graphgolf_learn.h"
+#include "gg_common.hpp"
 
 #define GG_FEATS 32
 #define GG_HIDDEN 128
@@ -30,8 +31,25 @@ int mlp_forward_dev(gg_mlp *m, const float *d_x, int P, float *d_out, hipStream_
 int mlp_grad_dev(gg_mlp *m, const float *d_x, const float *d_y, const int *d_idx, int B,
                  hipStream_t st);
 int mlp_adam_dev(gg_mlp *m, float lr, hipStream_t st);
-int features_dev(gg_batch *b, const int *d_moves, int P, float *d_x, hipStream_t st);
+// diam_dev: the base diameter read on the device (null: the host's value)
+int features_dev(gg_batch *b, const int *d_moves, int P, float *d_x, hipStream_t st,
+                 const int *diam_dev = nullptr);
+int batch_enqueue_dev(gg_batch *b, int K);
+struct BatchOut {
+    const ResultRec *d_res;
+    ResultRec *h_res;      // pinned
+    gg_swap *h_swaps;      // pinned
+    int *qctl;
+    int nqctl;
+};
+BatchOut batch_out(gg_batch *b);
+int batch_finish_dev(gg_batch *b, int K, gg_result *out);
+int batch_push_state(gg_batch *b);
+const int *b_adjR(gg_batch *b);
+int batch_shape(gg_batch *b, int *N, int *d);
+gg_swap *b_swaps(gg_batch *b);
+const gg_swap *b_hswaps(gg_batch *b);
 int gather_rows_dev(const float *src, const int *idx, int n, float *dst, int dst_cap, int head,
-                    hipStream_t st);
+                    hipStream_t st, const float *ysrc = nullptr, float *ydst = nullptr);
 hipStream_t batch_stream(gg_batch *b);
 }  // namespace gg

@@ -17,21 +17,219 @@
 #include "gg_batch.hpp"
 #include "gg_learn.hpp"
 #include "gg_search_impl.hpp"
+#include "gg_devrng.hpp"
 
 namespace gg {
 
+// ---- proposals and selection on the GPU ------------------------------------------------
+// Proposal p of step t draws from stream (seed, t, p) until it finds a simple
+// 2-opt swap, with the host generator's attempt (propose_swap): a, ia, c, ic,
+// kind, edges (a, adj[a][ia]) and (c, adj[c][ic]) rewired to (a,d),(b,c)
+// (kind 1) or (a,c),(b,d) (kind 0), redrawn unless the four vertices are
+// distinct and neither new edge exists.
+#define GG_PROPOSE_MAX_ATTEMPTS 4096
+#define GG_SELECT_MAX_P 4096
+#define GG_EXPLORE_LANE 0xFFFFFFFFull
+
+// Row of v (d neighbours) in registers: 16-byte loads issued together when
+// d is a multiple of 4 (one memory latency), else one by one.
+template <int DM>
+struct PRow {
+    int u[DM];
+    __device__ __forceinline__ void load(const int *adjR, int v, int d) {
+        const int *r = adjR + (size_t)v * d;
+        // every index is a compile-time constant (predicated, no dynamic
+        // indexing), so the row stays in registers
+        if ((d & 3) == 0) {
+            const int4 *r4 = reinterpret_cast<const int4 *>(r);
+#pragma unroll
+            for (int q = 0; q < DM / 4; q++) {
+                int4 x = make_int4(-1, -1, -1, -1);
+                if (4 * q < d) x = r4[q];
+                u[4 * q] = x.x; u[4 * q + 1] = x.y; u[4 * q + 2] = x.z; u[4 * q + 3] = x.w;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < DM; i++) u[i] = i < d ? r[i] : -1;
+        }
+    }
+    __device__ __forceinline__ int slot_of(int d, int x) const {
+        int s = -1;
+#pragma unroll
+        for (int i = DM - 1; i >= 0; i--)
+            if (i < d && u[i] == x) s = i;
+        return s;
+    }
+};
+
+// Scalar fallback for degrees above the register row.
+__device__ __forceinline__ int row_slot_of(const int *row, int d, int x) {
+    for (int i = 0; i < d; i++)
+        if (row[i] == x) return i;
+    return -1;
+}
+
+#define GG_PROPOSE_ROW 32
+// ctl (pinned host memory): [0] step, [1] warm, [2] diameter; workgroup 0
+// copies them to dctl (read by the later kernels of the step) and clears the
+// error word dctl[3].
+__global__ void __launch_bounds__(64) k_propose(const int *adjR, int N, int d, uint64_t seed,
+                                                 const int *ctl, int P, int *moves, gg_swap *out,
+                                                 int *dctl) {
+    __shared__ int s_step;
+    if (threadIdx.x == 0) {
+        s_step = ctl[0];
+        if (blockIdx.x == 0) { dctl[0] = ctl[0]; dctl[1] = ctl[1]; dctl[2] = ctl[2]; }
+    }
+    __syncthreads();
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    uint64_t st = sm64_stream(seed, (uint64_t)s_step, (uint64_t)p);
+    const bool reg = d <= GG_PROPOSE_ROW;
+    for (int att = 0; att < GG_PROPOSE_MAX_ATTEMPTS; att++) {
+        const int a = (int)sm64_below(st, N), ia = (int)sm64_below(st, d);
+        const int c = (int)sm64_below(st, N), ic = (int)sm64_below(st, d);
+        const int kind = (int)sm64_below(st, 2);
+        const int b = adjR[(size_t)a * d + ia], dd = adjR[(size_t)c * d + ic];
+        if (a == b || a == c || a == dd || b == c || b == dd || c == dd) continue;
+        const int xa = kind ? dd : c, xb = kind ? c : dd, xc = kind ? b : a, xd = kind ? a : b;
+        int ib, id;
+        if (reg) {
+            PRow<GG_PROPOSE_ROW> ra, rb, rd;
+            ra.load(adjR, a, d);
+            rb.load(adjR, b, d);
+            rd.load(adjR, dd, d);
+            if (ra.slot_of(d, xa) >= 0 || rb.slot_of(d, xb) >= 0) continue;
+            ib = rb.slot_of(d, a);
+            id = rd.slot_of(d, c);
+        } else {
+            const int *ra = adjR + (size_t)a * d, *rb = adjR + (size_t)b * d;
+            if (row_slot_of(ra, d, xa) >= 0 || row_slot_of(rb, d, xb) >= 0) continue;
+            ib = row_slot_of(rb, d, a);
+            id = row_slot_of(adjR + (size_t)dd * d, d, c);
+        }
+        if (ib < 0 || id < 0) continue;
+        gg_swap o;
+        o.v[0] = a;  o.slot[0] = ia; o.nbr[0] = xa;
+        o.v[1] = b;  o.slot[1] = ib; o.nbr[1] = xb;
+        o.v[2] = c;  o.slot[2] = ic; o.nbr[2] = xc;
+        o.v[3] = dd; o.slot[3] = id; o.nbr[3] = xd;
+        out[p] = o;
+        int *m = moves + (size_t)p * 5;
+        m[0] = a; m[1] = b; m[2] = c; m[3] = dd; m[4] = kind;
+        return;
+    }
+    atomicOr(dctl + 3, 1);  // no simple swap found: the host reports it
+    gg_swap none;
+    for (int j = 0; j < 4; j++) { none.v[j] = -1; none.slot[j] = 0; none.nbr[j] = 0; }
+    out[p] = none;
+    int *m = moves + (size_t)p * 5;
+    m[0] = m[1] = m[2] = m[3] = 0; m[4] = 0;
+}
+
+// Rank of every score among the P (NaN as +inf, ties by index).  A workgroup
+// of 256 threads ranks 64 proposals: all P scores staged in LDS, each wave
+// compares its proposals with a quarter of them, the quarters summed in LDS.
+__global__ void __launch_bounds__(256) k_rank(const float *score, int P, int *rank) {
+    __shared__ float s_sc[GG_SELECT_MAX_P];
+    __shared__ int s_part[4][64];
+    for (int q = threadIdx.x; q < P; q += 256) {
+        const float x = score[q];
+        s_sc[q] = x == x ? x : INFINITY;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, quarter = threadIdx.x >> 6;
+    const int p = blockIdx.x * 64 + lane;
+    int r = 0;
+    if (p < P) {
+        const float x = s_sc[p];
+        const int q0 = (P * quarter) >> 2, q1 = (P * (quarter + 1)) >> 2;
+#pragma unroll 8
+        for (int q = q0; q < q1; q++) {
+            const float y = s_sc[q];
+            r += (y < x || (y == x && q < p)) ? 1 : 0;
+        }
+    }
+    s_part[quarter][lane] = r;
+    __syncthreads();
+    if (quarter == 0 && p < P)
+        rank[p] = s_part[0][lane] + s_part[1][lane] + s_part[2][lane] + s_part[3][lane];
+}
+
+// One workgroup: the K candidates of the step.  While warm (ctl[1] != 0) the
+// first K proposals (uniform draws); otherwise the G = K - K/4 best ranked
+// (lowest predicted change) in rank order, then K/4 distinct uniform picks
+// among the rest from stream (seed, t, GG_EXPLORE_LANE).  Writes the chosen
+// swaps, their move rows and proposal indices.
+__global__ void __launch_bounds__(1024) k_select(const int *rank, int P, int K, uint64_t seed,
+                                                 const int *ctl, const gg_swap *psw,
+                                                 const int *pmoves, gg_swap *out_sw,
+                                                 int *out_moves, int *out_sel) {
+    __shared__ unsigned char s_used[GG_SELECT_MAX_P];
+    __shared__ int s_sel[GG_SELECT_MAX_P];
+    const int tid = threadIdx.x;
+    const bool warm = ctl[1] != 0;
+    if (warm) {
+        for (int k = tid; k < K; k += 1024) s_sel[k] = k;
+    } else {
+        const int G = K - K / 4;
+        for (int p = tid; p < P; p += 1024) {
+            const int r = rank[p];
+            const bool g = r < G;
+            s_used[p] = g ? 1 : 0;
+            if (g) s_sel[r] = p;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            uint64_t st = sm64_stream(seed, (uint64_t)ctl[0], GG_EXPLORE_LANE);
+            for (int n = G; n < K;) {
+                const int x = (int)sm64_below(st, P);
+                if (!s_used[x]) { s_used[x] = 1; s_sel[n++] = x; }
+            }
+        }
+    }
+    __syncthreads();
+    for (int k = tid; k < K; k += 1024) {
+        const int p = s_sel[k];
+        out_sw[k] = psw[p];
+        for (int j = 0; j < 5; j++) out_moves[k * 5 + j] = pmoves[(size_t)p * 5 + j];
+        out_sel[k] = p;
+    }
+}
+
+// End of a captured step: the K results, swaps and move rows and the control
+// words go straight to pinned host memory (no copy nodes), and the batch's
+// queue counters are reset for the next step.
+__global__ void __launch_bounds__(256) k_pack_out(int K, const ResultRec *res, ResultRec *h_res,
+                                                  const gg_swap *sw, gg_swap *h_sw,
+                                                  const int *msel, int *h_msel, const int *dctl,
+                                                  int *h_ctl, int *qctl, int nqctl) {
+    const int t = threadIdx.x;
+    for (int k = t; k < K; k += 256) {
+        h_res[k] = res[k];
+        h_sw[k] = sw[k];
+    }
+    for (int i = t; i < 5 * K; i += 256) h_msel[i] = msel[i];
+    if (t < 4) h_ctl[t] = dctl[t];
+    for (int i = t; i < nqctl; i += 256) qctl[i] = 0;
+}
+
+// Rows idx[r] of src into ring slots head + r of dst; with ysrc, the targets
+// ysrc[(head + r) % cap] (pinned host memory) into ydst at the same slots.
 __global__ void k_gather_rows(const float *src, const int *idx, int n, float *dst, int dst_cap,
-                              int head) {
+                              int head, const float *ysrc, float *ydst) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n * GG_FEATS) return;
     const int r = i / GG_FEATS, c = i - r * GG_FEATS;
-    dst[(size_t)((head + r) % dst_cap) * GG_FEATS + c] = src[(size_t)idx[r] * GG_FEATS + c];
+    const int slot = (head + r) % dst_cap;
+    dst[(size_t)slot * GG_FEATS + c] = src[(size_t)idx[r] * GG_FEATS + c];
+    if (ysrc && c == 0) ydst[slot] = ysrc[slot];
 }
 
 int gather_rows_dev(const float *src, const int *idx, int n, float *dst, int dst_cap, int head,
-                    hipStream_t st) {
+                    hipStream_t st, const float *ysrc, float *ydst) {
     hipLaunchKernelGGL(k_gather_rows, dim3((n * GG_FEATS + 255) / 256), dim3(256), 0, st, src, idx,
-                       n, dst, dst_cap, head);
+                       n, dst, dst_cap, head, ysrc, ydst);
     GG_HIP(hipGetLastError());
     return GG_OK;
 }
@@ -61,6 +259,17 @@ struct LearnState {
     std::vector<int> last_sel;                   // original indices of the chosen K
     std::vector<double> raw;                     // raw deltas of the replay buffer (host mirror)
     float *h_ynorm = nullptr;                    // normalised targets (pinned host mirror)
+    // device-drawn proposals: the whole step (draw, features, scores,
+    // selection, incremental evaluation) replays as one captured graph
+    bool dev = false;                            // this search draws on the GPU
+    bool sel_on_dev = false;                     // d_sel holds the last step's choice
+    gg_swap *d_psw = nullptr;                    // [P] proposals
+    int *d_rank = nullptr;                       // [P] score ranks
+    int *d_msel = nullptr, *h_msel = nullptr;    // [K][5] chosen move rows
+    int *d_ctl = nullptr, *h_ctl = nullptr;      // [4] step, warm, diameter, error; h_ctl[4..8) read back
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int g_push = -1;
 };
 
 static int alloc_learn(LearnState *L) {
@@ -82,6 +291,14 @@ static int alloc_learn(LearnState *L) {
         return GG_OOM;
     if (hipHostMalloc((void **)&L->h_ynorm, 4 * (size_t)L->cap, hipHostMallocDefault) != hipSuccess)
         return GG_OOM;
+    if (hipMalloc((void **)&L->d_psw, sizeof(gg_swap) * (size_t)P) != hipSuccess) return GG_OOM;
+    if (hipMalloc((void **)&L->d_rank, 4 * (size_t)P) != hipSuccess) return GG_OOM;
+    if (hipMalloc((void **)&L->d_msel, 4 * (size_t)K * 5) != hipSuccess) return GG_OOM;
+    if (hipHostMalloc((void **)&L->h_msel, 4 * (size_t)K * 5, hipHostMallocDefault) != hipSuccess)
+        return GG_OOM;
+    if (hipMalloc((void **)&L->d_ctl, 4 * 4) != hipSuccess) return GG_OOM;
+    if (hipHostMalloc((void **)&L->h_ctl, 4 * 8, hipHostMallocDefault) != hipSuccess)
+        return GG_OOM;
     return GG_OK;
 }
 
@@ -96,6 +313,7 @@ int learn_create(gg_search *s, int device) {
     L->K = s->cfg.K;
     L->cap = 1 << 17;  // 131072 pairs
     L->rng.reset(new Rng(s->cfg.seed ^ 0x9E3779B97F4A7C15ull));
+    L->dev = L->P <= GG_SELECT_MAX_P;
     int st = gg_mlp_create(device, s->cfg.seed + 1, &L->mlp);
     if (st != GG_OK) return st;
     return alloc_learn(L);
@@ -105,11 +323,14 @@ void learn_destroy(gg_search *s) {
     LearnState *L = s->learn;
     if (!L) return;
     if (L->st) (void)hipStreamSynchronize(L->st);
+    if (L->gexec) (void)hipGraphExecDestroy(L->gexec);
+    if (L->graph) (void)hipGraphDestroy(L->graph);
     gg_mlp_destroy(L->mlp);
-    void *dev[] = {L->d_moves, L->d_x, L->d_score, L->d_sel, L->d_bx, L->d_by, L->d_idx};
+    void *dev[] = {L->d_moves, L->d_x, L->d_score, L->d_sel, L->d_bx, L->d_by, L->d_idx,
+                   L->d_psw, L->d_msel, L->d_ctl, L->d_rank};
     for (void *p : dev)
         if (p) (void)hipFree(p);
-    void *host[] = {L->h_moves, L->h_score, L->h_sel, L->h_idx, L->h_ynorm};
+    void *host[] = {L->h_moves, L->h_score, L->h_sel, L->h_idx, L->h_ynorm, L->h_msel, L->h_ctl};
     for (void *p : host)
         if (p) (void)hipHostFree(p);
     delete L;
@@ -168,6 +389,69 @@ int learn_select(gg_search *s, int P, int K) {
     return GG_OK;
 }
 
+bool learn_on_device(const gg_search *s) { return s->learn && s->learn->dev; }
+
+// One learned step's candidates, drawn, scored, selected and evaluated on the
+// GPU (one captured graph, one wait): fills s->sw, s->mv and s->res for the K
+// chosen proposals.
+int learn_eval_device(gg_search *s) {
+    LearnState *L = s->learn;
+    gg_batch *b = L->batch;
+    const int P = L->P, K = L->K;
+    L->h_ctl[0] = (int)s->step;
+    L->h_ctl[1] = L->count < s->cfg.warmup_pairs ? 1 : 0;
+    L->h_ctl[2] = s->cur.diameter > 0 ? s->cur.diameter : 1;
+    L->h_ctl[3] = 0;
+    const int push = batch_push_state(b);
+    if (!L->gexec || L->g_push != push) {
+        if (L->gexec) { (void)hipGraphExecDestroy(L->gexec); L->gexec = nullptr; }
+        if (L->graph) { (void)hipGraphDestroy(L->graph); L->graph = nullptr; }
+        GG_HIP(hipStreamBeginCapture(L->st, hipStreamCaptureModeThreadLocal));
+        int st = GG_OK;
+        auto fail = [&](int code) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(L->st, &g);
+            if (g) (void)hipGraphDestroy(g);
+            return code;
+        };
+        // the control words are read from pinned host memory by k_propose
+        hipLaunchKernelGGL(k_propose, dim3((P + 63) / 64), dim3(64), 0, L->st, b_adjR(b),
+                           s->N, s->d, s->cfg.seed, L->h_ctl, P, L->d_moves, L->d_psw, L->d_ctl);
+        if (hipGetLastError() != hipSuccess) return fail(GG_HIP_ERROR);
+        if ((st = features_dev(b, L->d_moves, P, L->d_x, L->st, L->d_ctl + 2)) != GG_OK)
+            return fail(st);
+        if ((st = mlp_forward_dev(L->mlp, L->d_x, P, L->d_score, L->st)) != GG_OK) return fail(st);
+        hipLaunchKernelGGL(k_rank, dim3((P + 63) / 64), dim3(256), 0, L->st, L->d_score, P,
+                           L->d_rank);
+        hipLaunchKernelGGL(k_select, dim3(1), dim3(1024), 0, L->st, L->d_rank, P, K,
+                           s->cfg.seed, L->d_ctl, L->d_psw, L->d_moves, b_swaps(b), L->d_msel,
+                           L->d_sel);
+        if (hipGetLastError() != hipSuccess) return fail(GG_HIP_ERROR);
+        if ((st = batch_enqueue_dev(b, K)) != GG_OK) return fail(st);
+        BatchOut bo = batch_out(b);
+        hipLaunchKernelGGL(k_pack_out, dim3(1), dim3(256), 0, L->st, K, bo.d_res, bo.h_res,
+                           b_swaps(b), bo.h_swaps, L->d_msel, L->h_msel, L->d_ctl, L->h_ctl + 4,
+                           bo.qctl, bo.nqctl);
+        if (hipGetLastError() != hipSuccess) return fail(GG_HIP_ERROR);
+        hipGraph_t g = nullptr;
+        GG_HIP(hipStreamEndCapture(L->st, &g));
+        L->graph = g;
+        GG_HIP(hipGraphInstantiate(&L->gexec, g, nullptr, nullptr, 0));
+        L->g_push = push;
+    }
+    GG_HIP(hipGraphLaunch(L->gexec, L->st));
+    GG_HIP(hipStreamSynchronize(L->st));
+    if (L->h_ctl[4 + 3]) return GG_BAD_ARG;  // some proposal found no simple swap
+    const gg_swap *hs = b_hswaps(b);
+    for (int k = 0; k < K; k++) {
+        s->sw[k] = hs[k];
+        const int *m = L->h_msel + (size_t)k * 5;
+        s->mv[k] = MoveRec{m[0], m[1], m[2], m[3], m[4]};
+    }
+    L->sel_on_dev = true;
+    return batch_finish_dev(b, K, s->res.data());
+}
+
 static double raw_delta(const gg_search *s, const gg_result &r) {
     if (r.status != GG_OK) return 1e300;
     return (double)(r.total_scaled - s->cur.total_scaled) +
@@ -179,10 +463,11 @@ static int train_steps(gg_search *s, int steps) {
     const int B = std::min(std::max(s->cfg.train_batch, 1), GG_MAX_BATCH);
     if (L->count < B) return GG_OK;
     for (int t = 0; t < steps; t++) {
-        if (t > 0) GG_HIP(hipStreamSynchronize(L->st));  // h_idx is reused
+        // h_idx (pinned) is read by the gradient kernel itself; the step's
+        // wait for its candidates orders the next rewrite after that read
+        if (t > 0) GG_HIP(hipStreamSynchronize(L->st));
         for (int i = 0; i < B; i++) L->h_idx[i] = (int)L->rng->below(L->count);
-        GG_HIP(hipMemcpyAsync(L->d_idx, L->h_idx, 4 * B, hipMemcpyHostToDevice, L->st));
-        int st = mlp_grad_dev(L->mlp, L->d_bx, L->d_by, L->d_idx, B, L->st);
+        int st = mlp_grad_dev(L->mlp, L->d_bx, L->d_by, L->h_idx, B, L->st);
         if (st != GG_OK) return st;
         st = mlp_adam_dev(L->mlp, (float)(s->cfg.lr > 0 ? s->cfg.lr : 1e-3), L->st);
         if (st != GG_OK) return st;
@@ -192,13 +477,18 @@ static int train_steps(gg_search *s, int steps) {
     return GG_OK;
 }
 
-// Uploads normalised targets for ring positions [pos, pos + n) (n <= cap).
-static int upload_targets(LearnState *L, int pos, int n) {
-    int first = std::min(n, L->cap - pos);
+// Normalised targets of ring positions [pos, pos + n) into the pinned mirror.
+static void normalise_targets(LearnState *L, int pos, int n) {
     for (int i = 0; i < n; i++) {
         double y = (L->raw[(pos + i) % L->cap] - L->mu) / L->sigma;
         L->h_ynorm[(pos + i) % L->cap] = (float)std::max(-8.0, std::min(8.0, y));
     }
+}
+
+// Uploads normalised targets for ring positions [pos, pos + n) (n <= cap).
+static int upload_targets(LearnState *L, int pos, int n) {
+    int first = std::min(n, L->cap - pos);
+    normalise_targets(L, pos, n);
     GG_HIP(hipMemcpyAsync(L->d_by + pos, L->h_ynorm + pos, 4 * (size_t)first,
                           hipMemcpyHostToDevice, L->st));
     if (n > first)
@@ -214,15 +504,22 @@ int learn_observe(gg_search *s, int K) {
     for (int k = 0; k < K; k++) {
         double dlt = raw_delta(s, s->res[k]);
         L->raw[(pos + k) % L->cap] = dlt < 1e299 ? dlt : 1e30;
-        L->h_sel[k] = L->last_sel[k];
+        if (!L->sel_on_dev) L->h_sel[k] = L->last_sel[k];
     }
-    GG_HIP(hipMemcpyAsync(L->d_sel, L->h_sel, 4 * (size_t)K, hipMemcpyHostToDevice, L->st));
-    int st = gather_rows_dev(L->d_x, L->d_sel, K, L->d_bx, L->cap, pos, L->st);
+    if (!L->sel_on_dev)
+        GG_HIP(hipMemcpyAsync(L->d_sel, L->h_sel, 4 * (size_t)K, hipMemcpyHostToDevice, L->st));
+    L->sel_on_dev = false;
+    // once the normalisation is frozen the K new targets travel with the
+    // gathered feature rows (read from pinned memory by the gather kernel)
+    const bool frozen = L->norm_frozen;
+    if (frozen) normalise_targets(L, pos, K);
+    int st = gather_rows_dev(L->d_x, L->d_sel, K, L->d_bx, L->cap, pos, L->st,
+                             frozen ? L->h_ynorm : nullptr, frozen ? L->d_by : nullptr);
     if (st != GG_OK) return st;
     L->head = (L->head + K) % L->cap;
     L->count = std::min(L->count + K, L->cap);
     s->stats.train_pairs += K;
-    if (!L->norm_frozen) {
+    if (!frozen) {
         const int need = std::max(64, std::min(s->cfg.warmup_pairs, 4096));
         if (L->count < need) return GG_OK;
         // mean / std of the finite raw deltas seen so far, then normalise all
@@ -238,10 +535,8 @@ int learn_observe(gg_search *s, int K) {
         L->sigma = v > 0 ? std::sqrt(v) : 1.0;
         L->norm_frozen = true;
         st = upload_targets(L, 0, L->count);
-    } else {
-        st = upload_targets(L, pos, K);
+        if (st != GG_OK) return st;
     }
-    if (st != GG_OK) return st;
     const int every = std::max(1, s->cfg.train_every);
     if (s->step % every == 0) {
         st = train_steps(s, 1);
@@ -298,6 +593,41 @@ static double pearson(const std::vector<double> &a, const std::vector<double> &b
 }  // namespace gg
 
 using namespace gg;
+
+extern "C" int gg_batch_propose(gg_batch *b, uint64_t seed, int64_t step, int P,
+                                int32_t *moves_out) {
+    if (!b || !moves_out || P < 1 || step < 0 || step > INT32_MAX) return GG_BAD_ARG;
+    int N = 0, d = 0;
+    if (batch_shape(b, &N, &d) != GG_OK) return GG_BAD_ARG;
+    hipStream_t st = batch_stream(b);
+    // d_c[0..4): control words in (step, warm, diameter, -); d_c[4..8): the
+    // kernel's copy, [7] its error word
+    int *d_m = nullptr, *d_c = nullptr;
+    gg_swap *d_s = nullptr;
+    int h_c[8] = {(int)step, 0, 0, 0, 0, 0, 0, 0};
+    int rc = GG_OK;
+    if (hipMalloc((void **)&d_m, 4 * (size_t)P * 5) != hipSuccess ||
+        hipMalloc((void **)&d_s, sizeof(gg_swap) * (size_t)P) != hipSuccess ||
+        hipMalloc((void **)&d_c, sizeof(h_c)) != hipSuccess)
+        rc = GG_OOM;
+    if (rc == GG_OK && hipMemcpyAsync(d_c, h_c, sizeof(h_c), hipMemcpyHostToDevice, st) != hipSuccess)
+        rc = GG_HIP_ERROR;
+    if (rc == GG_OK) {
+        hipLaunchKernelGGL(k_propose, dim3((P + 63) / 64), dim3(64), 0, st, b_adjR(b), N, d,
+                           seed, d_c, P, d_m, d_s, d_c + 4);
+        if (hipGetLastError() != hipSuccess) rc = GG_HIP_ERROR;
+    }
+    if (rc == GG_OK &&
+        (hipMemcpyAsync(moves_out, d_m, 4 * (size_t)P * 5, hipMemcpyDeviceToHost, st) != hipSuccess ||
+         hipMemcpyAsync(h_c, d_c, sizeof(h_c), hipMemcpyDeviceToHost, st) != hipSuccess))
+        rc = GG_HIP_ERROR;
+    if (hipStreamSynchronize(st) != hipSuccess && rc == GG_OK) rc = GG_HIP_ERROR;
+    if (rc == GG_OK && h_c[7]) rc = GG_BAD_ARG;
+    if (d_m) (void)hipFree(d_m);
+    if (d_s) (void)hipFree(d_s);
+    if (d_c) (void)hipFree(d_c);
+    return rc;
+}
 
 // Audit: draw P fresh proposals (own RNG, the search state is unchanged),
 // score them, evaluate all P exactly, and report the scorer's Spearman rank

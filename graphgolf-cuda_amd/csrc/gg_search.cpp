@@ -104,6 +104,7 @@ struct BatchBackend : Backend {
         last_sw.clear();
         return gg_batch_commit(b, sw, out);
     }
+    void forget() override { last_sw.clear(); }
 };
 
 struct CallbackBackend : Backend {
@@ -349,16 +350,25 @@ static int search_step(gg_search *s) {
     int K = cfg.mode == GG_MODE_PLAIN ? 1 : cfg.K;
     int st;
     int nprop = cfg.mode == GG_MODE_LEARNED ? cfg.P : K;
-    if (!propose_swaps(s->adj, N, d, rng, nprop, s->sw.data(), s->mv.data(),
-                       1000000 + 4 * (int64_t)nprop))
-        return GG_BAD_ARG;  // no simple swap exists
-    s->stats.proposals += nprop;
-    if (cfg.mode == GG_MODE_LEARNED) {
-        st = learn_select(s, nprop, K);  // reorders sw/mv so the first K are the chosen ones
+    if (cfg.mode == GG_MODE_LEARNED && learn_on_device(s)) {
+        // drawn, scored, selected and evaluated on the GPU in one graph; a
+        // commit finds the candidates through the batch evaluator itself
+        s->be->forget();
+        st = learn_eval_device(s);
+        if (st != GG_OK) return st;
+        s->stats.proposals += nprop;
+    } else {
+        if (!propose_swaps(s->adj, N, d, rng, nprop, s->sw.data(), s->mv.data(),
+                           1000000 + 4 * (int64_t)nprop))
+            return GG_BAD_ARG;  // no simple swap exists
+        s->stats.proposals += nprop;
+        if (cfg.mode == GG_MODE_LEARNED) {
+            st = learn_select(s, nprop, K);  // reorders sw/mv so the first K are the chosen ones
+            if (st != GG_OK) return st;
+        }
+        st = s->be->eval(s->sw.data(), K, s->res.data());
         if (st != GG_OK) return st;
     }
-    st = s->be->eval(s->sw.data(), K, s->res.data());
-    if (st != GG_OK) return st;
     s->stats.exact_evals += K;
     if (cfg.mode == GG_MODE_LEARNED) {
         st = learn_observe(s, K);

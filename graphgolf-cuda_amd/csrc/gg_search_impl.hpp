@@ -21,6 +21,8 @@ struct Backend {
     virtual int set_base(const int32_t *adj, gg_result *out) = 0;
     virtual int eval(const gg_swap *sw, int K, gg_result *out) = 0;
     virtual int commit(const gg_swap *sw, gg_result *out) = 0;
+    // candidates evaluated outside eval() since: drop what eval() remembered
+    virtual void forget() {}
 };
 
 struct MoveRec {
@@ -82,6 +84,8 @@ namespace gg {
 int learn_create(gg_search *s, int device);
 void learn_destroy(gg_search *s);
 int learn_select(gg_search *s, int P, int K);
+bool learn_on_device(const gg_search *s);
+int learn_eval_device(gg_search *s);
 int learn_observe(gg_search *s, int K);
 void learn_fill_stats(gg_search *s, gg_search_stats *st);
 }  // namespace gg

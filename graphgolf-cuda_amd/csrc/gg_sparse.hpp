@@ -22,7 +22,12 @@
 // (written only at Delta positions), so the D neighbour lookups of an item
 // are independent loads.  Tasks whose Delta outgrows the list capacity, and
 // the columns that contain a swap endpoint as a source, are queued for the
-// dense kernel (k_dense_lds_queue).
+// dense kernel (k_dense_lds_queue) or run densely in the kernel's own tail.
+//
+// Every Delta_l (l <= S) is also logged as (vertex, V_l word) pairs, and the
+// tail keeps the endpoint columns' words after each level: committing a
+// candidate then rewrites the stored base levels in place (B_l := V_l on
+// Delta_l, endpoint columns copied whole) instead of running a new BFS.
 #pragma once
 #include "gg_dense.hpp"
 
@@ -50,12 +55,22 @@ struct SparseArgs {
     int *task_state;                 // [k0+K][W]
     int *task_end;                   // [k0+K][W] level of the fixed point
     int *task_flags;                 // [k0+K][W]
+    int *task_ecc;                   // [k0+K][W] the candidate's column eccentricity (finalize)
     int2 *queue;                     // dense fallback tasks (global k, w)
     int *queue_count;
     int *queue_next;
     int *ep_next;                    // endpoint tasks claimed (kernels with a dense tail)
     ColPart *dense_parts;            // [k0+K][W] written by the dense queue kernel
     ResultRec *res;                  // [k0+K]
+    // What committing one of the launch's candidates needs without a new BFS
+    // (all null: not kept).  Per task and level l <= S the Delta_l entries,
+    // and per endpoint task 4k+j of the dense tail the column's words after
+    // each level and its per-level new pairs.
+    int *lg_key;                     // [K*W][S][cap] Delta_l vertices
+    unsigned long long *lg_word;     // [K*W][S][cap] their candidate words V_l
+    int *lg_n;                       // [K*W][S] |Delta_l|
+    unsigned long long *ep_snap;     // [4K][S][N]
+    long long *ep_reach;             // [4K][lmax+1]
 };
 
 // Row of candidate vertex v with the swap's replacements applied: DT > 0
@@ -128,6 +143,7 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a, TopoRingQ topo, Den
     __shared__ int s_over[GG_SPARSE_MAX_K];
     __shared__ int s_diff[GG_SPARSE_MAX_K];
     __shared__ unsigned long long s_dc[GG_SPARSE_MAX_K];
+    __shared__ unsigned long long s_lost[GG_SPARSE_MAX_K];  // bits lost on Delta_{l-1}
     __shared__ int s_ended[GG_SPARSE_MAX_K];      // task ended last level: clear its bitmaps
     __shared__ int s_next;
     __shared__ int s_any;
@@ -205,6 +221,7 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a, TopoRingQ topo, Den
             s_over[k] = 0;
             s_diff[k] = 0;
             s_dc[k] = 0;
+            s_lost[k] = 0;
         }
         if (tid == 0) { s_next = 0; s_any = 0; }
         clear_ended();
@@ -256,7 +273,18 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a, TopoRingQ topo, Den
                 const int k = owner(s_koff, i);
                 const int x = a.dl_key[(lt(k) * 2 + par) * a.cap + (i - s_koff[k])];
                 atomicOr(&s_ubits[x >> 5], 1u << (x & 31));
-                if (stg) s_keys[i] = x;
+                int xs = x;
+                if (l >= ecc_w) {
+                    // B_l is full here: a bit can be missing from V_l[v] (v not a
+                    // swap endpoint) only if every closed neighbour of v that had
+                    // it in B_{l-1} is in Delta_{l-1} and lost it.  Keys that
+                    // lost nothing spawn no items; the lost bits are collected.
+                    const unsigned long long lost =
+                        Bprev[x] & ~a.dl_ovr[(lt(k) * 2 + par) * (size_t)N + x];
+                    if (lost) atomicOr(&s_lost[k], lost);
+                    else xs = -1;
+                }
+                if (stg) s_keys[i] = xs;
             }
         }
         __threadfence_block();
@@ -286,6 +314,7 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a, TopoRingQ topo, Den
             } else {
                 const int e = (loc - 4) / (D + 1), i = (loc - 4) - e * (D + 1);
                 const int u = staged ? s_keys[s_koff[k] + e] : a.dl_key[(tl * 2 + par) * a.cap + e];
+                if (u < 0) continue;  // lost no bit (levels where B_l is full)
                 v = i == 0 ? u : cand_nbr(a.adjR, D, u, i - 1, sw);
             }
             unsigned long long basew = ~0ull;  // issued before the gather
@@ -293,23 +322,31 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a, TopoRingQ topo, Den
             unsigned long long acc;
             const bool none = s_nprev[k] == 0;
             if constexpr (DT > 0) {
+                unsigned long long need = ~0ull;
+                if (l >= ecc_w && !none) {
+                    // B_l is full here, so the item only has to show V_l[v]
+                    // full.  Off the swap's endpoints only the task's lost bits
+                    // can be missing (none lost: nothing to show).
+                    const bool inE = v == sw.v[0] || v == sw.v[1] || v == sw.v[2] || v == sw.v[3];
+                    if (!inE) need = s_lost[k];
+                    if (!need) continue;
+                }
                 CandRow<DT> row;
                 row.load(a.adjR, v, sw);
                 if (l >= ecc_w && !none) {
-                    // B_l is full here, so the item only has to show V_l[v]
-                    // full: words of neighbours outside the Delta union are
-                    // B_{l-1} (LDS), a lower bound of the OR; groups of 8 until
-                    // it is full.  Otherwise the exact OR below.
+                    // words of neighbours outside the Delta union are B_{l-1}
+                    // (LDS), a lower bound of the OR; groups of 4 until the
+                    // needed bits are covered.  Otherwise the exact OR below.
                     auto outside = [&](int x) {
                         return ((s_ubits[x >> 5] >> (x & 31)) & 1u) ? 0ull : Bprev[x];
                     };
                     acc = outside(v);
-                    bool full = false;
+                    bool full = (acc & need) == need;
 #pragma unroll
-                    for (int g = 0; g < DT; g += 8) {
+                    for (int g = 0; g < DT && !full; g += 4) {
 #pragma unroll
-                        for (int i = g; i < (g + 8 < DT ? g + 8 : DT); i++) acc |= outside(row.u[i]);
-                        if (acc == ~0ull) { full = true; break; }
+                        for (int i = g; i < (g + 4 < DT ? g + 4 : DT); i++) acc |= outside(row.u[i]);
+                        full = (acc & need) == need;
                     }
                     if (full) continue;
                 }
@@ -342,6 +379,11 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a, TopoRingQ topo, Den
             if (idx >= a.cap) { s_over[k] = 1; continue; }
             a.dl_key[(tl * 2 + cur) * a.cap + idx] = v;
             a.dl_ovr[(tl * 2 + cur) * (size_t)N + v] = acc;
+            if (a.lg_key && l <= a.S) {
+                const size_t li = (tl * a.S + (l - 1)) * a.cap + idx;
+                a.lg_key[li] = v;
+                a.lg_word[li] = acc;
+            }
             atomicAdd(&s_dc[k], (unsigned long long)((long long)__popcll(acc) -
                                                       (long long)__popcll(basew)));
             if (none || !((pbits[v >> 5] >> (v & 31)) & 1u) || povr[v] != acc) s_diff[k] = 1;
@@ -352,8 +394,10 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a, TopoRingQ topo, Den
             if (!s_active[k]) continue;
             const size_t g = gt(k);
             if (s_over[k]) {
+                // its levels are not kept: committing this candidate needs a new BFS
                 s_state[k] = GG_TASK_DENSE;
                 a.task_state[g] = GG_TASK_DENSE;
+                a.task_flags[g] = GG_FLAG_QUEUED;
                 int q = atomicAdd(a.queue_count, 1);
                 a.queue[q] = make_int2(a.k0 + k, w);
                 s_ended[k] = 1;
@@ -361,6 +405,7 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a, TopoRingQ topo, Den
             }
             const int n_cur = s_ncur[k];
             a.dcnt[g * (a.lmax + 1) + l] = (long long)s_dc[k];
+            if (a.lg_n && l <= a.S) a.lg_n[lt(k) * a.S + (l - 1)] = n_cur;
             s_cnt[cur][k] = n_cur;
             const bool fixed = (l >= ecc_w && n_cur == 0) ||
                                (l > ecc_w && !s_diff[k] && n_cur == s_nprev[k]);
@@ -401,10 +446,16 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a, TopoRingQ topo, Den
                 s_task = t;
             }
             __syncthreads();
-            const int t = s_task;
+            // wave-uniform: the task's pointers live in scalar registers
+            const int t = __builtin_amdgcn_readfirstlane(s_task);
             if (t >= ntask) break;
             const int k = t >> 2, wv = s_sw[k].v[t & 3] >> 6;
-            const ColPart p = dense_column<TopoRingQ, BS, VPT, DT>(topo, qa, a.k0 + k, wv, Bprev);
+            ColOut o;
+            o.snap = a.ep_snap ? a.ep_snap + (size_t)t * a.S * N : nullptr;
+            o.lstride = (size_t)N;
+            o.levels = a.S;
+            o.reach = a.ep_reach ? a.ep_reach + (size_t)t * (a.lmax + 1) : nullptr;
+            const ColPart p = dense_column<TopoRingQ, BS, VPT, DT>(topo, qa, a.k0 + k, wv, Bprev, o);
             if (tid == 0) qa.parts[(size_t)(a.k0 + k) * qa.W + wv] = p;
         }
     }
@@ -425,10 +476,11 @@ __global__ void __launch_bounds__(BS) k_sparse_finalize(SparseArgs a, int nvalid
         const size_t t = (size_t)k * W + w;
         if (a.task_state[t] == GG_TASK_DENSE) {
             const ColPart &p = a.dense_parts[t];
+            a.task_ecc[t] = p.ecc;
             tot += p.total; rch += p.reached;
             diam = p.ecc > diam ? p.ecc : diam;
             lev = p.levels > lev ? p.levels : lev;
-            fl |= p.flags;
+            fl |= p.flags | (a.task_flags[t] & GG_FLAG_QUEUED);
             continue;
         }
         const int end = a.task_end[t];
@@ -444,6 +496,7 @@ __global__ void __launch_bounds__(BS) k_sparse_finalize(SparseArgs a, int nvalid
             if (c > cprev) { ecc = l; ctot += (long long)l * (c - cprev); }
             cprev = c;
         }
+        a.task_ecc[t] = ecc;
         tot += ctot;
         rch += cprev;
         diam = ecc > diam ? ecc : diam;

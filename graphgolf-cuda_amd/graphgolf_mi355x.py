@@ -40,8 +40,10 @@ GG_LEVEL_LIMIT_MAX = 199
 
 GG_BATCH_OPT_PATH = 1
 GG_BATCH_OPT_GRAPH = 2
+GG_BATCH_OPT_COMMIT = 3
 GG_BATCH_INFO_LAST_SPARSE = 100
 GG_BATCH_INFO_SPARSE_OK = 101
+GG_BATCH_INFO_LAST_COMMIT_INPLACE = 102
 PATH_AUTO, PATH_DENSE, PATH_SPARSE = 0, 1, 2
 
 
@@ -167,6 +169,7 @@ def lib() -> C.CDLL:
         "gg_batch_commit": (C.c_int, [vp, P(Swap), P(Result)]),
         "gg_batch_last_kernel_ms": (C.c_int, [vp, P(C.c_float)]),
         "gg_batch_base_ecc": (C.c_int, [vp, i32p]),
+        "gg_batch_levels_digest": (C.c_int, [vp, P(C.c_uint64)]),
         "gg_batch_set_option": (C.c_int, [vp, C.c_int, C.c_int]),
         "gg_batch_get_option": (C.c_int, [vp, C.c_int, P(C.c_int)]),
         "gg_search_create": (C.c_int, [C.c_int, C.c_int, i32p, P(SearchCfg), C.c_int, P(vp)]),
@@ -202,6 +205,7 @@ def lib() -> C.CDLL:
         "gg_mlp_train": (C.c_int, [vp, P(C.c_float), P(C.c_float), C.c_int, C.c_float,
                                    P(C.c_float)]),
         "gg_batch_features": (C.c_int, [vp, i32p, C.c_int, P(C.c_float)]),
+        "gg_batch_propose": (C.c_int, [vp, C.c_uint64, C.c_int64, C.c_int, i32p]),
         "gg_gen_random_regular": (C.c_int, [C.c_int, C.c_int, C.c_uint64, i32p]),
         "gg_gen_orbit_part": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_uint64, i32p]),
     }
@@ -400,6 +404,16 @@ class BatchEvaluator:
         check(lib().gg_batch_last_kernel_ms(self._h, C.byref(ms)), "gg_batch_last_kernel_ms")
         return ms.value
 
+    def base_ecc(self) -> np.ndarray:
+        e = np.empty(self.N, dtype=np.int32)
+        check(lib().gg_batch_base_ecc(self._h, _ptr(e)), "gg_batch_base_ecc")
+        return e
+
+    def levels_digest(self) -> int:
+        h = C.c_uint64(0)
+        check(lib().gg_batch_levels_digest(self._h, C.byref(h)), "gg_batch_levels_digest")
+        return h.value
+
     def features(self, moves) -> np.ndarray:
         """32 features per move row (a, b, c, d, kind) on the current base."""
         m = _i32(moves).reshape(-1, 5)
@@ -407,6 +421,12 @@ class BatchEvaluator:
         check(lib().gg_batch_features(self._h, _ptr(m), m.shape[0], _ptr(X, C.c_float)),
               "gg_batch_features")
         return X
+
+    def propose(self, seed: int, step: int, P: int) -> np.ndarray:
+        """P device-drawn 2-opt proposals {a, b, c, d, kind} on the current base."""
+        m = np.empty((P, 5), dtype=np.int32)
+        check(lib().gg_batch_propose(self._h, seed, step, P, _ptr(m)), "gg_batch_propose")
+        return m
 
     def set_option(self, opt: int, value: int):
         check(lib().gg_batch_set_option(self._h, opt, value), "gg_batch_set_option")

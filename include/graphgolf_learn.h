@@ -39,6 +39,17 @@ int gg_mlp_train(gg_mlp *m, const float *X, const float *Y, int B, float lr, flo
  * kind 0: (a,c),(b,d)) on the batch evaluator's current base. */
 int gg_batch_features(gg_batch *b, const int32_t *moves, int P, float *X);
 
+/* P uniform 2-opt proposals on the batch evaluator's current base, drawn on
+ * the GPU as the learned search draws them: proposal p of step `step` takes
+ * splitmix64 stream (seed, step, p) (start state
+ * mix(mix(seed ^ 0xA0761D6478BD642F) ^ step * 0xE7037ED1A0B428DB)
+ * ^ p * 0x8EBC6AF09C88C6E3) and redraws a, ia, c, ic, kind (Lemire bounded
+ * integers) until the rewired graph is simple, as createNeighbour's 2-opt
+ * does (cppcode/annealing.cpp:140-197, here with both rewirings and the
+ * simplicity test).  Writes P rows {a, b, c, d, kind}; GG_BAD_ARG when some
+ * proposal found no simple swap in 4096 attempts. */
+int gg_batch_propose(gg_batch *b, uint64_t seed, int64_t step, int P, int32_t *moves_out);
+
 #ifdef __cplusplus
 }
 #endif

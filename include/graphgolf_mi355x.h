@@ -172,12 +172,21 @@ void gg_batch_destroy(gg_batch *b);
 int gg_batch_set_base(gg_batch *b, const int32_t *adj, gg_result *base_out);
 /* Evaluate base + swaps[k] for k < K (K <= max_k); the base is unchanged. */
 int gg_batch_eval(gg_batch *b, const gg_swap *swaps, int K, gg_result *out);
-/* Apply one swap to the base and re-evaluate it. */
+/* Apply one swap to the base and re-evaluate it.  When the swap is one of
+ * the candidates of the last gg_batch_eval and that launch was incremental,
+ * its result is already known and the stored base levels are rewritten in
+ * place from what the launch kept (no new BFS; see GG_BATCH_OPT_COMMIT). */
 int gg_batch_commit(gg_batch *b, const gg_swap *sw, gg_result *base_out);
 /* Device time (ms) of the most recent gg_batch_eval launch. */
 int gg_batch_last_kernel_ms(gg_batch *b, float *ms);
 /* Per-vertex eccentricity of the current base (N entries). */
 int gg_batch_base_ecc(gg_batch *b, int32_t *ecc_out);
+/* Order-independent 64-bit checksum of the state the incremental path starts
+ * from: the stored words of every column below its eccentricity (levels at
+ * or above it count as full), the column parts and per-level counts.  Equal
+ * for equal bases however they were reached (set_base or commits).  No
+ * reference counterpart; for tests and diagnostics. */
+int gg_batch_levels_digest(gg_batch *b, uint64_t *digest);
 
 /* Evaluation path: 0 = automatic (incremental when the base allows it),
  * 1 = dense re-evaluation of every candidate, 2 = incremental when possible. */
@@ -188,6 +197,11 @@ int gg_batch_base_ecc(gg_batch *b, int32_t *ecc_out);
 #define GG_BATCH_INFO_LAST_SPARSE  100
 /* read-only: 1 if the current base allows the incremental path */
 #define GG_BATCH_INFO_SPARSE_OK    101
+/* Commit of an evaluated candidate: 0 = rewrite the stored levels in place
+ * when the last launch kept them (default), 1 = always a new base BFS. */
+#define GG_BATCH_OPT_COMMIT        3
+/* read-only: 1 if the last commit rewrote the stored levels in place */
+#define GG_BATCH_INFO_LAST_COMMIT_INPLACE 102
 int gg_batch_set_option(gg_batch *b, int option, int value);
 int gg_batch_get_option(gg_batch *b, int option, int *value);
 

@@ -64,3 +64,17 @@ def disjoint_cycles(n1, n2):
     a = cycle(n1)
     b = cycle(n2) + n1
     return np.concatenate([a, b]).astype(np.int32)
+
+
+def bridged(A, B):
+    """Disjoint union of two regular graphs joined by one 2-opt swap across
+    them: edges (a,b) of A and (c,d) of B become the bridges (a,c), (b,d).
+    Returns the adjacency and the two bridges."""
+    A = np.asarray(A, dtype=np.int32)
+    B = np.asarray(B, dtype=np.int32) + A.shape[0]
+    adj = np.concatenate([A, B]).copy()
+    a, c = 0, A.shape[0]
+    b, d = int(adj[a, 0]), int(adj[c, 0])
+    for v, old, new in ((a, b, c), (b, a, d), (c, d, a), (d, c, b)):
+        adj[v, int(np.nonzero(adj[v] == old)[0][0])] = new
+    return adj, ((a, c), (b, d))

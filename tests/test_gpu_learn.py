@@ -110,3 +110,29 @@ def test_learned_search_exact_acceptance_and_gain(gpu):
     print(f"learned audit: spearman {rho:.3f} miss {miss:.3f} "
           f"top-K mean delta {dl[top].mean():.1f} vs uniform {dl.mean():.1f}")
     s.close()
+
+
+def test_device_proposals_match_restatement(gpu):
+    """The learned search's GPU draws equal the restated streams, before and
+    after commits change the base."""
+    gg = gpu
+    import devrng_ref
+    from test_gpu_batch import _random_swaps
+    n, d = 512, 6
+    adj = gg.random_regular(n, d, 8)
+    rng = np.random.default_rng(4)
+    with gg.BatchEvaluator(n, d, 16) as be:
+        be.set_base(adj)
+        for step, seed in ((0, 77), (5, 77), (123456, 2 ** 63 + 9)):
+            mv = be.propose(seed, step, 300)
+            ref = [devrng_ref.propose(adj, seed, step, p) for p in range(300)]
+            assert [tuple(int(x) for x in r) for r in mv] == ref, step
+        cands = _random_swaps(gg, adj, 16, rng)
+        rs = be.eval([c[0] for c in cands])
+        i = min((i for i, r in enumerate(rs) if r.status == gg.GG_OK),
+                key=lambda i: rs[i].total_reps)
+        be.commit(cands[i][0])
+        adj = cands[i][1]
+        mv = be.propose(77, 6, 300)
+        ref = [devrng_ref.propose(adj, 77, 6, p) for p in range(300)]
+        assert [tuple(int(x) for x in r) for r in mv] == ref

@@ -64,18 +64,26 @@ def test_sparse_replays_and_commits(gpu):
 
 
 def test_sparse_disconnecting_candidates(gpu):
-    """Candidates that split a cycle-of-cliques-like sparse graph."""
+    """Two random halves joined by two bridge edges: the swap that rewires
+    both bridges back inside the halves disconnects the graph; random swaps
+    around it stay connected or stretch distances."""
     gg = gpu
     import graphs
-    n = 300
-    adj = graphs.circulant(n, [1, 2])  # 4-regular ring: many swaps disconnect or stretch it
+    h, d = 128, 10
+    adj, ((x1, y1), (x2, y2)) = graphs.bridged(gg.random_regular(h, d, 21),
+                                               gg.random_regular(h, d, 22))
+    n = 2 * h
+    ia = int(np.nonzero(adj[x1] == y1)[0][0])
+    ic = int(np.nonzero(adj[x2] == y2)[0][0])
+    cut, cut_adj = gg.make_swap(adj, x1, ia, x2, ic, 0)  # (x1,x2), (y1,y2)
+    assert cut is not None
     rng = np.random.default_rng(9)
-    cands = _random_swaps(gg, adj, 40, rng)
-    with gg.BatchEvaluator(n, 4, max_k=40) as be:
+    cands = [(cut, cut_adj)] + _random_swaps(gg, adj, 39, rng)
+    with gg.BatchEvaluator(n, d, max_k=40) as be:
         base = be.set_base(adj)
-        if be.get_option(gg.GG_BATCH_INFO_SPARSE_OK) != 1:
-            pytest.skip("base diameter beyond stored levels")
+        assert be.get_option(gg.GG_BATCH_INFO_SPARSE_OK) == 1, base.diameter
         rs, rd = _eval_both(gg, be, [c[0] for c in cands])
+        assert rs[0].status == gg.GG_UNCONNECTED
         for (sw, new), a, b in zip(cands, rs, rd):
             o = ol.adj_eval(new)
             assert a.total_reps == b.total_reps == o["total"]
