@@ -39,6 +39,8 @@ def cpu_baseline(adj, seconds_target=12.0):
     threads = min(os.cpu_count() or 1, 16)
     a = np.ascontiguousarray(adj, dtype=np.int32)
     tot, dia = C.c_int64(), C.c_int32()
+    # grow the source sample until one pass takes a measurable time, then
+    # repeat passes until about seconds_target of CPU work has been timed
     nsrc = 256
     while True:
         srcs = np.arange(nsrc, dtype=np.int32)
@@ -49,10 +51,18 @@ def cpu_baseline(adj, seconds_target=12.0):
         if dt > seconds_target / 4 or nsrc >= n:
             break
         nsrc = min(n, nsrc * 4)
-    return {"value": (nsrc / n) / dt, "unit": "exact evaluations/s", "cores": threads,
-            "kind": "port",
-            "sample": f"oracle queue BFS, OpenMP {threads} threads, {nsrc} of {n} sources of the "
-                      f"n={n} d={d} graph in {dt:.2f} s, scaled by {n}/{nsrc}"}
+    passes, t_all = 1, dt
+    while t_all < seconds_target:
+        t0 = time.perf_counter()
+        ol.oracle().oracle_bfs_omp(n, d, ol._p(a), nsrc, ol._p(srcs), threads, C.byref(tot),
+                                   C.byref(dia))
+        t_all += time.perf_counter() - t0
+        passes += 1
+    return {"value": passes * (nsrc / n) / t_all, "unit": "exact evaluations/s",
+            "cores": threads, "kind": "port",
+            "sample": f"oracle queue BFS, OpenMP {threads} threads, {passes} passes over {nsrc} "
+                      f"of {n} sources of the n={n} d={d} graph in {t_all:.2f} s, "
+                      f"scaled by {n}/{nsrc}"}
 
 
 def plain_kernel_roofline(gg, adj, reps=30):
