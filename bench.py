@@ -95,10 +95,22 @@ def plain_kernel_roofline(gg, adj, reps=30):
     bs = 256 if info["variant"] == 1 else 1024
     name = (f"k_dense_lds<TopoRingQ, {bs}, {info['vpt']}, {info['fixed']}>"
             if info["variant"] in (1, 2) else "k_dense_global<TopoRingQ>")
-    return {"bound": "lds", "achieved": ach, "peak": LDS_PEAK_GBS, "unit": "GB/s",
-            "frac": ach / LDS_PEAK_GBS, "traffic": hbm_traffic(n, d), "kernel": name,
+    # LDS ceiling: the larger of the width-times-clock figure and the measured
+    # conflict-free 16-byte read rate of the device
+    p16 = gg.probe_lds_read(16)
+    p8 = gg.probe_lds_read(8)
+    peak = max(LDS_PEAK_GBS, p16["gbs"])
+    basis = "measured 16-byte LDS read probe" if p16["gbs"] > LDS_PEAK_GBS else \
+        "256 CU x 128 B/clk x 2.4 GHz (above the measured probe)"
+    return {"bound": "lds", "achieved": ach, "peak": peak, "unit": "GB/s",
+            "frac": ach / peak, "traffic": hbm_traffic(n, d), "kernel": name,
             "kernel_ms": t * 1e3, "levels": L, "alg_bytes_per_launch": alg,
-            "push_cap": info["push_cap"], "hbm_frac": ach / HBM_PEAK_GBS}
+            "push_cap": info["push_cap"], "peak_basis": basis,
+            "lds_probe": {"b128_gbs": p16["gbs"], "b64_gbs": p8["gbs"],
+                          "clock_mhz": p16["clock_mhz"],
+                          "b128_bytes_per_clk_cu": p16["bytes_per_clk_cu"],
+                          "b64_bytes_per_clk_cu": p8["bytes_per_clk_cu"]},
+            "alg_rate_over_hbm_peak": ach / HBM_PEAK_GBS}
 
 
 def hbm_traffic(n, d):
@@ -136,6 +148,25 @@ def copy_calibration(nbytes=1 << 30, reps=5):
         hip.hipFree(p)
     print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "reps": reps}),
           flush=True)
+
+
+def trained_scorer(gg, adj, cfg, pairs, K, audits=3):
+    """The learned search run until the scorer has trained on `pairs` exact
+    (features, delta) pairs from the run itself, then audited: P fresh
+    proposals all evaluated exactly, Spearman rank correlation of score and
+    exact delta, fraction of the exact top-K the scorer's top-K misses
+    (mean over `audits` proposal draws)."""
+    steps = (pairs + K - 1) // K
+    cfg = gg.SearchCfg.from_buffer_copy(cfg)
+    cfg.total_steps = steps
+    s = gg.Search(adj, cfg)
+    st = s.run(steps)
+    res = [s.audit(1000 + i)[:2] for i in range(audits)]
+    s.close()
+    return {"spearman": float(np.mean([r[0] for r in res])),
+            "top_k_miss_rate": float(np.mean([r[1] for r in res])),
+            "train_pairs": st.train_pairs, "train_steps": st.train_steps,
+            "train_loss": st.train_loss, "search_seconds": st.seconds, "audits": audits}
 
 
 def batched_vs_sequential(gg, n=8192, d=16, K=256, steps=10, seed=4):
@@ -236,6 +267,7 @@ def main():
     learn_rate = el / tl
     rho, miss, _, _ = sl.audit(777)
     sl.close()
+    scorer_long = trained_scorer(gg, adj, learn_cfg, pairs=100_000, K=args.K)
 
     out = {
         "metric": "exact moves evaluated/sec (batched+learned search)",
@@ -249,7 +281,9 @@ def main():
                   "ms_per_step": tp / args.steps * 1e3},
         "speedup_vs_plain": learn_rate / plain_rate,
         "scorer": {"spearman": rho, "top_k_miss_rate": miss, "train_pairs": stl.train_pairs,
-                   "train_steps": stl.train_steps, "train_loss": stl.train_loss},
+                   "train_steps": stl.train_steps, "train_loss": stl.train_loss,
+                   "uniform_top_k_miss_rate": 1.0 - args.K / args.P},
+        "scorer_trained_1e5": scorer_long,
     }
     if not args.no_roofline:
         out["roofline"] = plain_kernel_roofline(gg, adj)

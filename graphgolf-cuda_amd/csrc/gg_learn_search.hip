@@ -202,7 +202,7 @@ __global__ void __launch_bounds__(1024) k_select(const int *rank, int P, int K, 
 // queue counters are reset for the next step.
 __global__ void __launch_bounds__(256) k_pack_out(int K, const ResultRec *res, ResultRec *h_res,
                                                   const gg_swap *sw, gg_swap *h_sw,
-                                                  const int *msel, int *h_msel, const int *dctl,
+                                                  const int *msel, int *h_msel, int *dctl,
                                                   int *h_ctl, int *qctl, int nqctl) {
     const int t = threadIdx.x;
     for (int k = t; k < K; k += 256) {
@@ -211,6 +211,7 @@ __global__ void __launch_bounds__(256) k_pack_out(int K, const ResultRec *res, R
     }
     for (int i = t; i < 5 * K; i += 256) h_msel[i] = msel[i];
     if (t < 4) h_ctl[t] = dctl[t];
+    if (t == 3) dctl[3] = 0;  // k_propose only ORs its error word: clear it for the next replay
     for (int i = t; i < nqctl; i += 256) qctl[i] = 0;
 }
 
@@ -297,6 +298,9 @@ static int alloc_learn(LearnState *L) {
     if (hipHostMalloc((void **)&L->h_msel, 4 * (size_t)K * 5, hipHostMallocDefault) != hipSuccess)
         return GG_OOM;
     if (hipMalloc((void **)&L->d_ctl, 4 * 4) != hipSuccess) return GG_OOM;
+    // the error word d_ctl[3] is only ever ORed by k_propose and cleared by
+    // k_pack_out, so it must start at zero (a fresh allocation is not)
+    if (hipMemset(L->d_ctl, 0, 4 * 4) != hipSuccess) return GG_HIP_ERROR;
     if (hipHostMalloc((void **)&L->h_ctl, 4 * 8, hipHostMallocDefault) != hipSuccess)
         return GG_OOM;
     return GG_OK;

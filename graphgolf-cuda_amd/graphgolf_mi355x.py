@@ -73,6 +73,11 @@ class TuneReport(C.Structure):
                 ("best_generic", C.c_int32), ("best_ms", C.c_float)]
 
 
+class LdsProbe(C.Structure):
+    _fields_ = [("gbs", C.c_double), ("kernel_ms", C.c_double), ("clock_mhz", C.c_double),
+                ("bytes_per_clk_cu", C.c_double), ("cus", C.c_int32)]
+
+
 class Swap(C.Structure):
     _fields_ = [("v", C.c_int32 * 4), ("slot", C.c_int32 * 4), ("nbr", C.c_int32 * 4)]
 
@@ -162,6 +167,7 @@ def lib() -> C.CDLL:
         "gg_eval_adj_swap": (C.c_int, [vp, P(Swap), P(Result)]),
         "gg_eval_autotune": (C.c_int, [vp, i32p, C.c_int, P(TuneReport)]),
         "gg_eval_adj_commit": (C.c_int, [vp, P(Swap)]),
+        "gg_probe_lds_read": (C.c_int, [C.c_int, C.c_int, P(LdsProbe)]),
         "gg_batch_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, P(vp)]),
         "gg_batch_destroy": (None, [vp]),
         "gg_batch_set_base": (C.c_int, [vp, i32p, P(Result)]),
@@ -239,6 +245,13 @@ def device_count() -> int:
     n = C.c_int(0)
     code = lib().gg_device_count(C.byref(n))
     return n.value if code == GG_OK else 0
+
+
+def probe_lds_read(bytes_per_read: int = 16, device: int = 0) -> dict:
+    """Measured LDS read bandwidth of the device (wave-consecutive reads)."""
+    r = LdsProbe()
+    check(lib().gg_probe_lds_read(device, bytes_per_read, C.byref(r)), "gg_probe_lds_read")
+    return {k: getattr(r, k) for k, _ in r._fields_}
 
 
 # ---- generators (host code inside the library) ----------------------------------

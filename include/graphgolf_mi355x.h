@@ -205,6 +205,20 @@ int gg_batch_levels_digest(gg_batch *b, uint64_t *digest);
 int gg_batch_set_option(gg_batch *b, int option, int value);
 int gg_batch_get_option(gg_batch *b, int option, int *value);
 
+/* ---- measurement helper (no reference counterpart) ------------------------
+ * LDS read bandwidth of the whole device: 16 workgroups per CU of 1024
+ * threads read 64 KiB of LDS each, wave-consecutive (conflict-free) reads of
+ * bytes_per_read (4, 8 or 16) bytes.  The ceiling the LDS-resident BFS
+ * kernel's roofline is quoted against. */
+typedef struct gg_lds_probe {
+    double gbs;              /* bytes read per second / 1e9                  */
+    double kernel_ms;
+    double clock_mhz;        /* shader clock during the run (cycle counter vs wall clock) */
+    double bytes_per_clk_cu; /* gbs / (clock * CUs)                          */
+    int32_t cus;
+} gg_lds_probe;
+int gg_probe_lds_read(int device, int bytes_per_read, gg_lds_probe *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -136,3 +136,22 @@ def test_device_proposals_match_restatement(gpu):
         mv = be.propose(77, 6, 300)
         ref = [devrng_ref.propose(adj, 77, 6, p) for p in range(300)]
         assert [tuple(int(x) for x in r) for r in mv] == ref
+
+
+def test_learned_searches_back_to_back_are_independent(gpu):
+    """Learned searches created one after another in one process (their device
+    buffers reuse freed memory) run without stale control words, and a
+    repeated configuration repeats its trajectory exactly."""
+    gg = gpu
+    n, d = 2048, 8
+    adj = gg.random_regular(n, d, 6)
+    runs = []
+    for tb in (256, 512, 256):
+        cfg = gg.search_cfg(mode=gg.MODE_LEARNED, K=16, P=256, total_steps=200, T0=1e-4,
+                            T1=1e-6, seed=9, warmup_pairs=512, train_batch=tb, lr=2e-3)
+        s = gg.Search(adj, cfg)
+        st = s.run(80)
+        assert st.exact_evals == 80 * 16
+        runs.append((st.total_cur, st.total_best, st.accepted, s.graph().tobytes()))
+        s.close()
+    assert runs[0] == runs[2]
