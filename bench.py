@@ -250,7 +250,7 @@ def main():
 
     learn_cfg = gg.search_cfg(mode=gg.MODE_LEARNED, K=args.K, P=args.P, total_steps=total,
                               T0=T0, T1=T1, seed=12345, warmup_pairs=2048, train_every=1,
-                              train_batch=256, lr=1e-3)
+                              train_batch=256, lr=3e-3)
     if args.only == "learned":
         sl, stl, el, tl = run_search(gg, adj, learn_cfg, args.warmup, args.steps)
         print(json.dumps({"learned_evals_per_s": el / tl, "ms_per_step": tl / args.steps * 1e3}),

@@ -18,7 +18,22 @@
 
 using namespace gg;
 
-// ---- features --------------------------------------------------------------------------
+// ---- features ----------------------------------------------------------------------------
+// Per proposal (added edges (p1,q1),(p2,q2), removed (a,b),(c,d)), 32 features:
+//  added edge t, X[8t..8t+8): pre-swap d(p,q)/diam, |L3(p)&L3(q)|/n,
+//    |~R3(p)&~R3(q)|/n, |L2(p)&L2(q)|/d^2, |L2(q)&~R3(p)|/d^2,
+//    |L2(p)&~R3(q)|/d^2, (|L1(q)&~R3(p)| + |L1(p)&~R3(q)|)/d,
+//    (|L1(q)&~R2(p)| + |L1(p)&~R2(q)|)/d -- vertices the new edge brings
+//    closer to p or q (first-order gain of the swap);
+//  removed edge t = (x,y), X[16+6t..16+6t+6): U(x,y)/d, U(y,x)/d,
+//    |L2(x)&L2(y)|/d^2, |R2(x)&R2(y)|/d^2, (|L1(x)&L2(y)| + |L2(x)&L1(y)|)/d^2,
+//    (ecc(x)+ecc(y)-2 diam)/2, where U(u,v) = #{z in N(v), z not in B1(u) :
+//    |N(z) & N(u)| = 1}, the second-layer vertices of u reached only through
+//    v (first-order loss of the swap);
+//  endpoint e of (a,b,c,d), X[28+e]: |~R3(x_e)|/n.
+// Bits of sources >= N are set in every stored level (they start reached);
+// level 0 sets them too, so no layer L_l (l >= 1) and no complement ~R_l
+// holds them; R2&R2 is masked explicitly.
 struct FeatArgs {
     int N, W, S, diam, d;
     const int *diam_dev;             // diameter on the device (overrides diam)
@@ -26,25 +41,84 @@ struct FeatArgs {
     const unsigned long long *rows;  // [RL][N][W] levels 1..RL row-major (full words stored)
     int RL;
     const ColPart *base_parts;       // [W]
+    const int *adjR;                 // [N][d] row-major neighbours of the base
     const int *moves;                // [P][5]: a, b, c, d, kind
     int P;
     float *X;                        // [P][GG_FEATS]
 };
 
-// B_l[w][x] with levels >= the column's eccentricity full and level 0 = {x}.
+__device__ __forceinline__ unsigned long long pad_bits(const FeatArgs &f, int w) {
+    const int r = f.N & 63;
+    return (w == f.W - 1 && r) ? (~0ull << r) : 0ull;
+}
+
+// B_l[w][x] with levels >= the column's eccentricity full and level 0 = {x}
+// plus the padding sources.
 __device__ __forceinline__ unsigned long long ball_word(const FeatArgs &f, int l, int w, int x,
                                                        int ecc_w) {
-    if (l <= 0) return ((x >> 6) == w) ? (1ull << (x & 63)) : 0ull;
+    if (l <= 0) return (((x >> 6) == w) ? (1ull << (x & 63)) : 0ull) | pad_bits(f, w);
     if (l <= f.RL) return f.rows[((size_t)(l - 1) * f.N + x) * f.W + w];  // lanes over w: coalesced
     if (l >= ecc_w) return ~0ull;
     return f.snap[((size_t)(l - 1) * f.W + w) * f.N + x];
 }
 
+// Bit z of the radius-1 ball row of u: z in B1(u).
+__device__ __forceinline__ bool in_ball1(const FeatArgs &f, int u, int z) {
+    const int w = z >> 6;
+    return (ball_word(f, 1, w, u, f.base_parts[w].ecc) >> (z & 63)) & 1ull;
+}
+
+// Radius-1 ball rows of the four endpoints staged in LDS per wave when the
+// row fits (W <= GG_FEAT_LDS_W words), else read from the stored levels.
+#define GG_FEAT_LDS_W 256
+
+// U(u,v) for the four directions (u,v) = (a,b), (b,a), (c,d), (d,c)
+// (u = endpoint e, v = endpoint e^1): lanes over the 4d (direction, entry z
+// of v's row) items; a lane reads z's row and tests its entries against u's
+// radius-1 ball row.
+__device__ void uniq_second_layer4(const FeatArgs &f, const int ends[4],
+                                   const unsigned long long (*r1)[GG_FEAT_LDS_W], int lane,
+                                   int out[4]) {
+    auto ball1 = [&](int e, int z) -> bool {
+        if (r1) return (r1[e][z >> 6] >> (z & 63)) & 1ull;
+        return in_ball1(f, ends[e], z);
+    };
+    int part[4] = {0, 0, 0, 0};
+    const int items = 4 * f.d;
+    for (int k = lane; k < items; k += 64) {
+        const int e = k / f.d, i = k - e * f.d;
+        const int u = ends[e], v = ends[e ^ 1];
+        const int z = f.adjR[(size_t)v * f.d + i];
+        if (z == u || ball1(e, z)) continue;
+        const int *rz = f.adjR + (size_t)z * f.d;
+        int common = 0;
+#pragma unroll 8
+        for (int j = 0; j < f.d; j++) common += ball1(e, rz[j]) ? 1 : 0;
+        const int hit = common == 1 ? 1 : 0;
+        part[0] += e == 0 ? hit : 0;
+        part[1] += e == 1 ? hit : 0;
+        part[2] += e == 2 ? hit : 0;
+        part[3] += e == 3 ? hit : 0;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        int x = part[q];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+        out[q] = x;
+    }
+}
+
 // One wave per proposal.  Counts are wave-reduced popcounts over the W words.
 __global__ void __launch_bounds__(256) k_features(FeatArgs f) {
-    const int lane = threadIdx.x & 63;
-    const int p = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (p >= f.P) return;
+    __shared__ unsigned long long s_r1[4][4][GG_FEAT_LDS_W];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    // a wave past the last proposal computes proposal 0 and writes nothing,
+    // so every wave reaches the workgroup barrier
+    const int p0 = blockIdx.x * 4 + wv;
+    const bool active = p0 < f.P;
+    const int p = active ? p0 : 0;
+    const bool stage = f.W <= GG_FEAT_LDS_W;
     const int *m = f.moves + (size_t)p * 5;
     const int a = m[0], b = m[1], c = m[2], dd = m[3], kind = m[4];
     // added edges (p1,q1),(p2,q2); removed (a,b),(c,dd)
@@ -59,11 +133,16 @@ __global__ void __launch_bounds__(256) k_features(FeatArgs f) {
     int eccl[4] = {0, 0, 0, 0};
     for (int w = lane; w < f.W; w += 64) {
         const int ew = f.base_parts[w].ecc;
+        const unsigned long long valid = ~pad_bits(f, w);
         unsigned long long R[4][4];  // R[endpoint][level 0..3]
 #pragma unroll
         for (int e = 0; e < 4; e++)
 #pragma unroll
             for (int l = 0; l < 4; l++) R[e][l] = ball_word(f, l, w, ends[e], ew);
+        if (stage) {
+#pragma unroll
+            for (int e = 0; e < 4; e++) s_r1[wv][e][w] = R[e][1];
+        }
 #pragma unroll
         for (int e = 0; e < 4; e++) {
             int fl = 0;
@@ -79,25 +158,24 @@ __global__ void __launch_bounds__(256) k_features(FeatArgs f) {
         const int pairs[2][2] = {{ip1, iq1}, {ip2, iq2}};
 #pragma unroll
         for (int t = 0; t < 2; t++) {
-            int x = pairs[t][0], y = pairs[t][1];
-            cnt[t * 6 + 0] += __popcll(L(x, 1) & L(y, 1));
-            cnt[t * 6 + 1] += __popcll(L(x, 1) & L(y, 2)) + __popcll(L(x, 2) & L(y, 1));
-            cnt[t * 6 + 2] += __popcll(L(x, 2) & L(y, 2));
-            cnt[t * 6 + 3] += __popcll(R[x][2] | R[y][2]);
-            cnt[t * 6 + 4] += __popcll(L(x, 3) & L(y, 3));
-            cnt[t * 6 + 5] += __popcll(~R[x][3] & ~R[y][3]);
+            const int x = pairs[t][0], y = pairs[t][1];
+            cnt[t * 7 + 0] += __popcll(L(x, 3) & L(y, 3));
+            cnt[t * 7 + 1] += __popcll(~R[x][3] & ~R[y][3]);
+            cnt[t * 7 + 2] += __popcll(L(x, 2) & L(y, 2));
+            cnt[t * 7 + 3] += __popcll(L(y, 2) & ~R[x][3]);
+            cnt[t * 7 + 4] += __popcll(L(x, 2) & ~R[y][3]);
+            cnt[t * 7 + 5] += __popcll(L(y, 1) & ~R[x][3]) + __popcll(L(x, 1) & ~R[y][3]);
+            cnt[t * 7 + 6] += __popcll(L(y, 1) & ~R[x][2]) + __popcll(L(x, 1) & ~R[y][2]);
         }
-        const int rem[2][2] = {{0, 1}, {2, 3}};
 #pragma unroll
         for (int t = 0; t < 2; t++) {
-            int x = rem[t][0], y = rem[t][1];
-            cnt[12 + t * 4 + 0] += __popcll(L(x, 1) & L(y, 1));
-            cnt[12 + t * 4 + 1] += __popcll(L(x, 1) & L(y, 2)) + __popcll(L(x, 2) & L(y, 1));
-            cnt[12 + t * 4 + 2] += __popcll(L(x, 2) & L(y, 2));
-            cnt[12 + t * 4 + 3] += __popcll(R[x][2] & R[y][2]);
+            const int x = 2 * t, y = 2 * t + 1;
+            cnt[14 + t * 3 + 0] += __popcll(L(x, 2) & L(y, 2));
+            cnt[14 + t * 3 + 1] += __popcll(R[x][2] & R[y][2] & valid);
+            cnt[14 + t * 3 + 2] += __popcll(L(x, 1) & L(y, 2)) + __popcll(L(x, 2) & L(y, 1));
         }
 #pragma unroll
-        for (int e = 0; e < 4; e++) cnt[20 + e] += __popcll(L(e, 2));
+        for (int e = 0; e < 4; e++) cnt[20 + e] += __popcll(~R[e][3]);
     }
 #pragma unroll
     for (int i = 0; i < 24; i++) {
@@ -108,7 +186,10 @@ __global__ void __launch_bounds__(256) k_features(FeatArgs f) {
     }
 #pragma unroll
     for (int e = 0; e < 4; e++) eccl[e] = wave_max32(eccl[e]);
-    if (lane != 0) return;
+    __syncthreads();
+    int uq[4];
+    uniq_second_layer4(f, ends, stage ? s_r1[wv] : nullptr, lane, uq);
+    if (lane != 0 || !active) return;
     const int diam = f.diam_dev ? (*f.diam_dev > 0 ? *f.diam_dev : 1) : f.diam;
     // distance of each added pair before the swap
     auto dist = [&](int x, int y) {
@@ -120,28 +201,25 @@ __global__ void __launch_bounds__(256) k_features(FeatArgs f) {
     float *X = f.X + (size_t)p * GG_FEATS;
     const float d1 = (float)f.d, d2 = d1 * d1, n = (float)f.N, inv_diam = 1.f / (float)diam;
     const int qs[2] = {q1, q2}, ps[2] = {p1, p2};
-    // eccentricity by endpoint: p1 = a, p2 = b; q1, q2 are d, c or c, d
-    const int eq[2] = {kind ? eccl[3] : eccl[2], kind ? eccl[2] : eccl[3]};
-    const int ep[2] = {eccl[0], eccl[1]};
     for (int t = 0; t < 2; t++) {
         X[t * 8 + 0] = dist(ps[t], qs[t]) * inv_diam;
-        X[t * 8 + 1] = cnt[t * 6 + 0] / d1;
-        X[t * 8 + 2] = cnt[t * 6 + 1] / d2;
-        X[t * 8 + 3] = cnt[t * 6 + 2] / d2;
-        X[t * 8 + 4] = cnt[t * 6 + 3] / (2.f * d2);
-        X[t * 8 + 5] = cnt[t * 6 + 4] / n;
-        X[t * 8 + 6] = cnt[t * 6 + 5] / n;
-        X[t * 8 + 7] = (float)(ep[t] + eq[t] - 2 * diam) * 0.5f;
+        X[t * 8 + 1] = cnt[t * 7 + 0] / n;
+        X[t * 8 + 2] = cnt[t * 7 + 1] / n;
+        X[t * 8 + 3] = cnt[t * 7 + 2] / d2;
+        X[t * 8 + 4] = cnt[t * 7 + 3] / d2;
+        X[t * 8 + 5] = cnt[t * 7 + 4] / d2;
+        X[t * 8 + 6] = cnt[t * 7 + 5] / d1;
+        X[t * 8 + 7] = cnt[t * 7 + 6] / d1;
     }
     for (int t = 0; t < 2; t++) {
-        X[16 + t * 6 + 0] = cnt[12 + t * 4 + 0] / d1;
-        X[16 + t * 6 + 1] = cnt[12 + t * 4 + 1] / d2;
-        X[16 + t * 6 + 2] = cnt[12 + t * 4 + 2] / d2;
-        X[16 + t * 6 + 3] = cnt[12 + t * 4 + 3] / d2;
-        X[16 + t * 6 + 4] = (float)(eccl[2 * t] - diam);
-        X[16 + t * 6 + 5] = (float)(eccl[2 * t + 1] - diam);
+        X[16 + t * 6 + 0] = (float)uq[2 * t] / d1;
+        X[16 + t * 6 + 1] = (float)uq[2 * t + 1] / d1;
+        X[16 + t * 6 + 2] = cnt[14 + t * 3 + 0] / d2;
+        X[16 + t * 6 + 3] = cnt[14 + t * 3 + 1] / d2;
+        X[16 + t * 6 + 4] = cnt[14 + t * 3 + 2] / d2;
+        X[16 + t * 6 + 5] = (float)(eccl[2 * t] + eccl[2 * t + 1] - 2 * diam) * 0.5f;
     }
-    for (int e = 0; e < 4; e++) X[28 + e] = cnt[20 + e] / d2;
+    for (int e = 0; e < 4; e++) X[28 + e] = cnt[20 + e] / n;
 }
 
 // ---- MLP ---------------------------------------------------------------------------------
@@ -508,6 +586,7 @@ int gg::features_dev(gg_batch *b, const int *d_moves, int P, float *d_x, hipStre
     f.rows = b->d_rows;
     f.RL = b->d_rows ? b->rows_levels : 0;
     f.base_parts = b->d_parts + (size_t)b->max_k * b->W;
+    f.adjR = b->d_adjR;
     f.moves = d_moves;
     f.P = P;
     f.X = d_x;

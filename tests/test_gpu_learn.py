@@ -73,7 +73,9 @@ def _moves(gg, adj, k, rng):
     return np.array(out, dtype=np.int32)
 
 
-@pytest.mark.parametrize("n,d", [(1024, 6), (4096, 12)])
+# n = 1000: the last column holds padding sources; n = 17000: rows wider than
+# the LDS-staged radius-1 rows (the kernel reads the stored levels instead)
+@pytest.mark.parametrize("n,d", [(1000, 6), (1024, 6), (4096, 12), (17000, 8)])
 def test_features_match_set_definition(gpu, n, d):
     gg = gpu
     rng = np.random.default_rng(n)
