@@ -21,6 +21,8 @@
 // reference keeps it.  Acceptance and schedule are those of the 2-opt driver
 // (annealing.cpp:406-412, HillClimb.cpp:239).
 #include <chrono>
+#include <climits>
+#include <cstdint>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -32,6 +34,10 @@
 
 struct gg_orbit_search {
     int N = 0, M = 0, d = 0, g = 0;
+    // lattice form (dims 2 = piece, 3 = box): per-axis orders and base
+    // extents; dims 1 is the part form (Ns[0] = N, Ms[0] = M)
+    int dims = 1;
+    int Ns[3] = {0, 0, 0}, Ms[3] = {1, 1, 1};
     gg_search_cfg cfg{};
     gg_eval *ev = nullptr;
     gg_table_callbacks cb{};
@@ -126,6 +132,159 @@ bool gg::propose_orbit(const std::vector<int32_t> &cur, int N, int M, int d, gg:
 
 namespace {
 
+// ---- lattice (piece / box) tables -------------------------------------------------------
+// t[(row*d + slot)*D + k]: offset along axis k of slot `slot` of base row
+// `row`; rows are mixed-radix over (M_0, .., M_{D-1}), axis 0 major
+// (x*My + y for piece, (x*My + y)*Mz + z for box; cudaASPLpiece.cu:184-195).
+struct Lattice {
+    int D, d;
+    int N[3], M[3], g[3];
+    int Mtot;
+    void coords(int row, int c[3]) const {
+        for (int k = D - 1; k >= 0; k--) { c[k] = row % M[k]; row /= M[k]; }
+    }
+    int row_of(const int c[3]) const {
+        int r = 0;
+        for (int k = 0; k < D; k++) r = r * M[k] + c[k];
+        return r;
+    }
+    // base row of the vertex reached from base row `row` by offset `off`
+    int target(int row, const int32_t *off) const {
+        int c[3];
+        coords(row, c);
+        for (int k = 0; k < D; k++) c[k] = (int)(mod_n((int64_t)c[k] + off[k], N[k]) % M[k]);
+        return row_of(c);
+    }
+    const int32_t *at(const std::vector<int32_t> &t, int row, int slot) const {
+        return t.data() + ((size_t)row * d + slot) * D;
+    }
+    int32_t *at(std::vector<int32_t> &t, int row, int slot) const {
+        return t.data() + ((size_t)row * d + slot) * D;
+    }
+    bool zero(const int32_t *o) const {
+        for (int k = 0; k < D; k++)
+            if (mod_n(o[k], N[k]) != 0) return false;
+        return true;
+    }
+    bool same(const int32_t *o, const int32_t *q) const {
+        for (int k = 0; k < D; k++)
+            if (mod_n((int64_t)o[k] - q[k], N[k]) != 0) return false;
+        return true;
+    }
+    bool reverse(const int32_t *o, const int32_t *q) const {  // o == -q per axis
+        for (int k = 0; k < D; k++)
+            if (mod_n((int64_t)o[k] + q[k], N[k]) != 0) return false;
+        return true;
+    }
+    // slot of `row` holding -diff seen from (from, slot), the same slot skipped
+    // (pieceannealing.cpp:127-133); -1 when there is none
+    int partner(const std::vector<int32_t> &t, int row, int from, int slot,
+                const int32_t *diff) const {
+        for (int i = 0; i < d; i++) {
+            if (row == from && i == slot) continue;
+            if (reverse(at(t, row, i), diff)) return i;
+        }
+        return -1;
+    }
+    // nonzero offsets, pairwise distinct (no self-loop or multi-edge in any copy)
+    bool row_simple(const std::vector<int32_t> &t, int r) const {
+        for (int i = 0; i < d; i++) {
+            if (zero(at(t, r, i))) return false;
+            for (int j = 0; j < i; j++)
+                if (same(at(t, r, i), at(t, r, j))) return false;
+        }
+        return true;
+    }
+    // every offset of row r has its reverse in the row it leads to
+    bool row_symmetric(const std::vector<int32_t> &t, int r) const {
+        for (int i = 0; i < d; i++)
+            if (partner(t, target(r, at(t, r, i)), r, i, at(t, r, i)) < 0) return false;
+        return true;
+    }
+    // new offset (to - from) + M*k along every axis, k drawn per axis (x first)
+    void lift(int from, int to, gg::Rng &rng, int32_t *out, int *kpack) const {
+        int cf[3], ct[3];
+        coords(from, cf);
+        coords(to, ct);
+        int kp = 0;
+        for (int k = 0; k < D; k++) {
+            const int kk = (int)rng.below(g[k]);
+            out[k] = (ct[k] - cf[k]) + M[k] * kk;
+            kp = kp * g[k] + kk;
+        }
+        *kpack = kp;
+    }
+};
+
+// One proposal attempt on a piece / box table (pieceannealing.cpp:113-204,
+// boxHillClimb.cpp:113-220): coin 0, when every axis has more than one copy,
+// re-lifts one edge orbit; otherwise orbit 2-opt (a,b),(c,d) -> (a,d),(b,c)
+// with fresh lifts.  The lifted pair keeps the reference's orientation (no
+// from <= to reordering, unlike the 1-D form).  Redrawn unless the four
+// touched rows stay simple and symmetric.
+bool propose_lattice(const Lattice &L, const std::vector<int32_t> &cur, gg::Rng &rng,
+                     std::vector<int32_t> &out, OrbitMove *mv) {
+    const int d = L.d;
+    bool lengths = true;
+    for (int k = 0; k < L.D; k++) lengths = lengths && L.g[k] > 1;
+    const int coin = (int)rng.below(2);
+    out = cur;
+    if (lengths && coin == 0) {
+        const int from = (int)rng.below(L.Mtot), ifrom = (int)rng.below(d);
+        const int32_t *diff = L.at(cur, from, ifrom);
+        const int to = L.target(from, diff);
+        const int ito = L.partner(cur, to, from, ifrom, diff);
+        if (ito < 0) return false;
+        int32_t nd[3];
+        int kp = 0;
+        L.lift(from, to, rng, nd, &kp);
+        for (int k = 0; k < L.D; k++) {
+            L.at(out, from, ifrom)[k] = nd[k];
+            L.at(out, to, ito)[k] = -nd[k];
+        }
+        *mv = OrbitMove{0, from, ifrom, to, ito, kp, 0};
+        return L.row_simple(out, from) && L.row_simple(out, to) && L.row_symmetric(out, from) &&
+               L.row_symmetric(out, to);
+    }
+    const int a = (int)rng.below(L.Mtot), ia = (int)rng.below(d);
+    const int c = (int)rng.below(L.Mtot), ic = (int)rng.below(d);
+    const int32_t *dab = L.at(cur, a, ia), *dcd = L.at(cur, c, ic);
+    const int b = L.target(a, dab), dd = L.target(c, dcd);
+    const int ib = L.partner(cur, b, a, ia, dab);
+    const int id = L.partner(cur, dd, c, ic, dcd);
+    if (ib < 0 || id < 0) return false;
+    if ((c == a && ic == ia) || (c == b && ic == ib)) return false;  // the same edge orbit
+    int32_t dad[3], dbc[3];
+    int ka = 0, kb = 0;
+    L.lift(a, dd, rng, dad, &ka);
+    L.lift(b, c, rng, dbc, &kb);
+    *mv = OrbitMove{1, a, ia, c, ic, ka, kb};
+    for (int k = 0; k < L.D; k++) {
+        L.at(out, a, ia)[k] = dad[k];
+        L.at(out, dd, id)[k] = -dad[k];
+        L.at(out, b, ib)[k] = dbc[k];
+        L.at(out, c, ic)[k] = -dbc[k];
+    }
+    const int rows[4] = {a, dd, b, c};
+    for (int r : rows)
+        if (!L.row_simple(out, r) || !L.row_symmetric(out, r)) return false;
+    return true;
+}
+
+Lattice lattice_of(const gg_orbit_search *s) {
+    Lattice L{};
+    L.D = s->dims;
+    L.d = s->d;
+    L.Mtot = 1;
+    for (int k = 0; k < 3; k++) {
+        L.N[k] = k < s->dims ? s->Ns[k] : 1;
+        L.M[k] = k < s->dims ? s->Ms[k] : 1;
+        L.g[k] = L.N[k] / L.M[k];
+        if (k < s->dims) L.Mtot *= L.M[k];
+    }
+    return L;
+}
+
 bool better(const gg_result &x, const gg_result &y) {
     bool xo = x.status == GG_OK, yo = y.status == GG_OK;
     if (xo != yo) return xo;
@@ -135,6 +294,8 @@ bool better(const gg_result &x, const gg_result &y) {
 
 int eval_table(gg_orbit_search *s, const int32_t *t, gg_result *out) {
     if (s->use_cb) return s->cb.eval(s->cb.ctx, t, out);
+    if (s->dims == 2) return gg_eval_piece(s->ev, t, out);
+    if (s->dims == 3) return gg_eval_box(s->ev, t, out);
     return gg_eval_part(s->ev, t, out);
 }
 
@@ -148,9 +309,28 @@ int validate(int N, int M, int d, const int32_t *table, const gg_search_cfg *cfg
     return GG_OK;
 }
 
+int validate_lattice(int dims, const int32_t *Ns, const int32_t *Ms, int d,
+                     const int32_t *table, const gg_search_cfg *cfg) {
+    if (dims < 2 || dims > 3 || !Ns || !Ms) return GG_BAD_ARG;
+    int64_t N = 1, M = 1;
+    for (int k = 0; k < dims; k++) {
+        if (Ms[k] < 1 || Ns[k] < 1 || Ns[k] % Ms[k] != 0) return GG_BAD_ARG;
+        N *= Ns[k];
+        M *= Ms[k];
+    }
+    if (N > INT32_MAX) return GG_BAD_ARG;
+    return validate((int)N, (int)M, d, table, cfg);
+}
+
+void set_lattice(gg_orbit_search *s, int dims, const int32_t *Ns, const int32_t *Ms) {
+    s->dims = dims;
+    for (int k = 0; k < dims; k++) { s->Ns[k] = Ns[k]; s->Ms[k] = Ms[k]; }
+}
+
 int init(gg_orbit_search *s, int N, int M, int d, const int32_t *table, const gg_search_cfg *cfg) {
     s->N = N; s->M = M; s->d = d; s->g = N / M; s->cfg = *cfg;
-    s->tab.assign(table, table + (size_t)M * d);
+    if (s->dims == 1) { s->Ns[0] = N; s->Ms[0] = M; }
+    s->tab.assign(table, table + (size_t)M * d * s->dims);
     s->rng.reset(new gg::Rng(cfg->seed));
     gg_result r;
     int st = eval_table(s, s->tab.data(), &r);
@@ -165,8 +345,14 @@ int orbit_step(gg_orbit_search *s) {
     gg::Rng &rng = *s->rng;
     OrbitMove mv{};
     int tries = 0;
-    while (!gg::propose_orbit(s->tab, s->N, s->M, s->d, rng, s->cand, &mv))
-        if (++tries > 1000000) return GG_BAD_ARG;  // no simple move exists
+    if (s->dims == 1) {
+        while (!gg::propose_orbit(s->tab, s->N, s->M, s->d, rng, s->cand, &mv))
+            if (++tries > 1000000) return GG_BAD_ARG;  // no simple move exists
+    } else {
+        const Lattice L = lattice_of(s);
+        while (!propose_lattice(L, s->tab, rng, s->cand, &mv))
+            if (++tries > 1000000) return GG_BAD_ARG;
+    }
     s->stats.proposals++;
     gg_result y;
     int st = eval_table(s, s->cand.data(), &y);
@@ -234,6 +420,59 @@ extern "C" int gg_orbit_search_create_cb(int N, int M, int d, const int32_t *tab
     s->cb = *cb;
     s->use_cb = true;
     st = init(s, N, M, d, table, cfg);
+    if (st != GG_OK) { gg_orbit_search_destroy(s); return st; }
+    *out = s;
+    return GG_OK;
+}
+
+static int lattice_create(int dims, const int32_t *Ns, const int32_t *Ms, int d,
+                          const int32_t *table, const gg_search_cfg *cfg, int device,
+                          gg_orbit_search **out) {
+    if (!out) return GG_BAD_ARG;
+    *out = nullptr;
+    int st = validate_lattice(dims, Ns, Ms, d, table, cfg);
+    if (st != GG_OK) return st;
+    gg_orbit_search *s = new gg_orbit_search();
+    set_lattice(s, dims, Ns, Ms);
+    const int N = Ns[0] * Ns[1] * (dims == 3 ? Ns[2] : 1);
+    const int M = Ms[0] * Ms[1] * (dims == 3 ? Ms[2] : 1);
+    st = dims == 2 ? gg_create_piece(Ns[0], Ns[1], Ms[0], Ms[1], d, device, &s->ev)
+                   : gg_create_box(Ns[0], Ns[1], Ns[2], Ms[0], Ms[1], Ms[2], d, device, &s->ev);
+    if (st == GG_OK) st = init(s, N, M, d, table, cfg);
+    if (st != GG_OK) { gg_orbit_search_destroy(s); return st; }
+    *out = s;
+    return GG_OK;
+}
+
+extern "C" int gg_piece_search_create(int Nx, int Ny, int Mx, int My, int degree,
+                                      const int32_t *table, const gg_search_cfg *cfg, int device,
+                                      gg_orbit_search **out) {
+    const int32_t Ns[2] = {Nx, Ny}, Ms[2] = {Mx, My};
+    return lattice_create(2, Ns, Ms, degree, table, cfg, device, out);
+}
+
+extern "C" int gg_box_search_create(int Nx, int Ny, int Nz, int Mx, int My, int Mz, int degree,
+                                    const int32_t *table, const gg_search_cfg *cfg, int device,
+                                    gg_orbit_search **out) {
+    const int32_t Ns[3] = {Nx, Ny, Nz}, Ms[3] = {Mx, My, Mz};
+    return lattice_create(3, Ns, Ms, degree, table, cfg, device, out);
+}
+
+extern "C" int gg_lattice_search_create_cb(int dims, const int32_t *Ns, const int32_t *Ms,
+                                           int degree, const int32_t *table,
+                                           const gg_search_cfg *cfg,
+                                           const gg_table_callbacks *cb, gg_orbit_search **out) {
+    if (!out || !cb || !cb->eval) return GG_BAD_ARG;
+    *out = nullptr;
+    int st = validate_lattice(dims, Ns, Ms, degree, table, cfg);
+    if (st != GG_OK) return st;
+    gg_orbit_search *s = new gg_orbit_search();
+    set_lattice(s, dims, Ns, Ms);
+    s->cb = *cb;
+    s->use_cb = true;
+    int N = 1, M = 1;
+    for (int k = 0; k < dims; k++) { N *= Ns[k]; M *= Ms[k]; }
+    st = init(s, N, M, degree, table, cfg);
     if (st != GG_OK) { gg_orbit_search_destroy(s); return st; }
     *out = s;
     return GG_OK;

@@ -192,6 +192,10 @@ def lib() -> C.CDLL:
                                              C.c_int, P(vp)]),
         "gg_orbit_search_create_cb": (C.c_int, [C.c_int, C.c_int, C.c_int, i32p, P(SearchCfg),
                                                 P(TableCallbacks), P(vp)]),
+        "gg_piece_search_create": (C.c_int, [C.c_int] * 5 + [i32p, P(SearchCfg), C.c_int, P(vp)]),
+        "gg_box_search_create": (C.c_int, [C.c_int] * 7 + [i32p, P(SearchCfg), C.c_int, P(vp)]),
+        "gg_lattice_search_create_cb": (C.c_int, [C.c_int, i32p, i32p, C.c_int, i32p,
+                                                  P(SearchCfg), P(TableCallbacks), P(vp)]),
         "gg_orbit_search_destroy": (None, [vp]),
         "gg_orbit_search_run": (C.c_int, [vp, C.c_int, P(SearchStats)]),
         "gg_orbit_search_table": (C.c_int, [vp, i32p, C.c_int]),
@@ -664,6 +668,61 @@ class OrbitSearch:
             self.close()
         except Exception:
             pass
+
+
+class LatticeSearch(OrbitSearch):
+    """The orbit search over 2-D (`piece`, Ns = (Nx, Ny)) or 3-D (`box`) symmetric
+    graphs: table [Mx*My(*Mz), degree, dims] of offset vectors, moves of
+    cppcode/pieceannealing.cpp:113-204 / boxHillClimb.cpp:113-220, candidates
+    evaluated by gg_eval_piece / gg_eval_box (or `table_eval`, a Python
+    function table[M, d, dims] -> (total_scaled, diameter, status))."""
+
+    def __init__(self, Ns, Ms, table, cfg: SearchCfg, device: int = 0, table_eval=None,
+                 log: int = 0):
+        dims = len(Ns)
+        assert dims in (2, 3) and len(Ms) == dims
+        M = int(np.prod(Ms))
+        t = _i32(table)
+        d = t.size // (M * dims)
+        t = t.reshape(M, d, dims)
+        self.N, self.M, self.d, self.dims = int(np.prod(Ns)), M, d, dims
+        self._h = C.c_void_p()
+        ns, ms = _i32(list(Ns)), _i32(list(Ms))
+        if table_eval is None:
+            self._cb = None
+            if dims == 2:
+                check(lib().gg_piece_search_create(Ns[0], Ns[1], Ms[0], Ms[1], d, _ptr(t),
+                                                   C.byref(cfg), device, C.byref(self._h)),
+                      "gg_piece_search_create")
+            else:
+                check(lib().gg_box_search_create(Ns[0], Ns[1], Ns[2], Ms[0], Ms[1], Ms[2], d,
+                                                 _ptr(t), C.byref(cfg), device,
+                                                 C.byref(self._h)), "gg_box_search_create")
+        else:
+            N = self.N
+
+            def ev(ctx, tab, out):
+                a = np.ctypeslib.as_array(tab, shape=(M * d * dims,)).reshape(M, d, dims).copy()
+                total, diam, status = table_eval(a)
+                r = out.contents
+                r.total_reps, r.total_scaled, r.diameter, r.wvc, r.levels, r.status = (
+                    total // (N // M), total, diam, 0, diam + 1, status)
+                return 0
+
+            self._fn = TABLE_CB(ev)
+            self._cb = TableCallbacks(None, self._fn)
+            check(lib().gg_lattice_search_create_cb(dims, _ptr(ns), _ptr(ms), d, _ptr(t),
+                                                    C.byref(cfg), C.byref(self._cb),
+                                                    C.byref(self._h)),
+                  "gg_lattice_search_create_cb")
+        if log:
+            check(lib().gg_orbit_search_set_log(self._h, log), "gg_orbit_search_set_log")
+
+    def table(self, best=False) -> np.ndarray:
+        out = np.empty((self.M, self.d, self.dims), dtype=np.int32)
+        check(lib().gg_orbit_search_table(self._h, _ptr(out), 1 if best else 0),
+              "gg_orbit_search_table")
+        return out
 
 
 class PyEvaluatorCallbacks:

@@ -117,9 +117,30 @@ int gg_orbit_search_run(gg_orbit_search *s, int steps, gg_search_stats *out);
 int gg_orbit_search_table(gg_orbit_search *s, int32_t *table_out, int best);
 /* Per-step log: 10 int64 per step {step, a*degree+slot_a, c*degree+slot_c,
  * k1, k2, kind (0 = length change, 1 = 2-opt), total, diameter, status,
- * accepted} (total = diameter = -1 when the status is not GG_OK). */
+ * accepted} (total = diameter = -1 when the status is not GG_OK).  Lattice
+ * searches pack a lift vector as k = (kx*gy + ky)*gz + kz. */
 int gg_orbit_search_set_log(gg_orbit_search *s, int capacity);
 int gg_orbit_search_get_log(gg_orbit_search *s, int64_t *buf, int capacity, int *count);
+
+/* The same search over 2-D (`piece`, cppcode/piece.hpp:6-53) and 3-D (`box`,
+ * cppcode/box.hpp:6-81) symmetric graphs, replacing the move generator and
+ * loop of pieceannealing.cpp / pieceHillClimb.cpp (createNeighbour
+ * pieceannealing.cpp:113-204, acceptance :252-260) and boxHillClimb.cpp
+ * (:113-220, :252-273) with the evaluators of gg_eval_piece / gg_eval_box.
+ * table = [Mx*My(*Mz)][degree][2 or 3] offsets in the evaluator's layout.
+ * Edge-length changes are drawn only when every axis has more than one copy
+ * (pieceannealing.cpp:115); lifts are drawn per axis, x first; a move whose
+ * touched rows are not simple and symmetric is redrawn.  The handle is a
+ * gg_orbit_search: run / table / log / destroy are shared. */
+int gg_piece_search_create(int Nx, int Ny, int Mx, int My, int degree, const int32_t *table,
+                           const gg_search_cfg *cfg, int device, gg_orbit_search **out);
+int gg_box_search_create(int Nx, int Ny, int Nz, int Mx, int My, int Mz, int degree,
+                         const int32_t *table, const gg_search_cfg *cfg, int device,
+                         gg_orbit_search **out);
+/* dims = 2 or 3; Ns / Ms per axis; the caller's exact table evaluator. */
+int gg_lattice_search_create_cb(int dims, const int32_t *Ns, const int32_t *Ms, int degree,
+                                const int32_t *table, const gg_search_cfg *cfg,
+                                const gg_table_callbacks *cb, gg_orbit_search **out);
 
 /* ---- seeded instances ----------------------------------------------------
  * gg_gen_random_regular: simple d-regular graph on n vertices as an n x d

@@ -200,8 +200,129 @@ def propose_orbit(t, N, rng):
     return rec, out
 
 
+# ---- 2-D / 3-D symmetric graphs (piece, box): pieceannealing.cpp:113-204,
+# boxHillClimb.cpp:113-220 -------------------------------------------------------------
+
+class Lattice:
+    """Per-axis orders Ns and base extents Ms; rows mixed-radix, axis 0 major."""
+
+    def __init__(self, Ns, Ms, d):
+        self.D, self.N, self.M, self.d = len(Ns), list(Ns), list(Ms), d
+        self.g = [n // m for n, m in zip(Ns, Ms)]
+        self.Mtot = int(np.prod(Ms))
+
+    def coords(self, row):
+        c = [0] * self.D
+        for k in reversed(range(self.D)):
+            c[k] = row % self.M[k]
+            row //= self.M[k]
+        return c
+
+    def row_of(self, c):
+        r = 0
+        for k in range(self.D):
+            r = r * self.M[k] + c[k]
+        return r
+
+    def target(self, row, off):
+        c = self.coords(row)
+        return self.row_of([((c[k] + int(off[k])) % self.N[k]) % self.M[k] for k in range(self.D)])
+
+    def reverse(self, o, q):
+        return all((int(o[k]) + int(q[k])) % self.N[k] == 0 for k in range(self.D))
+
+    def partner(self, t, row, frm, slot, diff):
+        for i in range(self.d):
+            if row == frm and i == slot:
+                continue
+            if self.reverse(t[row, i], diff):
+                return i
+        return -1
+
+    def row_simple(self, t, r):
+        xs = [tuple(int(t[r, i, k]) % self.N[k] for k in range(self.D)) for i in range(self.d)]
+        return all(any(x) for x in xs) and len(set(xs)) == len(xs)
+
+    def row_symmetric(self, t, r):
+        return all(self.partner(t, self.target(r, t[r, i]), r, i, t[r, i]) >= 0
+                   for i in range(self.d))
+
+    def lift(self, frm, to, rng):
+        cf, ct = self.coords(frm), self.coords(to)
+        out, kp = [], 0
+        for k in range(self.D):
+            kk = rng.below(self.g[k])
+            out.append((ct[k] - cf[k]) + self.M[k] * kk)
+            kp = kp * self.g[k] + kk
+        return np.array(out, dtype=np.int32), kp
+
+
+def propose_lattice(t, L, rng):
+    """One attempt on a [Mtot, d, D] table; (move record, new table) or None."""
+    d = L.d
+    lengths = all(g > 1 for g in L.g)
+    coin = rng.below(2)
+    out = t.copy()
+    if lengths and coin == 0:
+        frm, ifrm = rng.below(L.Mtot), rng.below(d)
+        diff = t[frm, ifrm].copy()
+        to = L.target(frm, diff)
+        ito = L.partner(t, to, frm, ifrm, diff)
+        if ito < 0:
+            return None
+        nd, kp = L.lift(frm, to, rng)
+        out[frm, ifrm] = nd
+        out[to, ito] = -nd
+        if not all(L.row_simple(out, r) and L.row_symmetric(out, r) for r in (frm, to)):
+            return None
+        return (frm * d + ifrm, to * d + ito, kp, 0, 0), out
+    a, ia = rng.below(L.Mtot), rng.below(d)
+    c, ic = rng.below(L.Mtot), rng.below(d)
+    dab, dcd = t[a, ia].copy(), t[c, ic].copy()
+    b, dd = L.target(a, dab), L.target(c, dcd)
+    ib = L.partner(t, b, a, ia, dab)
+    idd = L.partner(t, dd, c, ic, dcd)
+    if ib < 0 or idd < 0:
+        return None
+    if (c == a and ic == ia) or (c == b and ic == ib):
+        return None
+    dad, ka = L.lift(a, dd, rng)
+    dbc, kb = L.lift(b, c, rng)
+    out[a, ia] = dad
+    out[dd, idd] = -dad
+    out[b, ib] = dbc
+    out[c, ic] = -dbc
+    if not all(L.row_simple(out, r) and L.row_symmetric(out, r) for r in (a, dd, b, c)):
+        return None
+    return (a * d + ia, c * d + ic, ka, kb, 1), out
+
+
+def random_lattice_table(Ns, Ms, d, seed):
+    """Seeded simple symmetric table [Mtot, d, D]: the Mtot*d stubs paired at
+    random, each pair an edge orbit with a uniform lift per axis, resampled
+    until no self-loop, half-orbit or repeated offset remains."""
+    L = Lattice(Ns, Ms, d)
+    rng = np.random.default_rng(seed)
+    for _ in range(10000):
+        stubs = rng.permutation(L.Mtot * d)
+        t = np.zeros((L.Mtot, d, L.D), dtype=np.int32)
+        fill = [0] * L.Mtot
+        for i in range(0, len(stubs), 2):
+            r1, r2 = int(stubs[i]) // d, int(stubs[i + 1]) // d
+            c1, c2 = L.coords(r1), L.coords(r2)
+            off = np.array([(c2[k] - c1[k]) + L.M[k] * int(rng.integers(0, L.g[k]))
+                            for k in range(L.D)], dtype=np.int32)
+            t[r1, fill[r1]] = off
+            fill[r1] += 1
+            t[r2, fill[r2]] = -off
+            fill[r2] += 1
+        if all(L.row_simple(t, r) and L.row_symmetric(t, r) for r in range(L.Mtot)):
+            return t
+    raise RuntimeError("no simple table found")
+
+
 def run_orbit(N, table, evaluate, steps, rule="hill", T0=1e-3, T1=1e-6, total_steps=None,
-              seed=1):
+              seed=1, propose=None):
     """evaluate(table) -> (total_scaled, diameter, ok).  Returns the per-step
     log rows {step, a*d+slot_a, c*d+slot_c, k1, k2, kind, total, diameter,
     status, accepted} and the final table."""
@@ -213,7 +334,7 @@ def run_orbit(N, table, evaluate, steps, rule="hill", T0=1e-3, T1=1e-6, total_st
     log = []
     for step in range(1, steps + 1):
         while True:
-            p = propose_orbit(t, N, rng)
+            p = propose(t, rng) if propose else propose_orbit(t, N, rng)
             if p is not None:
                 break
         rec, new = p

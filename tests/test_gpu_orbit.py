@@ -1,4 +1,4 @@
-"""Orbit-graph search on the GPU evaluator, move for move against the same
+"""Orbit-graph searches on the GPU evaluators, move for move against the same
 driver on the CPU oracle: the cyclically symmetric configuration n = g*m with
 g = 32 copies of m = 128 base vertices, degree 16 (BFS from the 128
 representatives, total scaled by 32), hill climbing over 10^3 orbit moves
@@ -36,3 +36,47 @@ def test_orbit_search_gpu_matches_cpu_twin(gpu, rule, steps):
     tot, dia, status = oracle_eval.evaluate_part(N, final)
     assert status == 0 and tot == st.total_cur and dia == st.diam_cur
     assert log[:, 9].sum() > 0  # the run moved
+
+
+def _lattice_eval(Ns, Ms):
+    import oracle_lib as ol
+
+    def f(t):
+        if len(Ns) == 2:
+            r = ol.piece_eval(Ns[0], Ns[1], Ms[0], Ms[1], t)
+        else:
+            r = ol.box_eval(Ns[0], Ns[1], Ns[2], Ms[0], Ms[1], Ms[2], t)
+        ok = not r["unconnected"]
+        return (r["total_scaled"] if ok else 0), (r["diameter"] if ok else 0), (0 if ok else 1)
+    return f
+
+
+@pytest.mark.parametrize("Ns,Ms,d,rule,steps", [
+    ((16, 64), (4, 8), 8, "hill", 300),        # piece (pieceHillClimb.cpp)
+    ((16, 64), (4, 8), 8, "anneal", 200),      # piece (pieceannealing.cpp)
+    ((32, 128), (1, 128), 16, "hill", 200),    # the g = 32, m = 128 orbit graph as a piece
+    ((8, 8, 8), (2, 2, 2), 6, "hill", 200),    # box (boxHillClimb.cpp)
+])
+def test_lattice_search_gpu_matches_cpu_twin(gpu, Ns, Ms, d, rule, steps):
+    """Piece / box search (gg_piece_search_create / gg_box_search_create) on
+    the GPU evaluators, move for move against the same driver on the oracle."""
+    import driver_ref
+    gg = gpu
+    tab = driver_ref.random_lattice_table(Ns, Ms, d, 7)
+    cfg = gg.search_cfg(mode=gg.MODE_PLAIN,
+                        rule=gg.RULE_HILL if rule == "hill" else gg.RULE_ANNEAL,
+                        total_steps=steps, T0=1e-3, T1=1e-6, seed=11)
+    print("lattice search", Ns, Ms, d, rule, steps, flush=True)
+    with gg.LatticeSearch(Ns, Ms, tab, cfg, log=steps) as s:
+        st = s.run(steps)
+        log, final = s.log(), s.table()
+    ev = _lattice_eval(Ns, Ms)
+    with gg.LatticeSearch(Ns, Ms, tab, cfg, log=steps, table_eval=ev) as c:
+        sc = c.run(steps)
+        clog, cfinal = c.log(), c.table()
+    assert np.array_equal(log, clog)
+    assert np.array_equal(final, cfinal)
+    assert (st.total_cur, st.diam_cur, st.accepted) == (sc.total_cur, sc.diam_cur, sc.accepted)
+    tot, dia, status = ev(final)
+    assert status == 0 and tot == st.total_cur and dia == st.diam_cur
+    assert log[:, 9].sum() > 0

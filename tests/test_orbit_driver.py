@@ -74,3 +74,73 @@ def test_orbit_search_rejects_other_modes(gg):
     cfg = gg.search_cfg(mode=gg.MODE_BATCHED, K=4, total_steps=10)
     with pytest.raises(gg.GGError):
         gg.OrbitSearch(64, tab, cfg, table_eval=_tab_eval(64))
+
+
+def _lattice_eval(Ns, Ms):
+    def f(t):
+        if len(Ns) == 2:
+            r = ol.piece_eval(Ns[0], Ns[1], Ms[0], Ms[1], t)
+        else:
+            r = ol.box_eval(Ns[0], Ns[1], Ns[2], Ms[0], Ms[1], Ms[2], t)
+        ok = not r["unconnected"]
+        return (r["total_scaled"] if ok else 0), (r["diameter"] if ok else 0), (0 if ok else 1)
+    return f
+
+
+def _check_lattice_symmetric_simple(L, t):
+    for r in range(L.Mtot):
+        assert L.row_simple(t, r) and L.row_symmetric(t, r), r
+
+
+@pytest.mark.parametrize("Ns,Ms,d,rule", [
+    ((8, 12), (2, 3), 4, "hill"),      # piece, both axes lifted: length changes too
+    ((8, 12), (2, 3), 4, "anneal"),
+    ((6, 9), (1, 9), 4, "hill"),       # piece with one axis unlifted: 2-opt only
+    ((4, 4, 6), (2, 1, 2), 4, "hill"),  # box
+    ((4, 6, 4), (2, 2, 2), 3, "anneal"),
+])
+def test_lattice_driver_matches_python_restatement(gg, Ns, Ms, d, rule):
+    """pieceannealing.cpp:113-260 / boxHillClimb.cpp:113-273 moves and
+    acceptance: the C++ driver and the restatement log the same moves and
+    results step for step on the CPU oracle."""
+    L = driver_ref.Lattice(Ns, Ms, d)
+    tab = driver_ref.random_lattice_table(Ns, Ms, d, 5)
+    steps = 60
+    cfg = gg.search_cfg(mode=gg.MODE_PLAIN,
+                        rule=gg.RULE_HILL if rule == "hill" else gg.RULE_ANNEAL,
+                        total_steps=steps, T0=2e-2, T1=1e-4, seed=17)
+    ev = _lattice_eval(Ns, Ms)
+    N = int(np.prod(Ns))
+    with gg.LatticeSearch(Ns, Ms, tab, cfg, table_eval=ev, log=steps) as s:
+        st = s.run(steps)
+        log = s.log()
+        final = s.table()
+    ref_log, ref_tab = driver_ref.run_orbit(
+        N, tab, lambda t: (lambda r: (r[0], r[1], r[2] == 0))(ev(t)), steps, rule=rule,
+        T0=2e-2, T1=1e-4, seed=17, propose=lambda t, rng: driver_ref.propose_lattice(t, L, rng))
+    assert np.array_equal(log, ref_log)
+    assert np.array_equal(final, ref_tab)
+    assert st.steps == steps and st.exact_evals == steps
+    kinds = set(ref_log[:, 5].tolist())
+    assert kinds == ({0, 1} if all(g > 1 for g in L.g) else {1})
+    _check_lattice_symmetric_simple(L, final)
+    tot, dia, status = ev(final)
+    assert status == 0 and st.total_cur == tot and st.diam_cur == dia
+    if rule == "hill":
+        t0, d0, _ = ev(tab)
+        assert (st.diam_cur, st.total_cur) <= (d0, t0)
+
+
+def test_lattice_generator_tables_are_symmetric_simple():
+    for Ns, Ms, d in (((8, 12), (2, 3), 4), ((4, 4, 6), (2, 1, 2), 4)):
+        L = driver_ref.Lattice(Ns, Ms, d)
+        t = driver_ref.random_lattice_table(Ns, Ms, d, 1)
+        _check_lattice_symmetric_simple(L, t)
+        # expanded graph: every vertex has d distinct neighbours, undirected
+        if len(Ns) == 2:
+            adj = ol.expand_piece(Ns[0], Ns[1], Ms[0], Ms[1], t)
+        else:
+            adj = ol.expand_box(*Ns, *Ms, t)
+        e = {(v, int(u)) for v in range(adj.shape[0]) for u in adj[v]}
+        assert all((u, v) in e for v, u in e)
+        assert all(len(set(adj[v].tolist())) == d and v not in adj[v] for v in range(adj.shape[0]))
