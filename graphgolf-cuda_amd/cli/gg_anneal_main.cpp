@@ -6,11 +6,14 @@
 //
 //   gg-anneal [-c COUNT] [-s SEED] [-i IN] [-o OUT] [-l LOG] [-v VERBOSE] [-D DEVICE]
 //             [-t TEMP] [--hill] [--mode plain|batched|learned] [-K K] [-P P]
-//             [--schedule step|geometric] [--temp-end T]
+//             [--schedule step|geometric] [--temp-end T] [-f part|piece|box]
 //
-// Input and output are `part` tables (part.hpp:13-43).  M = N: an explicit
-// graph, 2-opt swaps (gg_search); M < N: a cyclically symmetric graph, orbit
-// moves (gg_orbit_search).  TEMP is the reference's initial temperature in
+// Input and output are `part` tables (part.hpp:13-43) by default.  M = N: an
+// explicit graph, 2-opt swaps (gg_search); M < N: a cyclically symmetric
+// graph, orbit moves (gg_orbit_search).  -f piece / -f box read and write the
+// 2-D / 3-D forms (piece.hpp:18-52, box.hpp:37-80) and search them with the
+// moves of pieceannealing.cpp / boxHillClimb.cpp (gg_piece_search_create,
+// gg_box_search_create; plain mode).  TEMP is the reference's initial temperature in
 // total-distance units (default 46.1793, annealing.cpp:314); the default
 // schedule is the reference's step schedule (annealing.cpp:442).  The moves
 // are this library's (gg_search.cpp / gg_orbit.cpp), so a run does not
@@ -39,7 +42,7 @@ namespace {
 int usage(int code) {
     std::cout << "gg-anneal [-c COUNT] [-s SEED] [-i IN] [-o OUT] [-l LOG] [-v VERBOSE] [-D DEVICE]\n"
                  "          [-t TEMP] [--hill] [--mode plain|batched|learned] [-K K] [-P P]\n"
-                 "          [--schedule step|geometric] [--temp-end T]\n";
+                 "          [--schedule step|geometric] [--temp-end T] [-f part|piece|box]\n";
     return code;
 }
 
@@ -54,6 +57,7 @@ int main(int argc, char **argv) {
     unsigned int seed = 0;
     double inittemp = 46.1793, temp_end = -1;
     std::string in_path, out_path, log_path, verbose_path, mode = "plain", schedule = "step";
+    std::string fmt = "part";
     for (int i = 1; i < argc; i++) {
         std::string a = argv[i];
         auto next = [&]() -> std::string {
@@ -75,6 +79,7 @@ int main(int argc, char **argv) {
         else if (a == "-P") P = std::stoi(next());
         else if (a == "--schedule") schedule = next();
         else if (a == "--temp-end") temp_end = std::stod(next());
+        else if (a == "-f" || a == "--format") fmt = next();
         else { std::cout << "unknown option " << a << "\n"; return usage(1); }
     }
     std::ifstream inputfs;
@@ -109,20 +114,95 @@ int main(int argc, char **argv) {
         if (!in_path.empty()) *f << "#input file: " << in_path << std::endl;
         if (!out_path.empty()) *f << "#output file: " << out_path << std::endl;
     }
+    if (fmt != "part" && fmt != "piece" && fmt != "box") return usage(1);
+    // the graph as a flat table: part [M][d]; piece / box [M][d][dims] in the
+    // evaluators' layout (cudaASPLpiece.cu:184-195)
     graphgolf::part init;
-    init.load(ist);
-    if (!init.regular) {
+    graphgolf::piece init_piece;
+    graphgolf::box init_box;
+    int dims = 1, N = 0, M = 0, d = 0;
+    bool regular = true;
+    std::vector<int32_t> tab;
+    if (fmt == "part") {
+        init.load(ist);
+        regular = init.regular;
+        N = init.N; M = init.M; d = init.degree;
+        tab.resize((size_t)M * d);
+        if (regular)
+            for (int r = 0; r < M; r++)
+                for (int j = 0; j < d; j++) tab[(size_t)r * d + j] = init.edges[r][j];
+    } else if (fmt == "piece") {
+        init_piece.load(ist);
+        regular = init_piece.regular;
+        dims = 2;
+        N = init_piece.Nx * init_piece.Ny; M = init_piece.Mx * init_piece.My;
+        d = init_piece.degree;
+        tab.resize((size_t)M * d * 2);
+        if (regular)
+            for (int x = 0; x < init_piece.Mx; x++)
+                for (int y = 0; y < init_piece.My; y++)
+                    for (int j = 0; j < d; j++) {
+                        const size_t o = (((size_t)x * init_piece.My + y) * d + j) * 2;
+                        tab[o] = init_piece.edges[x][y][j].first;
+                        tab[o + 1] = init_piece.edges[x][y][j].second;
+                    }
+    } else {
+        init_box.load(ist);
+        regular = init_box.regular;
+        dims = 3;
+        N = init_box.N; M = init_box.M; d = init_box.degree;
+        tab.resize((size_t)M * d * 3);
+        if (regular)
+            for (int x = 0; x < init_box.Mx; x++)
+                for (int y = 0; y < init_box.My; y++)
+                    for (int z = 0; z < init_box.Mz; z++)
+                        for (int j = 0; j < d; j++) {
+                            const size_t o =
+                                ((((size_t)x * init_box.My + y) * init_box.Mz + z) * d + j) * 3;
+                            const auto &e = init_box.edges[x][y][z][j];
+                            tab[o] = std::get<0>(e);
+                            tab[o + 1] = std::get<1>(e);
+                            tab[o + 2] = std::get<2>(e);
+                        }
+    }
+    if (!regular) {
         std::cout << "non-regular graph" << std::endl;
         return 1;
     }
-    if (verbose) verbosefs << "#N: " << init.N << " M: " << init.M << std::endl;
+    if (verbose) verbosefs << "#N: " << N << " M: " << M << std::endl;
     if (!have_seed) seed = std::random_device{}();
     if (verbose) verbosefs << "#seed: " << seed << std::endl;
-    const int N = init.N, M = init.M, d = init.degree;
     const double NN1 = (double)N * (N - 1);
-    std::vector<int32_t> tab((size_t)M * d);
-    for (int r = 0; r < M; r++)
-        for (int j = 0; j < d; j++) tab[(size_t)r * d + j] = init.edges[r][j];
+    // prints a table the search returns in the input's format (explicit
+    // graphs come back as neighbour ids: offsets v -> u as u - v)
+    auto print_table = [&](std::ostream &o, const std::vector<int32_t> &t) {
+        if (dims == 1) {
+            graphgolf::part p = init;
+            for (int r = 0; r < M; r++)
+                for (int j = 0; j < d; j++)
+                    p.edges[r][j] = M == N ? t[(size_t)r * d + j] - r : t[(size_t)r * d + j];
+            p.print(o);
+        } else if (dims == 2) {
+            graphgolf::piece p = init_piece;
+            for (int x = 0; x < p.Mx; x++)
+                for (int y = 0; y < p.My; y++)
+                    for (int j = 0; j < d; j++) {
+                        const size_t q = (((size_t)x * p.My + y) * d + j) * 2;
+                        p.edges[x][y][j] = std::make_pair(t[q], t[q + 1]);
+                    }
+            p.print(o);
+        } else {
+            graphgolf::box p = init_box;
+            for (int x = 0; x < p.Mx; x++)
+                for (int y = 0; y < p.My; y++)
+                    for (int z = 0; z < p.Mz; z++)
+                        for (int j = 0; j < d; j++) {
+                            const size_t q = ((((size_t)x * p.My + y) * p.Mz + z) * d + j) * 3;
+                            p.edges[x][y][z][j] = std::make_tuple(t[q], t[q + 1], t[q + 2]);
+                        }
+            p.print(o);
+        }
+    };
 
     gg_search_cfg cfg{};
     cfg.mode = mode == "learned" ? GG_MODE_LEARNED : mode == "batched" ? GG_MODE_BATCHED : GG_MODE_PLAIN;
@@ -141,10 +221,16 @@ int main(int argc, char **argv) {
 
     // evaluator of the initial solution (annealing.cpp:296-312)
     gg_eval *ev = nullptr;
-    int st = gg_create_part(N, M, d, device, &ev);
+    int st = dims == 1 ? gg_create_part(N, M, d, device, &ev)
+             : dims == 2 ? gg_create_piece(init_piece.Nx, init_piece.Ny, init_piece.Mx,
+                                           init_piece.My, d, device, &ev)
+                         : gg_create_box(init_box.Nx, init_box.Ny, init_box.Nz, init_box.Mx,
+                                         init_box.My, init_box.Mz, d, device, &ev);
     if (st != GG_OK) { std::cerr << "error: " << gg_strerror(st) << "\n"; return 1; }
     gg_result r0;
-    st = gg_eval_part(ev, tab.data(), &r0);
+    st = dims == 1 ? gg_eval_part(ev, tab.data(), &r0)
+         : dims == 2 ? gg_eval_piece(ev, tab.data(), &r0)
+                     : gg_eval_box(ev, tab.data(), &r0);
     gg_destroy(ev);
     if (st != GG_OK) { std::cerr << "error: " << gg_strerror(st) << "\n"; return 1; }
     if (r0.status != GG_OK) {
@@ -166,7 +252,15 @@ int main(int argc, char **argv) {
     gg_search *s = nullptr;
     gg_orbit_search *os = nullptr;
     std::vector<int32_t> adj;
-    if (M == N) {
+    if (dims == 2) {
+        cfg.mode = GG_MODE_PLAIN;
+        st = gg_piece_search_create(init_piece.Nx, init_piece.Ny, init_piece.Mx, init_piece.My, d,
+                                    tab.data(), &cfg, device, &os);
+    } else if (dims == 3) {
+        cfg.mode = GG_MODE_PLAIN;
+        st = gg_box_search_create(init_box.Nx, init_box.Ny, init_box.Nz, init_box.Mx, init_box.My,
+                                  init_box.Mz, d, tab.data(), &cfg, device, &os);
+    } else if (M == N) {
         adj.resize((size_t)N * d);
         for (int v = 0; v < N; v++)
             for (int j = 0; j < d; j++) {
@@ -237,15 +331,11 @@ int main(int argc, char **argv) {
         }
         done += n;
         if (improved && verbose) {
-            graphgolf::part cur = init;
-            std::vector<int32_t> out(M == N ? (size_t)N * d : (size_t)M * d);
+            std::vector<int32_t> out((size_t)M * d * dims);
             if (s) gg_search_graph(s, out.data(), 1);
             else gg_orbit_search_table(os, out.data(), 1);
-            for (int r = 0; r < M; r++)
-                for (int j = 0; j < d; j++)
-                    cur.edges[r][j] = M == N ? out[(size_t)r * d + j] - r : out[(size_t)r * d + j];
             verbosefs << "#update. new solution:" << std::endl;
-            cur.print(verbosefs);
+            print_table(verbosefs, out);
         }
     }
     const double elapsed =
@@ -254,14 +344,10 @@ int main(int argc, char **argv) {
     std::cout << std::endl << "total elapsed time: " << elapsed << "s" << std::endl;
     if (verbose) verbosefs << "#total elapsed time: " << elapsed << "s" << std::endl;
     // best solution in the input's format (offsets v -> u as u - v for M = N)
-    graphgolf::part best = init;
-    std::vector<int32_t> out(M == N ? (size_t)N * d : (size_t)M * d);
+    std::vector<int32_t> out((size_t)M * d * dims);
     if (s) gg_search_graph(s, out.data(), 1);
     else gg_orbit_search_table(os, out.data(), 1);
-    for (int r = 0; r < M; r++)
-        for (int j = 0; j < d; j++)
-            best.edges[r][j] = M == N ? out[(size_t)r * d + j] - r : out[(size_t)r * d + j];
-    best.print(ost);
+    print_table(ost, out);
     if (s) gg_search_destroy(s);
     if (os) gg_orbit_search_destroy(os);
     (void)dx;

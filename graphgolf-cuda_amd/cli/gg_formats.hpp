@@ -66,6 +66,16 @@ struct piece {
                 }
             }
     }
+    // piece.hpp:41-52
+    void print(std::ostream &out) const {
+        out << Nx << ' ' << Ny << ' ' << Mx << ' ' << My << '\n';
+        for (int x = 0; x < Mx; x++)
+            for (int y = 0; y < My; y++) {
+                out << edges[x][y].size();
+                for (const auto &e : edges[x][y]) out << ' ' << e.first << ' ' << e.second;
+                out << '\n';
+            }
+    }
 };
 
 struct box {
@@ -90,6 +100,18 @@ struct box {
                         in >> dx >> dy >> dz;
                         edges[x][y][z].emplace_back(dx, dy, dz);
                     }
+                }
+    }
+    // box.hpp:65-80
+    void print(std::ostream &out) const {
+        out << Nx << ' ' << Ny << ' ' << Nz << ' ' << Mx << ' ' << My << ' ' << Mz << '\n';
+        for (int x = 0; x < Mx; x++)
+            for (int y = 0; y < My; y++)
+                for (int z = 0; z < Mz; z++) {
+                    out << edges[x][y][z].size();
+                    for (const auto &e : edges[x][y][z])
+                        out << ' ' << std::get<0>(e) << ' ' << std::get<1>(e) << ' ' << std::get<2>(e);
+                    out << '\n';
                 }
     }
 };

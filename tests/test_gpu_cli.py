@@ -5,7 +5,7 @@ cppcode/annealing.cpp:321-324 and :415-427, and prints a best solution that
 the oracle finds connected and no worse than the input, with an ASPL no
 larger than any logged ASPL(x_best) when the diameter did not change, both for
 an explicit graph (M = N, 2-opt swaps) and a cyclically symmetric one (M < N,
-orbit moves)."""
+orbit moves); with -f piece / -f box it reads and writes the 2-D / 3-D forms."""
 import os
 import re
 import subprocess
@@ -136,3 +136,51 @@ def test_gg_anneal_learned_mode(gpu, tmp_path):
     n, d = 1024, 8
     adj = gg.random_regular(n, d, 9)
     _check_anneal(tmp_path, n, _part_of_adj(adj), ["--mode", "learned", "-K", "8", "-P", "64"])
+
+
+def _write_lattice(path, Ns, Ms, table):
+    M, d, D = table.shape
+    with open(path, "w") as f:
+        f.write(" ".join(str(x) for x in (*Ns, *Ms)) + "\n")
+        for r in range(M):
+            f.write(f"{d} " + " ".join(str(int(x)) for x in table[r].ravel()) + "\n")
+
+
+def _read_lattice(path, D):
+    with open(path) as f:
+        head = [int(x) for x in f.readline().split()]
+        rows = [[int(x) for x in line.split()] for line in f if line.strip()]
+    d = rows[0][0]
+    t = np.array([r[1:] for r in rows], dtype=np.int32).reshape(len(rows), d, D)
+    return head[:D], head[D:], t
+
+
+@pytest.mark.parametrize("fmt,Ns,Ms,d", [("piece", (16, 64), (4, 8), 8),
+                                         ("box", (8, 8, 8), (2, 2, 2), 6)])
+def test_gg_anneal_piece_and_box(gpu, tmp_path, fmt, Ns, Ms, d):
+    """-f piece / -f box: the 2-D / 3-D text formats in and out
+    (piece.hpp:18-52, box.hpp:37-80) and the pieceHillClimb / boxHillClimb
+    moves; the printed best is connected and no worse than the input."""
+    import driver_ref
+    import oracle_lib as ol
+    D = len(Ns)
+    tab = driver_ref.random_lattice_table(Ns, Ms, d, 3)
+    f, o = tmp_path / f"in.{fmt}", tmp_path / f"out.{fmt}"
+    _write_lattice(f, Ns, Ms, tab)
+
+    def ev(t):
+        r = (ol.piece_eval(*Ns, *Ms, t) if D == 2 else ol.box_eval(*Ns, *Ms, t))
+        assert not r["unconnected"]
+        return r["total_scaled"], r["diameter"]
+    N = int(np.prod(Ns))
+    tot0, dia0 = ev(tab)
+    out = _run([os.path.join(BIN, "gg-anneal"), "-f", fmt, "--hill", "-c", "400", "-s", "5",
+                "-i", str(f), "-o", str(o)])
+    init = float(re.search(r"ASPL\(init_x\): (\S+)", out).group(1))
+    assert abs(init - tot0 / (N * (N - 1))) < 1e-9
+    Ns2, Ms2, best = _read_lattice(o, D)
+    assert tuple(Ns2) == tuple(Ns) and tuple(Ms2) == tuple(Ms) and best.shape == tab.shape
+    L = driver_ref.Lattice(Ns, Ms, d)
+    assert all(L.row_simple(best, r) and L.row_symmetric(best, r) for r in range(L.Mtot))
+    tot, dia = ev(best)
+    assert (dia, tot) < (dia0, tot0)
