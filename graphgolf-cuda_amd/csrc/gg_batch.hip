@@ -365,6 +365,13 @@ static int enqueue_sparse(gg_batch *b, int K, bool reset = true) {
         c.k0 = k0;
         c.K = K - k0 < b->chunk ? K - k0 : b->chunk;
         c.ep_next = b->d_qctl + 2 + c_i;
+        // claiming grid: a workgroup per column, more (one per CU) while there
+        // are CUs left for the endpoint tasks
+        c.grid = W;
+        if (vpt > 0) {
+            const int want = W + 4 * c.K;
+            c.grid = want < b->dc.cu_count ? want : (W > b->dc.cu_count ? W : b->dc.cu_count);
+        }
         hipError_t e = launch_sparse(c, b->dc.ringq, qa, lds, vpt, b->stream);
         if (e != hipSuccess) {
             gg_set_last_hip_error(e, "k_sparse", __FILE__, __LINE__);

@@ -39,6 +39,7 @@
 struct SparseArgs {
     int N, D, W, K, lmax;
     int k0;                          // first candidate of this launch (swaps/results offset)
+    int grid;                        // workgroups of the dense-tail instances (>= W)
     const int *adjR;                 // [N][D] base neighbours, row-major
     const gg_swap *swaps;            // [k0 + K]
     const unsigned long long *snap;  // [S][W][N] base words after levels 1..S
@@ -124,8 +125,9 @@ __host__ __device__ inline size_t sparse_lds_bytes(int N, int skcap) {
 // VPT > 0: after its column, each workgroup runs dense BFS tasks of the
 // columns that hold a swap endpoint as a source (candidate k, endpoint j ->
 // task 4k+j, claimed from a.ep_next; the list follows from the swaps alone),
-// with the fixed-degree dense column of `qa` in the same LDS.  VPT = 0 queues
-// those columns for k_dense_lds_queue instead.
+// with the fixed-degree dense column of `qa` in the same LDS; workgroups
+// beyond the W columns (launched on spare CUs) start on those tasks at once.
+// VPT = 0 queues those columns for k_dense_lds_queue instead.
 template <int BS, int DT, int VPT>
 __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a, TopoRingQ topo, DenseArgs qa) {
     extern __shared__ unsigned long long Bprev[];
@@ -148,8 +150,11 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a, TopoRingQ topo, Den
     __shared__ int s_next;
     __shared__ int s_any;
     const int tid = threadIdx.x, lane = tid & 63;
-    const int w = blockIdx.x;
     const int N = a.N, D = a.D, W = a.W, K = a.K;
+    // workgroups beyond one per column (dense-tail instances launched with
+    // spare CUs) go straight to the endpoint tasks
+    const bool has_col = (int)blockIdx.x < W;
+    const int w = has_col ? blockIdx.x : 0;
     const int NB = (N + 31) >> 5;
     const int ecc_w = a.base_parts[w].ecc;
     const unsigned long long pad = pad_mask(w, N);
@@ -159,6 +164,7 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a, TopoRingQ topo, Den
     for (int k = tid; k < K; k += BS) s_sw[k] = a.swaps[a.k0 + k];
     auto gt = [&](int k) { return (size_t)(a.k0 + k) * W + w; };     // global (cand, col)
     auto lt = [&](int k) { return (size_t)k * W + w; };              // task slot of this launch
+    if (has_col) {
 
     // task init: columns holding a swap endpoint as a source go dense at once
     for (int k = tid; k < K; k += BS) {
@@ -424,6 +430,7 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a, TopoRingQ topo, Den
         if (!s_any) break;
     }
     clear_ended();
+    }  // has_col
     if constexpr (VPT > 0) {
         // dense tail: endpoint columns, each (candidate, column) once
         __shared__ int s_task;
@@ -542,26 +549,28 @@ static __global__ void k_queue_reset(int *ctl, int n) {
 template <int DT>
 hipError_t launch_sparse_deg(const SparseArgs &s, const TopoRingQ &topo, const DenseArgs &qa,
                              size_t lds, int vpt, hipStream_t st) {
-    auto go = [&](auto fn, int &configured) -> hipError_t {
+    auto go = [&](auto fn, int &configured, int grid) -> hipError_t {
         if ((int)lds > configured) {
             hipError_t e = hipFuncSetAttribute((const void *)fn,
                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
             if (e != hipSuccess) return e;
             configured = (int)lds;
         }
-        hipLaunchKernelGGL(fn, dim3(s.W), dim3(1024), lds, st, s, topo, qa);
+        hipLaunchKernelGGL(fn, dim3(grid), dim3(1024), lds, st, s, topo, qa);
         return hipGetLastError();
     };
     static int c0 = -1, c4 = -1, c8 = -1, c16 = -1;
+    // dense-tail instances may take extra workgroups; one per column otherwise
+    const int g = s.grid > s.W ? s.grid : s.W;
     if constexpr (DT > 0) {
         switch (vpt) {
-            case 4: return go(&k_sparse<1024, DT, 4>, c4);
-            case 8: return go(&k_sparse<1024, DT, 8>, c8);
-            case 16: return go(&k_sparse<1024, DT, 16>, c16);
+            case 4: return go(&k_sparse<1024, DT, 4>, c4, g);
+            case 8: return go(&k_sparse<1024, DT, 8>, c8, g);
+            case 16: return go(&k_sparse<1024, DT, 16>, c16, g);
             default: break;
         }
     }
-    return go(&k_sparse<1024, DT, 0>, c0);
+    return go(&k_sparse<1024, DT, 0>, c0, s.W);
 }
 
 namespace gg {
