@@ -10,8 +10,8 @@ Workload: random 24-regular graph, n = 16384 (seeded), simulated annealing over
 are only scored are not counted); the plain single-move GPU search is measured
 beside it.  Inputs (graph, tables, base BFS levels) are resident in HBM.
 
-Prints one JSON line.  Only the cpu_baseline leg loads the CPU oracle, to time
-it.
+Prints one JSON line.  Only the cpu_baseline leg loads the CPU oracle and the
+reference's CPU evaluator built from its sources (oracle/_ref), to time them.
 """
 import argparse
 import ctypes as C
@@ -63,6 +63,47 @@ def cpu_baseline(adj, seconds_target=12.0):
             "sample": f"oracle queue BFS, OpenMP {threads} threads, {passes} passes over {nsrc} "
                       f"of {n} sources of the n={n} d={d} graph in {t_all:.2f} s, "
                       f"scaled by {n}/{nsrc}"}
+
+
+def cpu_baseline_reference(gg, n, d, seconds_target=10.0, g=16):
+    """The reference's own CPU evaluator (cppcode/cpuASPLqueue.cpp
+    diameterASPL, compiled from the reference sources into oracle/_ref;
+    single-threaded as the reference runs it).  A whole evaluation at
+    n = 16384, d = 24 takes about a minute on one thread, so the sample is the
+    same evaluator on an n-vertex d-regular orbit graph with g copies: a queue
+    BFS from n/g representatives over all n vertices (the per-source work of
+    the explicit graph), scaled by n/g / n.  None when oracle/_ref is not
+    built."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import oracle_lib as ol
+    if not ol.ref_available() or n % g:
+        return None
+    m = n // g
+    # a simple d-regular graph on the m representatives, every edge {i, j}
+    # lifted by a random shift s: offsets (j - i) + m*s and its reverse
+    base = gg.random_regular(m, d, 11)
+    rng = np.random.default_rng(11)
+    table = np.zeros((m, d), dtype=np.int32)
+    for i in range(m):
+        for si, j in enumerate(base[i]):
+            j = int(j)
+            if j < i:
+                continue
+            off = (j - i) + m * int(rng.integers(0, g))
+            table[i, si] = off
+            table[j, int(np.nonzero(base[j] == i)[0][0])] = -off
+    passes, t_all = 0, 0.0
+    while t_all < seconds_target or passes == 0:
+        t0 = time.perf_counter()
+        ol.ref_part(n, m, table)
+        t_all += time.perf_counter() - t0
+        passes += 1
+    return {"value": passes * (m / n) / t_all, "unit": "exact evaluations/s", "cores": 1,
+            "kind": "reference",
+            "sample": f"cpuASPLqueue<16384>::diameterASPL of the reference, built from its "
+                      f"sources (oracle/_ref), on one host thread: {passes} evaluation(s) of an "
+                      f"n={n} d={d} orbit graph (BFS from {m} representatives over all {n} "
+                      f"vertices) in {t_all:.2f} s, scaled by {m}/{n}"}
 
 
 def plain_kernel_roofline(gg, adj, reps=30):
@@ -300,7 +341,11 @@ def main():
             s.close()
         out["equal_time"] = res
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(adj)
+        port = cpu_baseline(adj)
+        ref = cpu_baseline_reference(gg, n, d)
+        out["cpu_baseline"] = ref if ref is not None else port
+        if ref is not None:
+            out["cpu_baseline_port"] = port
     print(json.dumps(out), flush=True)
 
 
