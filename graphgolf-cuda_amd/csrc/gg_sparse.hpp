@@ -112,7 +112,12 @@ __device__ __forceinline__ unsigned long long prev_word(const unsigned *ubits, c
                                                         const unsigned long long *ovr,
                                                         const unsigned long long *Bprev, int x) {
     if (!((ubits[x >> 5] >> (x & 31)) & 1u)) return Bprev[x];
-    return ((bits[x >> 5] >> (x & 31)) & 1u) ? ovr[x] : Bprev[x];
+    // the membership word and the override are loaded together (one memory
+    // latency instead of two); ovr[x] is in bounds and simply discarded
+    // where the task's bit is clear
+    const unsigned bw = bits[x >> 5];
+    const unsigned long long o = ovr[x];
+    return ((bw >> (x & 31)) & 1u) ? o : Bprev[x];
 }
 
 // Dynamic LDS of k_sparse: B_{l-1}[N] words, the Delta union bitmap, the
