@@ -18,6 +18,14 @@
 
 using namespace gg;
 
+// MLP parameter layout (fp32 masters and bf16 copies): W1[H][F], b1[H], W2[H], b2.
+#define OFF_W1 0
+#define OFF_B1 (GG_HIDDEN * GG_FEATS)
+#define OFF_W2 (OFF_B1 + GG_HIDDEN)
+#define OFF_B2 (OFF_W2 + GG_HIDDEN)
+
+__device__ __forceinline__ float bf(const __hip_bfloat16 v) { return __bfloat162float(v); }
+
 // ---- features ----------------------------------------------------------------------------
 // Per proposal (added edges (p1,q1),(p2,q2), removed (a,b),(c,d)), 32 features:
 //  added edge t, X[8t..8t+8): pre-swap d(p,q)/diam, |L3(p)&L3(q)|/n,
@@ -45,6 +53,8 @@ struct FeatArgs {
     const int *moves;                // [P][5]: a, b, c, d, kind
     int P;
     float *X;                        // [P][GG_FEATS]
+    const __hip_bfloat16 *wq;        // MLP weights: with `score`, also the scores
+    float *score;                    // [P] (null: features only)
 };
 
 __device__ __forceinline__ unsigned long long pad_bits(const FeatArgs &f, int w) {
@@ -107,6 +117,45 @@ __device__ void uniq_second_layer4(const FeatArgs &f, const int ends[4],
         for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
         out[q] = x;
     }
+}
+
+// The 32 features of proposal p into xf (lane 0; the counts are wave totals).
+__device__ __forceinline__ void write_features(const FeatArgs &f, int p, int a, int b, int c, int dd, int kind,
+                               const float *cnt, const int *eccl, const int *uq, float *X) {
+    (void)p;
+    const int p1 = a, q1 = kind ? dd : c, p2 = b, q2 = kind ? c : dd;
+    const int diam = f.diam_dev ? (*f.diam_dev > 0 ? *f.diam_dev : 1) : f.diam;
+    // distance of each added pair before the swap
+    auto dist = [&](int x, int y) {
+        const int w = y >> 6, ew = f.base_parts[w].ecc;
+        for (int l = 1; l < 64; l++)
+            if ((ball_word(f, l, w, x, ew) >> (y & 63)) & 1ull) return l;
+        return 64;
+    };
+    const float d1 = (float)f.d, d2 = d1 * d1, n = (float)f.N, inv_diam = 1.f / (float)diam;
+    const int qs[2] = {q1, q2}, ps[2] = {p1, p2};
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+        X[t * 8 + 0] = dist(ps[t], qs[t]) * inv_diam;
+        X[t * 8 + 1] = cnt[t * 7 + 0] / n;
+        X[t * 8 + 2] = cnt[t * 7 + 1] / n;
+        X[t * 8 + 3] = cnt[t * 7 + 2] / d2;
+        X[t * 8 + 4] = cnt[t * 7 + 3] / d2;
+        X[t * 8 + 5] = cnt[t * 7 + 4] / d2;
+        X[t * 8 + 6] = cnt[t * 7 + 5] / d1;
+        X[t * 8 + 7] = cnt[t * 7 + 6] / d1;
+    }
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+        X[16 + t * 6 + 0] = (float)uq[2 * t] / d1;
+        X[16 + t * 6 + 1] = (float)uq[2 * t + 1] / d1;
+        X[16 + t * 6 + 2] = cnt[14 + t * 3 + 0] / d2;
+        X[16 + t * 6 + 3] = cnt[14 + t * 3 + 1] / d2;
+        X[16 + t * 6 + 4] = cnt[14 + t * 3 + 2] / d2;
+        X[16 + t * 6 + 5] = (float)(eccl[2 * t] + eccl[2 * t + 1] - 2 * diam) * 0.5f;
+    }
+#pragma unroll
+    for (int e = 0; e < 4; e++) X[28 + e] = cnt[20 + e] / n;
 }
 
 // One wave per proposal.  Counts are wave-reduced popcounts over the W words.
@@ -189,47 +238,42 @@ __global__ void __launch_bounds__(256) k_features(FeatArgs f) {
     __syncthreads();
     int uq[4];
     uniq_second_layer4(f, ends, stage ? s_r1[wv] : nullptr, lane, uq);
-    if (lane != 0 || !active) return;
-    const int diam = f.diam_dev ? (*f.diam_dev > 0 ? *f.diam_dev : 1) : f.diam;
-    // distance of each added pair before the swap
-    auto dist = [&](int x, int y) {
-        const int w = y >> 6, ew = f.base_parts[w].ecc;
-        for (int l = 1; l < 64; l++)
-            if ((ball_word(f, l, w, x, ew) >> (y & 63)) & 1ull) return l;
-        return 64;
-    };
-    float *X = f.X + (size_t)p * GG_FEATS;
-    const float d1 = (float)f.d, d2 = d1 * d1, n = (float)f.N, inv_diam = 1.f / (float)diam;
-    const int qs[2] = {q1, q2}, ps[2] = {p1, p2};
-    for (int t = 0; t < 2; t++) {
-        X[t * 8 + 0] = dist(ps[t], qs[t]) * inv_diam;
-        X[t * 8 + 1] = cnt[t * 7 + 0] / n;
-        X[t * 8 + 2] = cnt[t * 7 + 1] / n;
-        X[t * 8 + 3] = cnt[t * 7 + 2] / d2;
-        X[t * 8 + 4] = cnt[t * 7 + 3] / d2;
-        X[t * 8 + 5] = cnt[t * 7 + 4] / d2;
-        X[t * 8 + 6] = cnt[t * 7 + 5] / d1;
-        X[t * 8 + 7] = cnt[t * 7 + 6] / d1;
+    float xf[GG_FEATS];
+#pragma unroll
+    for (int i = 0; i < GG_FEATS; i++) xf[i] = 0.f;
+    if (lane == 0) write_features(f, p, a, b, c, dd, kind, cnt, eccl, uq, xf);
+    if (lane == 0 && active) {
+        float *X = f.X + (size_t)p * GG_FEATS;
+#pragma unroll
+        for (int i = 0; i < GG_FEATS; i++) X[i] = xf[i];
     }
-    for (int t = 0; t < 2; t++) {
-        X[16 + t * 6 + 0] = (float)uq[2 * t] / d1;
-        X[16 + t * 6 + 1] = (float)uq[2 * t + 1] / d1;
-        X[16 + t * 6 + 2] = cnt[14 + t * 3 + 0] / d2;
-        X[16 + t * 6 + 3] = cnt[14 + t * 3 + 1] / d2;
-        X[16 + t * 6 + 4] = cnt[14 + t * 3 + 2] / d2;
-        X[16 + t * 6 + 5] = (float)(eccl[2 * t] + eccl[2 * t + 1] - 2 * diam) * 0.5f;
+    if (!f.score) return;
+    // the MLP score of the proposal, with the arithmetic of k_mlp_forward
+    // (same products, fma order and sums; products rounded before they are
+    // summed, as k_mlp_forward's pass through LDS does): bit-identical
+    {
+#pragma clang fp contract(off)
+        float xs[GG_FEATS];
+#pragma unroll
+        for (int i = 0; i < GG_FEATS; i++) xs[i] = __shfl(xf[i], 0, 64);
+        float o = 0.f;
+#pragma unroll
+        for (int q = 0; q < GG_HIDDEN / 64; q++) {
+            const int h = lane + 64 * q;
+            const __hip_bfloat16 *w1 = f.wq + OFF_W1 + h * GG_FEATS;
+            float acc = bf(f.wq[OFF_B1 + h]);
+#pragma unroll
+            for (int i = 0; i < GG_FEATS; i++) acc = fmaf(bf(w1[i]), xs[i], acc);
+            const float r = bf(f.wq[OFF_W2 + h]) * (acc > 0.f ? acc : 0.f);
+            o = o + r;
+        }
+#pragma unroll
+        for (int d = 32; d > 0; d >>= 1) o = o + __shfl_xor(o, d, 64);
+        if (lane == 0 && active) f.score[p] = o + bf(f.wq[OFF_B2]);
     }
-    for (int e = 0; e < 4; e++) X[28 + e] = cnt[20 + e] / n;
 }
 
 // ---- MLP ---------------------------------------------------------------------------------
-// Parameter layout (fp32 masters and bf16 copies): W1[H][F], b1[H], W2[H], b2.
-#define OFF_W1 0
-#define OFF_B1 (GG_HIDDEN * GG_FEATS)
-#define OFF_W2 (OFF_B1 + GG_HIDDEN)
-#define OFF_B2 (OFF_W2 + GG_HIDDEN)
-
-__device__ __forceinline__ float bf(const __hip_bfloat16 v) { return __bfloat162float(v); }
 
 // Forward of P samples: workgroup g takes samples [16g, 16g+16), threads h and
 // h+128 compute hidden unit h for half of them each, one wave per four
@@ -576,6 +620,12 @@ extern "C" int gg_mlp_train(gg_mlp *m, const float *X, const float *Y, int B, fl
 // Features of P moves (a, b, c, d, kind) on the batch evaluator's base.
 int gg::features_dev(gg_batch *b, const int *d_moves, int P, float *d_x, hipStream_t st,
                      const int *diam_dev) {
+    return features_score_dev(b, nullptr, d_moves, P, d_x, nullptr, st, diam_dev);
+}
+
+// Features and, with m and d_score, the MLP scores of the P moves in one launch.
+int gg::features_score_dev(gg_batch *b, gg_mlp *m, const int *d_moves, int P, float *d_x,
+                           float *d_score, hipStream_t st, const int *diam_dev) {
     if (!b->sparse_ok) return GG_UNSUPPORTED;
     FeatArgs f{};
     f.diam_dev = diam_dev;
@@ -590,6 +640,8 @@ int gg::features_dev(gg_batch *b, const int *d_moves, int P, float *d_x, hipStre
     f.moves = d_moves;
     f.P = P;
     f.X = d_x;
+    f.wq = m && d_score ? m->d_wq : nullptr;
+    f.score = m ? d_score : nullptr;
     hipLaunchKernelGGL(k_features, dim3((P + 3) / 4), dim3(256), 0, st, f);
     GG_HIP(hipGetLastError());
     return GG_OK;
@@ -620,5 +672,40 @@ extern "C" int gg_batch_features(gg_batch *b, const int32_t *moves, int P, float
     if (hipStreamSynchronize(b->stream) != hipSuccess) st = GG_HIP_ERROR;
     (void)hipFree(d_m);
     (void)hipFree(d_x);
+    return st;
+}
+
+extern "C" int gg_batch_score(gg_batch *b, gg_mlp *m, const int32_t *moves, int P, float *X,
+                              float *scores) {
+    if (!b || !m || !moves || !X || !scores || P < 1 || !b->have_base) return GG_BAD_ARG;
+    if (m->device != b->device) return GG_BAD_ARG;
+    for (int p = 0; p < P; p++)
+        for (int j = 0; j < 4; j++)
+            if (moves[p * 5 + j] < 0 || moves[p * 5 + j] >= b->N) return GG_BAD_ARG;
+    GG_HIP(hipSetDevice(b->device));
+    int *d_m = nullptr;
+    float *d_x = nullptr, *d_s = nullptr;
+    int st = GG_OK;
+    if (hipMalloc((void **)&d_m, 4 * (size_t)P * 5) != hipSuccess ||
+        hipMalloc((void **)&d_x, 4 * (size_t)P * GG_FEATS) != hipSuccess ||
+        hipMalloc((void **)&d_s, 4 * (size_t)P) != hipSuccess)
+        st = GG_OOM;
+    // the weights are written on the MLP's stream: order after them
+    if (st == GG_OK && hipStreamSynchronize(m->stream) != hipSuccess) st = GG_HIP_ERROR;
+    if (st == GG_OK &&
+        hipMemcpyAsync(d_m, moves, 4 * (size_t)P * 5, hipMemcpyHostToDevice, b->stream) !=
+            hipSuccess)
+        st = GG_HIP_ERROR;
+    if (st == GG_OK) st = features_score_dev(b, m, d_m, P, d_x, d_s, b->stream);
+    if (st == GG_OK &&
+        (hipMemcpyAsync(X, d_x, 4 * (size_t)P * GG_FEATS, hipMemcpyDeviceToHost, b->stream) !=
+             hipSuccess ||
+         hipMemcpyAsync(scores, d_s, 4 * (size_t)P, hipMemcpyDeviceToHost, b->stream) !=
+             hipSuccess))
+        st = GG_HIP_ERROR;
+    if (hipStreamSynchronize(b->stream) != hipSuccess && st == GG_OK) st = GG_HIP_ERROR;
+    if (d_m) (void)hipFree(d_m);
+    if (d_x) (void)hipFree(d_x);
+    if (d_s) (void)hipFree(d_s);
     return st;
 }

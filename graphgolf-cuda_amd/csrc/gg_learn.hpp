@@ -34,6 +34,8 @@ int mlp_adam_dev(gg_mlp *m, float lr, hipStream_t st);
 // diam_dev: the base diameter read on the device (null: the host's value)
 int features_dev(gg_batch *b, const int *d_moves, int P, float *d_x, hipStream_t st,
                  const int *diam_dev = nullptr);
+int features_score_dev(gg_batch *b, gg_mlp *m, const int *d_moves, int P, float *d_x,
+                       float *d_score, hipStream_t st, const int *diam_dev = nullptr);
 int batch_enqueue_dev(gg_batch *b, int K);
 struct BatchOut {
     const ResultRec *d_res;

@@ -350,9 +350,7 @@ static int score_proposals(gg_search *s, int P) {
         r[0] = m.a; r[1] = m.b; r[2] = m.c; r[3] = m.d; r[4] = m.kind;
     }
     GG_HIP(hipMemcpyAsync(L->d_moves, L->h_moves, 4 * (size_t)P * 5, hipMemcpyHostToDevice, L->st));
-    int st = features_dev(L->batch, L->d_moves, P, L->d_x, L->st);
-    if (st != GG_OK) return st;
-    st = mlp_forward_dev(L->mlp, L->d_x, P, L->d_score, L->st);
+    int st = features_score_dev(L->batch, L->mlp, L->d_moves, P, L->d_x, L->d_score, L->st);
     if (st != GG_OK) return st;
     GG_HIP(hipMemcpyAsync(L->h_score, L->d_score, 4 * (size_t)P, hipMemcpyDeviceToHost, L->st));
     GG_HIP(hipStreamSynchronize(L->st));
@@ -422,9 +420,9 @@ int learn_eval_device(gg_search *s) {
         hipLaunchKernelGGL(k_propose, dim3((P + 63) / 64), dim3(64), 0, L->st, b_adjR(b),
                            s->N, s->d, s->cfg.seed, L->h_ctl, P, L->d_moves, L->d_psw, L->d_ctl);
         if (hipGetLastError() != hipSuccess) return fail(GG_HIP_ERROR);
-        if ((st = features_dev(b, L->d_moves, P, L->d_x, L->st, L->d_ctl + 2)) != GG_OK)
+        if ((st = features_score_dev(b, L->mlp, L->d_moves, P, L->d_x, L->d_score, L->st,
+                                     L->d_ctl + 2)) != GG_OK)
             return fail(st);
-        if ((st = mlp_forward_dev(L->mlp, L->d_x, P, L->d_score, L->st)) != GG_OK) return fail(st);
         hipLaunchKernelGGL(k_rank, dim3((P + 63) / 64), dim3(256), 0, L->st, L->d_score, P,
                            L->d_rank);
         hipLaunchKernelGGL(k_select, dim3(1), dim3(1024), 0, L->st, L->d_rank, P, K,

@@ -215,6 +215,7 @@ def lib() -> C.CDLL:
         "gg_mlp_train": (C.c_int, [vp, P(C.c_float), P(C.c_float), C.c_int, C.c_float,
                                    P(C.c_float)]),
         "gg_batch_features": (C.c_int, [vp, i32p, C.c_int, P(C.c_float)]),
+        "gg_batch_score": (C.c_int, [vp, vp, i32p, C.c_int, P(C.c_float), P(C.c_float)]),
         "gg_batch_propose": (C.c_int, [vp, C.c_uint64, C.c_int64, C.c_int, i32p]),
         "gg_gen_random_regular": (C.c_int, [C.c_int, C.c_int, C.c_uint64, i32p]),
         "gg_gen_orbit_part": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_uint64, i32p]),
@@ -438,6 +439,16 @@ class BatchEvaluator:
         check(lib().gg_batch_features(self._h, _ptr(m), m.shape[0], _ptr(X, C.c_float)),
               "gg_batch_features")
         return X
+
+    def score(self, mlp, moves):
+        """(features [P, 32], MLP scores [P]) of P moves in the learned
+        search's fused launch."""
+        m = _i32(moves).reshape(-1, 5)
+        X = np.empty((m.shape[0], MLP_FEATURES), dtype=np.float32)
+        sc = np.empty(m.shape[0], dtype=np.float32)
+        check(lib().gg_batch_score(self._h, mlp._h, _ptr(m), m.shape[0], _ptr(X, C.c_float),
+                                   _ptr(sc, C.c_float)), "gg_batch_score")
+        return X, sc
 
     def propose(self, seed: int, step: int, P: int) -> np.ndarray:
         """P device-drawn 2-opt proposals {a, b, c, d, kind} on the current base."""

@@ -38,6 +38,10 @@ int gg_mlp_train(gg_mlp *m, const float *X, const float *Y, int B, float lr, flo
 /* Features (P x 32) of P swap moves {a, b, c, d, kind} (kind 1: (a,d),(b,c);
  * kind 0: (a,c),(b,d)) on the batch evaluator's current base. */
 int gg_batch_features(gg_batch *b, const int32_t *moves, int P, float *X);
+/* The same features and the MLP's scores of the P moves, computed in one
+ * launch as the learned search computes them; scores equal gg_mlp_forward on
+ * X bit for bit.  No reference counterpart. */
+int gg_batch_score(gg_batch *b, gg_mlp *m, const int32_t *moves, int P, float *X, float *scores);
 
 /* P uniform 2-opt proposals on the batch evaluator's current base, drawn on
  * the GPU as the learned search draws them: proposal p of step `step` takes

@@ -157,3 +157,26 @@ def test_learned_searches_back_to_back_are_independent(gpu):
         runs.append((st.total_cur, st.total_best, st.accepted, s.graph().tobytes()))
         s.close()
     assert runs[0] == runs[2]
+
+
+@pytest.mark.parametrize("n,d", [(1000, 6), (4096, 12)])
+def test_fused_features_and_scores(gpu, n, d):
+    """The learned search's fused launch (features, then the MLP score in the
+    same wave) gives the set-defined features and exactly the scores of the
+    standalone forward kernel on them."""
+    gg = gpu
+    rng = np.random.default_rng(n + 1)
+    adj = gg.random_regular(n, d, 4)
+    mv = _moves(gg, adj, 70, rng)  # not a multiple of four: a partial workgroup
+    m = gg.MLP(seed=5)
+    m.set_params(_random_params(rng))
+    with gg.BatchEvaluator(n, d, 8) as be:
+        base = be.set_base(adj)
+        X, sc = be.score(m, mv)
+        X2 = be.features(mv)
+    assert np.array_equal(X, X2)
+    R = feature_ref.features(adj, mv, base.diameter)
+    assert np.allclose(X, R, rtol=1e-6, atol=1e-6)
+    ref = m.forward(X)
+    assert np.array_equal(sc, ref), np.abs(sc - ref).max()
+    m.close()
