@@ -108,7 +108,8 @@ def cpu_baseline_reference(gg, n, d, seconds_target=10.0, g=16):
 
 def plain_kernel_roofline(gg, adj, reps=30):
     """Dense column-BFS kernel (the plain evaluator) timed with HIP events on
-    its own stream; algorithmic bytes 8*(d+2) per word-level."""
+    its own stream: per swap, ten back-to-back launches of that swap's BFS
+    between one event pair; algorithmic bytes 8*(d+2) per word-level."""
     n, d = adj.shape
     rng = np.random.default_rng(7)
     ev = gg.SwapEvaluator(n, d)
@@ -117,7 +118,7 @@ def plain_kernel_roofline(gg, adj, reps=30):
                                                ("vpt", gg.GG_INFO_VPT),
                                                ("fixed", gg.GG_INFO_FIXED),
                                                ("push_cap", gg.GG_INFO_PUSH_CAP))}
-    kms, lev = [], []
+    kms, lev, single = [], [], []
     while len(kms) < reps:
         a, c = (int(x) for x in rng.integers(0, n, 2))
         ia, ic = (int(x) for x in rng.integers(0, d, 2))
@@ -125,7 +126,10 @@ def plain_kernel_roofline(gg, adj, reps=30):
         if sw is None:
             continue
         r = ev.eval_swap(sw)
-        kms.append(ev.last_kernel_ms())
+        single.append(ev.last_kernel_ms())
+        # the same launch repeated back to back between one event pair: the
+        # per-launch time without the event overhead of a single bracket
+        kms.append(ev.time_bfs(10))
         lev.append(r.levels)
     ev.close()
     W = (n + 63) // 64
@@ -145,7 +149,8 @@ def plain_kernel_roofline(gg, adj, reps=30):
         "256 CU x 128 B/clk x 2.4 GHz (above the measured probe)"
     return {"bound": "lds", "achieved": ach, "peak": peak, "unit": "GB/s",
             "frac": ach / peak, "traffic": hbm_traffic(n, d), "kernel": name,
-            "kernel_ms": t * 1e3, "levels": L, "alg_bytes_per_launch": alg,
+            "kernel_ms": t * 1e3, "kernel_ms_single_bracket": float(np.mean(single[3:])),
+            "levels": L, "alg_bytes_per_launch": alg,
             "push_cap": info["push_cap"], "peak_basis": basis,
             "lds_probe": {"b128_gbs": p16["gbs"], "b64_gbs": p8["gbs"],
                           "clock_mhz": p16["clock_mhz"],

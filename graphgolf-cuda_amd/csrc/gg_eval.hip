@@ -234,6 +234,7 @@ int run_dense(DenseCtx &c, const DenseArgs &a_in, hipStream_t st) {
             return GG_HIP_ERROR;
         }
     }
+    if (c.bfs_only) return GG_OK;
     hipLaunchKernelGGL(k_dense_finalize<1024>, dim3(a.K), dim3(1024), 0, st, a);
     e = hipGetLastError();
     if (e != hipSuccess) {
@@ -333,6 +334,7 @@ struct gg_eval {
     int W;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    DenseArgs last_a{};  // launch arguments of the last evaluation's BFS
     DenseCtx dc;
     void *d_tab = nullptr;
     size_t tab_bytes = 0;
@@ -623,6 +625,7 @@ static int eval_common(gg_eval *ev, gg_result *out, bool upload = true,
     a.reach_col = ev->d_reach_col;
     a.reach_out = ev->d_reach;
     a.want_wvc = ev->kind == EVAL_PART ? 1 : 0;
+    ev->last_a = a;
     GG_HIP(hipEventRecord(ev->ev0, ev->stream));
     int st = run_dense(ev->dc, a, ev->stream);  // records ev1 after the BFS kernel
     if (st != GG_OK) return st;
@@ -747,6 +750,30 @@ extern "C" int gg_eval_levels(gg_eval *ev, int32_t *ecc_out, int64_t *reach_out,
             reach_out[l] = (l < n && l <= ev->last_levels) ? tmp[l] : 0;
     }
     GG_HIP(hipStreamSynchronize(ev->stream));
+    return GG_OK;
+}
+
+// The last evaluation's BFS kernel launched `reps` times back to back
+// between one event pair (the kernel is idempotent: it rewrites the same
+// column parts); the mean per launch leaves out the event overhead that a
+// single bracketed launch carries.
+extern "C" int gg_eval_time_bfs(gg_eval *ev, int reps, float *ms) {
+    if (!ev || !ms || reps < 1 || reps > 100000 || !ev->have_result) return GG_BAD_ARG;
+    GG_HIP(hipSetDevice(ev->device));
+    DenseCtx &c = ev->dc;
+    hipEvent_t done = c.bfs_done;
+    c.bfs_done = nullptr;
+    c.bfs_only = 1;
+    int st = hipEventRecord(ev->ev0, ev->stream) == hipSuccess ? GG_OK : GG_HIP_ERROR;
+    for (int r = 0; r < reps && st == GG_OK; r++) st = run_dense(c, ev->last_a, ev->stream);
+    c.bfs_only = 0;
+    c.bfs_done = done;
+    if (st != GG_OK) return st;
+    GG_HIP(hipEventRecord(ev->ev1, ev->stream));
+    GG_HIP(hipEventSynchronize(ev->ev1));
+    float t = 0;
+    GG_HIP(hipEventElapsedTime(&t, ev->ev0, ev->ev1));
+    *ms = t / (float)reps;
     return GG_OK;
 }
 

@@ -23,6 +23,7 @@ struct DenseCtx {
     int push_off = 0;         // 1 = pull from level 1 (no push levels)
     int push_ok = 0;          // the graph of this launch is known to be undirected
     hipEvent_t bfs_done = nullptr;  // if set, recorded between the BFS and finalize kernels
+    int bfs_only = 0;         // 1 = launch the BFS kernel alone (no finalize): timing
     int last_variant = 0;     // 1 = LDS/256, 2 = LDS/1024, 3 = global
     int last_vpt = 0;         // vertices per thread of the last LDS launch
     int last_fixed = 0;       // fixed-degree instance of the last launch (0 = generic)

@@ -162,6 +162,7 @@ def lib() -> C.CDLL:
         "gg_eval_adj": (C.c_int, [vp, i32p, P(Result)]),
         "gg_eval_levels": (C.c_int, [vp, i32p, i64p, C.c_int]),
         "gg_eval_last_kernel_ms": (C.c_int, [vp, P(C.c_float)]),
+        "gg_eval_time_bfs": (C.c_int, [vp, C.c_int, P(C.c_float)]),
         "gg_eval_set_option": (C.c_int, [vp, C.c_int, C.c_int]),
         "gg_eval_get_option": (C.c_int, [vp, C.c_int, P(C.c_int)]),
         "gg_eval_adj_swap": (C.c_int, [vp, P(Swap), P(Result)]),
@@ -315,6 +316,13 @@ class _Evaluator:
     def last_kernel_ms(self) -> float:
         ms = C.c_float(0)
         check(lib().gg_eval_last_kernel_ms(self._h, C.byref(ms)), "gg_eval_last_kernel_ms")
+        return ms.value
+
+    def time_bfs(self, reps: int = 10) -> float:
+        """Mean ms per launch of the last evaluation's BFS kernel, `reps`
+        launches back to back between one event pair."""
+        ms = C.c_float(0)
+        check(lib().gg_eval_time_bfs(self._h, reps, C.byref(ms)), "gg_eval_time_bfs")
         return ms.value
 
     def set_option(self, opt: int, value: int):

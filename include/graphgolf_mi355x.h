@@ -126,6 +126,10 @@ int gg_eval_levels(gg_eval *ev, int32_t *ecc_out, int64_t *reach_out, int reach_
 /* Device time (ms, HIP events on the evaluator's stream) of the BFS kernel of
  * the most recent evaluation. */
 int gg_eval_last_kernel_ms(gg_eval *ev, float *ms);
+/* Mean time per launch of the last evaluation's BFS kernel, launched `reps`
+ * times back to back between one event pair (no reference counterpart; for
+ * the roofline measurement). */
+int gg_eval_time_bfs(gg_eval *ev, int reps, float *ms);
 
 /* Tuning knobs (0 = automatic choice). */
 #define GG_OPT_BLOCK      1   /* threads per workgroup: 256 or 1024                  */

@@ -347,3 +347,27 @@ def test_autotuner_picks_and_caches(gpu, tmp_path, monkeypatch):
     with gg.AdjEvaluator(4096, 12) as ev2:  # starts from the cached choice
         assert ev2.get_option(gg.GG_OPT_BLOCK) == rep.best_block
         assert ev2.get_option(gg.GG_OPT_GENERIC) == rep.best_generic
+
+
+def test_time_bfs_relaunches_leave_results_intact(gpu):
+    """gg_eval_time_bfs re-runs the last evaluation's BFS kernel back to back;
+    the kernel is idempotent, so later evaluations are unchanged."""
+    gg = gpu
+    n, d = 2048, 8
+    adj = gg.random_regular(n, d, 3)
+    rng = np.random.default_rng(5)
+    ev = gg.SwapEvaluator(n, d)
+    ev.set_base(adj)
+    sw = None
+    while sw is None:
+        a, c = (int(x) for x in rng.integers(0, n, 2))
+        ia, ic = (int(x) for x in rng.integers(0, d, 2))
+        sw, new = gg.make_swap(adj, a, ia, c, ic, 1)
+    r1 = ev.eval_swap(sw).as_dict()
+    t = ev.time_bfs(7)
+    assert t > 0
+    r2 = ev.eval_swap(sw).as_dict()
+    assert r1 == r2
+    o = ol.adj_eval(new)
+    assert r1["total_reps"] == o["total"] and r1["diameter"] == o["diameter"]
+    ev.close()
