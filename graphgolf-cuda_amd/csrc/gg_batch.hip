@@ -315,6 +315,9 @@ static hipError_t launch_sparse(const SparseArgs &s, const TopoRingQ &topo, cons
     }
 }
 
+// Workgroups of the dense fallback queue kernel after a sparse launch.
+static const int kQueueGridMax = 64;
+
 // Enqueues the sparse pipeline for K candidates whose swaps are in d_swaps.
 static int enqueue_sparse(gg_batch *b, int K, bool reset = true) {
     const int W = b->W, N = b->N;
@@ -327,6 +330,7 @@ static int enqueue_sparse(gg_batch *b, int K, bool reset = true) {
     s.base_parts = b->d_parts + (size_t)b->max_k * W;
     s.base_reach = b->d_base_reach;
     s.cap = b->cap;
+    s.climit = b->dlimit_opt > 0 && b->dlimit_opt < b->cap ? b->dlimit_opt : b->cap;
     s.skcap = b->skcap;
     s.dl_key = b->d_dl_key;
     s.dl_bits = b->d_dl_bits;
@@ -390,7 +394,11 @@ static int enqueue_sparse(gg_batch *b, int K, bool reset = true) {
     int per_cu = (int)((size_t)b->dc.lds_device / (lds + 4096));
     if (per_cu < 1) per_cu = 1;
     if (per_cu > 2) per_cu = 2;
+    // the queue holds only the tasks whose Delta overflowed (usually none):
+    // a small grid keeps the empty-queue launch cheap, and its workgroups
+    // loop over the queue when it does fill
     int grid = b->dc.cu_count * per_cu;
+    if (grid > kQueueGridMax) grid = kQueueGridMax;
     if (grid > K * W) grid = K * W;
     int st = run_dense_queue(b->dc, a, grid, b->stream);
     if (st != GG_OK) return st;
@@ -898,6 +906,11 @@ extern "C" int gg_batch_set_option(gg_batch *b, int option, int value) {
             if (value < 0 || value > 1) return GG_BAD_ARG;
             b->commit_mode = value;
             return GG_OK;
+        case GG_BATCH_OPT_DELTA_LIMIT:
+            if (value < 0) return GG_BAD_ARG;
+            b->dlimit_opt = value;
+            drop_graph(b);  // the captured step holds the old limit
+            return GG_OK;
         default: return GG_BAD_ARG;
     }
 }
@@ -910,6 +923,7 @@ extern "C" int gg_batch_get_option(gg_batch *b, int option, int *value) {
         case GG_BATCH_INFO_LAST_SPARSE: *value = b->last_sparse; return GG_OK;
         case GG_BATCH_INFO_SPARSE_OK: *value = b->sparse_ok; return GG_OK;
         case GG_BATCH_OPT_COMMIT: *value = b->commit_mode; return GG_OK;
+        case GG_BATCH_OPT_DELTA_LIMIT: *value = b->dlimit_opt; return GG_OK;
         case GG_BATCH_INFO_LAST_COMMIT_INPLACE: *value = b->last_commit_inplace; return GG_OK;
         default: return GG_BAD_ARG;
     }

@@ -48,6 +48,7 @@ struct gg_batch {
     int log_k0 = 0, log_K = 0;                // candidates the logs cover: [log_k0, log_K)
     int ecc_stale = 0;                        // d_ecc predates an in-place commit
     int commit_mode = 0;                      // 0 in place when possible, 1 always a new BFS
+    int dlimit_opt = 0;                       // Delta list limit below cap (0 = cap)
     int last_commit_inplace = 0;
     int mode_opt = 0;                         // 0 auto, 1 dense only, 2 sparse when possible
     int sparse_ok = 0;                        // base allows the sparse path

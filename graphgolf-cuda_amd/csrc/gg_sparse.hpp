@@ -46,7 +46,8 @@ struct SparseArgs {
     int S;
     const ColPart *base_parts;       // [W] base column parts (ecc, levels)
     const long long *base_reach;     // [W][lmax+1] base per-level new pairs
-    int cap;                         // Delta list capacity per task
+    int cap;                         // Delta list capacity per task (list stride)
+    int climit;                      // Delta entries kept before a task overflows (<= cap)
     int skcap;                       // Delta keys of the previous level staged in LDS (max)
     // per task t = k*W + w (k local to the launch), parity p = level & 1
     int *dl_key;                     // [T][2][cap] Delta vertices
@@ -387,7 +388,7 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a, TopoRingQ topo, Den
             const unsigned old = atomicOr(&cbits[v >> 5], m);
             if (old & m) continue;  // another item of this task recorded v
             const int idx = atomicAdd(&s_ncur[k], 1);
-            if (idx >= a.cap) { s_over[k] = 1; continue; }
+            if (idx >= a.climit) { s_over[k] = 1; continue; }
             a.dl_key[(tl * 2 + cur) * a.cap + idx] = v;
             a.dl_ovr[(tl * 2 + cur) * (size_t)N + v] = acc;
             if (a.lg_key && l <= a.S) {

@@ -41,6 +41,7 @@ GG_LEVEL_LIMIT_MAX = 199
 GG_BATCH_OPT_PATH = 1
 GG_BATCH_OPT_GRAPH = 2
 GG_BATCH_OPT_COMMIT = 3
+GG_BATCH_OPT_DELTA_LIMIT = 4
 GG_BATCH_INFO_LAST_SPARSE = 100
 GG_BATCH_INFO_SPARSE_OK = 101
 GG_BATCH_INFO_LAST_COMMIT_INPLACE = 102

@@ -206,6 +206,10 @@ int gg_batch_levels_digest(gg_batch *b, uint64_t *digest);
 #define GG_BATCH_OPT_COMMIT        3
 /* read-only: 1 if the last commit rewrote the stored levels in place */
 #define GG_BATCH_INFO_LAST_COMMIT_INPLACE 102
+/* Delta entries a (candidate, column) task may hold before it overflows to the
+ * dense fallback queue (0 = the allocated capacity, min(2048, N)); a small
+ * value sends most tasks through the queue (used by tests). */
+#define GG_BATCH_OPT_DELTA_LIMIT   4
 int gg_batch_set_option(gg_batch *b, int option, int value);
 int gg_batch_get_option(gg_batch *b, int option, int *value);
 

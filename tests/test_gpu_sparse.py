@@ -88,3 +88,47 @@ def test_sparse_disconnecting_candidates(gpu):
             o = ol.adj_eval(new)
             assert a.total_reps == b.total_reps == o["total"]
             assert a.status == b.status
+
+
+@pytest.mark.parametrize("limit", [1, 8])
+def test_sparse_overflow_goes_through_the_dense_queue(gpu, limit):
+    """With the Delta limit lowered, most (candidate, column) tasks overflow
+    and are evaluated by the dense fallback queue kernel (a bounded grid that
+    loops over the queue); every candidate still matches the dense path and
+    the oracle, replays included, and committing an overflowed candidate
+    re-evaluates the base."""
+    gg = gpu
+    n, d, k = 4096, 12, 32
+    print("overflow case", limit, flush=True)
+    rng = np.random.default_rng(77 + limit)
+    adj = gg.random_regular(n, d, 13)
+    cands = _random_swaps(gg, adj, k, rng)
+    with gg.BatchEvaluator(n, d, max_k=k) as be:
+        be.set_base(adj)
+        be.set_option(gg.GG_BATCH_OPT_DELTA_LIMIT, limit)
+        assert be.get_option(gg.GG_BATCH_OPT_DELTA_LIMIT) == limit
+        for rep in range(2):
+            rs, rd = _eval_both(gg, be, [c[0] for c in cands])
+            for (sw, new), a, b in zip(cands, rs, rd):
+                assert (a.total_reps, a.diameter, a.status) == (b.total_reps, b.diameter,
+                                                                  b.status), rep
+        for (sw, new), a in zip(cands[:6], rs):
+            o = ol.adj_eval(new)
+            assert a.total_reps == o["total"]
+        # commit the best sparse result, then the next batch on the new base
+        be.set_option(gg.GG_BATCH_OPT_PATH, gg.PATH_SPARSE)
+        rs = be.eval([c[0] for c in cands])
+        i = min((i for i, r in enumerate(rs) if r.status == gg.GG_OK),
+                key=lambda i: rs[i].total_reps)
+        be.commit(cands[i][0])
+        adj2 = cands[i][1]
+        cands2 = _random_swaps(gg, adj2, k, rng)
+        rs2, rd2 = _eval_both(gg, be, [c[0] for c in cands2])
+        for (sw, new), a, b in zip(cands2, rs2, rd2):
+            assert (a.total_reps, a.diameter, a.status) == (b.total_reps, b.diameter, b.status)
+        for (sw, new), a in zip(cands2[:4], rs2):
+            assert a.total_reps == ol.adj_eval(new)["total"]
+        # back to the full capacity
+        be.set_option(gg.GG_BATCH_OPT_DELTA_LIMIT, 0)
+        rs3, rd3 = _eval_both(gg, be, [c[0] for c in cands2])
+        assert [r.total_reps for r in rs3] == [r.total_reps for r in rs2]
