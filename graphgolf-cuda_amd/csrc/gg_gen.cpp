@@ -8,6 +8,8 @@
 // Orbit ('part' with M < N) graphs: orbit edges (i, j, s) give row i the
 // offset (j - i) + M*s and row j its negation (cppcode/annealing.cpp:132 form).
 #include <algorithm>
+#include <climits>
+#include <cstdint>
 #include <cstring>
 #include <vector>
 #include "graphgolf_mi355x.h"
@@ -140,6 +142,107 @@ extern "C" int gg_gen_orbit_part(int g, int m, int d, uint64_t seed, int32_t *ta
             for (int k = 0; k < d; k++) table_out[(size_t)r * d + k] = rows[r][k];
         (void)N;
         return GG_OK;
+    }
+    return GG_BAD_ARG;
+}
+
+// Symmetric tables of 1, 2 or 3 axes ('part', 'piece', 'box'; rows mixed-radix
+// over Ms with axis 0 major, offsets [M][d][dims]).  The base rows' stubs are
+// shuffled and paired as RNDpiecegen.cpp:89-127 / RNDboxgen.cpp:97-140 pair
+// them, each pair an edge orbit with a lift drawn per axis.  Pairs that give a
+// zero offset (a self-loop in every copy), or an offset equal to another of
+// the same row (a multi-edge, half orbits included) are repaired: such a pair
+// trades partners with a random other pair and both are lifted again, until
+// no bad pair remains (the reference keeps them).  GG_BAD_ARG when the
+// arguments are inconsistent or no simple table is reached.
+extern "C" int gg_gen_lattice(int dims, const int32_t *Ns, const int32_t *Ms, int d,
+                              uint64_t seed, int32_t *table_out) {
+    if (dims < 1 || dims > 3 || !Ns || !Ms || d < 1 || !table_out) return GG_BAD_ARG;
+    int64_t M = 1, N = 1;
+    int g[3] = {1, 1, 1}, nk[3] = {1, 1, 1}, mk[3] = {1, 1, 1};
+    for (int k = 0; k < dims; k++) {
+        if (Ms[k] < 1 || Ns[k] < 1 || Ns[k] % Ms[k] != 0) return GG_BAD_ARG;
+        nk[k] = Ns[k];
+        mk[k] = Ms[k];
+        g[k] = Ns[k] / Ms[k];
+        M *= Ms[k];
+        N *= Ns[k];
+    }
+    if (N > INT32_MAX || (M * d) % 2 != 0 || M * d > (int64_t)1 << 28) return GG_BAD_ARG;
+    auto coords = [&](int r, int *c) {
+        for (int k = dims - 1; k >= 0; k--) { c[k] = r % mk[k]; r /= mk[k]; }
+    };
+    // offset of pair (a -> b) with lifts kv, packed mod N per axis
+    auto offset = [&](int a, int b, const int *kv, int *off) {
+        int ca[3], cb[3];
+        coords(a, ca);
+        coords(b, cb);
+        for (int k = 0; k < dims; k++) off[k] = (cb[k] - ca[k]) + mk[k] * kv[k];
+    };
+    auto key = [&](const int *off, int sign) {
+        int64_t x = 0;
+        for (int k = 0; k < dims; k++) {
+            int64_t o = ((int64_t)sign * off[k]) % nk[k];
+            if (o < 0) o += nk[k];
+            x = x * nk[k] + o;
+        }
+        return x;
+    };
+    Rng rng(seed);
+    const int64_t S = M * d, P = S / 2;
+    std::vector<int> stubs((size_t)S);
+    for (int64_t i = 0; i < S; i++) stubs[(size_t)i] = (int)(i / d);
+    shuffle(stubs, rng);
+    std::vector<int> pa((size_t)P), pb((size_t)P), lift((size_t)P * 3, 0);
+    auto relift = [&](int64_t p) {
+        for (int k = 0; k < dims; k++) lift[(size_t)p * 3 + k] = (int)rng.below(g[k]);
+    };
+    for (int64_t p = 0; p < P; p++) {
+        pa[(size_t)p] = stubs[(size_t)(2 * p)];
+        pb[(size_t)p] = stubs[(size_t)(2 * p + 1)];
+        relift(p);
+    }
+    // entries (row, offset key, pair) of every pair's two ends
+    std::vector<std::pair<std::pair<int, int64_t>, int64_t>> ent((size_t)S);
+    std::vector<char> bad((size_t)P);
+    for (int round = 0; round < 100000; round++) {
+        for (int64_t p = 0; p < P; p++) {
+            int off[3];
+            offset(pa[(size_t)p], pb[(size_t)p], &lift[(size_t)p * 3], off);
+            ent[(size_t)(2 * p)] = {{pa[(size_t)p], key(off, 1)}, p};
+            ent[(size_t)(2 * p + 1)] = {{pb[(size_t)p], key(off, -1)}, p};
+        }
+        std::sort(ent.begin(), ent.end());
+        std::fill(bad.begin(), bad.end(), 0);
+        bool any = false;
+        for (size_t i = 0; i < ent.size(); i++) {
+            const bool zero = ent[i].first.second == 0;
+            const bool dup = i + 1 < ent.size() && ent[i + 1].first == ent[i].first;
+            if (zero || dup) {
+                bad[(size_t)ent[i].second] = 1;
+                if (dup) bad[(size_t)ent[i + 1].second] = 1;
+                any = true;
+            }
+        }
+        if (!any) {
+            std::vector<int> fill((size_t)M, 0);
+            for (int64_t p = 0; p < P; p++) {
+                int off[3];
+                const int a = pa[(size_t)p], b = pb[(size_t)p];
+                offset(a, b, &lift[(size_t)p * 3], off);
+                int32_t *ta = table_out + ((size_t)a * d + fill[(size_t)a]++) * dims;
+                int32_t *tb = table_out + ((size_t)b * d + fill[(size_t)b]++) * dims;
+                for (int k = 0; k < dims; k++) { ta[k] = off[k]; tb[k] = -off[k]; }
+            }
+            return GG_OK;
+        }
+        for (int64_t p = 0; p < P; p++) {
+            if (!bad[(size_t)p]) continue;
+            const int64_t q = (int64_t)rng.below((uint64_t)P);
+            if (q != p) std::swap(pb[(size_t)p], pb[(size_t)q]);
+            relift(p);
+            relift(q);
+        }
     }
     return GG_BAD_ARG;
 }

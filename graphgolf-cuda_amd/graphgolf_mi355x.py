@@ -221,6 +221,7 @@ def lib() -> C.CDLL:
         "gg_batch_propose": (C.c_int, [vp, C.c_uint64, C.c_int64, C.c_int, i32p]),
         "gg_gen_random_regular": (C.c_int, [C.c_int, C.c_int, C.c_uint64, i32p]),
         "gg_gen_orbit_part": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_uint64, i32p]),
+        "gg_gen_lattice": (C.c_int, [C.c_int, i32p, i32p, C.c_int, C.c_uint64, i32p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -274,6 +275,17 @@ def orbit_part(g: int, m: int, d: int, seed: int) -> np.ndarray:
     """Seeded orbit graph (N = g*m, M = m) as an int32 [m, d] 'part' offset table."""
     out = np.empty((m, d), dtype=np.int32)
     check(lib().gg_gen_orbit_part(g, m, d, seed, _ptr(out)), "gg_gen_orbit_part")
+    return out
+
+
+def gen_lattice(Ns, Ms, d: int, seed: int) -> np.ndarray:
+    """Seeded symmetric d-regular table [prod(Ms), d, len(Ns)] (part, piece or
+    box form) by stub pairing with a lift per axis, self-loops and repeated
+    offsets repaired."""
+    dims = len(Ns)
+    ns, ms = _i32(list(Ns)), _i32(list(Ms))
+    out = np.empty((int(np.prod(Ms)), d, dims), dtype=np.int32)
+    check(lib().gg_gen_lattice(dims, _ptr(ns), _ptr(ms), d, seed, _ptr(out)), "gg_gen_lattice")
     return out
 
 

@@ -154,6 +154,13 @@ int gg_lattice_search_create_cb(int dims, const int32_t *Ns, const int32_t *Ms, 
  * mt19937_64(seed) with explicit bounded-integer mappings. */
 int gg_gen_random_regular(int n, int d, uint64_t seed, int32_t *adj_out);
 int gg_gen_orbit_part(int g, int m, int d, uint64_t seed, int32_t *table_out);
+/* Cyclically symmetric d-regular table of `dims` axes (1 = part, 2 = piece,
+ * 3 = box): orders Ns and base extents Ms per axis, table [prod Ms][d][dims]
+ * in the evaluators' layout.  The stub pairing with a lift per axis of
+ * RNDpiecegen.cpp:89-127 / RNDboxgen.cpp:97-140, with pairs that give a
+ * self-loop or a repeated offset repaired (the reference keeps them). */
+int gg_gen_lattice(int dims, const int32_t *Ns, const int32_t *Ms, int d, uint64_t seed,
+                   int32_t *table_out);
 
 /* ---- initial temperature ------------------------------------------------
  * cppcode/initTemp.cpp:399-430 (pieceinitTemp.cpp:228-260): over `graphs`
