@@ -184,3 +184,19 @@ def test_gg_anneal_piece_and_box(gpu, tmp_path, fmt, Ns, Ms, d):
     assert all(L.row_simple(best, r) and L.row_symmetric(best, r) for r in range(L.Mtot))
     tot, dia = ev(best)
     assert (dia, tot) < (dia0, tot0)
+
+
+@pytest.mark.parametrize("N,M,d", [(512, 64, 6), (256, 256, 6)])
+def test_gg_inittemp_prints_the_library_estimate(gpu, N, M, d):
+    """gg-inittemp: initTemp.cpp's output lines (cnt, avg, inittemp =
+    -avg / ln 0.4) with the library's values for the same seed, for orbit
+    graphs (M < N) and random regular graphs (M = N)."""
+    gg = gpu
+    out = _run([os.path.join(BIN, "gg-inittemp"), "-n", str(N), "-m", str(M), "-d", str(d),
+                "-s", "4", "--graphs", "6", "--neighbours", "20"])
+    vals = dict(re.findall(r"^(cnt|avg|inittemp): (\S+)$", out, flags=re.M))
+    r = gg.init_temp(N, d, graphs=6, neighbours=20, seed=4, M=M)
+    assert int(vals["cnt"]) == r.count > 0
+    assert abs(float(vals["avg"]) - r.mean_increase_total) <= 1e-8 * abs(r.mean_increase_total)
+    assert abs(float(vals["inittemp"]) - r.temp_total) <= 1e-8 * abs(r.temp_total)
+    assert abs(r.temp_total + r.mean_increase_total / np.log(0.4)) <= 1e-9 * abs(r.temp_total)
