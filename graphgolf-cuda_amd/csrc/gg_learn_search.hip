@@ -128,39 +128,40 @@ __global__ void __launch_bounds__(64) k_propose(const int *adjR, int N, int d, u
 }
 
 // Rank of every score among the P (NaN as +inf, ties by index).  A workgroup
-// of 256 threads ranks 64 proposals: all P scores staged in LDS, each wave
-// compares its proposals with a quarter of them, the quarters summed in LDS.
+// of 256 threads ranks 16 proposals: all P scores staged in LDS, thread t
+// counts the predecessors of proposal 16g + (t mod 16) within the (t / 16)-th
+// sixteenth of the scores, and the sixteen partial counts are summed in LDS.
+#define GG_RANK_PER 16
 __global__ void __launch_bounds__(256) k_rank(const float *score, int P, int *rank) {
     __shared__ float s_sc[GG_SELECT_MAX_P];
-    __shared__ int s_part[4][64];
+    __shared__ int s_part[16][GG_RANK_PER];
     for (int q = threadIdx.x; q < P; q += 256) {
         const float x = score[q];
         s_sc[q] = x == x ? x : INFINITY;
     }
     __syncthreads();
-    const int lane = threadIdx.x & 63, quarter = threadIdx.x >> 6;
-    const int p = blockIdx.x * 64 + lane;
+    const int j = threadIdx.x & (GG_RANK_PER - 1), seg = threadIdx.x >> 4;
+    const int p = blockIdx.x * GG_RANK_PER + j;
     int r = 0;
     if (p < P) {
         const float x = s_sc[p];
-        const int q0 = (P * quarter) >> 2, q1 = (P * (quarter + 1)) >> 2;
+        const int q0 = (P * seg) >> 4, q1 = (P * (seg + 1)) >> 4;
 #pragma unroll 8
         for (int q = q0; q < q1; q++) {
             const float y = s_sc[q];
             r += (y < x || (y == x && q < p)) ? 1 : 0;
         }
     }
-    s_part[quarter][lane] = r;
+    s_part[seg][j] = r;
     __syncthreads();
-    if (quarter == 0 && p < P)
-        rank[p] = s_part[0][lane] + s_part[1][lane] + s_part[2][lane] + s_part[3][lane];
+    if (seg == 0 && p < P) {
+        int t = 0;
+#pragma unroll
+        for (int i = 0; i < 16; i++) t += s_part[i][j];
+        rank[p] = t;
+    }
 }
 
-// One workgroup: the K candidates of the step.  While warm (ctl[1] != 0) the
-// first K proposals (uniform draws); otherwise the G = K - K/4 best ranked
-// (lowest predicted change) in rank order, then K/4 distinct uniform picks
-// among the rest from stream (seed, t, GG_EXPLORE_LANE).  Writes the chosen
-// swaps, their move rows and proposal indices.
 __global__ void __launch_bounds__(1024) k_select(const int *rank, int P, int K, uint64_t seed,
                                                  const int *ctl, const gg_swap *psw,
                                                  const int *pmoves, gg_swap *out_sw,
@@ -423,8 +424,8 @@ int learn_eval_device(gg_search *s) {
         if ((st = features_score_dev(b, L->mlp, L->d_moves, P, L->d_x, L->d_score, L->st,
                                      L->d_ctl + 2)) != GG_OK)
             return fail(st);
-        hipLaunchKernelGGL(k_rank, dim3((P + 63) / 64), dim3(256), 0, L->st, L->d_score, P,
-                           L->d_rank);
+        hipLaunchKernelGGL(k_rank, dim3((P + GG_RANK_PER - 1) / GG_RANK_PER), dim3(256), 0, L->st,
+                           L->d_score, P, L->d_rank);
         hipLaunchKernelGGL(k_select, dim3(1), dim3(1024), 0, L->st, L->d_rank, P, K,
                            s->cfg.seed, L->d_ctl, L->d_psw, L->d_moves, b_swaps(b), L->d_msel,
                            L->d_sel);
