@@ -125,11 +125,18 @@ __device__ __forceinline__ void write_features(const FeatArgs &f, int p, int a, 
     (void)p;
     const int p1 = a, q1 = kind ? dd : c, p2 = b, q2 = kind ? c : dd;
     const int diam = f.diam_dev ? (*f.diam_dev > 0 ? *f.diam_dev : 1) : f.diam;
-    // distance of each added pair before the swap
+    // distance of each added pair before the swap: levels 1-4 of the ball row
+    // loaded together (one memory latency), later levels one by one
     auto dist = [&](int x, int y) {
-        const int w = y >> 6, ew = f.base_parts[w].ecc;
-        for (int l = 1; l < 64; l++)
-            if ((ball_word(f, l, w, x, ew) >> (y & 63)) & 1ull) return l;
+        const int w = y >> 6, ew = f.base_parts[w].ecc, bit = y & 63;
+        unsigned long long b[4];
+#pragma unroll
+        for (int l = 1; l <= 4; l++) b[l - 1] = ball_word(f, l, w, x, ew);
+#pragma unroll
+        for (int l = 1; l <= 4; l++)
+            if ((b[l - 1] >> bit) & 1ull) return l;
+        for (int l = 5; l < 64; l++)
+            if ((ball_word(f, l, w, x, ew) >> bit) & 1ull) return l;
         return 64;
     };
     const float d1 = (float)f.d, d2 = d1 * d1, n = (float)f.N, inv_diam = 1.f / (float)diam;
