@@ -44,11 +44,8 @@ __global__ void __launch_bounds__(kProbeThreads) k_lds_read(int iters, unsigned 
 }
 
 template <typename T>
-int run_probe(int blocks, int iters, hipStream_t st, unsigned int *d_out, long long *d_clk,
-              float *ms) {
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    GG_HIP(hipEventCreate(&e0));
-    GG_HIP(hipEventCreate(&e1));
+int timed_launches(int blocks, int iters, hipStream_t st, unsigned int *d_out, long long *d_clk,
+                   hipEvent_t e0, hipEvent_t e1, float *ms) {
     hipLaunchKernelGGL(k_lds_read<T>, dim3(blocks), dim3(kProbeThreads), 0, st, 8, d_out, d_clk);
     GG_HIP(hipGetLastError());
     GG_HIP(hipEventRecord(e0, st));
@@ -58,9 +55,20 @@ int run_probe(int blocks, int iters, hipStream_t st, unsigned int *d_out, long l
     GG_HIP(hipEventRecord(e1, st));
     GG_HIP(hipEventSynchronize(e1));
     GG_HIP(hipEventElapsedTime(ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     return GG_OK;
+}
+
+// A warm-up launch, then `iters` iterations timed between two events.
+template <typename T>
+int run_probe(int blocks, int iters, hipStream_t st, unsigned int *d_out, long long *d_clk,
+              float *ms) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = GG_OK;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = GG_HIP_ERROR;
+    if (rc == GG_OK) rc = timed_launches<T>(blocks, iters, st, d_out, d_clk, e0, e1, ms);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    return rc;
 }
 
 }  // namespace
