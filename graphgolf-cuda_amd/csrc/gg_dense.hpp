@@ -25,6 +25,19 @@
 #include <type_traits>
 #include "gg_common.hpp"
 
+namespace gg {
+// Dynamic-LDS size a kernel has been allowed on each device (the attribute
+// only grows).  ensure_dynamic_lds raises it to at least `lds` bytes on the
+// current device, under a process-wide lock: launchers on different host
+// threads share these per-kernel records.
+constexpr int kMaxDevices = 64;
+struct LdsAttr {
+    int bytes[kMaxDevices];
+    LdsAttr() { for (int &b : bytes) b = -1; }
+};
+hipError_t ensure_dynamic_lds(const void *fn, size_t lds, LdsAttr &attr);
+}  // namespace gg
+
 // ---- topologies ---------------------------------------------------------------
 // Offset form, 1-D ring: neighbour i of v is v + tab[i*R + v % R] (mod N) with
 // offsets stored normalised to [0, N).  R = M for 'part' (cppcode/part.hpp),

@@ -57,17 +57,26 @@ extern "C" double gg_aspl(const gg_result *r, int N) {
 // ---- kernel dispatch ------------------------------------------------------------------
 namespace gg {
 
+hipError_t ensure_dynamic_lds(const void *fn, size_t lds, LdsAttr &attr) {
+    static std::mutex mu;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if ((int)lds <= attr.bytes[dev]) return hipSuccess;
+    e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e == hipSuccess) attr.bytes[dev] = (int)lds;
+    return e;
+}
+
 template <class Topo, int BS, int VPT, int DT>
 static hipError_t launch_lds_inst(const Topo &topo, const DenseArgs &a, int grid, size_t lds,
                                   hipStream_t st) {
-    static int configured_bytes = -1;
+    static LdsAttr attr;
     auto fn = &k_dense_lds<Topo, BS, VPT, DT>;
-    if ((int)lds > configured_bytes) {
-        hipError_t e = hipFuncSetAttribute((const void *)fn,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return e;
-        configured_bytes = (int)lds;
-    }
+    hipError_t e = ensure_dynamic_lds((const void *)fn, lds, attr);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(fn, dim3(grid), dim3(BS), lds, st, topo, a);
     return hipGetLastError();
 }
@@ -102,14 +111,10 @@ static hipError_t launch_fixed(const TopoRingQ &topo, const DenseArgs &a, int gr
 template <class Topo, int BS, int VPT, int DT>
 static hipError_t launch_lds_queue_inst(const Topo &topo, const DenseArgs &a, int grid, size_t lds,
                                         hipStream_t st) {
-    static int configured_bytes = -1;
+    static LdsAttr attr;
     auto fn = &k_dense_lds_queue<Topo, BS, VPT, DT>;
-    if ((int)lds > configured_bytes) {
-        hipError_t e = hipFuncSetAttribute((const void *)fn,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return e;
-        configured_bytes = (int)lds;
-    }
+    hipError_t e = ensure_dynamic_lds((const void *)fn, lds, attr);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(fn, dim3(grid), dim3(BS), lds, st, topo, a);
     return hipGetLastError();
 }

@@ -7,29 +7,19 @@
 
 namespace gg {
 
-template <class K>
-static hipError_t set_lds(K fn, size_t lds, int &configured) {
-    if ((int)lds > configured) {
-        hipError_t e = hipFuncSetAttribute((const void *)fn,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return e;
-        configured = (int)lds;
-    }
-    return hipSuccess;
-}
 
 template <int VPT, int DT>
 static hipError_t launch_fixed_inst(const TopoRingQ &topo, const DenseArgs &a, int grid,
                                     size_t lds, bool queue, hipStream_t st) {
-    static int cfg_lds = -1, cfg_queue = -1;
+    static LdsAttr cfg_lds, cfg_queue;
     hipError_t e;
     if (queue) {
         auto fn = &k_dense_lds_queue<TopoRingQ, 1024, VPT, DT>;
-        if ((e = set_lds(fn, lds, cfg_queue)) != hipSuccess) return e;
+        if ((e = ensure_dynamic_lds((const void *)fn, lds, cfg_queue)) != hipSuccess) return e;
         hipLaunchKernelGGL(fn, dim3(grid), dim3(1024), lds, st, topo, a);
     } else {
         auto fn = &k_dense_lds<TopoRingQ, 1024, VPT, DT>;
-        if ((e = set_lds(fn, lds, cfg_lds)) != hipSuccess) return e;
+        if ((e = ensure_dynamic_lds((const void *)fn, lds, cfg_lds)) != hipSuccess) return e;
         hipLaunchKernelGGL(fn, dim3(grid), dim3(1024), lds, st, topo, a);
     }
     return hipGetLastError();

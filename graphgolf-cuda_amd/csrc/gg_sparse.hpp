@@ -555,17 +555,13 @@ static __global__ void k_queue_reset(int *ctl, int n) {
 template <int DT>
 hipError_t launch_sparse_deg(const SparseArgs &s, const TopoRingQ &topo, const DenseArgs &qa,
                              size_t lds, int vpt, hipStream_t st) {
-    auto go = [&](auto fn, int &configured, int grid) -> hipError_t {
-        if ((int)lds > configured) {
-            hipError_t e = hipFuncSetAttribute((const void *)fn,
-                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-            if (e != hipSuccess) return e;
-            configured = (int)lds;
-        }
+    auto go = [&](auto fn, gg::LdsAttr &attr, int grid) -> hipError_t {
+        hipError_t e = gg::ensure_dynamic_lds((const void *)fn, lds, attr);
+        if (e != hipSuccess) return e;
         hipLaunchKernelGGL(fn, dim3(grid), dim3(1024), lds, st, s, topo, qa);
         return hipGetLastError();
     };
-    static int c0 = -1, c4 = -1, c8 = -1, c16 = -1;
+    static gg::LdsAttr c0, c4, c8, c16;
     // dense-tail instances may take extra workgroups; one per column otherwise
     const int g = s.grid > s.W ? s.grid : s.W;
     if constexpr (DT > 0) {
