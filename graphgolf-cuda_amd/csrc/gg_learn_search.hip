@@ -525,17 +525,26 @@ int learn_observe(gg_search *s, int K) {
     if (!frozen) {
         const int need = std::max(64, std::min(s->cfg.warmup_pairs, 4096));
         if (L->count < need) return GG_OK;
-        // mean / std of the finite raw deltas seen so far, then normalise all
-        double m = 0, v = 0;
-        int n = 0;
+        // Median and 1.4826 x MAD of the finite raw deltas seen so far, then
+        // normalise all.  A diameter change adds (+-1) N^2 to a delta (about
+        // 1e6 times the spread of the ordinary ones): a single such move in the
+        // sample would set a mean / std scale that flattens every other target
+        // to ~0.  With the robust scale those moves land in the +-8 clip.
+        std::vector<double> fin;
+        fin.reserve(L->count);
         for (int i = 0; i < L->count; i++)
-            if (L->raw[i] < 1e29) { m += L->raw[i]; n++; }
-        m = n ? m / n : 0.0;
-        for (int i = 0; i < L->count; i++)
-            if (L->raw[i] < 1e29) v += (L->raw[i] - m) * (L->raw[i] - m);
-        v = n > 1 ? v / n : 1.0;
-        L->mu = m;
-        L->sigma = v > 0 ? std::sqrt(v) : 1.0;
+            if (L->raw[i] < 1e29) fin.push_back(L->raw[i]);
+        double med = 0, mad = 0;
+        if (!fin.empty()) {
+            auto mid = fin.begin() + fin.size() / 2;
+            std::nth_element(fin.begin(), mid, fin.end());
+            med = *mid;
+            for (double &x : fin) x = std::fabs(x - med);
+            std::nth_element(fin.begin(), mid, fin.end());
+            mad = *mid;
+        }
+        L->mu = med;
+        L->sigma = mad > 0 ? 1.4826 * mad : 1.0;
         L->norm_frozen = true;
         st = upload_targets(L, 0, L->count);
         if (st != GG_OK) return st;
