@@ -764,3 +764,4 @@ __global__ void __launch_bounds__(1024) k_dense_global(Topo topo, DenseArgs a, i
         __syncthreads();
     }
 }
+

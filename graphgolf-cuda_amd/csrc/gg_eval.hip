@@ -136,6 +136,217 @@ static hipError_t launch_global(const Topo &topo, const DenseArgs &a, int grid, 
     return hipGetLastError();
 }
 
+// ---- level-synchronous BFS over the whole grid (N too large for LDS, few columns) ----
+// The persistent kernel above runs each column on one workgroup, so with a
+// handful of columns (a symmetric graph of 10^6 vertices has M <= 128
+// representatives: two columns) all but a few CUs idle.  Here one launch is
+// one level of every column: thread (v, w) owns word w of vertex v, the words
+// of a vertex are contiguous ([N][Wp], Wp = the power of two >= W), so the
+// lanes reading one neighbour's words share its cache lines.  Per-level
+// new-pair counts and new-bit masks per column go to cnt/lor [lmax+1][Wp].
+struct WideArgs {
+    int N, M, D, W, wsh, lmax;          // Wp = 1 << wsh
+    const int *src_vertex;              // [M]
+    const gg_swap *swaps;               // [1] or nullptr
+    unsigned long long *cnt, *lor;      // [lmax+1][Wp]
+};
+
+// cur := padding bits (columns >= W: all ones), counters of levels 1..lmax
+// cleared, level 0 holds each column's valid source bits and their count.
+__global__ void __launch_bounds__(256) k_wide_init(WideArgs a, unsigned long long *cur) {
+    const size_t Wp = (size_t)1 << a.wsh;
+    const size_t total = (size_t)a.N << a.wsh;
+    const size_t stride = (size_t)gridDim.x * 256;
+    for (size_t t = (size_t)blockIdx.x * 256 + threadIdx.x; t < total; t += stride) {
+        const int w = (int)(t & (Wp - 1));
+        cur[t] = w < a.W ? pad_mask(w, a.M) : ~0ull;
+    }
+    const size_t nc = (size_t)(a.lmax + 1) * Wp;
+    for (size_t t = (size_t)blockIdx.x * 256 + threadIdx.x; t < nc; t += stride) {
+        const int w = (int)(t & (Wp - 1));
+        const unsigned long long src = w < a.W ? ~pad_mask(w, a.M) : 0ull;
+        const bool l0 = t < Wp;
+        a.cnt[t] = l0 ? (unsigned long long)__popcll(src) : 0ull;
+        a.lor[t] = l0 ? src : 0ull;
+    }
+}
+
+// Source s sets bit s & 63 of word s >> 6 of its vertex (after k_wide_init).
+__global__ void __launch_bounds__(256) k_wide_sources(WideArgs a, unsigned long long *cur) {
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    if (s >= a.M) return;
+    const size_t i = ((size_t)a.src_vertex[s] << a.wsh) + (size_t)(s >> 6);
+    cur[i] |= 1ull << (s & 63);
+}
+
+// Level `level` of every column: nxt = cur | (OR of the listed neighbours'
+// words & ~cur).  A column with no new bits at the previous level is closed
+// and skipped (its counters stay zero); a launch after every column closed
+// returns at once.  Each thread keeps one column w and strides over vertices,
+// so counts are summed in registers, then over the lanes and waves holding
+// the same w, then added to the level's counters (one atomic per column and
+// workgroup).
+template <class Topo>
+__global__ void __launch_bounds__(256) k_wide_level(Topo topo, WideArgs a, int level,
+                                                    const unsigned long long *cur,
+                                                    unsigned long long *nxt) {
+    __shared__ unsigned long long s_cnt[256], s_lor[256];
+    __shared__ int s_live;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int Wp = 1 << a.wsh;
+    const size_t t0 = (size_t)blockIdx.x * 256 + tid;
+    const int w = (int)(t0 & (size_t)(Wp - 1));
+    const unsigned long long *prev_lor = a.lor + (size_t)(level - 1) * Wp;
+    if (tid == 0) s_live = 0;
+    if (tid < Wp) { s_cnt[tid] = 0; s_lor[tid] = 0; }
+    __syncthreads();
+    for (int i = tid; i < a.W; i += 256)
+        if (prev_lor[i]) s_live = 1;  // benign race: every writer stores 1
+    __syncthreads();
+    if (!s_live) return;  // uniform: every column closed
+    SwapRegs sw;
+    bool has_sw;
+    load_swap(a.swaps, 0, sw, has_sw);
+    const bool live = w < a.W && prev_lor[w] != 0;
+    unsigned long long cnt = 0, lor = 0;
+    if (live) {
+        const size_t vstride = ((size_t)gridDim.x * 256) >> a.wsh;
+        for (size_t v = t0 >> a.wsh; v < (size_t)a.N; v += vstride) {
+            const size_t i = (v << a.wsh) + (size_t)w;
+            const unsigned long long own = cur[i];
+            unsigned long long n = 0;
+            if (~own) {
+                unsigned long long acc = 0;
+                for (int j = 0; j < a.D; j++) {
+                    const unsigned u = nbr_patched(topo, (unsigned)v, j, sw, has_sw);
+                    acc |= cur[((size_t)u << a.wsh) + (size_t)w];
+                }
+                n = acc & ~own;
+            }
+            nxt[i] = own | n;
+            cnt += (unsigned long long)__popcll(n);
+            lor |= n;
+        }
+    }
+    // lanes l and l ^ o (o >= Wp) hold the same column
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        if (o < Wp) break;
+        cnt += __shfl_xor(cnt, o, 64);
+        lor |= __shfl_xor(lor, o, 64);
+    }
+    if (live && lane < Wp && (cnt || lor)) {
+        atomicAdd(&s_cnt[w], cnt);
+        atomicOr(&s_lor[w], lor);
+    }
+    __syncthreads();
+    if (tid < Wp && tid < a.W && s_cnt[tid]) {
+        atomicAdd(&a.cnt[(size_t)level * Wp + tid], s_cnt[tid]);
+        atomicOr(&a.lor[(size_t)level * Wp + tid], s_lor[tid]);
+    }
+}
+
+// Column parts (and per-source eccentricities / per-level counts) from the
+// level counters, in the form the persistent kernels write them.
+__global__ void __launch_bounds__(256) k_wide_parts(WideArgs a, ColPart *parts, int *ecc_out,
+                                                    long long *reach_col) {
+    const int Wp = 1 << a.wsh;
+    for (int w = threadIdx.x; w < a.W; w += 256) {
+        long long total = 0, reached = (long long)a.cnt[w];
+        int ecc = 0;
+        for (int l = 1; l <= a.lmax; l++) {
+            const long long c = (long long)a.cnt[(size_t)l * Wp + w];
+            total += (long long)l * c;
+            reached += c;
+            if (c) ecc = l;
+            if (reach_col) reach_col[(size_t)w * (a.lmax + 1) + l] = c;
+        }
+        if (reach_col) reach_col[(size_t)w * (a.lmax + 1)] = (long long)a.cnt[w];
+        ColPart p;
+        p.total = total;
+        p.reached = reached;
+        p.ecc = ecc;
+        p.levels = ecc < a.lmax ? ecc + 1 : a.lmax;
+        p.flags = ecc >= a.lmax ? GG_FLAG_LIMIT : 0;
+        p.pad = 0;
+        parts[w] = p;
+    }
+    if (ecc_out) {
+        for (int s = threadIdx.x; s < a.M; s += 256) {
+            const int w = s >> 6;
+            const unsigned long long bit = 1ull << (s & 63);
+            int e = 0;
+            for (int l = a.lmax; l >= 1; l--)
+                if (a.lor[(size_t)l * Wp + w] & bit) { e = l; break; }
+            ecc_out[s] = e;
+        }
+    }
+}
+
+// Level-synchronous path: columns of at most 256 words per vertex.
+static const int kWideMaxCols = 256;
+static const int kWideChunk = 4;  // levels launched between reads of the counters
+
+// Grows the evaluator's global scratch to `need` bytes.
+static int ensure_gscratch(DenseCtx &c, size_t need) {
+    if (need <= c.gscratch_bytes) return GG_OK;
+    if (c.gscratch) { (void)hipFree(c.gscratch); c.gscratch = nullptr; c.gscratch_bytes = 0; }
+    if (hipMalloc((void **)&c.gscratch, need) != hipSuccess) return GG_OOM;
+    c.gscratch_bytes = need;
+    return GG_OK;
+}
+
+// One launch per level over all (vertex, column) words; between chunks of
+// kWideChunk levels the host reads the last level's new-bit masks and stops
+// once every column is closed.  Words [N][Wp], then the level counters.
+template <class Topo>
+static int run_wide(DenseCtx &c, const DenseArgs &a, const Topo &topo, hipStream_t st) {
+    int wsh = 0;
+    while ((1 << wsh) < a.W) wsh++;
+    const size_t Wp = (size_t)1 << wsh;
+    const size_t words = (size_t)a.N * Wp, nctr = (size_t)(a.lmax + 1) * Wp;
+    int rc = ensure_gscratch(c, (2 * words + 2 * nctr) * sizeof(unsigned long long));
+    if (rc != GG_OK) return rc;
+    unsigned long long *cur = c.gscratch, *nxt = cur + words;
+    WideArgs wa;
+    wa.N = a.N; wa.M = a.M; wa.D = a.D; wa.W = a.W; wa.wsh = wsh; wa.lmax = a.lmax;
+    wa.src_vertex = a.src_vertex;
+    wa.swaps = a.swaps;
+    wa.cnt = nxt + words;
+    wa.lor = wa.cnt + nctr;
+    const size_t want = (words + 255) / 256;
+    const int grid = (int)(want < (size_t)8 * c.cu_count ? want : (size_t)8 * c.cu_count);
+    hipLaunchKernelGGL(k_wide_init, dim3(grid), dim3(256), 0, st, wa, cur);
+    hipLaunchKernelGGL(k_wide_sources, dim3((a.M + 255) / 256), dim3(256), 0, st, wa, cur);
+    std::vector<unsigned long long> h(a.W);
+    int level = 1;
+    while (level <= a.lmax) {
+        const int end = level + kWideChunk - 1 < a.lmax ? level + kWideChunk - 1 : a.lmax;
+        for (; level <= end; level++) {
+            hipLaunchKernelGGL(k_wide_level<Topo>, dim3(grid), dim3(256), 0, st, topo, wa, level,
+                               (const unsigned long long *)cur, nxt);
+            unsigned long long *t = cur; cur = nxt; nxt = t;
+        }
+        if (level > a.lmax) break;
+        if (hipMemcpyAsync(h.data(), wa.lor + (size_t)(level - 1) * Wp, sizeof(h[0]) * a.W,
+                           hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess) {
+            gg_set_last_hip_error(hipGetLastError(), "level counters", __FILE__, __LINE__);
+            return GG_HIP_ERROR;
+        }
+        bool any = false;
+        for (int w = 0; w < a.W; w++) any = any || h[w] != 0;
+        if (!any) break;
+    }
+    hipLaunchKernelGGL(k_wide_parts, dim3(1), dim3(256), 0, st, wa, a.parts, a.ecc_out, a.reach_col);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        gg_set_last_hip_error(e, "level-synchronous BFS launch", __FILE__, __LINE__);
+        return GG_HIP_ERROR;
+    }
+    return GG_OK;
+}
+
 // Smallest supported VPT >= need for block size bs (0 if none).
 int pick_vpt(int bs, int need) {
     static const int v256[] = {1, 2, 4, 8, 16};
@@ -205,15 +416,28 @@ int run_dense(DenseCtx &c, const DenseArgs &a_in, hipStream_t st) {
             case TOPO_PIECE: e = launch_lds(c.piece, a, tasks, lds, bs, vpt, st); break;
             default: e = launch_lds(c.box, a, tasks, lds, bs, vpt, st); break;
         }
+    } else if (c.wide_ok && !c.wide_off && a.K == 1 && !a.queue && !a.snap &&
+               a.W <= kWideMaxCols) {
+        // few columns of a large graph: every level spread over the whole grid
+        c.last_variant = 4;
+        c.last_vpt = 0;
+        c.last_fixed = 0;
+        c.last_push_cap = 0;
+        int rc;
+        switch (c.topo_kind) {
+            case TOPO_RINGQ: rc = run_wide(c, a, c.ringq, st); break;
+            case TOPO_RING32: rc = run_wide(c, a, c.ring32, st); break;
+            case TOPO_PIECE: rc = run_wide(c, a, c.piece, st); break;
+            default: rc = run_wide(c, a, c.box, st); break;
+        }
+        if (rc != GG_OK) return rc;
+        e = hipSuccess;
     } else {
         // global-memory variant: persistent grid, 2*N words of scratch per workgroup
         int grid = tasks < 2 * c.cu_count ? tasks : 2 * c.cu_count;
         size_t need = (size_t)grid * 2 * a.N * sizeof(unsigned long long);
-        if (need > c.gscratch_bytes) {
-            if (c.gscratch) { (void)hipFree(c.gscratch); c.gscratch = nullptr; c.gscratch_bytes = 0; }
-            if (hipMalloc((void **)&c.gscratch, need) != hipSuccess) return GG_OOM;
-            c.gscratch_bytes = need;
-        }
+        int rc = ensure_gscratch(c, need);
+        if (rc != GG_OK) return rc;
         DenseArgs g = a;
         g.gscratch = c.gscratch;
         g.push_cap = 0;
@@ -410,6 +634,7 @@ static int eval_alloc(gg_eval *ev, size_t tab_elem_bytes) {
     GG_HIP(hipStreamSynchronize(ev->stream));
     ev->dc.lds_device = device_lds_limit(ev->device);
     ev->dc.cu_count = device_cu_count(ev->device);
+    ev->dc.wide_ok = 1;  // single evaluations, never captured
     return GG_OK;
 }
 
@@ -809,6 +1034,9 @@ extern "C" int gg_eval_set_option(gg_eval *ev, int option, int value) {
         case GG_OPT_PUSH:
             ev->dc.push_off = value ? 0 : 1;
             return GG_OK;
+        case GG_OPT_WIDE:
+            ev->dc.wide_off = value ? 0 : 1;
+            return GG_OK;
         default: return GG_BAD_ARG;
     }
 }
@@ -821,6 +1049,7 @@ extern "C" int gg_eval_get_option(gg_eval *ev, int option, int *value) {
         case GG_OPT_GENERIC: *value = ev->dc.generic_opt; return GG_OK;
         case GG_OPT_LEVEL_LIMIT: *value = ev->lmax; return GG_OK;
         case GG_OPT_PUSH: *value = ev->dc.push_off ? 0 : 1; return GG_OK;
+        case GG_OPT_WIDE: *value = ev->dc.wide_off ? 0 : 1; return GG_OK;
         case 100: *value = ev->dc.last_variant; return GG_OK;  // kernel variant of the last call
         case 101: *value = ev->dc.last_vpt; return GG_OK;      // its vertices per thread
         case 102: *value = ev->dc.last_fixed; return GG_OK;    // its fixed degree (0 = generic)

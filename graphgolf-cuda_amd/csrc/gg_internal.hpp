@@ -22,9 +22,12 @@ struct DenseCtx {
     int generic_opt = 0;      // 1 = runtime-degree kernels even when a specialisation exists
     int push_off = 0;         // 1 = pull from level 1 (no push levels)
     int push_ok = 0;          // the graph of this launch is known to be undirected
+    int wide_ok = 0;          // level-synchronous path allowed (never under graph capture:
+                              // the host reads the level counters between launches)
+    int wide_off = 0;         // 1 = persistent global kernel only (GG_OPT_WIDE 0)
     hipEvent_t bfs_done = nullptr;  // if set, recorded between the BFS and finalize kernels
     int bfs_only = 0;         // 1 = launch the BFS kernel alone (no finalize): timing
-    int last_variant = 0;     // 1 = LDS/256, 2 = LDS/1024, 3 = global
+    int last_variant = 0;     // 1 = LDS/256, 2 = LDS/1024, 3 = global, 4 = level-synchronous
     int last_vpt = 0;         // vertices per thread of the last LDS launch
     int last_fixed = 0;       // fixed-degree instance of the last launch (0 = generic)
     int last_push_cap = 0;    // frontier list capacity of the last launch (0 = pull only)

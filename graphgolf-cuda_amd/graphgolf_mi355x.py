@@ -32,6 +32,7 @@ GG_OPT_LDS_LIMIT = 2
 GG_OPT_GENERIC = 3
 GG_OPT_LEVEL_LIMIT = 4
 GG_OPT_PUSH = 5
+GG_OPT_WIDE = 6
 GG_INFO_VARIANT = 100
 GG_INFO_VPT = 101
 GG_INFO_FIXED = 102

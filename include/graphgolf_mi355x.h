@@ -141,8 +141,14 @@ int gg_eval_time_bfs(gg_eval *ev, int reps, float *ms);
 #define GG_OPT_PUSH       5   /* 1 (default) = levels 1-2 push from the frontier when
                                  the graph is undirected (adjacency evaluators verify
                                  it on upload and per swap); 0 = pull only           */
+#define GG_OPT_WIDE       6   /* 1 (default) = a single evaluation whose column bitset
+                                 does not fit in LDS and that has at most 256 columns
+                                 runs level-synchronously over the whole grid (one
+                                 launch per level, GG_INFO_VARIANT 4); 0 = one
+                                 workgroup per column (GG_INFO_VARIANT 3)            */
 /* Read-only facts about the most recent launch (gg_eval_get_option). */
-#define GG_INFO_VARIANT   100 /* 1 = LDS bitset / 256 threads, 2 = LDS / 1024, 3 = HBM */
+#define GG_INFO_VARIANT   100 /* 1 = LDS bitset / 256 threads, 2 = LDS / 1024, 3 = HBM,
+                                 4 = HBM level-synchronous                          */
 #define GG_INFO_VPT       101 /* vertices per thread of the LDS kernel             */
 #define GG_INFO_FIXED     102 /* compile-time degree of the kernel (0 = runtime)     */
 #define GG_INFO_PUSH_CAP  103 /* frontier list capacity (0 = pull-only launch)      */

@@ -1,0 +1,121 @@
+"""GPU parity of the level-synchronous path: a single evaluation whose column
+bitset does not fit in LDS and that has at most 256 columns runs one launch
+per level over the whole grid (GG_INFO_VARIANT 4).  Checked bit for bit
+against the CPU oracle and against the persistent one-workgroup-per-column
+global kernel (GG_OPT_WIDE 0, GG_INFO_VARIANT 3): distance totals, diameters,
+per-source eccentricities, per-level counts, WVC, status codes."""
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+
+pytestmark = pytest.mark.gpu
+
+
+def _both(gg, make, evaluate, lds_limit=1024):
+    """(result dict, ecc, reach, variant) with the level-synchronous path and
+    with the persistent global kernel; lds_limit forces the global kernels."""
+    out = []
+    for wide in (1, 0):
+        with make() as ev:
+            ev.set_option(gg.GG_OPT_LDS_LIMIT, lds_limit)
+            ev.set_option(gg.GG_OPT_WIDE, wide)
+            r = evaluate(ev)
+            ecc, reach = ev.levels()
+            out.append((r.as_dict(), ecc, reach, ev.get_option(gg.GG_INFO_VARIANT)))
+    (rw, ew, cw, vw), (rp, ep, cp, vp) = out
+    assert (vw, vp) == (4, 3)
+    assert rw == rp
+    assert np.array_equal(ew, ep) and np.array_equal(cw, cp)
+    return rw, ew, cw
+
+
+@pytest.mark.parametrize("n,d,seed", [(2048, 8, 17), (3000, 5, 2), (1000, 6, 3)])
+def test_adjacency_matches_oracle(gpu, n, d, seed):
+    """Full and ragged last columns (3000 and 1000 sources), W = 47 pads to 64."""
+    gg = gpu
+    adj = gg.random_regular(n, d, seed)
+    o = ol.adj_eval(adj)
+    r, ecc, reach = _both(gg, lambda: gg.AdjEvaluator(n, d), lambda ev: ev.eval(adj))
+    assert r["status"] == gg.GG_OK
+    assert r["total_reps"] == o["total"] and r["diameter"] == o["diameter"]
+    assert r["levels"] == o["diameter"] + 1
+    assert np.array_equal(ecc, o["ecc"])
+    assert np.array_equal(reach[:len(o["reach"])], o["reach"][:len(reach)])
+
+
+def test_part_piece_box_match_oracle(gpu):
+    gg = gpu
+    g, m, d = 64, 96, 6
+    N = g * m
+    tab = gg.orbit_part(g, m, d, 5)
+    o = ol.part_eval(N, m, tab)
+    r, ecc, _ = _both(gg, lambda: gg.PartEvaluator(N, m, d), lambda ev: ev.eval(tab))
+    assert r["total_scaled"] == o["total_scaled"] and r["diameter"] == o["diameter"]
+    assert r["wvc"] == o["wvc"]
+    assert np.array_equal(ecc, o["ecc"])
+
+    Nx, Ny, Mx, My, d = 32, 64, 2, 64, 6
+    tab = gg.gen_lattice((Nx, Ny), (Mx, My), d, 7)
+    o = ol.piece_eval(Nx, Ny, Mx, My, tab)
+    r, ecc, _ = _both(gg, lambda: gg.PieceEvaluator(Nx, Ny, Mx, My, d), lambda ev: ev.eval(tab))
+    assert r["total_scaled"] == o["total_scaled"] and r["diameter"] == o["diameter"]
+    assert np.array_equal(ecc, o["ecc"])
+
+    Ns, Ms, d = (8, 8, 16), (2, 2, 16), 6
+    tab = gg.gen_lattice(Ns, Ms, d, 9)
+    o = ol.box_eval(*Ns, *Ms, tab)
+    r, ecc, _ = _both(gg, lambda: gg.BoxEvaluator(*Ns, *Ms, d), lambda ev: ev.eval(tab))
+    assert r["total_scaled"] == o["total_scaled"] and r["diameter"] == o["diameter"]
+    assert np.array_equal(ecc, o["ecc"])
+
+
+def test_swap_unconnected_and_level_limit(gpu):
+    gg = gpu
+    n, d = 2048, 8
+    adj = gg.random_regular(n, d, 6)
+    c = 700
+    sw, new = gg.make_swap(adj, 5, 1, c, 2, 0)
+    while sw is None:
+        c += 1
+        sw, new = gg.make_swap(adj, 5, 1, c, 2, 0)
+    o = ol.adj_eval(new)
+
+    def swap_eval(ev):
+        ev.set_base(adj)
+        return ev.eval_swap(sw)
+    r, ecc, _ = _both(gg, lambda: gg.SwapEvaluator(n, d), swap_eval)
+    assert r["total_reps"] == o["total"] and r["diameter"] == o["diameter"]
+    assert np.array_equal(ecc, o["ecc"])
+
+    # two disjoint random regular graphs of 600 vertices
+    h = gg.random_regular(600, 4, 3)
+    two = np.concatenate([h, h + 600]).astype(np.int32)
+    o = ol.adj_eval(two)
+    r, _, _ = _both(gg, lambda: gg.AdjEvaluator(1200, 4), lambda ev: ev.eval(two))
+    assert o["unconnected"] and r["status"] == gg.GG_UNCONNECTED
+    assert r["total_reps"] == o["total"]
+
+    def limited(ev):
+        ev.set_option(gg.GG_OPT_LEVEL_LIMIT, 2)
+        return ev.eval(adj)
+    r, _, _ = _both(gg, lambda: gg.AdjEvaluator(n, d), limited)
+    assert r["status"] == gg.GG_DIAMETER_LIMIT
+
+
+def test_large_orbit_graph_matches_oracle(gpu):
+    """N = 2^18 with 128 representatives (two columns): the level-synchronous
+    path is the automatic choice and matches the oracle."""
+    gg = gpu
+    g, m, d = 2048, 128, 16
+    N = g * m
+    tab = gg.orbit_part(g, m, d, 3)
+    o = ol.part_eval(N, m, tab)
+    with gg.PartEvaluator(N, m, d) as ev:
+        r = ev.eval(tab)
+        assert ev.get_option(gg.GG_INFO_VARIANT) == 4
+        ecc, _ = ev.levels()
+    assert r.status == gg.GG_OK
+    assert r.total_scaled == o["total_scaled"] and r.diameter == o["diameter"]
+    assert r.wvc == o["wvc"]
+    assert np.array_equal(ecc, o["ecc"])
