@@ -258,6 +258,37 @@ def run_search(gg, adj, cfg, warmup, steps):
     return s, st, st.exact_evals - e0, st.seconds - t0
 
 
+def large_n_orbit(gg, g=8192, m=128, d=16, seed=3, reps=5):
+    """Large-N evaluation: an orbit graph of N = g*m = 2^20 vertices with m = 128
+    representatives (two 64-source columns; the reference's own large-N design
+    point, cudaASPLconv.cu:157).  The level-synchronous path (one launch per
+    level over the whole grid) against the persistent one-workgroup-per-column
+    global kernel; algorithmic bytes 8*(d+2)*N*W*L as for the dense kernel."""
+    N = g * m
+    tab = gg.orbit_part(g, m, d, seed)
+    out = {"workload": f"orbit graph N={N}, {m} representatives, d={d}"}
+    with gg.PartEvaluator(N, m, d) as ev:
+        r = ev.eval(tab)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            r2 = ev.eval(tab)
+        wide_ms = (time.perf_counter() - t0) / reps * 1e3
+        assert r2.total_scaled == r.total_scaled
+        bfs_ms = ev.time_bfs(reps)
+        W = (m + 63) // 64
+        alg = 8 * (d + 2) * N * W * r.levels
+        out.update({"diameter": r.diameter, "levels": r.levels, "total_scaled": r.total_scaled,
+                    "level_sync_ms_per_eval": wide_ms, "level_sync_bfs_ms": bfs_ms,
+                    "level_sync_alg_GB_per_s": alg / (bfs_ms * 1e-3) / 1e9})
+        ev.set_option(gg.GG_OPT_WIDE, 0)
+        t0 = time.perf_counter()
+        r3 = ev.eval(tab)
+        pers_ms = (time.perf_counter() - t0) * 1e3
+        assert ev.get_option(gg.GG_INFO_VARIANT) == 3 and r3.total_scaled == r.total_scaled
+        out.update({"persistent_ms_per_eval": pers_ms, "speedup": pers_ms / wide_ms})
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -270,10 +301,11 @@ def main():
     ap.add_argument("--seed", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
-    ap.add_argument("--only", choices=["roofline", "copy", "learned"], default=None,
+    ap.add_argument("--only", choices=["roofline", "copy", "learned", "large"], default=None,
                     help="run only the dense-kernel roofline launches, only a device copy of "
                          "known size (the two legs of the rocprofv3 --pmc traffic runs), or "
-                         "only the learned search (for kernel profiles of its step)")
+                         "only the learned search (for kernel profiles of its step), or only the "
+                         "large-N orbit evaluation")
     ap.add_argument("--equal-time", type=float, default=0.0,
                     help="also run plain and learned for this many seconds each and report "
                          "the final ASPL of both")
@@ -286,6 +318,9 @@ def main():
     n, d = args.n, args.d
     if args.only == "copy":
         copy_calibration()
+        return
+    if args.only == "large":
+        print(json.dumps({"large_n": large_n_orbit(gg)}), flush=True)
         return
     adj = gg.random_regular(n, d, args.seed)
     if args.only == "roofline":
@@ -334,6 +369,7 @@ def main():
     if not args.no_roofline:
         out["roofline"] = plain_kernel_roofline(gg, adj)
     out["batched_vs_sequential"] = batched_vs_sequential(gg)
+    out["large_n"] = large_n_orbit(gg)
     if args.equal_time > 0:
         res = {}
         for name, cfg in (("plain", plain_cfg), ("learned", learn_cfg)):

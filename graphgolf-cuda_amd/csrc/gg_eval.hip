@@ -247,15 +247,16 @@ __global__ void __launch_bounds__(256) k_wide_level(Topo topo, WideArgs a, int l
 }
 
 // Column parts (and per-source eccentricities / per-level counts) from the
-// level counters, in the form the persistent kernels write them.
-__global__ void __launch_bounds__(256) k_wide_parts(WideArgs a, ColPart *parts, int *ecc_out,
-                                                    long long *reach_col) {
+// counters of levels 1..lexec (the levels launched), in the form the
+// persistent kernels write them.
+__global__ void __launch_bounds__(256) k_wide_parts(WideArgs a, int lexec, ColPart *parts,
+                                                    int *ecc_out, long long *reach_col) {
     const int Wp = 1 << a.wsh;
     for (int w = threadIdx.x; w < a.W; w += 256) {
         long long total = 0, reached = (long long)a.cnt[w];
         int ecc = 0;
         for (int l = 1; l <= a.lmax; l++) {
-            const long long c = (long long)a.cnt[(size_t)l * Wp + w];
+            const long long c = l <= lexec ? (long long)a.cnt[(size_t)l * Wp + w] : 0;
             total += (long long)l * c;
             reached += c;
             if (c) ecc = l;
@@ -276,7 +277,7 @@ __global__ void __launch_bounds__(256) k_wide_parts(WideArgs a, ColPart *parts, 
             const int w = s >> 6;
             const unsigned long long bit = 1ull << (s & 63);
             int e = 0;
-            for (int l = a.lmax; l >= 1; l--)
+            for (int l = lexec; l >= 1; l--)
                 if (a.lor[(size_t)l * Wp + w] & bit) { e = l; break; }
             ecc_out[s] = e;
         }
@@ -338,7 +339,9 @@ static int run_wide(DenseCtx &c, const DenseArgs &a, const Topo &topo, hipStream
         for (int w = 0; w < a.W; w++) any = any || h[w] != 0;
         if (!any) break;
     }
-    hipLaunchKernelGGL(k_wide_parts, dim3(1), dim3(256), 0, st, wa, a.parts, a.ecc_out, a.reach_col);
+    const int lexec = level - 1 < a.lmax ? level - 1 : a.lmax;  // last level launched
+    hipLaunchKernelGGL(k_wide_parts, dim3(1), dim3(256), 0, st, wa, lexec, a.parts, a.ecc_out,
+                       a.reach_col);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
         gg_set_last_hip_error(e, "level-synchronous BFS launch", __FILE__, __LINE__);
