@@ -149,6 +149,16 @@ struct WideArgs {
     const int *src_vertex;              // [M]
     const gg_swap *swaps;               // [1] or nullptr
     unsigned long long *cnt, *lor;      // [lmax+1][Wp]
+    // push levels (undirected graphs only; fcap = 0: pull from level 1).
+    // Frontier lists [2][fcap] of (word index v*Wp + w, new bits); list
+    // (l & 1) holds what level l added, list 0 starts with the sources.
+    // ctl[2]: entries appended to each list (may exceed fcap: overflow).
+    // mode[l] = 1: level l pushes from list (l-1) & 1; a level that pulls, or
+    // whose push overflows its list, sets mode[l+1] = 0, so pulling persists.
+    int fcap;
+    unsigned *fv;
+    unsigned long long *fb;
+    int *ctl, *mode;
 };
 
 // cur := padding bits (columns >= W: all ones), counters of levels 1..lmax
@@ -169,6 +179,11 @@ __global__ void __launch_bounds__(256) k_wide_init(WideArgs a, unsigned long lon
         a.cnt[t] = l0 ? (unsigned long long)__popcll(src) : 0ull;
         a.lor[t] = l0 ? src : 0ull;
     }
+    if (a.fcap) {
+        const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+        if (t < (size_t)a.lmax + 2) a.mode[t] = t >= 1 ? 1 : 0;
+        if (t < 2) a.ctl[t] = t == 0 ? a.M : 0;
+    }
 }
 
 // Source s sets bit s & 63 of word s >> 6 of its vertex (after k_wide_init).
@@ -177,6 +192,94 @@ __global__ void __launch_bounds__(256) k_wide_sources(WideArgs a, unsigned long 
     if (s >= a.M) return;
     const size_t i = ((size_t)a.src_vertex[s] << a.wsh) + (size_t)(s >> 6);
     cur[i] |= 1ull << (s & 63);
+    if (a.fcap) {
+        a.fv[s] = (unsigned)i;
+        a.fb[s] = 1ull << (s & 63);
+    }
+}
+
+// Push level: nxt := cur (the push ORs into it), and the list this level
+// fills starts empty.
+__global__ void __launch_bounds__(256) k_wide_copy(WideArgs a, int level,
+                                                   const unsigned long long *cur,
+                                                   unsigned long long *nxt) {
+    if (!a.fcap || !a.mode[level]) return;
+    const size_t total = (size_t)a.N << a.wsh;
+    const size_t stride = (size_t)gridDim.x * 256;
+    for (size_t t = (size_t)blockIdx.x * 256 + threadIdx.x; t < total; t += stride) nxt[t] = cur[t];
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.ctl[level & 1] = 0;
+}
+
+// Push level `level`: every (entry, neighbour) item ORs the entry's new bits
+// into the neighbour's word of the same column.  The old word returned by
+// the atomic tells which bits this item set first, so counts are exact and
+// each bit new at this level is appended (wave-aggregated) exactly once.
+// Equal to the pull level only on undirected graphs (pull reads listed
+// neighbours, push follows listed edges), which the host checks.
+template <class Topo>
+__global__ void __launch_bounds__(256) k_wide_push(Topo topo, WideArgs a, int level,
+                                                   unsigned long long *nxt) {
+    if (!a.fcap || !a.mode[level]) return;  // uniform
+    __shared__ unsigned long long s_cnt[256], s_lor[256];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int Wp = 1 << a.wsh;
+    if (tid < Wp) { s_cnt[tid] = 0; s_lor[tid] = 0; }
+    __syncthreads();
+    SwapRegs sw;
+    bool has_sw;
+    load_swap(a.swaps, 0, sw, has_sw);
+    const int pin = (level - 1) & 1, pout = level & 1;
+    const unsigned *fv_in = a.fv + (size_t)pin * a.fcap;
+    const unsigned long long *fb_in = a.fb + (size_t)pin * a.fcap;
+    unsigned *fv_out = a.fv + (size_t)pout * a.fcap;
+    unsigned long long *fb_out = a.fb + (size_t)pout * a.fcap;
+    const size_t items = (size_t)a.ctl[pin] * a.D;
+    const size_t nthr = (size_t)gridDim.x * 256;
+    bool over = false;
+    // wave-uniform trip count: the ballot below sees all 64 lanes
+    for (size_t base = (size_t)blockIdx.x * 256 + (size_t)(tid & ~63); base < items; base += nthr) {
+        const size_t it = base + (size_t)lane;
+        unsigned long long nb = 0;
+        unsigned ui = 0;
+        if (it < items) {
+            const size_t e = it / (size_t)a.D;
+            const int j = (int)(it - e * (size_t)a.D);
+            const unsigned idx = fv_in[e];
+            const unsigned long long bits = fb_in[e];
+            const unsigned v = idx >> a.wsh;
+            const int w = (int)(idx & (unsigned)(Wp - 1));
+            const unsigned u = nbr_patched(topo, v, j, sw, has_sw);
+            ui = (u << a.wsh) + (unsigned)w;
+            const unsigned long long old = atomicOr(&nxt[ui], bits);
+            nb = bits & ~old;
+            if (nb) {
+                atomicAdd(&s_cnt[w], (unsigned long long)__popcll(nb));
+                atomicOr(&s_lor[w], nb);
+            }
+        }
+        const unsigned long long m = __ballot(nb != 0);
+        if (m) {
+            const int first = __builtin_ctzll(m);
+            int pos0 = 0;
+            if (lane == first) pos0 = atomicAdd(&a.ctl[pout], __popcll(m));
+            pos0 = __shfl(pos0, first, 64);
+            if (nb) {
+                const int pos = pos0 + __popcll(m & ((1ull << lane) - 1ull));
+                if (pos < a.fcap) {
+                    fv_out[pos] = ui;
+                    fb_out[pos] = nb;
+                } else {
+                    over = true;
+                }
+            }
+        }
+    }
+    if (over) a.mode[level + 1] = 0;  // the next level pulls (benign race: all store 0)
+    __syncthreads();
+    if (tid < Wp && tid < a.W && s_cnt[tid]) {
+        atomicAdd(&a.cnt[(size_t)level * Wp + tid], s_cnt[tid]);
+        atomicOr(&a.lor[(size_t)level * Wp + tid], s_lor[tid]);
+    }
 }
 
 // Level `level` of every column: nxt = cur | (OR of the listed neighbours'
@@ -190,9 +293,11 @@ template <class Topo>
 __global__ void __launch_bounds__(256) k_wide_level(Topo topo, WideArgs a, int level,
                                                     const unsigned long long *cur,
                                                     unsigned long long *nxt) {
+    if (a.fcap && a.mode[level]) return;  // a push level (uniform)
     __shared__ unsigned long long s_cnt[256], s_lor[256];
     __shared__ int s_live;
     const int tid = threadIdx.x, lane = tid & 63;
+    if (a.fcap && blockIdx.x == 0 && tid == 0) a.mode[level + 1] = 0;  // pulling persists
     const int Wp = 1 << a.wsh;
     const size_t t0 = (size_t)blockIdx.x * 256 + tid;
     const int w = (int)(t0 & (size_t)(Wp - 1));
@@ -306,7 +411,17 @@ static int run_wide(DenseCtx &c, const DenseArgs &a, const Topo &topo, hipStream
     while ((1 << wsh) < a.W) wsh++;
     const size_t Wp = (size_t)1 << wsh;
     const size_t words = (size_t)a.N * Wp, nctr = (size_t)(a.lmax + 1) * Wp;
-    int rc = ensure_gscratch(c, (2 * words + 2 * nctr) * sizeof(unsigned long long));
+    // push levels 1 and 2: level 1 adds at most M*D entries, level 2 pushes
+    // them (M*D*D device-scope atomics: the workgroups span every XCD) and
+    // overflows into pulling from level 3 on, where the atomics would cost
+    // more than the pull gather (N = 2^18, d = 16: 0.38 ms pulling only,
+    // 0.58 ms pushing while a level adds at most N*Wp/8 entries)
+    const bool push = c.wide_push_ok && !c.push_off && words < (1ull << 31);
+    const size_t fcap = push ? std::max((size_t)a.M,
+                                        std::min(words / 8, (size_t)a.M * (size_t)a.D)) : 0;
+    const size_t ints = 2 * fcap + 2 + (size_t)a.lmax + 2;
+    int rc = ensure_gscratch(c, (2 * words + 2 * nctr + 2 * fcap) * sizeof(unsigned long long) +
+                                    ints * sizeof(int));
     if (rc != GG_OK) return rc;
     unsigned long long *cur = c.gscratch, *nxt = cur + words;
     WideArgs wa;
@@ -315,6 +430,12 @@ static int run_wide(DenseCtx &c, const DenseArgs &a, const Topo &topo, hipStream
     wa.swaps = a.swaps;
     wa.cnt = nxt + words;
     wa.lor = wa.cnt + nctr;
+    wa.fcap = (int)fcap;
+    wa.fb = wa.lor + nctr;
+    wa.fv = reinterpret_cast<unsigned *>(wa.fb + 2 * fcap);
+    wa.ctl = reinterpret_cast<int *>(wa.fv + 2 * fcap);
+    wa.mode = wa.ctl + 2;
+    c.last_push_cap = (int)fcap;
     const size_t want = (words + 255) / 256;
     const int grid = (int)(want < (size_t)8 * c.cu_count ? want : (size_t)8 * c.cu_count);
     hipLaunchKernelGGL(k_wide_init, dim3(grid), dim3(256), 0, st, wa, cur);
@@ -324,6 +445,12 @@ static int run_wide(DenseCtx &c, const DenseArgs &a, const Topo &topo, hipStream
     while (level <= a.lmax) {
         const int end = level + kWideChunk - 1 < a.lmax ? level + kWideChunk - 1 : a.lmax;
         for (; level <= end; level++) {
+            if (fcap) {
+                hipLaunchKernelGGL(k_wide_copy, dim3(grid), dim3(256), 0, st, wa, level,
+                                   (const unsigned long long *)cur, nxt);
+                hipLaunchKernelGGL(k_wide_push<Topo>, dim3(grid), dim3(256), 0, st, topo, wa, level,
+                                   nxt);
+            }
             hipLaunchKernelGGL(k_wide_level<Topo>, dim3(grid), dim3(256), 0, st, topo, wa, level,
                                (const unsigned long long *)cur, nxt);
             unsigned long long *t = cur; cur = nxt; nxt = t;
@@ -425,7 +552,7 @@ int run_dense(DenseCtx &c, const DenseArgs &a_in, hipStream_t st) {
         c.last_variant = 4;
         c.last_vpt = 0;
         c.last_fixed = 0;
-        c.last_push_cap = 0;
+        c.last_push_cap = 0;  // run_wide sets its frontier capacity
         int rc;
         switch (c.topo_kind) {
             case TOPO_RINGQ: rc = run_wide(c, a, c.ringq, st); break;
@@ -871,11 +998,39 @@ static int eval_common(gg_eval *ev, gg_result *out, bool upload = true,
     return GG_OK;
 }
 
+// A part table is undirected when every offset o of row r is matched, with
+// the same multiplicity, by -o in row (r + o) mod M (the row of vertex r + o).
+// Checked only where the level-synchronous path could push (O(M D^2)).
+static bool part_table_undirected(const int32_t *tab, int N, int M, int D) {
+    std::vector<int32_t> t((size_t)M * D);  // offsets reduced to [0, N)
+    for (size_t i = 0; i < t.size(); i++) t[i] = (int32_t)((((long long)tab[i] % N) + N) % N);
+    for (int r = 0; r < M; r++) {
+        const int32_t *row = t.data() + (size_t)r * D;
+        for (int i = 0; i < D; i++) {
+            const int32_t o = row[i];
+            const int ru = (int)(((long long)r + o) % N % M);
+            const int32_t neg = o ? N - o : 0;
+            const int32_t *back = t.data() + (size_t)ru * D;
+            int c1 = 0, c2 = 0;
+            for (int j = 0; j < D; j++) {
+                c1 += row[j] == o;
+                c2 += back[j] == neg;
+            }
+            if (c1 != c2) return false;
+        }
+    }
+    return true;
+}
+
 extern "C" int gg_eval_part(gg_eval *ev, const int32_t *table, gg_result *out) {
     if (!ev || !out || (!table && ev->degree > 0) || ev->kind != EVAL_PART) return GG_BAD_ARG;
     GG_HIP(hipSetDevice(ev->device));
     int st = pack_ring(ev, table, ev->M, false);
     if (st != GG_OK) return st;
+    ev->dc.wide_push_ok = (size_t)ev->N * 8 > (size_t)(ev->dc.lds_limit_opt > 0 ? ev->dc.lds_limit_opt
+                                                                   : ev->dc.lds_device) &&
+                          (long long)ev->M * ev->degree * ev->degree <= (1ll << 24) &&
+                          part_table_undirected(table, ev->N, ev->M, ev->degree);
     return eval_common(ev, out);
 }
 
@@ -887,6 +1042,7 @@ extern "C" int gg_eval_adj(gg_eval *ev, const int32_t *adj, gg_result *out) {
     ev->h_adj.assign(adj, adj + (size_t)ev->N * ev->degree);
     ev->base_sym = adj_symmetric(adj, ev->N, ev->degree);
     ev->dc.push_ok = ev->base_sym;
+    ev->dc.wide_push_ok = ev->dc.push_ok;
     st = eval_common(ev, out);
     if (st == GG_OK) ev->have_base = 1;
     return st;
@@ -912,6 +1068,7 @@ extern "C" int gg_eval_adj_swap(gg_eval *ev, const gg_swap *sw, gg_result *out) 
     GG_HIP(hipMemcpyAsync(ev->d_swap, ev->h_swap, sizeof(gg_swap), hipMemcpyHostToDevice,
                           ev->stream));
     ev->dc.push_ok = ev->base_sym && swap_keeps_symmetry(ev->h_adj.data(), ev->degree, *sw);
+    ev->dc.wide_push_ok = ev->dc.push_ok;
     return eval_common(ev, out, false, ev->d_swap);
 }
 

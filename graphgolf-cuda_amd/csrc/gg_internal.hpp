@@ -25,6 +25,7 @@ struct DenseCtx {
     int wide_ok = 0;          // level-synchronous path allowed (never under graph capture:
                               // the host reads the level counters between launches)
     int wide_off = 0;         // 1 = persistent global kernel only (GG_OPT_WIDE 0)
+    int wide_push_ok = 0;     // level-synchronous path may push (graph verified undirected)
     hipEvent_t bfs_done = nullptr;  // if set, recorded between the BFS and finalize kernels
     int bfs_only = 0;         // 1 = launch the BFS kernel alone (no finalize): timing
     int last_variant = 0;     // 1 = LDS/256, 2 = LDS/1024, 3 = global, 4 = level-synchronous

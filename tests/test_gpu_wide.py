@@ -12,21 +12,27 @@ import oracle_lib as ol
 pytestmark = pytest.mark.gpu
 
 
-def _both(gg, make, evaluate, lds_limit=1024):
-    """(result dict, ecc, reach, variant) with the level-synchronous path and
-    with the persistent global kernel; lds_limit forces the global kernels."""
+def _both(gg, make, evaluate, lds_limit=1024, pushes=True):
+    """(result dict, ecc, reach) of the level-synchronous path with push levels
+    allowed, with pull only, and of the persistent global kernel; all three
+    must agree.  lds_limit forces the global kernels; `pushes`: whether the
+    graph is one the host lets push (verified undirected)."""
     out = []
-    for wide in (1, 0):
+    for wide, push in ((1, 1), (1, 0), (0, 1)):
         with make() as ev:
             ev.set_option(gg.GG_OPT_LDS_LIMIT, lds_limit)
             ev.set_option(gg.GG_OPT_WIDE, wide)
+            ev.set_option(gg.GG_OPT_PUSH, push)
             r = evaluate(ev)
             ecc, reach = ev.levels()
-            out.append((r.as_dict(), ecc, reach, ev.get_option(gg.GG_INFO_VARIANT)))
-    (rw, ew, cw, vw), (rp, ep, cp, vp) = out
-    assert (vw, vp) == (4, 3)
-    assert rw == rp
-    assert np.array_equal(ew, ep) and np.array_equal(cw, cp)
+            out.append((r.as_dict(), ecc, reach, ev.get_option(gg.GG_INFO_VARIANT),
+                        ev.get_option(gg.GG_INFO_PUSH_CAP)))
+    (rw, ew, cw, vw, pw), (rq, eq, cq, vq, pq), (rp, ep, cp, vp, pp) = out
+    assert (vw, vq, vp) == (4, 4, 3)
+    assert (pw > 0) == pushes and pq == 0 and pp == 0
+    assert rw == rq == rp
+    assert np.array_equal(ew, eq) and np.array_equal(ew, ep)
+    assert np.array_equal(cw, cq) and np.array_equal(cw, cp)
     return rw, ew, cw
 
 
@@ -58,14 +64,16 @@ def test_part_piece_box_match_oracle(gpu):
     Nx, Ny, Mx, My, d = 32, 64, 2, 64, 6
     tab = gg.gen_lattice((Nx, Ny), (Mx, My), d, 7)
     o = ol.piece_eval(Nx, Ny, Mx, My, tab)
-    r, ecc, _ = _both(gg, lambda: gg.PieceEvaluator(Nx, Ny, Mx, My, d), lambda ev: ev.eval(tab))
+    r, ecc, _ = _both(gg, lambda: gg.PieceEvaluator(Nx, Ny, Mx, My, d), lambda ev: ev.eval(tab),
+                      pushes=False)
     assert r["total_scaled"] == o["total_scaled"] and r["diameter"] == o["diameter"]
     assert np.array_equal(ecc, o["ecc"])
 
     Ns, Ms, d = (8, 8, 16), (2, 2, 16), 6
     tab = gg.gen_lattice(Ns, Ms, d, 9)
     o = ol.box_eval(*Ns, *Ms, tab)
-    r, ecc, _ = _both(gg, lambda: gg.BoxEvaluator(*Ns, *Ms, d), lambda ev: ev.eval(tab))
+    r, ecc, _ = _both(gg, lambda: gg.BoxEvaluator(*Ns, *Ms, d), lambda ev: ev.eval(tab),
+                      pushes=False)
     assert r["total_scaled"] == o["total_scaled"] and r["diameter"] == o["diameter"]
     assert np.array_equal(ecc, o["ecc"])
 
@@ -103,6 +111,18 @@ def test_swap_unconnected_and_level_limit(gpu):
     assert r["status"] == gg.GG_DIAMETER_LIMIT
 
 
+def test_directed_part_table_pulls(gpu):
+    """Random offsets (a directed table, which no reference driver makes): the
+    level-synchronous path must not push there, and agrees with the
+    persistent kernel (both pull from listed neighbours)."""
+    gg = gpu
+    g, m, d = 32, 96, 5
+    N = g * m
+    rng = np.random.default_rng(11)
+    tab = rng.integers(1, N, size=(m, d)).astype(np.int32)
+    _both(gg, lambda: gg.PartEvaluator(N, m, d), lambda ev: ev.eval(tab), pushes=False)
+
+
 def test_large_orbit_graph_matches_oracle(gpu):
     """N = 2^18 with 128 representatives (two columns): the level-synchronous
     path is the automatic choice and matches the oracle."""
@@ -114,6 +134,7 @@ def test_large_orbit_graph_matches_oracle(gpu):
     with gg.PartEvaluator(N, m, d) as ev:
         r = ev.eval(tab)
         assert ev.get_option(gg.GG_INFO_VARIANT) == 4
+        assert ev.get_option(gg.GG_INFO_PUSH_CAP) > 0
         ecc, _ = ev.levels()
     assert r.status == gg.GG_OK
     assert r.total_scaled == o["total_scaled"] and r.diameter == o["diameter"]
