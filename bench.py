@@ -26,8 +26,31 @@ sys.path[:0] = [os.path.join(REPO, "graphgolf-cuda_amd")]
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# LDS: 256 CUs x 128 B/clk x 2.4 GHz (CDNA3/CDNA4 LDS read width per CU)
-LDS_PEAK_GBS = 256 * 128 * 2.4
+# LDS: 256 CUs x 256 B/clk x 2.4 GHz peak engine clock.  256 B/clk/CU is the
+# smallest power of two above the 16-byte read rate the LDS probe measures on
+# gfx950 (about 230 B/clk/CU); the probe's figure is reported beside it as
+# what is achievable.
+LDS_PEAK_GBS = 256 * 256 * 2.4
+# exact (features, delta) pairs the scorer trains on before the timed window
+TRAIN_PAIRS = 100_000
+
+
+def host_cpu():
+    """Model name and logical CPU count of the host the CPU legs run on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count()
+    return {"cpu_model": model, "nproc": avail, "logical_cpus": os.cpu_count()}
 
 
 def cpu_baseline(adj, seconds_target=12.0):
@@ -140,18 +163,19 @@ def plain_kernel_roofline(gg, adj, reps=30):
     bs = 256 if info["variant"] == 1 else 1024
     name = (f"k_dense_lds<TopoRingQ, {bs}, {info['vpt']}, {info['fixed']}>"
             if info["variant"] in (1, 2) else "k_dense_global<TopoRingQ>")
-    # LDS ceiling: the larger of the width-times-clock figure and the measured
-    # conflict-free 16-byte read rate of the device
+    # LDS ceiling: the fixed width-times-clock figure; the measured
+    # conflict-free read rates of the device are what is achievable
     p16 = gg.probe_lds_read(16)
     p8 = gg.probe_lds_read(8)
-    peak = max(LDS_PEAK_GBS, p16["gbs"])
-    basis = "measured 16-byte LDS read probe" if p16["gbs"] > LDS_PEAK_GBS else \
-        "256 CU x 128 B/clk x 2.4 GHz (above the measured probe)"
+    peak = LDS_PEAK_GBS
     return {"bound": "lds", "achieved": ach, "peak": peak, "unit": "GB/s",
             "frac": ach / peak, "traffic": hbm_traffic(n, d), "kernel": name,
             "kernel_ms": t * 1e3, "kernel_ms_single_bracket": float(np.mean(single[3:])),
             "levels": L, "alg_bytes_per_launch": alg,
-            "push_cap": info["push_cap"], "peak_basis": basis,
+            "push_cap": info["push_cap"],
+            "peak_basis": "256 CU x 256 B/clk x 2.4 GHz",
+            "achievable": {"gbs": p16["gbs"], "frac": ach / p16["gbs"],
+                           "basis": "measured conflict-free 16-byte LDS read probe"},
             "lds_probe": {"b128_gbs": p16["gbs"], "b64_gbs": p8["gbs"],
                           "clock_mhz": p16["clock_mhz"],
                           "b128_bytes_per_clk_cu": p16["bytes_per_clk_cu"],
@@ -196,25 +220,6 @@ def copy_calibration(nbytes=1 << 30, reps=5):
           flush=True)
 
 
-def trained_scorer(gg, adj, cfg, pairs, K, audits=3):
-    """The learned search run until the scorer has trained on `pairs` exact
-    (features, delta) pairs from the run itself, then audited: P fresh
-    proposals all evaluated exactly, Spearman rank correlation of score and
-    exact delta, fraction of the exact top-K the scorer's top-K misses
-    (mean over `audits` proposal draws)."""
-    steps = (pairs + K - 1) // K
-    cfg = gg.SearchCfg.from_buffer_copy(cfg)
-    cfg.total_steps = steps
-    s = gg.Search(adj, cfg)
-    st = s.run(steps)
-    res = [s.audit(1000 + i)[:2] for i in range(audits)]
-    s.close()
-    return {"spearman": float(np.mean([r[0] for r in res])),
-            "top_k_miss_rate": float(np.mean([r[1] for r in res])),
-            "train_pairs": st.train_pairs, "train_steps": st.train_steps,
-            "train_loss": st.train_loss, "search_seconds": st.seconds, "audits": audits}
-
-
 def batched_vs_sequential(gg, n=8192, d=16, K=256, steps=10, seed=4):
     """K candidate 2-opt swaps on a random d-regular graph, one batched launch
     per step (incremental path, HIP graph) against K sequential single-move
@@ -249,13 +254,15 @@ def batched_vs_sequential(gg, n=8192, d=16, K=256, steps=10, seed=4):
 
 
 def run_search(gg, adj, cfg, warmup, steps):
+    """`warmup` untimed steps, then `steps` timed ones (wall time measured by
+    the driver around its own loop).  Returns the search, the stats before
+    and after the timed window, the exact evaluations and the seconds."""
     s = gg.Search(adj, cfg)
     if warmup:
         s.run(warmup)
     st0 = s.run(0)
-    e0, t0 = st0.exact_evals, st0.seconds
     st = s.run(steps)
-    return s, st, st.exact_evals - e0, st.seconds - t0
+    return s, st0, st, st.exact_evals - st0.exact_evals, st.seconds - st0.seconds
 
 
 def large_n_orbit(gg, g=8192, m=128, d=16, seed=3, reps=5):
@@ -289,11 +296,34 @@ def large_n_orbit(gg, g=8192, m=128, d=16, seed=3, reps=5):
     return out
 
 
+def equal_time(gg, adj, cfgs, seconds):
+    """Each search for `seconds` of wall time from the same graph (its
+    temperature schedule sized to the steps it makes in that time), best
+    ASPL and diameter reached."""
+    n = adj.shape[0]
+    res = {}
+    for name, cfg, chunk in cfgs:
+        s = gg.Search(adj, cfg)
+        t_end = time.perf_counter() + seconds
+        st = s.run(0)
+        while time.perf_counter() < t_end:
+            st = s.run(chunk)
+        res[name] = {"aspl_best": st.total_best / (n * (n - 1)), "diam_best": st.diam_best,
+                     "exact_evals": st.exact_evals, "steps": st.steps, "seconds": st.seconds,
+                     "schedule_steps": cfg.total_steps}
+        s.close()
+    res["learned_aspl_le_plain"] = ((res["learned"]["diam_best"], res["learned"]["aspl_best"])
+                                    <= (res["plain"]["diam_best"], res["plain"]["aspl_best"]))
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200, help="timed steps of each search")
-    ap.add_argument("--warmup", type=int, default=100, help="untimed steps of each search")
+    ap.add_argument("--warmup", type=int, default=100,
+                    help="untimed steps of each search (the learned search also runs until "
+                         "its scorer has trained on %d exact pairs)" % TRAIN_PAIRS)
     ap.add_argument("--n", type=int, default=16384)
     ap.add_argument("--d", type=int, default=24)
     ap.add_argument("--K", type=int, default=32)
@@ -306,9 +336,9 @@ def main():
                          "known size (the two legs of the rocprofv3 --pmc traffic runs), or "
                          "only the learned search (for kernel profiles of its step), or only the "
                          "large-N orbit evaluation")
-    ap.add_argument("--equal-time", type=float, default=0.0,
-                    help="also run plain and learned for this many seconds each and report "
-                         "the final ASPL of both")
+    ap.add_argument("--equal-time", type=float, default=10.0,
+                    help="seconds each of plain and learned search from the same graph for "
+                         "the equal-wall-time ASPL comparison (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU: run with --gpus 1"}))
@@ -327,66 +357,83 @@ def main():
         print(json.dumps({"roofline": plain_kernel_roofline(gg, adj)}), flush=True)
         return
     T0, T1 = 1e-3, 1e-6
-    total = args.warmup + args.steps
-
-    learn_cfg = gg.search_cfg(mode=gg.MODE_LEARNED, K=args.K, P=args.P, total_steps=total,
-                              T0=T0, T1=T1, seed=12345, warmup_pairs=2048, train_every=1,
-                              train_batch=256, lr=3e-3)
+    K = args.K
+    # the learned search trains its scorer on TRAIN_PAIRS exact pairs of its own
+    # run before the timed window (the warm-up pairs are 2048 of them), and
+    # keeps training (one optimiser step per search step) inside it
+    learn_warm = max(args.warmup, -(-TRAIN_PAIRS // K))
+    learn_cfg = gg.search_cfg(mode=gg.MODE_LEARNED, K=K, P=args.P,
+                              total_steps=learn_warm + args.steps, T0=T0, T1=T1, seed=12345,
+                              warmup_pairs=2048, train_every=1, train_batch=256, lr=3e-3)
     if args.only == "learned":
-        sl, stl, el, tl = run_search(gg, adj, learn_cfg, args.warmup, args.steps)
+        sl, _, _, el, tl = run_search(gg, adj, learn_cfg, learn_warm, args.steps)
         print(json.dumps({"learned_evals_per_s": el / tl, "ms_per_step": tl / args.steps * 1e3}),
               flush=True)
         return
 
-    plain_cfg = gg.search_cfg(mode=gg.MODE_PLAIN, total_steps=total * args.K, T0=T0, T1=T1,
-                              seed=12345)
-    sp, stp, ep, tp = run_search(gg, adj, plain_cfg, args.warmup, args.steps)
+    plain_cfg = gg.search_cfg(mode=gg.MODE_PLAIN, total_steps=(args.warmup + args.steps) * K,
+                              T0=T0, T1=T1, seed=12345)
+    sp, _, stp, ep, tp = run_search(gg, adj, plain_cfg, args.warmup, args.steps)
     plain_rate = ep / tp
     sp.close()
 
-    sl, stl, el, tl = run_search(gg, adj, learn_cfg, args.warmup, args.steps)
+    sl, stl0, stl, el, tl = run_search(gg, adj, learn_cfg, learn_warm, args.steps)
     learn_rate = el / tl
-    rho, miss, _, _ = sl.audit(777)
+    trained_in_window = stl.train_steps - stl0.train_steps
+    # the scorer of the timed search, audited after the window: all P of a
+    # fresh proposal draw evaluated exactly (mean over three draws)
+    aud = [sl.audit(777 + i)[:2] for i in range(3)]
     sl.close()
-    scorer_long = trained_scorer(gg, adj, learn_cfg, pairs=100_000, K=args.K)
+    rho = float(np.mean([a[0] for a in aud]))
+    miss = float(np.mean([a[1] for a in aud]))
+    uniform_miss = 1.0 - K / args.P
+    valid = trained_in_window > 0 and stl0.train_steps > 0 and miss < uniform_miss
 
     out = {
         "metric": "exact moves evaluated/sec (batched+learned search)",
-        "value": learn_rate, "unit": "exact evaluations/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": tl / args.steps * 1e3,
+        "value": learn_rate if valid else None, "unit": "exact evaluations/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": tl / args.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
         "data": "synthetic: seeded random regular graph, seeded 2-opt proposals",
         "config": {"workload": f"annealing over 2-opt swaps, random {d}-regular graph n={n}",
-                   "n": n, "d": d, "K_exact_per_step": args.K, "P_proposals_per_step": args.P},
+                   "n": n, "d": d, "K_exact_per_step": K, "P_proposals_per_step": args.P},
+        "learned_warmup_steps": learn_warm,
         "plain": {"value": plain_rate, "unit": "exact evaluations/s",
                   "ms_per_step": tp / args.steps * 1e3},
         "speedup_vs_plain": learn_rate / plain_rate,
-        "scorer": {"spearman": rho, "top_k_miss_rate": miss, "train_pairs": stl.train_pairs,
-                   "train_steps": stl.train_steps, "train_loss": stl.train_loss,
-                   "uniform_top_k_miss_rate": 1.0 - args.K / args.P},
-        "scorer_trained_1e5": scorer_long,
+        "scorer": {"spearman": rho, "top_k_miss_rate": miss, "audits": len(aud),
+                   "uniform_top_k_miss_rate": uniform_miss,
+                   "train_pairs": stl.train_pairs, "train_steps": stl.train_steps,
+                   "train_steps_before_window": stl0.train_steps,
+                   "train_steps_in_window": trained_in_window, "train_loss": stl.train_loss},
     }
+    if not valid:
+        out["invalid_reason"] = ("the scorer did not train inside the timed window or does not "
+                                 "beat uniform selection")
+    print(json.dumps({"progress": "searches timed", "value": out["value"]}), file=sys.stderr,
+          flush=True)
     if not args.no_roofline:
         out["roofline"] = plain_kernel_roofline(gg, adj)
     out["batched_vs_sequential"] = batched_vs_sequential(gg)
     out["large_n"] = large_n_orbit(gg)
+    print(json.dumps({"progress": "kernels timed"}), file=sys.stderr, flush=True)
     if args.equal_time > 0:
-        res = {}
-        for name, cfg in (("plain", plain_cfg), ("learned", learn_cfg)):
-            s = gg.Search(adj, cfg)
-            t_end = time.perf_counter() + args.equal_time
-            while time.perf_counter() < t_end:
-                st = s.run(10 if name == "learned" else 200)
-            res[name] = {"aspl_best": st.total_best / (n * (n - 1)), "diam_best": st.diam_best,
-                         "exact_evals": st.exact_evals, "seconds": st.seconds}
-            s.close()
-        out["equal_time"] = res
+        sec = args.equal_time
+        cfgs = (("plain", gg.search_cfg(mode=gg.MODE_PLAIN,
+                                        total_steps=max(1, int(sec * plain_rate)),
+                                        T0=T0, T1=T1, seed=12345), 200),
+                ("learned", gg.search_cfg(mode=gg.MODE_LEARNED, K=K, P=args.P,
+                                          total_steps=max(1, int(sec * learn_rate / K)),
+                                          T0=T0, T1=T1, seed=12345, warmup_pairs=2048,
+                                          train_every=1, train_batch=256, lr=3e-3), 10))
+        out["equal_time"] = equal_time(gg, adj, cfgs, sec)
     if not args.no_cpu:
+        host = host_cpu()
         port = cpu_baseline(adj)
         ref = cpu_baseline_reference(gg, n, d)
-        out["cpu_baseline"] = ref if ref is not None else port
+        out["cpu_baseline"] = dict(ref if ref is not None else port, **host)
         if ref is not None:
-            out["cpu_baseline_port"] = port
+            out["cpu_baseline_port"] = dict(port, **host)
     print(json.dumps(out), flush=True)
 
 

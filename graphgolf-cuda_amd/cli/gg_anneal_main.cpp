@@ -116,7 +116,7 @@ int main(int argc, char **argv) {
     }
     if (fmt != "part" && fmt != "piece" && fmt != "box") return usage(1);
     // the graph as a flat table: part [M][d]; piece / box [M][d][dims] in the
-    // evaluators' layout (cudaASPLpiece.cu:184-195)
+    // evaluators' layout (cudaASPLpiece.cu:163-174)
     graphgolf::part init;
     graphgolf::piece init_piece;
     graphgolf::box init_box;

@@ -2,11 +2,12 @@
 // encodings, with the public fields its drivers and evaluators use.
 // Formats (whitespace separated):
 //   part  : "N M", then M rows "E off_1 .. off_E"              (cppcode/part.hpp:13-43)
-//   piece : "Nx Ny Mx My", then Mx*My rows "E dx dy .. "      (cppcode/piece.hpp:62-96)
-//   box   : "Nx Ny Nz Mx My Mz", then rows "E dx dy dz .."    (cppcode/box.hpp:134-177)
+//   piece : "Nx Ny Mx My", then Mx*My rows "E dx dy .. "      (cppcode/piece.hpp:18-52)
+//   box   : "Nx Ny Nz Mx My Mz", then rows "E dx dy dz .."    (cppcode/box.hpp:37-80)  
 //   graph : one edge "u v" per line, N = max id + 1          (cppcode/graph.hpp:15-51)
 // 'degree' is the largest row length for part and the first row's length for
-// piece and box, as the reference computes it (part.hpp:21-25, piece.hpp:72-75).
+// piece and box, as the reference computes it (part.hpp:21-25, piece.hpp:28-31,
+// box.hpp:51-54).
 #pragma once
 #include <algorithm>
 #include <iostream>

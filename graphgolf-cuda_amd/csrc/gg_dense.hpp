@@ -6,7 +6,7 @@
 // bitset (one uint64 per vertex) lives in LDS, so every level is an LDS
 // neighbour-OR gather and the level barrier is a workgroup barrier -- no host
 // round trip per level (the reference synchronises and copies to the host
-// after every level, cudaASPLconv.cu:232-240).
+// after every level, cudaASPLconv.cu:208-217).
 //
 //   new[v]  = (OR_{u in adj(v)} vis[u]) & ~vis[v]     (pull over visited)
 //   vis[v] |= new[v];  total += level * popc(new[v])
@@ -77,7 +77,7 @@ struct TopoRingQ {
     }
 };
 
-// 2-D torus 'piece' (cppcode/piece.hpp, cudacode/cudaASPLpiece.cu:56-75):
+// 2-D torus 'piece' (cppcode/piece.hpp, cudacode/cudaASPLpiece.cu:35-54):
 // v = x*Ny + y; offsets (dx,dy) normalised, packed dx | dy << 16.
 struct TopoPiece {
     const uint32_t *tab;

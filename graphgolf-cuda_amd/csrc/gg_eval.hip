@@ -411,11 +411,15 @@ static int run_wide(DenseCtx &c, const DenseArgs &a, const Topo &topo, hipStream
     while ((1 << wsh) < a.W) wsh++;
     const size_t Wp = (size_t)1 << wsh;
     const size_t words = (size_t)a.N * Wp, nctr = (size_t)(a.lmax + 1) * Wp;
-    // push levels 1 and 2: level 1 adds at most M*D entries, level 2 pushes
-    // them (M*D*D device-scope atomics: the workgroups span every XCD) and
-    // overflows into pulling from level 3 on, where the atomics would cost
-    // more than the pull gather (N = 2^18, d = 16: 0.38 ms pulling only,
-    // 0.58 ms pushing while a level adds at most N*Wp/8 entries)
+    // push levels: a level pushes while the level before it fit its list
+    // (fcap entries); the first level whose list overflows makes every later
+    // level pull.  fcap = M*D (at most N*Wp/8, at least M): level 1 adds at
+    // most M*D entries, so on expander graphs level 2 (M*D*D device-scope
+    // atomics: the workgroups span every XCD) is the one that overflows and
+    // pulling starts at level 3, where the atomics would cost more than the
+    // pull gather (N = 2^18, d = 16: 0.38 ms pulling only, 0.58 ms pushing
+    // with lists of N*Wp/8 entries); graphs with slowly growing frontiers
+    // (long cycles with few chords) keep pushing
     const bool push = c.wide_push_ok && !c.push_off && words < (1ull << 31);
     const size_t fcap = push ? std::max((size_t)a.M,
                                         std::min(words / 8, (size_t)a.M * (size_t)a.D)) : 0;
@@ -877,7 +881,7 @@ static inline long long norm_mod(long long x, long long n) {
     return r < 0 ? r + n : r;
 }
 
-// The reference adds N before % (cudaASPLconv.cu:66); for every offset where
+// The reference adds N before % (cudaASPLconv.cu:44); for every offset where
 // that is well defined the value equals the mathematical residue used here.
 static int pack_ring(gg_eval *ev, const int32_t *table, int R, bool adj) {
     const int N = ev->N, D = ev->degree;
@@ -889,7 +893,7 @@ static int pack_ring(gg_eval *ev, const int32_t *table, int R, bool adj) {
                 long long x = table[(size_t)r * D + i];
                 long long off;
                 if (adj) {
-                    if (x < 0) off = 0;  // padding = self-loop (cudaASPLgraph.cu:167-170)
+                    if (x < 0) off = 0;  // padding = self-loop (cudaASPLgraph.cu:148-151)
                     else if (x >= N) return GG_BAD_ARG;
                     else off = norm_mod(x - r, N);
                 } else {
@@ -1000,8 +1004,12 @@ static int eval_common(gg_eval *ev, gg_result *out, bool upload = true,
 
 // A part table is undirected when every offset o of row r is matched, with
 // the same multiplicity, by -o in row (r + o) mod M (the row of vertex r + o).
-// Checked only where the level-synchronous path could push (O(M D^2)).
+// Checked only where the level-synchronous path could push (O(M D^2)).  The
+// row of vertex v is v mod M, so checking rows 0..M-1 covers every vertex
+// only when M divides N: past the last full copy the rows wrap differently,
+// and such tables are treated as directed (they always pull).
 static bool part_table_undirected(const int32_t *tab, int N, int M, int D) {
+    if (M < 1 || N % M != 0) return false;
     std::vector<int32_t> t((size_t)M * D);  // offsets reduced to [0, N)
     for (size_t i = 0; i < t.size(); i++) t[i] = (int32_t)((((long long)tab[i] % N) + N) % N);
     for (int r = 0; r < M; r++) {

@@ -673,8 +673,14 @@ extern "C" int gg_search_audit(gg_search *s, uint64_t seed, double *spearman, do
     std::vector<int> a(P), b(P);
     std::iota(a.begin(), a.end(), 0);
     std::iota(b.begin(), b.end(), 0);
-    std::partial_sort(a.begin(), a.begin() + K, a.end(), [&](int x, int y) { return sc[x] < sc[y]; });
-    std::partial_sort(b.begin(), b.begin() + K, b.end(), [&](int x, int y) { return dl[x] < dl[y]; });
+    // ties broken by proposal index, so the sets are a function of (sc, dl)
+    // alone (a stable argsort of either array gives the same K)
+    std::partial_sort(a.begin(), a.begin() + K, a.end(), [&](int x, int y) {
+        return sc[x] < sc[y] || (sc[x] == sc[y] && x < y);
+    });
+    std::partial_sort(b.begin(), b.begin() + K, b.end(), [&](int x, int y) {
+        return dl[x] < dl[y] || (dl[x] == dl[y] && x < y);
+    });
     std::vector<char> inA(P, 0);
     for (int k = 0; k < K; k++) inA[a[k]] = 1;
     int hit = 0;
@@ -683,5 +689,25 @@ extern "C" int gg_search_audit(gg_search *s, uint64_t seed, double *spearman, do
     L->miss = 1.0 - (double)hit / K;
     if (spearman) *spearman = rho;
     if (miss_rate) *miss_rate = L->miss;
+    return GG_OK;
+}
+
+extern "C" int gg_search_last_selection(gg_search *s, int64_t *step, int32_t *warm,
+                                        float *scores_out, int32_t *moves_out, int32_t *sel_out) {
+    if (!s || !s->learn) return GG_BAD_ARG;
+    LearnState *L = s->learn;
+    if (!L->dev) return GG_UNSUPPORTED;
+    if (s->step < 1) return GG_BAD_ARG;  // no step has run
+    const int P = L->P, K = L->K;
+    // the device buffers hold the last captured step's draws, scores and
+    // choice until the next step: nothing after the step rewrites them
+    GG_HIP(hipStreamSynchronize(L->st));
+    if (scores_out)
+        GG_HIP(hipMemcpy(scores_out, L->d_score, 4 * (size_t)P, hipMemcpyDeviceToHost));
+    if (moves_out)
+        GG_HIP(hipMemcpy(moves_out, L->d_moves, 4 * (size_t)P * 5, hipMemcpyDeviceToHost));
+    if (sel_out) GG_HIP(hipMemcpy(sel_out, L->d_sel, 4 * (size_t)K, hipMemcpyDeviceToHost));
+    if (step) *step = L->h_ctl[0];
+    if (warm) *warm = L->h_ctl[1];
     return GG_OK;
 }

@@ -135,7 +135,7 @@ namespace {
 // ---- lattice (piece / box) tables -------------------------------------------------------
 // t[(row*d + slot)*D + k]: offset along axis k of slot `slot` of base row
 // `row`; rows are mixed-radix over (M_0, .., M_{D-1}), axis 0 major
-// (x*My + y for piece, (x*My + y)*Mz + z for box; cudaASPLpiece.cu:184-195).
+// (x*My + y for piece, (x*My + y)*Mz + z for box; cudaASPLpiece.cu:163-174).
 struct Lattice {
     int D, d;
     int N[3], M[3], g[3];

@@ -191,6 +191,7 @@ def lib() -> C.CDLL:
         "gg_search_get_log": (C.c_int, [vp, i64p, C.c_int, P(C.c_int)]),
         "gg_search_audit": (C.c_int, [vp, C.c_uint64, P(C.c_double), P(C.c_double),
                                       P(C.c_float), P(C.c_double)]),
+        "gg_search_last_selection": (C.c_int, [vp, i64p, i32p, P(C.c_float), i32p, i32p]),
         "gg_orbit_search_create": (C.c_int, [C.c_int, C.c_int, C.c_int, i32p, P(SearchCfg),
                                              C.c_int, P(vp)]),
         "gg_orbit_search_create_cb": (C.c_int, [C.c_int, C.c_int, C.c_int, i32p, P(SearchCfg),
@@ -542,7 +543,7 @@ class BoxEvaluator(_Evaluator):
 
 
 def aspl(result: Result, N: int) -> float:
-    """ASPL as the reference derives it (cudacode/cudaASPLconv.cu:265)."""
+    """ASPL as the reference derives it (cudacode/cudaASPLconv.cu:242)."""
     return float(result.total_scaled) / (N * (N - 1))
 
 
@@ -564,7 +565,7 @@ class Search:
     def __init__(self, adj, cfg: SearchCfg, device: int = 0, callbacks=None, log: int = 0):
         a = _i32(adj)
         self.N, self.d = a.shape
-        self.P = cfg.P
+        self.P, self.K = cfg.P, cfg.K
         self._h = C.c_void_p()
         self._cb = callbacks  # keep the ctypes callbacks alive
         if callbacks is None:
@@ -604,6 +605,19 @@ class Search:
         check(lib().gg_search_audit(self._h, seed, C.byref(rho), C.byref(miss),
                                     _ptr(sc, C.c_float), _ptr(dl, C.c_double)), "gg_search_audit")
         return rho.value, miss.value, sc, dl
+
+    def last_selection(self):
+        """Learned mode (proposals drawn on the GPU): the last step's
+        (step index, warm flag, scores[P], move rows[P][5], chosen indices[K])."""
+        P, K = self.P, self.K
+        step, warm = C.c_int64(), C.c_int32()
+        sc = np.empty(P, dtype=np.float32)
+        mv = np.empty((P, 5), dtype=np.int32)
+        sel = np.empty(K, dtype=np.int32)
+        check(lib().gg_search_last_selection(self._h, C.byref(step), C.byref(warm),
+                                             _ptr(sc, C.c_float), _ptr(mv), _ptr(sel)),
+              "gg_search_last_selection")
+        return step.value, warm.value, sc, mv, sel
 
     def close(self):
         if self._h:

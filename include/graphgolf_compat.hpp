@@ -28,7 +28,8 @@ namespace detail {
 inline void check(int st, const char *what) {
     if (st < 0) throw std::runtime_error(std::string(what) + ": " + gg_strerror(st));
 }
-// Reference sentinels: diameterASPL (cudacode/cudaASPLconv.cu:230,252,257).
+// Reference sentinels: diameterASPL (cudacode/cudaASPLconv.cu:229,234),
+// WVCdiameterASPL (:293,298).
 constexpr int kBadDiameter = 100000000;
 constexpr double kBadASPL = 100000000.0;
 }  // namespace detail
@@ -40,7 +41,7 @@ class cudaASPLconv {
 
     template <class Part>
     gg_result run(Part &p, int level_limit) {
-        // packing of cudaASPLconv.cu:222-225: rows padded with offset 0
+        // packing of cudaASPLconv.cu:198-202: rows padded with offset 0
         for (int i = 0; i < M; i++)
             for (int j = 0; j < degree; j++)
                 tab_[(size_t)i * degree + j] = j < (int)p.edges[i].size() ? p.edges[i][j] : 0;
@@ -60,21 +61,21 @@ public:
     cudaASPLconv(const cudaASPLconv &) = delete;
     cudaASPLconv &operator=(const cudaASPLconv &) = delete;
 
-    // cudaASPLconv.cu:174-219: ASPL, or 1e9 when unconnected / too deep
+    // cudaASPLconv.cu:151-196: ASPL, or 1e9 when unconnected (:182) / too deep
     template <class Part>
     double calc(Part &p) {
         gg_result r = run(p, GG_LEVEL_LIMIT);
         if (r.status != GG_OK) return 1e9;
         return gg_aspl(&r, N);
     }
-    // cudaASPLconv.cu:220-266
+    // cudaASPLconv.cu:197-243
     template <class Part>
     std::pair<int, double> diameterASPL(Part &p) {
         gg_result r = run(p, GG_LEVEL_LIMIT);
         if (r.status != GG_OK) return {detail::kBadDiameter, detail::kBadASPL};
         return {r.diameter, gg_aspl(&r, N)};
     }
-    // cudaASPLconv.cu:267-331 (levels up to 100)
+    // cudaASPLconv.cu:244-308 (levels up to 100, :264)
     template <class Part>
     std::tuple<int, int, double> WVCdiameterASPL(Part &p) {
         gg_result r = run(p, 100);
@@ -100,7 +101,7 @@ public:
     cudaASPLpiece(const cudaASPLpiece &) = delete;
     cudaASPLpiece &operator=(const cudaASPLpiece &) = delete;
 
-    // cudaASPLpiece.cu:180-237; packing :184-195 (rows truncated / padded to degree)
+    // cudaASPLpiece.cu:159-216; packing :163-174 (rows truncated / padded to degree)
     template <class Piece>
     std::pair<int, double> diameterASPL(Piece &p) {
         for (int x = 0; x < Mx; x++)
@@ -159,7 +160,7 @@ public:
 
 // cudacode/cudaASPLgraph.hpp:5-19: explicit adjacency, all N sources,
 // returns (diameter, unscaled total); (-1,-1) unconnected, (-2,-2) too deep
-// (cudaASPLgraph.cu:199-209).
+// (cudaASPLgraph.cu:182,190); rows padded with self-loops (:144-153).
 class cudaASPLgraph {
     gg_eval *ev_ = nullptr;
     std::vector<int32_t> adj_;

@@ -20,12 +20,13 @@ extern "C" {
 
 /* ---- status codes --------------------------------------------------------
  * The reference checks no device status and reports graph conditions through
- * sentinels (cudacode/cudaASPLconv.cu:228-229, :252, :257; cppcode/
+ * sentinels (cudacode/cudaASPLconv.cu:182,187 calc; :229,234 diameterASPL;
+ * :293,298 WVCdiameterASPL; cudaASPLgraph.cu:182,190; cppcode/
  * cpuASPLqueue.cpp:92).  Here graph conditions are positive codes carried in
  * gg_result.status; errors are negative return values. */
 #define GG_OK               0
 #define GG_UNCONNECTED      1   /* some source does not reach every vertex          */
-#define GG_DIAMETER_LIMIT   2   /* level 99 still found new vertices (cudaASPLconv.cu:231) */
+#define GG_DIAMETER_LIMIT   2   /* level 99 still found new vertices (cudaASPLconv.cu:208) */
 #define GG_BAD_ARG        (-1)
 #define GG_HIP_ERROR      (-2)
 #define GG_OOM            (-3)
@@ -33,9 +34,9 @@ extern "C" {
 #define GG_UNSUPPORTED    (-5)
 
 /* Highest BFS level evaluated by default: the reference loops step = 1..99
- * (cudacode/cudaASPLconv.cu:231) and reports a sentinel when step 99 still
- * finds new vertices.  WVCdiameterASPL loops to 100 (cudaASPLconv.cu:287) and
- * cudaASPLgraph::calc to 199 (cudaASPLgraph.cu:182); GG_OPT_LEVEL_LIMIT sets
+ * (cudacode/cudaASPLconv.cu:208) and reports a sentinel when step 99 still
+ * finds new vertices.  WVCdiameterASPL loops to 100 (cudaASPLconv.cu:264,296)
+ * and cudaASPLgraph::calc to 199 (cudaASPLgraph.cu:163,187); GG_OPT_LEVEL_LIMIT sets
  * the limit per evaluator, up to GG_LEVEL_LIMIT_MAX. */
 #define GG_LEVEL_LIMIT 99
 #define GG_LEVEL_LIMIT_MAX 199
@@ -46,9 +47,9 @@ typedef struct gg_batch gg_batch;
 /* Result of one exact evaluation. */
 typedef struct gg_result {
     int64_t total_reps;   /* sum over the M sources s of sum_v d(s,v) (pairs reached) */
-    int64_t total_scaled; /* total_reps * (N / M), integer N/M as cudaASPLconv.cu:264 */
+    int64_t total_scaled; /* total_reps * (N / M), integer N/M as cudaASPLconv.cu:240 */
     int32_t diameter;     /* max eccentricity over the sources (reached part)         */
-    int32_t wvc;          /* WVC as cudaASPLconv.cu:258-308 computes it (part only)   */
+    int32_t wvc;          /* WVC as cudaASPLconv.cu:256-285 computes it (part only)   */
     int32_t levels;       /* BFS levels executed (diameter + 1 when connected)        */
     int32_t status;       /* GG_OK, GG_UNCONNECTED or GG_DIAMETER_LIMIT                */
 } gg_result;
@@ -58,7 +59,7 @@ typedef struct gg_result {
  *   {a, idx(a->b), d}, {b, idx(b->a), c}, {c, idx(c->d), b}, {d, idx(d->c), a}.
  * Rows may repeat (entries are applied in order j = 0..3).  Unused entries have
  * v = -1.  No reference counterpart: the reference re-uploads the whole table
- * per move (cudacode/cudaASPLconv.cu:222-225). */
+ * per move (cudacode/cudaASPLconv.cu:198-202). */
 typedef struct gg_swap {
     int32_t v[4];
     int32_t slot[4];
@@ -69,40 +70,41 @@ typedef struct gg_swap {
 const char *gg_strerror(int status);
 int gg_device_count(int *count);
 /* ASPL exactly as the reference derives it: double(total)/(int64(N)*(N-1))
- * (cudacode/cudaASPLconv.cu:265; cppcode/cpuASPLqueue.cpp:94). */
+ * (cudacode/cudaASPLconv.cu:242; cppcode/cpuASPLqueue.cpp:94). */
 double gg_aspl(const gg_result *r, int N);
 
 /* ---- single-graph evaluators ---------------------------------------------
  * Replace the constructors cudaASPLconv(N,M,degree,device)
- * (cudacode/cudaASPLconv.hpp:18, .cu:145-160), cudaASPLpiece(Nx,Ny,Mx,My,
- * degree,device) (cudaASPLpiece.hpp:18, .cu:149-166), cudaASPLbox(Nx,Ny,Nz,Mx,
- * My,Mz,degree,device) (cudaASPLbox.hpp:18) and cudaASPLgraph(N,degree_max,
- * device) (cudaASPLgraph.hpp:16, .cu:133-146).  N need not be a multiple of
- * M: the scaled total uses integer N/M as the reference does
- * (cudaASPLconv.cu:264). */
+ * (cudacode/cudaASPLconv.hpp:18, .cu:122-137), cudaASPLpiece(Nx,Ny,Mx,My,
+ * degree,device) (cudaASPLpiece.hpp:18, .cu:128-145), cudaASPLbox(Nx,Ny,Nz,Mx,
+ * My,Mz,degree,device) (cudaASPLbox.hpp:18, .cu:135-152) and
+ * cudaASPLgraph(N,degree_max,device) (cudaASPLgraph.hpp:16, .cu:114-127).
+ * N need not be a multiple of M: the scaled total uses integer N/M as the
+ * reference does (cudaASPLconv.cu:240). */
 int gg_create_part(int N, int M, int degree, int device, gg_eval **out);
 int gg_create_piece(int Nx, int Ny, int Mx, int My, int degree, int device, gg_eval **out);
 int gg_create_box(int Nx, int Ny, int Nz, int Mx, int My, int Mz, int degree,
                   int device, gg_eval **out);
 int gg_create_adj(int N, int degree_max, int device, gg_eval **out);
-/* Replaces the destructors (cudacode/cudaASPLconv.cu:161-173 and siblings). */
+/* Replaces the destructors (cudacode/cudaASPLconv.cu:138-150, cudaASPLpiece.cu:146-158,
+ * cudaASPLbox.cu:153-165, cudaASPLgraph.cu:128-140). */
 void gg_destroy(gg_eval *ev);
 
-/* Replaces cudaASPLconv::diameterASPL / calc / WVCdiameterASPL
- * (cudacode/cudaASPLconv.cu:174-331).  table is the reference's packed h_edges:
- * M rows of `degree` int32 offsets, short rows padded with 0
- * (cudaASPLconv.cu:222-224).  Vertex v's neighbours are (v + off + N) % N for
+/* Replaces cudaASPLconv::calc / diameterASPL / WVCdiameterASPL
+ * (cudacode/cudaASPLconv.cu:151-196, 197-243, 244-308).  table is the
+ * reference's packed h_edges: M rows of `degree` int32 offsets, short rows
+ * padded with 0 (cudaASPLconv.cu:198-202).  Vertex v's neighbours are (v + off + N) % N for
  * the offsets of row v % M. */
 int gg_eval_part(gg_eval *ev, const int32_t *table, gg_result *out);
-/* Replaces cudaASPLpiece::diameterASPL (cudacode/cudaASPLpiece.cu:180-237);
- * table = h_edges[((x*My+y)*degree+e)*2 + {0,1}] (cudaASPLpiece.cu:184-195). */
+/* Replaces cudaASPLpiece::diameterASPL (cudacode/cudaASPLpiece.cu:159-216);
+ * table = h_edges[((x*My+y)*degree+e)*2 + {0,1}] (cudaASPLpiece.cu:163-174). */
 int gg_eval_piece(gg_eval *ev, const int32_t *table, gg_result *out);
 /* Replaces cudaASPLbox::diameterASPL (cudacode/cudaASPLbox.cu:166-226);
- * table = h_edges[((x*My*Mz+y*Mz+z)*degree+e)*3 + {0,1,2}]. */
+ * table = h_edges[((x*My*Mz+y*Mz+z)*degree+e)*3 + {0,1,2}] (cudaASPLbox.cu:170-183). */
 int gg_eval_box(gg_eval *ev, const int32_t *table, gg_result *out);
-/* Replaces cudaASPLgraph::calc (cudacode/cudaASPLgraph.cu:160-218): explicit
+/* Replaces cudaASPLgraph::calc (cudacode/cudaASPLgraph.cu:141-199): explicit
  * adjacency N x degree_max, entries < 0 are padding (the reference pads with
- * self-loops, cudaASPLgraph.cu:163-172).  All N vertices are sources. */
+ * self-loops, cudaASPLgraph.cu:144-153).  All N vertices are sources. */
 int gg_eval_adj(gg_eval *ev, const int32_t *adj, gg_result *out);
 
 /* Plain single-move path on a resident adjacency (the table of the last
@@ -116,7 +118,7 @@ int gg_eval_adj_commit(gg_eval *ev, const gg_swap *sw);
 /* Per-source eccentricities (M entries, source order: part/adj = vertex id;
  * piece = x*My+y; box = (x*My+y)*Mz+z) and per-level reach counts (pairs at
  * distance exactly l, l < reach_cap) of the most recent evaluation.  The BFS
- * pulls from each vertex's listed neighbours (as cudaASPLconv.cu:65-67 does),
+ * pulls from each vertex's listed neighbours (as cudaASPLconv.cu:35-48 does),
  * so on a table that describes a directed graph these are in-eccentricities;
  * on undirected graphs (all the reference's drivers produce) in = out.  Either
  * pointer may be NULL.  No reference counterpart (the reference never exposes

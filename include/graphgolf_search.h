@@ -88,6 +88,19 @@ int gg_search_get_log(gg_search *s, int64_t *buf, int capacity, int *count);
 int gg_search_audit(gg_search *s, uint64_t seed, double *spearman, double *miss_rate,
                     float *scores_out, double *delta_out);
 
+/* Learned mode with proposals drawn on the GPU (P <= 4096) only: what the
+ * last step drew, scored and chose, for checking the selection rule from
+ * outside.  step = the step index the draws used (stream (seed, step, p) of
+ * proposal p, stream (seed, step, 2^32 - 1) of the exploration draws),
+ * warm = 1 when all K were taken uniformly (proposals 0..K-1, before
+ * warmup_pairs exact pairs existed); scores_out P floats, moves_out P rows
+ * (a, b, c, d, kind), sel_out the K chosen proposal indices in evaluation
+ * order.  Any output pointer may be NULL.  Valid until the next step or
+ * audit (an audit scores its own proposals in the same buffers).  Counterpart of the reference's
+ * one draw per move (cppcode/annealing.cpp:399-403), which has no selection. */
+int gg_search_last_selection(gg_search *s, int64_t *step, int32_t *warm, float *scores_out,
+                             int32_t *moves_out, int32_t *sel_out);
+
 /* ---- search over cyclically symmetric graphs ------------------------------
  * The reference's `part` form (cppcode/part.hpp:6-44): M representative rows
  * of `degree` offsets, N = g*M, neighbour i of vertex v is

@@ -175,7 +175,7 @@ int oracle_bfs(int N, int dmax, const int32_t *adj, int nsrc, const int32_t *src
     return unconnected;
 }
 
-/* WVC as the reference GPU computes it (cudacode/cudaASPLconv.cu:258-308):
+/* WVC as the reference GPU computes it (cudacode/cudaASPLconv.cu:256-285):
  * classes j in [0,32) merge sources j, j+32, ... (< M); count the classes whose
  * largest eccentricity equals the diameter. ecc[s] is indexed by source id. */
 int oracle_wvc(int M, const int32_t *ecc, int diameter)

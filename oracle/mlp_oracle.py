@@ -51,3 +51,18 @@ def loss_and_grad(w, X, Y):
     gW1 = gh.T @ X
     gb1 = gh.sum(0)
     return loss, np.concatenate([gW1.ravel(), gb1, gW2, [gb2]])
+
+
+BETA1, BETA2, EPS = 0.9, 0.999, 1e-8
+
+
+def adam_step(w, m, v, g, t, lr):
+    """One Adam update (Kingma and Ba 2015, Algorithm 1) in float64: moments
+    m, v from gradient g, bias correction by 1 - beta^t, then
+    w - lr * m_hat / (sqrt(v_hat) + eps).  Returns (w, m, v)."""
+    g = np.asarray(g, dtype=np.float64)
+    m = BETA1 * np.asarray(m, dtype=np.float64) + (1 - BETA1) * g
+    v = BETA2 * np.asarray(v, dtype=np.float64) + (1 - BETA2) * g * g
+    mh = m / (1 - BETA1 ** t)
+    vh = v / (1 - BETA2 ** t)
+    return np.asarray(w, dtype=np.float64) - lr * mh / (np.sqrt(vh) + EPS), m, v

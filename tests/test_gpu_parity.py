@@ -163,7 +163,7 @@ def test_symmetric_part_parity(gpu, g, m, d, seed):
 def test_part_with_irregular_offsets(gpu):
     """Random offset tables, offsets beyond +-N included.  Such tables describe
     directed graphs: the reference GPU kernel pulls from out-neighbours
-    (cudacode/cudaASPLconv.cu:65-67), so its per-source eccentricities are
+    (cudacode/cudaASPLconv.cu:35-48), so its per-source eccentricities are
     in-eccentricities, while the CPU queue BFS (cpuASPLqueue.cpp:80-84) yields
     out-eccentricities.  The totals and the diameter agree by the cyclic
     symmetry of 'part' graphs; WVC (a per-source quantity) is only compared on
@@ -266,7 +266,7 @@ def test_piece_and_box_parity(gpu):
 
 def test_piece_directed_pull_semantics(gpu):
     """A directed piece table with Mx not dividing Nx: the kernel pulls from
-    listed neighbours (cudacode/cudaASPLpiece.cu:64-71), i.e. it runs the BFS
+    listed neighbours (cudacode/cudaASPLpiece.cu:35-54), i.e. it runs the BFS
     of the reversed graph from the representatives."""
     gg = gpu
     rng = np.random.default_rng(11)

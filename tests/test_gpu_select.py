@@ -1,0 +1,88 @@
+"""GPU checks of the learned search's candidate selection and audit against
+their restatements (select_ref.py, devrng_ref.py), and of the learned search
+at the n = 16384, d = 24 shape (K = 32 of P = 1024) past its warm-up, its
+final graph checked on the oracle."""
+import numpy as np
+import pytest
+
+import devrng_ref
+import oracle_eval
+import select_ref as sr
+
+pytestmark = pytest.mark.gpu
+
+
+def _check_step(s, seed, adj_before, K, P, check_moves=True):
+    step, warm, sc, mv, sel = s.last_selection()
+    if check_moves:
+        ref = [devrng_ref.propose(adj_before, seed, step, p) for p in range(P)]
+        assert [tuple(int(x) for x in r) for r in mv] == ref, step
+    want = sr.select(sc, K, seed, step, warm)
+    assert sel.tolist() == want, (step, warm)
+    return step, warm, sc
+
+
+def test_selection_matches_restated_rule(gpu):
+    gg = gpu
+    n, d, K, P, seed = 2048, 8, 16, 256, 9
+    adj = gg.random_regular(n, d, 6)
+    cfg = gg.search_cfg(mode=gg.MODE_LEARNED, K=K, P=P, total_steps=200, T0=1e-4, T1=1e-6,
+                        seed=seed, warmup_pairs=512, train_batch=256, lr=2e-3)
+    s = gg.Search(adj, cfg)
+    warm_seen, greedy_seen = 0, 0
+    for t in range(48):  # 32 warm-up steps (512 pairs / 16), then selection by score
+        before = s.graph()
+        s.run(1)
+        step, warm, sc = _check_step(s, seed, before, K, P, check_moves=(t % 4 == 0))
+        assert step == t
+        warm_seen += warm
+        greedy_seen += 1 - warm
+        assert np.isfinite(sc).all()
+    assert warm_seen == 32 and greedy_seen == 16
+    s.close()
+
+
+def test_audit_figures_recomputed(gpu):
+    gg = gpu
+    n, d, K, P = 2048, 8, 16, 256
+    adj = gg.random_regular(n, d, 6)
+    cfg = gg.search_cfg(mode=gg.MODE_LEARNED, K=K, P=P, total_steps=400, T0=1e-4, T1=1e-6,
+                        seed=9, warmup_pairs=1024, train_batch=256, lr=2e-3)
+    s = gg.Search(adj, cfg)
+    s.run(150)
+    for seed in (1, 2):
+        rho, miss, sc, dl = s.audit(seed)
+        assert abs(rho - sr.spearman(sc, dl)) < 1e-9, (rho, sr.spearman(sc, dl))
+        assert miss == sr.miss_rate(sc, dl, K)
+    s.close()
+
+
+def test_learned_search_at_n16384_d24(gpu):
+    """Learned search at its benchmark shape (K = 32 of P = 1024, 2048 warm-up
+    pairs = 64 steps) for 300 steps: training runs in every step after the
+    warm-up, one step's selection follows the restated rule, and the driver's
+    current and best totals and diameters are the oracle's on its graphs."""
+    gg = gpu
+    n, d, K, P, seed = 16384, 24, 32, 1024, 12345
+    adj = gg.random_regular(n, d, 5)
+    cfg = gg.search_cfg(mode=gg.MODE_LEARNED, K=K, P=P, total_steps=300, T0=1e-3, T1=1e-6,
+                        seed=seed, warmup_pairs=2048, train_every=1, train_batch=256, lr=3e-3)
+    s = gg.Search(adj, cfg)
+    st1 = s.run(99)
+    before = s.graph()
+    s.run(1)
+    step, warm, _ = _check_step(s, seed, before, K, P)
+    assert step == 99 and warm == 0
+    st = s.run(200)
+    assert st.exact_evals == 300 * K and st.proposals == 300 * P
+    assert st1.train_steps > 0 and st.train_steps - st1.train_steps == 201
+    cur = oracle_eval.evaluate(s.graph())
+    assert (cur[0], cur[1], cur[2]) == (st.total_cur, st.diam_cur, 0)
+    best = oracle_eval.evaluate(s.graph(best=True))
+    assert (best[0], best[1]) == (st.total_best, st.diam_best)
+    assert st.accepted > 0
+    rho, miss, sc, dl = s.audit(777)
+    assert abs(rho - sr.spearman(sc, dl)) < 1e-9 and miss == sr.miss_rate(sc, dl, K)
+    print(f"n=16384 d=24 learned, 300 steps: accepted {st.accepted}, total {st.total_cur}, "
+          f"train steps {st.train_steps}, audit spearman {rho:.3f} miss {miss:.3f}")
+    s.close()

@@ -140,3 +140,32 @@ def test_large_orbit_graph_matches_oracle(gpu):
     assert r.total_scaled == o["total_scaled"] and r.diameter == o["diameter"]
     assert r.wvc == o["wvc"]
     assert np.array_equal(ecc, o["ecc"])
+
+
+def test_push_continues_while_lists_fit(gpu):
+    """A ring of 256 vertices with the diameter chord, as a part table of 16
+    rows (g = 16): from 16 consecutive sources every level adds about four
+    frontier words, so no list of the push levels overflows and every level
+    up to the diameter pushes (level 3 onwards reads the list level 2 wrote).
+    Push, pull and the persistent kernel agree with the oracle."""
+    gg = gpu
+    N, M = 256, 16
+    tab = np.tile(np.array([1, N - 1, N // 2], dtype=np.int32), (M, 1))
+    o = ol.part_eval(N, M, tab)
+    r, ecc, reach = _both(gg, lambda: gg.PartEvaluator(N, M, 3), lambda ev: ev.eval(tab))
+    assert o["diameter"] > 8
+    assert r["total_scaled"] == o["total_scaled"] and r["diameter"] == o["diameter"]
+    assert np.array_equal(ecc, o["ecc"])
+    assert np.array_equal(reach[:len(o["reach"])], o["reach"][:len(reach)])
+
+
+def test_part_table_with_partial_copy_pulls(gpu):
+    """M does not divide N: the rows of the vertices past the last full copy
+    wrap differently, so a table whose rows 0..M-1 pass the undirected test
+    is still treated as directed (push capacity 0); push-allowed, pull-only
+    and persistent runs agree."""
+    gg = gpu
+    g, m, d = 32, 96, 6
+    N = g * m + 24
+    tab = gg.orbit_part(g, m, d, 5)
+    _both(gg, lambda: gg.PartEvaluator(N, m, d), lambda ev: ev.eval(tab), pushes=False)

@@ -35,3 +35,21 @@ def test_gradient_matches_finite_differences():
         wm = w.copy(); wm[i] -= h
         fd = (mo.loss_and_grad(wp, X, Y)[0] - mo.loss_and_grad(wm, X, Y)[0]) / (2 * h)
         assert abs(fd - g[i]) <= 1e-6 + 1e-5 * abs(g[i]), (i, fd, g[i])
+
+
+def test_adam_step_closed_forms():
+    """Constant gradient: the bias-corrected moments are g and g^2 at every
+    step, so each update is lr * g / (|g| + eps); zero gradient: no move; a
+    sign flip +1, -1: m_hat_2 = -0.01 / 0.19 and v_hat_2 = 1 exactly."""
+    lr = 1e-2
+    w0 = np.array([0.5, -0.25, 0.0, 2.0])
+    g = np.array([3.0, -0.5, 0.0, 1e-3])
+    w, m, v = w0.copy(), np.zeros(4), np.zeros(4)
+    for t in range(1, 6):
+        w, m, v = mo.adam_step(w, m, v, g, t, lr)
+        step = lr * g / (np.abs(g) + mo.EPS)
+        assert np.allclose(w, w0 - t * step, rtol=0, atol=1e-12)
+    w, m, v = mo.adam_step([1.0], [0.0], [0.0], [1.0], 1, lr)
+    w, m, v = mo.adam_step(w, m, v, [-1.0], 2, lr)
+    mh, vh = -0.01 / 0.19, 1.0
+    assert abs(w[0] - (1.0 - lr / (1 + mo.EPS) - lr * mh / (np.sqrt(vh) + mo.EPS))) < 1e-15

@@ -149,7 +149,7 @@ def test_oracle_vs_reference_build_unconnected():
 
 def _wvc_by_reference_process(adj, M):
     """WVC by replaying the reference GPU process with 32-bit words
-    (cudacode/cudaASPLconv.cu:267-331): AND over all words of all vertices each
+    (cudacode/cudaASPLconv.cu:244-308): AND over all words of all vertices each
     level, WVC = popcount(mask ^ complete) at each level with updates."""
     N = adj.shape[0]
     width = 1
