@@ -331,11 +331,12 @@ def main():
     ap.add_argument("--seed", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
-    ap.add_argument("--only", choices=["roofline", "copy", "learned", "large"], default=None,
+    ap.add_argument("--only", choices=["roofline", "copy", "learned", "plain", "large"],
+                    default=None,
                     help="run only the dense-kernel roofline launches, only a device copy of "
                          "known size (the two legs of the rocprofv3 --pmc traffic runs), or "
-                         "only the learned search (for kernel profiles of its step), or only the "
-                         "large-N orbit evaluation")
+                         "only the learned or the plain search (for kernel profiles of their "
+                         "steps), or only the large-N orbit evaluation")
     ap.add_argument("--equal-time", type=float, default=10.0,
                     help="seconds each of plain and learned search from the same graph for "
                          "the equal-wall-time ASPL comparison (0 skips it)")
@@ -373,6 +374,11 @@ def main():
 
     plain_cfg = gg.search_cfg(mode=gg.MODE_PLAIN, total_steps=(args.warmup + args.steps) * K,
                               T0=T0, T1=T1, seed=12345)
+    if args.only == "plain":
+        sp, _, _, ep, tp = run_search(gg, adj, plain_cfg, args.warmup, args.steps)
+        print(json.dumps({"plain_evals_per_s": ep / tp, "ms_per_step": tp / args.steps * 1e3}),
+              flush=True)
+        return
     sp, _, stp, ep, tp = run_search(gg, adj, plain_cfg, args.warmup, args.steps)
     plain_rate = ep / tp
     sp.close()
@@ -414,6 +420,13 @@ def main():
           flush=True)
     if not args.no_roofline:
         out["roofline"] = plain_kernel_roofline(gg, adj)
+        # the plain path against its own BFS kernel: a move costs at least one
+        # launch of it, so this is the plain rate no host overhead could beat
+        kms = out["roofline"]["kernel_ms"]
+        out["plain"]["kernel_ms"] = kms
+        out["plain"]["kernel_bound_value"] = 1e3 / kms
+        out["plain"]["ms_per_step_over_kernel_ms"] = out["plain"]["ms_per_step"] / kms
+        out["speedup_vs_plain_kernel"] = learn_rate / (1e3 / kms)
     out["batched_vs_sequential"] = batched_vs_sequential(gg)
     out["large_n"] = large_n_orbit(gg)
     print(json.dumps({"progress": "kernels timed"}), file=sys.stderr, flush=True)

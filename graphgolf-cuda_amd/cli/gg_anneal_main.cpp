@@ -6,7 +6,7 @@
 //
 //   gg-anneal [-c COUNT] [-s SEED] [-i IN] [-o OUT] [-l LOG] [-v VERBOSE] [-D DEVICE]
 //             [-t TEMP] [--hill] [--mode plain|batched|learned] [-K K] [-P P]
-//             [--schedule step|geometric] [--temp-end T] [-f part|piece|box]
+//             [--schedule step|geometric] [--temp-end T] [-f part|piece|box] [--wvc]
 //
 // Input and output are `part` tables (part.hpp:13-43) by default.  M = N: an
 // explicit graph, 2-opt swaps (gg_search); M < N: a cyclically symmetric
@@ -23,7 +23,10 @@
 // (annealing.cpp:429-436), which differs only when a later solution of
 // smaller diameter has a larger ASPL.  A best-solution dump in the verbose
 // file is written once per 1000 iterations (for the last improvement among
-// them) rather than at every improvement.
+// them) rather than at every improvement.  --wvc compares candidates on
+// (diameter, WVC, ASPL) as annealingWVC.cpp:412-417 / HillClimbWVC.cpp:240
+// do (part input, plain mode), with annealingWVC's schedule by default:
+// temp = TEMP^(1 - i/COUNT), geometric from TEMP to 1 (annealingWVC.cpp:411).
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -42,7 +45,8 @@ namespace {
 int usage(int code) {
     std::cout << "gg-anneal [-c COUNT] [-s SEED] [-i IN] [-o OUT] [-l LOG] [-v VERBOSE] [-D DEVICE]\n"
                  "          [-t TEMP] [--hill] [--mode plain|batched|learned] [-K K] [-P P]\n"
-                 "          [--schedule step|geometric] [--temp-end T] [-f part|piece|box]\n";
+                 "          [--schedule step|geometric] [--temp-end T] [-f part|piece|box]\n"
+                 "          [--wvc]\n";
     return code;
 }
 
@@ -53,7 +57,7 @@ const char *kClear = "\x1b[F\x1b[E\x1b[K";  // the reference's line rewrite (ann
 int main(int argc, char **argv) {
     auto t_init = std::chrono::steady_clock::now();
     int count = 1000, device = 0, K = 32, P = 1024;
-    bool have_seed = false, hill = false;
+    bool have_seed = false, hill = false, wvc = false, have_schedule = false;
     unsigned int seed = 0;
     double inittemp = 46.1793, temp_end = -1;
     std::string in_path, out_path, log_path, verbose_path, mode = "plain", schedule = "step";
@@ -77,7 +81,8 @@ int main(int argc, char **argv) {
         else if (a == "--mode") mode = next();
         else if (a == "-K") K = std::stoi(next());
         else if (a == "-P") P = std::stoi(next());
-        else if (a == "--schedule") schedule = next();
+        else if (a == "--schedule") { schedule = next(); have_schedule = true; }
+        else if (a == "--wvc") wvc = true;
         else if (a == "--temp-end") temp_end = std::stod(next());
         else if (a == "-f" || a == "--format") fmt = next();
         else { std::cout << "unknown option " << a << "\n"; return usage(1); }
@@ -206,7 +211,10 @@ int main(int argc, char **argv) {
 
     gg_search_cfg cfg{};
     cfg.mode = mode == "learned" ? GG_MODE_LEARNED : mode == "batched" ? GG_MODE_BATCHED : GG_MODE_PLAIN;
-    cfg.rule = hill ? GG_RULE_HILL : GG_RULE_ANNEAL;
+    cfg.rule = wvc ? (hill ? GG_RULE_HILL_WVC : GG_RULE_ANNEAL_WVC)
+                   : (hill ? GG_RULE_HILL : GG_RULE_ANNEAL);
+    if (wvc && !have_schedule) schedule = "geometric";
+    if (wvc && temp_end <= 0) temp_end = 1.0;
     cfg.K = K;
     cfg.P = P;
     cfg.total_steps = count > 0 ? count : 1;

@@ -711,13 +711,34 @@ struct gg_eval {
     long long *d_reach = nullptr;
     gg_swap *d_swap = nullptr;
     gg_swap *h_swap = nullptr;  // pinned
+    gg_swap *h_commit = nullptr;  // pinned: the swap a queued commit kernel reads
+    int commit_pending = 0;       // a commit kernel may not have run yet
     std::vector<int32_t> h_adj;  // adjacency evaluators: host copy of the resident base
     bool base_sym = false;       // the resident base is undirected
     int have_base = 0;  // adjacency evaluators: a table is resident
     int lmax = GG_LEVEL_LIMIT;
     int have_result = 0;
     int last_levels = 0;
+    int levels_on = 1;        // GG_OPT_LEVELS
+    int wvc_on = 0;           // GG_OPT_WVC
+    int swap_graph = 0;       // GG_OPT_SWAP_GRAPH: 0 copies + events, 1 graph, 2 launches
+    int levels_valid = 0;     // the last evaluation kept eccentricities and level counts
+    int events_valid = 0;     // the last evaluation recorded its kernel-time events
+    // captured single-swap evaluations, one per push setting of the swap
+    hipGraph_t sg_graph[2] = {nullptr, nullptr};
+    hipGraphExec_t sg_exec[2] = {nullptr, nullptr};
 };
+
+// Captured swap graphs hold the launch arguments chosen at capture time:
+// every option change drops them.
+static void drop_swap_graphs(gg_eval *ev) {
+    for (int k = 0; k < 2; k++) {
+        if (ev->sg_exec[k]) (void)hipGraphExecDestroy(ev->sg_exec[k]);
+        if (ev->sg_graph[k]) (void)hipGraphDestroy(ev->sg_graph[k]);
+        ev->sg_exec[k] = nullptr;
+        ev->sg_graph[k] = nullptr;
+    }
+}
 
 static int eval_alloc(gg_eval *ev, size_t tab_elem_bytes) {
     GG_HIP(hipSetDevice(ev->device));
@@ -744,6 +765,8 @@ static int eval_alloc(gg_eval *ev, size_t tab_elem_bytes) {
     if (hipMalloc((void **)&ev->d_reach, sizeof(long long) * L) != hipSuccess) return GG_OOM;
     if (hipMalloc((void **)&ev->d_swap, sizeof(gg_swap)) != hipSuccess) return GG_OOM;
     if (hipHostMalloc((void **)&ev->h_swap, sizeof(gg_swap), hipHostMallocDefault) != hipSuccess)
+        return GG_OOM;
+    if (hipHostMalloc((void **)&ev->h_commit, sizeof(gg_swap), hipHostMallocDefault) != hipSuccess)
         return GG_OOM;
     // source -> vertex map
     std::vector<int> src(ev->M);
@@ -798,6 +821,7 @@ extern "C" int gg_create_part(int N, int M, int degree, int device, gg_eval **ou
     gg_eval *ev = new gg_eval();
     ev->kind = EVAL_PART;
     ev->N = N; ev->M = M; ev->degree = degree; ev->device = device;
+    ev->wvc_on = 1;  // cudaASPLconv::WVCdiameterASPL
     bool narrow = N <= 65535;
     ev->dc.topo_kind = narrow ? TOPO_RINGQ : TOPO_RING32;
     return finish_create(ev, narrow ? 2 : 4, out);
@@ -857,6 +881,7 @@ extern "C" void gg_destroy(gg_eval *ev) {
     if (!ev) return;
     (void)hipSetDevice(ev->device);
     if (ev->stream) (void)hipStreamSynchronize(ev->stream);
+    drop_swap_graphs(ev);
     (void)hipFree(ev->d_tab);
     (void)hipHostFree(ev->h_tab);
     (void)hipFree(ev->d_src);
@@ -868,6 +893,7 @@ extern "C" void gg_destroy(gg_eval *ev) {
     (void)hipFree(ev->d_reach);
     (void)hipFree(ev->d_swap);
     (void)hipHostFree(ev->h_swap);
+    (void)hipHostFree(ev->h_commit);
     if (ev->dc.gscratch) (void)hipFree(ev->dc.gscratch);
     if (ev->ev0) (void)hipEventDestroy(ev->ev0);
     if (ev->ev1) (void)hipEventDestroy(ev->ev1);
@@ -985,10 +1011,10 @@ static int eval_common(gg_eval *ev, gg_result *out, bool upload = true,
     a.swaps = d_swap;
     a.parts = ev->d_parts;
     a.res = ev->d_res;
-    a.ecc_out = ev->d_ecc;
-    a.reach_col = ev->d_reach_col;
-    a.reach_out = ev->d_reach;
-    a.want_wvc = ev->kind == EVAL_PART ? 1 : 0;
+    a.ecc_out = ev->levels_on || ev->wvc_on ? ev->d_ecc : nullptr;
+    a.reach_col = ev->levels_on ? ev->d_reach_col : nullptr;
+    a.reach_out = ev->levels_on ? ev->d_reach : nullptr;
+    a.want_wvc = ev->wvc_on;
     ev->last_a = a;
     GG_HIP(hipEventRecord(ev->ev0, ev->stream));
     int st = run_dense(ev->dc, a, ev->stream);  // records ev1 after the BFS kernel
@@ -998,6 +1024,8 @@ static int eval_common(gg_eval *ev, gg_result *out, bool upload = true,
     GG_HIP(hipStreamSynchronize(ev->stream));
     fill_result(ev, *ev->h_res, out);
     ev->have_result = 1;
+    ev->levels_valid = ev->levels_on;
+    ev->events_valid = 1;
     ev->last_levels = ev->h_res->levels;
     return GG_OK;
 }
@@ -1067,16 +1095,84 @@ static int check_swap(const gg_eval *ev, const gg_swap *sw) {
     return GG_OK;
 }
 
+// One swap evaluation as a replayed HIP graph: the BFS kernel reads the swap
+// from pinned host memory and finalize writes the result there, so a call is
+// a host store, one graph launch and one wait.  The graph is captured on
+// first use for each push setting (the launch arguments depend on it) and
+// dropped by any option change.  Only where the column bitset fits in LDS
+// (the persistent global and level-synchronous kernels allocate scratch or
+// read counters back between launches).
+static int eval_swap_graph(gg_eval *ev, gg_result *out) {
+    const int key = ev->dc.push_ok ? 1 : 0;
+    DenseArgs a{};
+    a.N = ev->N; a.M = ev->M; a.D = ev->degree; a.W = ev->W; a.K = 1;
+    a.lmax = ev->lmax;
+    a.src_vertex = ev->d_src;
+    a.swaps = ev->h_swap;
+    a.parts = ev->d_parts;
+    a.res = ev->h_res;
+    a.ecc_out = ev->levels_on || ev->wvc_on ? ev->d_ecc : nullptr;
+    a.reach_col = ev->levels_on ? ev->d_reach_col : nullptr;
+    a.reach_out = ev->levels_on ? ev->d_reach : nullptr;
+    a.want_wvc = ev->wvc_on;
+    if (ev->swap_graph == 2) {
+        // the same two kernels launched one by one (no graph)
+        bind_topology(ev);
+        DenseCtx &c = ev->dc;
+        hipEvent_t done = c.bfs_done;
+        c.bfs_done = nullptr;
+        const int wide = c.wide_ok;
+        c.wide_ok = 0;
+        const int st = run_dense(c, a, ev->stream);
+        c.bfs_done = done;
+        c.wide_ok = wide;
+        if (st != GG_OK) return st;
+    } else if (!ev->sg_exec[key]) {
+        bind_topology(ev);
+        DenseCtx &c = ev->dc;
+        hipEvent_t done = c.bfs_done;
+        c.bfs_done = nullptr;  // no event nodes in the graph
+        const int wide = c.wide_ok;
+        c.wide_ok = 0;
+        GG_HIP(hipStreamBeginCapture(ev->stream, hipStreamCaptureModeThreadLocal));
+        int st = run_dense(c, a, ev->stream);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(ev->stream, &g);
+        c.bfs_done = done;
+        c.wide_ok = wide;
+        if (st != GG_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return st;
+        }
+        GG_HIP(e);
+        ev->sg_graph[key] = g;
+        GG_HIP(hipGraphInstantiate(&ev->sg_exec[key], g, nullptr, nullptr, 0));
+    }
+    if (ev->swap_graph == 1) GG_HIP(hipGraphLaunch(ev->sg_exec[key], ev->stream));
+    GG_HIP(hipStreamSynchronize(ev->stream));
+    ev->commit_pending = 0;
+    fill_result(ev, *ev->h_res, out);
+    ev->last_a = a;
+    ev->have_result = 1;
+    ev->levels_valid = ev->levels_on;
+    ev->events_valid = 0;
+    ev->last_levels = ev->h_res->levels;
+    return GG_OK;
+}
+
 extern "C" int gg_eval_adj_swap(gg_eval *ev, const gg_swap *sw, gg_result *out) {
     if (!ev || !sw || !out || ev->kind != EVAL_ADJ || !ev->have_base) return GG_BAD_ARG;
     int st = check_swap(ev, sw);
     if (st != GG_OK) return st;
     GG_HIP(hipSetDevice(ev->device));
-    *ev->h_swap = *sw;
-    GG_HIP(hipMemcpyAsync(ev->d_swap, ev->h_swap, sizeof(gg_swap), hipMemcpyHostToDevice,
-                          ev->stream));
     ev->dc.push_ok = ev->base_sym && swap_keeps_symmetry(ev->h_adj.data(), ev->degree, *sw);
     ev->dc.wide_push_ok = ev->dc.push_ok;
+    // the previous call's kernels have finished (every call waits), so the
+    // pinned swap can be rewritten
+    *ev->h_swap = *sw;
+    if (ev->swap_graph && dense_fits_lds(ev->dc, ev->N)) return eval_swap_graph(ev, out);
+    GG_HIP(hipMemcpyAsync(ev->d_swap, ev->h_swap, sizeof(gg_swap), hipMemcpyHostToDevice,
+                          ev->stream));
     return eval_common(ev, out, false, ev->d_swap);
 }
 
@@ -1102,10 +1198,24 @@ extern "C" int gg_eval_adj_commit(gg_eval *ev, const gg_swap *sw) {
     for (int j = 0; j < 4; j++)
         if (sw->v[j] >= 0) ev->h_adj[(size_t)sw->v[j] * ev->degree + sw->slot[j]] = sw->nbr[j];
     GG_HIP(hipSetDevice(ev->device));
+    const bool q = ev->dc.topo_kind == TOPO_RINGQ;
+    if (ev->swap_graph) {
+        // queued, not waited for: the kernel reads the swap from pinned
+        // memory, which is rewritten only after it has run (the next
+        // evaluation waits on the stream, a second commit waits here)
+        if (ev->commit_pending) GG_HIP(hipStreamSynchronize(ev->stream));
+        *ev->h_commit = *sw;
+        hipLaunchKernelGGL(k_commit_swap, dim3(1), dim3(64), 0, ev->stream,
+                           q ? (uint16_t *)ev->d_tab : nullptr,
+                           q ? nullptr : (uint32_t *)ev->d_tab, ev->N,
+                           (const gg_swap *)ev->h_commit);
+        GG_HIP(hipGetLastError());
+        ev->commit_pending = 1;
+        return GG_OK;
+    }
     *ev->h_swap = *sw;
     GG_HIP(hipMemcpyAsync(ev->d_swap, ev->h_swap, sizeof(gg_swap), hipMemcpyHostToDevice,
                           ev->stream));
-    const bool q = ev->dc.topo_kind == TOPO_RINGQ;
     hipLaunchKernelGGL(k_commit_swap, dim3(1), dim3(64), 0, ev->stream,
                        q ? (uint16_t *)ev->d_tab : nullptr, q ? nullptr : (uint32_t *)ev->d_tab,
                        ev->N, (const gg_swap *)ev->d_swap);
@@ -1133,6 +1243,7 @@ extern "C" int gg_eval_box(gg_eval *ev, const int32_t *table, gg_result *out) {
 extern "C" int gg_eval_levels(gg_eval *ev, int32_t *ecc_out, int64_t *reach_out, int reach_cap) {
     if (!ev || reach_cap < 0) return GG_BAD_ARG;
     if (!ev->have_result) return GG_BAD_ARG;
+    if (!ev->levels_valid) return GG_UNSUPPORTED;  // evaluated with GG_OPT_LEVELS 0
     GG_HIP(hipSetDevice(ev->device));
     if (ecc_out)
         GG_HIP(hipMemcpyAsync(ecc_out, ev->d_ecc, sizeof(int) * ev->M, hipMemcpyDeviceToHost,
@@ -1177,12 +1288,14 @@ extern "C" int gg_eval_time_bfs(gg_eval *ev, int reps, float *ms) {
 
 extern "C" int gg_eval_last_kernel_ms(gg_eval *ev, float *ms) {
     if (!ev || !ms || !ev->have_result) return GG_BAD_ARG;
+    if (!ev->events_valid) return GG_UNSUPPORTED;  // a captured swap evaluation
     GG_HIP(hipEventElapsedTime(ms, ev->ev0, ev->ev1));
     return GG_OK;
 }
 
 extern "C" int gg_eval_set_option(gg_eval *ev, int option, int value) {
     if (!ev) return GG_BAD_ARG;
+    drop_swap_graphs(ev);
     switch (option) {
         case GG_OPT_BLOCK:
             if (value != 0 && value != 256 && value != 1024) return GG_BAD_ARG;
@@ -1205,6 +1318,18 @@ extern "C" int gg_eval_set_option(gg_eval *ev, int option, int value) {
         case GG_OPT_WIDE:
             ev->dc.wide_off = value ? 0 : 1;
             return GG_OK;
+        case GG_OPT_SWAP_GRAPH:
+            if (value < 0 || value > 2) return GG_BAD_ARG;
+            if (value && ev->kind != EVAL_ADJ) return GG_UNSUPPORTED;
+            ev->swap_graph = value;
+            return GG_OK;
+        case GG_OPT_LEVELS:
+            ev->levels_on = value ? 1 : 0;
+            return GG_OK;
+        case GG_OPT_WVC:
+            if (value && ev->kind != EVAL_PART && ev->kind != EVAL_ADJ) return GG_UNSUPPORTED;
+            ev->wvc_on = value ? 1 : 0;
+            return GG_OK;
         default: return GG_BAD_ARG;
     }
 }
@@ -1218,6 +1343,9 @@ extern "C" int gg_eval_get_option(gg_eval *ev, int option, int *value) {
         case GG_OPT_LEVEL_LIMIT: *value = ev->lmax; return GG_OK;
         case GG_OPT_PUSH: *value = ev->dc.push_off ? 0 : 1; return GG_OK;
         case GG_OPT_WIDE: *value = ev->dc.wide_off ? 0 : 1; return GG_OK;
+        case GG_OPT_SWAP_GRAPH: *value = ev->swap_graph; return GG_OK;
+        case GG_OPT_LEVELS: *value = ev->levels_on; return GG_OK;
+        case GG_OPT_WVC: *value = ev->wvc_on; return GG_OK;
         case 100: *value = ev->dc.last_variant; return GG_OK;  // kernel variant of the last call
         case 101: *value = ev->dc.last_vpt; return GG_OK;      // its vertices per thread
         case 102: *value = ev->dc.last_fixed; return GG_OK;    // its fixed degree (0 = generic)
@@ -1294,6 +1422,7 @@ extern "C" int gg_eval_autotune(gg_eval *ev, const int32_t *table, int reps,
                                 gg_tune_report *report) {
     if (!ev || !table || reps < 1) return GG_BAD_ARG;
     GG_HIP(hipSetDevice(ev->device));
+    drop_swap_graphs(ev);  // the kernel choice changes
     int st;
     switch (ev->kind) {
         case EVAL_PART: st = pack_ring(ev, table, ev->M, false); break;

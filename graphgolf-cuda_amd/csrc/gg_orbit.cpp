@@ -285,13 +285,6 @@ Lattice lattice_of(const gg_orbit_search *s) {
     return L;
 }
 
-bool better(const gg_result &x, const gg_result &y) {
-    bool xo = x.status == GG_OK, yo = y.status == GG_OK;
-    if (xo != yo) return xo;
-    if (x.diameter != y.diameter) return x.diameter < y.diameter;
-    return x.total_scaled < y.total_scaled;
-}
-
 int eval_table(gg_orbit_search *s, const int32_t *t, gg_result *out) {
     if (s->use_cb) return s->cb.eval(s->cb.ctx, t, out);
     if (s->dims == 2) return gg_eval_piece(s->ev, t, out);
@@ -302,8 +295,10 @@ int eval_table(gg_orbit_search *s, const int32_t *t, gg_result *out) {
 int validate(int N, int M, int d, const int32_t *table, const gg_search_cfg *cfg) {
     if (!cfg || !table || M < 1 || N < 5 || d < 2 || N % M != 0) return GG_BAD_ARG;
     if (cfg->mode != GG_MODE_PLAIN) return GG_UNSUPPORTED;
-    if (cfg->rule != GG_RULE_ANNEAL && cfg->rule != GG_RULE_HILL) return GG_BAD_ARG;
-    if (cfg->rule == GG_RULE_ANNEAL && (!(cfg->T0 > 0) || !(cfg->T1 > 0))) return GG_BAD_ARG;
+    if (cfg->rule < GG_RULE_ANNEAL || cfg->rule > GG_RULE_HILL_WVC) return GG_BAD_ARG;
+    if ((cfg->rule == GG_RULE_ANNEAL || cfg->rule == GG_RULE_ANNEAL_WVC) &&
+        (!(cfg->T0 > 0) || !(cfg->T1 > 0)))
+        return GG_BAD_ARG;
     if (cfg->schedule != GG_SCHED_GEOMETRIC && cfg->schedule != GG_SCHED_STEP) return GG_BAD_ARG;
     if (cfg->total_steps < 1) return GG_BAD_ARG;
     return GG_OK;
@@ -319,6 +314,8 @@ int validate_lattice(int dims, const int32_t *Ns, const int32_t *Ms, int d,
         M *= Ms[k];
     }
     if (N > INT32_MAX) return GG_BAD_ARG;
+    // WVC exists for part tables only (cudaASPLconv::WVCdiameterASPL)
+    if (cfg && gg::rule_uses_wvc(cfg->rule)) return GG_UNSUPPORTED;
     return validate((int)N, (int)M, d, table, cfg);
 }
 
@@ -359,21 +356,9 @@ int orbit_step(gg_orbit_search *s) {
     if (st != GG_OK) return st;
     s->stats.exact_evals++;
     s->step++;
-    const double NN1 = (double)s->N * (s->N - 1);
-    const double fx = (double)s->cur.total_scaled / NN1;
-    const int dx = s->cur.diameter;
     const bool ok = y.status == GG_OK;
-    const double fy = ok ? (double)y.total_scaled / NN1 : 1e8;
-    const int dy = ok ? y.diameter : 100000000;
-    bool accept = false;
-    if (cfg.rule == GG_RULE_HILL) {
-        accept = ok && (dy < dx || (dy == dx && fy < fx));
-    } else {
-        const double T = gg::temperature(cfg, s->step);
-        if (fy < fx || dy < dx) accept = true;
-        else if (rng.uniform() < std::exp((fx - fy) * NN1 / (T * NN1))) accept = true;
-        if (dy > dx || !ok) accept = false;
-    }
+    const bool accept =
+        gg::accept_move(cfg, s->step, (double)s->N * (s->N - 1), s->cur, y, rng);
     if (s->log_cap > 0 && (int64_t)s->log.size() < (int64_t)s->log_cap * 10) {
         int64_t rec[10] = {s->step, mv.a * (int64_t)s->d + mv.ia, mv.c * (int64_t)s->d + mv.ic,
                            mv.k1, mv.k2, mv.kind, ok ? y.total_scaled : -1, ok ? y.diameter : -1,
@@ -384,7 +369,7 @@ int orbit_step(gg_orbit_search *s) {
         s->tab.swap(s->cand);
         s->cur = y;
         s->stats.accepted++;
-        if (better(s->cur, s->best)) {
+        if (gg::better_by_rule(cfg.rule, s->cur, s->best)) {
             s->best = s->cur;
             s->best_tab = s->tab;
         }
@@ -403,6 +388,10 @@ extern "C" int gg_orbit_search_create(int N, int M, int d, const int32_t *table,
     if (st != GG_OK) return st;
     gg_orbit_search *s = new gg_orbit_search();
     st = gg_create_part(N, M, d, device, &s->ev);
+    // the WVC rules evaluate as WVCdiameterASPL does: levels up to 100
+    // (cudaASPLconv.cu:264)
+    if (st == GG_OK && gg::rule_uses_wvc(cfg->rule))
+        st = gg_eval_set_option(s->ev, GG_OPT_LEVEL_LIMIT, 100);
     if (st == GG_OK) st = init(s, N, M, d, table, cfg);
     if (st != GG_OK) { gg_orbit_search_destroy(s); return st; }
     *out = s;

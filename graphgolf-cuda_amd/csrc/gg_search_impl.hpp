@@ -45,6 +45,43 @@ inline double temperature(const gg_search_cfg &cfg, int64_t i) {
     return cfg.T0 * std::pow(cfg.T1 / cfg.T0, (double)i / cfg.total_steps);
 }
 
+inline bool rule_uses_wvc(int rule) { return rule == GG_RULE_ANNEAL_WVC || rule == GG_RULE_HILL_WVC; }
+
+// Acceptance of candidate y against the current solution x at step `step`
+// (1-based), cppcode/annealing.cpp:406-412, HillClimb.cpp:239,
+// annealingWVC.cpp:412-417, HillClimbWVC.cpp:240.  A failed evaluation
+// (unconnected, level limit) stands for the reference's sentinels: diameter
+// 100000000 and ASPL 1e8, WVC 100000000.  The uniform is drawn only on the
+// branch where the reference draws it.
+inline bool accept_move(const gg_search_cfg &cfg, int64_t step, double NN1, const gg_result &x,
+                        const gg_result &y, Rng &rng) {
+    const double fx = (double)x.total_scaled / NN1;
+    const int dx = x.diameter, wx = x.wvc;
+    const bool ok = y.status == GG_OK;
+    const double fy = ok ? (double)y.total_scaled / NN1 : 1e8;
+    const int dy = ok ? y.diameter : 100000000, wy = ok ? y.wvc : 100000000;
+    const bool wvc = rule_uses_wvc(cfg.rule);
+    const bool lex = wvc ? (dy < dx || (dy == dx && wy < wx) || (dy == dx && wy == wx && fy < fx))
+                         : (dy < dx || (dy == dx && fy < fx));
+    if (cfg.rule == GG_RULE_HILL || cfg.rule == GG_RULE_HILL_WVC) return ok && lex;
+    bool accept = false;
+    if (wvc ? lex : (fy < fx || dy < dx)) accept = true;
+    else if (rng.uniform() < std::exp((fx - fy) * NN1 / (temperature(cfg, step) * NN1)))
+        accept = true;
+    if (dy > dx || (wvc && dy == dx && wy > wx) || !ok) accept = false;
+    return accept;
+}
+
+// Best-solution order: connected first, then (diameter, total), or
+// (diameter, WVC, total) under the WVC rules.
+inline bool better_by_rule(int rule, const gg_result &x, const gg_result &y) {
+    const bool xo = x.status == GG_OK, yo = y.status == GG_OK;
+    if (xo != yo) return xo;
+    if (x.diameter != y.diameter) return x.diameter < y.diameter;
+    if (rule_uses_wvc(rule) && x.wvc != y.wvc) return x.wvc < y.wvc;
+    return x.total_scaled < y.total_scaled;
+}
+
 // Symmetric `part` tables (gg_orbit.cpp): one move attempt (edge-length change
 // or orbit 2-opt) written to `out`; false when it must be redrawn.
 struct OrbitMove {

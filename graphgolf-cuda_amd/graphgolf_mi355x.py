@@ -33,6 +33,9 @@ GG_OPT_GENERIC = 3
 GG_OPT_LEVEL_LIMIT = 4
 GG_OPT_PUSH = 5
 GG_OPT_WIDE = 6
+GG_OPT_SWAP_GRAPH = 7
+GG_OPT_LEVELS = 8
+GG_OPT_WVC = 9
 GG_INFO_VARIANT = 100
 GG_INFO_VPT = 101
 GG_INFO_FIXED = 102
@@ -132,7 +135,7 @@ MODE_PLAIN, MODE_BATCHED, MODE_LEARNED = 0, 1, 2
 MLP_FEATURES, MLP_HIDDEN = 32, 128
 MLP_PARAMS = MLP_HIDDEN * MLP_FEATURES + MLP_HIDDEN + MLP_HIDDEN + 1
 MLP_MAX_BATCH = 512
-RULE_ANNEAL, RULE_HILL = 0, 1
+RULE_ANNEAL, RULE_HILL, RULE_ANNEAL_WVC, RULE_HILL_WVC = 0, 1, 2, 3
 SCHED_GEOMETRIC, SCHED_STEP = 0, 1
 
 _lib = None
@@ -650,7 +653,7 @@ class OrbitSearch:
     edge-length change and orbit 2-opt (cppcode/annealing.cpp:112-197), each
     candidate table evaluated exactly from the M representatives.  With
     `table_eval` (a Python function table[M][d] -> (total_scaled, diameter,
-    status)), the exact evaluator is the caller's."""
+    status[, wvc])), the exact evaluator is the caller's."""
 
     def __init__(self, N, table, cfg: SearchCfg, device: int = 0, table_eval=None, log: int = 0):
         t = _i32(table)
@@ -666,10 +669,12 @@ class OrbitSearch:
 
             def ev(ctx, tab, out):
                 a = np.ctypeslib.as_array(tab, shape=(M * d,)).reshape(M, d).copy()
-                total, diam, status = table_eval(a)
+                res = tuple(table_eval(a))
+                total, diam, status = res[:3]
                 r = out.contents
                 r.total_reps, r.total_scaled, r.diameter, r.wvc, r.levels, r.status = (
-                    total // (N // M), total, diam, 0, diam + 1, status)
+                    total // (N // M), total, diam, res[3] if len(res) > 3 else 0, diam + 1,
+                    status)
                 return 0
 
             self._fn = TABLE_CB(ev)
@@ -774,15 +779,15 @@ class LatticeSearch(OrbitSearch):
 
 class PyEvaluatorCallbacks:
     """Wraps a Python evaluator object exposing set_base(adj) -> (total, diam,
-    status), eval(list of Swap) -> list of (total, diam, status) and
-    commit(Swap) -> (total, diam, status) as gg_eval_callbacks."""
+    status[, wvc]), eval(list of Swap) -> list of such tuples and
+    commit(Swap) -> one as gg_eval_callbacks."""
 
     def __init__(self, impl, N, d):
         self.impl, self.N, self.d = impl, N, d
 
         def _fill(r, t):
             r.total_reps, r.total_scaled, r.diameter, r.wvc, r.levels, r.status = (
-                t[0], t[0], t[1], 0, t[1] + 1, t[2])
+                t[0], t[0], t[1], t[3] if len(t) > 3 else 0, t[1] + 1, t[2])
 
         def set_base(ctx, adj, out):
             a = np.ctypeslib.as_array(adj, shape=(N * d,)).reshape(N, d).copy()

@@ -49,7 +49,7 @@ typedef struct gg_result {
     int64_t total_reps;   /* sum over the M sources s of sum_v d(s,v) (pairs reached) */
     int64_t total_scaled; /* total_reps * (N / M), integer N/M as cudaASPLconv.cu:240 */
     int32_t diameter;     /* max eccentricity over the sources (reached part)         */
-    int32_t wvc;          /* WVC as cudaASPLconv.cu:256-285 computes it (part only)   */
+    int32_t wvc;          /* WVC as cudaASPLconv.cu:256-285 computes it (GG_OPT_WVC)  */
     int32_t levels;       /* BFS levels executed (diameter + 1 when connected)        */
     int32_t status;       /* GG_OK, GG_UNCONNECTED or GG_DIAMETER_LIMIT                */
 } gg_result;
@@ -148,6 +148,21 @@ int gg_eval_time_bfs(gg_eval *ev, int reps, float *ms);
                                  runs level-synchronously over the whole grid (one
                                  launch per level, GG_INFO_VARIANT 4); 0 = one
                                  workgroup per column (GG_INFO_VARIANT 3)            */
+#define GG_OPT_SWAP_GRAPH 7   /* adjacency evaluators: how gg_eval_adj_swap runs its
+                                 BFS and finalize kernels.  0 (default) = swap and
+                                 result copied, kernel-time events recorded; 1 = a
+                                 captured HIP graph of the two kernels, which read
+                                 the swap from and write the result to pinned host
+                                 memory; 2 = the same two kernels launched directly
+                                 (no graph).  1 and 2 record no events
+                                 (gg_eval_last_kernel_ms is then unavailable),
+                                 queue gg_eval_adj_commit without waiting, and
+                                 apply where the column bitset fits in LDS          */
+#define GG_OPT_LEVELS     8   /* 1 (default) = keep per-source eccentricities and
+                                 per-level counts for gg_eval_levels; 0 = skip them
+                                 (gg_eval_levels then returns GG_UNSUPPORTED)        */
+#define GG_OPT_WVC        9   /* 1 = compute gg_result.wvc (default 1 for part, 0 for
+                                 adjacency evaluators; piece and box have no WVC)    */
 /* Read-only facts about the most recent launch (gg_eval_get_option). */
 #define GG_INFO_VARIANT   100 /* 1 = LDS bitset / 256 threads, 2 = LDS / 1024, 3 = HBM,
                                  4 = HBM level-synchronous                          */

@@ -25,6 +25,18 @@ typedef struct gg_search gg_search;
 
 #define GG_RULE_ANNEAL  0  /* cppcode/annealing.cpp:406-412                   */
 #define GG_RULE_HILL    1  /* cppcode/HillClimb.cpp:239                        */
+/* The same two rules on (diameter, WVC, ASPL) compared lexicographically, WVC
+ * as WVCdiameterASPL computes it (cudacode/cudaASPLconv.cu:256-285):
+ * annealing accepts a lexicographic improvement, else draws the uniform; a
+ * larger diameter, or an equal diameter with a larger WVC, is always rejected
+ * (cppcode/annealingWVC.cpp:412-417); hill climbing accepts only a
+ * lexicographic improvement (cppcode/HillClimbWVC.cpp:240).  The best
+ * solution is kept in the same order (annealingWVC.cpp:433).  Plain mode of
+ * gg_search (explicit graphs, all N vertices as sources) and part tables of
+ * gg_orbit_search only: the reference has WVC for cudaASPLconv alone.  The
+ * level limit of these searches is WVCdiameterASPL's, 100 (:264). */
+#define GG_RULE_ANNEAL_WVC 2
+#define GG_RULE_HILL_WVC   3
 
 #define GG_SCHED_GEOMETRIC 0  /* T(i) = T0 (T1/T0)^(i/total_steps)              */
 #define GG_SCHED_STEP      1  /* T(i) = T0 0.995^floor((i-1)/10^4), updated after

@@ -86,17 +86,46 @@ def temperature(step, T0, T1, total_steps, schedule=0):
     return T0 * math.pow(T1 / T0, step / total_steps)
 
 
+def accept(rule, rng, step, NN1, x, y, T0, T1, total_steps, schedule):
+    """Acceptance of y = (total, diameter, ok, wvc) against x = (total,
+    diameter, wvc): cppcode/annealing.cpp:406-412, HillClimb.cpp:239,
+    annealingWVC.cpp:412-417, HillClimbWVC.cpp:240.  Failed candidates carry
+    the sentinels 100000000 / 1e8."""
+    tx, dx, wx = x
+    ty, dy, ok, wy = y
+    fx = tx / NN1
+    fy = ty / NN1 if ok else 1e8
+    dyv = dy if ok else 100000000
+    wyv = wy if ok else 100000000
+    wvc = rule.endswith("_wvc")
+    lex = ((dyv, wyv, fy) < (dx, wx, fx)) if wvc else ((dyv, fy) < (dx, fx))
+    if rule.startswith("hill"):
+        return ok and lex
+    acc = False
+    if (lex if wvc else (fy < fx or dyv < dx)):
+        acc = True
+    elif rng.uniform() < math.exp((fx - fy) * NN1 /
+                                  (temperature(step, T0, T1, total_steps, schedule) * NN1)):
+        acc = True
+    if dyv > dx or (wvc and dyv == dx and wyv > wx) or not ok:
+        acc = False
+    return acc
+
+
 def run(adj, evaluate, steps, mode_K=1, rule="anneal", T0=1e-3, T1=1e-6, total_steps=None,
         seed=12345, schedule=0):
-    """evaluate(adj) -> (total_scaled, diameter, ok).  Returns the per-step log
-    rows {step, a, b, c, d, kind, total, diameter, status, accepted}."""
+    """evaluate(adj) -> (total_scaled, diameter, ok[, wvc]).  rule: anneal,
+    hill, anneal_wvc, hill_wvc.  Returns the per-step log rows {step, a, b,
+    c, d, kind, total, diameter, status, accepted} and the final graph."""
     adj = np.array(adj, dtype=np.int32)
     n, d = adj.shape
     total_steps = total_steps or steps
     rng = MT64(seed)
-    tx, dx, _ = evaluate(adj)
+    ev = lambda g: (tuple(evaluate(g)) + (0,))[:4]  # noqa: E731
+    tx, dx, _, wx = ev(adj)
     log = []
     NN1 = float(n) * (n - 1)
+    wvc = rule.endswith("_wvc")
     for step in range(1, steps + 1):
         cands = []
         for _ in range(mode_K):
@@ -110,29 +139,18 @@ def run(adj, evaluate, steps, mode_K=1, rule="anneal", T0=1e-3, T1=1e-6, total_s
             new = adj.copy()
             for v, s, x in ents:
                 new[v, s] = x
-            evals.append(evaluate(new) + (new,))
-        # best by (ok, diameter, total)
-        bi = min(range(len(evals)), key=lambda k: (not evals[k][2], evals[k][1], evals[k][0]))
-        ty, dy, ok, new = evals[bi]
-        fx = tx / NN1
-        fy = ty / NN1 if ok else 1e8
-        dyv = dy if ok else 100000000
-        if rule == "hill":
-            acc = ok and (dyv < dx or (dyv == dx and fy < fx))
-        else:
-            T = temperature(step, T0, T1, total_steps, schedule)
-            acc = False
-            if fy < fx or dyv < dx:
-                acc = True
-            elif rng.uniform() < math.exp((fx - fy) * NN1 / (T * NN1)):
-                acc = True
-            if dyv > dx or not ok:
-                acc = False
+            evals.append(ev(new) + (new,))
+        # best by (ok, diameter, [wvc,] total)
+        bi = min(range(len(evals)), key=lambda k: (not evals[k][2], evals[k][1],
+                                                   evals[k][3] if wvc else 0, evals[k][0]))
+        ty, dy, ok, wy, new = evals[bi]
+        acc = accept(rule, rng, step, NN1, (tx, dx, wx), (ty, dy, ok, wy), T0, T1, total_steps,
+                     schedule)
         mv = cands[bi][0]
         log.append([step, *mv, ty if ok else -1, dy if ok else -1, 0 if ok else 1, int(acc)])
         if acc:
             adj = new
-            tx, dx = ty, dy
+            tx, dx, wx = ty, dy, wy
     return np.array(log, dtype=np.int64), adj
 
 
@@ -323,13 +341,14 @@ def random_lattice_table(Ns, Ms, d, seed):
 
 def run_orbit(N, table, evaluate, steps, rule="hill", T0=1e-3, T1=1e-6, total_steps=None,
               seed=1, propose=None):
-    """evaluate(table) -> (total_scaled, diameter, ok).  Returns the per-step
-    log rows {step, a*d+slot_a, c*d+slot_c, k1, k2, kind, total, diameter,
-    status, accepted} and the final table."""
+    """evaluate(table) -> (total_scaled, diameter, ok[, wvc]).  Returns the
+    per-step log rows {step, a*d+slot_a, c*d+slot_c, k1, k2, kind, total,
+    diameter, status, accepted} and the final table."""
     t = np.array(table, dtype=np.int32)
     total_steps = total_steps or steps
     rng = MT64(seed)
-    tx, dx, _ = evaluate(t)
+    ev = lambda g: (tuple(evaluate(g)) + (0,))[:4]  # noqa: E731
+    tx, dx, _, wx = ev(t)
     NN1 = float(N) * (N - 1)
     log = []
     for step in range(1, steps + 1):
@@ -338,21 +357,10 @@ def run_orbit(N, table, evaluate, steps, rule="hill", T0=1e-3, T1=1e-6, total_st
             if p is not None:
                 break
         rec, new = p
-        ty, dy, ok = evaluate(new)
-        fx = tx / NN1
-        fy = ty / NN1 if ok else 1e8
-        dyv = dy if ok else 100000000
-        if rule == "hill":
-            acc = ok and (dyv < dx or (dyv == dx and fy < fx))
-        else:
-            T = T0 * math.pow(T1 / T0, step / total_steps)
-            acc = fy < fx or dyv < dx
-            if not acc and rng.uniform() < math.exp((fx - fy) * NN1 / (T * NN1)):
-                acc = True
-            if dyv > dx or not ok:
-                acc = False
+        ty, dy, ok, wy = ev(new)
+        acc = accept(rule, rng, step, NN1, (tx, dx, wx), (ty, dy, ok, wy), T0, T1, total_steps, 0)
         log.append([step, *rec, ty if ok else -1, dy if ok else -1, 0 if ok else 1, int(acc)])
         if acc:
             t = new
-            tx, dx = ty, dy
+            tx, dx, wx = ty, dy, wy
     return np.array(log, dtype=np.int64), t

@@ -22,13 +22,27 @@ def evaluate(adj, threads=None):
     return tot.value, (dia.value if rc == 0 else 0), (0 if rc == 0 else 1)
 
 
+def evaluate_wvc(adj):
+    """(total, diameter, status, WVC) of an explicit adjacency: queue BFS from
+    every vertex with per-source eccentricities, WVC by its definition over
+    the 32 source classes (cudacode/cudaASPLconv.cu:256-285)."""
+    r = ol.adj_eval(adj)
+    ok = not r["unconnected"]
+    w = ol.wvc(adj.shape[0], r["ecc"], r["diameter"]) if ok else 0
+    return r["total"], (r["diameter"] if ok else 0), (0 if ok else 1), w
+
+
 class OracleSwapEvaluator:
-    def __init__(self):
+    """The search-callback evaluator on the oracle; with wvc=True the results
+    carry WVC (single-threaded BFS with eccentricities)."""
+
+    def __init__(self, wvc=False):
         self.adj = None
+        self.fn = evaluate_wvc if wvc else evaluate
 
     def set_base(self, adj):
         self.adj = np.array(adj, dtype=np.int32)
-        return evaluate(self.adj)
+        return self.fn(self.adj)
 
     def _apply(self, sw, adj):
         new = adj.copy()
@@ -38,11 +52,11 @@ class OracleSwapEvaluator:
         return new
 
     def eval(self, swaps):
-        return [evaluate(self._apply(sw, self.adj)) for sw in swaps]
+        return [self.fn(self._apply(sw, self.adj)) for sw in swaps]
 
     def commit(self, sw):
         self.adj = self._apply(sw, self.adj)
-        return evaluate(self.adj)
+        return self.fn(self.adj)
 
 
 def evaluate_part(N, table, threads=None):

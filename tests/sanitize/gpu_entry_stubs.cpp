@@ -22,6 +22,7 @@ int gg_eval_box(gg_eval *, const int32_t *, gg_result *) { return GG_UNSUPPORTED
 int gg_eval_adj(gg_eval *, const int32_t *, gg_result *) { return GG_UNSUPPORTED; }
 int gg_eval_adj_swap(gg_eval *, const gg_swap *, gg_result *) { return GG_UNSUPPORTED; }
 int gg_eval_adj_commit(gg_eval *, const gg_swap *) { return GG_UNSUPPORTED; }
+int gg_eval_set_option(gg_eval *, int, int) { return GG_UNSUPPORTED; }
 int gg_batch_create(int, int, int, int, gg_batch **out) { if (out) *out = nullptr; return GG_UNSUPPORTED; }
 void gg_batch_destroy(gg_batch *) {}
 int gg_batch_set_base(gg_batch *, const int32_t *, gg_result *) { return GG_UNSUPPORTED; }

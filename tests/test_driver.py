@@ -56,3 +56,40 @@ def test_step_schedule_matches_python_restatement(gg):
                                 seed=5, schedule=1)
     assert np.array_equal(log, ref_log)
     s.close()
+
+
+@pytest.mark.parametrize("rule", ["anneal_wvc", "hill_wvc"])
+def test_wvc_rules_match_python_restatement(gg, rule):
+    """Lexicographic (diameter, WVC, ASPL) acceptance of annealingWVC.cpp:412-417
+    and HillClimbWVC.cpp:240, with WVC from the oracle's eccentricities."""
+    adj = gg.random_regular(64, 4, 9)
+    steps = 80
+    r = gg.RULE_ANNEAL_WVC if rule == "anneal_wvc" else gg.RULE_HILL_WVC
+    cfg = gg.search_cfg(mode=gg.MODE_PLAIN, rule=r, total_steps=steps, T0=2e-2, T1=1e-4,
+                        seed=31)
+    cb = gg.PyEvaluatorCallbacks(oracle_eval.OracleSwapEvaluator(wvc=True), *adj.shape)
+    s = gg.Search(adj, cfg, callbacks=cb, log=steps)
+    st = s.run(steps)
+    log = s.log()
+
+    def ev(a):
+        t, d, status, w = oracle_eval.evaluate_wvc(a)
+        return t, d, status == 0, w
+    ref_log, ref_adj = driver_ref.run(adj, ev, steps, rule=rule, T0=2e-2, T1=1e-4, seed=31)
+    assert np.array_equal(log, ref_log)
+    assert np.array_equal(s.graph(), ref_adj)
+    assert st.accepted == int(ref_log[:, 9].sum()) > 0
+    # the rule differs from the plain one on this run: some step decided by WVC
+    plain_log, _ = driver_ref.run(adj, lambda a: ev(a)[:3], steps,
+                                  rule=rule.replace("_wvc", ""), T0=2e-2, T1=1e-4, seed=31)
+    assert not np.array_equal(plain_log, ref_log)
+    s.close()
+
+
+def test_wvc_rules_need_the_single_candidate_mode(gg):
+    adj = gg.random_regular(64, 4, 9)
+    cfg = gg.search_cfg(mode=gg.MODE_BATCHED, rule=gg.RULE_HILL_WVC, K=4, total_steps=10)
+    cb = gg.PyEvaluatorCallbacks(oracle_eval.OracleSwapEvaluator(wvc=True), *adj.shape)
+    with pytest.raises(gg.GGError) as e:
+        gg.Search(adj, cfg, callbacks=cb)
+    assert e.value.code == gg.GG_UNSUPPORTED

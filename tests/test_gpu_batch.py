@@ -76,3 +76,43 @@ def test_batch_disconnecting_swap(gpu):
         be.set_base(adj)
         r = be.eval([sw])[0]
     assert o["unconnected"] and r.status == gg.GG_UNCONNECTED and r.total_reps == o["total"]
+
+
+def test_swap_graph_matches_launched_path(gpu):
+    """GG_OPT_SWAP_GRAPH: the captured single-swap evaluation (swap read from
+    and result written to pinned memory) gives the results of the launched
+    path on swaps that keep the graph undirected and on swaps that do not
+    (the two push settings are two captured graphs), before and after
+    commits; with GG_OPT_LEVELS 0 gg_eval_levels reports GG_UNSUPPORTED, and
+    kernel-time events exist only for launched evaluations."""
+    gg = gpu
+    n, d = 2048, 8
+    adj = gg.random_regular(n, d, 12)
+    rng = np.random.default_rng(3)
+    swaps = [c[0] for c in _random_swaps(gg, adj, 24, rng)]
+    # a directed swap: one slot rewritten without its partner
+    bad = gg.Swap()
+    for j in range(4):
+        bad.v[j], bad.slot[j], bad.nbr[j] = -1, 0, 0
+    bad.v[0], bad.slot[0], bad.nbr[0] = 5, 0, int(adj[5][0]) ^ 1
+    swaps.insert(7, bad)
+    with gg.SwapEvaluator(n, d) as a, gg.SwapEvaluator(n, d) as b, \
+            gg.SwapEvaluator(n, d) as c:
+        for ev in (a, b, c):
+            ev.set_base(adj)
+        b.set_option(gg.GG_OPT_SWAP_GRAPH, 1)
+        b.set_option(gg.GG_OPT_LEVELS, 0)
+        c.set_option(gg.GG_OPT_SWAP_GRAPH, 2)
+        for i, sw in enumerate(swaps):
+            ra, rb, rc = a.eval_swap(sw), b.eval_swap(sw), c.eval_swap(sw)
+            assert ra.as_dict() == rb.as_dict() == rc.as_dict(), i
+            if i in (3, 11, 12):  # commits (two in a row) of undirected swaps
+                for ev in (a, b, c):
+                    ev.commit(sw)
+        assert np.array_equal(a.levels()[0], c.levels()[0])
+        with pytest.raises(gg.GGError) as e:
+            b.levels()
+        assert e.value.code == gg.GG_UNSUPPORTED
+        with pytest.raises(gg.GGError):
+            b.last_kernel_ms()
+        assert a.last_kernel_ms() > 0

@@ -200,3 +200,25 @@ def test_gg_inittemp_prints_the_library_estimate(gpu, N, M, d):
     assert abs(float(vals["avg"]) - r.mean_increase_total) <= 1e-8 * abs(r.mean_increase_total)
     assert abs(float(vals["inittemp"]) - r.temp_total) <= 1e-8 * abs(r.temp_total)
     assert abs(r.temp_total + r.mean_increase_total / np.log(0.4)) <= 1e-9 * abs(r.temp_total)
+
+
+@pytest.mark.parametrize("extra", [["--wvc", "--hill"], ["--wvc"]])
+def test_gg_anneal_wvc(gpu, tmp_path, extra):
+    """--wvc: annealingWVC / HillClimbWVC acceptance on (diameter, WVC, ASPL)
+    for an explicit graph and for an orbit graph; the printed best is no
+    worse than the input in that order (WVC from the oracle)."""
+    import oracle_lib as ol
+    gg = gpu
+    for N, tab in ((256, _part_of_adj(gg.random_regular(256, 6, 5))),
+                   (16 * 64, gg.orbit_part(16, 64, 8, 5))):
+        M = tab.shape[0]
+        f, o = tmp_path / "in.part", tmp_path / "out.part"
+        _write_part(f, N, tab)
+        r0 = ol.part_eval(N, M, tab)
+        _run([os.path.join(BIN, "gg-anneal"), "-c", "300", "-s", "3", "-i", str(f), "-o", str(o),
+              "-t", "20"] + extra)
+        N2, best = _read_part(o)
+        r1 = ol.part_eval(N, M, best)
+        assert not r1["unconnected"]
+        assert (r1["diameter"], r1["wvc"], r1["total_scaled"]) <= \
+            (r0["diameter"], r0["wvc"], r0["total_scaled"])

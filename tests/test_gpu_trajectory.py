@@ -1,7 +1,8 @@
 """Annealing trajectories on the GPU evaluator match, move for move, the same
 driver run on the CPU oracle (the reference's compile-time evaluator swap,
 cppcode/annealing.cpp:296 vs annealing_cpu.cpp:292): config C2 (random
-12-regular n=4096, 2-opt, geometric T from 1e-3 to 1e-6 over 10^4 moves)."""
+12-regular n=4096, 2-opt, geometric T from 1e-3 to 1e-6 over 10^4 moves),
+a 500-move prefix."""
 import numpy as np
 import pytest
 
@@ -19,7 +20,7 @@ def _cfg(gg, mode, K=1, total=10000):
 def test_c2_plain_trajectory_prefix_matches_cpu(gpu):
     gg = gpu
     adj = gg.random_regular(4096, 12, 2)
-    steps = 200
+    steps = 500
     g = gg.Search(adj, _cfg(gg, gg.MODE_PLAIN), log=steps)
     g.run(steps)
     cb = gg.PyEvaluatorCallbacks(oracle_eval.OracleSwapEvaluator(), *adj.shape)

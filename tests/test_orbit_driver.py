@@ -144,3 +144,29 @@ def test_lattice_generator_tables_are_symmetric_simple():
         e = {(v, int(u)) for v in range(adj.shape[0]) for u in adj[v]}
         assert all((u, v) in e for v, u in e)
         assert all(len(set(adj[v].tolist())) == d and v not in adj[v] for v in range(adj.shape[0]))
+
+
+@pytest.mark.parametrize("rule", ["anneal_wvc", "hill_wvc"])
+def test_orbit_wvc_rules_match_python_restatement(gg, rule):
+    g, m, d = 8, 32, 6
+    N = g * m
+    tab = gg.orbit_part(g, m, d, 21)
+    steps = 80
+    r = gg.RULE_ANNEAL_WVC if rule == "anneal_wvc" else gg.RULE_HILL_WVC
+    cfg = gg.search_cfg(mode=gg.MODE_PLAIN, rule=r, total_steps=steps, T0=2e-2, T1=1e-4,
+                        seed=99)
+
+    def ev(t):
+        o = ol.part_eval(N, m, t)
+        ok = not o["unconnected"]
+        return o["total_scaled"], o["diameter"] if ok else 0, 0 if ok else 1, o["wvc"]
+    with gg.OrbitSearch(N, tab, cfg, table_eval=ev, log=steps) as s:
+        st = s.run(steps)
+        log = s.log()
+        final = s.table()
+    ref_log, ref_tab = driver_ref.run_orbit(
+        N, tab, lambda t: (lambda x: (x[0], x[1], x[2] == 0, x[3]))(ev(t)), steps, rule=rule,
+        T0=2e-2, T1=1e-4, seed=99)
+    assert np.array_equal(log, ref_log)
+    assert np.array_equal(final, ref_tab)
+    assert st.accepted == int(ref_log[:, 9].sum())
