@@ -28,7 +28,7 @@ static const int kRowLevels = 3;  // levels kept row-major (the features use 1..
 
 // The sparse kernel keeps one column (N words) in LDS plus small bookkeeping.
 // Static LDS of k_sparse (bookkeeping and the launch's swaps), rounded up.
-static const int kSparseStaticLds = 8192;
+static const int kSparseStaticLds = 10240;
 static const int kStagedKeysMax = 8192;
 
 static bool sparse_fits(const gg_batch *b) {
@@ -116,19 +116,26 @@ static int batch_alloc(gg_batch *b) {
         if (hipMalloc((void **)&b->d_qctl, sizeof(int) * b->nqctl) != hipSuccess) return GG_OOM;
         if (hipMalloc((void **)&b->d_dense_parts, sizeof(ColPart) * T) != hipSuccess)
             return GG_OOM;
+        if (hipMalloc((void **)&b->d_ss_cnt, sizeof(int) * 4 * (size_t)K * L) != hipSuccess ||
+            hipMalloc((void **)&b->d_ss_ecc, sizeof(int) * 4 * (size_t)K) != hipSuccess ||
+            hipMalloc((void **)&b->d_ss_flags, sizeof(int) * 4 * (size_t)K) != hipSuccess ||
+            hipMalloc((void **)&b->d_mb_cnt, sizeof(int) * T * (cap + 1)) != hipSuccess)
+            return GG_OOM;
         // commit logs: optional (without them every commit runs a new BFS)
         const size_t lg = TC * cap * (size_t)b->cap;
         if (hipMalloc((void **)&b->d_lg_key, 4 * lg) != hipSuccess ||
             hipMalloc((void **)&b->d_lg_word, 8 * lg) != hipSuccess ||
             hipMalloc((void **)&b->d_lg_n, 4 * TC * cap) != hipSuccess ||
             hipMalloc((void **)&b->d_ep_snap, 8 * (size_t)4 * chunk * cap * N) != hipSuccess ||
-            hipMalloc((void **)&b->d_ep_reach, 8 * (size_t)4 * chunk * L) != hipSuccess) {
+            hipMalloc((void **)&b->d_ep_reach, 8 * (size_t)4 * chunk * L) != hipSuccess ||
+            hipMalloc((void **)&b->d_ss_vis, 4 * (size_t)4 * chunk * cap * NB) != hipSuccess) {
             (void)hipGetLastError();
-            void *lgb[] = {b->d_lg_key, b->d_lg_word, b->d_lg_n, b->d_ep_snap, b->d_ep_reach};
+            void *lgb[] = {b->d_lg_key, b->d_lg_word, b->d_lg_n, b->d_ep_snap, b->d_ep_reach,
+                           b->d_ss_vis};
             for (void *p : lgb)
                 if (p) (void)hipFree(p);
             b->d_lg_key = nullptr; b->d_lg_word = nullptr; b->d_lg_n = nullptr;
-            b->d_ep_snap = nullptr; b->d_ep_reach = nullptr;
+            b->d_ep_snap = nullptr; b->d_ep_reach = nullptr; b->d_ss_vis = nullptr;
         }
         GG_HIP(hipMemsetAsync(b->d_qctl, 0, sizeof(int) * b->nqctl, b->stream));
         GG_HIP(hipMemsetAsync(b->d_dl_bits, 0, 4 * TC * 2 * NB, b->stream));
@@ -169,7 +176,8 @@ extern "C" void gg_batch_destroy(gg_batch *b) {
                     b->d_ecc, b->d_base_reach, b->d_snap, b->d_dl_key, b->d_dl_bits, b->d_dl_ovr,
                     b->d_task_state, b->d_task_end, b->d_task_flags, b->d_dcnt, b->d_queue,
                     b->d_qctl, b->d_dense_parts, b->d_lg_key, b->d_lg_word, b->d_lg_n,
-                    b->d_ep_snap, b->d_ep_reach, b->d_rows, b->d_task_ecc};
+                    b->d_ep_snap, b->d_ep_reach, b->d_rows, b->d_task_ecc, b->d_ss_cnt,
+                    b->d_ss_ecc, b->d_ss_flags, b->d_ss_vis, b->d_mb_cnt};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     if (b->h_tab) (void)hipHostFree(b->h_tab);
@@ -303,16 +311,19 @@ static int check_swaps(const gg_batch *b, const gg_swap *sw, int K) {
 
 // Sparse kernel for the degree (fixed-degree instances with a dense tail
 // when vpt > 0, the runtime-degree one without).
-static hipError_t launch_sparse(const SparseArgs &s, const TopoRingQ &topo, const DenseArgs &qa,
-                                size_t lds, int vpt, hipStream_t st) {
+static hipError_t launch_sparse(const SparseArgs &s, size_t lds, int tail, hipStream_t st) {
     switch (s.D) {
-        case 4: return launch_sparse_d4(s, topo, qa, lds, vpt, st);
-        case 8: return launch_sparse_d8(s, topo, qa, lds, vpt, st);
-        case 12: return launch_sparse_d12(s, topo, qa, lds, vpt, st);
-        case 16: return launch_sparse_d16(s, topo, qa, lds, vpt, st);
-        case 24: return launch_sparse_d24(s, topo, qa, lds, vpt, st);
-        default: return launch_sparse_deg<0>(s, topo, qa, lds, 0, st);
+        case 4: return launch_sparse_d4(s, lds, tail, st);
+        case 8: return launch_sparse_d8(s, lds, tail, st);
+        case 12: return launch_sparse_d12(s, lds, tail, st);
+        case 16: return launch_sparse_d16(s, lds, tail, st);
+        case 24: return launch_sparse_d24(s, lds, tail, st);
+        default: return launch_sparse_deg<0>(s, lds, 0, st);
     }
+}
+
+static bool fixed_sparse_degree(int D) {
+    return D == 4 || D == 8 || D == 12 || D == 16 || D == 24;
 }
 
 // Workgroups of the dense fallback queue kernel after a sparse launch.
@@ -350,17 +361,16 @@ static int enqueue_sparse(gg_batch *b, int K, bool reset = true) {
     s.lg_n = b->d_lg_n;
     s.ep_snap = b->d_ep_snap;
     s.ep_reach = b->d_ep_reach;
+    s.ss_cnt = b->d_ss_cnt;
+    s.ss_ecc = b->d_ss_ecc;
+    s.ss_flags = b->d_ss_flags;
+    s.ss_vis = b->d_ss_vis;
+    s.mb_cnt = b->d_mb_cnt;
+    s.undirected = b->dc.push_ok ? 1 : 0;
     const size_t lds = sparse_lds_bytes(N, b->skcap);
-    // dense tail of the sparse kernel (endpoint columns): fixed-degree ring
-    // table, its vertices-per-thread instance, a push list within the LDS
-    // the sparse layout leaves after B_{l-1}
-    int vpt = 0;
-    if (b->narrow && b->dc.ringq.R == b->dc.ringq.N && !b->dc.generic_opt) {
-        const int v = pick_vpt(1024, (N + 1023) / 1024);
-        if (v == 4 || v == 8 || v == 16) vpt = v;
-    }
-    DenseArgs qa = dense_args(b, K, b->d_swaps, b->d_dense_parts, nullptr, nullptr);
-    qa.push_cap = push_capacity(b->dc, N, b->degree, (int)lds);
+    // single-source tail of the sparse kernel (endpoint sources): the
+    // fixed-degree instances
+    const int tail = fixed_sparse_degree(b->degree) && !b->dc.generic_opt ? 1 : 0;
     // the logs of the last launch survive: they cover its candidates
     b->log_k0 = ((K - 1) / b->chunk) * b->chunk;
     b->log_K = K;
@@ -372,11 +382,11 @@ static int enqueue_sparse(gg_batch *b, int K, bool reset = true) {
         // claiming grid: a workgroup per column, more (one per CU) while there
         // are CUs left for the endpoint tasks
         c.grid = W;
-        if (vpt > 0) {
+        if (tail) {
             const int want = W + 4 * c.K;
             c.grid = want < b->dc.cu_count ? want : (W > b->dc.cu_count ? W : b->dc.cu_count);
         }
-        hipError_t e = launch_sparse(c, b->dc.ringq, qa, lds, vpt, b->stream);
+        hipError_t e = launch_sparse(c, lds, tail, b->stream);
         if (e != hipSuccess) {
             gg_set_last_hip_error(e, "k_sparse", __FILE__, __LINE__);
             return GG_HIP_ERROR;
@@ -436,7 +446,11 @@ extern "C" int gg_batch_eval(gg_batch *b, const gg_swap *swaps, int K, gg_result
     for (int k = 0; k < K && sym; k++) sym = swap_keeps_symmetry(b->h_adj.data(), b->degree, swaps[k]);
     if ((b->dc.push_ok != 0) != sym) drop_graph(b);  // the captured launches bake the push choice
     b->dc.push_ok = sym;
-    const bool sparse = b->sparse_ok && b->mode_opt != 1;
+    // the incremental path finds the vertices a changed word can reach
+    // through the changed vertices' own rows, which lists the vertices that
+    // pull from them only on an undirected graph: directed candidates (and
+    // bases) run densely
+    const bool sparse = b->sparse_ok && b->mode_opt != 1 && sym;
     GG_HIP(hipEventRecord(b->ev0, b->stream));
     if (sparse && b->use_graph) {
         if (b->graph_k != K) {
@@ -476,7 +490,8 @@ namespace gg {
 // The caller reads the results and swaps back (batch_out) and resets the
 // queue counters after the step.
 int batch_enqueue_dev(gg_batch *b, int K) {
-    if (!b->have_base || !b->sparse_ok || b->mode_opt == 1 || K < 1 || K > b->max_k)
+    if (!b->have_base || !b->sparse_ok || b->mode_opt == 1 || K < 1 || K > b->max_k ||
+        !b->base_sym)
         return GG_UNSUPPORTED;
     // swaps drawn as 2-opt rewirings of an undirected base keep it undirected
     b->dc.push_ok = b->base_sym;
@@ -553,10 +568,29 @@ struct CommitArgs {
     const unsigned long long *ep_snap;
     const long long *ep_reach;
     const ColPart *dense_parts;      // [K][W]
+    const int *ss_cnt, *ss_ecc;      // single-source counts / eccentricities (global index)
+    const int *mb_cnt;               // [K][W][S+1] base pairs of the masked endpoint bits
+    const unsigned *ss_vis;          // [4K][S][NB] visited bitmaps (index in the logs' launch)
+    int NB;
     uint16_t *tab16;                 // resident tables: the swap's slots are rewritten
     uint32_t *tab32;
     int *adjR;
 };
+
+// Column w's word after level l of the committed candidate, for a column
+// holding endpoint sources (bits m, run masked): the masked word x with each
+// endpoint's bit from its single-source visited bitmap.
+__device__ __forceinline__ unsigned long long commit_unmask(const CommitArgs &c, int w, int l,
+                                                            unsigned long long m, int v,
+                                                            unsigned long long x) {
+    x &= ~m;
+    for (int j = 0; j < 4; j++) {
+        if (!endpoint_first(c.sw, j) || (c.sw.v[j] >> 6) != w) continue;
+        const unsigned *vis = c.ss_vis + ((size_t)(4 * c.k + j) * c.S + (l - 1)) * c.NB;
+        if ((vis[v >> 5] >> (v & 31)) & 1u) x |= 1ull << (c.sw.v[j] & 63);
+    }
+    return x;
+}
 
 // Endpoint task (4k + j) of the dense tail that ran column w: the first
 // endpoint of the swap in that column.
@@ -575,17 +609,76 @@ __device__ __forceinline__ int commit_col_ecc(const CommitArgs &c, int w) {
 // w < W, rewrites level l of column w (and of the row-major copy for l <= RL
 // when the column is incremental); workgroups w >= W write the row-major copy
 // of the endpoint columns, 1024 vertices each (endpoint j, vertex chunk).
+// Masked column (endpoint sources run masked, their bits from the
+// single-source bitmaps): level l of vertices [v0, v0 + 1024) of column w.
+// Every word of the chunk is recomputed (logged words where the Delta log
+// holds them, the stored word elsewhere) and written to the stored level and
+// its row-major copy (full words at every level) where it changes.  One
+// workgroup per chunk: the chunks are disjoint, so no word is written twice.
+__device__ void commit_masked_chunk(const CommitArgs &c, int w, int l, int v0,
+                                    unsigned long long m) {
+    __shared__ unsigned long long s_dw[1024];
+    __shared__ unsigned s_dm[32];
+    const int tid = threadIdx.x, N = c.N;
+    const size_t t = (size_t)c.gk * c.W + w;
+    const int ecc_new = commit_col_ecc(c, w), end = c.task_end[t];
+    const bool store = l < ecc_new, full = l >= c.base_parts[w].ecc;
+    const bool rw = c.rows && l <= c.RL;
+    if (!store && !rw) return;  // nothing kept at this level
+    if (tid < 32) s_dm[tid] = 0u;
+    __syncthreads();
+    if (l <= end) {
+        const size_t lt = ((size_t)c.k * c.W + w) * c.S + (l - 1);
+        const int *key = c.lg_key + lt * c.cap;
+        const unsigned long long *word = c.lg_word + lt * c.cap;
+        const int n = c.lg_n[lt];
+        for (int i = tid; i < n; i += 1024) {
+            const int x = key[i] - v0;
+            if (x < 0 || x >= 1024) continue;
+            s_dw[x] = word[i];
+            atomicOr(&s_dm[x >> 5], 1u << (x & 31));
+        }
+    }
+    __syncthreads();
+    const int v = v0 + tid;
+    if (v >= N) return;
+    unsigned long long *dst = c.snap + ((size_t)(l - 1) * c.W + w) * N;
+    // the word as stored (levels at or above the base column's eccentricity
+    // are not stored: all ones)
+    const unsigned long long o = full ? ~0ull : dst[v];
+    unsigned long long x = ~0ull;
+    if (store) {
+        const bool logged = (s_dm[tid >> 5] >> (tid & 31)) & 1u;
+        x = commit_unmask(c, w, l, m, v, logged ? s_dw[tid] : o);
+        if (full || x != o) dst[v] = x;
+    }
+    if (rw && x != o) c.rows[((size_t)(l - 1) * N + v) * c.W + w] = x;
+}
+
+// Grid (W + 4 * chunks, S) with chunks = ceil(N / 1024): workgroup (w, l-1),
+// w < W, rewrites level l of column w (and of the row-major copy for l <= RL
+// when the column is incremental); workgroups w >= W handle the columns
+// holding endpoint sources, 1024 vertices each (endpoint j, vertex chunk):
+// the masked columns' words, and the row-major copy of columns that ran
+// densely.
 __global__ void __launch_bounds__(1024) k_commit_levels(CommitArgs c) {
     const int l = blockIdx.y + 1, tid = threadIdx.x;
     const int N = c.N;
     if ((int)blockIdx.x >= c.W) {
-        if (!c.rows || l > c.RL) return;
         const int chunks = (N + 1023) >> 10;
         const int e = ((int)blockIdx.x - c.W) / chunks, ch = ((int)blockIdx.x - c.W) - e * chunks;
         const int v0 = c.sw.v[e];
         if (v0 < 0) return;
         const int w = v0 >> 6;
-        if (commit_ep_slot(c, w) != 4 * c.k + e) return;  // column written by a lower endpoint
+        // each column once: by its first endpoint
+        for (int j = 0; j < e; j++)
+            if (c.sw.v[j] >= 0 && (c.sw.v[j] >> 6) == w) return;
+        if (c.task_state[(size_t)c.gk * c.W + w] != GG_TASK_DENSE) {
+            commit_masked_chunk(c, w, l, ch * 1024, endpoint_mask(c.sw, w));
+            return;
+        }
+        // a column that ran densely (no single-source tail): its copy
+        if (!c.rows || l > c.RL) return;
         const int ecc_new = commit_col_ecc(c, w);
         const unsigned long long *src = c.ep_snap + ((size_t)(4 * c.k + e) * c.S + (l - 1)) * N;
         const int v = ch * 1024 + tid;
@@ -611,6 +704,7 @@ __global__ void __launch_bounds__(1024) k_commit_levels(CommitArgs c) {
         }
         return;
     }
+    if (endpoint_mask(c.sw, w)) return;  // the endpoint chunks rewrite this column
     const int end = c.task_end[t];
     const size_t lt = ((size_t)c.k * c.W + w) * c.S + (l - 1);
     const int *key = c.lg_key + lt * c.cap;
@@ -659,13 +753,29 @@ __global__ void __launch_bounds__(64) k_commit_parts(CommitArgs c) {
     }
     if (lane != 0) return;
     const int ecc = commit_col_ecc(c, w);
-    // one pass in place: level l's old count is read before it is replaced
+    // one pass in place: level l's old count is read before it is replaced;
+    // endpoint sources (masked columns) add their single-source counts in
+    // place of the base's, as the finalize does
     const int blev = c.base_parts[w].levels;
     const long long *dd = c.dcnt + t * c.L1;
-    long long cb = br[0], cprev = cb, tot = 0;
+    const int end = c.task_end[t];
+    const unsigned long long em = endpoint_mask(c.sw, w);
+    const long long nm = __popcll(em);
+    const int becc = c.base_parts[w].ecc;
+    long long cb = br[0], cprev = cb, tot = 0, cx = nm;
     for (int l = 1; l <= ecc; l++) {
         if (l <= blev) cb += br[l];
-        const long long cur = cb + dd[l];
+        long long corr = 0;
+        if (em) {
+            for (int j = 0; j < 4; j++) {
+                if (!endpoint_first(c.sw, j) || (c.sw.v[j] >> 6) != w) continue;
+                const size_t t4 = (size_t)4 * c.gk + j;
+                if (l <= c.ss_ecc[t4]) cx += c.ss_cnt[t4 * c.L1 + l];
+            }
+            const long long bx = l < becc ? c.mb_cnt[t * (c.S + 1) + l] : (long long)c.N * nm;
+            corr = cx - bx;
+        }
+        const long long cur = cb + dd[l <= end ? l : end] + corr;
         br[l] = cur - cprev;
         tot += (long long)l * (cur - cprev);
         cprev = cur;
@@ -766,6 +876,11 @@ extern "C" int gg_batch_commit_known(gg_batch *b, const gg_swap *sw, const gg_re
         c.ep_snap = b->d_ep_snap;
         c.ep_reach = b->d_ep_reach;
         c.dense_parts = b->d_dense_parts;
+        c.ss_cnt = b->d_ss_cnt;
+        c.ss_ecc = b->d_ss_ecc;
+        c.mb_cnt = b->d_mb_cnt;
+        c.ss_vis = b->d_ss_vis;
+        c.NB = (b->N + 31) / 32;
         c.tab16 = b->narrow ? (uint16_t *)b->d_tab : nullptr;
         c.tab32 = b->narrow ? nullptr : (uint32_t *)b->d_tab;
         c.adjR = b->d_adjR;

@@ -44,6 +44,12 @@ struct gg_batch {
     int *d_lg_n = nullptr;                    // [chunk*W][snap_cap]
     unsigned long long *d_ep_snap = nullptr;  // [4*chunk][snap_cap][N]
     long long *d_ep_reach = nullptr;          // [4*chunk][LEVEL_LIMIT+1]
+    // single-source BFSes of the swap endpoints (sparse kernels with a tail)
+    int *d_ss_cnt = nullptr;                  // [4*max_k][LEVEL_LIMIT+1]
+    int *d_ss_ecc = nullptr;                  // [4*max_k]
+    int *d_mb_cnt = nullptr;                  // [max_k][W][snap_cap+1]
+    int *d_ss_flags = nullptr;                // [4*max_k]
+    unsigned *d_ss_vis = nullptr;             // [4*chunk][snap_cap][ceil(N/32)] (commit)
     int log_valid = 0;                        // the last eval was sparse with logs kept
     int log_k0 = 0, log_K = 0;                // candidates the logs cover: [log_k0, log_K)
     int ecc_stale = 0;                        // d_ecc predates an in-place commit

@@ -24,10 +24,28 @@
 // the columns that contain a swap endpoint as a source, are queued for the
 // dense kernel (k_dense_lds_queue) or run densely in the kernel's own tail.
 //
+// A column that holds a swap endpoint x as a source differs from the base in
+// x's bit nearly everywhere (x's own BFS tree changes), so its Delta would
+// overflow.  Bits are independent BFSes, though: in fixed-degree kernels
+// with a tail (TAIL = 1) such a column runs incrementally with the endpoint
+// sources' bits masked out of every word compared, counted and logged (the
+// base's counts of those bits are taken from the stored levels as they are
+// staged), and the tail runs a single-source BFS from each endpoint on the
+// candidate graph (top-down while the frontier is small,
+// bottom-up after it, as Beamer et al. 2012 "Direction-optimizing
+// breadth-first search"; bottom-up only where the candidate graph may be
+// directed, since the evaluators pull from listed neighbours).  The column's
+// reached count after level l is then
+//     base count - base counts of the masked sources + masked delta
+//                + candidate counts of the masked sources,
+// which k_sparse_finalize combines.  Without a tail (TAIL = 0) those columns
+// run densely in the queue kernel.
+//
 // Every Delta_l (l <= S) is also logged as (vertex, V_l word) pairs, and the
-// tail keeps the endpoint columns' words after each level: committing a
-// candidate then rewrites the stored base levels in place (B_l := V_l on
-// Delta_l, endpoint columns copied whole) instead of running a new BFS.
+// single-source BFSes log the candidate's visited bitmap after each level:
+// committing a candidate then rewrites the stored base levels in place
+// (B_l := V_l on Delta_l, the endpoint sources' bits from their bitmaps)
+// instead of running a new BFS.
 #pragma once
 #include "gg_dense.hpp"
 
@@ -73,7 +91,35 @@ struct SparseArgs {
     int *lg_n;                       // [K*W][S] |Delta_l|
     unsigned long long *ep_snap;     // [4K][S][N]
     long long *ep_reach;             // [4K][lmax+1]
+    // single-source BFS of endpoint j of candidate g on the candidate graph
+    // (task 4g + j, g global)
+    int *ss_cnt;                     // [4(k0+K)][lmax+1] new vertices per level
+    int *ss_ecc;                     // [4(k0+K)] eccentricity of the source
+    int *ss_flags;                   // [4(k0+K)] GG_FLAG_LIMIT
+    // per task (g, w) with masked endpoint sources: their base reached pairs
+    // after levels 0..S (popcount of the masked bits of B_l)
+    int *mb_cnt;                     // [(k0+K)][W][S+1]
+    unsigned *ss_vis;                // [4K][S][ceil(N/32)] candidate visited after level l
+    int undirected;                  // every candidate graph is undirected (top-down allowed)
 };
+
+// The endpoint sources of swap `sw` among column w's 64 sources (bits of the
+// column word); rows may repeat in a swap, vertices count once.
+__host__ __device__ inline unsigned long long endpoint_mask(const gg_swap &sw, int w) {
+    unsigned long long m = 0;
+    for (int j = 0; j < 4; j++)
+        if (sw.v[j] >= 0 && (sw.v[j] >> 6) == w) m |= 1ull << (sw.v[j] & 63);
+    return m;
+}
+
+// Task index of endpoint j of a swap, or -1 when it is unused or repeats an
+// earlier endpoint's vertex.
+__host__ __device__ inline bool endpoint_first(const gg_swap &sw, int j) {
+    if (sw.v[j] < 0) return false;
+    for (int jj = 0; jj < j; jj++)
+        if (sw.v[jj] == sw.v[j]) return false;
+    return true;
+}
 
 // Row of candidate vertex v with the swap's replacements applied: DT > 0
 // loads the whole row as 16-byte vectors (one memory latency), DT = 0 loads
@@ -109,9 +155,10 @@ __device__ __forceinline__ int cand_nbr(const int *adjR, int D, int v, int i, co
 // holds x, else the base word in LDS.  `ubits` (LDS) is the union of
 // Delta_{l-1} over the column's tasks: outside it the base word is the
 // answer without touching the task's bitmap in global memory.
+template <class BW>
 __device__ __forceinline__ unsigned long long prev_word(const unsigned *ubits, const unsigned *bits,
                                                         const unsigned long long *ovr,
-                                                        const unsigned long long *Bprev, int x) {
+                                                        const BW &Bprev, int x) {
     if (!((ubits[x >> 5] >> (x & 31)) & 1u)) return Bprev[x];
     // the membership word and the override are loaded together (one memory
     // latency instead of two); ovr[x] is in bounds and simply discarded
@@ -121,6 +168,161 @@ __device__ __forceinline__ unsigned long long prev_word(const unsigned *ubits, c
     return ((bw >> (x & 31)) & 1u) ? o : Bprev[x];
 }
 
+// Single-source BFS from x, by the whole workgroup, in `lds` (at least
+// 12 * ceil(N/32) + 8 + 8 * lc bytes): visited / frontier / next bitmaps and
+// two frontier lists of lc entries.  A level runs top-down from the frontier
+// list (each listed vertex ORs its neighbours' bits in; allowed only on an
+// undirected graph) while the list holds the frontier and the frontier's
+// edges are fewer than twice the unvisited vertices, else bottom-up (each
+// unvisited vertex looks for a neighbour in the frontier bitmap: the pull
+// semantics of the evaluators).  Writes the new vertices per level to
+// cnt[0..lmax] (cnt[0] = 1), the visited bitmap after levels 1..S to
+// vis_log (when set; levels after the last are the final set) and returns
+// the eccentricity of x; *limit = 1 when level lmax still found vertices.
+template <int BS, int DT>
+__device__ int ss_bfs(const int *adjR, int N, int D, int lmax, int S, int x,
+                      const SwapRegs &sw, bool topdown_ok, unsigned *lds, int lc, int *cnt,
+                      unsigned *vis_log, int *limit) {
+    __shared__ int s_new, s_len[2], s_un;
+    const int tid = threadIdx.x;
+    const int NB = (N + 31) >> 5, NBE = (NB + 1) & ~1;
+    unsigned *vis = lds, *front = lds + NBE, *nxt = lds + 2 * NBE;
+    int *lst0 = reinterpret_cast<int *>(lds + 3 * NBE), *lst1 = lst0 + lc;
+    for (int i = tid; i < NBE; i += BS) { vis[i] = 0u; front[i] = 0u; nxt[i] = 0u; }
+    __syncthreads();
+    if (tid == 0) {
+        vis[x >> 5] = 1u << (x & 31);
+        front[x >> 5] = 1u << (x & 31);
+        lst0[0] = x;
+        s_len[0] = 1;
+        cnt[0] = 1;
+    }
+    __syncthreads();
+    int reached = 1, fsize = 1, in = 0, ecc = 0, l = 1;
+    bool list_ok = true;
+    *limit = 0;
+    for (; l <= lmax; l++) {
+        const bool td = topdown_ok && list_ok && (long long)fsize * D <= 2LL * (N - reached);
+        int *lin = in ? lst1 : lst0, *lout = in ? lst0 : lst1;
+        if (tid == 0) { s_new = 0; s_len[in ^ 1] = 0; }
+        __syncthreads();
+        // a new vertex: counted, appended to the next list while it fits
+        auto append = [&](bool isnew, int y, bool list) {
+            if (!isnew) return;
+            atomicAdd(&s_new, 1);
+            if (!list) return;
+            const int pos = atomicAdd(&s_len[in ^ 1], 1);
+            if (pos < lc) lout[pos] = y;
+        };
+        auto claim = [&](int y) {
+            const unsigned bit = 1u << (y & 31);
+            if (vis[y >> 5] & bit) return false;
+            const unsigned old = atomicOr(&vis[y >> 5], bit);
+            if (old & bit) return false;
+            atomicOr(&nxt[y >> 5], bit);
+            return true;
+        };
+        // the lanes of a wave take consecutive entries
+        const int wbase = tid & ~63, lane = tid & 63;
+        if (td) {
+            if constexpr (DT > 0) {
+                // a frontier vertex per lane, its row in one vector load
+                for (int e0 = wbase; e0 < fsize; e0 += BS) {
+                    const int e = e0 + lane;
+                    const bool ok = e < fsize;
+                    CandRow<DT> row;
+                    if (ok) row.load(adjR, lin[e], sw);
+#pragma unroll
+                    for (int i = 0; i < DT; i++) {
+                        const int y = ok ? row.u[i] : 0;
+                        append(ok && claim(y), y, true);
+                    }
+                }
+            } else {
+                const int items = fsize * D;
+                for (int i0 = wbase; i0 < items; i0 += BS) {
+                    const int it = i0 + lane;
+                    const bool ok = it < items;
+                    int y = 0;
+                    if (ok) {
+                        const int e = it / D, i = it - e * D;
+                        y = cand_nbr(adjR, D, lin[e], i, sw);
+                    }
+                    append(ok && claim(y), y, true);
+                }
+            }
+        } else {
+            // the unvisited vertices listed first (a lane per vertex then
+            // waits for one row load, not one per vertex it scans)
+            if (tid == 0) s_un = 0;
+            __syncthreads();
+            for (int i = tid; i < NB; i += BS) {
+                unsigned u = ~vis[i];
+                if (i == NB - 1 && (N & 31)) u &= (1u << (N & 31)) - 1u;
+                if (!u) continue;
+                int pos = atomicAdd(&s_un, __popc(u));
+                while (u) {
+                    const int b = __builtin_ctz(u);
+                    u &= u - 1u;
+                    if (pos < lc) lout[pos] = 32 * i + b;
+                    pos++;
+                }
+            }
+            __syncthreads();
+            const int nun = s_un;
+            const bool listed = nun <= lc;
+            const int span = listed ? nun : N;
+            for (int v0 = wbase; v0 < span; v0 += BS) {
+                const int e = v0 + lane;
+                const int v = listed ? (e < nun ? lout[e] : N) : e;
+                bool hit = false;
+                if (v < N && !((vis[v >> 5] >> (v & 31)) & 1u)) {
+                    if constexpr (DT > 0) {
+                        CandRow<DT> row;
+                        row.load(adjR, v, sw);
+#pragma unroll
+                        for (int i = 0; i < DT; i++)
+                            hit = hit || ((front[row.u[i] >> 5] >> (row.u[i] & 31)) & 1u);
+                    } else {
+                        for (int i = 0; i < D && !hit; i++) {
+                            const int y = cand_nbr(adjR, D, v, i, sw);
+                            hit = (front[y >> 5] >> (y & 31)) & 1u;
+                        }
+                    }
+                    if (hit) atomicOr(&nxt[v >> 5], 1u << (v & 31));
+                }
+                append(hit, v, false);
+            }
+        }
+        __syncthreads();
+        const int nnew = s_new;
+        const bool lok = td && s_len[in ^ 1] <= lc;
+        for (int i = tid; i < NBE; i += BS) {
+            const unsigned n = nxt[i];
+            const unsigned vv = td ? vis[i] : (vis[i] | n);
+            vis[i] = vv;
+            front[i] = n;
+            nxt[i] = 0u;
+            if (vis_log && l <= S && i < NB) vis_log[(size_t)(l - 1) * NB + i] = vv;
+        }
+        if (tid == 0) cnt[l] = nnew;
+        __syncthreads();
+        if (nnew == 0) break;
+        ecc = l;
+        reached += nnew;
+        fsize = nnew;
+        list_ok = lok;
+        in ^= 1;
+    }
+    if (l > lmax) *limit = 1;
+    // levels after the last: the final visited set
+    if (vis_log)
+        for (int ll = (l > lmax ? lmax : l) + 1; ll <= S; ll++)
+            for (int i = tid; i < NB; i += BS) vis_log[(size_t)(ll - 1) * NB + i] = vis[i];
+    __syncthreads();
+    return ecc;
+}
+
 // Dynamic LDS of k_sparse: B_{l-1}[N] words, the Delta union bitmap, the
 // staged Delta keys.
 __host__ __device__ inline size_t sparse_lds_bytes(int N, int skcap) {
@@ -128,14 +330,14 @@ __host__ __device__ inline size_t sparse_lds_bytes(int N, int skcap) {
     return (size_t)N * 8 + nb * 4 + (size_t)skcap * 4;
 }
 
-// VPT > 0: after its column, each workgroup runs dense BFS tasks of the
-// columns that hold a swap endpoint as a source (candidate k, endpoint j ->
-// task 4k+j, claimed from a.ep_next; the list follows from the swaps alone),
-// with the fixed-degree dense column of `qa` in the same LDS; workgroups
-// beyond the W columns (launched on spare CUs) start on those tasks at once.
-// VPT = 0 queues those columns for k_dense_lds_queue instead.
-template <int BS, int DT, int VPT>
-__global__ void __launch_bounds__(BS) k_sparse(SparseArgs a, TopoRingQ topo, DenseArgs qa) {
+// TAIL = 1 (fixed-degree instances): after its column, each workgroup runs
+// the single-source BFS tasks of the swap endpoints (candidate k, endpoint
+// j -> task 4k+j, claimed from a.ep_next; the list follows from the swaps
+// alone) in the same LDS; workgroups beyond the W columns (launched on spare
+// CUs) start on those tasks at once.  TAIL = 0 queues the columns holding an
+// endpoint for k_dense_lds_queue instead.
+template <int BS, int DT, int TAIL>
+__global__ void __launch_bounds__(BS) k_sparse(SparseArgs a) {
     extern __shared__ unsigned long long Bprev[];
     __shared__ gg_swap s_sw[GG_SPARSE_MAX_K];     // the launch's swaps
     __shared__ int s_state[GG_SPARSE_MAX_K];      // task state of this column
@@ -153,6 +355,9 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a, TopoRingQ topo, Den
     __shared__ unsigned long long s_dc[GG_SPARSE_MAX_K];
     __shared__ unsigned long long s_lost[GG_SPARSE_MAX_K];  // bits lost on Delta_{l-1}
     __shared__ int s_ended[GG_SPARSE_MAX_K];      // task ended last level: clear its bitmaps
+    __shared__ unsigned long long s_mask[GG_SPARSE_MAX_K];  // endpoint sources' bits (masked)
+    __shared__ int s_mcnt[GG_SPARSE_MAX_K];       // base pairs of the masked bits in B_{l-1}
+    __shared__ int s_nmasked;
     __shared__ int s_next;
     __shared__ int s_any;
     const int tid = threadIdx.x, lane = tid & 63;
@@ -170,21 +375,26 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a, TopoRingQ topo, Den
     for (int k = tid; k < K; k += BS) s_sw[k] = a.swaps[a.k0 + k];
     auto gt = [&](int k) { return (size_t)(a.k0 + k) * W + w; };     // global (cand, col)
     auto lt = [&](int k) { return (size_t)k * W + w; };              // task slot of this launch
+    if (tid == 0) s_nmasked = 0;
+    __syncthreads();
     if (has_col) {
 
-    // task init: columns holding a swap endpoint as a source go dense at once
+    // task init: a column holding a swap endpoint as a source runs with the
+    // endpoint bits masked (the tail's single-source BFSes supply them), or
+    // densely in the queue kernel when there is no tail
     for (int k = tid; k < K; k += BS) {
         const gg_swap &s = a.swaps[a.k0 + k];
-        bool dense = false;
-        for (int j = 0; j < 4; j++)
-            if (s.v[j] >= 0 && (s.v[j] >> 6) == w) dense = true;
+        const unsigned long long em = endpoint_mask(s, w);
+        const bool dense = TAIL == 0 && em != 0;
+        s_mask[k] = TAIL ? em : 0ull;
+        if (TAIL && em) atomicAdd(&s_nmasked, 1);
         s_cnt[0][k] = 0;
         s_cnt[1][k] = 0;
         a.task_flags[gt(k)] = 0;
         a.task_end[gt(k)] = 0;
         s_state[k] = dense ? GG_TASK_DENSE : GG_TASK_ACTIVE;
         a.task_state[gt(k)] = s_state[k];
-        if (dense && VPT == 0) {
+        if (dense) {
             int q = atomicAdd(a.queue_count, 1);
             a.queue[q] = make_int2(a.k0 + k, w);
         }
@@ -204,11 +414,12 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a, TopoRingQ topo, Den
 
     for (int l = 1; l <= a.lmax; l++) {
         const int par = (l - 1) & 1, cur = l & 1;
+        const int lp = l - 1;
         // B_{l-1} of this column into LDS (stored levels: 16-byte loads,
         // eight per thread in flight before the LDS stores)
-        const int lp = l - 1;
         if (lp >= 1 && lp < ecc_w && (N & 1) == 0) {
-            const uint4 *src = reinterpret_cast<const uint4 *>(a.snap + ((size_t)(lp - 1) * W + w) * N);
+            const uint4 *src =
+                reinterpret_cast<const uint4 *>(a.snap + ((size_t)(lp - 1) * W + w) * N);
             uint4 *dst = reinterpret_cast<uint4 *>(Bprev);
             const int n2 = N >> 1;
 #pragma unroll 8
@@ -234,11 +445,25 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a, TopoRingQ topo, Den
             s_diff[k] = 0;
             s_dc[k] = 0;
             s_lost[k] = 0;
+            s_mcnt[k] = 0;
         }
         if (tid == 0) { s_next = 0; s_any = 0; }
         clear_ended();
         __syncthreads();
         for (int k = tid; k < K; k += BS) s_ended[k] = 0;
+        if (s_nmasked) {
+            // base pairs of the masked endpoint bits after level l-1: what the
+            // finalize replaces with the single-source counts
+            for (int k = 0; k < K; k++) {
+                const unsigned long long m = s_mask[k];
+                if (!m || !s_active[k]) continue;
+                int c = 0;
+                for (int v = tid; v < N; v += BS) c += __popcll(Bprev[v] & m);
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+                if (lane == 0 && c) atomicAdd(&s_mcnt[k], c);
+            }
+        }
         if (tid < 64) {  // prefix sums over the candidates (K <= 64), one wave
             const bool act = tid < K && s_active[tid];
             int it = act ? 4 + s_nprev[tid] * (D + 1) : 0;
@@ -286,15 +511,17 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a, TopoRingQ topo, Den
                 const int x = a.dl_key[(lt(k) * 2 + par) * a.cap + (i - s_koff[k])];
                 atomicOr(&s_ubits[x >> 5], 1u << (x & 31));
                 int xs = x;
-                if (l >= ecc_w) {
-                    // B_l is full here: a bit can be missing from V_l[v] (v not a
-                    // swap endpoint) only if every closed neighbour of v that had
-                    // it in B_{l-1} is in Delta_{l-1} and lost it.  Keys that
-                    // lost nothing spawn no items; the lost bits are collected.
+                if (l >= ecc_w - 1) {
+                    // Where B_l[v] is full, a bit can be missing from V_l[v] (v
+                    // not a swap endpoint) only if every closed neighbour of v
+                    // that had it in B_{l-1} is in Delta_{l-1} and lost it: the
+                    // lost bits are collected.  At and above the column's
+                    // eccentricity every B_l word is full, and keys that lost
+                    // nothing spawn no items; one level below most are.
                     const unsigned long long lost =
-                        Bprev[x] & ~a.dl_ovr[(lt(k) * 2 + par) * (size_t)N + x];
+                        Bprev[x] & ~a.dl_ovr[(lt(k) * 2 + par) * (size_t)N + x] & ~s_mask[k];
                     if (lost) atomicOr(&s_lost[k], lost);
-                    else xs = -1;
+                    else if (l >= ecc_w) xs = -1;
                 }
                 if (stg) s_keys[i] = xs;
             }
@@ -329,23 +556,35 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a, TopoRingQ topo, Den
                 if (u < 0) continue;  // lost no bit (levels where B_l is full)
                 v = i == 0 ? u : cand_nbr(a.adjR, D, u, i - 1, sw);
             }
+            // masked endpoint sources: their bits take no part in any word
+            // compared, counted or logged
+            const unsigned long long keep = ~s_mask[k];
             unsigned long long basew = ~0ull;  // issued before the gather
             if (l < ecc_w) basew = a.snap[((size_t)(l - 1) * W + w) * N + v];
+            basew &= keep;
             unsigned long long acc;
             const bool none = s_nprev[k] == 0;
             if constexpr (DT > 0) {
+                const bool inE = v == sw.v[0] || v == sw.v[1] || v == sw.v[2] || v == sw.v[3];
                 unsigned long long need = ~0ull;
                 if (l >= ecc_w && !none) {
                     // B_l is full here, so the item only has to show V_l[v]
                     // full.  Off the swap's endpoints only the task's lost bits
                     // can be missing (none lost: nothing to show).
-                    const bool inE = v == sw.v[0] || v == sw.v[1] || v == sw.v[2] || v == sw.v[3];
-                    if (!inE) need = s_lost[k];
+                    need = inE ? keep : s_lost[k];
                     if (!need) continue;
                 }
                 CandRow<DT> row;
                 row.load(a.adjR, v, sw);
-                if (l >= ecc_w && !none) {
+                // one level below the column's eccentricity: the same for the
+                // vertices whose B_l word is full (the base word and the row
+                // are loaded together)
+                const bool bfull = !none && (l >= ecc_w || (l == ecc_w - 1 && basew == keep));
+                if (bfull && l < ecc_w) {
+                    need = inE ? keep : s_lost[k];
+                    if (!need) continue;
+                }
+                if (bfull) {
                     // words of neighbours outside the Delta union are B_{l-1}
                     // (LDS), a lower bound of the OR; groups of 4 until the
                     // needed bits are covered.  Otherwise the exact OR below.
@@ -382,6 +621,7 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a, TopoRingQ topo, Den
                         acc |= prev_word(s_ubits, pbits, povr, Bprev, cand_nbr(a.adjR, D, v, i, sw));
                 }
             }
+            acc &= keep;
             if (acc == basew) continue;
             unsigned *cbits = a.dl_bits + (tl * 2 + cur) * NB;
             const unsigned m = 1u << (v & 31);
@@ -417,6 +657,7 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a, TopoRingQ topo, Den
             }
             const int n_cur = s_ncur[k];
             a.dcnt[g * (a.lmax + 1) + l] = (long long)s_dc[k];
+            if (s_mask[k] && l - 1 <= a.S) a.mb_cnt[g * (a.S + 1) + (l - 1)] = s_mcnt[k];
             if (a.lg_n && l <= a.S) a.lg_n[lt(k) * a.S + (l - 1)] = n_cur;
             s_cnt[cur][k] = n_cur;
             const bool fixed = (l >= ecc_w && n_cur == 0) ||
@@ -437,10 +678,15 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a, TopoRingQ topo, Den
     }
     clear_ended();
     }  // has_col
-    if constexpr (VPT > 0) {
-        // dense tail: endpoint columns, each (candidate, column) once
+    if constexpr (TAIL > 0) {
+        // tail: a single-source BFS from every endpoint of every candidate
+        // on the candidate graph (task 4k + j)
         __shared__ int s_task;
         const int ntask = 4 * K;
+        const int NB = (N + 31) >> 5;
+        // frontier lists in the LDS left after the three bitmaps
+        const int lc = (int)(((size_t)N * 8 - 12 * (size_t)((NB + 1) & ~1)) / 8);
+        unsigned *scratch = reinterpret_cast<unsigned *>(Bprev);
         for (;;) {
             __syncthreads();  // the LDS is free again
             if (tid == 0) {
@@ -448,28 +694,28 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a, TopoRingQ topo, Den
                 for (;;) {
                     t = atomicAdd(a.ep_next, 1);
                     if (t >= ntask) break;
-                    const gg_swap &sk = s_sw[t >> 2];
-                    const int j = t & 3, v = sk.v[j];
-                    if (v < 0) continue;
-                    bool dup = false;
-                    for (int jj = 0; jj < j; jj++)
-                        if (sk.v[jj] >= 0 && (sk.v[jj] >> 6) == (v >> 6)) dup = true;
-                    if (!dup) break;
+                    if (endpoint_first(s_sw[t >> 2], t & 3)) break;
                 }
                 s_task = t;
             }
             __syncthreads();
-            // wave-uniform: the task's pointers live in scalar registers
             const int t = __builtin_amdgcn_readfirstlane(s_task);
             if (t >= ntask) break;
-            const int k = t >> 2, wv = s_sw[k].v[t & 3] >> 6;
-            ColOut o;
-            o.snap = a.ep_snap ? a.ep_snap + (size_t)t * a.S * N : nullptr;
-            o.lstride = (size_t)N;
-            o.levels = a.S;
-            o.reach = a.ep_reach ? a.ep_reach + (size_t)t * (a.lmax + 1) : nullptr;
-            const ColPart p = dense_column<TopoRingQ, BS, VPT, DT>(topo, qa, a.k0 + k, wv, Bprev, o);
-            if (tid == 0) qa.parts[(size_t)(a.k0 + k) * qa.W + wv] = p;
+            const int k = t >> 2, x = s_sw[k].v[t & 3];
+            const size_t gt4 = (size_t)4 * a.k0 + t;
+            const int L1 = a.lmax + 1;
+            SwapRegs sw;
+            bool has;
+            load_swap(s_sw, k, sw, has);
+            int lim_c = 0;
+            const int ec = ss_bfs<BS, DT>(a.adjR, N, D, a.lmax, a.S, x, sw, a.undirected,
+                                          scratch, lc, a.ss_cnt + gt4 * L1,
+                                          a.ss_vis ? a.ss_vis + (size_t)t * a.S * NB : nullptr,
+                                          &lim_c);
+            if (tid == 0) {
+                a.ss_ecc[gt4] = ec;
+                a.ss_flags[gt4] = lim_c ? GG_FLAG_LIMIT : 0;
+            }
         }
     }
 }
@@ -497,15 +743,71 @@ __global__ void __launch_bounds__(BS) k_sparse_finalize(SparseArgs a, int nvalid
             continue;
         }
         const int end = a.task_end[t];
-        const ColPart &bp = a.base_parts[w];
+        const ColPart bp = a.base_parts[w];
         const long long *br = a.base_reach + (size_t)w * L1;
         const long long *dd = a.dcnt + t * L1;
+        // endpoint sources of this column (masked in the sparse pass): their
+        // single-source counts replace the base's
+        const gg_swap &sk = a.swaps[k];
+        const unsigned long long em = endpoint_mask(sk, w);
+        const long long nm = __popcll(em);
+        int top = end, sfl = 0;
+        if (em)
+            for (int j = 0; j < 4; j++) {
+                if (!endpoint_first(sk, j) || (sk.v[j] >> 6) != w) continue;
+                const int ec = a.ss_ecc[(size_t)4 * k + j];
+                top = ec > top ? ec : top;
+                sfl |= a.ss_flags[(size_t)4 * k + j];
+            }
+        if (top > a.lmax) top = a.lmax;
+        // the first kLU levels unrolled (their loads issued together, every
+        // array index a constant), later levels one by one
+        constexpr int kLU = 8;
         long long cb = (w == W - 1) ? nvalid_last : 64;  // base reached after level 0
-        long long cprev = cb, ctot = 0;
+        long long cprev = cb, ctot = 0, cx = nm, dend = 0;
         int ecc = 0;
-        for (int l = 1; l <= end; l++) {
+        long long pbr[kLU], pdd[kLU], pcx[kLU], pbx[kLU];
+#pragma unroll
+        for (int i = 0; i < kLU; i++) {
+            const int l = i + 1;
+            pbr[i] = (l <= bp.levels && l < L1) ? br[l] : 0;
+            pdd[i] = (l <= end) ? dd[l] : 0;
+            pcx[i] = 0;
+            pbx[i] = 0;
+            if (em && l <= top) {
+                for (int j = 0; j < 4; j++) {
+                    if (!endpoint_first(sk, j) || (sk.v[j] >> 6) != w) continue;
+                    const size_t t4 = (size_t)4 * k + j;
+                    if (l <= a.ss_ecc[t4]) pcx[i] += a.ss_cnt[t4 * L1 + l];
+                }
+                pbx[i] = l < bp.ecc ? a.mb_cnt[t * (a.S + 1) + l] : (long long)a.N * nm;
+            }
+            if (l == end) dend = pdd[i];
+        }
+        if (end > kLU) dend = dd[end];
+#pragma unroll
+        for (int i = 0; i < kLU; i++) {
+            const int l = i + 1;
+            if (l > top) break;
+            cb += pbr[i];
+            cx += pcx[i];
+            // after the masked pass's fixed point its count delta stays
+            const long long c = cb + (l <= end ? pdd[i] : dend) + (em ? cx - pbx[i] : 0);
+            if (c > cprev) { ecc = l; ctot += (long long)l * (c - cprev); }
+            cprev = c;
+        }
+        for (int l = kLU + 1; l <= top; l++) {
             if (l <= bp.levels) cb += br[l];
-            long long c = cb + dd[l];
+            long long bx = 0;
+            if (em) {
+                for (int j = 0; j < 4; j++) {
+                    if (!endpoint_first(sk, j) || (sk.v[j] >> 6) != w) continue;
+                    const size_t t4 = (size_t)4 * k + j;
+                    if (l <= a.ss_ecc[t4]) cx += a.ss_cnt[t4 * L1 + l];
+                }
+                bx = l < bp.ecc ? a.mb_cnt[t * (a.S + 1) + l] : (long long)a.N * nm;
+            }
+            const long long c = cb + dd[l <= end ? l : end] + (em ? cx - bx : 0);
             if (c > cprev) { ecc = l; ctot += (long long)l * (c - cprev); }
             cprev = c;
         }
@@ -513,9 +815,10 @@ __global__ void __launch_bounds__(BS) k_sparse_finalize(SparseArgs a, int nvalid
         tot += ctot;
         rch += cprev;
         diam = ecc > diam ? ecc : diam;
-        int lv = a.task_flags[t] ? end : ecc + 1;
+        const int tf = a.task_flags[t] | sfl;
+        int lv = tf ? top : ecc + 1;
         lev = lv > lev ? lv : lev;
-        fl |= a.task_flags[t];
+        fl |= tf;
     }
     tot = wave_sum64(tot);
     rch = wave_sum64(rch);
@@ -551,39 +854,28 @@ static __global__ void k_queue_reset(int *ctl, int n) {
 }
 
 // Launch of the sparse kernel of degree DT (fixed-degree instances live in
-// gg_sparse_d*.hip); vpt picks the dense tail (0 = none).
+// gg_sparse_d*.hip); tail = 1 picks the instance with the single-source BFS
+// tail (fixed degrees only).
 template <int DT>
-hipError_t launch_sparse_deg(const SparseArgs &s, const TopoRingQ &topo, const DenseArgs &qa,
-                             size_t lds, int vpt, hipStream_t st) {
+hipError_t launch_sparse_deg(const SparseArgs &s, size_t lds, int tail, hipStream_t st) {
     auto go = [&](auto fn, gg::LdsAttr &attr, int grid) -> hipError_t {
         hipError_t e = gg::ensure_dynamic_lds((const void *)fn, lds, attr);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(fn, dim3(grid), dim3(1024), lds, st, s, topo, qa);
+        hipLaunchKernelGGL(fn, dim3(grid), dim3(1024), lds, st, s);
         return hipGetLastError();
     };
-    static gg::LdsAttr c0, c4, c8, c16;
-    // dense-tail instances may take extra workgroups; one per column otherwise
-    const int g = s.grid > s.W ? s.grid : s.W;
+    static gg::LdsAttr c0, c1;
     if constexpr (DT > 0) {
-        switch (vpt) {
-            case 4: return go(&k_sparse<1024, DT, 4>, c4, g);
-            case 8: return go(&k_sparse<1024, DT, 8>, c8, g);
-            case 16: return go(&k_sparse<1024, DT, 16>, c16, g);
-            default: break;
-        }
+        // tail instances may take extra workgroups; one per column otherwise
+        if (tail) return go(&k_sparse<1024, DT, 1>, c1, s.grid > s.W ? s.grid : s.W);
     }
     return go(&k_sparse<1024, DT, 0>, c0, s.W);
 }
 
 namespace gg {
-hipError_t launch_sparse_d4(const SparseArgs &, const TopoRingQ &, const DenseArgs &, size_t, int,
-                            hipStream_t);
-hipError_t launch_sparse_d8(const SparseArgs &, const TopoRingQ &, const DenseArgs &, size_t, int,
-                            hipStream_t);
-hipError_t launch_sparse_d12(const SparseArgs &, const TopoRingQ &, const DenseArgs &, size_t, int,
-                             hipStream_t);
-hipError_t launch_sparse_d16(const SparseArgs &, const TopoRingQ &, const DenseArgs &, size_t, int,
-                             hipStream_t);
-hipError_t launch_sparse_d24(const SparseArgs &, const TopoRingQ &, const DenseArgs &, size_t, int,
-                             hipStream_t);
+hipError_t launch_sparse_d4(const SparseArgs &, size_t, int, hipStream_t);
+hipError_t launch_sparse_d8(const SparseArgs &, size_t, int, hipStream_t);
+hipError_t launch_sparse_d12(const SparseArgs &, size_t, int, hipStream_t);
+hipError_t launch_sparse_d16(const SparseArgs &, size_t, int, hipStream_t);
+hipError_t launch_sparse_d24(const SparseArgs &, size_t, int, hipStream_t);
 }  // namespace gg

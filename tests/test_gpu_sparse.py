@@ -132,3 +132,38 @@ def test_sparse_overflow_goes_through_the_dense_queue(gpu, limit):
         be.set_option(gg.GG_BATCH_OPT_DELTA_LIMIT, 0)
         rs3, rd3 = _eval_both(gg, be, [c[0] for c in cands2])
         assert [r.total_reps for r in rs3] == [r.total_reps for r in rs2]
+
+
+@pytest.mark.parametrize("n,d", [(4096, 12), (2048, 8)])
+def test_sparse_directed_candidates_match_dense(gpu, n, d):
+    """Swaps that rewrite single slots (directed candidate graphs): the
+    incremental path follows changed words through the changed vertices' own
+    rows, which is exact only on undirected graphs, so such a batch runs on
+    the dense path even where the incremental one is asked for."""
+    gg = gpu
+    rng = np.random.default_rng(n)
+    adj = gg.random_regular(n, d, 3)
+    swaps = []
+    for _ in range(24):
+        sw = gg.Swap()
+        for j in range(4):
+            sw.v[j], sw.slot[j], sw.nbr[j] = -1, 0, 0
+        for j in range(int(rng.integers(1, 3))):
+            v = int(rng.integers(0, n))
+            sw.v[j], sw.slot[j], sw.nbr[j] = v, int(rng.integers(0, d)), int(rng.integers(0, n))
+        swaps.append(sw)
+    with gg.BatchEvaluator(n, d, max_k=24) as be:
+        be.set_base(adj)
+        be.set_option(gg.GG_BATCH_OPT_PATH, gg.PATH_SPARSE)
+        rs = be.eval(swaps)
+        assert be.get_option(gg.GG_BATCH_INFO_LAST_SPARSE) == 0
+        be.set_option(gg.GG_BATCH_OPT_PATH, gg.PATH_DENSE)
+        rd = be.eval(swaps)
+        for i, (a, b) in enumerate(zip(rs, rd)):
+            assert (a.total_reps, a.diameter, a.status) == (b.total_reps, b.diameter, b.status), i
+        # one undirected swap among them: the batch still runs densely; a
+        # batch of undirected swaps alone runs incrementally again
+        cands = _random_swaps(gg, adj, 8, rng)
+        be.set_option(gg.GG_BATCH_OPT_PATH, gg.PATH_SPARSE)
+        be.eval([c[0] for c in cands])
+        assert be.get_option(gg.GG_BATCH_INFO_LAST_SPARSE) == 1
