@@ -36,6 +36,7 @@ extern "C" const char *gg_strerror(int status) {
         case GG_OOM: return "out of device memory";
         case GG_NO_DEVICE: return "no HIP device";
         case GG_UNSUPPORTED: return "unsupported shape";
+        case GG_SCRATCH_LIMIT: return "input exceeds the fixed scratch of the no-allocation evaluator";
         default: return "unknown status";
     }
 }

@@ -25,6 +25,7 @@ GG_HIP_ERROR = -2
 GG_OOM = -3
 GG_NO_DEVICE = -4
 GG_UNSUPPORTED = -5
+GG_SCRATCH_LIMIT = -6
 GG_LEVEL_LIMIT = 99
 
 GG_OPT_BLOCK = 1
@@ -162,6 +163,7 @@ def lib() -> C.CDLL:
         "gg_create_adj": (C.c_int, [C.c_int, C.c_int, C.c_int, P(vp)]),
         "gg_destroy": (None, [vp]),
         "gg_eval_part": (C.c_int, [vp, i32p, P(Result)]),
+        "gg_eval_part_noalloc": (C.c_int, [C.c_int] * 4 + [i32p, P(Result)]),
         "gg_eval_piece": (C.c_int, [vp, i32p, P(Result)]),
         "gg_eval_box": (C.c_int, [vp, i32p, P(Result)]),
         "gg_eval_adj": (C.c_int, [vp, i32p, P(Result)]),

@@ -32,6 +32,7 @@ extern "C" {
 #define GG_OOM            (-3)
 #define GG_NO_DEVICE      (-4)
 #define GG_UNSUPPORTED    (-5)
+#define GG_SCRATCH_LIMIT  (-6)   /* input larger than gg_eval_part_noalloc's fixed scratch  */
 
 /* Highest BFS level evaluated by default: the reference loops step = 1..99
  * (cudacode/cudaASPLconv.cu:208) and reports a sentinel when step 99 still
@@ -106,6 +107,18 @@ int gg_eval_box(gg_eval *ev, const int32_t *table, gg_result *out);
  * adjacency N x degree_max, entries < 0 are padding (the reference pads with
  * self-loops, cudaASPLgraph.cu:144-153).  All N vertices are sources. */
 int gg_eval_adj(gg_eval *ev, const int32_t *adj, gg_result *out);
+
+/* gg_create_part + gg_eval_part + gg_destroy as one call that allocates
+ * nothing (no hipMalloc, hipFree, pinned or library allocation): no
+ * counterpart in the reference; for callers that cannot allocate (serving
+ * loops, code that forbids device allocation).  Same table and result as
+ * gg_eval_part with the default options (level limit 99, WVC computed).  Its
+ * buffers live in one fixed 4 MiB device array per GPU, kept until the
+ * process ends: N <= 16384 and M * degree <= 524288, else GG_SCRATCH_LIMIT.
+ * Runs on the device's null stream and returns when the result is on the
+ * host.  Calls that overlap on one device share that array and race. */
+int gg_eval_part_noalloc(int N, int M, int degree, int device, const int32_t *table,
+                         gg_result *out);
 
 /* Plain single-move path on a resident adjacency (the table of the last
  * gg_eval_adj call stays on the device): evaluate base + one swap in one

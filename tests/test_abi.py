@@ -48,6 +48,13 @@ def test_bad_args_rejected_before_device(gg):
     assert L.gg_create_adj(-5, 3, 0, C.byref(h)) == gg.GG_BAD_ARG
     assert L.gg_create_piece(70000, 1, 1, 1, 2, 0, C.byref(h)) == gg.GG_UNSUPPORTED
     assert L.gg_create_part(10, 5, 3, 0, None) == gg.GG_BAD_ARG
+    r = gg.Result()
+    t = np.zeros((1, 2), dtype=np.int32)
+    tp = t.ctypes.data_as(C.POINTER(C.c_int32))
+    assert L.gg_eval_part_noalloc(16385, 1, 2, 0, tp, C.byref(r)) == gg.GG_SCRATCH_LIMIT
+    assert L.gg_eval_part_noalloc(16384, 16384, 33, 0, tp, C.byref(r)) == gg.GG_SCRATCH_LIMIT
+    assert L.gg_eval_part_noalloc(16, 32, 2, 0, tp, C.byref(r)) == gg.GG_BAD_ARG
+    assert L.gg_eval_part_noalloc(16, 1, 2, 0, tp, None) == gg.GG_BAD_ARG
 
 
 def test_no_device_is_an_error_not_a_fallback(gg):
