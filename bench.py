@@ -18,6 +18,7 @@ import ctypes as C
 import json
 import os
 import sys
+import tempfile
 import time
 
 REPO = os.path.dirname(os.path.abspath(__file__))
@@ -233,13 +234,26 @@ def batched_vs_sequential(gg, n=8192, d=16, K=256, steps=10, seed=4):
         sw, _ = gg.make_swap(adj, a, ia, c, ic, int(rng.integers(0, 2)))
         if sw is not None:
             cands.append(sw)
-    with gg.BatchEvaluator(n, d, max_k=K) as be:
-        be.set_base(adj)
-        rb = be.eval(cands)  # warm-up (captures the graph)
+    def timed(be):
+        r = be.eval(cands)  # warm-up (captures the graph)
         t0 = time.perf_counter()
         for _ in range(steps):
-            rb = be.eval(cands)
-        tb = (time.perf_counter() - t0) / steps
+            r = be.eval(cands)
+        return r, (time.perf_counter() - t0) / steps
+
+    with gg.BatchEvaluator(n, d, max_k=K) as be:
+        be.set_base(adj)
+        # the automatic launch shape (all K in one launch), then the tuned one
+        for opt in (gg.GG_BATCH_OPT_LAUNCH_K, gg.GG_BATCH_OPT_GROUP_K,
+                    gg.GG_BATCH_OPT_SPARSE_BLOCK, gg.GG_BATCH_OPT_STAGED_KEYS):
+            be.set_option(opt, 0)
+        rd, td = timed(be)
+        default_shape = be.shape()
+        rep = be.autotune(cands, reps=3)
+        rb, tb = timed(be)
+        tuned_shape = be.shape()
+    assert [(a.total_reps, a.diameter) for a in rd] == [(a.total_reps, a.diameter) for a in rb]
+    tune = rep.as_dict()
     ev = gg.SwapEvaluator(n, d)
     ev.set_base(adj)
     ev.eval_swap(cands[0])
@@ -250,7 +264,36 @@ def batched_vs_sequential(gg, n=8192, d=16, K=256, steps=10, seed=4):
     assert all(a.total_reps == b.total_reps and a.diameter == b.diameter for a, b in zip(rb, rs))
     return {"workload": f"K={K} 2-opt candidates, random {d}-regular graph n={n}",
             "batched_ms_per_step": tb * 1e3, "sequential_ms": ts * 1e3,
-            "speedup": ts / tb, "batched_evals_per_s": K / tb, "sequential_evals_per_s": K / ts}
+            "speedup": ts / tb, "batched_evals_per_s": K / tb, "sequential_evals_per_s": K / ts,
+            "launches_per_step": -(-K // tuned_shape["launch_k"]),
+            "default_shape": default_shape, "default_ms_per_step": td * 1e3,
+            "tuned_shape": tuned_shape,
+            "autotune": {"configs": len(tune["configs"]), "mismatches": tune["mismatches"],
+                         "best": tune["best"], "default_ms": tune["default_ms"],
+                         "base": tune["base"], "best_base": tune["best_base"]}}
+
+
+def tune_step_shape(gg, adj, K, seed=11):
+    """Autotunes the incremental step's launch shape for (n, d, K) on K random
+    2-opt candidates of the search's start graph; the cached choice applies
+    to every batch evaluator of that shape created afterwards (the learned
+    search's included)."""
+    n, d = adj.shape
+    rng = np.random.default_rng(seed)
+    cands = []
+    while len(cands) < K:
+        a, c = (int(x) for x in rng.integers(0, n, 2))
+        ia, ic = (int(x) for x in rng.integers(0, d, 2))
+        sw, _ = gg.make_swap(adj, a, ia, c, ic, int(rng.integers(0, 2)))
+        if sw is not None:
+            cands.append(sw)
+    with gg.BatchEvaluator(n, d, max_k=K) as be:
+        be.set_base(adj)
+        rep = be.autotune(cands, reps=3).as_dict()
+        shape = be.shape()
+    return {"shape": shape, "configs": len(rep["configs"]), "mismatches": rep["mismatches"],
+            "best_ms": rep["best"]["ms"], "default_ms": rep["default_ms"],
+            "best_base": rep["best_base"]}
 
 
 def run_search(gg, adj, cfg, warmup, steps):
@@ -331,6 +374,8 @@ def main():
     ap.add_argument("--seed", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the automatic launch shape of the batched step")
     ap.add_argument("--only", choices=["roofline", "copy", "learned", "plain", "large"],
                     default=None,
                     help="run only the dense-kernel roofline launches, only a device copy of "
@@ -344,6 +389,9 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU: run with --gpus 1"}))
         sys.exit(2)
+    # launch shapes the autotuner picks are cached per (device, n, d, K)
+    os.environ.setdefault("GG_TUNE_CACHE",
+                          os.path.join(tempfile.gettempdir(), "gg_mi355x_tune_bench.txt"))
     import graphgolf_mi355x as gg
     assert gg.device_count() >= 1, "no HIP device"
     n, d = args.n, args.d
@@ -366,10 +414,11 @@ def main():
     learn_cfg = gg.search_cfg(mode=gg.MODE_LEARNED, K=K, P=args.P,
                               total_steps=learn_warm + args.steps, T0=T0, T1=T1, seed=12345,
                               warmup_pairs=2048, train_every=1, train_batch=256, lr=3e-3)
+    tune = None if args.no_tune else tune_step_shape(gg, adj, K)
     if args.only == "learned":
         sl, _, _, el, tl = run_search(gg, adj, learn_cfg, learn_warm, args.steps)
-        print(json.dumps({"learned_evals_per_s": el / tl, "ms_per_step": tl / args.steps * 1e3}),
-              flush=True)
+        print(json.dumps({"learned_evals_per_s": el / tl, "ms_per_step": tl / args.steps * 1e3,
+                          "step_shape": tune}), flush=True)
         return
 
     plain_cfg = gg.search_cfg(mode=gg.MODE_PLAIN, total_steps=(args.warmup + args.steps) * K,
@@ -404,6 +453,7 @@ def main():
         "config": {"workload": f"annealing over 2-opt swaps, random {d}-regular graph n={n}",
                    "n": n, "d": d, "K_exact_per_step": K, "P_proposals_per_step": args.P},
         "learned_warmup_steps": learn_warm,
+        "step_shape_autotune": tune,
         "plain": {"value": plain_rate, "unit": "exact evaluations/s",
                   "ms_per_step": tp / args.steps * 1e3},
         "speedup_vs_plain": learn_rate / plain_rate,

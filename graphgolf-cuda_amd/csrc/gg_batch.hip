@@ -8,7 +8,9 @@
 //           (gg_sparse.hpp) with a dense fallback queue, captured as one HIP
 //           graph: H2D swaps -> k_sparse -> k_dense_lds_queue ->
 //           k_sparse_finalize -> queue reset -> D2H results.
+#include <algorithm>
 #include <cstring>
+#include <sstream>
 #include <vector>
 #include "gg_common.hpp"
 #include "gg_dense.hpp"
@@ -22,7 +24,7 @@ using namespace gg;
 static const int kSparseBlock = 1024;
 static const int kSnapCapDefault = 8;
 static const size_t kSnapBytesMax = (size_t)8 << 30;
-static const size_t kOverrideBytesMax = (size_t)8 << 30;
+static const size_t kOverrideBytesMax = (size_t)16 << 30;
 static const int kDeltaCap = 2048;
 static const int kRowLevels = 3;  // levels kept row-major (the features use 1..3)
 
@@ -88,16 +90,21 @@ static int batch_alloc(gg_batch *b) {
         while (cap > 1 && per_level * cap > kSnapBytesMax) cap--;
         b->snap_cap = cap;
         b->cap = kDeltaCap < N ? kDeltaCap : N;
-        b->skcap = sparse_staged_keys(b);
+        b->skcap_max = sparse_staged_keys(b);
+        b->skcap = b->skcap_max;
         const int NB = (N + 31) / 32;
         // per task: Delta state of two parities and the commit logs of `cap`
         // levels; per candidate: four endpoint columns' words per level
         const size_t per_task = 2 * ((size_t)N * 8 + (size_t)NB * 4 + (size_t)b->cap * 4) +
                                 (size_t)cap * ((size_t)b->cap * 12 + 4);
         const size_t per_cand = per_task * W + 4 * ((size_t)cap * N * 8 + (size_t)L * 8);
-        int chunk = K < GG_SPARSE_MAX_K ? K : GG_SPARSE_MAX_K;
-        while (chunk > 1 && per_cand * chunk > kOverrideBytesMax) chunk--;
+        // every candidate in one launch unless the per-launch state would
+        // pass kOverrideBytesMax
+        size_t fit = kOverrideBytesMax / per_cand;
+        int chunk = fit < 1 ? 1 : (fit < (size_t)K ? (int)fit : K);
+        b->chunk_max = chunk;
         b->chunk = chunk;
+        b->kg = chunk < GG_SPARSE_MAX_K ? chunk : GG_SPARSE_MAX_K;
         const size_t T = (size_t)K * W, TC = (size_t)chunk * W;
         if (hipMalloc((void **)&b->d_snap, per_level * cap) != hipSuccess) return GG_OOM;
         b->rows_levels = cap < kRowLevels ? cap : kRowLevels;
@@ -112,7 +119,7 @@ static int batch_alloc(gg_batch *b) {
         if (hipMalloc((void **)&b->d_task_ecc, sizeof(int) * T) != hipSuccess) return GG_OOM;
         if (hipMalloc((void **)&b->d_dcnt, sizeof(long long) * T * L) != hipSuccess) return GG_OOM;
         if (hipMalloc((void **)&b->d_queue, sizeof(int2) * T) != hipSuccess) return GG_OOM;
-        b->nqctl = 2 + (K + chunk - 1) / chunk;  // queue count, queue next, endpoint counters
+        b->nqctl = 2 + K;  // queue count, queue next, endpoint counters (one per launch)
         if (hipMalloc((void **)&b->d_qctl, sizeof(int) * b->nqctl) != hipSuccess) return GG_OOM;
         if (hipMalloc((void **)&b->d_dense_parts, sizeof(ColPart) * T) != hipSuccess)
             return GG_OOM;
@@ -144,6 +151,8 @@ static int batch_alloc(gg_batch *b) {
     return GG_OK;
 }
 
+static void batch_tune_lookup(gg_batch *b);
+
 extern "C" int gg_batch_create(int N, int degree, int max_k, int device, gg_batch **out) {
     if (!out) return GG_BAD_ARG;
     *out = nullptr;
@@ -155,6 +164,7 @@ extern "C" int gg_batch_create(int N, int degree, int max_k, int device, gg_batc
     b->N = N; b->degree = degree; b->max_k = max_k; b->device = device;
     int st = batch_alloc(b);
     if (st != GG_OK) { gg_batch_destroy(b); return st; }
+    if (b->chunk_max > 0) batch_tune_lookup(b);
     *out = b;
     return GG_OK;
 }
@@ -312,6 +322,7 @@ static int check_swaps(const gg_batch *b, const gg_swap *sw, int K) {
 // Sparse kernel for the degree (fixed-degree instances with a dense tail
 // when vpt > 0, the runtime-degree one without).
 static hipError_t launch_sparse(const SparseArgs &s, size_t lds, int tail, hipStream_t st) {
+    // block = 512 exists for the tail instances only
     switch (s.D) {
         case 4: return launch_sparse_d4(s, lds, tail, st);
         case 8: return launch_sparse_d8(s, lds, tail, st);
@@ -378,13 +389,16 @@ static int enqueue_sparse(gg_batch *b, int K, bool reset = true) {
         SparseArgs c = s;
         c.k0 = k0;
         c.K = K - k0 < b->chunk ? K - k0 : b->chunk;
+        c.kg = b->kg < c.K ? b->kg : c.K;
+        c.block = tail ? b->sparse_block : 1024;
         c.ep_next = b->d_qctl + 2 + c_i;
-        // claiming grid: a workgroup per column, more (one per CU) while there
-        // are CUs left for the endpoint tasks
-        c.grid = W;
+        // claiming grid: a workgroup per (group, column), more (one per CU)
+        // while there are CUs left for the endpoint tasks
+        const int cols = (c.K + c.kg - 1) / c.kg * W;
+        c.grid = cols;
         if (tail) {
-            const int want = W + 4 * c.K;
-            c.grid = want < b->dc.cu_count ? want : (W > b->dc.cu_count ? W : b->dc.cu_count);
+            const int want = cols + 4 * c.K;
+            c.grid = want < b->dc.cu_count ? want : (cols > b->dc.cu_count ? cols : b->dc.cu_count);
         }
         hipError_t e = launch_sparse(c, lds, tail, b->stream);
         if (e != hipSuccess) {
@@ -1026,6 +1040,37 @@ extern "C" int gg_batch_set_option(gg_batch *b, int option, int value) {
             b->dlimit_opt = value;
             drop_graph(b);  // the captured step holds the old limit
             return GG_OK;
+        // launch shape: the captured step holds the old one
+        case GG_BATCH_OPT_LAUNCH_K:
+            if (value < 0 || value > b->chunk_max) return GG_BAD_ARG;
+            b->chunk = value ? value : b->chunk_max;
+            drop_graph(b);
+            return GG_OK;
+        case GG_BATCH_OPT_GROUP_K:
+            if (value < 0 || value > GG_SPARSE_MAX_K) return GG_BAD_ARG;
+            b->kg = value ? value : (b->chunk_max < GG_SPARSE_MAX_K ? b->chunk_max : GG_SPARSE_MAX_K);
+            drop_graph(b);
+            return GG_OK;
+        case GG_BATCH_OPT_SPARSE_BLOCK:
+            if (value != 0 && value != 512 && value != 1024) return GG_BAD_ARG;
+            b->sparse_block = value ? value : 1024;
+            drop_graph(b);
+            return GG_OK;
+        case GG_BATCH_OPT_STAGED_KEYS:
+            if (value < 0 || value > b->skcap_max) return GG_BAD_ARG;
+            b->skcap = value ? value : b->skcap_max;
+            drop_graph(b);
+            return GG_OK;
+        case GG_BATCH_OPT_BASE_BLOCK:
+            if (value != 0 && value != 256 && value != 1024) return GG_BAD_ARG;
+            b->dc.block_opt = value;
+            drop_graph(b);  // the dense fallback queue shares the setting
+            return GG_OK;
+        case GG_BATCH_OPT_BASE_PUSH:
+            if (value < 0 || value > 1) return GG_BAD_ARG;
+            b->dc.push_off = value ? 0 : 1;
+            drop_graph(b);
+            return GG_OK;
         default: return GG_BAD_ARG;
     }
 }
@@ -1040,6 +1085,200 @@ extern "C" int gg_batch_get_option(gg_batch *b, int option, int *value) {
         case GG_BATCH_OPT_COMMIT: *value = b->commit_mode; return GG_OK;
         case GG_BATCH_OPT_DELTA_LIMIT: *value = b->dlimit_opt; return GG_OK;
         case GG_BATCH_INFO_LAST_COMMIT_INPLACE: *value = b->last_commit_inplace; return GG_OK;
+        case GG_BATCH_OPT_LAUNCH_K: *value = b->chunk; return GG_OK;
+        case GG_BATCH_OPT_GROUP_K: *value = b->kg; return GG_OK;
+        case GG_BATCH_OPT_SPARSE_BLOCK: *value = b->sparse_block; return GG_OK;
+        case GG_BATCH_OPT_STAGED_KEYS: *value = b->skcap; return GG_OK;
+        case GG_BATCH_OPT_BASE_BLOCK: *value = b->dc.block_opt; return GG_OK;
+        case GG_BATCH_OPT_BASE_PUSH: *value = b->dc.push_off ? 0 : 1; return GG_OK;
+        case GG_BATCH_INFO_LAUNCH_MAX: *value = b->chunk_max; return GG_OK;
+        case GG_BATCH_INFO_STAGED_MAX: *value = b->skcap_max; return GG_OK;
         default: return GG_BAD_ARG;
     }
+}
+
+// ---- autotuner ------------------------------------------------------------------------
+static std::string batch_tune_key(const gg_batch *b) {
+    std::ostringstream k;
+    k << "batch " << device_arch(b->device) << ' ' << b->N << ' ' << b->degree << ' ' << b->max_k;
+    return k.str();
+}
+
+// Applies a cached launch shape (silently ignored when it does not fit these
+// buffers).
+static void batch_tune_lookup(gg_batch *b) {
+    std::istringstream in(tune_cache_get(batch_tune_key(b)));
+    int lk, gk, blk, sk, bb, bp;
+    if (!(in >> lk >> gk >> blk >> sk >> bb >> bp)) return;
+    if (lk < 1 || lk > b->chunk_max || gk < 1 || gk > GG_SPARSE_MAX_K ||
+        (blk != 512 && blk != 1024) || sk < 0 || sk > b->skcap_max ||
+        (bb != 0 && bb != 256 && bb != 1024) || (bp != 0 && bp != 1))
+        return;
+    b->chunk = lk;
+    b->kg = gk;
+    b->sparse_block = blk;
+    b->skcap = sk;
+    b->dc.block_opt = bb;
+    b->dc.push_off = bp ? 0 : 1;
+}
+
+static bool same_results(const std::vector<gg_result> &x, const std::vector<gg_result> &y) {
+    if (x.size() != y.size()) return false;
+    for (size_t i = 0; i < x.size(); i++)
+        if (x[i].total_reps != y[i].total_reps || x[i].diameter != y[i].diameter ||
+            x[i].levels != y[i].levels || x[i].status != y[i].status)
+            return false;
+    return true;
+}
+
+extern "C" int gg_batch_autotune(gg_batch *b, const gg_swap *swaps, int K, int reps,
+                                 gg_batch_tune_report *report) {
+    if (!b || !swaps || K < 1 || K > b->max_k || reps < 1 || !b->have_base) return GG_BAD_ARG;
+    GG_HIP(hipSetDevice(b->device));
+    if (!b->sparse_ok || !b->base_sym || b->mode_opt == 1) return GG_UNSUPPORTED;
+    gg_batch_tune_report rep{};
+    const int saved[6] = {b->chunk, b->kg, b->sparse_block, b->skcap, b->dc.block_opt,
+                          b->dc.push_off};
+    auto restore = [&]() {
+        b->chunk = saved[0]; b->kg = saved[1]; b->sparse_block = saved[2]; b->skcap = saved[3];
+        b->dc.block_opt = saved[4]; b->dc.push_off = saved[5];
+        drop_graph(b);
+    };
+    std::vector<gg_result> ref(K), res(K);
+    // one configuration: a first (capturing) step, then `reps` timed ones
+    auto time_step = [&](float *med) -> int {
+        drop_graph(b);
+        std::vector<float> t;
+        for (int i = 0; i <= reps; i++) {
+            int st = gg_batch_eval(b, swaps, K, res.data());
+            if (st != GG_OK) return st;
+            float ms = 0;
+            GG_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+            if (i > 0) t.push_back(ms);
+        }
+        std::sort(t.begin(), t.end());
+        *med = t[t.size() / 2];
+        return GG_OK;
+    };
+
+    // the automatic shape first: its results are the reference
+    const int kt = K < b->chunk_max ? K : b->chunk_max;
+    b->chunk = b->chunk_max;
+    b->kg = b->chunk_max < GG_SPARSE_MAX_K ? b->chunk_max : GG_SPARSE_MAX_K;
+    b->sparse_block = 1024;
+    b->skcap = b->skcap_max;
+    int st = time_step(&rep.default_ms);
+    if (st != GG_OK) { restore(); return st; }
+    ref = res;
+    if (!b->last_sparse) { restore(); return GG_UNSUPPORTED; }
+
+    std::vector<int> launches;
+    for (int d : {1, 2, 4}) {
+        const int l = (kt + d - 1) / d;
+        if (std::find(launches.begin(), launches.end(), l) == launches.end()) launches.push_back(l);
+    }
+    const bool tail = fixed_sparse_degree(b->degree) && !b->dc.generic_opt;
+    std::vector<int> blocks = tail ? std::vector<int>{1024, 512} : std::vector<int>{1024};
+    std::vector<int> staged{b->skcap_max};
+    {
+        // keys that leave room for two workgroups per CU
+        long long two = ((long long)b->dc.lds_device / 2 - kSparseStaticLds -
+                         (long long)sparse_lds_bytes(b->N, 0)) / 4;
+        if (two >= 256 && two < b->skcap_max) staged.push_back((int)two);
+    }
+    float best = rep.default_ms;
+    int best_cfg[4] = {b->chunk_max, b->kg, 1024, b->skcap_max};
+    for (int l : launches)
+        for (int g : {8, 16, 32, 64}) {
+            if (g > l && g != 8) continue;
+            for (int blk : blocks)
+                for (int sk : staged) {
+                    if (rep.count >= GG_BATCH_TUNE_MAX) break;
+                    b->chunk = l;
+                    b->kg = g < l ? g : l;
+                    b->sparse_block = blk;
+                    b->skcap = sk;
+                    float ms = 0;
+                    st = time_step(&ms);
+                    if (st != GG_OK) { restore(); return st; }
+                    const int i = rep.count++;
+                    rep.launch_k[i] = l;
+                    rep.group_k[i] = b->kg;
+                    rep.block[i] = blk;
+                    rep.staged_keys[i] = sk;
+                    rep.ms[i] = ms;
+                    if (!same_results(res, ref) || !b->last_sparse) {
+                        rep.mismatches++;
+                        continue;
+                    }
+                    if (ms < best) {
+                        best = ms;
+                        best_cfg[0] = l; best_cfg[1] = b->kg; best_cfg[2] = blk; best_cfg[3] = sk;
+                    }
+                }
+        }
+    b->chunk = best_cfg[0]; b->kg = best_cfg[1]; b->sparse_block = best_cfg[2];
+    b->skcap = best_cfg[3];
+    rep.best_launch_k = best_cfg[0]; rep.best_group_k = best_cfg[1];
+    rep.best_block = best_cfg[2]; rep.best_staged_keys = best_cfg[3];
+    rep.best_ms = best;
+    drop_graph(b);
+
+    // dense base BFS: block size (the vertices per thread follow) x push
+    // levels, timed from the BFS kernel's start to its end, levels compared
+    uint64_t ref_digest = 0;
+    ResultRec ref_base = b->base_res;
+    st = gg_batch_levels_digest(b, &ref_digest);
+    if (st != GG_OK) { restore(); return st; }
+    float best_base = 1e30f;
+    int best_bb = b->dc.block_opt, best_bp = b->dc.push_off ? 0 : 1;
+    for (int bb : {1024, 256})
+        for (int bp : {1, 0}) {
+            if (bp && !b->base_sym) continue;
+            b->dc.block_opt = bb;
+            b->dc.push_off = bp ? 0 : 1;
+            std::vector<float> t;
+            bool same = true;
+            for (int i = 0; i <= reps; i++) {
+                GG_HIP(hipEventRecord(b->ev0, b->stream));
+                b->dc.bfs_done = b->ev1;
+                st = eval_base(b, nullptr);
+                b->dc.bfs_done = nullptr;
+                if (st != GG_OK) { restore(); return st; }
+                float ms = 0;
+                GG_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+                if (i > 0) t.push_back(ms);
+                const ResultRec &r = b->base_res;
+                same = same && r.total == ref_base.total && r.diameter == ref_base.diameter &&
+                       r.levels == ref_base.levels && r.reached == ref_base.reached;
+            }
+            uint64_t dg = 0;
+            st = gg_batch_levels_digest(b, &dg);
+            if (st != GG_OK) { restore(); return st; }
+            std::sort(t.begin(), t.end());
+            const float med = t[t.size() / 2];
+            if (rep.base_count < 4) {
+                rep.base_block[rep.base_count] = bb;
+                rep.base_push[rep.base_count] = bp;
+                rep.base_ms[rep.base_count] = med;
+                rep.base_count++;
+            }
+            if (!same || dg != ref_digest) { rep.mismatches++; continue; }
+            if (med < best_base) { best_base = med; best_bb = bb; best_bp = bp; }
+        }
+    b->dc.block_opt = best_bb;
+    b->dc.push_off = best_bp ? 0 : 1;
+    rep.best_base_block = best_bb;
+    rep.best_base_push = best_bp;
+    rep.best_base_ms = best_base;
+    // leave the stored levels as the chosen base configuration computes them
+    st = eval_base(b, nullptr);
+    if (st != GG_OK) { restore(); return st; }
+    drop_graph(b);
+    std::ostringstream v;
+    v << b->chunk << ' ' << b->kg << ' ' << b->sparse_block << ' ' << b->skcap << ' '
+      << b->dc.block_opt << ' ' << best_bp << ' ' << best << ' ' << best_base;
+    tune_cache_put(batch_tune_key(b), v.str());
+    if (report) *report = rep;
+    return GG_OK;
 }

@@ -28,28 +28,32 @@ struct gg_batch {
     int cap = 0;                              // Delta list capacity per task
     int skcap = 0;                            // Delta keys staged in LDS per column and level
     int nqctl = 0;                            // d_qctl words: queue count/next + endpoint counters
-    int chunk = 0;                            // candidates per sparse launch
+    int chunk_max = 0;                        // candidates per sparse launch the buffers hold
+    int chunk = 0;                            // candidates per sparse launch (<= chunk_max)
+    int kg = 0;                               // candidates per column workgroup
+    int sparse_block = 1024;                  // threads per sparse workgroup (512 / 1024)
+    int skcap_max = 0;                        // staged keys the LDS left over allows
     int *d_dl_key = nullptr;
-    unsigned *d_dl_bits = nullptr;            // [chunk*W][2][ceil(N/32)]
-    unsigned long long *d_dl_ovr = nullptr;   // [chunk*W][2][N]
+    unsigned *d_dl_bits = nullptr;            // [chunk_max*W][2][ceil(N/32)]
+    unsigned long long *d_dl_ovr = nullptr;   // [chunk_max*W][2][N]
     int *d_task_state = nullptr, *d_task_end = nullptr,
         *d_task_flags = nullptr, *d_task_ecc = nullptr;
     long long *d_dcnt = nullptr;
     int2 *d_queue = nullptr;
-    int *d_qctl = nullptr;                    // [2] queue count, queue next
+    int *d_qctl = nullptr;                    // queue count, queue next, endpoint counters
     ColPart *d_dense_parts = nullptr;         // [max_k * W]
     // kept by the sparse launch for an in-place commit (null: not kept)
-    int *d_lg_key = nullptr;                  // [chunk*W][snap_cap][cap]
-    unsigned long long *d_lg_word = nullptr;  // [chunk*W][snap_cap][cap]
-    int *d_lg_n = nullptr;                    // [chunk*W][snap_cap]
-    unsigned long long *d_ep_snap = nullptr;  // [4*chunk][snap_cap][N]
-    long long *d_ep_reach = nullptr;          // [4*chunk][LEVEL_LIMIT+1]
+    int *d_lg_key = nullptr;                  // [chunk_max*W][snap_cap][cap]
+    unsigned long long *d_lg_word = nullptr;  // [chunk_max*W][snap_cap][cap]
+    int *d_lg_n = nullptr;                    // [chunk_max*W][snap_cap]
+    unsigned long long *d_ep_snap = nullptr;  // [4*chunk_max][snap_cap][N]
+    long long *d_ep_reach = nullptr;          // [4*chunk_max][LEVEL_LIMIT+1]
     // single-source BFSes of the swap endpoints (sparse kernels with a tail)
     int *d_ss_cnt = nullptr;                  // [4*max_k][LEVEL_LIMIT+1]
     int *d_ss_ecc = nullptr;                  // [4*max_k]
     int *d_mb_cnt = nullptr;                  // [max_k][W][snap_cap+1]
     int *d_ss_flags = nullptr;                // [4*max_k]
-    unsigned *d_ss_vis = nullptr;             // [4*chunk][snap_cap][ceil(N/32)] (commit)
+    unsigned *d_ss_vis = nullptr;             // [4*chunk_max][snap_cap][ceil(N/32)] (commit)
     int log_valid = 0;                        // the last eval was sparse with logs kept
     int log_k0 = 0, log_K = 0;                // candidates the logs cover: [log_k0, log_K)
     int ecc_stale = 0;                        // d_ecc predates an in-place commit

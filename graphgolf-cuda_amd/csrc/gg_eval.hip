@@ -1370,37 +1370,28 @@ static std::string tune_cache_path() {
     return std::string(h ? h : "/tmp") + "/.cache/gg_mi355x_tune.txt";
 }
 
-static std::string tune_key(const gg_eval *ev) {
+namespace gg {
+std::string device_arch(int device) {
     hipDeviceProp_t prop;
-    std::string arch = "unknown";
-    if (hipGetDeviceProperties(&prop, ev->device) == hipSuccess) arch = prop.gcnArchName;
-    std::ostringstream k;
-    k << arch << ' ' << (int)ev->kind << ' ' << ev->N << ' ' << ev->M << ' ' << ev->degree;
-    return k.str();
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) return prop.gcnArchName;
+    return "unknown";
 }
 
-static void tune_lookup(gg_eval *ev) {
+// The values stored under `key` in the tuning cache ("" if none).
+std::string tune_cache_get(const std::string &key) {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     std::ifstream f(tune_cache_path());
-    if (!f) return;
-    const std::string key = tune_key(ev);
-    std::string line;
-    while (std::getline(f, line)) {
-        if (line.compare(0, key.size(), key) != 0 || line.size() <= key.size() ||
-            line[key.size()] != ' ')
-            continue;
-        std::istringstream in(line.substr(key.size()));
-        int block = 0, generic = 0;
-        if (in >> block >> generic) {
-            ev->dc.block_opt = block;
-            ev->dc.generic_opt = generic;
-        }
-    }
+    std::string line, found;
+    while (f && std::getline(f, line))
+        if (line.size() > key.size() && line.compare(0, key.size() + 1, key + " ") == 0)
+            found = line.substr(key.size() + 1);
+    return found;
 }
 
-static void tune_store(gg_eval *ev, int block, int generic, float ms) {
+// Replaces the line of `key` in the tuning cache.
+void tune_cache_put(const std::string &key, const std::string &values) {
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    const std::string path = tune_cache_path(), key = tune_key(ev);
+    const std::string path = tune_cache_path();
     std::vector<std::string> keep;
     {
         std::ifstream f(path);
@@ -1416,7 +1407,30 @@ static void tune_store(gg_eval *ev, int block, int generic, float ms) {
     }
     std::ofstream f(path, std::ios::trunc);
     for (auto &l : keep) f << l << '\n';
-    f << key << ' ' << block << ' ' << generic << ' ' << ms << '\n';
+    f << key << ' ' << values << '\n';
+}
+}  // namespace gg
+
+static std::string tune_key(const gg_eval *ev) {
+    std::ostringstream k;
+    k << device_arch(ev->device) << ' ' << (int)ev->kind << ' ' << ev->N << ' ' << ev->M << ' '
+      << ev->degree;
+    return k.str();
+}
+
+static void tune_lookup(gg_eval *ev) {
+    std::istringstream in(tune_cache_get(tune_key(ev)));
+    int block = 0, generic = 0;
+    if (in >> block >> generic) {
+        ev->dc.block_opt = block;
+        ev->dc.generic_opt = generic;
+    }
+}
+
+static void tune_store(gg_eval *ev, int block, int generic, float ms) {
+    std::ostringstream v;
+    v << block << ' ' << generic << ' ' << ms;
+    tune_cache_put(tune_key(ev), v.str());
 }
 
 extern "C" int gg_eval_autotune(gg_eval *ev, const int32_t *table, int reps,

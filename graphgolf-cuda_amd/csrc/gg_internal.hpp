@@ -1,6 +1,7 @@
 // gg_internal.hpp -- host-side types shared by the evaluator translation units.
 #pragma once
 #include "gg_common.hpp"
+#include <string>
 #include "gg_dense.hpp"
 
 namespace gg {
@@ -50,6 +51,12 @@ int push_capacity(const DenseCtx &c, int N, int D, int lds_cap);
 // Undirected-graph checks that gate the push levels (host side).
 bool adj_symmetric(const int32_t *adj, int N, int D);
 bool swap_keeps_symmetry(const int32_t *adj, int D, const gg_swap &sw);
+
+// Tuning cache (GG_TUNE_CACHE, default $HOME/.cache/gg_mi355x_tune.txt): one
+// line per key, "key values".
+std::string device_arch(int device);
+std::string tune_cache_get(const std::string &key);
+void tune_cache_put(const std::string &key, const std::string &values);
 
 // Quad-packed table size / index helpers (TopoRingQ).
 inline int quads(int D) { return (D + 3) / 4; }

@@ -52,12 +52,15 @@
 #define GG_TASK_ACTIVE 0
 #define GG_TASK_DONE 1
 #define GG_TASK_DENSE 2
-#define GG_SPARSE_MAX_K 64  // candidates per sparse launch (LDS bookkeeping)
+#define GG_SPARSE_MAX_K 64  // candidates per column workgroup (LDS bookkeeping)
 
 struct SparseArgs {
     int N, D, W, K, lmax;
     int k0;                          // first candidate of this launch (swaps/results offset)
-    int grid;                        // workgroups of the dense-tail instances (>= W)
+    int kg;                          // candidates per column workgroup (<= GG_SPARSE_MAX_K):
+                                     // ceil(K / kg) groups of W column workgroups
+    int grid;                        // workgroups of the dense-tail instances (>= groups * W)
+    int block;                       // threads per workgroup of the tail instances (512, 1024)
     const int *adjR;                 // [N][D] base neighbours, row-major
     const gg_swap *swaps;            // [k0 + K]
     const unsigned long long *snap;  // [S][W][N] base words after levels 1..S
@@ -339,7 +342,7 @@ __host__ __device__ inline size_t sparse_lds_bytes(int N, int skcap) {
 template <int BS, int DT, int TAIL>
 __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a) {
     extern __shared__ unsigned long long Bprev[];
-    __shared__ gg_swap s_sw[GG_SPARSE_MAX_K];     // the launch's swaps
+    __shared__ gg_swap s_sw[GG_SPARSE_MAX_K];     // the group's swaps
     __shared__ int s_state[GG_SPARSE_MAX_K];      // task state of this column
     __shared__ int s_cnt[2][GG_SPARSE_MAX_K];     // |Delta| per parity
     __shared__ int s_koff[GG_SPARSE_MAX_K + 1];   // staged Delta keys: offset per candidate
@@ -361,20 +364,27 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a) {
     __shared__ int s_next;
     __shared__ int s_any;
     const int tid = threadIdx.x, lane = tid & 63;
-    const int N = a.N, D = a.D, W = a.W, K = a.K;
-    // workgroups beyond one per column (dense-tail instances launched with
-    // spare CUs) go straight to the endpoint tasks
-    const bool has_col = (int)blockIdx.x < W;
-    const int w = has_col ? blockIdx.x : 0;
+    const int N = a.N, D = a.D, W = a.W;
+    // workgroup (group gi, column w) runs the group's candidates kb..kb+K-1
+    // of the launch on column w; workgroups beyond the groups' columns
+    // (dense-tail instances launched with spare CUs) go straight to the
+    // endpoint tasks
+    const int groups = (a.K + a.kg - 1) / a.kg;
+    const bool has_col = (int)blockIdx.x < groups * W;
+    const int gi = has_col ? (int)blockIdx.x / W : 0;
+    const int w = has_col ? (int)blockIdx.x - gi * W : 0;
+    const int kb = gi * a.kg;                                     // launch-local
+    const int K = has_col ? (a.K - kb < a.kg ? a.K - kb : a.kg) : 0;
+    const int kc = a.k0 + kb;                                     // global
     const int NB = (N + 31) >> 5;
     const int ecc_w = a.base_parts[w].ecc;
     const unsigned long long pad = pad_mask(w, N);
     const int NBE = (NB + 1) & ~1;
     unsigned *s_ubits = reinterpret_cast<unsigned *>(Bprev + N);  // [NBE]
     int *s_keys = reinterpret_cast<int *>(s_ubits + NBE);          // [skcap]
-    for (int k = tid; k < K; k += BS) s_sw[k] = a.swaps[a.k0 + k];
-    auto gt = [&](int k) { return (size_t)(a.k0 + k) * W + w; };     // global (cand, col)
-    auto lt = [&](int k) { return (size_t)k * W + w; };              // task slot of this launch
+    for (int k = tid; k < K; k += BS) s_sw[k] = a.swaps[kc + k];
+    auto gt = [&](int k) { return (size_t)(kc + k) * W + w; };       // global (cand, col)
+    auto lt = [&](int k) { return (size_t)(kb + k) * W + w; };       // task slot of this launch
     if (tid == 0) s_nmasked = 0;
     __syncthreads();
     if (has_col) {
@@ -383,7 +393,7 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a) {
     // endpoint bits masked (the tail's single-source BFSes supply them), or
     // densely in the queue kernel when there is no tail
     for (int k = tid; k < K; k += BS) {
-        const gg_swap &s = a.swaps[a.k0 + k];
+        const gg_swap &s = a.swaps[kc + k];
         const unsigned long long em = endpoint_mask(s, w);
         const bool dense = TAIL == 0 && em != 0;
         s_mask[k] = TAIL ? em : 0ull;
@@ -396,7 +406,7 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a) {
         a.task_state[gt(k)] = s_state[k];
         if (dense) {
             int q = atomicAdd(a.queue_count, 1);
-            a.queue[q] = make_int2(a.k0 + k, w);
+            a.queue[q] = make_int2(kc + k, w);
         }
         s_ended[k] = 0;
     }
@@ -651,7 +661,7 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a) {
                 a.task_state[g] = GG_TASK_DENSE;
                 a.task_flags[g] = GG_FLAG_QUEUED;
                 int q = atomicAdd(a.queue_count, 1);
-                a.queue[q] = make_int2(a.k0 + k, w);
+                a.queue[q] = make_int2(kc + k, w);
                 s_ended[k] = 1;
                 continue;
             }
@@ -680,9 +690,10 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a) {
     }  // has_col
     if constexpr (TAIL > 0) {
         // tail: a single-source BFS from every endpoint of every candidate
-        // on the candidate graph (task 4k + j)
+        // of the launch on the candidate graph (task 4k + j)
         __shared__ int s_task;
-        const int ntask = 4 * K;
+        const int ntask = 4 * a.K;
+        const gg_swap *lsw = a.swaps + a.k0;  // the launch's swaps
         const int NB = (N + 31) >> 5;
         // frontier lists in the LDS left after the three bitmaps
         const int lc = (int)(((size_t)N * 8 - 12 * (size_t)((NB + 1) & ~1)) / 8);
@@ -694,19 +705,19 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a) {
                 for (;;) {
                     t = atomicAdd(a.ep_next, 1);
                     if (t >= ntask) break;
-                    if (endpoint_first(s_sw[t >> 2], t & 3)) break;
+                    if (endpoint_first(lsw[t >> 2], t & 3)) break;
                 }
                 s_task = t;
             }
             __syncthreads();
             const int t = __builtin_amdgcn_readfirstlane(s_task);
             if (t >= ntask) break;
-            const int k = t >> 2, x = s_sw[k].v[t & 3];
+            const int k = t >> 2, x = lsw[k].v[t & 3];
             const size_t gt4 = (size_t)4 * a.k0 + t;
             const int L1 = a.lmax + 1;
             SwapRegs sw;
             bool has;
-            load_swap(s_sw, k, sw, has);
+            load_swap(lsw, k, sw, has);
             int lim_c = 0;
             const int ec = ss_bfs<BS, DT>(a.adjR, N, D, a.lmax, a.S, x, sw, a.undirected,
                                           scratch, lc, a.ss_cnt + gt4 * L1,
@@ -858,18 +869,20 @@ static __global__ void k_queue_reset(int *ctl, int n) {
 // tail (fixed degrees only).
 template <int DT>
 hipError_t launch_sparse_deg(const SparseArgs &s, size_t lds, int tail, hipStream_t st) {
-    auto go = [&](auto fn, gg::LdsAttr &attr, int grid) -> hipError_t {
+    auto go = [&](auto fn, gg::LdsAttr &attr, int grid, int bs) -> hipError_t {
         hipError_t e = gg::ensure_dynamic_lds((const void *)fn, lds, attr);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(fn, dim3(grid), dim3(1024), lds, st, s);
+        hipLaunchKernelGGL(fn, dim3(grid), dim3(bs), lds, st, s);
         return hipGetLastError();
     };
-    static gg::LdsAttr c0, c1;
+    static gg::LdsAttr c0, c1, c2;
     if constexpr (DT > 0) {
-        // tail instances may take extra workgroups; one per column otherwise
-        if (tail) return go(&k_sparse<1024, DT, 1>, c1, s.grid > s.W ? s.grid : s.W);
+        // tail instances may take extra workgroups; one per (group, column)
+        // otherwise
+        if (tail && s.block == 512) return go(&k_sparse<512, DT, 1>, c2, s.grid, 512);
+        if (tail) return go(&k_sparse<1024, DT, 1>, c1, s.grid, 1024);
     }
-    return go(&k_sparse<1024, DT, 0>, c0, s.W);
+    return go(&k_sparse<1024, DT, 0>, c0, (s.K + s.kg - 1) / s.kg * s.W, 1024);
 }
 
 namespace gg {

@@ -50,6 +50,15 @@ GG_BATCH_OPT_DELTA_LIMIT = 4
 GG_BATCH_INFO_LAST_SPARSE = 100
 GG_BATCH_INFO_SPARSE_OK = 101
 GG_BATCH_INFO_LAST_COMMIT_INPLACE = 102
+GG_BATCH_OPT_LAUNCH_K = 5
+GG_BATCH_OPT_GROUP_K = 6
+GG_BATCH_OPT_SPARSE_BLOCK = 7
+GG_BATCH_OPT_STAGED_KEYS = 8
+GG_BATCH_OPT_BASE_BLOCK = 9
+GG_BATCH_OPT_BASE_PUSH = 10
+GG_BATCH_INFO_LAUNCH_MAX = 103
+GG_BATCH_INFO_STAGED_MAX = 104
+GG_BATCH_TUNE_MAX = 64
 PATH_AUTO, PATH_DENSE, PATH_SPARSE = 0, 1, 2
 
 
@@ -77,6 +86,36 @@ class TuneReport(C.Structure):
     _fields_ = [("count", C.c_int32), ("block", C.c_int32 * 4), ("generic", C.c_int32 * 4),
                 ("variant", C.c_int32 * 4), ("ms", C.c_float * 4), ("best_block", C.c_int32),
                 ("best_generic", C.c_int32), ("best_ms", C.c_float)]
+
+
+class BatchTuneReport(C.Structure):
+    _T = GG_BATCH_TUNE_MAX
+    _fields_ = [("count", C.c_int32), ("launch_k", C.c_int32 * _T), ("group_k", C.c_int32 * _T),
+                ("block", C.c_int32 * _T), ("staged_keys", C.c_int32 * _T),
+                ("ms", C.c_float * _T), ("mismatches", C.c_int32),
+                ("best_launch_k", C.c_int32), ("best_group_k", C.c_int32),
+                ("best_block", C.c_int32), ("best_staged_keys", C.c_int32),
+                ("best_ms", C.c_float), ("default_ms", C.c_float), ("base_count", C.c_int32),
+                ("base_block", C.c_int32 * 4), ("base_push", C.c_int32 * 4),
+                ("base_ms", C.c_float * 4), ("best_base_block", C.c_int32),
+                ("best_base_push", C.c_int32), ("best_base_ms", C.c_float)]
+
+    def as_dict(self):
+        n, nb = self.count, self.base_count
+        return {
+            "configs": [{"launch_k": self.launch_k[i], "group_k": self.group_k[i],
+                         "block": self.block[i], "staged_keys": self.staged_keys[i],
+                         "ms": round(self.ms[i], 4)} for i in range(n)],
+            "mismatches": self.mismatches,
+            "best": {"launch_k": self.best_launch_k, "group_k": self.best_group_k,
+                     "block": self.best_block, "staged_keys": self.best_staged_keys,
+                     "ms": round(self.best_ms, 4)},
+            "default_ms": round(self.default_ms, 4),
+            "base": [{"block": self.base_block[i], "push": self.base_push[i],
+                      "ms": round(self.base_ms[i], 4)} for i in range(nb)],
+            "best_base": {"block": self.best_base_block, "push": self.best_base_push,
+                          "ms": round(self.best_base_ms, 4)},
+        }
 
 
 class LdsProbe(C.Structure):
@@ -186,6 +225,7 @@ def lib() -> C.CDLL:
         "gg_batch_levels_digest": (C.c_int, [vp, P(C.c_uint64)]),
         "gg_batch_set_option": (C.c_int, [vp, C.c_int, C.c_int]),
         "gg_batch_get_option": (C.c_int, [vp, C.c_int, P(C.c_int)]),
+        "gg_batch_autotune": (C.c_int, [vp, P(Swap), C.c_int, C.c_int, P(BatchTuneReport)]),
         "gg_search_create": (C.c_int, [C.c_int, C.c_int, i32p, P(SearchCfg), C.c_int, P(vp)]),
         "gg_search_create_cb": (C.c_int, [C.c_int, C.c_int, i32p, P(SearchCfg),
                                           P(EvalCallbacks), P(vp)]),
@@ -449,6 +489,22 @@ class BatchEvaluator:
         ms = C.c_float(0)
         check(lib().gg_batch_last_kernel_ms(self._h, C.byref(ms)), "gg_batch_last_kernel_ms")
         return ms.value
+
+    def autotune(self, swaps, reps: int = 3) -> BatchTuneReport:
+        """Time the incremental step's launch shapes and the base BFS variants
+        on these candidates, apply and cache the fastest."""
+        K = len(swaps)
+        arr = (Swap * K)(*swaps)
+        rep = BatchTuneReport()
+        check(lib().gg_batch_autotune(self._h, arr, K, reps, C.byref(rep)), "gg_batch_autotune")
+        return rep
+
+    def shape(self) -> dict:
+        """The incremental step's current launch shape."""
+        names = {"launch_k": GG_BATCH_OPT_LAUNCH_K, "group_k": GG_BATCH_OPT_GROUP_K,
+                 "block": GG_BATCH_OPT_SPARSE_BLOCK, "staged_keys": GG_BATCH_OPT_STAGED_KEYS,
+                 "base_block": GG_BATCH_OPT_BASE_BLOCK, "base_push": GG_BATCH_OPT_BASE_PUSH}
+        return {k: self.get_option(v) for k, v in names.items()}
 
     def base_ecc(self) -> np.ndarray:
         e = np.empty(self.N, dtype=np.int32)

@@ -246,8 +246,54 @@ int gg_batch_levels_digest(gg_batch *b, uint64_t *digest);
  * dense fallback queue (0 = the allocated capacity, min(2048, N)); a small
  * value sends most tasks through the queue (used by tests). */
 #define GG_BATCH_OPT_DELTA_LIMIT   4
+/* Launch shape of the incremental step; 0 restores the automatic value (all
+ * candidates the buffers hold in one launch, min(64, that) per workgroup,
+ * 1024 threads, as many staged keys as the LDS left over holds).  A shape
+ * gg_batch_autotune cached for the device, N, degree and max_k applies at
+ * creation.  The reference has no such knob: its launch shape is fixed
+ * (cudacode/cudaASPLconv.cu:123-124). */
+#define GG_BATCH_OPT_LAUNCH_K      5   /* candidates per launch (<= GG_BATCH_INFO_LAUNCH_MAX) */
+#define GG_BATCH_OPT_GROUP_K       6   /* candidates per column workgroup (1..64)            */
+#define GG_BATCH_OPT_SPARSE_BLOCK  7   /* threads per workgroup: 512 or 1024 (512: degrees
+                                          4, 8, 12, 16, 24 only)                             */
+#define GG_BATCH_OPT_STAGED_KEYS   8   /* Delta keys staged in LDS per column and level     */
+#define GG_BATCH_OPT_BASE_BLOCK    9   /* dense base BFS threads per workgroup: 256 / 1024  */
+#define GG_BATCH_OPT_BASE_PUSH    10   /* dense base BFS push levels: 1 allowed, 0 pull only */
+/* read-only: candidates per launch the buffers hold */
+#define GG_BATCH_INFO_LAUNCH_MAX   103
+/* read-only: staged keys the LDS left over allows */
+#define GG_BATCH_INFO_STAGED_MAX   104
 int gg_batch_set_option(gg_batch *b, int option, int value);
 int gg_batch_get_option(gg_batch *b, int option, int *value);
+
+/* Times the incremental step on the caller's K candidates (the resident base
+ * must allow it) over candidates per launch {K, K/2, K/4}, candidates per
+ * workgroup {8, 16, 32, 64}, 512 / 1024 threads (fixed degrees) and the
+ * staged-key count (all the LDS left over, or what leaves room for two
+ * workgroups per CU), and the dense base BFS over 256 / 1024 threads (the
+ * vertices per thread follow) with and without push levels.  Every
+ * configuration's results must equal the first one's, bit for bit
+ * (`mismatches` counts those that did not; they are never chosen).  Applies
+ * the fastest step and base configurations and caches them (GG_TUNE_CACHE,
+ * default $HOME/.cache/gg_mi355x_tune.txt) under the device architecture, N,
+ * degree and max_k.  The base is unchanged.  No reference counterpart. */
+#define GG_BATCH_TUNE_MAX 64
+typedef struct gg_batch_tune_report {
+    int32_t count;                       /* step configurations timed        */
+    int32_t launch_k[GG_BATCH_TUNE_MAX], group_k[GG_BATCH_TUNE_MAX];
+    int32_t block[GG_BATCH_TUNE_MAX], staged_keys[GG_BATCH_TUNE_MAX];
+    float ms[GG_BATCH_TUNE_MAX];         /* median step time (ms)            */
+    int32_t mismatches;
+    int32_t best_launch_k, best_group_k, best_block, best_staged_keys;
+    float best_ms, default_ms;           /* fastest and automatic shapes     */
+    int32_t base_count;
+    int32_t base_block[4], base_push[4];
+    float base_ms[4];                    /* median base BFS kernel time (ms) */
+    int32_t best_base_block, best_base_push;
+    float best_base_ms;
+} gg_batch_tune_report;
+int gg_batch_autotune(gg_batch *b, const gg_swap *swaps, int K, int reps,
+                      gg_batch_tune_report *report);
 
 /* ---- measurement helper (no reference counterpart) ------------------------
  * LDS read bandwidth of the whole device: 16 workgroups per CU of 1024

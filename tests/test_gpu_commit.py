@@ -89,8 +89,9 @@ def test_inplace_commit_of_worse_candidates(gpu):
 
 def test_commit_mode_new_bfs_and_chunks(gpu):
     """GG_BATCH_OPT_COMMIT = 1 always runs a new BFS; a candidate outside the
-    last launch of a chunked batch is committed with a new BFS too.  Every
-    path ends in the same stored state."""
+    last launch of a batch split into launches of 64 is committed with a new
+    BFS too, one of a single launch in place.  Every path ends in the same
+    stored state."""
     gg = gpu
     n, d = 1024, 6
     rng = np.random.default_rng(23)
@@ -102,7 +103,8 @@ def test_commit_mode_new_bfs_and_chunks(gpu):
         assert be.get_option(gg.GG_BATCH_INFO_LAST_COMMIT_INPLACE) == 0
         be.set_option(gg.GG_BATCH_OPT_COMMIT, 0)
         _same_as_fresh(gg, be, adj, 16, rng)
-        # 100 candidates: launches of at most 64; the logs cover the last one
+        # 100 candidates in launches of at most 64: the logs cover the last one
+        be.set_option(gg.GG_BATCH_OPT_LAUNCH_K, 64)
         cands = _random_swaps(gg, adj, 100, rng)
         rs = be.eval([c[0] for c in cands])
         ok = [i for i, r in enumerate(rs) if r.status == gg.GG_OK]
@@ -122,6 +124,15 @@ def test_commit_mode_new_bfs_and_chunks(gpu):
         assert base.total_reps == rs[i].total_reps
         adj = cands[i][1]
         _same_as_fresh(gg, be, adj, 100, rng)
+        # by default all 100 run in one launch and every candidate commits in place
+        be.set_option(gg.GG_BATCH_OPT_LAUNCH_K, 0)
+        cands = _random_swaps(gg, adj, 100, rng)
+        rs = be.eval([c[0] for c in cands])
+        i = [i for i, r in enumerate(rs) if r.status == gg.GG_OK][0]
+        base = be.commit(cands[i][0])
+        assert be.get_option(gg.GG_BATCH_INFO_LAST_COMMIT_INPLACE) == 1
+        assert base.total_reps == rs[i].total_reps
+        _same_as_fresh(gg, be, cands[i][1], 100, rng)
 
 
 def test_features_after_inplace_commits(gpu):
