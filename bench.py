@@ -339,6 +339,35 @@ def large_n_orbit(gg, g=8192, m=128, d=16, seed=3, reps=5):
     return out
 
 
+def large_n_plain(gg, g=1024, m=128, d=16, seed=3, reps=3):
+    """Plain large-N evaluation with every vertex a source: the orbit graph of
+    N = g*m = 2^17 vertices as an explicit adjacency (the reference's
+    cudaASPLgraph, 1024-source passes, cudaASPLgraph.cu:155-160); here passes
+    of 256 columns, each level one launch over the whole grid, against the
+    one-workgroup-per-column global kernel.  Algorithmic bytes 8*(d+2)*N*W*L."""
+    N = g * m
+    tab = gg.orbit_part(g, m, d, seed)
+    # the orbit graph's explicit adjacency: vertex v's neighbours are
+    # (v + o) mod N for the offsets o of row v mod m
+    v = np.arange(N, dtype=np.int64)[:, None]
+    adj = ((v + tab[v[:, 0] % m]) % N).astype(np.int32)
+    out = {"workload": f"explicit adjacency N={N} (all sources, {N // 64} columns), d={d}"}
+    with gg.AdjEvaluator(N, d) as ev:
+        r = ev.eval(adj)
+        bfs_ms = ev.time_bfs(reps)
+        W = N // 64
+        alg = 8 * (d + 2) * N * W * r.levels
+        out.update({"diameter": r.diameter, "levels": r.levels, "total": r.total_reps,
+                    "passes": -(-W // 256), "level_sync_bfs_ms": bfs_ms,
+                    "level_sync_alg_GB_per_s": alg / (bfs_ms * 1e-3) / 1e9})
+        ev.set_option(gg.GG_OPT_WIDE, 0)
+        r2 = ev.eval(adj)
+        assert ev.get_option(gg.GG_INFO_VARIANT) == 3 and r2.total_reps == r.total_reps
+        pers_ms = ev.time_bfs(1)
+        out.update({"persistent_bfs_ms": pers_ms, "speedup": pers_ms / bfs_ms})
+    return out
+
+
 def equal_time(gg, adj, cfgs, seconds):
     """Each search for `seconds` of wall time from the same graph (its
     temperature schedule sized to the steps it makes in that time), best
@@ -399,7 +428,8 @@ def main():
         copy_calibration()
         return
     if args.only == "large":
-        print(json.dumps({"large_n": large_n_orbit(gg)}), flush=True)
+        print(json.dumps({"large_n": large_n_orbit(gg), "large_n_plain": large_n_plain(gg)}),
+              flush=True)
         return
     adj = gg.random_regular(n, d, args.seed)
     if args.only == "roofline":
@@ -479,6 +509,7 @@ def main():
         out["speedup_vs_plain_kernel"] = learn_rate / (1e3 / kms)
     out["batched_vs_sequential"] = batched_vs_sequential(gg)
     out["large_n"] = large_n_orbit(gg)
+    out["large_n_plain"] = large_n_plain(gg)
     print(json.dumps({"progress": "kernels timed"}), file=sys.stderr, flush=True)
     if args.equal_time > 0:
         sec = args.equal_time

@@ -160,7 +160,26 @@ struct WideArgs {
     unsigned *fv;
     unsigned long long *fb;
     int *ctl, *mode;
+    // vertex order of the words: v itself (pcopies = 0), or for a table of
+    // 2^pshift rows repeated pcopies = N >> pshift times, vertex v = c*R + r
+    // (copy c of row r) at position r*pcopies + c.  A row's offsets are the
+    // same in every copy, so the lanes holding consecutive copies of one row
+    // gather consecutive words of a neighbour row.
+    int pshift;
+    unsigned pcopies;
 };
+
+__device__ __forceinline__ size_t wide_pos(const WideArgs &a, unsigned v) {
+    if (!a.pcopies) return v;
+    return (size_t)(v & ((1u << a.pshift) - 1u)) * a.pcopies + (v >> a.pshift);
+}
+
+__device__ __forceinline__ unsigned wide_vertex(const WideArgs &a, size_t p) {
+    if (!a.pcopies) return (unsigned)p;
+    const unsigned r = (unsigned)(p / a.pcopies);
+    const unsigned c = (unsigned)(p - (size_t)r * a.pcopies);
+    return (c << a.pshift) + r;
+}
 
 // cur := padding bits (columns >= W: all ones), counters of levels 1..lmax
 // cleared, level 0 holds each column's valid source bits and their count.
@@ -191,7 +210,7 @@ __global__ void __launch_bounds__(256) k_wide_init(WideArgs a, unsigned long lon
 __global__ void __launch_bounds__(256) k_wide_sources(WideArgs a, unsigned long long *cur) {
     const int s = blockIdx.x * 256 + threadIdx.x;
     if (s >= a.M) return;
-    const size_t i = ((size_t)a.src_vertex[s] << a.wsh) + (size_t)(s >> 6);
+    const size_t i = (wide_pos(a, (unsigned)a.src_vertex[s]) << a.wsh) + (size_t)(s >> 6);
     cur[i] |= 1ull << (s & 63);
     if (a.fcap) {
         a.fv[s] = (unsigned)i;
@@ -247,10 +266,10 @@ __global__ void __launch_bounds__(256) k_wide_push(Topo topo, WideArgs a, int le
             const int j = (int)(it - e * (size_t)a.D);
             const unsigned idx = fv_in[e];
             const unsigned long long bits = fb_in[e];
-            const unsigned v = idx >> a.wsh;
+            const unsigned v = wide_vertex(a, idx >> a.wsh);
             const int w = (int)(idx & (unsigned)(Wp - 1));
             const unsigned u = nbr_patched(topo, v, j, sw, has_sw);
-            ui = (u << a.wsh) + (unsigned)w;
+            ui = (unsigned)(wide_pos(a, u) << a.wsh) + (unsigned)w;
             const unsigned long long old = atomicOr(&nxt[ui], bits);
             nb = bits & ~old;
             if (nb) {
@@ -317,17 +336,27 @@ __global__ void __launch_bounds__(256) k_wide_level(Topo topo, WideArgs a, int l
     unsigned long long cnt = 0, lor = 0;
     if (live) {
         const size_t vstride = ((size_t)gridDim.x * 256) >> a.wsh;
-        for (size_t v = t0 >> a.wsh; v < (size_t)a.N; v += vstride) {
-            const size_t i = (v << a.wsh) + (size_t)w;
+        for (size_t p = t0 >> a.wsh; p < (size_t)a.N; p += vstride) {
+            const unsigned v = wide_vertex(a, p);
+            const size_t i = (p << a.wsh) + (size_t)w;
             const unsigned long long own = cur[i];
             unsigned long long n = 0;
             if (~own) {
+                // neighbours in groups of four loads; stop once every missing
+                // bit has been found (the OR cannot add more)
+                const unsigned long long need = ~own;
                 unsigned long long acc = 0;
-                for (int j = 0; j < a.D; j++) {
-                    const unsigned u = nbr_patched(topo, (unsigned)v, j, sw, has_sw);
-                    acc |= cur[((size_t)u << a.wsh) + (size_t)w];
+                for (int j0 = 0; j0 < a.D; j0 += 4) {
+#pragma unroll
+                    for (int j = j0; j < j0 + 4; j++) {
+                        if (j < a.D) {
+                            const unsigned u = nbr_patched(topo, v, j, sw, has_sw);
+                            acc |= cur[(wide_pos(a, u) << a.wsh) + (size_t)w];
+                        }
+                    }
+                    if ((acc & need) == need) break;
                 }
-                n = acc & ~own;
+                n = acc & need;
             }
             nxt[i] = own | n;
             cnt += (unsigned long long)__popcll(n);
@@ -390,7 +419,7 @@ __global__ void __launch_bounds__(256) k_wide_parts(WideArgs a, int lexec, ColPa
     }
 }
 
-// Level-synchronous path: columns of at most 256 words per vertex.
+// Level-synchronous path: at most 256 words per vertex in one pass.
 static const int kWideMaxCols = 256;
 static const int kWideChunk = 4;  // levels launched between reads of the counters
 
@@ -402,6 +431,11 @@ static int ensure_gscratch(DenseCtx &c, size_t need) {
     c.gscratch_bytes = need;
     return GG_OK;
 }
+
+// Rows of a ring table repeated over the vertices (0: other topologies).
+template <class Topo> static unsigned topo_rows(const Topo &) { return 0; }
+template <> unsigned topo_rows(const TopoRingQ &t) { return t.R; }
+template <> unsigned topo_rows(const TopoRing<uint32_t> &t) { return t.R; }
 
 // One launch per level over all (vertex, column) words; between chunks of
 // kWideChunk levels the host reads the last level's new-bit masks and stops
@@ -440,6 +474,13 @@ static int run_wide(DenseCtx &c, const DenseArgs &a, const Topo &topo, hipStream
     wa.fv = reinterpret_cast<unsigned *>(wa.fb + 2 * fcap);
     wa.ctl = reinterpret_cast<int *>(wa.fv + 2 * fcap);
     wa.mode = wa.ctl + 2;
+    wa.pshift = 0;
+    wa.pcopies = 0;
+    const unsigned R = topo_rows(topo);
+    if (R >= 1 && R < (unsigned)a.N && (R & (R - 1)) == 0 && a.N % R == 0) {
+        while ((1u << wa.pshift) < R) wa.pshift++;
+        wa.pcopies = (unsigned)a.N / R;
+    }
     c.last_push_cap = (int)fcap;
     const size_t want = (words + 255) / 256;
     const int grid = (int)(want < (size_t)8 * c.cu_count ? want : (size_t)8 * c.cu_count);
@@ -551,21 +592,32 @@ int run_dense(DenseCtx &c, const DenseArgs &a_in, hipStream_t st) {
             case TOPO_PIECE: e = launch_lds(c.piece, a, tasks, lds, bs, vpt, st); break;
             default: e = launch_lds(c.box, a, tasks, lds, bs, vpt, st); break;
         }
-    } else if (c.wide_ok && !c.wide_off && a.K == 1 && !a.queue && !a.snap &&
-               a.W <= kWideMaxCols) {
-        // few columns of a large graph: every level spread over the whole grid
+    } else if (c.wide_ok && !c.wide_off && a.K == 1 && !a.queue && !a.snap) {
+        // a large graph: every level spread over the whole grid, the words
+        // of up to kWideMaxCols columns of a vertex side by side (so the
+        // lanes gathering one neighbour read its consecutive words); more
+        // columns run as passes of kWideMaxCols, each writing its own parts
         c.last_variant = 4;
         c.last_vpt = 0;
         c.last_fixed = 0;
         c.last_push_cap = 0;  // run_wide sets its frontier capacity
-        int rc;
-        switch (c.topo_kind) {
-            case TOPO_RINGQ: rc = run_wide(c, a, c.ringq, st); break;
-            case TOPO_RING32: rc = run_wide(c, a, c.ring32, st); break;
-            case TOPO_PIECE: rc = run_wide(c, a, c.piece, st); break;
-            default: rc = run_wide(c, a, c.box, st); break;
+        for (int w0 = 0; w0 < a.W; w0 += kWideMaxCols) {
+            DenseArgs p = a;
+            p.W = a.W - w0 < kWideMaxCols ? a.W - w0 : kWideMaxCols;
+            p.M = a.M - 64 * w0 < 64 * p.W ? a.M - 64 * w0 : 64 * p.W;
+            p.src_vertex = a.src_vertex + (size_t)64 * w0;
+            p.parts = a.parts + w0;
+            p.ecc_out = a.ecc_out ? a.ecc_out + (size_t)64 * w0 : nullptr;
+            p.reach_col = a.reach_col ? a.reach_col + (size_t)w0 * (a.lmax + 1) : nullptr;
+            int rc;
+            switch (c.topo_kind) {
+                case TOPO_RINGQ: rc = run_wide(c, p, c.ringq, st); break;
+                case TOPO_RING32: rc = run_wide(c, p, c.ring32, st); break;
+                case TOPO_PIECE: rc = run_wide(c, p, c.piece, st); break;
+                default: rc = run_wide(c, p, c.box, st); break;
+            }
+            if (rc != GG_OK) return rc;
         }
-        if (rc != GG_OK) return rc;
         e = hipSuccess;
     } else {
         // global-memory variant: persistent grid, 2*N words of scratch per workgroup

@@ -157,9 +157,11 @@ int gg_eval_time_bfs(gg_eval *ev, int reps, float *ms);
                                  the graph is undirected (adjacency evaluators verify
                                  it on upload and per swap); 0 = pull only           */
 #define GG_OPT_WIDE       6   /* 1 (default) = a single evaluation whose column bitset
-                                 does not fit in LDS and that has at most 256 columns
-                                 runs level-synchronously over the whole grid (one
-                                 launch per level, GG_INFO_VARIANT 4); 0 = one
+                                 does not fit in LDS runs level-synchronously over the
+                                 whole grid (one launch per level, passes of 256
+                                 columns = 16384 sources, as the reference's
+                                 cudaASPLgraph runs passes of 1024 sources,
+                                 cudaASPLgraph.cu:155-160; GG_INFO_VARIANT 4); 0 = one
                                  workgroup per column (GG_INFO_VARIANT 3)            */
 #define GG_OPT_SWAP_GRAPH 7   /* adjacency evaluators: how gg_eval_adj_swap runs its
                                  BFS and finalize kernels.  0 (default) = swap and

@@ -1,6 +1,6 @@
 """GPU parity of the level-synchronous path: a single evaluation whose column
-bitset does not fit in LDS and that has at most 256 columns runs one launch
-per level over the whole grid (GG_INFO_VARIANT 4).  Checked bit for bit
+bitset does not fit in LDS runs one launch per level over the whole grid
+(GG_INFO_VARIANT 4), in passes of 256 columns when it has more.  Checked bit for bit
 against the CPU oracle and against the persistent one-workgroup-per-column
 global kernel (GG_OPT_WIDE 0, GG_INFO_VARIANT 3): distance totals, diameters,
 per-source eccentricities, per-level counts, WVC, status codes."""
@@ -48,6 +48,53 @@ def test_adjacency_matches_oracle(gpu, n, d, seed):
     assert r["levels"] == o["diameter"] + 1
     assert np.array_equal(ecc, o["ecc"])
     assert np.array_equal(reach[:len(o["reach"])], o["reach"][:len(reach)])
+
+
+def test_adjacency_in_several_passes_matches_oracle(gpu):
+    """17000 sources = 266 columns: a pass of 256 columns, then one of 10
+    with a ragged last column."""
+    gg = gpu
+    n, d = 17000, 6
+    adj = gg.random_regular(n, d, 21)
+    o = ol.adj_eval(adj)
+    r, ecc, reach = _both(gg, lambda: gg.AdjEvaluator(n, d), lambda ev: ev.eval(adj))
+    assert r["status"] == gg.GG_OK
+    assert r["total_reps"] == o["total"] and r["diameter"] == o["diameter"]
+    assert np.array_equal(ecc, o["ecc"])
+    assert np.array_equal(reach[:len(o["reach"])], o["reach"][:len(reach)])
+
+
+def test_plain_large_n_every_vertex_a_source(gpu):
+    """The plain evaluator on a graph of N = 131840 > 2^17 vertices, every
+    vertex a source (2060 columns: eight passes of 256 and one of 12), given
+    as an explicit adjacency: an orbit graph of g = 1030 copies of 128
+    representatives, so that translation by 128 is an automorphism and the
+    oracle's BFS from the 128 representatives fixes everything: the total
+    over all N sources is g times theirs, the diameter is theirs, vertex v's
+    eccentricity is that of representative v mod 128, and the per-level
+    counts are g times theirs.  The one-workgroup-per-column kernel agrees."""
+    gg = gpu
+    g, m, d = 1030, 128, 16
+    N = g * m
+    tab = gg.orbit_part(g, m, d, 3)
+    o = ol.part_eval(N, m, tab)
+    adj = ol.expand_part(N, m, tab)
+    got = []
+    for wide in (1, 0):
+        with gg.AdjEvaluator(N, d) as ev:
+            ev.set_option(gg.GG_OPT_WIDE, wide)
+            print(f"large plain N={N} wide={wide}", flush=True)
+            r = ev.eval(adj)
+            assert ev.get_option(gg.GG_INFO_VARIANT) == (4 if wide else 3)
+            ecc, reach = ev.levels()
+            got.append((r.as_dict(), ecc, reach))
+    (r, ecc, reach), (rg, eccg, reachg) = got
+    assert r == rg and np.array_equal(ecc, eccg) and np.array_equal(reach, reachg)
+    assert r["status"] == gg.GG_OK
+    assert r["total_reps"] == g * o["total"] and r["diameter"] == o["diameter"]
+    assert np.array_equal(ecc, np.tile(o["ecc"], g))
+    L = len(o["reach"])
+    assert np.array_equal(reach[:L], g * o["reach"][:len(reach)])
 
 
 def test_part_piece_box_match_oracle(gpu):
@@ -121,6 +168,29 @@ def test_directed_part_table_pulls(gpu):
     rng = np.random.default_rng(11)
     tab = rng.integers(1, N, size=(m, d)).astype(np.int32)
     _both(gg, lambda: gg.PartEvaluator(N, m, d), lambda ev: ev.eval(tab), pushes=False)
+
+
+@pytest.mark.parametrize("g,m,d,directed", [(64, 64, 6, False), (48, 128, 5, True),
+                                             (2, 2048, 4, False)])
+def test_copy_major_word_order(gpu, g, m, d, directed):
+    """Tables of 2^k rows repeated over the vertices keep their words copy by
+    copy (row r's copies side by side); push, pull and the persistent kernel
+    agree with each other and the oracle, for orbit graphs and for a
+    directed table (pull only)."""
+    gg = gpu
+    N = g * m
+    if directed:
+        rng = np.random.default_rng(g)
+        tab = rng.integers(1, N, size=(m, d)).astype(np.int32)
+    else:
+        tab = gg.orbit_part(g, m, d, 7)
+    r, ecc, reach = _both(gg, lambda: gg.PartEvaluator(N, m, d), lambda ev: ev.eval(tab),
+                          pushes=not directed)
+    if not directed:
+        o = ol.part_eval(N, m, tab)
+        assert r["total_scaled"] == o["total_scaled"] and r["diameter"] == o["diameter"]
+        assert np.array_equal(ecc, o["ecc"])
+        assert np.array_equal(reach[:len(o["reach"])], o["reach"][:len(reach)])
 
 
 def test_large_orbit_graph_matches_oracle(gpu):
