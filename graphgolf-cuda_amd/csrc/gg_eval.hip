@@ -342,18 +342,21 @@ __global__ void __launch_bounds__(256) k_wide_level(Topo topo, WideArgs a, int l
             const unsigned long long own = cur[i];
             unsigned long long n = 0;
             if (~own) {
-                // neighbours in groups of four loads; stop once every missing
-                // bit has been found (the OR cannot add more)
+                // neighbours in groups of eight loads issued together (slots
+                // past the degree repeat the last one); stop once every
+                // missing bit has been found (the OR cannot add more)
                 const unsigned long long need = ~own;
                 unsigned long long acc = 0;
-                for (int j0 = 0; j0 < a.D; j0 += 4) {
+                for (int j0 = 0; j0 < a.D; j0 += 8) {
+                    unsigned long long x[8];
 #pragma unroll
-                    for (int j = j0; j < j0 + 4; j++) {
-                        if (j < a.D) {
-                            const unsigned u = nbr_patched(topo, v, j, sw, has_sw);
-                            acc |= cur[(wide_pos(a, u) << a.wsh) + (size_t)w];
-                        }
+                    for (int k = 0; k < 8; k++) {
+                        const int j = j0 + k < a.D ? j0 + k : a.D - 1;
+                        const unsigned u = nbr_patched(topo, v, j, sw, has_sw);
+                        x[k] = cur[(wide_pos(a, u) << a.wsh) + (size_t)w];
                     }
+#pragma unroll
+                    for (int k = 0; k < 8; k++) acc |= x[k];
                     if ((acc & need) == need) break;
                 }
                 n = acc & need;
@@ -489,7 +492,10 @@ static int run_wide(DenseCtx &c, const DenseArgs &a, const Topo &topo, hipStream
     std::vector<unsigned long long> h(a.W);
     int level = 1;
     while (level <= a.lmax) {
-        const int end = level + kWideChunk - 1 < a.lmax ? level + kWideChunk - 1 : a.lmax;
+        // the first read of the counters after 2 * kWideChunk levels (most
+        // graphs this path serves close by then), then every kWideChunk
+        const int chunk = level == 1 ? 2 * kWideChunk : kWideChunk;
+        const int end = level + chunk - 1 < a.lmax ? level + chunk - 1 : a.lmax;
         for (; level <= end; level++) {
             if (fcap) {
                 hipLaunchKernelGGL(k_wide_copy, dim3(grid), dim3(256), 0, st, wa, level,
