@@ -6,7 +6,7 @@
 //
 //   gg-anneal [-c COUNT] [-s SEED] [-i IN] [-o OUT] [-l LOG] [-v VERBOSE] [-D DEVICE]
 //             [-t TEMP] [--hill] [--mode plain|batched|learned] [-K K] [-P P]
-//             [--schedule step|geometric] [--temp-end T] [-f part|piece|box] [--wvc]
+//             [--schedule step|geometric] [--temp-end T] [-f part|piece|box] [--wvc] [--nr]
 //
 // Input and output are `part` tables (part.hpp:13-43) by default.  M = N: an
 // explicit graph, 2-opt swaps (gg_search); M < N: a cyclically symmetric
@@ -27,6 +27,10 @@
 // (diameter, WVC, ASPL) as annealingWVC.cpp:412-417 / HillClimbWVC.cpp:240
 // do (part input, plain mode), with annealingWVC's schedule by default:
 // temp = TEMP^(1 - i/COUNT), geometric from TEMP to 1 (annealingWVC.cpp:411).
+// --nr accepts a symmetric part table whose rows have different numbers of
+// entries, as annealingNR does: slots are drawn per row among its entries
+// (annealingNR.cpp:117,146-147), rows keep their lengths, short rows are
+// evaluated padded with offset 0 (cudaASPLconv.cu:199-201).
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -46,7 +50,7 @@ int usage(int code) {
     std::cout << "gg-anneal [-c COUNT] [-s SEED] [-i IN] [-o OUT] [-l LOG] [-v VERBOSE] [-D DEVICE]\n"
                  "          [-t TEMP] [--hill] [--mode plain|batched|learned] [-K K] [-P P]\n"
                  "          [--schedule step|geometric] [--temp-end T] [-f part|piece|box]\n"
-                 "          [--wvc]\n";
+                 "          [--wvc] [--nr]\n";
     return code;
 }
 
@@ -57,7 +61,7 @@ const char *kClear = "\x1b[F\x1b[E\x1b[K";  // the reference's line rewrite (ann
 int main(int argc, char **argv) {
     auto t_init = std::chrono::steady_clock::now();
     int count = 1000, device = 0, K = 32, P = 1024;
-    bool have_seed = false, hill = false, wvc = false, have_schedule = false;
+    bool have_seed = false, hill = false, wvc = false, have_schedule = false, nr = false;
     unsigned int seed = 0;
     double inittemp = 46.1793, temp_end = -1;
     std::string in_path, out_path, log_path, verbose_path, mode = "plain", schedule = "step";
@@ -83,6 +87,7 @@ int main(int argc, char **argv) {
         else if (a == "-P") P = std::stoi(next());
         else if (a == "--schedule") { schedule = next(); have_schedule = true; }
         else if (a == "--wvc") wvc = true;
+        else if (a == "--nr") nr = true;
         else if (a == "--temp-end") temp_end = std::stod(next());
         else if (a == "-f" || a == "--format") fmt = next();
         else { std::cout << "unknown option " << a << "\n"; return usage(1); }
@@ -127,15 +132,25 @@ int main(int argc, char **argv) {
     graphgolf::box init_box;
     int dims = 1, N = 0, M = 0, d = 0;
     bool regular = true;
-    std::vector<int32_t> tab;
+    std::vector<int32_t> tab, lens;
     if (fmt == "part") {
         init.load(ist);
         regular = init.regular;
         N = init.N; M = init.M; d = init.degree;
-        tab.resize((size_t)M * d);
-        if (regular)
-            for (int r = 0; r < M; r++)
-                for (int j = 0; j < d; j++) tab[(size_t)r * d + j] = init.edges[r][j];
+        tab.assign((size_t)M * d, 0);
+        if (regular || nr) {
+            // --nr: each row's entries first, the rest offset 0
+            for (int r = 0; r < M; r++) {
+                lens.push_back((int32_t)init.edges[r].size());
+                for (size_t j = 0; j < init.edges[r].size(); j++)
+                    tab[(size_t)r * d + j] = init.edges[r][j];
+            }
+            if (!regular && M == N) {
+                std::cout << "--nr needs a symmetric table (M < N)" << std::endl;
+                return 1;
+            }
+            regular = true;
+        }
     } else if (fmt == "piece") {
         init_piece.load(ist);
         regular = init_piece.regular;
@@ -184,7 +199,7 @@ int main(int argc, char **argv) {
         if (dims == 1) {
             graphgolf::part p = init;
             for (int r = 0; r < M; r++)
-                for (int j = 0; j < d; j++)
+                for (size_t j = 0; j < p.edges[r].size(); j++)
                     p.edges[r][j] = M == N ? t[(size_t)r * d + j] - r : t[(size_t)r * d + j];
             p.print(o);
         } else if (dims == 2) {
@@ -279,6 +294,7 @@ int main(int argc, char **argv) {
     } else {
         cfg.mode = GG_MODE_PLAIN;
         st = gg_orbit_search_create(N, M, d, tab.data(), &cfg, device, &os);
+        if (st == GG_OK && nr) st = gg_orbit_search_set_row_lengths(os, lens.data());
     }
     if (st != GG_OK) { std::cerr << "error: " << gg_strerror(st) << "\n"; return 1; }
 

@@ -44,6 +44,9 @@ struct gg_orbit_search {
     bool use_cb = false;
     std::unique_ptr<gg::Rng> rng;
     std::vector<int32_t> tab, cand, best_tab;
+    // part form: entries per row (empty = every row has `d`); slots past a
+    // row's length are padding, offset 0
+    std::vector<int32_t> lens;
     gg_result cur{}, best{};
     int64_t step = 0;
     gg_search_stats stats{};
@@ -60,21 +63,22 @@ inline int64_t mod_n(int64_t x, int64_t N) {
     return x < 0 ? x + N : x;
 }
 
-// Slot of row `row` holding the reverse of offset `diff` seen from row
-// `from`/slot `slot` (the same slot is skipped, as annealing.cpp:121 skips it
-// for diff == 0); -1 when there is none.
+// Slot among the n entries of row `row` holding the reverse of offset `diff`
+// seen from row `from`/slot `slot` (the same slot is skipped, as
+// annealing.cpp:121 skips it for diff == 0); -1 when there is none.
 int partner_slot(const std::vector<int32_t> &t, int N, int d, int row, int from, int slot,
-                 int32_t diff) {
-    for (int i = 0; i < d; i++) {
+                 int32_t diff, int n) {
+    for (int i = 0; i < n; i++) {
         if (row == from && i == slot) continue;
         if (mod_n((int64_t)t[(size_t)row * d + i] + diff, N) == 0) return i;
     }
     return -1;
 }
 
-// Row `r` describes simple neighbourhoods: offsets nonzero and distinct mod N.
-bool row_simple(const std::vector<int32_t> &t, int N, int d, int r) {
-    for (int i = 0; i < d; i++) {
+// The n entries of row `r` describe simple neighbourhoods: offsets nonzero
+// and distinct mod N.
+bool row_simple(const std::vector<int32_t> &t, int N, int d, int r, int n) {
+    for (int i = 0; i < n; i++) {
         int64_t x = mod_n(t[(size_t)r * d + i], N);
         if (x == 0) return false;
         for (int j = 0; j < i; j++)
@@ -86,17 +90,20 @@ bool row_simple(const std::vector<int32_t> &t, int N, int d, int r) {
 }  // namespace
 
 // One proposal attempt on `cur`, written to `out` (a copy of cur).  Returns
-// false when the move is degenerate or its table is not simple.
+// false when the move is degenerate or its table is not simple.  With row
+// lengths (annealingNR, cppcode/annealingNR.cpp:117,146-147) a slot is drawn
+// among its row's entries and partners are looked for among theirs.
 bool gg::propose_orbit(const std::vector<int32_t> &cur, int N, int M, int d, gg::Rng &rng,
-                       std::vector<int32_t> &out, OrbitMove *mv) {
+                       std::vector<int32_t> &out, OrbitMove *mv, const int32_t *lens) {
     const int g = N / M;
+    auto len = [&](int r) { return lens ? (int)lens[r] : d; };
     const int coin = (int)rng.below(2);
     out = cur;
     if (g > 1 && coin == 0) {
-        int from = (int)rng.below(M), ifrom = (int)rng.below(d);
+        int from = (int)rng.below(M), ifrom = (int)rng.below(len(from));
         const int32_t diff = cur[(size_t)from * d + ifrom];
         int to = (int)(mod_n((int64_t)from + diff, N) % M);
-        int ito = partner_slot(cur, N, d, to, from, ifrom, diff);
+        int ito = partner_slot(cur, N, d, to, from, ifrom, diff, len(to));
         if (ito < 0) return false;
         if (from > to) { std::swap(from, to); std::swap(ifrom, ito); }
         const int k = (int)rng.below(g);
@@ -104,15 +111,15 @@ bool gg::propose_orbit(const std::vector<int32_t> &cur, int N, int M, int d, gg:
         out[(size_t)from * d + ifrom] = nd;
         out[(size_t)to * d + ito] = -nd;
         *mv = OrbitMove{0, from, ifrom, to, ito, k, 0};
-        return row_simple(out, N, d, from) && row_simple(out, N, d, to);
+        return row_simple(out, N, d, from, len(from)) && row_simple(out, N, d, to, len(to));
     }
-    int a = (int)rng.below(M), ia = (int)rng.below(d);
-    int c = (int)rng.below(M), ic = (int)rng.below(d);
+    int a = (int)rng.below(M), ia = (int)rng.below(len(a));
+    int c = (int)rng.below(M), ic = (int)rng.below(len(c));
     const int32_t dab = cur[(size_t)a * d + ia], dcd = cur[(size_t)c * d + ic];
     int b = (int)(mod_n((int64_t)a + dab, N) % M);
     int dd = (int)(mod_n((int64_t)c + dcd, N) % M);
-    int ib = partner_slot(cur, N, d, b, a, ia, dab);
-    int id = partner_slot(cur, N, d, dd, c, ic, dcd);
+    int ib = partner_slot(cur, N, d, b, a, ia, dab, len(b));
+    int id = partner_slot(cur, N, d, dd, c, ic, dcd, len(dd));
     if (ib < 0 || id < 0) return false;
     // the same edge orbit drawn twice (annealing.cpp:168-172)
     if ((c == a && ic == ia) || (c == b && ic == ib)) return false;
@@ -126,8 +133,8 @@ bool gg::propose_orbit(const std::vector<int32_t> &cur, int N, int M, int d, gg:
     const int32_t dbc = (c - b) + M * kb;
     out[(size_t)b * d + ib] = dbc;
     out[(size_t)c * d + ic] = -dbc;
-    return row_simple(out, N, d, a) && row_simple(out, N, d, dd) && row_simple(out, N, d, b) &&
-           row_simple(out, N, d, c);
+    return row_simple(out, N, d, a, len(a)) && row_simple(out, N, d, dd, len(dd)) &&
+           row_simple(out, N, d, b, len(b)) && row_simple(out, N, d, c, len(c));
 }
 
 namespace {
@@ -343,7 +350,8 @@ int orbit_step(gg_orbit_search *s) {
     OrbitMove mv{};
     int tries = 0;
     if (s->dims == 1) {
-        while (!gg::propose_orbit(s->tab, s->N, s->M, s->d, rng, s->cand, &mv))
+        while (!gg::propose_orbit(s->tab, s->N, s->M, s->d, rng, s->cand, &mv,
+                                  s->lens.empty() ? nullptr : s->lens.data()))
             if (++tries > 1000000) return GG_BAD_ARG;  // no simple move exists
     } else {
         const Lattice L = lattice_of(s);
@@ -395,6 +403,17 @@ extern "C" int gg_orbit_search_create(int N, int M, int d, const int32_t *table,
     if (st == GG_OK) st = init(s, N, M, d, table, cfg);
     if (st != GG_OK) { gg_orbit_search_destroy(s); return st; }
     *out = s;
+    return GG_OK;
+}
+
+extern "C" int gg_orbit_search_set_row_lengths(gg_orbit_search *s, const int32_t *lens) {
+    if (!s || !lens || s->dims != 1) return GG_BAD_ARG;
+    for (int r = 0; r < s->M; r++) {
+        if (lens[r] < 1 || lens[r] > s->d) return GG_BAD_ARG;
+        for (int i = lens[r]; i < s->d; i++)
+            if (s->tab[(size_t)r * s->d + i] != 0) return GG_BAD_ARG;  // padding is offset 0
+    }
+    s->lens.assign(lens, lens + s->M);
     return GG_OK;
 }
 

@@ -83,12 +83,13 @@ inline bool better_by_rule(int rule, const gg_result &x, const gg_result &y) {
 }
 
 // Symmetric `part` tables (gg_orbit.cpp): one move attempt (edge-length change
-// or orbit 2-opt) written to `out`; false when it must be redrawn.
+// or orbit 2-opt) written to `out`; false when it must be redrawn.  lens:
+// entries per row (nullptr = d in every row).
 struct OrbitMove {
     int kind, a, ia, c, ic, k1, k2;
 };
 bool propose_orbit(const std::vector<int32_t> &cur, int N, int M, int d, Rng &rng,
-                   std::vector<int32_t> &out, OrbitMove *mv);
+                   std::vector<int32_t> &out, OrbitMove *mv, const int32_t *lens = nullptr);
 void apply_swap(std::vector<int32_t> &adj, int d, const gg_swap &sw);
 
 }  // namespace gg

@@ -245,6 +245,7 @@ def lib() -> C.CDLL:
         "gg_box_search_create": (C.c_int, [C.c_int] * 7 + [i32p, P(SearchCfg), C.c_int, P(vp)]),
         "gg_lattice_search_create_cb": (C.c_int, [C.c_int, i32p, i32p, C.c_int, i32p,
                                                   P(SearchCfg), P(TableCallbacks), P(vp)]),
+        "gg_orbit_search_set_row_lengths": (C.c_int, [vp, i32p]),
         "gg_orbit_search_destroy": (None, [vp]),
         "gg_orbit_search_run": (C.c_int, [vp, C.c_int, P(SearchStats)]),
         "gg_orbit_search_table": (C.c_int, [vp, i32p, C.c_int]),
@@ -713,7 +714,8 @@ class OrbitSearch:
     `table_eval` (a Python function table[M][d] -> (total_scaled, diameter,
     status[, wvc])), the exact evaluator is the caller's."""
 
-    def __init__(self, N, table, cfg: SearchCfg, device: int = 0, table_eval=None, log: int = 0):
+    def __init__(self, N, table, cfg: SearchCfg, device: int = 0, table_eval=None, log: int = 0,
+                 row_lengths=None):
         t = _i32(table)
         self.N = N
         self.M, self.d = t.shape
@@ -740,6 +742,10 @@ class OrbitSearch:
             check(lib().gg_orbit_search_create_cb(N, self.M, self.d, _ptr(t), C.byref(cfg),
                                                   C.byref(self._cb), C.byref(self._h)),
                   "gg_orbit_search_create_cb")
+        if row_lengths is not None:
+            lens = _i32(row_lengths)
+            check(lib().gg_orbit_search_set_row_lengths(self._h, _ptr(lens)),
+                  "gg_orbit_search_set_row_lengths")
         if log:
             check(lib().gg_orbit_search_set_log(self._h, log), "gg_orbit_search_set_log")
 

@@ -136,6 +136,13 @@ int gg_orbit_search_create(int N, int M, int degree, const int32_t *table,
 int gg_orbit_search_create_cb(int N, int M, int degree, const int32_t *table,
                               const gg_search_cfg *cfg, const gg_table_callbacks *cb,
                               gg_orbit_search **out);
+/* Rows of different lengths, as annealingNR draws them (a row, then a slot
+ * among that row's entries; partners among the partner row's entries:
+ * cppcode/annealingNR.cpp:117,146-147): lens[r] in [1, degree] entries of row
+ * r are edges, the slots after them padding that must hold offset 0 (a
+ * self-loop, as cudaASPLconv.cu:199-201 pads short rows).  Row lengths never
+ * change.  Part form only; call before the first run. */
+int gg_orbit_search_set_row_lengths(gg_orbit_search *s, const int32_t *lens);
 void gg_orbit_search_destroy(gg_orbit_search *s);
 int gg_orbit_search_run(gg_orbit_search *s, int steps, gg_search_stats *out);
 /* Current (best = 0) or best (best = 1) table, M x degree offsets. */

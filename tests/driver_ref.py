@@ -156,8 +156,8 @@ def run(adj, evaluate, steps, mode_K=1, rule="anneal", T0=1e-3, T1=1e-6, total_s
 
 # ---- symmetric ("orbit") part graphs: cppcode/annealing.cpp:112-197 ----------------------
 
-def _partner_slot(t, N, row, frm, slot, diff):
-    for i in range(t.shape[1]):
+def _partner_slot(t, N, row, frm, slot, diff, n=None):
+    for i in range(t.shape[1] if n is None else n):
         if row == frm and i == slot:
             continue
         if (int(t[row, i]) + diff) % N == 0:
@@ -165,22 +165,26 @@ def _partner_slot(t, N, row, frm, slot, diff):
     return -1
 
 
-def _row_simple(t, N, r):
-    xs = [int(x) % N for x in t[r]]
+def _row_simple(t, N, r, n=None):
+    xs = [int(x) % N for x in t[r][:n]]
     return 0 not in xs and len(set(xs)) == len(xs)
 
 
-def propose_orbit(t, N, rng):
-    """One attempt; returns (move record, new table) or None (redraw)."""
+def propose_orbit(t, N, rng, lens=None):
+    """One attempt; returns (move record, new table) or None (redraw).  With
+    row lengths (annealingNR, cppcode/annealingNR.cpp:117,146-147) the slot is
+    drawn among the row's entries and partners among the partner row's."""
     M, d = t.shape
     g = N // M
+    ln = (lambda r: d) if lens is None else (lambda r: int(lens[r]))  # noqa: E731
     coin = rng.below(2)
     out = t.copy()
     if g > 1 and coin == 0:
-        frm, ifrm = rng.below(M), rng.below(d)
+        frm = rng.below(M)
+        ifrm = rng.below(ln(frm))
         diff = int(t[frm, ifrm])
         to = ((frm + diff) % N) % M
-        ito = _partner_slot(t, N, to, frm, ifrm, diff)
+        ito = _partner_slot(t, N, to, frm, ifrm, diff, ln(to))
         if ito < 0:
             return None
         if frm > to:
@@ -189,16 +193,18 @@ def propose_orbit(t, N, rng):
         nd = (to - frm) + M * k
         out[frm, ifrm] = nd
         out[to, ito] = -nd
-        if not (_row_simple(out, N, frm) and _row_simple(out, N, to)):
+        if not (_row_simple(out, N, frm, ln(frm)) and _row_simple(out, N, to, ln(to))):
             return None
         return (frm * d + ifrm, to * d + ito, k, 0, 0), out
-    a, ia = rng.below(M), rng.below(d)
-    c, ic = rng.below(M), rng.below(d)
+    a = rng.below(M)
+    ia = rng.below(ln(a))
+    c = rng.below(M)
+    ic = rng.below(ln(c))
     dab, dcd = int(t[a, ia]), int(t[c, ic])
     b = ((a + dab) % N) % M
     dd = ((c + dcd) % N) % M
-    ib = _partner_slot(t, N, b, a, ia, dab)
-    idd = _partner_slot(t, N, dd, c, ic, dcd)
+    ib = _partner_slot(t, N, b, a, ia, dab, ln(b))
+    idd = _partner_slot(t, N, dd, c, ic, dcd, ln(dd))
     if ib < 0 or idd < 0:
         return None
     if (c == a and ic == ia) or (c == b and ic == ib):
@@ -213,7 +219,7 @@ def propose_orbit(t, N, rng):
         b, c, ib, ic = c, b, ic, ib
     out[b, ib] = (c - b) + M * kb
     out[c, ic] = -((c - b) + M * kb)
-    if not all(_row_simple(out, N, r) for r in (a, dd, b, c)):
+    if not all(_row_simple(out, N, r, ln(r)) for r in (a, dd, b, c)):
         return None
     return rec, out
 
@@ -339,8 +345,41 @@ def random_lattice_table(Ns, Ms, d, seed):
     raise RuntimeError("no simple table found")
 
 
+def irregular_part(t, N, drop, seed):
+    """A symmetric part table whose rows have different lengths (annealingNR's
+    input): `drop` edge orbits of t removed (an entry and its reverse in the
+    partner row), each row's remaining entries moved to its front, padding
+    offset 0.  Returns (table, row lengths)."""
+    rng = np.random.default_rng(seed)
+    M, d = t.shape
+    rows = [[int(x) for x in r] for r in t]
+    removed = 0
+    while removed < drop:
+        r = int(rng.integers(M))
+        if len(rows[r]) <= 1:
+            continue
+        i = int(rng.integers(len(rows[r])))
+        x = rows[r][i]
+        p = ((r + x) % N) % M
+        js = [k for k in range(len(rows[p])) if (p != r or k != i) and (rows[p][k] + x) % N == 0]
+        if not js or len(rows[p]) <= 1:
+            continue
+        if p == r:
+            for k in sorted((i, js[0]), reverse=True):
+                del rows[r][k]
+        else:
+            del rows[r][i]
+            del rows[p][js[0]]
+        removed += 1
+    lens = np.array([len(r) for r in rows], dtype=np.int32)
+    out = np.zeros((M, d), dtype=np.int32)
+    for r, row in enumerate(rows):
+        out[r, :len(row)] = row
+    return out, lens
+
+
 def run_orbit(N, table, evaluate, steps, rule="hill", T0=1e-3, T1=1e-6, total_steps=None,
-              seed=1, propose=None):
+              seed=1, propose=None, lens=None):
     """evaluate(table) -> (total_scaled, diameter, ok[, wvc]).  Returns the
     per-step log rows {step, a*d+slot_a, c*d+slot_c, k1, k2, kind, total,
     diameter, status, accepted} and the final table."""
@@ -353,7 +392,7 @@ def run_orbit(N, table, evaluate, steps, rule="hill", T0=1e-3, T1=1e-6, total_st
     log = []
     for step in range(1, steps + 1):
         while True:
-            p = propose(t, rng) if propose else propose_orbit(t, N, rng)
+            p = propose(t, rng) if propose else propose_orbit(t, N, rng, lens)
             if p is not None:
                 break
         rec, new = p

@@ -222,3 +222,41 @@ def test_gg_anneal_wvc(gpu, tmp_path, extra):
         assert not r1["unconnected"]
         assert (r1["diameter"], r1["wvc"], r1["total_scaled"]) <= \
             (r0["diameter"], r0["wvc"], r0["total_scaled"])
+
+
+def test_gg_anneal_nr_rows_of_different_lengths(gpu, tmp_path):
+    """--nr (annealingNR): a symmetric part table whose rows have different
+    numbers of entries is searched with the rows' lengths kept; the output
+    table has the input's row lengths and the oracle's ASPL and diameter
+    match the printed best; without --nr the input is refused as
+    non-regular."""
+    import driver_ref
+    import oracle_lib as ol
+    gg = gpu
+    g, m, d = 16, 64, 8
+    N = g * m
+    tab, lens = driver_ref.irregular_part(gg.orbit_part(g, m, d, 5), N, 40, 3)
+    f, o = tmp_path / "nr.part", tmp_path / "out.part"
+    with open(f, "w") as fh:
+        fh.write(f"{N} {m}\n")
+        for r in range(m):
+            fh.write(" ".join(str(int(x)) for x in [lens[r]] + tab[r, :lens[r]].tolist()) + "\n")
+    p = subprocess.run([os.path.join(BIN, "gg-anneal"), "-c", "50", "-i", str(f)],
+                       capture_output=True, text=True, timeout=120)
+    assert p.returncode != 0 and "non-regular" in p.stdout
+    out = _run([os.path.join(BIN, "gg-anneal"), "--nr", "--hill", "-c", "400", "-s", "5",
+                "-i", str(f), "-o", str(o)])
+    toks = open(o).read().split()
+    assert (int(toks[0]), int(toks[1])) == (N, m)
+    k, rows = 2, []
+    for r in range(m):
+        E = int(toks[k])
+        assert E == lens[r]
+        rows.append([int(x) for x in toks[k + 1:k + 1 + E]] + [0] * (d - E))
+        k += 1 + E
+    best = np.array(rows, dtype=np.int32)
+    r1 = ol.part_eval(N, m, best)
+    r0 = ol.part_eval(N, m, tab)
+    assert not r1["unconnected"]
+    assert (r1["diameter"], r1["total_scaled"]) <= (r0["diameter"], r0["total_scaled"])
+    assert "total elapsed time" in out

@@ -38,6 +38,33 @@ def test_orbit_search_gpu_matches_cpu_twin(gpu, rule, steps):
     assert log[:, 9].sum() > 0  # the run moved
 
 
+def test_orbit_search_rows_of_different_lengths_gpu_matches_cpu_twin(gpu):
+    """annealingNR's rows of different lengths (cppcode/annealingNR.cpp:117,
+    146-147) on the g = 32, m = 128 orbit graph with 300 edge orbits removed:
+    the GPU evaluator sees short rows padded with offset 0 (self-loops, as
+    cudaASPLconv.cu:199-201 pads them) and agrees move for move with the CPU
+    oracle."""
+    import driver_ref
+    gg = gpu
+    g, m, d = 32, 128, 16
+    N = g * m
+    tab, lens = driver_ref.irregular_part(gg.orbit_part(g, m, d, 3), N, 300, 9)
+    steps = 300
+    cfg = gg.search_cfg(mode=gg.MODE_PLAIN, rule=gg.RULE_ANNEAL, total_steps=steps,
+                        T0=1e-3, T1=1e-6, seed=4)
+    with gg.OrbitSearch(N, tab, cfg, log=steps, row_lengths=lens) as s:
+        st = s.run(steps)
+        log, final = s.log(), s.table()
+    with gg.OrbitSearch(N, tab, cfg, log=steps, row_lengths=lens,
+                        table_eval=lambda t: oracle_eval.evaluate_part(N, t)) as c:
+        c.run(steps)
+        clog, cfinal = c.log(), c.table()
+    assert np.array_equal(log, clog) and np.array_equal(final, cfinal)
+    tot, dia, status = oracle_eval.evaluate_part(N, final)
+    assert status == 0 and tot == st.total_cur and dia == st.diam_cur
+    assert log[:, 9].sum() > 0
+
+
 def _lattice_eval(Ns, Ms):
     import oracle_lib as ol
 

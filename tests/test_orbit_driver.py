@@ -63,6 +63,55 @@ def test_orbit_driver_matches_python_restatement(gg, g, m, d, rule):
         assert (st.diam_cur, st.total_cur) <= (d0, t0)
 
 
+@pytest.mark.parametrize("rule", ["hill", "anneal"])
+def test_orbit_driver_rows_of_different_lengths(gg, rule):
+    """annealingNR's input: rows with different numbers of entries
+    (cppcode/annealingNR.cpp:117,146-147).  Slots are drawn among a row's
+    entries, the padding stays offset 0 and the per-step logs and final table
+    equal the restatement's."""
+    g, m, d = 8, 32, 6
+    N = g * m
+    tab, lens = driver_ref.irregular_part(gg.orbit_part(g, m, d, 21), N, 20, 5)
+    assert len(set(lens.tolist())) > 1
+    steps = 120
+    cfg = gg.search_cfg(mode=gg.MODE_PLAIN,
+                        rule=gg.RULE_HILL if rule == "hill" else gg.RULE_ANNEAL,
+                        total_steps=steps, T0=2e-2, T1=1e-4, seed=7)
+    ev = _tab_eval(N)
+    with gg.OrbitSearch(N, tab, cfg, table_eval=ev, log=steps, row_lengths=lens) as s:
+        st = s.run(steps)
+        log = s.log()
+        final = s.table()
+    ref_log, ref_tab = driver_ref.run_orbit(
+        N, tab, lambda t: (lambda r: (r[0], r[1], r[2] == 0))(ev(t)), steps, rule=rule,
+        T0=2e-2, T1=1e-4, seed=7, lens=lens)
+    assert np.array_equal(log, ref_log)
+    assert np.array_equal(final, ref_tab)
+    assert st.accepted == int(ref_log[:, 9].sum()) and st.accepted > 0
+    # every drawn slot lies inside its row; padding untouched; still symmetric
+    for a_slot in np.concatenate([log[:, 1], log[:, 2]]):
+        r, i = divmod(int(a_slot), d)
+        assert i < lens[r]
+    for r in range(m):
+        assert not final[r, lens[r]:].any()
+        xs = [int(x) % N for x in final[r, :lens[r]]]
+        assert 0 not in xs and len(set(xs)) == len(xs)
+        for x in xs:
+            p = (r + x) % N % m
+            assert (-x) % N in [int(y) % N for y in final[p, :lens[p]]]
+
+
+def test_row_lengths_are_validated(gg):
+    g, m, d = 8, 32, 6
+    N = g * m
+    tab = gg.orbit_part(g, m, d, 21)
+    cfg = gg.search_cfg(mode=gg.MODE_PLAIN, rule=gg.RULE_HILL, total_steps=10, seed=1)
+    ev = _tab_eval(N)
+    for bad in (np.full(m, 0), np.full(m, d + 1), np.full(m, d - 1)):  # last: padding not 0
+        with pytest.raises(gg.GGError):
+            gg.OrbitSearch(N, tab, cfg, table_eval=ev, row_lengths=bad.astype(np.int32))
+
+
 def ol_table_plain(gg, m, d):
     """A plain graph (g = 1) as a part table: offsets v -> u as (u - v)."""
     adj = gg.random_regular(m, d, 4)
