@@ -117,6 +117,23 @@ struct TopoBox {
 };
 
 // ---- kernel arguments ---------------------------------------------------------
+// Running totals of a fused single-candidate launch, updated with
+// device-scope atomics by each column workgroup as it finishes: the
+// workgroups spread over kFuseSlots slots of their own 64-byte line (fewer
+// atomics queue on one line), the arrivals on a line of their own.
+constexpr int kFuseSlots = 16;
+struct FuseSlot {
+    unsigned long long total, reached;
+    int ecc_levels;  // max of ecc * 256 + levels (levels follows ecc)
+    int flags;
+    int pad[10];
+};
+struct FuseAcc {
+    FuseSlot slot[kFuseSlots];
+    int arrived;
+    int pad[15];
+};
+
 struct DenseArgs {
     int N, M, D;            // vertices, sources, slots per row
     int W;                  // 64-source columns per candidate
@@ -149,6 +166,10 @@ struct DenseArgs {
     unsigned long long *ep_snap;
     long long *ep_reach;
     int ep_k0, ep_S;
+    // fused finalize (k_dense_lds, single candidate, no eccentricities or
+    // level counts asked for): the candidate's accumulators, zero between
+    // launches; nullptr = separate finalize kernel
+    FuseAcc *fuse;
 };
 
 struct SwapRegs {
@@ -279,18 +300,16 @@ __device__ __forceinline__ unsigned long long pad_mask(int w, int M) {
     return ~0ull << (M - base);
 }
 
-// Reduces the column parts of candidate k = blockIdx.x into its ResultRec
-// (and the WVC and per-level reach of a single-candidate launch).  A kernel
-// of its own: the launch boundary orders the parts' stores before these
-// loads, where an in-kernel last-workgroup reduction needs an agent-scope
-// release per workgroup (an L2 writeback per XCD each time).
+// Reduces the column parts of candidate k into its ResultRec (and the WVC
+// and per-level reach of a single-candidate launch), by the whole workgroup:
+// k_dense_finalize (one workgroup per candidate, after the BFS launch), or
+// the last column workgroup of a fused single-candidate BFS launch.
 template <int BS>
-__global__ void __launch_bounds__(BS) k_dense_finalize(DenseArgs a) {
+__device__ void finalize_candidate(const DenseArgs &a, int k) {
     __shared__ long long s_tot[BS / 64], s_rch[BS / 64];
     __shared__ int s_diam[BS / 64], s_lev[BS / 64], s_fl[BS / 64];
     __shared__ int s_cls[32];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const int k = blockIdx.x;
     if (tid < 32) s_cls[tid] = -1;
     __syncthreads();
     long long tot = 0, rch = 0;
@@ -360,6 +379,12 @@ __global__ void __launch_bounds__(BS) k_dense_finalize(DenseArgs a) {
             __syncthreads();
         }
     }
+}
+
+// The launch boundary orders the BFS kernel's parts before these loads.
+template <int BS>
+__global__ void __launch_bounds__(BS) k_dense_finalize(DenseArgs a) {
+    finalize_candidate<BS>(a, blockIdx.x);
 }
 
 // Where a column BFS stores its per-level words and counts: the words after
@@ -644,7 +669,14 @@ __device__ __forceinline__ ColPart dense_column(const Topo &topo, const DenseArg
     return p;
 }
 
-// One workgroup per (candidate, column); k_dense_finalize then reduces the parts.
+// One workgroup per (candidate, column); k_dense_finalize then reduces the
+// parts.  With a.fuse (one candidate) each workgroup instead adds its part
+// into its slot of the accumulators with device-scope atomics, waits until
+// they have completed, and only then counts itself in; the last one to
+// arrive takes the slots' sums with device-scope atomic exchanges (leaving
+// them zero for the next launch) and writes the result.  No release fence
+// (an L2 writeback per workgroup) is needed: every value crosses workgroups
+// through atomics.
 template <class Topo, int BS, int VPT, int DT>
 __global__ void __launch_bounds__(BS) k_dense_lds(Topo topo, DenseArgs a) {
     extern __shared__ unsigned long long vis[];
@@ -652,6 +684,48 @@ __global__ void __launch_bounds__(BS) k_dense_lds(Topo topo, DenseArgs a) {
     const int k = t / a.W, w = t - k * a.W;
     ColPart p = dense_column<Topo, BS, VPT, DT>(topo, a, k, w, vis, col_out(a, w));
     if (threadIdx.x == 0) a.parts[(size_t)k * a.W + w] = p;
+    if (a.fuse) {
+        __shared__ int s_last;
+        FuseAcc *f = a.fuse;
+        if (threadIdx.x == 0) {
+            FuseSlot *q = &f->slot[t % kFuseSlots];
+            atomicAdd(&q->total, (unsigned long long)p.total);
+            atomicAdd(&q->reached, (unsigned long long)p.reached);
+            atomicMax(&q->ecc_levels, p.ecc * 256 + p.levels);
+            if (p.flags) atomicOr(&q->flags, p.flags);
+            __builtin_amdgcn_s_waitcnt(0);  // the atomics above have completed
+            s_last = atomicAdd(&f->arrived, 1) == a.W - 1;
+        }
+        __syncthreads();
+        if (s_last && threadIdx.x < 64) {
+            const int i = threadIdx.x;
+            unsigned long long tot = 0, rch = 0;
+            int el = 0, fl = 0;
+            if (i < kFuseSlots) {
+                FuseSlot *q = &f->slot[i];
+                tot = atomicExch(&q->total, 0ull);
+                rch = atomicExch(&q->reached, 0ull);
+                el = atomicExch(&q->ecc_levels, 0);
+                fl = atomicExch(&q->flags, 0);
+            }
+            tot = (unsigned long long)wave_sum64((long long)tot);
+            rch = (unsigned long long)wave_sum64((long long)rch);
+            el = wave_max32(el);
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) fl |= __shfl_xor(fl, o, 64);
+            if (i == 0) {
+                atomicExch(&f->arrived, 0);
+                ResultRec r;
+                r.total = (long long)tot;
+                r.reached = (long long)rch;
+                r.diameter = el >> 8;
+                r.levels = el & 255;
+                r.flags = fl;
+                r.wvc = 0;
+                a.res[0] = r;
+            }
+        }
+    }
 }
 
 // Persistent workgroups draining a device-side task queue; parts only.

@@ -580,6 +580,11 @@ int run_dense(DenseCtx &c, const DenseArgs &a_in, hipStream_t st) {
     if (vpt == 0 && bs == 256) { bs = 1024; vpt = pick_vpt(bs, (a.N + bs - 1) / bs); }
     hipError_t e;
     if ((size_t)a.N * 8 <= (size_t)lds_cap && vpt > 0) {
+        // one candidate: its last column workgroup reduces the parts (no
+        // separate finalize launch); timing launches keep the BFS alone
+        if (c.fuse && a.K == 1 && !c.bfs_only && !a.ecc_out && !a.reach_col && !a.reach_out &&
+            !a.snap)
+            a.fuse = c.fuse;
         c.last_variant = bs == 256 ? 1 : 2;
         c.last_vpt = vpt;
         c.last_fixed = 0;
@@ -656,7 +661,7 @@ int run_dense(DenseCtx &c, const DenseArgs &a_in, hipStream_t st) {
             return GG_HIP_ERROR;
         }
     }
-    if (c.bfs_only) return GG_OK;
+    if (c.bfs_only || a.fuse) return GG_OK;
     hipLaunchKernelGGL(k_dense_finalize<1024>, dim3(a.K), dim3(1024), 0, st, a);
     e = hipGetLastError();
     if (e != hipSuccess) {
@@ -823,6 +828,8 @@ static int eval_alloc(gg_eval *ev, size_t tab_elem_bytes) {
         return GG_OOM;
     if (hipMalloc((void **)&ev->d_reach, sizeof(long long) * L) != hipSuccess) return GG_OOM;
     if (hipMalloc((void **)&ev->d_swap, sizeof(gg_swap)) != hipSuccess) return GG_OOM;
+    if (hipMalloc((void **)&ev->dc.fuse, sizeof(FuseAcc)) != hipSuccess) return GG_OOM;
+    GG_HIP(hipMemsetAsync(ev->dc.fuse, 0, sizeof(FuseAcc), ev->stream));
     if (hipHostMalloc((void **)&ev->h_swap, sizeof(gg_swap), hipHostMallocDefault) != hipSuccess)
         return GG_OOM;
     if (hipHostMalloc((void **)&ev->h_commit, sizeof(gg_swap), hipHostMallocDefault) != hipSuccess)
@@ -954,6 +961,7 @@ extern "C" void gg_destroy(gg_eval *ev) {
     (void)hipHostFree(ev->h_swap);
     (void)hipHostFree(ev->h_commit);
     if (ev->dc.gscratch) (void)hipFree(ev->dc.gscratch);
+    if (ev->dc.fuse) (void)hipFree(ev->dc.fuse);
     if (ev->ev0) (void)hipEventDestroy(ev->ev0);
     if (ev->ev1) (void)hipEventDestroy(ev->ev1);
     if (ev->stream) (void)hipStreamDestroy(ev->stream);

@@ -35,6 +35,8 @@ struct DenseCtx {
     int last_push_cap = 0;    // frontier list capacity of the last launch (0 = pull only)
     unsigned long long *gscratch = nullptr;
     size_t gscratch_bytes = 0;
+    FuseAcc *fuse = nullptr;  // accumulators (zero between launches): single-candidate LDS
+                              // launches without ecc / level outputs finalize in the BFS kernel
 };
 
 int run_dense(DenseCtx &c, const DenseArgs &a, hipStream_t st);
