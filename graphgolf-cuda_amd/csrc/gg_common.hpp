@@ -26,6 +26,9 @@ struct ResultRec {
     int levels;         // max levels executed by a column
     int wvc;
     int flags;          // bit 0: level limit hit
+    int seq;            // fused single evaluations: the launch's sequence number, stored
+                        // last (after the other fields are visible system-wide)
+    int pad;
 };
 
 #define GG_FLAG_LIMIT 1

@@ -170,6 +170,7 @@ struct DenseArgs {
     // level counts asked for): the candidate's accumulators, zero between
     // launches; nullptr = separate finalize kernel
     FuseAcc *fuse;
+    int seq;  // fused launches: written to res->seq after the result
 };
 
 struct SwapRegs {
@@ -715,14 +716,17 @@ __global__ void __launch_bounds__(BS) k_dense_lds(Topo topo, DenseArgs a) {
             for (int o = 32; o > 0; o >>= 1) fl |= __shfl_xor(fl, o, 64);
             if (i == 0) {
                 atomicExch(&f->arrived, 0);
-                ResultRec r;
-                r.total = (long long)tot;
-                r.reached = (long long)rch;
-                r.diameter = el >> 8;
-                r.levels = el & 255;
-                r.flags = fl;
-                r.wvc = 0;
-                a.res[0] = r;
+                // the fields first, then the sequence number a host may be
+                // polling (res can be pinned host memory)
+                ResultRec *r = a.res;
+                r->total = (long long)tot;
+                r->reached = (long long)rch;
+                r->diameter = el >> 8;
+                r->levels = el & 255;
+                r->flags = fl;
+                r->wvc = 0;
+                __threadfence_system();
+                __hip_atomic_store(&r->seq, a.seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
             }
         }
     }

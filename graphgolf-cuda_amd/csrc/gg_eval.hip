@@ -2,8 +2,10 @@
 //
 // Replaces cudaASPLconv / cudaASPLpiece / cudaASPLbox / cudaASPLgraph
 // (cudacode/*.cu).  One call = pack the table on the host, one async H2D, one
-// kernel launch (all levels, all columns, final reduction), one 48-byte D2H.
+// kernel launch (all levels, all columns, final reduction), one 40-byte D2H.
 #include <algorithm>
+#include <atomic>
+#include <climits>
 #include <cstring>
 #include <cstdlib>
 #include <fstream>
@@ -583,8 +585,10 @@ int run_dense(DenseCtx &c, const DenseArgs &a_in, hipStream_t st) {
         // one candidate: its last column workgroup reduces the parts (no
         // separate finalize launch); timing launches keep the BFS alone
         if (c.fuse && a.K == 1 && !c.bfs_only && !a.ecc_out && !a.reach_col && !a.reach_out &&
-            !a.snap)
+            !a.snap) {
             a.fuse = c.fuse;
+            a.seq = c.seq;
+        }
         c.last_variant = bs == 256 ? 1 : 2;
         c.last_vpt = vpt;
         c.last_fixed = 0;
@@ -661,6 +665,7 @@ int run_dense(DenseCtx &c, const DenseArgs &a_in, hipStream_t st) {
             return GG_HIP_ERROR;
         }
     }
+    c.last_fused = a.fuse ? 1 : 0;
     if (c.bfs_only || a.fuse) return GG_OK;
     hipLaunchKernelGGL(k_dense_finalize<1024>, dim3(a.K), dim3(1024), 0, st, a);
     e = hipGetLastError();
@@ -777,6 +782,7 @@ struct gg_eval {
     gg_swap *h_swap = nullptr;  // pinned
     gg_swap *h_commit = nullptr;  // pinned: the swap a queued commit kernel reads
     int commit_pending = 0;       // a commit kernel may not have run yet
+    int seq = 0;                  // sequence number of the last polled evaluation
     std::vector<int32_t> h_adj;  // adjacency evaluators: host copy of the resident base
     bool base_sym = false;       // the resident base is undirected
     int have_base = 0;  // adjacency evaluators: a table is resident
@@ -823,6 +829,7 @@ static int eval_alloc(gg_eval *ev, size_t tab_elem_bytes) {
     if (hipMalloc((void **)&ev->d_res, sizeof(ResultRec)) != hipSuccess) return GG_OOM;
     if (hipHostMalloc((void **)&ev->h_res, sizeof(ResultRec), hipHostMallocDefault) != hipSuccess)
         return GG_OOM;
+    memset(ev->h_res, 0, sizeof(ResultRec));  // seq 0: no evaluation polled yet
     if (hipMalloc((void **)&ev->d_ecc, sizeof(int) * ev->M) != hipSuccess) return GG_OOM;
     if (hipMalloc((void **)&ev->d_reach_col, sizeof(long long) * (size_t)ev->W * L) != hipSuccess)
         return GG_OOM;
@@ -1169,6 +1176,27 @@ static int check_swap(const gg_eval *ev, const gg_swap *sw) {
 // dropped by any option change.  Only where the column bitset fits in LDS
 // (the persistent global and level-synchronous kernels allocate scratch or
 // read counters back between launches).
+// Waits until the pinned result carries sequence number `seq` (the launch
+// that writes it, and everything queued before it on `st`, has then
+// finished its work); the stream's status is checked every few thousand
+// polls so that a failed launch is reported instead of waited on forever.
+static hipError_t poll_result(const ResultRec *h_res, int seq, hipStream_t st) {
+    const volatile int *flag = &h_res->seq;
+    for (unsigned it = 1;; it++) {
+        if (*flag == seq) break;
+        if ((it & 4095) == 0) {
+            const hipError_t q = hipStreamQuery(st);
+            if (q == hipSuccess) {
+                if (*flag == seq) break;
+                return hipErrorUnknown;  // the stream drained without the mark
+            }
+            if (q != hipErrorNotReady) return q;
+        }
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+    return hipSuccess;
+}
+
 static int eval_swap_graph(gg_eval *ev, gg_result *out) {
     const int key = ev->dc.push_ok ? 1 : 0;
     DenseArgs a{};
@@ -1182,18 +1210,28 @@ static int eval_swap_graph(gg_eval *ev, gg_result *out) {
     a.reach_col = ev->levels_on ? ev->d_reach_col : nullptr;
     a.reach_out = ev->levels_on ? ev->d_reach : nullptr;
     a.want_wvc = ev->wvc_on;
+    bool polled = false;
     if (ev->swap_graph == 2) {
-        // the same two kernels launched one by one (no graph)
+        // the same kernels launched one by one (no graph); a BFS launch that
+        // finalizes in its last workgroup marks the pinned result with its
+        // sequence number, which the host polls instead of waiting for the
+        // stream
         bind_topology(ev);
         DenseCtx &c = ev->dc;
         hipEvent_t done = c.bfs_done;
         c.bfs_done = nullptr;
         const int wide = c.wide_ok;
         c.wide_ok = 0;
+        ev->seq = ev->seq == INT_MAX ? 1 : ev->seq + 1;
+        c.seq = ev->seq;
         const int st = run_dense(c, a, ev->stream);
         c.bfs_done = done;
         c.wide_ok = wide;
         if (st != GG_OK) return st;
+        if (c.last_fused) {
+            GG_HIP(poll_result(ev->h_res, ev->seq, ev->stream));
+            polled = true;
+        }
     } else if (!ev->sg_exec[key]) {
         bind_topology(ev);
         DenseCtx &c = ev->dc;
@@ -1216,7 +1254,7 @@ static int eval_swap_graph(gg_eval *ev, gg_result *out) {
         GG_HIP(hipGraphInstantiate(&ev->sg_exec[key], g, nullptr, nullptr, 0));
     }
     if (ev->swap_graph == 1) GG_HIP(hipGraphLaunch(ev->sg_exec[key], ev->stream));
-    GG_HIP(hipStreamSynchronize(ev->stream));
+    if (!polled) GG_HIP(hipStreamSynchronize(ev->stream));
     ev->commit_pending = 0;
     fill_result(ev, *ev->h_res, out);
     ev->last_a = a;

@@ -37,6 +37,8 @@ struct DenseCtx {
     size_t gscratch_bytes = 0;
     FuseAcc *fuse = nullptr;  // accumulators (zero between launches): single-candidate LDS
                               // launches without ecc / level outputs finalize in the BFS kernel
+    int seq = 0;              // sequence number the next fused launch writes with its result
+    int last_fused = 0;       // the last launch finalized in its BFS kernel
 };
 
 int run_dense(DenseCtx &c, const DenseArgs &a, hipStream_t st);

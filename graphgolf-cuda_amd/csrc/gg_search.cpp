@@ -39,13 +39,13 @@ struct PlainBackend : Backend {
     gg_eval *ev = nullptr;
     std::vector<gg_swap> last_sw;     // candidates of the last eval and their results:
     std::vector<gg_result> last_res;  // committing one of them needs no new evaluation
-    // One exact evaluation per move as two kernel launches (BFS + finalize,
-    // swap read from and result written to pinned memory: no copies, no
-    // events; at n = 16384, d = 24 this measured 6 us per move below the
-    // same two kernels replayed as a HIP graph), without the per-source
-    // eccentricities and level counts the search never reads; WVC (and the
-    // level limit of WVCdiameterASPL) under the WVC rules.  Commits are
-    // queued, not waited for.
+    // One exact evaluation per move as one BFS kernel launch that finalizes
+    // in its last workgroup (swap read from and result written to pinned
+    // memory, the result's sequence number polled: no copies, no events, no
+    // stream wait), without the per-source eccentricities and level counts
+    // the search never reads; WVC (and the level limit of WVCdiameterASPL)
+    // under the WVC rules, which add a finalize kernel and a stream wait.
+    // Commits are queued, not waited for.
     PlainBackend(int N, int d, int device, bool wvc, int *st) {
         *st = gg_create_adj(N, d, device, &ev);
         if (*st == GG_OK) *st = gg_eval_set_option(ev, GG_OPT_SWAP_GRAPH, 2);
