@@ -170,7 +170,8 @@ def plain_kernel_roofline(gg, adj, reps=30):
     p8 = gg.probe_lds_read(8)
     peak = LDS_PEAK_GBS
     return {"bound": "lds", "achieved": ach, "peak": peak, "unit": "GB/s",
-            "frac": ach / peak, "traffic": hbm_traffic(n, d), "kernel": name,
+            "frac": ach / peak, "traffic": hbm_traffic(n, d), "lds_counters": lds_counters(n, d),
+            "kernel": name,
             "kernel_ms": t * 1e3, "kernel_ms_single_bracket": float(np.mean(single[3:])),
             "levels": L, "alg_bytes_per_launch": alg,
             "push_cap": info["push_cap"],
@@ -184,23 +185,58 @@ def plain_kernel_roofline(gg, adj, reps=30):
             "alg_rate_over_hbm_peak": ach / HBM_PEAK_GBS}
 
 
-def hbm_traffic(n, d):
-    """HBM bytes per launch of the dense kernel from the committed rocprofv3
-    --pmc summary for this shape (FETCH_SIZE + WRITE_SIZE, scaled by the
-    copy-kernel calibration recorded beside them), or None."""
-    path = os.path.join(REPO, "profiles", f"pmc_k_dense_lds_n{n}_d{d}.txt")
-    if not os.path.exists(path):
-        return None
+def pmc_summary(name):
+    """The `key: value` figures of a committed rocprofv3 --pmc summary under
+    profiles/ (counters scaled by the copy-kernel calibration recorded in
+    it), or {} when there is none."""
+    path = os.path.join(REPO, "profiles", name)
     vals = {}
-    with open(path) as f:
-        for line in f:
-            if ":" in line and not line.startswith("#"):
-                k, v = line.split(":", 1)
-                vals[k.strip()] = v.strip()
-    try:
-        return float(vals["hbm_bytes_per_launch"])
-    except (KeyError, ValueError):
+    if os.path.exists(path):
+        with open(path) as f:
+            for line in f:
+                if ":" in line and not line.startswith("#"):
+                    k, v = line.split(":", 1)
+                    try:
+                        vals[k.strip()] = float(v)
+                    except ValueError:
+                        pass
+    return vals
+
+
+def hbm_traffic(n, d):
+    """HBM bytes per launch of the dense kernel (FETCH_SIZE + WRITE_SIZE) from
+    the committed summary for this shape, or None."""
+    return pmc_summary(f"pmc_k_dense_lds_n{n}_d{d}.txt").get("hbm_bytes_per_launch")
+
+
+def lds_counters(n, d):
+    """LDS instructions and cycles per launch of the dense kernel from the
+    committed summary, or None."""
+    v = pmc_summary(f"pmc_k_dense_lds_n{n}_d{d}.txt")
+    if "lds_insts_per_launch" not in v:
         return None
+    return {"lds_insts_per_launch": v["lds_insts_per_launch"],
+            "bank_conflict_cycles": v["lds_bank_conflict_cycles_per_launch"],
+            "lds_active_cycles": v["lds_active_cycles_per_launch"],
+            "bank_conflict_frac_of_active": v["lds_bank_conflict_cycles_per_launch"]
+            / v["lds_active_cycles_per_launch"]}
+
+
+def sparse_roofline(n, d, K):
+    """HBM roofline of the incremental kernel at the learned search's shape
+    from its committed counter summary: bytes fetched and written per launch
+    over the kernel time of the same profiled command."""
+    v = pmc_summary(f"pmc_k_sparse_n{n}_d{d}_k{K}.txt")
+    if "hbm_bytes_per_launch" not in v:
+        return None
+    t = v["kernel_us"] * 1e-6
+    ach = v["hbm_bytes_per_launch"] / t / 1e9
+    return {"bound": "hbm", "kernel": "k_sparse<1024, 24, 1>", "achieved": ach,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+            "traffic": v["hbm_bytes_per_launch"], "kernel_ms": v["kernel_us"] * 1e-3,
+            "lds_insts_per_launch": v.get("lds_insts_per_launch"),
+            "source": f"profiles/pmc_k_sparse_n{n}_d{d}_k{K}.txt (FETCH_SIZE + WRITE_SIZE and "
+                      "kernel time of bench.py --only sparse under rocprofv3)"}
 
 
 def copy_calibration(nbytes=1 << 30, reps=5):
@@ -273,6 +309,34 @@ def batched_vs_sequential(gg, n=8192, d=16, K=256, steps=10, seed=4):
                          "base": tune["base"], "best_base": tune["best_base"]}}
 
 
+def sparse_steps(gg, adj, K, steps=20, seed=13):
+    """Incremental steps of K random 2-opt candidates on the start graph, each
+    launched without a HIP graph (so a counter run can attribute every
+    kernel), the results checked against the dense path once."""
+    n, d = adj.shape
+    rng = np.random.default_rng(seed)
+    cands = []
+    while len(cands) < K:
+        a, c = (int(x) for x in rng.integers(0, n, 2))
+        ia, ic = (int(x) for x in rng.integers(0, d, 2))
+        sw, _ = gg.make_swap(adj, a, ia, c, ic, int(rng.integers(0, 2)))
+        if sw is not None:
+            cands.append(sw)
+    with gg.BatchEvaluator(n, d, max_k=K) as be:
+        be.set_base(adj)
+        be.set_option(gg.GG_BATCH_OPT_GRAPH, 0)
+        r = be.eval(cands)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            r = be.eval(cands)
+        ms = (time.perf_counter() - t0) / steps * 1e3
+        be.set_option(gg.GG_BATCH_OPT_PATH, gg.PATH_DENSE)
+        rd = be.eval(cands)
+    assert [(x.total_reps, x.diameter) for x in r] == [(x.total_reps, x.diameter) for x in rd]
+    return {"workload": f"K={K} random 2-opt candidates, random {d}-regular graph n={n}",
+            "ms_per_step": ms, "steps": steps}
+
+
 def tune_step_shape(gg, adj, K, seed=11):
     """Autotunes the incremental step's launch shape for (n, d, K) on K random
     2-opt candidates of the search's start graph; the cached choice applies
@@ -329,7 +393,13 @@ def large_n_orbit(gg, g=8192, m=128, d=16, seed=3, reps=5):
         alg = 8 * (d + 2) * N * W * r.levels
         out.update({"diameter": r.diameter, "levels": r.levels, "total_scaled": r.total_scaled,
                     "level_sync_ms_per_eval": wide_ms, "level_sync_bfs_ms": bfs_ms,
-                    "level_sync_alg_GB_per_s": alg / (bfs_ms * 1e-3) / 1e9})
+                    "level_sync_alg_GB_per_s": alg / (bfs_ms * 1e-3) / 1e9,
+                    "level_sync_alg_frac_of_hbm_peak": alg / (bfs_ms * 1e-3) / 1e9 / HBM_PEAK_GBS})
+        v = pmc_summary("pmc_k_wide_level_n1048576_orbit_n131072_adjacency_d16.txt")
+        if N == 1 << 20 and "orbit_fetch_bytes_per_eval" in v:
+            miss = v["orbit_fetch_bytes_per_eval"] + v["orbit_write_bytes_per_eval"]
+            out.update({"l2_miss_bytes_per_eval": miss, "l2_hit_rate": v["orbit_tcc_hit_rate"],
+                        "l2_miss_GB_per_s": miss / (bfs_ms * 1e-3) / 1e9})
         ev.set_option(gg.GG_OPT_WIDE, 0)
         t0 = time.perf_counter()
         r3 = ev.eval(tab)
@@ -405,12 +475,16 @@ def main():
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the automatic launch shape of the batched step")
-    ap.add_argument("--only", choices=["roofline", "copy", "learned", "plain", "large"],
+    ap.add_argument("--only", choices=["roofline", "copy", "learned", "plain", "large",
+                                       "sparse", "batched"],
                     default=None,
                     help="run only the dense-kernel roofline launches, only a device copy of "
-                         "known size (the two legs of the rocprofv3 --pmc traffic runs), or "
+                         "known size (the two legs of the rocprofv3 --pmc traffic runs), "
                          "only the learned or the plain search (for kernel profiles of their "
-                         "steps), or only the large-N orbit evaluation")
+                         "steps), only the large-N evaluations, only incremental steps of K "
+                         "random candidates at the learned search's shape launched without "
+                         "a graph (the incremental kernel's counter runs), or only the "
+                         "K = 256 batched step")
     ap.add_argument("--equal-time", type=float, default=10.0,
                     help="seconds each of plain and learned search from the same graph for "
                          "the equal-wall-time ASPL comparison (0 skips it)")
@@ -431,7 +505,13 @@ def main():
         print(json.dumps({"large_n": large_n_orbit(gg), "large_n_plain": large_n_plain(gg)}),
               flush=True)
         return
+    if args.only == "batched":
+        print(json.dumps({"batched_vs_sequential": batched_vs_sequential(gg)}), flush=True)
+        return
     adj = gg.random_regular(n, d, args.seed)
+    if args.only == "sparse":
+        print(json.dumps({"sparse_steps": sparse_steps(gg, adj, args.K)}), flush=True)
+        return
     if args.only == "roofline":
         print(json.dumps({"roofline": plain_kernel_roofline(gg, adj)}), flush=True)
         return
@@ -453,12 +533,14 @@ def main():
 
     plain_cfg = gg.search_cfg(mode=gg.MODE_PLAIN, total_steps=(args.warmup + args.steps) * K,
                               T0=T0, T1=T1, seed=12345)
+    # the plain search times as many exact evaluations as the learned one
+    # (steps * K moves, after warmup * K untimed ones)
     if args.only == "plain":
-        sp, _, _, ep, tp = run_search(gg, adj, plain_cfg, args.warmup, args.steps)
-        print(json.dumps({"plain_evals_per_s": ep / tp, "ms_per_step": tp / args.steps * 1e3}),
-              flush=True)
+        sp, _, _, ep, tp = run_search(gg, adj, plain_cfg, args.warmup * K, args.steps * K)
+        print(json.dumps({"plain_evals_per_s": ep / tp, "ms_per_move": tp / ep * 1e3,
+                          "moves_timed": ep}), flush=True)
         return
-    sp, _, stp, ep, tp = run_search(gg, adj, plain_cfg, args.warmup, args.steps)
+    sp, _, stp, ep, tp = run_search(gg, adj, plain_cfg, args.warmup * K, args.steps * K)
     plain_rate = ep / tp
     sp.close()
 
@@ -485,7 +567,7 @@ def main():
         "learned_warmup_steps": learn_warm,
         "step_shape_autotune": tune,
         "plain": {"value": plain_rate, "unit": "exact evaluations/s",
-                  "ms_per_step": tp / args.steps * 1e3},
+                  "moves_timed": ep, "ms_per_step": tp / ep * 1e3},
         "speedup_vs_plain": learn_rate / plain_rate,
         "scorer": {"spearman": rho, "top_k_miss_rate": miss, "audits": len(aud),
                    "uniform_top_k_miss_rate": uniform_miss,
@@ -507,6 +589,7 @@ def main():
         out["plain"]["kernel_bound_value"] = 1e3 / kms
         out["plain"]["ms_per_step_over_kernel_ms"] = out["plain"]["ms_per_step"] / kms
         out["speedup_vs_plain_kernel"] = learn_rate / (1e3 / kms)
+    out["roofline_sparse"] = sparse_roofline(n, d, K)
     out["batched_vs_sequential"] = batched_vs_sequential(gg)
     out["large_n"] = large_n_orbit(gg)
     out["large_n_plain"] = large_n_plain(gg)
