@@ -228,18 +228,30 @@ __device__ int ss_bfs(const int *adjR, int N, int D, int lmax, int S, int x,
         // the lanes of a wave take consecutive entries
         const int wbase = tid & ~63, lane = tid & 63;
         if (td) {
-            if constexpr (DT > 0) {
-                // a frontier vertex per lane, its row in one vector load
-                for (int e0 = wbase; e0 < fsize; e0 += BS) {
-                    const int e = e0 + lane;
-                    const bool ok = e < fsize;
-                    CandRow<DT> row;
-                    if (ok) row.load(adjR, lin[e], sw);
+            // a frontier whose edges fit one pass of the workgroup: a lane
+            // per (vertex, slot) item; a larger one (fixed degrees): a lane
+            // per four consecutive slots of a frontier vertex's row (one
+            // 16-byte load)
+            if (DT > 0 && fsize * D > BS) {
+                constexpr int Q = DT > 0 ? DT / 4 : 1;
+                const int items = fsize * Q;
+                for (int i0 = wbase; i0 < items; i0 += BS) {
+                    const int it = i0 + lane;
+                    const bool ok = it < items;
+                    int y4[4] = {0, 0, 0, 0};
+                    if (ok) {
+                        const int e = it / Q, q = it - e * Q;
+                        const int v = lin[e];
+                        const int4 x = reinterpret_cast<const int4 *>(adjR + (size_t)v * D)[q];
+                        y4[0] = x.x; y4[1] = x.y; y4[2] = x.z; y4[3] = x.w;
 #pragma unroll
-                    for (int i = 0; i < DT; i++) {
-                        const int y = ok ? row.u[i] : 0;
-                        append(ok && claim(y), y, true);
+                        for (int i = 0; i < 4; i++)
+#pragma unroll
+                            for (int j = 0; j < 4; j++)
+                                if (sw.v[j] == v && sw.s[j] == 4 * q + i) y4[i] = sw.n[j];
                     }
+#pragma unroll
+                    for (int i = 0; i < 4; i++) append(ok && claim(y4[i]), y4[i], true);
                 }
             } else {
                 const int items = fsize * D;
@@ -357,7 +369,8 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a) {
     __shared__ int s_diff[GG_SPARSE_MAX_K];
     __shared__ unsigned long long s_dc[GG_SPARSE_MAX_K];
     __shared__ unsigned long long s_lost[GG_SPARSE_MAX_K];  // bits lost on Delta_{l-1}
-    __shared__ int s_ended[GG_SPARSE_MAX_K];      // task ended last level: clear its bitmaps
+    __shared__ unsigned long long s_endmask;      // tasks that ended last level: clear their bitmaps
+    __shared__ unsigned long long s_maskset;      // tasks with masked endpoint sources
     __shared__ unsigned long long s_mask[GG_SPARSE_MAX_K];  // endpoint sources' bits (masked)
     __shared__ int s_mcnt[GG_SPARSE_MAX_K];       // base pairs of the masked bits in B_{l-1}
     __shared__ int s_nmasked;
@@ -385,7 +398,7 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a) {
     for (int k = tid; k < K; k += BS) s_sw[k] = a.swaps[kc + k];
     auto gt = [&](int k) { return (size_t)(kc + k) * W + w; };       // global (cand, col)
     auto lt = [&](int k) { return (size_t)(kb + k) * W + w; };       // task slot of this launch
-    if (tid == 0) s_nmasked = 0;
+    if (tid == 0) { s_nmasked = 0; s_endmask = 0; s_maskset = 0; }
     __syncthreads();
     if (has_col) {
 
@@ -397,7 +410,10 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a) {
         const unsigned long long em = endpoint_mask(s, w);
         const bool dense = TAIL == 0 && em != 0;
         s_mask[k] = TAIL ? em : 0ull;
-        if (TAIL && em) atomicAdd(&s_nmasked, 1);
+        if (TAIL && em) {
+            atomicAdd(&s_nmasked, 1);
+            atomicOr(&s_maskset, 1ull << k);
+        }
         s_cnt[0][k] = 0;
         s_cnt[1][k] = 0;
         a.task_flags[gt(k)] = 0;
@@ -408,16 +424,14 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a) {
             int q = atomicAdd(a.queue_count, 1);
             a.queue[q] = make_int2(kc + k, w);
         }
-        s_ended[k] = 0;
     }
     __threadfence_block();
     __syncthreads();
     // Bitmaps are all zero between launches: a task clears both of its
     // bitmaps at the level after it ends (and after the last level).
     auto clear_ended = [&]() {
-        for (int k = 0; k < K; k++) {
-            if (!s_ended[k]) continue;
-            unsigned *b0 = a.dl_bits + lt(k) * 2 * NB;
+        for (unsigned long long m = s_endmask; m; m &= m - 1) {
+            unsigned *b0 = a.dl_bits + lt(__builtin_ctzll(m)) * 2 * NB;
             for (int i = tid; i < 2 * NB; i += BS) b0[i] = 0u;
         }
     };
@@ -460,13 +474,14 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a) {
         if (tid == 0) { s_next = 0; s_any = 0; }
         clear_ended();
         __syncthreads();
-        for (int k = tid; k < K; k += BS) s_ended[k] = 0;
+        if (tid == 0) s_endmask = 0;
         if (s_nmasked) {
             // base pairs of the masked endpoint bits after level l-1: what the
             // finalize replaces with the single-source counts
-            for (int k = 0; k < K; k++) {
+            for (unsigned long long mm = s_maskset; mm; mm &= mm - 1) {
+                const int k = __builtin_ctzll(mm);
+                if (!s_active[k]) continue;
                 const unsigned long long m = s_mask[k];
-                if (!m || !s_active[k]) continue;
                 int c = 0;
                 for (int v = tid; v < N; v += BS) c += __popcll(Bprev[v] & m);
 #pragma unroll
@@ -662,7 +677,7 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a) {
                 a.task_flags[g] = GG_FLAG_QUEUED;
                 int q = atomicAdd(a.queue_count, 1);
                 a.queue[q] = make_int2(kc + k, w);
-                s_ended[k] = 1;
+                atomicOr(&s_endmask, 1ull << k);
                 continue;
             }
             const int n_cur = s_ncur[k];
@@ -677,7 +692,7 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a) {
                 a.task_state[g] = GG_TASK_DONE;
                 a.task_end[g] = l;
                 if (!fixed) a.task_flags[g] = GG_FLAG_LIMIT;
-                s_ended[k] = 1;
+                atomicOr(&s_endmask, 1ull << k);
             } else {
                 atomicOr(&s_any, 1);
             }
