@@ -426,8 +426,14 @@ static int enqueue_sparse(gg_batch *b, int K, bool reset = true) {
     if (grid > K * W) grid = K * W;
     int st = run_dense_queue(b->dc, a, grid, b->stream);
     if (st != GG_OK) return st;
-    hipLaunchKernelGGL(k_sparse_finalize<256>, dim3(K), dim3(256), 0, b->stream, s,
-                       N - 64 * (W - 1));
+    if (b->fin_args) {
+        // the caller launches the finalize (fused with its own epilogue)
+        *b->fin_args = s;
+        b->fin_nvalid = N - 64 * (W - 1);
+    } else {
+        hipLaunchKernelGGL(k_sparse_finalize<256>, dim3(K), dim3(256), 0, b->stream, s,
+                           N - 64 * (W - 1));
+    }
     if (reset)
         hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, b->stream, b->d_qctl, b->nqctl);
     GG_HIP(hipGetLastError());
@@ -503,13 +509,17 @@ namespace gg {
 // and swaps to the host buffers.
 // The caller reads the results and swaps back (batch_out) and resets the
 // queue counters after the step.
-int batch_enqueue_dev(gg_batch *b, int K) {
+int batch_enqueue_dev(gg_batch *b, int K, SparseArgs *fin, int *fin_nvalid) {
     if (!b->have_base || !b->sparse_ok || b->mode_opt == 1 || K < 1 || K > b->max_k ||
         !b->base_sym)
         return GG_UNSUPPORTED;
     // swaps drawn as 2-opt rewirings of an undirected base keep it undirected
     b->dc.push_ok = b->base_sym;
-    return enqueue_sparse(b, K, false);
+    b->fin_args = fin;
+    const int st = enqueue_sparse(b, K, false);
+    b->fin_args = nullptr;
+    if (fin_nvalid) *fin_nvalid = b->fin_nvalid;
+    return st;
 }
 
 BatchOut batch_out(gg_batch *b) {
@@ -568,7 +578,7 @@ __global__ void k_batch_commit(uint16_t *tab16, uint32_t *tab32, int *adjR, int 
 struct CommitArgs {
     int N, W, S, L1, cap, D;
     int k, gk;                       // candidate: index in the logs' launch, index in the batch
-    gg_swap sw;
+    const gg_swap *swaps;            // the batch's candidates (device): swaps[gk] is committed
     unsigned long long *snap;        // [S][W][N]
     unsigned long long *rows;        // [RL][N][W] row-major levels 1..RL (null: none)
     int RL;
@@ -589,34 +599,55 @@ struct CommitArgs {
     uint16_t *tab16;                 // resident tables: the swap's slots are rewritten
     uint32_t *tab32;
     int *adjR;
+    const CommitSel *sel;            // non-null: the candidate is read from here at launch
+                                     // (k and gk above ignored; k < 0 commits nothing)
 };
+
+// The candidate being committed: the launch's own (k, gk), or the one a
+// device-side decision wrote to *sel (false: it chose none).  Its swap is read
+// from the batch's candidate array in device memory.
+struct CommitPick {
+    int k, gk;
+    const gg_swap *sw;
+};
+__device__ __forceinline__ bool commit_pick(const CommitArgs &c, CommitPick &p) {
+    p.k = c.k;
+    p.gk = c.gk;
+    if (c.sel) {
+        p.k = p.gk = c.sel->k;
+        if (p.k < 0) return false;
+    }
+    p.sw = c.swaps + p.gk;
+    return true;
+}
 
 // Column w's word after level l of the committed candidate, for a column
 // holding endpoint sources (bits m, run masked): the masked word x with each
 // endpoint's bit from its single-source visited bitmap.
-__device__ __forceinline__ unsigned long long commit_unmask(const CommitArgs &c, int w, int l,
+__device__ __forceinline__ unsigned long long commit_unmask(const CommitArgs &c,
+                                                            const CommitPick &p, int w, int l,
                                                             unsigned long long m, int v,
                                                             unsigned long long x) {
     x &= ~m;
     for (int j = 0; j < 4; j++) {
-        if (!endpoint_first(c.sw, j) || (c.sw.v[j] >> 6) != w) continue;
-        const unsigned *vis = c.ss_vis + ((size_t)(4 * c.k + j) * c.S + (l - 1)) * c.NB;
-        if ((vis[v >> 5] >> (v & 31)) & 1u) x |= 1ull << (c.sw.v[j] & 63);
+        if (!endpoint_first(*p.sw, j) || (p.sw->v[j] >> 6) != w) continue;
+        const unsigned *vis = c.ss_vis + ((size_t)(4 * p.k + j) * c.S + (l - 1)) * c.NB;
+        if ((vis[v >> 5] >> (v & 31)) & 1u) x |= 1ull << (p.sw->v[j] & 63);
     }
     return x;
 }
 
 // Endpoint task (4k + j) of the dense tail that ran column w: the first
 // endpoint of the swap in that column.
-__device__ __forceinline__ int commit_ep_slot(const CommitArgs &c, int w) {
+__device__ __forceinline__ int commit_ep_slot(const CommitArgs &c, const CommitPick &p, int w) {
     for (int j = 0; j < 4; j++)
-        if (c.sw.v[j] >= 0 && (c.sw.v[j] >> 6) == w) return 4 * c.k + j;
+        if (p.sw->v[j] >= 0 && (p.sw->v[j] >> 6) == w) return 4 * p.k + j;
     return -1;
 }
 
 // New eccentricity of the candidate's column w (recorded by the finalize).
-__device__ __forceinline__ int commit_col_ecc(const CommitArgs &c, int w) {
-    return c.task_ecc[(size_t)c.gk * c.W + w];
+__device__ __forceinline__ int commit_col_ecc(const CommitArgs &c, const CommitPick &p, int w) {
+    return c.task_ecc[(size_t)p.gk * c.W + w];
 }
 
 // Grid (W + 4 * chunks, S) with chunks = ceil(N / 1024): workgroup (w, l-1),
@@ -629,20 +660,20 @@ __device__ __forceinline__ int commit_col_ecc(const CommitArgs &c, int w) {
 // holds them, the stored word elsewhere) and written to the stored level and
 // its row-major copy (full words at every level) where it changes.  One
 // workgroup per chunk: the chunks are disjoint, so no word is written twice.
-__device__ void commit_masked_chunk(const CommitArgs &c, int w, int l, int v0,
-                                    unsigned long long m) {
+__device__ void commit_masked_chunk(const CommitArgs &c, const CommitPick &p, int w, int l,
+                                    int v0, unsigned long long m) {
     __shared__ unsigned long long s_dw[1024];
     __shared__ unsigned s_dm[32];
     const int tid = threadIdx.x, N = c.N;
-    const size_t t = (size_t)c.gk * c.W + w;
-    const int ecc_new = commit_col_ecc(c, w), end = c.task_end[t];
+    const size_t t = (size_t)p.gk * c.W + w;
+    const int ecc_new = commit_col_ecc(c, p, w), end = c.task_end[t];
     const bool store = l < ecc_new, full = l >= c.base_parts[w].ecc;
     const bool rw = c.rows && l <= c.RL;
     if (!store && !rw) return;  // nothing kept at this level
     if (tid < 32) s_dm[tid] = 0u;
     __syncthreads();
     if (l <= end) {
-        const size_t lt = ((size_t)c.k * c.W + w) * c.S + (l - 1);
+        const size_t lt = ((size_t)p.k * c.W + w) * c.S + (l - 1);
         const int *key = c.lg_key + lt * c.cap;
         const unsigned long long *word = c.lg_word + lt * c.cap;
         const int n = c.lg_n[lt];
@@ -663,7 +694,7 @@ __device__ void commit_masked_chunk(const CommitArgs &c, int w, int l, int v0,
     unsigned long long x = ~0ull;
     if (store) {
         const bool logged = (s_dm[tid >> 5] >> (tid & 31)) & 1u;
-        x = commit_unmask(c, w, l, m, v, logged ? s_dw[tid] : o);
+        x = commit_unmask(c, p, w, l, m, v, logged ? s_dw[tid] : o);
         if (full || x != o) dst[v] = x;
     }
     if (rw && x != o) c.rows[((size_t)(l - 1) * N + v) * c.W + w] = x;
@@ -676,36 +707,38 @@ __device__ void commit_masked_chunk(const CommitArgs &c, int w, int l, int v0,
 // the masked columns' words, and the row-major copy of columns that ran
 // densely.
 __global__ void __launch_bounds__(1024) k_commit_levels(CommitArgs c) {
+    CommitPick p;
+    if (!commit_pick(c, p)) return;
     const int l = blockIdx.y + 1, tid = threadIdx.x;
     const int N = c.N;
     if ((int)blockIdx.x >= c.W) {
         const int chunks = (N + 1023) >> 10;
         const int e = ((int)blockIdx.x - c.W) / chunks, ch = ((int)blockIdx.x - c.W) - e * chunks;
-        const int v0 = c.sw.v[e];
+        const int v0 = p.sw->v[e];
         if (v0 < 0) return;
         const int w = v0 >> 6;
         // each column once: by its first endpoint
         for (int j = 0; j < e; j++)
-            if (c.sw.v[j] >= 0 && (c.sw.v[j] >> 6) == w) return;
-        if (c.task_state[(size_t)c.gk * c.W + w] != GG_TASK_DENSE) {
-            commit_masked_chunk(c, w, l, ch * 1024, endpoint_mask(c.sw, w));
+            if (p.sw->v[j] >= 0 && (p.sw->v[j] >> 6) == w) return;
+        if (c.task_state[(size_t)p.gk * c.W + w] != GG_TASK_DENSE) {
+            commit_masked_chunk(c, p, w, l, ch * 1024, endpoint_mask(*p.sw, w));
             return;
         }
         // a column that ran densely (no single-source tail): its copy
         if (!c.rows || l > c.RL) return;
-        const int ecc_new = commit_col_ecc(c, w);
-        const unsigned long long *src = c.ep_snap + ((size_t)(4 * c.k + e) * c.S + (l - 1)) * N;
+        const int ecc_new = commit_col_ecc(c, p, w);
+        const unsigned long long *src = c.ep_snap + ((size_t)(4 * p.k + e) * c.S + (l - 1)) * N;
         const int v = ch * 1024 + tid;
         if (v < N) c.rows[((size_t)(l - 1) * N + v) * c.W + w] = l < ecc_new ? src[v] : ~0ull;
         return;
     }
     const int w = blockIdx.x;
-    const size_t t = (size_t)c.gk * c.W + w;
-    const int ecc_new = commit_col_ecc(c, w);
+    const size_t t = (size_t)p.gk * c.W + w;
+    const int ecc_new = commit_col_ecc(c, p, w);
     unsigned long long *__restrict__ dst = c.snap + ((size_t)(l - 1) * c.W + w) * N;
     if (c.task_state[t] == GG_TASK_DENSE) {
         if (l >= ecc_new) return;
-        const int slot = commit_ep_slot(c, w);
+        const int slot = commit_ep_slot(c, p, w);
         const unsigned long long *__restrict__ src =
             c.ep_snap + ((size_t)slot * c.S + (l - 1)) * N;
         if ((N & 1) == 0) {  // 16-byte words, eight per thread in flight
@@ -718,9 +751,9 @@ __global__ void __launch_bounds__(1024) k_commit_levels(CommitArgs c) {
         }
         return;
     }
-    if (endpoint_mask(c.sw, w)) return;  // the endpoint chunks rewrite this column
+    if (endpoint_mask(*p.sw, w)) return;  // the endpoint chunks rewrite this column
     const int end = c.task_end[t];
-    const size_t lt = ((size_t)c.k * c.W + w) * c.S + (l - 1);
+    const size_t lt = ((size_t)p.k * c.W + w) * c.S + (l - 1);
     const int *key = c.lg_key + lt * c.cap;
     const unsigned long long *word = c.lg_word + lt * c.cap;
     // the row-major copy stores full words explicitly: the Delta entries are
@@ -741,16 +774,16 @@ __global__ void __launch_bounds__(1024) k_commit_levels(CommitArgs c) {
 }
 
 // Grid W, one wave: column w's new part and per-level counts; workgroup 0
-// also rewrites the swap's table slots.  Runs after k_commit_levels, which
-// reads the old parts.
+// also rewrites the swap's slots in the resident tables.  Runs after
+// k_commit_levels, which reads the old parts.
 __global__ void __launch_bounds__(64) k_commit_parts(CommitArgs c) {
+    CommitPick p;
+    if (!commit_pick(c, p)) return;
     const int w = blockIdx.x, lane = threadIdx.x;
-    const size_t t = (size_t)c.gk * c.W + w;
-    long long *br = c.base_reach + (size_t)w * c.L1;
     if (w == 0 && lane < 4) {
-        const int v = c.sw.v[lane];
+        const int v = p.sw->v[lane];
         if (v >= 0) {
-            const int slot = c.sw.slot[lane], u = c.sw.nbr[lane];
+            const int slot = p.sw->slot[lane], u = p.sw->nbr[lane];
             int off = u - v;
             if (off < 0) off += c.N;
             if (c.tab16) c.tab16[quad_index(slot, c.N, v)] = (uint16_t)u;
@@ -758,22 +791,24 @@ __global__ void __launch_bounds__(64) k_commit_parts(CommitArgs c) {
             c.adjR[(size_t)v * c.D + slot] = u;
         }
     }
+    const size_t t = (size_t)p.gk * c.W + w;
+    long long *br = c.base_reach + (size_t)w * c.L1;
     if (c.task_state[t] == GG_TASK_DENSE) {
-        const ColPart p = c.dense_parts[t];
-        const long long *er = c.ep_reach + (size_t)commit_ep_slot(c, w) * c.L1;
-        for (int l = lane; l <= p.levels && l < c.L1; l += 64) br[l] = er[l];
-        if (lane == 0) c.base_parts[w] = p;
+        const ColPart dp = c.dense_parts[t];
+        const long long *er = c.ep_reach + (size_t)commit_ep_slot(c, p, w) * c.L1;
+        for (int l = lane; l <= dp.levels && l < c.L1; l += 64) br[l] = er[l];
+        if (lane == 0) c.base_parts[w] = dp;
         return;
     }
     if (lane != 0) return;
-    const int ecc = commit_col_ecc(c, w);
+    const int ecc = commit_col_ecc(c, p, w);
     // one pass in place: level l's old count is read before it is replaced;
     // endpoint sources (masked columns) add their single-source counts in
     // place of the base's, as the finalize does
     const int blev = c.base_parts[w].levels;
     const long long *dd = c.dcnt + t * c.L1;
     const int end = c.task_end[t];
-    const unsigned long long em = endpoint_mask(c.sw, w);
+    const unsigned long long em = endpoint_mask(*p.sw, w);
     const long long nm = __popcll(em);
     const int becc = c.base_parts[w].ecc;
     long long cb = br[0], cprev = cb, tot = 0, cx = nm;
@@ -782,8 +817,8 @@ __global__ void __launch_bounds__(64) k_commit_parts(CommitArgs c) {
         long long corr = 0;
         if (em) {
             for (int j = 0; j < 4; j++) {
-                if (!endpoint_first(c.sw, j) || (c.sw.v[j] >> 6) != w) continue;
-                const size_t t4 = (size_t)4 * c.gk + j;
+                if (!endpoint_first(*p.sw, j) || (p.sw->v[j] >> 6) != w) continue;
+                const size_t t4 = (size_t)4 * p.gk + j;
                 if (l <= c.ss_ecc[t4]) cx += c.ss_cnt[t4 * c.L1 + l];
             }
             const long long bx = l < becc ? c.mb_cnt[t * (c.S + 1) + l] : (long long)c.N * nm;
@@ -795,14 +830,14 @@ __global__ void __launch_bounds__(64) k_commit_parts(CommitArgs c) {
         cprev = cur;
     }
     if (ecc + 1 < c.L1) br[ecc + 1] = 0;
-    ColPart p;
-    p.total = tot;
-    p.reached = cprev;
-    p.ecc = ecc;
-    p.levels = ecc + 1;
-    p.flags = 0;
-    p.pad = 0;
-    c.base_parts[w] = p;
+    ColPart np;
+    np.total = tot;
+    np.reached = cprev;
+    np.ecc = ecc;
+    np.levels = ecc + 1;
+    np.flags = 0;
+    np.pad = 0;
+    c.base_parts[w] = np;
 }
 
 // Applies the swap to the resident tables (host mirror and device).
@@ -842,6 +877,96 @@ extern "C" int gg_batch_commit(gg_batch *b, const gg_swap *sw, gg_result *out) {
     return eval_base(b, out);
 }
 
+// Arguments of the in-place commit kernels for candidate kc of the last
+// launch (sel: read the candidate from device memory instead).
+static CommitArgs commit_args(gg_batch *b, int kc, const CommitSel *sel) {
+    CommitArgs c{};
+    c.N = b->N; c.W = b->W; c.S = b->snap_cap; c.L1 = GG_LEVEL_LIMIT + 1; c.cap = b->cap;
+    c.D = b->degree;
+    c.k = kc - b->log_k0;
+    c.gk = kc;
+    c.swaps = b->d_swaps;
+    c.sel = sel;
+    c.snap = b->d_snap;
+    c.rows = b->d_rows;
+    c.RL = b->rows_levels;
+    c.base_parts = b->d_parts + (size_t)b->max_k * b->W;
+    c.base_reach = b->d_base_reach;
+    c.task_state = b->d_task_state;
+    c.task_end = b->d_task_end;
+    c.task_ecc = b->d_task_ecc;
+    c.dcnt = b->d_dcnt;
+    c.lg_key = b->d_lg_key;
+    c.lg_word = b->d_lg_word;
+    c.lg_n = b->d_lg_n;
+    c.ep_snap = b->d_ep_snap;
+    c.ep_reach = b->d_ep_reach;
+    c.dense_parts = b->d_dense_parts;
+    c.ss_cnt = b->d_ss_cnt;
+    c.ss_ecc = b->d_ss_ecc;
+    c.mb_cnt = b->d_mb_cnt;
+    c.ss_vis = b->d_ss_vis;
+    c.NB = (b->N + 31) / 32;
+    c.tab16 = b->narrow ? (uint16_t *)b->d_tab : nullptr;
+    c.tab32 = b->narrow ? nullptr : (uint32_t *)b->d_tab;
+    c.adjR = b->d_adjR;
+    return c;
+}
+
+static hipError_t launch_commit(gg_batch *b, const CommitArgs &c) {
+    const int chunks = (b->N + 1023) / 1024;
+    hipLaunchKernelGGL(k_commit_levels, dim3(b->W + 4 * chunks, b->snap_cap), dim3(1024), 0,
+                       b->stream, c);
+    hipLaunchKernelGGL(k_commit_parts, dim3(b->W), dim3(64), 0, b->stream, c);
+    return hipGetLastError();
+}
+
+// The resident base now holds `known` (exact result of the swapped graph).
+static void set_base_result(gg_batch *b, const gg_result *known) {
+    ResultRec r{};
+    r.total = known->total_reps;
+    r.reached = known->status == GG_OK ? (long long)b->N * b->N : 0;
+    r.diameter = known->diameter;
+    r.levels = known->levels;
+    r.flags = known->status == GG_DIAMETER_LIMIT ? GG_FLAG_LIMIT : 0;
+    r.wvc = known->wvc;
+    b->base_res = r;
+    b->sparse_ok = b->d_snap && known->status == GG_OK && known->diameter - 1 <= b->snap_cap &&
+                   b->dc.last_variant != 3;
+}
+
+// Host mirror of an in-place commit (the kernels ran or are queued).
+static void note_inplace_commit(gg_batch *b, const gg_swap *sw, const gg_result *known) {
+    b->log_valid = 0;
+    b->last_commit_inplace = 1;
+    b->base_sym = b->base_sym && swap_keeps_symmetry(b->h_adj.data(), b->degree, *sw);
+    for (int j = 0; j < 4; j++)
+        if (sw->v[j] >= 0) b->h_adj[(size_t)sw->v[j] * b->degree + sw->slot[j]] = sw->nbr[j];
+    b->dc.push_ok = b->base_sym;
+    b->ecc_stale = 1;
+    set_base_result(b, known);
+}
+
+// Commit by a new BFS of the swapped base (stored levels, rows and parts
+// rebuilt), queued on the batch stream.
+static int commit_rebuild(gg_batch *b, const gg_swap *sw, const gg_result *known) {
+    b->log_valid = 0;
+    b->last_commit_inplace = 0;
+    int st = apply_commit(b, sw);
+    if (st != GG_OK) return st;
+    DenseArgs a = dense_args(b, 1, nullptr, b->d_parts + (size_t)b->max_k * b->W,
+                             b->d_res + b->max_k, b->d_ecc);
+    a.reach_col = b->d_base_reach;
+    a.snap = b->d_snap;
+    a.snap_levels = b->d_snap ? b->snap_cap : 0;
+    st = run_dense(b->dc, a, b->stream);
+    if (st != GG_OK) return st;
+    if ((st = enqueue_rows(b)) != GG_OK) return st;
+    b->ecc_stale = 0;
+    set_base_result(b, known);
+    return GG_OK;
+}
+
 // Commit of a candidate whose exact result the caller already holds (one of
 // the last batch's).  When the last launch kept its logs for that candidate
 // the stored base levels are rewritten in place (two small launches);
@@ -853,7 +978,6 @@ extern "C" int gg_batch_commit_known(gg_batch *b, const gg_swap *sw, const gg_re
     int st = check_swaps(b, sw, 1);
     if (st != GG_OK) return st;
     GG_HIP(hipSetDevice(b->device));
-    const int K = b->max_k;
     int kc = -1;
     if (b->log_valid && b->commit_mode == 0)
         for (int k = b->log_k0; k < b->log_K; k++)
@@ -862,73 +986,57 @@ extern "C" int gg_batch_commit_known(gg_batch *b, const gg_swap *sw, const gg_re
                          known->diameter - 1 <= b->snap_cap &&
                          !(b->h_res[kc].flags & (GG_FLAG_QUEUED | GG_FLAG_LIMIT)) &&
                          b->h_res[kc].total == known->total_reps;
-    b->log_valid = 0;
-    b->last_commit_inplace = inplace ? 1 : 0;
-    if (inplace) {
-        b->base_sym = b->base_sym && swap_keeps_symmetry(b->h_adj.data(), b->degree, *sw);
-        for (int j = 0; j < 4; j++)
-            if (sw->v[j] >= 0) b->h_adj[(size_t)sw->v[j] * b->degree + sw->slot[j]] = sw->nbr[j];
-        b->dc.push_ok = b->base_sym;
-        CommitArgs c{};
-        c.N = b->N; c.W = b->W; c.S = b->snap_cap; c.L1 = GG_LEVEL_LIMIT + 1; c.cap = b->cap;
-        c.D = b->degree;
-        c.k = kc - b->log_k0;
-        c.gk = kc;
-        c.sw = *sw;
-        c.snap = b->d_snap;
-        c.rows = b->d_rows;
-        c.RL = b->rows_levels;
-        c.base_parts = b->d_parts + (size_t)K * b->W;
-        c.base_reach = b->d_base_reach;
-        c.task_state = b->d_task_state;
-        c.task_end = b->d_task_end;
-        c.task_ecc = b->d_task_ecc;
-        c.dcnt = b->d_dcnt;
-        c.lg_key = b->d_lg_key;
-        c.lg_word = b->d_lg_word;
-        c.lg_n = b->d_lg_n;
-        c.ep_snap = b->d_ep_snap;
-        c.ep_reach = b->d_ep_reach;
-        c.dense_parts = b->d_dense_parts;
-        c.ss_cnt = b->d_ss_cnt;
-        c.ss_ecc = b->d_ss_ecc;
-        c.mb_cnt = b->d_mb_cnt;
-        c.ss_vis = b->d_ss_vis;
-        c.NB = (b->N + 31) / 32;
-        c.tab16 = b->narrow ? (uint16_t *)b->d_tab : nullptr;
-        c.tab32 = b->narrow ? nullptr : (uint32_t *)b->d_tab;
-        c.adjR = b->d_adjR;
-        const int chunks = (b->N + 1023) / 1024;
-        hipLaunchKernelGGL(k_commit_levels, dim3(b->W + 4 * chunks, b->snap_cap), dim3(1024), 0,
-                           b->stream, c);
-        hipLaunchKernelGGL(k_commit_parts, dim3(b->W), dim3(64), 0, b->stream, c);
-        GG_HIP(hipGetLastError());
-        b->ecc_stale = 1;
-    } else {
-        st = apply_commit(b, sw);
-        if (st != GG_OK) return st;
-        DenseArgs a = dense_args(b, 1, nullptr, b->d_parts + (size_t)K * b->W, b->d_res + K,
-                                 b->d_ecc);
-        a.reach_col = b->d_base_reach;
-        a.snap = b->d_snap;
-        a.snap_levels = b->d_snap ? b->snap_cap : 0;
-        st = run_dense(b->dc, a, b->stream);
-        if (st != GG_OK) return st;
-        if ((st = enqueue_rows(b)) != GG_OK) return st;
-        b->ecc_stale = 0;
+    if (!inplace) return commit_rebuild(b, sw, known);
+    // d_swaps[kc] holds the swap the launch evaluated (equal to *sw)
+    const CommitArgs c = commit_args(b, kc, nullptr);
+    hipError_t e = launch_commit(b, c);
+    if (e != hipSuccess) {
+        gg_set_last_hip_error(e, "k_commit_levels", __FILE__, __LINE__);
+        return GG_HIP_ERROR;
     }
-    ResultRec r{};
-    r.total = known->total_reps;
-    r.reached = known->status == GG_OK ? (long long)b->N * b->N : 0;
-    r.diameter = known->diameter;
-    r.levels = known->levels;
-    r.flags = known->status == GG_DIAMETER_LIMIT ? GG_FLAG_LIMIT : 0;
-    r.wvc = known->wvc;
-    b->base_res = r;
-    b->sparse_ok = b->d_snap && known->status == GG_OK && known->diameter - 1 <= b->snap_cap &&
-                   b->dc.last_variant != 3;
+    note_inplace_commit(b, sw, known);
     return GG_OK;
 }
+
+namespace gg {
+// Under capture: the in-place commit of the candidate a device-side decision
+// writes to *sel (k < 0: none) for the last launch of K candidates in one
+// sparse launch (log_k0 = 0).
+int batch_enqueue_commit_dev(gg_batch *b, const CommitSel *sel) {
+    if (!b->d_lg_key || b->commit_mode != 0) return GG_UNSUPPORTED;
+    const CommitArgs c = commit_args(b, 0, sel);
+    hipError_t e = launch_commit(b, c);
+    if (e != hipSuccess) {
+        gg_set_last_hip_error(e, "k_commit_levels", __FILE__, __LINE__);
+        return GG_HIP_ERROR;
+    }
+    return GG_OK;
+}
+
+// Host mirror of an in-place commit the device made.
+void batch_note_device_commit(gg_batch *b, const gg_swap *sw, const gg_result *known) {
+    note_inplace_commit(b, sw, known);
+}
+
+// Commit that the device left to the host (no kept logs for the candidate).
+int batch_commit_rebuild(gg_batch *b, const gg_swap *sw, const gg_result *known) {
+    return commit_rebuild(b, sw, known);
+}
+
+// Whether one sparse launch covers K candidates (the device-side commit
+// indexes the logs of that launch) with the logs kept.
+bool batch_single_launch(gg_batch *b, int K) {
+    return b->d_lg_key && b->commit_mode == 0 && K <= b->chunk && b->snap_cap > 0;
+}
+
+int batch_snap_cap(gg_batch *b) { return b->snap_cap; }
+bool batch_sparse_ok(gg_batch *b) { return b->have_base && b->sparse_ok && b->mode_opt != 1; }
+int *batch_qctl(gg_batch *b, int *n) {
+    *n = b->nqctl;
+    return b->d_qctl;
+}
+const ResultRec *batch_dres(gg_batch *b) { return b->d_res; }
+}  // namespace gg
 
 extern "C" int gg_batch_last_kernel_ms(gg_batch *b, float *ms) {
     if (!b || !ms || !b->have_timing) return GG_BAD_ARG;

@@ -3,6 +3,8 @@
 #include <vector>
 #include "gg_internal.hpp"
 
+struct SparseArgs;
+
 struct gg_batch {
     int N = 0, degree = 0, max_k = 0, device = 0, W = 0;
     bool narrow = true;
@@ -71,4 +73,6 @@ struct gg_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     int graph_k = 0;
+    SparseArgs *fin_args = nullptr;           // set: enqueue_sparse leaves the finalize to the
+    int fin_nvalid = 0;                       // caller, handing over its arguments here
 };

@@ -34,6 +34,13 @@ struct ResultRec {
 #define GG_FLAG_LIMIT 1
 #define GG_FLAG_QUEUED 2  // a column ran in the dense fallback queue (no stored levels)
 
+// The candidate an in-place commit writes into the stored base, chosen on the
+// device: k < 0 commits nothing.  k indexes the last incremental launch.
+struct CommitSel {
+    int k;
+    int pad[3];
+};
+
 // Per (candidate, column) partial written by the workgroup that ran it.
 struct ColPart {
     long long total;
@@ -71,4 +78,12 @@ __device__ __forceinline__ int wave_max32(int x) {
 template <class T>
 __device__ __forceinline__ T ld_agent(const T *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Stores another workgroup reads in the same launch: written through to the
+// device-coherent level (no L2 writeback fence), complete after
+// __builtin_amdgcn_s_waitcnt(0); the reader drops stale lines of its own
+// caches with an agent-scope acquire fence before plain loads.
+template <class T>
+__device__ __forceinline__ void st_agent(T *p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }

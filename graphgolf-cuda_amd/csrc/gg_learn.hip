@@ -331,9 +331,8 @@ __global__ void __launch_bounds__(256) k_mlp_forward(const float *X, int P,
 // sums the partials in workgroup order, so the result is deterministic.
 #define GG_GRAD_SAMPLES 16
 #define GG_PART_STRIDE (GG_NPARAMS + 1)
-__global__ void __launch_bounds__(256) k_mlp_grad_part(const float *X, const float *Y,
-                                                       const int *idx, int B,
-                                                       const __hip_bfloat16 *wq, float *part) {
+__device__ __forceinline__ void grad_part_body(const float *X, const float *Y, const int *idx,
+                                               int B, const __hip_bfloat16 *wq, float *part) {
     __shared__ float sw[GG_NPARAMS];
     __shared__ float sx[GG_GRAD_SAMPLES][GG_FEATS];
     __shared__ float sa[GG_GRAD_SAMPLES][GG_HIDDEN];  // hidden pre-activations
@@ -416,25 +415,18 @@ __global__ void __launch_bounds__(256) k_mlp_grad_part(const float *X, const flo
     }
 }
 
-// Sums the G partial gradients (fixed order) into grad[] and the mean loss.
-__global__ void __launch_bounds__(256) k_mlp_reduce(const float *part, int G, int B, float *grad,
-                                                    float *loss) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i > GG_NPARAMS) return;
-    float acc = 0.f;
-    for (int g = 0; g < G; g++) acc += part[(size_t)g * GG_PART_STRIDE + i];
-    if (i < GG_NPARAMS) grad[i] = acc;
-    else if (loss) *loss = acc / (float)B;
+__global__ void __launch_bounds__(256) k_mlp_grad_part(const float *X, const float *Y,
+                                                       const int *idx, int B,
+                                                       const __hip_bfloat16 *wq, float *part,
+                                                       const TrainGate *gate) {
+    if (gate && !gate->on) return;
+    grad_part_body(X, Y, idx, B, wq, part);
 }
 
-// Adam step on the fp32 masters, then refresh the bf16 copies.
-__global__ void __launch_bounds__(256) k_mlp_adam(float *w, float *m, float *v,
-                                                  const float *grad, __hip_bfloat16 *wq, float lr,
-                                                  float b1t, float b2t) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= GG_NPARAMS) return;
+// Adam update of parameter i from its gradient (fp32 masters, bf16 copy).
+__device__ __forceinline__ void adam_update(float *w, float *m, float *v, __hip_bfloat16 *wq,
+                                            int i, float gi, float lr, float b1t, float b2t) {
     const float beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
-    const float gi = grad[i];
     const float mi = beta1 * m[i] + (1.f - beta1) * gi;
     const float vi = beta2 * v[i] + (1.f - beta2) * gi * gi;
     m[i] = mi;
@@ -443,6 +435,58 @@ __global__ void __launch_bounds__(256) k_mlp_adam(float *w, float *m, float *v,
     const float wi = w[i] - lr * mh / (sqrtf(vh) + eps);
     w[i] = wi;
     wq[i] = __float2bfloat16(wi);
+}
+
+// Gradient reduction and Adam step in one launch, one thread per parameter:
+// the G partial gradients summed in workgroup order (k_mlp_reduce's
+// arithmetic) and the update applied (k_mlp_adam's).  Same results as the
+// two kernels.
+__global__ void __launch_bounds__(256) k_mlp_reduce_adam(const float *__restrict__ part, int G,
+                                                         int B, float *__restrict__ grad,
+                                                         float *__restrict__ loss,
+                                                         float *__restrict__ w,
+                                                         float *__restrict__ m,
+                                                         float *__restrict__ v,
+                                                         __hip_bfloat16 *__restrict__ wq,
+                                                         float lr, const TrainGate *gate) {
+    if (!gate->on) return;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i > GG_NPARAMS) return;
+    float acc = 0.f;
+    for (int g = 0; g < G; g++) acc += part[(size_t)g * GG_PART_STRIDE + i];
+    if (i == GG_NPARAMS) {
+        *loss = acc / (float)B;
+        return;
+    }
+    grad[i] = acc;
+    adam_update(w, m, v, wq, i, acc, lr, gate->b1t, gate->b2t);
+}
+
+// Sums the G partial gradients (fixed order) into grad[] and the mean loss.
+__global__ void __launch_bounds__(256) k_mlp_reduce(const float *part, int G, int B, float *grad,
+                                                    float *loss, const TrainGate *gate) {
+    if (gate && !gate->on) return;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i > GG_NPARAMS) return;
+    float acc = 0.f;
+    for (int g = 0; g < G; g++) acc += part[(size_t)g * GG_PART_STRIDE + i];
+    if (i < GG_NPARAMS) grad[i] = acc;
+    else if (loss) *loss = acc / (float)B;
+}
+
+// Adam step on the fp32 masters, then refresh the bf16 copies.  With a gate
+// the bias corrections beta^t come from it (the step count lives on the GPU).
+__global__ void __launch_bounds__(256) k_mlp_adam(float *w, float *m, float *v,
+                                                  const float *grad, __hip_bfloat16 *wq, float lr,
+                                                  float b1t, float b2t, const TrainGate *gate) {
+    if (gate) {
+        if (!gate->on) return;
+        b1t = gate->b1t;
+        b2t = gate->b2t;
+    }
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= GG_NPARAMS) return;
+    adam_update(w, m, v, wq, i, grad[i], lr, b1t, b2t);
 }
 
 __global__ void k_to_bf16(const float *w, __hip_bfloat16 *wq, int n) {
@@ -562,9 +606,21 @@ int gg::mlp_grad_dev(gg_mlp *m, const float *d_x, const float *d_y, const int *d
     if (B < 1 || B > GG_MAX_BATCH) return GG_BAD_ARG;
     const int G = (B + GG_GRAD_SAMPLES - 1) / GG_GRAD_SAMPLES;
     hipLaunchKernelGGL(k_mlp_grad_part, dim3(G), dim3(256), 0, st, d_x, d_y, d_idx, B, m->d_wq,
-                       m->d_pre);
+                       m->d_pre, (const TrainGate *)nullptr);
     hipLaunchKernelGGL(k_mlp_reduce, dim3(GG_NPARAMS / 256 + 1), dim3(256), 0, st, m->d_pre, G,
-                       B, m->d_g, m->d_loss);
+                       B, m->d_g, m->d_loss, (const TrainGate *)nullptr);
+    GG_HIP(hipGetLastError());
+    return GG_OK;
+}
+
+int gg::mlp_train_gated_dev(gg_mlp *m, const float *d_x, const float *d_y, const int *d_idx,
+                            int B, float lr, const TrainGate *gate, hipStream_t st) {
+    if (B < 1 || B > GG_MAX_BATCH || !gate) return GG_BAD_ARG;
+    const int G = (B + GG_GRAD_SAMPLES - 1) / GG_GRAD_SAMPLES;
+    hipLaunchKernelGGL(k_mlp_grad_part, dim3(G), dim3(256), 0, st, d_x, d_y, d_idx, B, m->d_wq,
+                       m->d_pre, gate);
+    hipLaunchKernelGGL(k_mlp_reduce_adam, dim3(GG_NPARAMS / 256 + 1), dim3(256), 0, st, m->d_pre,
+                       G, B, m->d_g, m->d_loss, m->d_w, m->d_m, m->d_v, m->d_wq, lr, gate);
     GG_HIP(hipGetLastError());
     return GG_OK;
 }
@@ -573,7 +629,7 @@ int gg::mlp_adam_dev(gg_mlp *m, float lr, hipStream_t st) {
     m->t++;
     const float b1t = std::pow(0.9f, (float)m->t), b2t = std::pow(0.999f, (float)m->t);
     hipLaunchKernelGGL(k_mlp_adam, dim3((GG_NPARAMS + 255) / 256), dim3(256), 0, st, m->d_w,
-                       m->d_m, m->d_v, m->d_g, m->d_wq, lr, b1t, b2t);
+                       m->d_m, m->d_v, m->d_g, m->d_wq, lr, b1t, b2t, (const TrainGate *)nullptr);
     GG_HIP(hipGetLastError());
     return GG_OK;
 }

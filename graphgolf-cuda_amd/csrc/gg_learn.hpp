@@ -24,6 +24,15 @@ struct gg_mlp {
 };
 
 struct gg_batch;
+struct SparseArgs;
+
+// Device-side training decision of a step (the learned search's resident
+// loop): on = take one Adam step, b1t / b2t = 0.9^t and 0.999^t.
+struct TrainGate {
+    int on;
+    float b1t, b2t;
+    int pad;
+};
 
 namespace gg {
 int mlp_init_params(std::vector<float> &w, uint64_t seed);
@@ -31,12 +40,17 @@ int mlp_forward_dev(gg_mlp *m, const float *d_x, int P, float *d_out, hipStream_
 int mlp_grad_dev(gg_mlp *m, const float *d_x, const float *d_y, const int *d_idx, int B,
                  hipStream_t st);
 int mlp_adam_dev(gg_mlp *m, float lr, hipStream_t st);
+// gradient + Adam step gated on the device (*gate read by the kernels)
+int mlp_train_gated_dev(gg_mlp *m, const float *d_x, const float *d_y, const int *d_idx, int B,
+                        float lr, const TrainGate *gate, hipStream_t st);
 // diam_dev: the base diameter read on the device (null: the host's value)
 int features_dev(gg_batch *b, const int *d_moves, int P, float *d_x, hipStream_t st,
                  const int *diam_dev = nullptr);
 int features_score_dev(gg_batch *b, gg_mlp *m, const int *d_moves, int P, float *d_x,
                        float *d_score, hipStream_t st, const int *diam_dev = nullptr);
-int batch_enqueue_dev(gg_batch *b, int K);
+// fin: null launches the finalize; else its arguments are returned there (and
+// *fin_nvalid its second argument) for the caller's own finalize launch
+int batch_enqueue_dev(gg_batch *b, int K, SparseArgs *fin = nullptr, int *fin_nvalid = nullptr);
 struct BatchOut {
     const ResultRec *d_res;
     ResultRec *h_res;      // pinned
@@ -54,4 +68,12 @@ const gg_swap *b_hswaps(gg_batch *b);
 int gather_rows_dev(const float *src, const int *idx, int n, float *dst, int dst_cap, int head,
                     hipStream_t st, const float *ysrc = nullptr, float *ydst = nullptr);
 hipStream_t batch_stream(gg_batch *b);
+int batch_enqueue_commit_dev(gg_batch *b, const CommitSel *sel);
+void batch_note_device_commit(gg_batch *b, const gg_swap *sw, const gg_result *known);
+int batch_commit_rebuild(gg_batch *b, const gg_swap *sw, const gg_result *known);
+bool batch_single_launch(gg_batch *b, int K);
+int batch_snap_cap(gg_batch *b);
+bool batch_sparse_ok(gg_batch *b);
+int *batch_qctl(gg_batch *b, int *n);
+const ResultRec *batch_dres(gg_batch *b);
 }  // namespace gg

@@ -9,6 +9,7 @@
 // takes Adam steps on random minibatches of that buffer.  Acceptance is always
 // decided on the exact results.
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -18,8 +19,46 @@
 #include "gg_learn.hpp"
 #include "gg_search_impl.hpp"
 #include "gg_devrng.hpp"
+#include "gg_sparse.hpp"
 
 namespace gg {
+
+// ---- the resident loop's device state --------------------------------------------------
+// After the scorer's warm-up a learned step replays as one captured graph that
+// also decides acceptance, writes the training targets, draws the minibatch,
+// takes the Adam step and commits in place: the host launches steps back to
+// back and reads one StepRec per step when a chunk of steps ends.
+#define GG_LOOP_HOST_COMMIT 1  // the accepted candidate has no kept logs: host commit
+#define GG_LOOP_NO_SWAP 2      // a proposal found no simple swap
+#define GG_LOOP_REC_CAP 512    // steps per chunk (records in pinned memory)
+#define GG_LOOP_INFLIGHT 4     // graph launches enqueued ahead of the GPU
+#define GG_LOOP_UNROLL 4       // steps per graph launch
+#define GG_ACCEPT_LANE 0xFFFFFFFEull
+#define GG_BATCH_SALT 0x5851F42D4C957F2Dull
+
+struct LoopCtl {
+    int hdr[4];            // step before this one (low 32 bits), warm-up flag, diameter (>= 1), -
+    long long step;        // search step before this one
+    long long cur_total;   // current graph: distance total, diameter, levels, status, WVC
+    int cur_diam, cur_levels, cur_status, cur_wvc;
+    int head, count;       // replay ring position and fill
+    int halted;            // a step of this chunk stopped it: the later steps are void
+    int reason;            // GG_LOOP_*
+    int nrec;              // steps recorded in this chunk
+    int trained;           // Adam steps taken in this chunk
+    int adam_t;            // Adam step count (bias corrections)
+    int pad;
+};
+
+struct StepRec {
+    long long step;        // the step's number, 1-based
+    long long total;       // the best candidate's exact result
+    int diameter, levels, status, wvc;
+    int bi, accept, inplace, pad;
+    int mv[5];
+    int pad2;
+    gg_swap sw;
+};
 
 // ---- proposals and selection on the GPU ------------------------------------------------
 // Proposal p of step t draws from stream (seed, t, p) until it finds a simple
@@ -228,6 +267,216 @@ __global__ void k_gather_rows(const float *src, const int *idx, int n, float *ds
     if (ysrc && c == 0) ydst[slot] = ysrc[slot];
 }
 
+struct AcceptArgs {
+    int N, K, rule, sched, total_steps, S, warmup_pairs, every, B, cap, nqctl;
+    double T0, T1, mu, sigma;
+    uint64_t seed;
+    const ResultRec *res;    // the launch's K exact results
+    const gg_swap *swaps;    // their swaps
+    const int *msel;         // their moves (a, b, c, d, kind)
+    const float *x;          // the P proposals' features
+    const int *psel;         // the K chosen proposals (indices into x)
+    LoopCtl *ctl;
+    int *dctl;               // k_propose's copy of the header: [3] its error word
+    CommitSel *sel;
+    TrainGate *gate;
+    float *bx, *by;          // replay ring: features, targets
+    int *idx;                // minibatch rows
+    StepRec *rec;            // pinned, GG_LOOP_REC_CAP entries
+    int *qctl;               // the incremental launch's queue counters
+    unsigned *done;          // finalize workgroups done (zero between launches)
+};
+
+__device__ __forceinline__ int rec_status(const ResultRec &r, long long pairs) {
+    if (r.flags & GG_FLAG_LIMIT) return GG_DIAMETER_LIMIT;
+    return r.reached != pairs ? GG_UNCONNECTED : GG_OK;
+}
+
+// Acceptance of one resident-loop step, by one wave: the K exact results become
+// training pairs (feature rows and normalised targets into the replay ring);
+// the best candidate by (connected, diameter, total, index) goes through the
+// annealing or hill-climbing rule (cppcode/annealing.cpp:406-412,
+// HillClimb.cpp:239; the uniform from the counter stream (seed, step,
+// GG_ACCEPT_LANE)); an accepted candidate is committed in place by the next
+// kernels, or, when its levels were not kept, halts the chunk for a commit by
+// the host.  Then the Adam step's gate and minibatch (rows drawn from
+// (seed ^ GG_BATCH_SALT, t, row)), the next step's header and the step's
+// record.  A halted chunk's later steps are void: only the queue counters of
+// their launches are reset.
+__device__ __forceinline__ void loop_accept_body(const AcceptArgs &a) {
+    LoopCtl *__restrict__ c = a.ctl;
+    const int lane = threadIdx.x, K = a.K;  // K <= 64: lane k holds candidate k
+    // every load first (stores below could alias them)
+    const int halted = c->halted, err = a.dctl[3];
+    const long long cur_total = c->cur_total, step = c->step;
+    const int cur_diam = c->cur_diam, head = c->head, count0 = c->count, adam_t = c->adam_t,
+              nrec = c->nrec, trained = c->trained;
+    ResultRec r{};
+    if (lane < K) r = a.res[lane];
+    // feature rows of the chosen proposals: lane = (row, half), 16 floats
+    float4 fx[2][4];
+    int frow[2];
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+        const int hr = lane + 64 * u;
+        frow[u] = hr < 2 * K ? hr : -1;
+        if (frow[u] < 0) continue;
+        const float4 *src = reinterpret_cast<const float4 *>(
+            a.x + (size_t)a.psel[hr >> 1] * GG_FEATS + (hr & 1) * (GG_FEATS / 2));
+#pragma unroll
+        for (int q = 0; q < 4; q++) fx[u][q] = src[q];
+    }
+    if (halted || err) {
+        for (int i = lane; i < a.nqctl; i += 64) a.qctl[i] = 0;
+        if (lane == 0) {
+            if (!halted) { c->halted = 1; c->reason = GG_LOOP_NO_SWAP; }
+            a.sel->k = -1;
+            a.gate->on = 0;
+            a.dctl[3] = 0;
+        }
+        return;
+    }
+    const long long pairs = (long long)a.N * a.N;
+    // the best candidate by (connected, diameter, total, index)
+    const bool rok = lane < K && rec_status(r, pairs) == GG_OK;
+    int key_bad = lane < K ? (rok ? 0 : 1) : 2, key_d = r.diameter, key_k = lane;
+    long long key_t = r.total;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const int ob = __shfl_xor(key_bad, o, 64), od = __shfl_xor(key_d, o, 64),
+                  okk = __shfl_xor(key_k, o, 64);
+        const long long ot = __shfl_xor(key_t, o, 64);
+        const bool better = ob != key_bad ? ob < key_bad
+                            : od != key_d ? od < key_d
+                            : ot != key_t ? ot < key_t
+                                          : okk < key_k;
+        if (better) { key_bad = ob; key_d = od; key_t = ot; key_k = okk; }
+    }
+    const int bi = key_k;
+    const int mv = lane < 5 ? a.msel[bi * 5 + lane] : 0;
+    const int swv = lane < 12 ? reinterpret_cast<const int *>(a.swaps + bi)[lane] : 0;
+    ResultRec y;
+    y.total = __shfl(r.total, bi, 64);
+    y.reached = __shfl(r.reached, bi, 64);
+    y.diameter = __shfl(r.diameter, bi, 64);
+    y.levels = __shfl(r.levels, bi, 64);
+    y.wvc = __shfl(r.wvc, bi, 64);
+    y.flags = __shfl(r.flags, bi, 64);
+    const int ys = rec_status(y, pairs);
+    const long long step1 = step + 1;
+    const int count = count0 + K < a.cap ? count0 + K : a.cap;
+    const bool train = step % a.every == 0 && count >= a.B;
+    const int t = adam_t + (train ? 1 : 0);
+    // acceptance (every lane computes the same)
+    const double NN1 = (double)a.N * (a.N - 1);
+    const double fxv = (double)cur_total / NN1;
+    const bool ok = ys == GG_OK;
+    const double fy = ok ? (double)y.total / NN1 : 1e8;
+    const int dy = ok ? y.diameter : 100000000;
+    bool accept;
+    if (a.rule == GG_RULE_HILL) {
+        accept = ok && (dy < cur_diam || (dy == cur_diam && fy < fxv));
+    } else {
+        accept = fy < fxv || dy < cur_diam;
+        if (!accept) {
+            const double T = a.sched == GG_SCHED_STEP
+                                 ? a.T0 * pow(0.995, (double)((step1 - 1) / 10000))
+                                 : a.T0 * pow(a.T1 / a.T0, (double)step1 / a.total_steps);
+            uint64_t st = sm64_stream(a.seed, (uint64_t)step1, GG_ACCEPT_LANE);
+            const double u = (double)(sm64_next(st) >> 11) * (1.0 / 9007199254740992.0);
+            accept = u < exp((fxv - fy) * NN1 / (T * NN1));
+        }
+        if (dy > cur_diam || !ok) accept = false;
+    }
+    const bool inplace = accept && ok && y.diameter - 1 <= a.S &&
+                         !(y.flags & (GG_FLAG_QUEUED | GG_FLAG_LIMIT));
+    // ---- stores: training pairs (feature rows and normalised targets)
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+        if (frow[u] < 0) continue;
+        const int hr = frow[u];
+        float4 *dst = reinterpret_cast<float4 *>(
+            a.bx + (size_t)((head + (hr >> 1)) % a.cap) * GG_FEATS + (hr & 1) * (GG_FEATS / 2));
+#pragma unroll
+        for (int q = 0; q < 4; q++) dst[q] = fx[u][q];
+    }
+    if (lane < K) {
+        const double raw = rok ? (double)(r.total - cur_total) +
+                                     (double)(r.diameter - cur_diam) * (double)a.N * a.N
+                               : 1e30;
+        double yv = (raw - a.mu) / a.sigma;
+        yv = yv < 8.0 ? yv : 8.0;
+        yv = -8.0 < yv ? yv : -8.0;
+        a.by[(head + lane) % a.cap] = (float)yv;
+    }
+    // the step's record
+    StepRec *rec = a.rec + nrec % GG_LOOP_REC_CAP;
+    if (lane < 5) rec->mv[lane] = mv;
+    if (lane < 12) reinterpret_cast<int *>(&rec->sw)[lane] = swv;
+    if (lane == 0) {
+        rec->step = step1;
+        rec->total = y.total;
+        rec->diameter = y.diameter;
+        rec->levels = y.levels;
+        rec->status = ys;
+        rec->wvc = y.wvc;
+        rec->bi = bi;
+        rec->accept = accept;
+        rec->inplace = inplace;
+        a.sel->k = inplace ? bi : -1;
+        const int diam = accept ? y.diameter : cur_diam;
+        if (accept) {
+            c->cur_total = y.total;
+            c->cur_diam = y.diameter;
+            c->cur_levels = y.levels;
+            c->cur_status = ys;
+            c->cur_wvc = y.wvc;
+            if (!inplace) { c->halted = 1; c->reason = GG_LOOP_HOST_COMMIT; }
+        }
+        c->head = (head + K) % a.cap;
+        c->count = count;
+        if (train) {
+            c->adam_t = t;
+            c->trained = trained + 1;
+            a.gate->b1t = powf(0.9f, (float)t);
+            a.gate->b2t = powf(0.999f, (float)t);
+        }
+        a.gate->on = train ? 1 : 0;
+        c->step = step1;
+        c->hdr[0] = (int)step1;
+        c->hdr[1] = count < a.warmup_pairs ? 1 : 0;
+        c->hdr[2] = diam > 0 ? diam : 1;
+        c->nrec = nrec + 1;
+    }
+    // the Adam step's minibatch rows
+    if (train)
+        for (int i = lane; i < a.B; i += 64) {
+            uint64_t st = sm64_stream(a.seed ^ GG_BATCH_SALT, (uint64_t)t, (uint64_t)i);
+            a.idx[i] = (int)sm64_below(st, (uint64_t)count);
+        }
+    for (int i = lane; i < a.nqctl; i += 64) a.qctl[i] = 0;
+}
+
+// The resident loop's finalize: workgroup k merges candidate k's result
+// (sparse_finalize_body); the last workgroup to finish (counter) runs the
+// step's acceptance with its first wave once every result is visible.
+__global__ void __launch_bounds__(256) k_finalize_accept(SparseArgs sa, int nvalid_last,
+                                                         AcceptArgs a) {
+    __shared__ int s_last;
+    // the results cross workgroups as stores through to the device-coherent
+    // level (no L2 writeback fence per workgroup)
+    sparse_finalize_body<256, true>(sa, nvalid_last, blockIdx.x);
+    __syncthreads();
+    if (threadIdx.x == 0) s_last = atomicAdd(a.done, 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (!s_last) return;
+    // the other workgroups' results were stored through to the coherent
+    // level: drop stale lines of this XCD's caches before reading them
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if (threadIdx.x == 0) *a.done = 0u;
+    if (threadIdx.x < 64) loop_accept_body(a);
+}
+
 int gather_rows_dev(const float *src, const int *idx, int n, float *dst, int dst_cap, int head,
                     hipStream_t st, const float *ysrc, float *ydst) {
     hipLaunchKernelGGL(k_gather_rows, dim3((n * GG_FEATS + 255) / 256), dim3(256), 0, st, src, idx,
@@ -272,6 +521,19 @@ struct LearnState {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int g_push = -1;
+    // resident loop (after the warm-up): every decision of a step on the GPU
+    LoopCtl *d_loop = nullptr, *h_loop = nullptr;  // control block, pinned staging copy
+    CommitSel *d_csel = nullptr;                   // the step's in-place commit
+    TrainGate *d_gate = nullptr;                   // the step's Adam step
+    int *d_bidx = nullptr;                         // minibatch rows drawn on the GPU
+    unsigned *d_done = nullptr;                    // finalize workgroups done
+    StepRec *h_rec = nullptr;                      // one record per step (pinned)
+    hipGraph_t lgraph = nullptr;
+    hipGraphExec_t lexec = nullptr;
+    int l_push = -1;
+    hipEvent_t lev[GG_LOOP_INFLIGHT] = {};          // steps in flight: one event each
+    hipGraph_t lgraph_u = nullptr;                 // GG_LOOP_UNROLL steps in one graph
+    hipGraphExec_t lexec_u = nullptr;
 };
 
 static int alloc_learn(LearnState *L) {
@@ -304,6 +566,19 @@ static int alloc_learn(LearnState *L) {
     if (hipMemset(L->d_ctl, 0, 4 * 4) != hipSuccess) return GG_HIP_ERROR;
     if (hipHostMalloc((void **)&L->h_ctl, 4 * 8, hipHostMallocDefault) != hipSuccess)
         return GG_OOM;
+    if (hipMalloc((void **)&L->d_loop, sizeof(LoopCtl)) != hipSuccess) return GG_OOM;
+    if (hipHostMalloc((void **)&L->h_loop, sizeof(LoopCtl), hipHostMallocDefault) != hipSuccess)
+        return GG_OOM;
+    if (hipMalloc((void **)&L->d_csel, sizeof(CommitSel)) != hipSuccess) return GG_OOM;
+    if (hipMalloc((void **)&L->d_gate, sizeof(TrainGate)) != hipSuccess) return GG_OOM;
+    if (hipMalloc((void **)&L->d_bidx, 4 * GG_MAX_BATCH) != hipSuccess) return GG_OOM;
+    if (hipMalloc((void **)&L->d_done, 4) != hipSuccess) return GG_OOM;
+    if (hipMemset(L->d_done, 0, 4) != hipSuccess) return GG_HIP_ERROR;
+    if (hipHostMalloc((void **)&L->h_rec, sizeof(StepRec) * GG_LOOP_REC_CAP,
+                      hipHostMallocDefault) != hipSuccess)
+        return GG_OOM;
+    for (hipEvent_t &e : L->lev)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return GG_HIP_ERROR;
     return GG_OK;
 }
 
@@ -330,12 +605,20 @@ void learn_destroy(gg_search *s) {
     if (L->st) (void)hipStreamSynchronize(L->st);
     if (L->gexec) (void)hipGraphExecDestroy(L->gexec);
     if (L->graph) (void)hipGraphDestroy(L->graph);
+    if (L->lexec) (void)hipGraphExecDestroy(L->lexec);
+    if (L->lgraph) (void)hipGraphDestroy(L->lgraph);
+    for (hipEvent_t e : L->lev)
+        if (e) (void)hipEventDestroy(e);
+    if (L->lexec_u) (void)hipGraphExecDestroy(L->lexec_u);
+    if (L->lgraph_u) (void)hipGraphDestroy(L->lgraph_u);
     gg_mlp_destroy(L->mlp);
-    void *dev[] = {L->d_moves, L->d_x, L->d_score, L->d_sel, L->d_bx, L->d_by, L->d_idx,
-                   L->d_psw, L->d_msel, L->d_ctl, L->d_rank};
+    void *dev[] = {L->d_moves, L->d_x, L->d_score, L->d_sel, L->d_bx,   L->d_by,   L->d_idx,
+                   L->d_psw,   L->d_msel, L->d_ctl, L->d_rank, L->d_loop, L->d_csel, L->d_gate,
+                   L->d_bidx,  L->d_done};
     for (void *p : dev)
         if (p) (void)hipFree(p);
-    void *host[] = {L->h_moves, L->h_score, L->h_sel, L->h_idx, L->h_ynorm, L->h_msel, L->h_ctl};
+    void *host[] = {L->h_moves, L->h_score, L->h_sel,  L->h_idx,
+                    L->h_ynorm, L->h_msel,  L->h_ctl,  L->h_loop, L->h_rec};
     for (void *p : host)
         if (p) (void)hipHostFree(p);
     delete L;
@@ -453,6 +736,191 @@ int learn_eval_device(gg_search *s) {
     }
     L->sel_on_dev = true;
     return batch_finish_dev(b, K, s->res.data());
+}
+
+// The resident loop runs once the targets' normalisation is frozen, on the
+// plain annealing / hill-climbing rules, while the base allows the
+// incremental path with one launch whose logs are kept (in-place commits).
+bool learn_loop_ready(const gg_search *s) {
+    const LearnState *L = s->learn;
+    if (!L || !L->dev || !L->norm_frozen) return false;
+    if (s->cfg.rule != GG_RULE_ANNEAL && s->cfg.rule != GG_RULE_HILL) return false;
+    if (s->cfg.schedule != GG_SCHED_GEOMETRIC && s->cfg.schedule != GG_SCHED_STEP) return false;
+    gg_batch *b = L->batch;
+    // the acceptance wave holds one candidate per lane
+    return L->K <= 64 && batch_push_state(b) == 1 && batch_sparse_ok(b) &&
+           batch_single_launch(b, L->K);
+}
+
+// One resident-loop step's launches on the learned search's stream.
+static int loop_enqueue(gg_search *s) {
+    LearnState *L = s->learn;
+    gg_batch *b = L->batch;
+    const int P = L->P, K = L->K;
+    int nq = 0;
+    int *qctl = batch_qctl(b, &nq);
+    AcceptArgs a{};
+    a.N = s->N; a.K = K; a.rule = s->cfg.rule; a.sched = s->cfg.schedule;
+    a.total_steps = s->cfg.total_steps; a.S = batch_snap_cap(b);
+    a.warmup_pairs = s->cfg.warmup_pairs; a.every = std::max(1, s->cfg.train_every);
+    a.B = std::min(std::max(s->cfg.train_batch, 1), GG_MAX_BATCH); a.cap = L->cap;
+    a.nqctl = nq;
+    a.T0 = s->cfg.T0; a.T1 = s->cfg.T1; a.mu = L->mu; a.sigma = L->sigma; a.seed = s->cfg.seed;
+    a.res = batch_dres(b); a.swaps = b_swaps(b); a.msel = L->d_msel; a.x = L->d_x;
+    a.psel = L->d_sel; a.ctl = L->d_loop; a.dctl = L->d_ctl; a.sel = L->d_csel;
+    a.gate = L->d_gate; a.bx = L->d_bx; a.by = L->d_by; a.idx = L->d_bidx; a.rec = L->h_rec;
+    a.qctl = qctl;
+    a.done = L->d_done;
+    const float lr = (float)(s->cfg.lr > 0 ? s->cfg.lr : 1e-3);
+    int st = GG_OK;
+    // the header (step, warm-up flag, diameter) is read from the control
+    // block and copied to d_ctl, which the later kernels of the step read
+    hipLaunchKernelGGL(k_propose, dim3((P + 63) / 64), dim3(64), 0, L->st, b_adjR(b), s->N, s->d,
+                       s->cfg.seed, (const int *)L->d_loop->hdr, P, L->d_moves, L->d_psw,
+                       L->d_ctl);
+    GG_HIP(hipGetLastError());
+    if ((st = features_score_dev(b, L->mlp, L->d_moves, P, L->d_x, L->d_score, L->st,
+                                 L->d_ctl + 2)) != GG_OK)
+        return st;
+    hipLaunchKernelGGL(k_rank, dim3((P + GG_RANK_PER - 1) / GG_RANK_PER), dim3(256), 0, L->st,
+                       L->d_score, P, L->d_rank);
+    hipLaunchKernelGGL(k_select, dim3(1), dim3(1024), 0, L->st, L->d_rank, P, K, s->cfg.seed,
+                       L->d_ctl, L->d_psw, L->d_moves, b_swaps(b), L->d_msel, L->d_sel);
+    GG_HIP(hipGetLastError());
+    SparseArgs fin{};
+    int nvalid = 0;
+    if ((st = batch_enqueue_dev(b, K, &fin, &nvalid)) != GG_OK) return st;
+    hipLaunchKernelGGL(k_finalize_accept, dim3(K), dim3(256), 0, L->st, fin, nvalid, a);
+    GG_HIP(hipGetLastError());
+    if ((st = mlp_train_gated_dev(L->mlp, L->d_bx, L->d_by, L->d_bidx, a.B, lr, L->d_gate,
+                                  L->st)) != GG_OK)
+        return st;
+    return batch_enqueue_commit_dev(b, L->d_csel);
+}
+
+// Captures `steps` consecutive steps into one executable graph.
+static int loop_capture_n(gg_search *s, int steps, hipGraph_t *gr, hipGraphExec_t *ex) {
+    LearnState *L = s->learn;
+    if (*ex) { (void)hipGraphExecDestroy(*ex); *ex = nullptr; }
+    if (*gr) { (void)hipGraphDestroy(*gr); *gr = nullptr; }
+    GG_HIP(hipStreamBeginCapture(L->st, hipStreamCaptureModeThreadLocal));
+    int st = GG_OK;
+    for (int i = 0; i < steps && st == GG_OK; i++) st = loop_enqueue(s);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(L->st, &g);
+    if (st != GG_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    GG_HIP(e);
+    *gr = g;
+    GG_HIP(hipGraphInstantiate(ex, g, nullptr, nullptr, 0));
+    return GG_OK;
+}
+
+static int loop_capture(gg_search *s) {
+    LearnState *L = s->learn;
+    int st = loop_capture_n(s, 1, &L->lgraph, &L->lexec);
+    if (st == GG_OK) st = loop_capture_n(s, GG_LOOP_UNROLL, &L->lgraph_u, &L->lexec_u);
+    if (st != GG_OK) return st;
+    L->l_push = batch_push_state(L->batch);
+    return GG_OK;
+}
+
+// Runs up to `steps` steps of the resident loop (one chunk: the graph
+// launched back to back, one wait), then replays the records on the host:
+// statistics, log, current and best graph, the batch evaluator's mirror of
+// every commit the device made.  A step whose accepted candidate had no kept
+// logs ends the chunk and is committed here by a new BFS.  *done = steps
+// completed (>= 1 unless an error is returned).
+int learn_loop_run(gg_search *s, int steps, int *done) {
+    LearnState *L = s->learn;
+    gg_batch *b = L->batch;
+    *done = 0;
+    s->be->forget();
+    // GG_LOOP_GRAPH=0: the step's kernels launched one by one instead of a
+    // graph replay (for tools that do not follow graph launches)
+    const char *lg = getenv("GG_LOOP_GRAPH");
+    const bool use_graph = !(lg && lg[0] == '0');
+    if (use_graph && (!L->lexec || L->l_push != batch_push_state(b))) {
+        int st = loop_capture(s);
+        if (st != GG_OK) return st;
+    }
+    LoopCtl &h = *L->h_loop;
+    memset(&h, 0, sizeof(h));
+    h.step = s->step;
+    h.hdr[0] = (int)s->step;
+    h.hdr[1] = L->count < s->cfg.warmup_pairs ? 1 : 0;
+    h.hdr[2] = s->cur.diameter > 0 ? s->cur.diameter : 1;
+    h.cur_total = s->cur.total_scaled;
+    h.cur_diam = s->cur.diameter;
+    h.cur_levels = s->cur.levels;
+    h.cur_status = s->cur.status;
+    h.cur_wvc = s->cur.wvc;
+    h.head = L->head;
+    h.count = L->count;
+    h.adam_t = L->mlp->t;
+    const int n = std::min(steps, GG_LOOP_REC_CAP);
+    GG_HIP(hipMemcpyAsync(L->d_loop, L->h_loop, sizeof(LoopCtl), hipMemcpyHostToDevice, L->st));
+    // at most GG_LOOP_INFLIGHT launches queued ahead of the one running
+    for (int i = 0, j = 0; i < n; j++) {
+        hipEvent_t &e = L->lev[j % GG_LOOP_INFLIGHT];
+        if (j >= GG_LOOP_INFLIGHT) GG_HIP(hipEventSynchronize(e));
+        const int u = n - i >= GG_LOOP_UNROLL ? GG_LOOP_UNROLL : 1;
+        if (use_graph) {
+            GG_HIP(hipGraphLaunch(u > 1 ? L->lexec_u : L->lexec, L->st));
+        } else {
+            for (int q = 0; q < u; q++) {
+                int st = loop_enqueue(s);
+                if (st != GG_OK) return st;
+            }
+        }
+        GG_HIP(hipEventRecord(e, L->st));
+        i += u;
+    }
+    GG_HIP(hipMemcpyAsync(L->h_loop, L->d_loop, sizeof(LoopCtl), hipMemcpyDeviceToHost, L->st));
+    GG_HIP(hipStreamSynchronize(L->st));
+    const int P = L->P, K = L->K;
+    for (int i = 0; i < h.nrec; i++) {
+        const StepRec &r = L->h_rec[i];
+        s->step = r.step;
+        s->stats.proposals += P;
+        s->stats.exact_evals += K;
+        s->stats.train_pairs += K;
+        const bool ok = r.status == GG_OK;
+        if (s->log_cap > 0 && (int64_t)s->log.size() < s->log_cap * 10) {
+            int64_t rec[10] = {r.step, r.mv[0], r.mv[1], r.mv[2], r.mv[3], r.mv[4],
+                               ok ? r.total : -1, ok ? r.diameter : -1, r.status, r.accept};
+            s->log.insert(s->log.end(), rec, rec + 10);
+        }
+        if (!r.accept) continue;
+        gg_result y{};
+        y.total_reps = r.total;
+        y.total_scaled = r.total;
+        y.diameter = r.diameter;
+        y.wvc = r.wvc;
+        y.levels = r.levels;
+        y.status = r.status;
+        if (r.inplace) {
+            batch_note_device_commit(b, &r.sw, &y);
+        } else {
+            int st = batch_commit_rebuild(b, &r.sw, &y);
+            if (st != GG_OK) return st;
+        }
+        apply_swap(s->adj, s->d, r.sw);
+        s->cur = y;
+        s->stats.accepted++;
+        track_best(s, r.sw);
+    }
+    L->head = h.head;
+    L->count = h.count;
+    L->mlp->t = h.adam_t;
+    L->train_steps += h.trained;
+    if (h.trained) L->loss_pending = true;
+    L->sel_on_dev = false;
+    *done = h.nrec;
+    if (h.halted && h.reason == GG_LOOP_NO_SWAP) return GG_BAD_ARG;
+    return GG_OK;
 }
 
 static double raw_delta(const gg_search *s, const gg_result &r) {
@@ -707,7 +1175,9 @@ extern "C" int gg_search_last_selection(gg_search *s, int64_t *step, int32_t *wa
     if (moves_out)
         GG_HIP(hipMemcpy(moves_out, L->d_moves, 4 * (size_t)P * 5, hipMemcpyDeviceToHost));
     if (sel_out) GG_HIP(hipMemcpy(sel_out, L->d_sel, 4 * (size_t)K, hipMemcpyDeviceToHost));
-    if (step) *step = L->h_ctl[0];
-    if (warm) *warm = L->h_ctl[1];
+    int hdr[4];
+    GG_HIP(hipMemcpy(hdr, L->d_ctl, sizeof(hdr), hipMemcpyDeviceToHost));
+    if (step) *step = hdr[0];
+    if (warm) *warm = hdr[1];
     return GG_OK;
 }

@@ -327,7 +327,7 @@ extern "C" void gg_search_destroy(gg_search *s) {
 // copy of the graph per improvement.  The trail is folded into best_base when
 // it outgrows kTrailCap; when most of it lies past the best it is dropped, and
 // the next improvement copies the graph.
-static void track_best(gg_search *s, const gg_swap &sw) {
+void gg::track_best(gg_search *s, const gg_swap &sw) {
     constexpr size_t kTrailCap = 1 << 16;
     if (s->trail_live) s->trail.push_back(sw);
     if (better_by_rule(s->cfg.rule, s->cur, s->best)) {
@@ -415,9 +415,18 @@ static int search_step(gg_search *s) {
 extern "C" int gg_search_run(gg_search *s, int steps, gg_search_stats *out) {
     if (!s || steps < 0) return GG_BAD_ARG;
     auto t0 = std::chrono::steady_clock::now();
-    for (int i = 0; i < steps; i++) {
+    for (int i = 0; i < steps;) {
+        if (learn_loop_ready(s)) {
+            // after the scorer's warm-up: steps replayed on the GPU in chunks
+            int done = 0;
+            int st = learn_loop_run(s, steps - i, &done);
+            if (st != GG_OK) return st;
+            i += done;
+            continue;
+        }
         int st = search_step(s);
         if (st != GG_OK) return st;
+        i++;
     }
     auto t1 = std::chrono::steady_clock::now();
     s->stats.seconds += std::chrono::duration<double>(t1 - t0).count();

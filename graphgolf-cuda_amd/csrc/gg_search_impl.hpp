@@ -126,4 +126,9 @@ bool learn_on_device(const gg_search *s);
 int learn_eval_device(gg_search *s);
 int learn_observe(gg_search *s, int K);
 void learn_fill_stats(gg_search *s, gg_search_stats *st);
+// the learned search's resident loop (every decision of a step on the GPU)
+bool learn_loop_ready(const gg_search *s);
+int learn_loop_run(gg_search *s, int steps, int *done);
+// best-graph bookkeeping after an accepted swap (s->cur already updated)
+void track_best(gg_search *s, const gg_swap &sw);
 }  // namespace gg

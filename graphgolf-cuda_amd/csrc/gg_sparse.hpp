@@ -746,14 +746,14 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a) {
     }
 }
 
-// One workgroup per candidate: merges sparse column results (base counts +
-// deltas) and dense fallback parts into the candidate's ResultRec.
-template <int BS>
-__global__ void __launch_bounds__(BS) k_sparse_finalize(SparseArgs a, int nvalid_last) {
+// Candidate k's result, by one workgroup: merges sparse column results (base
+// counts + deltas) and dense fallback parts into its ResultRec (AGENT: stored
+// for another workgroup of the same launch, complete on return).
+template <int BS, bool AGENT = false>
+__device__ __forceinline__ void sparse_finalize_body(const SparseArgs &a, int nvalid_last, int k) {
     __shared__ long long s_tot[BS / 64], s_rch[BS / 64];
     __shared__ int s_diam[BS / 64], s_lev[BS / 64], s_fl[BS / 64];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const int k = blockIdx.x;
     const int W = a.W, L1 = a.lmax + 1;
     long long tot = 0, rch = 0;
     int diam = 0, lev = 0, fl = 0;
@@ -868,8 +868,27 @@ __global__ void __launch_bounds__(BS) k_sparse_finalize(SparseArgs a, int nvalid
         r.levels = s_lev[0];
         r.flags = s_fl[0];
         r.wvc = 0;
-        a.res[k] = r;
+        r.seq = 0;
+        r.pad = 0;
+        if (AGENT) {
+            ResultRec *d = a.res + k;
+            st_agent(&d->total, r.total);
+            st_agent(&d->reached, r.reached);
+            st_agent(&d->diameter, r.diameter);
+            st_agent(&d->levels, r.levels);
+            st_agent(&d->wvc, r.wvc);
+            st_agent(&d->flags, r.flags);
+            __builtin_amdgcn_s_waitcnt(0);
+        } else {
+            a.res[k] = r;
+        }
     }
+}
+
+// One workgroup per candidate (sparse_finalize_body).
+template <int BS>
+__global__ void __launch_bounds__(BS) k_sparse_finalize(SparseArgs a, int nvalid_last) {
+    sparse_finalize_body<BS>(a, nvalid_last, blockIdx.x);
 }
 
 // Resets the dense queue counters and the endpoint-task counters (after the

@@ -645,6 +645,10 @@ class Search:
         check(lib().gg_search_run(self._h, steps, C.byref(st)), "gg_search_run")
         return st
 
+    def set_log(self, capacity: int):
+        """Log the next `capacity` steps (clears what was logged)."""
+        check(lib().gg_search_set_log(self._h, capacity), "gg_search_set_log")
+
     def graph(self, best=False) -> np.ndarray:
         out = np.empty((self.N, self.d), dtype=np.int32)
         check(lib().gg_search_graph(self._h, _ptr(out), 1 if best else 0), "gg_search_graph")

@@ -39,4 +39,6 @@ bool learn_on_device(const gg_search *) { return false; }
 int learn_eval_device(gg_search *) { return GG_UNSUPPORTED; }
 int learn_observe(gg_search *, int) { return GG_UNSUPPORTED; }
 void learn_fill_stats(gg_search *, gg_search_stats *) {}
+bool learn_loop_ready(const gg_search *) { return false; }
+int learn_loop_run(gg_search *, int, int *) { return GG_UNSUPPORTED; }
 }  // namespace gg

@@ -81,3 +81,40 @@ def miss_rate(sc, dl, K):
     A = set(np.argsort(np.asarray(sc, dtype=np.float64), kind="stable")[:K].tolist())
     B = np.argsort(np.asarray(dl, dtype=np.float64), kind="stable")[:K].tolist()
     return 1.0 - sum(1 for b in B if b in A) / K
+
+
+# ---- acceptance of the resident loop ---------------------------------------------------
+# k_loop_accept (gg_learn_search.hip): after the scorer's warm-up every step's
+# acceptance is decided on the GPU.  The best candidate (connected first, then
+# diameter, then distance total) goes through the rule of
+# cppcode/annealing.cpp:406-412 (HillClimb.cpp:239 for hill climbing) with the
+# uniform drawn from the counter stream (seed, step, 2^32 - 2), 53-bit mantissa.
+ACCEPT_LANE = 0xFFFFFFFE
+
+
+def accept_uniform(seed, step):
+    return (devrng_ref.Stream(seed, step, ACCEPT_LANE).next() >> 11) * (1.0 / 9007199254740992.0)
+
+
+def temperature(T0, T1, total_steps, step):
+    """Geometric schedule T(i) = T0 (T1/T0)^(i/total_steps), step 1-based."""
+    return T0 * (T1 / T0) ** (step / total_steps)
+
+
+def accept(rule_hill, N, cur_total, cur_diam, ok, total, diam, step, seed, T0, T1, total_steps):
+    """Acceptance of a candidate (ok: connected and within the level limit)
+    against the current graph at `step` (1-based)."""
+    import math
+    NN1 = float(N) * (N - 1)
+    fx = cur_total / NN1
+    fy = total / NN1 if ok else 1e8
+    dy = diam if ok else 100000000
+    if rule_hill:
+        return ok and (dy < cur_diam or (dy == cur_diam and fy < fx))
+    acc = fy < fx or dy < cur_diam
+    if not acc:
+        T = temperature(T0, T1, total_steps, step)
+        acc = accept_uniform(seed, step) < math.exp((fx - fy) * NN1 / (T * NN1))
+    if dy > cur_diam or not ok:
+        acc = False
+    return acc

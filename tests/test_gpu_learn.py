@@ -114,6 +114,42 @@ def test_learned_search_exact_acceptance_and_gain(gpu):
     s.close()
 
 
+def test_learned_search_at_n16384_d24_k32(gpu):
+    """The learned configuration at full size (random 24-regular graph,
+    n = 16384, 1024 proposals scored per step, the top 32 evaluated exactly):
+    past the 2048-pair warm-up and 300 steps of online training, the driver's
+    current total and diameter equal the oracle's on the final graph, so every
+    acceptance was decided on exact values (annealing.cpp:399-412)."""
+    import ctypes as C
+    gg = gpu
+    n, d, K, P = 16384, 24, 32, 1024
+    adj = gg.random_regular(n, d, 5)
+    cfg = gg.search_cfg(mode=gg.MODE_LEARNED, K=K, P=P, total_steps=400, T0=1e-3, T1=1e-6,
+                        seed=12345, warmup_pairs=2048, train_every=1, train_batch=256, lr=3e-3)
+    s = gg.Search(adj, cfg)
+    warm = -(-2048 // K)
+    st0 = s.run(warm)
+    st = s.run(300)
+    cur = s.graph()
+    rho, miss, sc, dl = s.audit(4242)
+    s.close()
+    assert st.exact_evals == (warm + 300) * K and st.proposals == (warm + 300) * P
+    assert st.train_steps - st0.train_steps >= 290, (st0.train_steps, st.train_steps)
+    assert st.accepted > 0
+    a = np.ascontiguousarray(cur, dtype=np.int32)
+    assert a.shape == (n, d)
+    srcs = np.arange(n, dtype=np.int32)
+    tot, dia = C.c_int64(), C.c_int32()
+    rc = ol.oracle().oracle_bfs_omp(n, d, ol._p(a), n, ol._p(srcs), min(16, os.cpu_count() or 1),
+                                    C.byref(tot), C.byref(dia))
+    assert rc == 0
+    assert (tot.value, dia.value) == (st.total_cur, st.diam_cur)
+    # the trained scorer ranks proposals better than a uniform choice of K
+    assert miss < 1.0 - K / P, miss
+    print(f"n=16384 d=24 K=32: {st.accepted} accepted, total {st.total_cur} diam {st.diam_cur}, "
+          f"spearman {rho:.3f}, top-32 miss {miss:.3f}")
+
+
 def test_device_proposals_match_restatement(gpu):
     """The learned search's GPU draws equal the restated streams, before and
     after commits change the base."""

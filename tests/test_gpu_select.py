@@ -86,3 +86,47 @@ def test_learned_search_at_n16384_d24(gpu):
     print(f"n=16384 d=24 learned, 300 steps: accepted {st.accepted}, total {st.total_cur}, "
           f"train steps {st.train_steps}, audit spearman {rho:.3f} miss {miss:.3f}")
     s.close()
+
+
+@pytest.mark.parametrize("hill", [False, True])
+def test_resident_loop_acceptance_matches_restated_rule(gpu, hill):
+    """After the warm-up the learned search decides acceptance on the GPU: every
+    logged step's decision equals the restated rule applied to the logged best
+    candidate, the current graph following the accepted ones, and the final
+    current graph's oracle total equals the driver's."""
+    gg = gpu
+    # few candidates per step, so some steps hold no improving one and the
+    # annealing rule draws its uniform
+    n, d, K, P, seed = 2048, 8, 4, 16, 31
+    T0, T1, steps = 5e-3, 1e-5, 400
+    adj = gg.random_regular(n, d, 6)
+    cfg = gg.search_cfg(mode=gg.MODE_LEARNED, rule=gg.RULE_HILL if hill else gg.RULE_ANNEAL,
+                        K=K, P=P, total_steps=steps, T0=T0, T1=T1, seed=seed,
+                        warmup_pairs=64, train_batch=64, lr=2e-3)
+    s = gg.Search(adj, cfg)
+    s.run(40)  # past the warm-up (64 pairs = 16 steps): the loop runs from here on
+    st0 = s.run(0)
+    s.set_log(1000)
+    st = s.run(300)
+    log = s.log()
+    assert len(log) == 300 and int(log[0, 0]) == st0.steps + 1
+    cur_total, cur_diam = st0.total_cur, st0.diam_cur
+    uphill = drawn = 0
+    for row in log:
+        step, total, diam, status, acc = (int(row[0]), int(row[6]), int(row[7]), int(row[8]),
+                                          int(row[9]))
+        ok = status == 0
+        want = sr.accept(hill, n, cur_total, cur_diam, ok, total, diam, step, seed, T0, T1,
+                         steps)
+        assert bool(acc) == want, (step, total, diam, cur_total, cur_diam)
+        drawn += ok and (diam, total) >= (cur_diam, cur_total)
+        if acc:
+            uphill += (diam, total) > (cur_diam, cur_total)
+            cur_total, cur_diam = total, diam
+    assert (cur_total, cur_diam) == (st.total_cur, st.diam_cur)
+    assert drawn > 0
+    if not hill:
+        assert uphill > 0  # the uniform branch accepted
+    o = oracle_eval.evaluate(s.graph())
+    assert (o[0], o[1]) == (st.total_cur, st.diam_cur)
+    s.close()
