@@ -54,6 +54,8 @@ struct FeatArgs {
     int P;
     float *X;                        // [P][GG_FEATS]
     const __hip_bfloat16 *wq;        // MLP weights: with `score`, also the scores
+    const __hip_bfloat16 *wq2;       // with wpar: the copy used when *wpar is odd
+    const int *wpar;
     float *score;                    // [P] (null: features only)
 };
 
@@ -165,13 +167,14 @@ __device__ __forceinline__ void write_features(const FeatArgs &f, int p, int a, 
     for (int e = 0; e < 4; e++) X[28 + e] = cnt[20 + e] / n;
 }
 
-// One wave per proposal.  Counts are wave-reduced popcounts over the W words.
-__global__ void __launch_bounds__(256) k_features(FeatArgs f) {
+// One wave per proposal (workgroup bid: proposals 4 bid ..).  Counts are
+// wave-reduced popcounts over the W words.
+__device__ __forceinline__ void features_body(const FeatArgs &f, int bid) {
     __shared__ unsigned long long s_r1[4][4][GG_FEAT_LDS_W];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     // a wave past the last proposal computes proposal 0 and writes nothing,
     // so every wave reaches the workgroup barrier
-    const int p0 = blockIdx.x * 4 + wv;
+    const int p0 = bid * 4 + wv;
     const bool active = p0 < f.P;
     const int p = active ? p0 : 0;
     const bool stage = f.W <= GG_FEAT_LDS_W;
@@ -263,22 +266,25 @@ __global__ void __launch_bounds__(256) k_features(FeatArgs f) {
         float xs[GG_FEATS];
 #pragma unroll
         for (int i = 0; i < GG_FEATS; i++) xs[i] = __shfl(xf[i], 0, 64);
+        const __hip_bfloat16 *wq = (f.wpar && (*f.wpar & 1)) ? f.wq2 : f.wq;
         float o = 0.f;
 #pragma unroll
         for (int q = 0; q < GG_HIDDEN / 64; q++) {
             const int h = lane + 64 * q;
-            const __hip_bfloat16 *w1 = f.wq + OFF_W1 + h * GG_FEATS;
-            float acc = bf(f.wq[OFF_B1 + h]);
+            const __hip_bfloat16 *w1 = wq + OFF_W1 + h * GG_FEATS;
+            float acc = bf(wq[OFF_B1 + h]);
 #pragma unroll
             for (int i = 0; i < GG_FEATS; i++) acc = fmaf(bf(w1[i]), xs[i], acc);
-            const float r = bf(f.wq[OFF_W2 + h]) * (acc > 0.f ? acc : 0.f);
+            const float r = bf(wq[OFF_W2 + h]) * (acc > 0.f ? acc : 0.f);
             o = o + r;
         }
 #pragma unroll
         for (int d = 32; d > 0; d >>= 1) o = o + __shfl_xor(o, d, 64);
-        if (lane == 0 && active) f.score[p] = o + bf(f.wq[OFF_B2]);
+        if (lane == 0 && active) f.score[p] = o + bf(wq[OFF_B2]);
     }
 }
+
+__global__ void __launch_bounds__(256) k_features(FeatArgs f) { features_body(f, blockIdx.x); }
 
 // ---- MLP ---------------------------------------------------------------------------------
 
@@ -326,13 +332,12 @@ __global__ void __launch_bounds__(256) k_mlp_forward(const float *X, int P,
 
 // Gradient of the mean squared error over a minibatch of B samples (rows
 // idx[0..B) of X/Y), forward with the bf16 weights, fp32 accumulation.
-// Workgroup g takes samples [16g, 16g+16) and writes its partial gradient
-// (and partial squared error at index GG_NPARAMS) to part[g][]; k_mlp_reduce
-// sums the partials in workgroup order, so the result is deterministic.
-#define GG_GRAD_SAMPLES 16
-#define GG_PART_STRIDE (GG_NPARAMS + 1)
+// Group g takes samples [16g, 16g+16) and writes its partial gradient (and
+// partial squared error at index GG_NPARAMS) to part[g][]; the reduction
+// sums the partials in group order, so the result is deterministic.
 __device__ __forceinline__ void grad_part_body(const float *X, const float *Y, const int *idx,
-                                               int B, const __hip_bfloat16 *wq, float *part) {
+                                               int B, const __hip_bfloat16 *wq, float *part,
+                                               int g) {
     __shared__ float sw[GG_NPARAMS];
     __shared__ float sx[GG_GRAD_SAMPLES][GG_FEATS];
     __shared__ float sa[GG_GRAD_SAMPLES][GG_HIDDEN];  // hidden pre-activations
@@ -340,7 +345,7 @@ __device__ __forceinline__ void grad_part_body(const float *X, const float *Y, c
     __shared__ float se[GG_GRAD_SAMPLES];             // squared error
     __shared__ int si[GG_GRAD_SAMPLES];               // sample rows (idx may be host memory)
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int s0 = blockIdx.x * GG_GRAD_SAMPLES;
+    const int s0 = g * GG_GRAD_SAMPLES;
     const int ns = min(GG_GRAD_SAMPLES, B - s0);
     if (tid < ns) si[tid] = idx ? idx[s0 + tid] : s0 + tid;
     for (int i = tid; i < GG_NPARAMS; i += 256) sw[i] = bf(wq[i]);
@@ -399,7 +404,7 @@ __device__ __forceinline__ void grad_part_body(const float *X, const float *Y, c
         for (int i = 0; i < GG_FEATS / 2; i++)
             gw1[i] = fmaf(gh, sx[s][half * (GG_FEATS / 2) + i], gw1[i]);
     }
-    float *out = part + (size_t)blockIdx.x * GG_PART_STRIDE;
+    float *out = part + (size_t)g * GG_PART_STRIDE;
 #pragma unroll
     for (int i = 0; i < GG_FEATS / 2; i++)
         out[OFF_W1 + h * GG_FEATS + half * (GG_FEATS / 2) + i] = gw1[i];
@@ -418,48 +423,30 @@ __device__ __forceinline__ void grad_part_body(const float *X, const float *Y, c
 __global__ void __launch_bounds__(256) k_mlp_grad_part(const float *X, const float *Y,
                                                        const int *idx, int B,
                                                        const __hip_bfloat16 *wq, float *part,
-                                                       const TrainGate *gate) {
+                                                       const TrainGate *gate,
+                                                       const __hip_bfloat16 *wq2,
+                                                       const int *par) {
     if (gate && !gate->on) return;
-    grad_part_body(X, Y, idx, B, wq, part);
+    grad_part_body(X, Y, idx, B, (par && (*par & 1)) ? wq2 : wq, part, blockIdx.x);
 }
 
-// Adam update of parameter i from its gradient (fp32 masters, bf16 copy).
-__device__ __forceinline__ void adam_update(float *w, float *m, float *v, __hip_bfloat16 *wq,
-                                            int i, float gi, float lr, float b1t, float b2t) {
-    const float beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
-    const float mi = beta1 * m[i] + (1.f - beta1) * gi;
-    const float vi = beta2 * v[i] + (1.f - beta2) * gi * gi;
-    m[i] = mi;
-    v[i] = vi;
-    const float mh = mi / (1.f - b1t), vh = vi / (1.f - b2t);
-    const float wi = w[i] - lr * mh / (sqrtf(vh) + eps);
-    w[i] = wi;
-    wq[i] = __float2bfloat16(wi);
-}
-
-// Gradient reduction and Adam step in one launch, one thread per parameter:
-// the G partial gradients summed in workgroup order (k_mlp_reduce's
-// arithmetic) and the update applied (k_mlp_adam's).  Same results as the
-// two kernels.
-__global__ void __launch_bounds__(256) k_mlp_reduce_adam(const float *__restrict__ part, int G,
-                                                         int B, float *__restrict__ grad,
-                                                         float *__restrict__ loss,
-                                                         float *__restrict__ w,
-                                                         float *__restrict__ m,
-                                                         float *__restrict__ v,
-                                                         __hip_bfloat16 *__restrict__ wq,
-                                                         float lr, const TrainGate *gate) {
-    if (!gate->on) return;
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i > GG_NPARAMS) return;
-    float acc = 0.f;
-    for (int g = 0; g < G; g++) acc += part[(size_t)g * GG_PART_STRIDE + i];
-    if (i == GG_NPARAMS) {
-        *loss = acc / (float)B;
+// Features and scores of the P proposals (workgroups < nfeat) beside the
+// gradient of a gated training step (the workgroups after them): a kernel that
+// leaves most CUs idle carries the learned search's training on them.
+__global__ void __launch_bounds__(256) k_features_grad(FeatArgs f, GradArgs g, int nfeat) {
+    if ((int)blockIdx.x < nfeat) {
+        features_body(f, blockIdx.x);
         return;
     }
-    grad[i] = acc;
-    adam_update(w, m, v, wq, i, acc, lr, gate->b1t, gate->b2t);
+    if (!g.gate->on) return;
+    grad_part_body(g.X, g.Y, g.idx, g.B, (*g.par & 1) ? g.wq2 : g.wq, g.part,
+                   (int)blockIdx.x - nfeat);
+}
+
+// Gradient reduction and Adam step in one launch, one thread per parameter
+// (reduce_adam_body).  Same results as the two kernels.
+__global__ void __launch_bounds__(256) k_mlp_reduce_adam(AdamArgs a) {
+    reduce_adam_body(a, blockIdx.x * 256 + threadIdx.x);
 }
 
 // Sums the G partial gradients (fixed order) into grad[] and the mean loss.
@@ -504,6 +491,7 @@ static int mlp_alloc(gg_mlp *m) {
     if (hipMalloc((void **)&m->d_v, 4 * P) != hipSuccess) return GG_OOM;
     if (hipMalloc((void **)&m->d_g, 4 * P) != hipSuccess) return GG_OOM;
     if (hipMalloc((void **)&m->d_wq, 2 * P) != hipSuccess) return GG_OOM;
+    if (hipMalloc((void **)&m->d_wq2, 2 * P) != hipSuccess) return GG_OOM;
     if (hipMalloc((void **)&m->d_pre,
                   4 * (size_t)(GG_MAX_BATCH / GG_GRAD_SAMPLES) * GG_PART_STRIDE) != hipSuccess)
         return GG_OOM;
@@ -563,7 +551,7 @@ extern "C" void gg_mlp_destroy(gg_mlp *m) {
     if (!m) return;
     (void)hipSetDevice(m->device);
     if (m->stream) (void)hipStreamSynchronize(m->stream);
-    void *bufs[] = {m->d_w, m->d_m, m->d_v, m->d_g, m->d_wq, m->d_pre, m->d_loss, m->d_x,
+    void *bufs[] = {m->d_w, m->d_m, m->d_v, m->d_g, m->d_wq, m->d_wq2, m->d_pre, m->d_loss, m->d_x,
                     m->d_y, m->d_out};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
@@ -606,7 +594,8 @@ int gg::mlp_grad_dev(gg_mlp *m, const float *d_x, const float *d_y, const int *d
     if (B < 1 || B > GG_MAX_BATCH) return GG_BAD_ARG;
     const int G = (B + GG_GRAD_SAMPLES - 1) / GG_GRAD_SAMPLES;
     hipLaunchKernelGGL(k_mlp_grad_part, dim3(G), dim3(256), 0, st, d_x, d_y, d_idx, B, m->d_wq,
-                       m->d_pre, (const TrainGate *)nullptr);
+                       m->d_pre, (const TrainGate *)nullptr, (const __hip_bfloat16 *)nullptr,
+                       (const int *)nullptr);
     hipLaunchKernelGGL(k_mlp_reduce, dim3(GG_NPARAMS / 256 + 1), dim3(256), 0, st, m->d_pre, G,
                        B, m->d_g, m->d_loss, (const TrainGate *)nullptr);
     GG_HIP(hipGetLastError());
@@ -614,13 +603,39 @@ int gg::mlp_grad_dev(gg_mlp *m, const float *d_x, const float *d_y, const int *d
 }
 
 int gg::mlp_train_gated_dev(gg_mlp *m, const float *d_x, const float *d_y, const int *d_idx,
-                            int B, float lr, const TrainGate *gate, hipStream_t st) {
+                            int B, float lr, const TrainGate *gate, hipStream_t st,
+                            const int *par) {
     if (B < 1 || B > GG_MAX_BATCH || !gate) return GG_BAD_ARG;
     const int G = (B + GG_GRAD_SAMPLES - 1) / GG_GRAD_SAMPLES;
     hipLaunchKernelGGL(k_mlp_grad_part, dim3(G), dim3(256), 0, st, d_x, d_y, d_idx, B, m->d_wq,
-                       m->d_pre, gate);
-    hipLaunchKernelGGL(k_mlp_reduce_adam, dim3(GG_NPARAMS / 256 + 1), dim3(256), 0, st, m->d_pre,
-                       G, B, m->d_g, m->d_loss, m->d_w, m->d_m, m->d_v, m->d_wq, lr, gate);
+                       m->d_pre, gate, (const __hip_bfloat16 *)m->d_wq2, par);
+    hipLaunchKernelGGL(k_mlp_reduce_adam, dim3(GG_ADAM_GROUPS), dim3(256), 0, st,
+                       mlp_adam_args(m, B, lr, gate, par));
+    GG_HIP(hipGetLastError());
+    return GG_OK;
+}
+
+GradArgs gg::mlp_grad_args(gg_mlp *m, const float *d_x, const float *d_y, const int *d_idx,
+                           int B, const TrainGate *gate, const int *par) {
+    GradArgs g{};
+    g.X = d_x; g.Y = d_y; g.idx = d_idx; g.B = B;
+    g.wq = m->d_wq; g.wq2 = m->d_wq2; g.par = par; g.part = m->d_pre; g.gate = gate;
+    return g;
+}
+
+AdamArgs gg::mlp_adam_args(gg_mlp *m, int B, float lr, const TrainGate *gate, const int *par) {
+    AdamArgs a{};
+    a.part = m->d_pre; a.G = (B + GG_GRAD_SAMPLES - 1) / GG_GRAD_SAMPLES; a.B = B;
+    a.grad = m->d_g; a.loss = m->d_loss; a.w = m->d_w; a.m = m->d_m; a.v = m->d_v;
+    a.wq = m->d_wq; a.wq2 = m->d_wq2; a.lr = lr; a.gate = gate; a.par = par;
+    return a;
+}
+
+int gg::mlp_refresh_dev(gg_mlp *m, hipStream_t st) {
+    hipLaunchKernelGGL(k_to_bf16, dim3((GG_NPARAMS + 255) / 256), dim3(256), 0, st, m->d_w,
+                       m->d_wq, GG_NPARAMS);
+    hipLaunchKernelGGL(k_to_bf16, dim3((GG_NPARAMS + 255) / 256), dim3(256), 0, st, m->d_w,
+                       m->d_wq2, GG_NPARAMS);
     GG_HIP(hipGetLastError());
     return GG_OK;
 }
@@ -688,7 +703,8 @@ int gg::features_dev(gg_batch *b, const int *d_moves, int P, float *d_x, hipStre
 
 // Features and, with m and d_score, the MLP scores of the P moves in one launch.
 int gg::features_score_dev(gg_batch *b, gg_mlp *m, const int *d_moves, int P, float *d_x,
-                           float *d_score, hipStream_t st, const int *diam_dev) {
+                           float *d_score, hipStream_t st, const int *diam_dev,
+                           const int *wpar, const GradArgs *grad) {
     if (!b->sparse_ok) return GG_UNSUPPORTED;
     FeatArgs f{};
     f.diam_dev = diam_dev;
@@ -704,8 +720,16 @@ int gg::features_score_dev(gg_batch *b, gg_mlp *m, const int *d_moves, int P, fl
     f.P = P;
     f.X = d_x;
     f.wq = m && d_score ? m->d_wq : nullptr;
+    f.wq2 = m && d_score ? m->d_wq2 : nullptr;
+    f.wpar = m && d_score ? wpar : nullptr;
     f.score = m ? d_score : nullptr;
-    hipLaunchKernelGGL(k_features, dim3((P + 3) / 4), dim3(256), 0, st, f);
+    const int nfeat = (P + 3) / 4;
+    if (grad) {
+        const int G = (grad->B + GG_GRAD_SAMPLES - 1) / GG_GRAD_SAMPLES;
+        hipLaunchKernelGGL(k_features_grad, dim3(nfeat + G), dim3(256), 0, st, f, *grad, nfeat);
+    } else {
+        hipLaunchKernelGGL(k_features, dim3(nfeat), dim3(256), 0, st, f);
+    }
     GG_HIP(hipGetLastError());
     return GG_OK;
 }

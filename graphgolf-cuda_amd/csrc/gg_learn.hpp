@@ -11,12 +11,17 @@
 #define GG_HIDDEN 128
 #define GG_NPARAMS (GG_HIDDEN * GG_FEATS + GG_HIDDEN + GG_HIDDEN + 1)
 #define GG_MAX_BATCH 512
+#define GG_GRAD_SAMPLES 16                          // minibatch samples per gradient group
+#define GG_PART_STRIDE (GG_NPARAMS + 1)             // partial gradient + squared error
+#define GG_ADAM_GROUPS (GG_NPARAMS / 256 + 1)       // 256-thread groups of the update
 
 struct gg_mlp {
     int device = 0;
     hipStream_t stream = nullptr;
     float *d_w = nullptr, *d_m = nullptr, *d_v = nullptr, *d_g = nullptr;  // fp32 masters, Adam, grad
     __hip_bfloat16 *d_wq = nullptr;  // bf16 weights used by every forward
+    __hip_bfloat16 *d_wq2 = nullptr; // second bf16 copy (the learned search's resident loop
+                                     // alternates the two by step parity)
     float *d_pre = nullptr, *d_loss = nullptr;  // d_pre: per-workgroup partial gradients
     float *d_x = nullptr, *d_y = nullptr, *d_out = nullptr;
     int xcap = 0;
@@ -34,20 +39,89 @@ struct TrainGate {
     int pad;
 };
 
+// A gated training step's gradient (groups of GG_GRAD_SAMPLES samples): rows
+// idx[0..B) of X / Y, the bf16 copy *par & 1 (wq, wq2), partials into part.
+struct GradArgs {
+    const float *X, *Y;
+    const int *idx;
+    int B;
+    const __hip_bfloat16 *wq, *wq2;
+    const int *par;
+    float *part;
+    const TrainGate *gate;
+};
+
+// Its reduction and Adam update: the G partials summed in group order, the
+// fp32 masters updated and the bf16 copy (*par - 1) & 1 written (from the
+// unchanged masters when the gate is off); without par, copy wq.
+struct AdamArgs {
+    const float *part;
+    int G, B;
+    float *grad, *loss, *w, *m, *v;
+    __hip_bfloat16 *wq, *wq2;
+    float lr;
+    const TrainGate *gate;
+    const int *par;
+};
+
+__device__ __forceinline__ void adam_update(float *w, float *m, float *v, __hip_bfloat16 *wq,
+                                            int i, float gi, float lr, float b1t, float b2t) {
+    const float beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
+    const float mi = beta1 * m[i] + (1.f - beta1) * gi;
+    const float vi = beta2 * v[i] + (1.f - beta2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float mh = mi / (1.f - b1t), vh = vi / (1.f - b2t);
+    const float wi = w[i] - lr * mh / (sqrtf(vh) + eps);
+    w[i] = wi;
+    wq[i] = __float2bfloat16(wi);
+}
+
+// Parameter i (i == GG_NPARAMS: the loss) of an Adam step.
+__device__ __forceinline__ void reduce_adam_body(const AdamArgs &a, int i) {
+    __hip_bfloat16 *wq = (a.par && ((*a.par - 1) & 1)) ? a.wq2 : a.wq;
+    if (!a.gate->on) {
+        if (a.par && i < GG_NPARAMS) wq[i] = __float2bfloat16(a.w[i]);
+        return;
+    }
+    if (i > GG_NPARAMS) return;
+    float acc = 0.f;
+    for (int g = 0; g < a.G; g++) acc += a.part[(size_t)g * GG_PART_STRIDE + i];
+    if (i == GG_NPARAMS) {
+        *a.loss = acc / (float)a.B;
+        return;
+    }
+    a.grad[i] = acc;
+    adam_update(a.w, a.m, a.v, wq, i, acc, a.lr, a.gate->b1t, a.gate->b2t);
+}
+
 namespace gg {
 int mlp_init_params(std::vector<float> &w, uint64_t seed);
 int mlp_forward_dev(gg_mlp *m, const float *d_x, int P, float *d_out, hipStream_t st);
 int mlp_grad_dev(gg_mlp *m, const float *d_x, const float *d_y, const int *d_idx, int B,
                  hipStream_t st);
 int mlp_adam_dev(gg_mlp *m, float lr, hipStream_t st);
-// gradient + Adam step gated on the device (*gate read by the kernels)
+// gradient + Adam step gated on the device (*gate read by the kernels).  With
+// par (a device step counter p, read at launch): the gradient's forward uses
+// the bf16 copy p & 1 and the update writes copy (p - 1) & 1 (also when the
+// gate is off, refreshed from the fp32 masters).
 int mlp_train_gated_dev(gg_mlp *m, const float *d_x, const float *d_y, const int *d_idx, int B,
-                        float lr, const TrainGate *gate, hipStream_t st);
+                        float lr, const TrainGate *gate, hipStream_t st,
+                        const int *par = nullptr);
+// both bf16 copies from the fp32 masters
+int mlp_refresh_dev(gg_mlp *m, hipStream_t st);
+// arguments of a gated step's gradient / update kernels carried by other launches
+GradArgs mlp_grad_args(gg_mlp *m, const float *d_x, const float *d_y, const int *d_idx, int B,
+                       const TrainGate *gate, const int *par);
+AdamArgs mlp_adam_args(gg_mlp *m, int B, float lr, const TrainGate *gate, const int *par);
 // diam_dev: the base diameter read on the device (null: the host's value)
 int features_dev(gg_batch *b, const int *d_moves, int P, float *d_x, hipStream_t st,
                  const int *diam_dev = nullptr);
+// wpar: device step counter p; the scores use bf16 copy p & 1 (null: d_wq).
+// grad: a gated training step's gradient carried by the same launch.
 int features_score_dev(gg_batch *b, gg_mlp *m, const int *d_moves, int P, float *d_x,
-                       float *d_score, hipStream_t st, const int *diam_dev = nullptr);
+                       float *d_score, hipStream_t st, const int *diam_dev = nullptr,
+                       const int *wpar = nullptr, const GradArgs *grad = nullptr);
 // fin: null launches the finalize; else its arguments are returned there (and
 // *fin_nvalid its second argument) for the caller's own finalize launch
 int batch_enqueue_dev(gg_batch *b, int K, SparseArgs *fin = nullptr, int *fin_nvalid = nullptr);

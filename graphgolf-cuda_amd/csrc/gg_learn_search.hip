@@ -171,7 +171,7 @@ __global__ void __launch_bounds__(64) k_propose(const int *adjR, int N, int d, u
 // counts the predecessors of proposal 16g + (t mod 16) within the (t / 16)-th
 // sixteenth of the scores, and the sixteen partial counts are summed in LDS.
 #define GG_RANK_PER 16
-__global__ void __launch_bounds__(256) k_rank(const float *score, int P, int *rank) {
+__device__ __forceinline__ void rank_body(const float *score, int P, int *rank, int bid) {
     __shared__ float s_sc[GG_SELECT_MAX_P];
     __shared__ int s_part[16][GG_RANK_PER];
     for (int q = threadIdx.x; q < P; q += 256) {
@@ -180,7 +180,7 @@ __global__ void __launch_bounds__(256) k_rank(const float *score, int P, int *ra
     }
     __syncthreads();
     const int j = threadIdx.x & (GG_RANK_PER - 1), seg = threadIdx.x >> 4;
-    const int p = blockIdx.x * GG_RANK_PER + j;
+    const int p = bid * GG_RANK_PER + j;
     int r = 0;
     if (p < P) {
         const float x = s_sc[p];
@@ -199,6 +199,21 @@ __global__ void __launch_bounds__(256) k_rank(const float *score, int P, int *ra
         for (int i = 0; i < 16; i++) t += s_part[i][j];
         rank[p] = t;
     }
+}
+
+__global__ void __launch_bounds__(256) k_rank(const float *score, int P, int *rank) {
+    rank_body(score, P, rank, blockIdx.x);
+}
+
+// The ranks (workgroups < nrank) beside the reduction and Adam update of a
+// gated training step whose gradient the last feature launch carried.
+__global__ void __launch_bounds__(256) k_rank_adam(const float *score, int P, int *rank,
+                                                   int nrank, AdamArgs a) {
+    if ((int)blockIdx.x < nrank) {
+        rank_body(score, P, rank, blockIdx.x);
+        return;
+    }
+    reduce_adam_body(a, ((int)blockIdx.x - nrank) * 256 + threadIdx.x);
 }
 
 __global__ void __launch_bounds__(1024) k_select(const int *rank, int P, int K, uint64_t seed,
@@ -779,11 +794,19 @@ static int loop_enqueue(gg_search *s) {
                        s->cfg.seed, (const int *)L->d_loop->hdr, P, L->d_moves, L->d_psw,
                        L->d_ctl);
     GG_HIP(hipGetLastError());
+    // The last step's training step runs in this step's feature and rank
+    // launches, on the CUs their own workgroups leave idle: its gradient
+    // beside the features, its update beside the ranks.  The scores use the
+    // bf16 copy of step parity, trained through two steps back, so the
+    // update writing the other copy does not race them.
+    const int *par = (const int *)L->d_loop->hdr;
+    const GradArgs ga = mlp_grad_args(L->mlp, L->d_bx, L->d_by, L->d_bidx, a.B, L->d_gate, par);
     if ((st = features_score_dev(b, L->mlp, L->d_moves, P, L->d_x, L->d_score, L->st,
-                                 L->d_ctl + 2)) != GG_OK)
+                                 L->d_ctl + 2, par, &ga)) != GG_OK)
         return st;
-    hipLaunchKernelGGL(k_rank, dim3((P + GG_RANK_PER - 1) / GG_RANK_PER), dim3(256), 0, L->st,
-                       L->d_score, P, L->d_rank);
+    const int nrank = (P + GG_RANK_PER - 1) / GG_RANK_PER;
+    hipLaunchKernelGGL(k_rank_adam, dim3(nrank + GG_ADAM_GROUPS), dim3(256), 0, L->st, L->d_score,
+                       P, L->d_rank, nrank, mlp_adam_args(L->mlp, a.B, lr, L->d_gate, par));
     hipLaunchKernelGGL(k_select, dim3(1), dim3(1024), 0, L->st, L->d_rank, P, K, s->cfg.seed,
                        L->d_ctl, L->d_psw, L->d_moves, b_swaps(b), L->d_msel, L->d_sel);
     GG_HIP(hipGetLastError());
@@ -792,9 +815,6 @@ static int loop_enqueue(gg_search *s) {
     if ((st = batch_enqueue_dev(b, K, &fin, &nvalid)) != GG_OK) return st;
     hipLaunchKernelGGL(k_finalize_accept, dim3(K), dim3(256), 0, L->st, fin, nvalid, a);
     GG_HIP(hipGetLastError());
-    if ((st = mlp_train_gated_dev(L->mlp, L->d_bx, L->d_by, L->d_bidx, a.B, lr, L->d_gate,
-                                  L->st)) != GG_OK)
-        return st;
     return batch_enqueue_commit_dev(b, L->d_csel);
 }
 
@@ -862,6 +882,11 @@ int learn_loop_run(gg_search *s, int steps, int *done) {
     h.adam_t = L->mlp->t;
     const int n = std::min(steps, GG_LOOP_REC_CAP);
     GG_HIP(hipMemcpyAsync(L->d_loop, L->h_loop, sizeof(LoopCtl), hipMemcpyHostToDevice, L->st));
+    // both bf16 copies of the weights from the masters (the loop alternates
+    // them), and no training step pending for the first step to carry
+    int rst = mlp_refresh_dev(L->mlp, L->st);
+    if (rst != GG_OK) return rst;
+    GG_HIP(hipMemsetAsync(L->d_gate, 0, sizeof(TrainGate), L->st));
     // at most GG_LOOP_INFLIGHT launches queued ahead of the one running
     for (int i = 0, j = 0; i < n; j++) {
         hipEvent_t &e = L->lev[j % GG_LOOP_INFLIGHT];
@@ -878,6 +903,15 @@ int learn_loop_run(gg_search *s, int steps, int *done) {
         GG_HIP(hipEventRecord(e, L->st));
         i += u;
     }
+    // the last step's training step (no later step carries it), then the
+    // host path's single copy holds the latest weights again
+    const float lr = (float)(s->cfg.lr > 0 ? s->cfg.lr : 1e-3);
+    rst = mlp_train_gated_dev(L->mlp, L->d_bx, L->d_by, L->d_bidx,
+                              std::min(std::max(s->cfg.train_batch, 1), GG_MAX_BATCH), lr,
+                              L->d_gate, L->st, (const int *)L->d_loop->hdr);
+    if (rst != GG_OK) return rst;
+    rst = mlp_refresh_dev(L->mlp, L->st);
+    if (rst != GG_OK) return rst;
     GG_HIP(hipMemcpyAsync(L->h_loop, L->d_loop, sizeof(LoopCtl), hipMemcpyDeviceToHost, L->st));
     GG_HIP(hipStreamSynchronize(L->st));
     const int P = L->P, K = L->K;

@@ -362,6 +362,7 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a) {
     __shared__ int s_nold[GG_SPARSE_MAX_K];
     __shared__ int s_staged;
     __shared__ int s_items[GG_SPARSE_MAX_K + 1];  // prefix sums of items per candidate
+    __shared__ int s_kc[GG_SPARSE_MAX_K];         // keys that spawn items (compacted levels)
     __shared__ int s_nprev[GG_SPARSE_MAX_K];
     __shared__ int s_ncur[GG_SPARSE_MAX_K];
     __shared__ int s_active[GG_SPARSE_MAX_K];     // 1 if the task runs this level
@@ -470,6 +471,7 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a) {
             s_dc[k] = 0;
             s_lost[k] = 0;
             s_mcnt[k] = 0;
+            s_kc[k] = 0;
         }
         if (tid == 0) { s_next = 0; s_any = 0; }
         clear_ended();
@@ -523,6 +525,11 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a) {
         // flat over (candidate, entry): clear the stale bits of the cur parity
         // (Delta_{l-2}) so Delta_l starts empty; build the union of the
         // Delta_{l-1} (LDS bitmap) and stage their keys
+        // At and above the column's eccentricity only the keys that lost bits
+        // spawn items: with staged keys those are packed to the front of their
+        // candidate's key range and the item counts recomputed, so the pool
+        // deals out no items that would be skipped.
+        const bool compact = s_staged && l >= ecc_w;
         {
             const int nold_all = s_ooff[K], nprev_all = s_koff[K];
             const bool stg = s_staged;
@@ -548,11 +555,30 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a) {
                     if (lost) atomicOr(&s_lost[k], lost);
                     else if (l >= ecc_w) xs = -1;
                 }
-                if (stg) s_keys[i] = xs;
+                if (compact) {
+                    if (xs >= 0) s_keys[s_koff[k] + atomicAdd(&s_kc[k], 1)] = xs;
+                } else if (stg) {
+                    s_keys[i] = xs;
+                }
             }
         }
         __threadfence_block();
         __syncthreads();
+        if (compact) {
+            if (tid < 64) {  // item prefix sums over the packed keys, one wave
+                const bool act = tid < K && s_active[tid];
+                int it = act ? 4 + s_kc[tid] * (D + 1) : 0;
+                const int it0 = it;
+#pragma unroll
+                for (int o = 1; o < 64; o <<= 1) {
+                    const int t1 = __shfl_up(it, o, 64);
+                    if (lane >= o) it += t1;
+                }
+                if (tid < K) s_items[tid] = it - it0;
+                if (tid == 63) s_items[K] = it;
+            }
+            __syncthreads();
+        }
         const int total = s_items[K];
         const bool staged = s_staged;
         // shared pool of items over all active candidates
