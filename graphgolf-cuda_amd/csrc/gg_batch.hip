@@ -15,6 +15,7 @@
 #include "gg_common.hpp"
 #include "gg_dense.hpp"
 #include "gg_sparse.hpp"
+#include "gg_propose.hpp"
 #include "gg_internal.hpp"
 #include "gg_batch.hpp"
 #include "gg_learn.hpp"
@@ -710,6 +711,19 @@ __global__ void __launch_bounds__(1024) k_commit_levels(CommitArgs c) {
     CommitPick p;
     if (!commit_pick(c, p)) return;
     const int l = blockIdx.y + 1, tid = threadIdx.x;
+    // the swap's slots in the resident tables (nothing in this launch or the
+    // parts launch reads them)
+    if (blockIdx.x == 0 && blockIdx.y == 0 && tid < 4) {
+        const int v = p.sw->v[tid];
+        if (v >= 0) {
+            const int slot = p.sw->slot[tid], u = p.sw->nbr[tid];
+            int off = u - v;
+            if (off < 0) off += c.N;
+            if (c.tab16) c.tab16[quad_index(slot, c.N, v)] = (uint16_t)u;
+            else c.tab32[(size_t)slot * c.N + v] = (uint32_t)off;
+            c.adjR[(size_t)v * c.D + slot] = u;
+        }
+    }
     const int N = c.N;
     if ((int)blockIdx.x >= c.W) {
         const int chunks = (N + 1023) >> 10;
@@ -773,24 +787,12 @@ __global__ void __launch_bounds__(1024) k_commit_levels(CommitArgs c) {
     for (int i = tid; i < n; i += 1024) dst[key[i]] = word[i];
 }
 
-// Grid W, one wave: column w's new part and per-level counts; workgroup 0
-// also rewrites the swap's slots in the resident tables.  Runs after
+// One wave: column w's new part and per-level counts.  Runs after
 // k_commit_levels, which reads the old parts.
-__global__ void __launch_bounds__(64) k_commit_parts(CommitArgs c) {
+__device__ __forceinline__ void commit_parts_body(const CommitArgs &c, int w) {
     CommitPick p;
     if (!commit_pick(c, p)) return;
-    const int w = blockIdx.x, lane = threadIdx.x;
-    if (w == 0 && lane < 4) {
-        const int v = p.sw->v[lane];
-        if (v >= 0) {
-            const int slot = p.sw->slot[lane], u = p.sw->nbr[lane];
-            int off = u - v;
-            if (off < 0) off += c.N;
-            if (c.tab16) c.tab16[quad_index(slot, c.N, v)] = (uint16_t)u;
-            else c.tab32[(size_t)slot * c.N + v] = (uint32_t)off;
-            c.adjR[(size_t)v * c.D + slot] = u;
-        }
-    }
+    const int lane = threadIdx.x;
     const size_t t = (size_t)p.gk * c.W + w;
     long long *br = c.base_reach + (size_t)w * c.L1;
     if (c.task_state[t] == GG_TASK_DENSE) {
@@ -838,6 +840,21 @@ __global__ void __launch_bounds__(64) k_commit_parts(CommitArgs c) {
     np.flags = 0;
     np.pad = 0;
     c.base_parts[w] = np;
+}
+
+__global__ void __launch_bounds__(64) k_commit_parts(CommitArgs c) {
+    commit_parts_body(c, blockIdx.x);
+}
+
+// The parts (workgroups < W) beside the next step's proposal draw (the
+// workgroups after them): the draw reads the resident rows k_commit_levels
+// has already rewritten.
+__global__ void __launch_bounds__(64) k_commit_parts_propose(CommitArgs c, ProposeArgs pa) {
+    if ((int)blockIdx.x < c.W) {
+        commit_parts_body(c, blockIdx.x);
+        return;
+    }
+    propose_body(pa, (int)blockIdx.x - c.W);
 }
 
 // Applies the swap to the resident tables (host mirror and device).
@@ -913,11 +930,16 @@ static CommitArgs commit_args(gg_batch *b, int kc, const CommitSel *sel) {
     return c;
 }
 
-static hipError_t launch_commit(gg_batch *b, const CommitArgs &c) {
+static hipError_t launch_commit(gg_batch *b, const CommitArgs &c,
+                                const ProposeArgs *next = nullptr) {
     const int chunks = (b->N + 1023) / 1024;
     hipLaunchKernelGGL(k_commit_levels, dim3(b->W + 4 * chunks, b->snap_cap), dim3(1024), 0,
                        b->stream, c);
-    hipLaunchKernelGGL(k_commit_parts, dim3(b->W), dim3(64), 0, b->stream, c);
+    if (next)
+        hipLaunchKernelGGL(k_commit_parts_propose, dim3(b->W + (next->P + 63) / 64), dim3(64), 0,
+                           b->stream, c, *next);
+    else
+        hipLaunchKernelGGL(k_commit_parts, dim3(b->W), dim3(64), 0, b->stream, c);
     return hipGetLastError();
 }
 
@@ -1002,10 +1024,10 @@ namespace gg {
 // Under capture: the in-place commit of the candidate a device-side decision
 // writes to *sel (k < 0: none) for the last launch of K candidates in one
 // sparse launch (log_k0 = 0).
-int batch_enqueue_commit_dev(gg_batch *b, const CommitSel *sel) {
+int batch_enqueue_commit_dev(gg_batch *b, const CommitSel *sel, const ProposeArgs *next) {
     if (!b->d_lg_key || b->commit_mode != 0) return GG_UNSUPPORTED;
     const CommitArgs c = commit_args(b, 0, sel);
-    hipError_t e = launch_commit(b, c);
+    hipError_t e = launch_commit(b, c, next);
     if (e != hipSuccess) {
         gg_set_last_hip_error(e, "k_commit_levels", __FILE__, __LINE__);
         return GG_HIP_ERROR;

@@ -142,7 +142,10 @@ const gg_swap *b_hswaps(gg_batch *b);
 int gather_rows_dev(const float *src, const int *idx, int n, float *dst, int dst_cap, int head,
                     hipStream_t st, const float *ysrc = nullptr, float *ydst = nullptr);
 hipStream_t batch_stream(gg_batch *b);
-int batch_enqueue_commit_dev(gg_batch *b, const CommitSel *sel);
+struct ProposeArgs;
+// next: the commit's parts launch also draws those proposals (extra workgroups)
+int batch_enqueue_commit_dev(gg_batch *b, const CommitSel *sel,
+                             const ProposeArgs *next = nullptr);
 void batch_note_device_commit(gg_batch *b, const gg_swap *sw, const gg_result *known);
 int batch_commit_rebuild(gg_batch *b, const gg_swap *sw, const gg_result *known);
 bool batch_single_launch(gg_batch *b, int K);

@@ -20,6 +20,7 @@
 #include "gg_search_impl.hpp"
 #include "gg_devrng.hpp"
 #include "gg_sparse.hpp"
+#include "gg_propose.hpp"
 
 namespace gg {
 
@@ -32,7 +33,7 @@ namespace gg {
 #define GG_LOOP_NO_SWAP 2      // a proposal found no simple swap
 #define GG_LOOP_REC_CAP 512    // steps per chunk (records in pinned memory)
 #define GG_LOOP_INFLIGHT 4     // graph launches enqueued ahead of the GPU
-#define GG_LOOP_UNROLL 4       // steps per graph launch
+#define GG_LOOP_UNROLL 8       // steps per graph launch
 #define GG_ACCEPT_LANE 0xFFFFFFFEull
 #define GG_BATCH_SALT 0x5851F42D4C957F2Dull
 
@@ -61,110 +62,10 @@ struct StepRec {
 };
 
 // ---- proposals and selection on the GPU ------------------------------------------------
-// Proposal p of step t draws from stream (seed, t, p) until it finds a simple
-// 2-opt swap, with the host generator's attempt (propose_swap): a, ia, c, ic,
-// kind, edges (a, adj[a][ia]) and (c, adj[c][ic]) rewired to (a,d),(b,c)
-// (kind 1) or (a,c),(b,d) (kind 0), redrawn unless the four vertices are
-// distinct and neither new edge exists.
-#define GG_PROPOSE_MAX_ATTEMPTS 4096
 #define GG_SELECT_MAX_P 4096
 #define GG_EXPLORE_LANE 0xFFFFFFFFull
 
-// Row of v (d neighbours) in registers: 16-byte loads issued together when
-// d is a multiple of 4 (one memory latency), else one by one.
-template <int DM>
-struct PRow {
-    int u[DM];
-    __device__ __forceinline__ void load(const int *adjR, int v, int d) {
-        const int *r = adjR + (size_t)v * d;
-        // every index is a compile-time constant (predicated, no dynamic
-        // indexing), so the row stays in registers
-        if ((d & 3) == 0) {
-            const int4 *r4 = reinterpret_cast<const int4 *>(r);
-#pragma unroll
-            for (int q = 0; q < DM / 4; q++) {
-                int4 x = make_int4(-1, -1, -1, -1);
-                if (4 * q < d) x = r4[q];
-                u[4 * q] = x.x; u[4 * q + 1] = x.y; u[4 * q + 2] = x.z; u[4 * q + 3] = x.w;
-            }
-        } else {
-#pragma unroll
-            for (int i = 0; i < DM; i++) u[i] = i < d ? r[i] : -1;
-        }
-    }
-    __device__ __forceinline__ int slot_of(int d, int x) const {
-        int s = -1;
-#pragma unroll
-        for (int i = DM - 1; i >= 0; i--)
-            if (i < d && u[i] == x) s = i;
-        return s;
-    }
-};
-
-// Scalar fallback for degrees above the register row.
-__device__ __forceinline__ int row_slot_of(const int *row, int d, int x) {
-    for (int i = 0; i < d; i++)
-        if (row[i] == x) return i;
-    return -1;
-}
-
-#define GG_PROPOSE_ROW 32
-// ctl (pinned host memory): [0] step, [1] warm, [2] diameter; workgroup 0
-// copies them to dctl (read by the later kernels of the step) and clears the
-// error word dctl[3].
-__global__ void __launch_bounds__(64) k_propose(const int *adjR, int N, int d, uint64_t seed,
-                                                 const int *ctl, int P, int *moves, gg_swap *out,
-                                                 int *dctl) {
-    __shared__ int s_step;
-    if (threadIdx.x == 0) {
-        s_step = ctl[0];
-        if (blockIdx.x == 0) { dctl[0] = ctl[0]; dctl[1] = ctl[1]; dctl[2] = ctl[2]; }
-    }
-    __syncthreads();
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= P) return;
-    uint64_t st = sm64_stream(seed, (uint64_t)s_step, (uint64_t)p);
-    const bool reg = d <= GG_PROPOSE_ROW;
-    for (int att = 0; att < GG_PROPOSE_MAX_ATTEMPTS; att++) {
-        const int a = (int)sm64_below(st, N), ia = (int)sm64_below(st, d);
-        const int c = (int)sm64_below(st, N), ic = (int)sm64_below(st, d);
-        const int kind = (int)sm64_below(st, 2);
-        const int b = adjR[(size_t)a * d + ia], dd = adjR[(size_t)c * d + ic];
-        if (a == b || a == c || a == dd || b == c || b == dd || c == dd) continue;
-        const int xa = kind ? dd : c, xb = kind ? c : dd, xc = kind ? b : a, xd = kind ? a : b;
-        int ib, id;
-        if (reg) {
-            PRow<GG_PROPOSE_ROW> ra, rb, rd;
-            ra.load(adjR, a, d);
-            rb.load(adjR, b, d);
-            rd.load(adjR, dd, d);
-            if (ra.slot_of(d, xa) >= 0 || rb.slot_of(d, xb) >= 0) continue;
-            ib = rb.slot_of(d, a);
-            id = rd.slot_of(d, c);
-        } else {
-            const int *ra = adjR + (size_t)a * d, *rb = adjR + (size_t)b * d;
-            if (row_slot_of(ra, d, xa) >= 0 || row_slot_of(rb, d, xb) >= 0) continue;
-            ib = row_slot_of(rb, d, a);
-            id = row_slot_of(adjR + (size_t)dd * d, d, c);
-        }
-        if (ib < 0 || id < 0) continue;
-        gg_swap o;
-        o.v[0] = a;  o.slot[0] = ia; o.nbr[0] = xa;
-        o.v[1] = b;  o.slot[1] = ib; o.nbr[1] = xb;
-        o.v[2] = c;  o.slot[2] = ic; o.nbr[2] = xc;
-        o.v[3] = dd; o.slot[3] = id; o.nbr[3] = xd;
-        out[p] = o;
-        int *m = moves + (size_t)p * 5;
-        m[0] = a; m[1] = b; m[2] = c; m[3] = dd; m[4] = kind;
-        return;
-    }
-    atomicOr(dctl + 3, 1);  // no simple swap found: the host reports it
-    gg_swap none;
-    for (int j = 0; j < 4; j++) { none.v[j] = -1; none.slot[j] = 0; none.nbr[j] = 0; }
-    out[p] = none;
-    int *m = moves + (size_t)p * 5;
-    m[0] = m[1] = m[2] = m[3] = 0; m[4] = 0;
-}
+__global__ void __launch_bounds__(64) k_propose(ProposeArgs pa) { propose_body(pa, blockIdx.x); }
 
 // Rank of every score among the P (NaN as +inf, ties by index).  A workgroup
 // of 256 threads ranks 16 proposals: all P scores staged in LDS, thread t
@@ -716,8 +617,9 @@ int learn_eval_device(gg_search *s) {
             return code;
         };
         // the control words are read from pinned host memory by k_propose
-        hipLaunchKernelGGL(k_propose, dim3((P + 63) / 64), dim3(64), 0, L->st, b_adjR(b),
-                           s->N, s->d, s->cfg.seed, L->h_ctl, P, L->d_moves, L->d_psw, L->d_ctl);
+        hipLaunchKernelGGL(k_propose, dim3((P + 63) / 64), dim3(64), 0, L->st,
+                           ProposeArgs{b_adjR(b), s->N, s->d, s->cfg.seed, L->h_ctl, P,
+                                       L->d_moves, L->d_psw, L->d_ctl});
         if (hipGetLastError() != hipSuccess) return fail(GG_HIP_ERROR);
         if ((st = features_score_dev(b, L->mlp, L->d_moves, P, L->d_x, L->d_score, L->st,
                                      L->d_ctl + 2)) != GG_OK)
@@ -767,8 +669,19 @@ bool learn_loop_ready(const gg_search *s) {
            batch_single_launch(b, L->K);
 }
 
-// One resident-loop step's launches on the learned search's stream.
-static int loop_enqueue(gg_search *s) {
+// The resident loop's draw: the step number, warm-up flag and diameter are
+// read from the control block and copied to d_ctl, which the later kernels of
+// the step read.
+static ProposeArgs loop_propose_args(gg_search *s) {
+    LearnState *L = s->learn;
+    return ProposeArgs{b_adjR(L->batch), s->N, s->d, s->cfg.seed, (const int *)L->d_loop->hdr,
+                       L->P, L->d_moves, L->d_psw, L->d_ctl};
+}
+
+// One resident-loop step's launches on the learned search's stream.  first:
+// the step draws its own proposals (else the last step's commit launch drew
+// them); next: the commit launch also draws the next step's.
+static int loop_enqueue(gg_search *s, bool first, bool next) {
     LearnState *L = s->learn;
     gg_batch *b = L->batch;
     const int P = L->P, K = L->K;
@@ -788,12 +701,10 @@ static int loop_enqueue(gg_search *s) {
     a.done = L->d_done;
     const float lr = (float)(s->cfg.lr > 0 ? s->cfg.lr : 1e-3);
     int st = GG_OK;
-    // the header (step, warm-up flag, diameter) is read from the control
-    // block and copied to d_ctl, which the later kernels of the step read
-    hipLaunchKernelGGL(k_propose, dim3((P + 63) / 64), dim3(64), 0, L->st, b_adjR(b), s->N, s->d,
-                       s->cfg.seed, (const int *)L->d_loop->hdr, P, L->d_moves, L->d_psw,
-                       L->d_ctl);
-    GG_HIP(hipGetLastError());
+    if (first) {
+        hipLaunchKernelGGL(k_propose, dim3((P + 63) / 64), dim3(64), 0, L->st, loop_propose_args(s));
+        GG_HIP(hipGetLastError());
+    }
     // The last step's training step runs in this step's feature and rank
     // launches, on the CUs their own workgroups leave idle: its gradient
     // beside the features, its update beside the ranks.  The scores use the
@@ -815,7 +726,8 @@ static int loop_enqueue(gg_search *s) {
     if ((st = batch_enqueue_dev(b, K, &fin, &nvalid)) != GG_OK) return st;
     hipLaunchKernelGGL(k_finalize_accept, dim3(K), dim3(256), 0, L->st, fin, nvalid, a);
     GG_HIP(hipGetLastError());
-    return batch_enqueue_commit_dev(b, L->d_csel);
+    const ProposeArgs pn = loop_propose_args(s);
+    return batch_enqueue_commit_dev(b, L->d_csel, next ? &pn : nullptr);
 }
 
 // Captures `steps` consecutive steps into one executable graph.
@@ -825,7 +737,7 @@ static int loop_capture_n(gg_search *s, int steps, hipGraph_t *gr, hipGraphExec_
     if (*gr) { (void)hipGraphDestroy(*gr); *gr = nullptr; }
     GG_HIP(hipStreamBeginCapture(L->st, hipStreamCaptureModeThreadLocal));
     int st = GG_OK;
-    for (int i = 0; i < steps && st == GG_OK; i++) st = loop_enqueue(s);
+    for (int i = 0; i < steps && st == GG_OK; i++) st = loop_enqueue(s, i == 0, i + 1 < steps);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(L->st, &g);
     if (st != GG_OK) {
@@ -896,7 +808,7 @@ int learn_loop_run(gg_search *s, int steps, int *done) {
             GG_HIP(hipGraphLaunch(u > 1 ? L->lexec_u : L->lexec, L->st));
         } else {
             for (int q = 0; q < u; q++) {
-                int st = loop_enqueue(s);
+                int st = loop_enqueue(s, q == 0, q + 1 < u);
                 if (st != GG_OK) return st;
             }
         }
@@ -1127,8 +1039,8 @@ extern "C" int gg_batch_propose(gg_batch *b, uint64_t seed, int64_t step, int P,
     if (rc == GG_OK && hipMemcpyAsync(d_c, h_c, sizeof(h_c), hipMemcpyHostToDevice, st) != hipSuccess)
         rc = GG_HIP_ERROR;
     if (rc == GG_OK) {
-        hipLaunchKernelGGL(k_propose, dim3((P + 63) / 64), dim3(64), 0, st, b_adjR(b), N, d,
-                           seed, d_c, P, d_m, d_s, d_c + 4);
+        hipLaunchKernelGGL(k_propose, dim3((P + 63) / 64), dim3(64), 0, st,
+                           ProposeArgs{b_adjR(b), N, d, seed, d_c, P, d_m, d_s, d_c + 4});
         if (hipGetLastError() != hipSuccess) rc = GG_HIP_ERROR;
     }
     if (rc == GG_OK &&

@@ -1,0 +1,125 @@
+// gg_propose.hpp -- 2-opt proposals drawn on the GPU (the learned search's
+// draw, also carried by the commit launch of the step before).
+//
+// Proposal p of step t draws from stream (seed, t, p) until it finds a simple
+// 2-opt swap, with the host generator's attempt (propose_swap): a, ia, c, ic,
+// kind, edges (a, adj[a][ia]) and (c, adj[c][ic]) rewired to (a,d),(b,c)
+// (kind 1) or (a,c),(b,d) (kind 0), redrawn unless the four vertices are
+// distinct and neither new edge exists.
+#pragma once
+#include "gg_common.hpp"
+#include "gg_devrng.hpp"
+
+#define GG_PROPOSE_MAX_ATTEMPTS 4096
+#define GG_PROPOSE_ROW 32
+
+namespace gg {
+
+// Row of v (d neighbours) in registers: 16-byte loads issued together when
+// d is a multiple of 4 (one memory latency), else one by one.
+template <int DM>
+struct PRow {
+    int u[DM];
+    __device__ __forceinline__ void load(const int *adjR, int v, int d) {
+        const int *r = adjR + (size_t)v * d;
+        // every index is a compile-time constant (predicated, no dynamic
+        // indexing), so the row stays in registers
+        if ((d & 3) == 0) {
+            const int4 *r4 = reinterpret_cast<const int4 *>(r);
+#pragma unroll
+            for (int q = 0; q < DM / 4; q++) {
+                int4 x = make_int4(-1, -1, -1, -1);
+                if (4 * q < d) x = r4[q];
+                u[4 * q] = x.x; u[4 * q + 1] = x.y; u[4 * q + 2] = x.z; u[4 * q + 3] = x.w;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < DM; i++) u[i] = i < d ? r[i] : -1;
+        }
+    }
+    __device__ __forceinline__ int slot_of(int d, int x) const {
+        int s = -1;
+#pragma unroll
+        for (int i = DM - 1; i >= 0; i--)
+            if (i < d && u[i] == x) s = i;
+        return s;
+    }
+};
+
+// Scalar fallback for degrees above the register row.
+__device__ __forceinline__ int row_slot_of(const int *row, int d, int x) {
+    for (int i = 0; i < d; i++)
+        if (row[i] == x) return i;
+    return -1;
+}
+
+// ctl: [0] step, [1] warm, [2] diameter; workgroup 0 copies them to dctl
+// (read by the later kernels of the step).  A proposal with no simple swap
+// after GG_PROPOSE_MAX_ATTEMPTS ORs dctl[3] and writes an empty swap.
+struct ProposeArgs {
+    const int *adjR;
+    int N, d;
+    uint64_t seed;
+    const int *ctl;
+    int P;
+    int *moves;      // [P][5]: a, b, c, d, kind
+    gg_swap *out;    // [P]
+    int *dctl;
+};
+
+// The proposals of workgroup bid (64 threads, one proposal each).
+__device__ __forceinline__ void propose_body(const ProposeArgs &pa, int bid) {
+    __shared__ int s_step;
+    const int *adjR = pa.adjR;
+    const int N = pa.N, d = pa.d;
+    if (threadIdx.x == 0) {
+        s_step = pa.ctl[0];
+        if (bid == 0) { pa.dctl[0] = pa.ctl[0]; pa.dctl[1] = pa.ctl[1]; pa.dctl[2] = pa.ctl[2]; }
+    }
+    __syncthreads();
+    const int p = bid * 64 + threadIdx.x;
+    if (p >= pa.P) return;
+    uint64_t st = sm64_stream(pa.seed, (uint64_t)s_step, (uint64_t)p);
+    const bool reg = d <= GG_PROPOSE_ROW;
+    for (int att = 0; att < GG_PROPOSE_MAX_ATTEMPTS; att++) {
+        const int a = (int)sm64_below(st, N), ia = (int)sm64_below(st, d);
+        const int c = (int)sm64_below(st, N), ic = (int)sm64_below(st, d);
+        const int kind = (int)sm64_below(st, 2);
+        const int b = adjR[(size_t)a * d + ia], dd = adjR[(size_t)c * d + ic];
+        if (a == b || a == c || a == dd || b == c || b == dd || c == dd) continue;
+        const int xa = kind ? dd : c, xb = kind ? c : dd, xc = kind ? b : a, xd = kind ? a : b;
+        int ib, id;
+        if (reg) {
+            PRow<GG_PROPOSE_ROW> ra, rb, rd;
+            ra.load(adjR, a, d);
+            rb.load(adjR, b, d);
+            rd.load(adjR, dd, d);
+            if (ra.slot_of(d, xa) >= 0 || rb.slot_of(d, xb) >= 0) continue;
+            ib = rb.slot_of(d, a);
+            id = rd.slot_of(d, c);
+        } else {
+            const int *ra = adjR + (size_t)a * d, *rb = adjR + (size_t)b * d;
+            if (row_slot_of(ra, d, xa) >= 0 || row_slot_of(rb, d, xb) >= 0) continue;
+            ib = row_slot_of(rb, d, a);
+            id = row_slot_of(adjR + (size_t)dd * d, d, c);
+        }
+        if (ib < 0 || id < 0) continue;
+        gg_swap o;
+        o.v[0] = a;  o.slot[0] = ia; o.nbr[0] = xa;
+        o.v[1] = b;  o.slot[1] = ib; o.nbr[1] = xb;
+        o.v[2] = c;  o.slot[2] = ic; o.nbr[2] = xc;
+        o.v[3] = dd; o.slot[3] = id; o.nbr[3] = xd;
+        pa.out[p] = o;
+        int *m = pa.moves + (size_t)p * 5;
+        m[0] = a; m[1] = b; m[2] = c; m[3] = dd; m[4] = kind;
+        return;
+    }
+    atomicOr(pa.dctl + 3, 1);  // no simple swap found: the host reports it
+    gg_swap none;
+    for (int j = 0; j < 4; j++) { none.v[j] = -1; none.slot[j] = 0; none.nbr[j] = 0; }
+    pa.out[p] = none;
+    int *m = pa.moves + (size_t)p * 5;
+    m[0] = m[1] = m[2] = m[3] = 0; m[4] = 0;
+}
+
+}  // namespace gg
