@@ -233,25 +233,34 @@ __device__ int ss_bfs(const int *adjR, int N, int D, int lmax, int S, int x,
             // per four consecutive slots of a frontier vertex's row (one
             // 16-byte load)
             if (DT > 0 && fsize * D > BS) {
+                // two quads per lane, both row loads in flight together
                 constexpr int Q = DT > 0 ? DT / 4 : 1;
                 const int items = fsize * Q;
-                for (int i0 = wbase; i0 < items; i0 += BS) {
-                    const int it = i0 + lane;
-                    const bool ok = it < items;
-                    int y4[4] = {0, 0, 0, 0};
-                    if (ok) {
-                        const int e = it / Q, q = it - e * Q;
-                        const int v = lin[e];
-                        const int4 x = reinterpret_cast<const int4 *>(adjR + (size_t)v * D)[q];
-                        y4[0] = x.x; y4[1] = x.y; y4[2] = x.z; y4[3] = x.w;
+                for (int i0 = 2 * wbase; i0 < items; i0 += 2 * BS) {
+                    int y4[2][4];
+                    bool ok[2];
 #pragma unroll
-                        for (int i = 0; i < 4; i++)
+                    for (int h = 0; h < 2; h++) {
+                        const int it = i0 + 64 * h + lane;
+                        ok[h] = it < items;
+                        y4[h][0] = y4[h][1] = y4[h][2] = y4[h][3] = 0;
+                        if (ok[h]) {
+                            const int e = it / Q, q = it - e * Q;
+                            const int v = lin[e];
+                            const int4 x = reinterpret_cast<const int4 *>(adjR + (size_t)v * D)[q];
+                            y4[h][0] = x.x; y4[h][1] = x.y; y4[h][2] = x.z; y4[h][3] = x.w;
 #pragma unroll
-                            for (int j = 0; j < 4; j++)
-                                if (sw.v[j] == v && sw.s[j] == 4 * q + i) y4[i] = sw.n[j];
+                            for (int i = 0; i < 4; i++)
+#pragma unroll
+                                for (int j = 0; j < 4; j++)
+                                    if (sw.v[j] == v && sw.s[j] == 4 * q + i) y4[h][i] = sw.n[j];
+                        }
                     }
 #pragma unroll
-                    for (int i = 0; i < 4; i++) append(ok && claim(y4[i]), y4[i], true);
+                    for (int h = 0; h < 2; h++)
+#pragma unroll
+                        for (int i = 0; i < 4; i++)
+                            append(ok[h] && claim(y4[h][i]), y4[h][i], true);
                 }
             } else {
                 const int items = fsize * D;
@@ -287,26 +296,66 @@ __device__ int ss_bfs(const int *adjR, int N, int D, int lmax, int S, int x,
             const int nun = s_un;
             const bool listed = nun <= lc;
             const int span = listed ? nun : N;
-            for (int v0 = wbase; v0 < span; v0 += BS) {
-                const int e = v0 + lane;
-                const int v = listed ? (e < nun ? lout[e] : N) : e;
-                bool hit = false;
-                if (v < N && !((vis[v >> 5] >> (v & 31)) & 1u)) {
-                    if constexpr (DT > 0) {
-                        CandRow<DT> row;
-                        row.load(adjR, v, sw);
+            if constexpr (DT > 0) {
+                // two vertices per lane: the first eight slots of both rows
+                // loaded together (a bottom-up frontier is large, so a vertex
+                // nearly always meets it there), the rest only where not
+                auto quad_hit = [&](int v, int q, int4 x) {
+                    int y[4] = {x.x, x.y, x.z, x.w};
+                    bool hit = false;
 #pragma unroll
-                        for (int i = 0; i < DT; i++)
-                            hit = hit || ((front[row.u[i] >> 5] >> (row.u[i] & 31)) & 1u);
-                    } else {
+                    for (int i = 0; i < 4; i++) {
+#pragma unroll
+                        for (int j = 0; j < 4; j++)
+                            if (sw.v[j] == v && sw.s[j] == 4 * q + i) y[i] = sw.n[j];
+                        hit = hit || ((front[y[i] >> 5] >> (y[i] & 31)) & 1u);
+                    }
+                    return hit;
+                };
+                constexpr int Q = DT / 4;
+                for (int v0 = 2 * wbase; v0 < span; v0 += 2 * BS) {
+                    int vv[2];
+                    bool un[2];
+                    int4 x0[2], x1[2];
+#pragma unroll
+                    for (int h = 0; h < 2; h++) {
+                        const int e = v0 + 64 * h + lane;
+                        vv[h] = listed ? (e < nun ? lout[e] : N) : e;
+                        un[h] = vv[h] < N && !((vis[vv[h] >> 5] >> (vv[h] & 31)) & 1u);
+                        x0[h] = x1[h] = make_int4(0, 0, 0, 0);
+                        if (un[h]) {
+                            const int4 *r4 = reinterpret_cast<const int4 *>(adjR + (size_t)vv[h] * D);
+                            x0[h] = r4[0];
+                            if (Q > 1) x1[h] = r4[1];
+                        }
+                    }
+#pragma unroll
+                    for (int h = 0; h < 2; h++) {
+                        bool hit = false;
+                        if (un[h]) {
+                            const int v = vv[h];
+                            hit = quad_hit(v, 0, x0[h]) || (Q > 1 && quad_hit(v, 1, x1[h]));
+                            const int4 *r4 = reinterpret_cast<const int4 *>(adjR + (size_t)v * D);
+                            for (int q = 2; q < Q && !hit; q++) hit = quad_hit(v, q, r4[q]);
+                            if (hit) atomicOr(&nxt[v >> 5], 1u << (v & 31));
+                        }
+                        append(hit, vv[h], false);
+                    }
+                }
+            } else {
+                for (int v0 = wbase; v0 < span; v0 += BS) {
+                    const int e = v0 + lane;
+                    const int v = listed ? (e < nun ? lout[e] : N) : e;
+                    bool hit = false;
+                    if (v < N && !((vis[v >> 5] >> (v & 31)) & 1u)) {
                         for (int i = 0; i < D && !hit; i++) {
                             const int y = cand_nbr(adjR, D, v, i, sw);
                             hit = (front[y >> 5] >> (y & 31)) & 1u;
                         }
+                        if (hit) atomicOr(&nxt[v >> 5], 1u << (v & 31));
                     }
-                    if (hit) atomicOr(&nxt[v >> 5], 1u << (v & 31));
+                    append(hit, v, false);
                 }
-                append(hit, v, false);
             }
         }
         __syncthreads();
