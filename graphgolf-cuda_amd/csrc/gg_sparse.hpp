@@ -738,7 +738,10 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a) {
             }
             atomicAdd(&s_dc[k], (unsigned long long)((long long)__popcll(acc) -
                                                       (long long)__popcll(basew)));
-            if (none || !((pbits[v >> 5] >> (v & 31)) & 1u) || povr[v] != acc) s_diff[k] = 1;
+            // whether Delta_l differs from Delta_{l-1} matters only above the
+            // column's eccentricity (the fixed-point test)
+            if (l > ecc_w && (none || !((pbits[v >> 5] >> (v & 31)) & 1u) || povr[v] != acc))
+                s_diff[k] = 1;
         }
         __threadfence_block();
         __syncthreads();
