@@ -835,55 +835,74 @@ __device__ __forceinline__ void sparse_finalize_body(const SparseArgs &a, int nv
     const int W = a.W, L1 = a.lmax + 1;
     long long tot = 0, rch = 0;
     int diam = 0, lev = 0, fl = 0;
+    // the candidate's swap and its endpoints' single-source results (the
+    // same for every thread): loaded once, up front
+    constexpr int kLU = 8;
+    static_assert(GG_LEVEL_LIMIT >= kLU, "rows hold at least kLU + 1 levels");
+    const gg_swap sk = a.swaps[k];
+    int sse[4], ssf[4], ssc[4][kLU];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const size_t t4 = (size_t)4 * k + j;
+        sse[j] = a.ss_ecc[t4];
+        ssf[j] = a.ss_flags[t4];
+#pragma unroll
+        for (int i = 0; i < kLU; i++) ssc[j][i] = a.ss_cnt[t4 * L1 + i + 1];
+    }
     for (int w = tid; w < W; w += BS) {
         const size_t t = (size_t)k * W + w;
-        if (a.task_state[t] == GG_TASK_DENSE) {
+        const long long *br = a.base_reach + (size_t)w * L1;
+        const long long *dd = a.dcnt + t * L1;
+        // every load of the first kLU levels issued with the task's state (the
+        // entries past the column's levels are masked below, not skipped)
+        const int ts = a.task_state[t];
+        const int end = a.task_end[t];
+        const ColPart bp = a.base_parts[w];
+        const int tflags = a.task_flags[t];
+        long long pbr[kLU], pdd[kLU];
+#pragma unroll
+        for (int i = 0; i < kLU; i++) {
+            pbr[i] = br[i + 1];
+            pdd[i] = dd[i + 1];
+        }
+        if (ts == GG_TASK_DENSE) {
             const ColPart &p = a.dense_parts[t];
             a.task_ecc[t] = p.ecc;
             tot += p.total; rch += p.reached;
             diam = p.ecc > diam ? p.ecc : diam;
             lev = p.levels > lev ? p.levels : lev;
-            fl |= p.flags | (a.task_flags[t] & GG_FLAG_QUEUED);
+            fl |= p.flags | (tflags & GG_FLAG_QUEUED);
             continue;
         }
-        const int end = a.task_end[t];
-        const ColPart bp = a.base_parts[w];
-        const long long *br = a.base_reach + (size_t)w * L1;
-        const long long *dd = a.dcnt + t * L1;
         // endpoint sources of this column (masked in the sparse pass): their
         // single-source counts replace the base's
-        const gg_swap &sk = a.swaps[k];
         const unsigned long long em = endpoint_mask(sk, w);
         const long long nm = __popcll(em);
         int top = end, sfl = 0;
-        if (em)
-            for (int j = 0; j < 4; j++) {
-                if (!endpoint_first(sk, j) || (sk.v[j] >> 6) != w) continue;
-                const int ec = a.ss_ecc[(size_t)4 * k + j];
-                top = ec > top ? ec : top;
-                sfl |= a.ss_flags[(size_t)4 * k + j];
-            }
+        bool mine[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            mine[j] = em && endpoint_first(sk, j) && (sk.v[j] >> 6) == w;
+            if (!mine[j]) continue;
+            top = sse[j] > top ? sse[j] : top;
+            sfl |= ssf[j];
+        }
         if (top > a.lmax) top = a.lmax;
-        // the first kLU levels unrolled (their loads issued together, every
-        // array index a constant), later levels one by one
-        constexpr int kLU = 8;
         long long cb = (w == W - 1) ? nvalid_last : 64;  // base reached after level 0
         long long cprev = cb, ctot = 0, cx = nm, dend = 0;
         int ecc = 0;
-        long long pbr[kLU], pdd[kLU], pcx[kLU], pbx[kLU];
+        long long pcx[kLU], pbx[kLU];
 #pragma unroll
         for (int i = 0; i < kLU; i++) {
             const int l = i + 1;
-            pbr[i] = (l <= bp.levels && l < L1) ? br[l] : 0;
-            pdd[i] = (l <= end) ? dd[l] : 0;
+            if (!(l <= bp.levels && l < L1)) pbr[i] = 0;
+            if (!(l <= end)) pdd[i] = 0;
             pcx[i] = 0;
             pbx[i] = 0;
             if (em && l <= top) {
-                for (int j = 0; j < 4; j++) {
-                    if (!endpoint_first(sk, j) || (sk.v[j] >> 6) != w) continue;
-                    const size_t t4 = (size_t)4 * k + j;
-                    if (l <= a.ss_ecc[t4]) pcx[i] += a.ss_cnt[t4 * L1 + l];
-                }
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    if (mine[j] && l <= sse[j]) pcx[i] += ssc[j][i];
                 pbx[i] = l < bp.ecc ? a.mb_cnt[t * (a.S + 1) + l] : (long long)a.N * nm;
             }
             if (l == end) dend = pdd[i];
@@ -905,9 +924,9 @@ __device__ __forceinline__ void sparse_finalize_body(const SparseArgs &a, int nv
             long long bx = 0;
             if (em) {
                 for (int j = 0; j < 4; j++) {
-                    if (!endpoint_first(sk, j) || (sk.v[j] >> 6) != w) continue;
+                    if (!mine[j]) continue;
                     const size_t t4 = (size_t)4 * k + j;
-                    if (l <= a.ss_ecc[t4]) cx += a.ss_cnt[t4 * L1 + l];
+                    if (l <= sse[j]) cx += a.ss_cnt[t4 * L1 + l];
                 }
                 bx = l < bp.ecc ? a.mb_cnt[t * (a.S + 1) + l] : (long long)a.N * nm;
             }
@@ -919,7 +938,7 @@ __device__ __forceinline__ void sparse_finalize_body(const SparseArgs &a, int nv
         tot += ctot;
         rch += cprev;
         diam = ecc > diam ? ecc : diam;
-        const int tf = a.task_flags[t] | sfl;
+        const int tf = tflags | sfl;
         int lv = tf ? top : ecc + 1;
         lev = lv > lev ? lv : lev;
         fl |= tf;
