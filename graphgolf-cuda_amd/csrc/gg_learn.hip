@@ -84,6 +84,12 @@ __device__ __forceinline__ bool in_ball1(const FeatArgs &f, int u, int z) {
 // row fits (W <= GG_FEAT_LDS_W words), else read from the stored levels.
 #define GG_FEAT_LDS_W 256
 
+// Value 0..3 selected by a wave-uniform or per-lane index without an indexed
+// register array.
+__device__ __forceinline__ int pick4(int v0, int v1, int v2, int v3, int i) {
+    return i == 0 ? v0 : i == 1 ? v1 : i == 2 ? v2 : v3;
+}
+
 // U(u,v) for the four directions (u,v) = (a,b), (b,a), (c,d), (d,c)
 // (u = endpoint e, v = endpoint e^1): lanes over the 4d (direction, entry z
 // of v's row) items; a lane reads z's row and tests its entries against u's
@@ -93,13 +99,14 @@ __device__ void uniq_second_layer4(const FeatArgs &f, const int ends[4],
                                    int out[4]) {
     auto ball1 = [&](int e, int z) -> bool {
         if (r1) return (r1[e][z >> 6] >> (z & 63)) & 1ull;
-        return in_ball1(f, ends[e], z);
+        return in_ball1(f, pick4(ends[0], ends[1], ends[2], ends[3], e), z);
     };
     int part[4] = {0, 0, 0, 0};
     const int items = 4 * f.d;
     for (int k = lane; k < items; k += 64) {
         const int e = k / f.d, i = k - e * f.d;
-        const int u = ends[e], v = ends[e ^ 1];
+        const int u = pick4(ends[0], ends[1], ends[2], ends[3], e),
+                  v = pick4(ends[1], ends[0], ends[3], ends[2], e);
         const int z = f.adjR[(size_t)v * f.d + i];
         if (z == u || ball1(e, z)) continue;
         const int *rz = f.adjR + (size_t)z * f.d;
@@ -121,31 +128,42 @@ __device__ void uniq_second_layer4(const FeatArgs &f, const int ends[4],
     }
 }
 
-// The 32 features of proposal p into xf (lane 0; the counts are wave totals).
-__device__ __forceinline__ void write_features(const FeatArgs &f, int p, int a, int b, int c, int dd, int kind,
-                               const float *cnt, const int *eccl, const int *uq, float *X) {
-    (void)p;
-    const int p1 = a, q1 = kind ? dd : c, p2 = b, q2 = kind ? c : dd;
-    const int diam = f.diam_dev ? (*f.diam_dev > 0 ? *f.diam_dev : 1) : f.diam;
-    // distance of each added pair before the swap: levels 1-4 of the ball row
-    // loaded together (one memory latency), later levels one by one
-    auto dist = [&](int x, int y) {
-        const int w = y >> 6, ew = f.base_parts[w].ecc, bit = y & 63;
-        unsigned long long b[4];
+// Diameter the features normalise by (the device copy when given).
+__device__ __forceinline__ int feat_diam(const FeatArgs &f) {
+    return f.diam_dev ? (*f.diam_dev > 0 ? *f.diam_dev : 1) : f.diam;
+}
+
+// d(x, y) before the swap given the words of x's ball rows at y's column for
+// levels 1..3 (b3): the first level whose word holds y, levels past 3 read one
+// by one (levels at or above the column's eccentricity are full).
+__device__ __forceinline__ int pair_dist(const FeatArgs &f, int x, int y, int ew,
+                                         const unsigned long long b3[3]) {
+    const int w = y >> 6, bit = y & 63;
 #pragma unroll
-        for (int l = 1; l <= 4; l++) b[l - 1] = ball_word(f, l, w, x, ew);
+    for (int l = 1; l <= 3; l++)
+        if ((b3[l - 1] >> bit) & 1ull) return l;
+    for (int l = 4; l < 64; l++)
+        if ((ball_word(f, l, w, x, ew) >> bit) & 1ull) return l;
+    return 64;
+}
+
+__device__ __forceinline__ int pair_dist(const FeatArgs &f, int x, int y) {
+    const int w = y >> 6, ew = f.base_parts[w].ecc;
+    unsigned long long b3[3];
 #pragma unroll
-        for (int l = 1; l <= 4; l++)
-            if ((b[l - 1] >> bit) & 1ull) return l;
-        for (int l = 5; l < 64; l++)
-            if ((ball_word(f, l, w, x, ew) >> bit) & 1ull) return l;
-        return 64;
-    };
+    for (int l = 1; l <= 3; l++) b3[l - 1] = ball_word(f, l, w, x, ew);
+    return pair_dist(f, x, y, ew, b3);
+}
+
+// The 32 features of a proposal into X (lane 0; the counts are wave totals,
+// dist[t] the pre-swap distance of added edge t).
+__device__ __forceinline__ void write_features(const FeatArgs &f, int diam, const int dist[2],
+                                               const float *cnt, const int *eccl, const int *uq,
+                                               float *X) {
     const float d1 = (float)f.d, d2 = d1 * d1, n = (float)f.N, inv_diam = 1.f / (float)diam;
-    const int qs[2] = {q1, q2}, ps[2] = {p1, p2};
 #pragma unroll
     for (int t = 0; t < 2; t++) {
-        X[t * 8 + 0] = dist(ps[t], qs[t]) * inv_diam;
+        X[t * 8 + 0] = dist[t] * inv_diam;
         X[t * 8 + 1] = cnt[t * 7 + 0] / n;
         X[t * 8 + 2] = cnt[t * 7 + 1] / n;
         X[t * 8 + 3] = cnt[t * 7 + 2] / d2;
@@ -167,8 +185,360 @@ __device__ __forceinline__ void write_features(const FeatArgs &f, int p, int a, 
     for (int e = 0; e < 4; e++) X[28 + e] = cnt[20 + e] / n;
 }
 
-// One wave per proposal (workgroup bid: proposals 4 bid ..).  Counts are
-// wave-reduced popcounts over the W words.
+// Counts of one column w given the endpoints' ball words of levels 0..3
+// (R[endpoint][level]) and the column's eccentricity ew: the per-lane partial
+// counts and the endpoints' full levels.
+template <class T>
+__device__ __forceinline__ void column_counts(const FeatArgs &f, int w, int ew, const int ends[4],
+                                              int kind, const unsigned long long (&R)[4][4],
+                                              T *cnt, int *eccl) {
+    const unsigned long long valid = ~pad_bits(f, w);
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        int fl = 0;
+        for (int l = 0; l <= ew; l++) {
+            const unsigned long long x = l < 4 ? R[e][l] : ball_word(f, l, w, ends[e], ew);
+            if (x == ~0ull) { fl = l; break; }
+        }
+        eccl[e] = fl > eccl[e] ? fl : eccl[e];
+    }
+    auto L = [&](int e, int l) { return R[e][l] & ~R[e][l - 1]; };
+    // index of p1,q1,p2,q2 in ends: a=0,b=1,c=2,dd=3
+    const int ip1 = 0, iq1 = kind ? 3 : 2, ip2 = 1, iq2 = kind ? 2 : 3;
+    const int pairs[2][2] = {{ip1, iq1}, {ip2, iq2}};
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+        const int x = pairs[t][0], y = pairs[t][1];
+        cnt[t * 7 + 0] += __popcll(L(x, 3) & L(y, 3));
+        cnt[t * 7 + 1] += __popcll(~R[x][3] & ~R[y][3]);
+        cnt[t * 7 + 2] += __popcll(L(x, 2) & L(y, 2));
+        cnt[t * 7 + 3] += __popcll(L(y, 2) & ~R[x][3]);
+        cnt[t * 7 + 4] += __popcll(L(x, 2) & ~R[y][3]);
+        cnt[t * 7 + 5] += __popcll(L(y, 1) & ~R[x][3]) + __popcll(L(x, 1) & ~R[y][3]);
+        cnt[t * 7 + 6] += __popcll(L(y, 1) & ~R[x][2]) + __popcll(L(x, 1) & ~R[y][2]);
+    }
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+        const int x = 2 * t, y = 2 * t + 1;
+        cnt[14 + t * 3 + 0] += __popcll(L(x, 2) & L(y, 2));
+        cnt[14 + t * 3 + 1] += __popcll(R[x][2] & R[y][2] & valid);
+        cnt[14 + t * 3 + 2] += __popcll(L(x, 1) & L(y, 2)) + __popcll(L(x, 2) & L(y, 1));
+    }
+#pragma unroll
+    for (int e = 0; e < 4; e++) cnt[20 + e] += __popcll(~R[e][3]);
+}
+
+// bf16 pair of a 32-bit word as floats (element 0 in the low half).
+__device__ __forceinline__ float bf_lo(unsigned u) { return __uint_as_float(u << 16); }
+__device__ __forceinline__ float bf_hi(unsigned u) { return __uint_as_float(u & 0xffff0000u); }
+
+// The MLP score of a proposal from its features xs (every lane), with the
+// arithmetic of k_mlp_forward (same products, fma order and sums; products
+// rounded before they are summed, as k_mlp_forward's pass through LDS does):
+// bit-identical.  w1 holds lane's hidden units lane and lane + 64 (rows of
+// W1 as bf16 pairs), b1/w2 their bias and output weight.
+__device__ __forceinline__ float mlp_score(const float xs[GG_FEATS],
+                                           const uint4 (&w1)[GG_HIDDEN / 64][GG_FEATS / 8],
+                                           const float (&b1)[GG_HIDDEN / 64],
+                                           const float (&w2)[GG_HIDDEN / 64], float b2) {
+#pragma clang fp contract(off)
+    float o = 0.f;
+#pragma unroll
+    for (int q = 0; q < GG_HIDDEN / 64; q++) {
+        float acc = b1[q];
+#pragma unroll
+        for (int i = 0; i < GG_FEATS / 8; i++) {
+            const uint4 u = w1[q][i];
+            acc = fmaf(bf_lo(u.x), xs[8 * i + 0], acc);
+            acc = fmaf(bf_hi(u.x), xs[8 * i + 1], acc);
+            acc = fmaf(bf_lo(u.y), xs[8 * i + 2], acc);
+            acc = fmaf(bf_hi(u.y), xs[8 * i + 3], acc);
+            acc = fmaf(bf_lo(u.z), xs[8 * i + 4], acc);
+            acc = fmaf(bf_hi(u.z), xs[8 * i + 5], acc);
+            acc = fmaf(bf_lo(u.w), xs[8 * i + 6], acc);
+            acc = fmaf(bf_hi(u.w), xs[8 * i + 7], acc);
+        }
+        const float r = w2[q] * (acc > 0.f ? acc : 0.f);
+        o = o + r;
+    }
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) o = o + __shfl_xor(o, d, 64);
+    return o + b2;
+}
+
+// Lane's MLP weights (hidden units lane, lane + 64) from the bf16 copy.
+struct LaneWeights {
+    uint4 w1[GG_HIDDEN / 64][GG_FEATS / 8];
+    float b1[GG_HIDDEN / 64], w2[GG_HIDDEN / 64], b2;
+};
+__device__ __forceinline__ void load_lane_weights(const __hip_bfloat16 *wq, int lane,
+                                                  LaneWeights &lw) {
+#pragma unroll
+    for (int q = 0; q < GG_HIDDEN / 64; q++) {
+        const int h = lane + 64 * q;
+        const uint4 *r = reinterpret_cast<const uint4 *>(wq + OFF_W1 + h * GG_FEATS);
+#pragma unroll
+        for (int i = 0; i < GG_FEATS / 8; i++) lw.w1[q][i] = r[i];
+        lw.b1[q] = bf(wq[OFF_B1 + h]);
+        lw.w2[q] = bf(wq[OFF_W2 + h]);
+    }
+    lw.b2 = bf(wq[OFF_B2]);
+}
+
+// Shape of the feature pass with one workgroup per proposal: row-major
+// levels 1..3 kept, at most 256 columns (one per thread, the radius-1 rows
+// staged in LDS), degree a multiple of 4 up to 32 (the 4d second-layer
+// entries one per thread, their rows as 16-byte loads).
+#define GG_FEAT_FAST_D 32
+__host__ __device__ inline bool features_fast_ok(const FeatArgs &f) {
+    return f.rows && f.RL >= 3 && f.W <= GG_FEAT_LDS_W && f.d <= GG_FEAT_FAST_D &&
+           (f.d & 3) == 0;
+}
+
+// Workgroups of the feature pass for P proposals.
+__host__ inline int features_groups(const FeatArgs &f) {
+    return features_fast_ok(f) ? f.P : (f.P + 3) / 4;
+}
+
+// Proposal p by one workgroup of 256 threads: thread t takes column t and
+// second-layer entry t, threads 0 and 1 the added edges' distances, threads
+// below 128 the hidden units of the score.  Every global load is issued
+// before any is used (three dependent rounds: the move, the level words and
+// entries, the entries' rows).  Counts are summed per wave, then over the
+// four waves (integers held exactly in fp32: any order gives the same sum).
+// Wave sums of 12 words of two 16-bit counts each, reduce-scattered: on
+// return lane L holds the wave total of word (L&32 ? 6 : 0) + (L&16 ? 3 : 0) +
+// (L&8 ? 2 : 0) + (L&4 ? 1 : 0) (no word where L&8 and L&4 are both set).
+// Halves cannot carry: every lane's half is at most 128, so a total is at
+// most 8192.
+__device__ __forceinline__ unsigned wave_sum12_scatter(unsigned (&pk)[12], int lane) {
+    const bool b5 = lane & 32, b4 = lane & 16, b3 = lane & 8, b2 = lane & 4;
+#pragma unroll
+    for (int j = 0; j < 6; j++) {
+        const unsigned mine = b5 ? pk[j + 6] : pk[j], other = b5 ? pk[j] : pk[j + 6];
+        pk[j] = mine + (unsigned)__shfl_xor((int)other, 32, 64);
+    }
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        const unsigned mine = b4 ? pk[j + 3] : pk[j], other = b4 ? pk[j] : pk[j + 3];
+        pk[j] = mine + (unsigned)__shfl_xor((int)other, 16, 64);
+    }
+    unsigned q[2];
+    {
+        const unsigned hi0 = pk[2], hi1 = 0u;
+        const unsigned m0 = b3 ? hi0 : pk[0], o0 = b3 ? pk[0] : hi0;
+        const unsigned m1 = b3 ? hi1 : pk[1], o1 = b3 ? pk[1] : hi1;
+        q[0] = m0 + (unsigned)__shfl_xor((int)o0, 8, 64);
+        q[1] = m1 + (unsigned)__shfl_xor((int)o1, 8, 64);
+    }
+    unsigned x;
+    {
+        const unsigned mine = b2 ? q[1] : q[0], other = b2 ? q[0] : q[1];
+        x = mine + (unsigned)__shfl_xor((int)other, 4, 64);
+    }
+    x += (unsigned)__shfl_xor((int)x, 2, 64);
+    x += (unsigned)__shfl_xor((int)x, 1, 64);
+    return x;
+}
+
+struct FeatWgLds {
+    unsigned long long r1[4][GG_FEAT_LDS_W];  // R_1 of the endpoints
+    unsigned cnt[4][12];                       // per wave: counts 2j, 2j+1 as 16-bit halves
+    int ecc[4][4];
+    unsigned uq[4];                            // per wave: U counts as bytes
+    int dist[2];
+    float x[GG_FEATS];
+    float rh[GG_HIDDEN];
+};
+
+// Proposal p by one workgroup of 256 threads: thread t takes column t and
+// second-layer entry t, threads 0 and 1 the added edges' distances, threads
+// below 128 the hidden units of the score.  Every global load is issued
+// before any is used (three dependent rounds: the move, the level words and
+// entries, the entries' rows).  Counts are integers summed per wave, then
+// over the four waves; lane i of wave 0 writes feature i.
+__device__ __forceinline__ void features_wg_body(const FeatArgs &f, int p, FeatWgLds &L) {
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int W = f.W, N = f.N, d = f.d, items = 4 * d;
+    const int *m = f.moves + (size_t)p * 5;
+    const int a = m[0], b = m[1], c = m[2], dd = m[3], kind = m[4];
+    const bool wodd = f.score && f.wpar && (*f.wpar & 1);
+    const int p1 = a, q1 = kind ? dd : c, p2 = b, q2 = kind ? c : dd;
+    const int ends[4] = {a, b, c, dd};
+    const int diam = feat_diam(f);
+    // (1) second-layer entry z (item k = tid: endpoint e = k / d, entry k % d
+    // of the row of e's partner e ^ 1); index clamped so every thread loads
+    const int k = tid < items ? tid : items - 1;
+    const int ue = k / d;
+    const int uz = f.adjR[(size_t)pick4(b, a, dd, c, ue) * d + (k - ue * d)];
+    // (2) the endpoints' ball words of levels 1..3 at column tid
+    const int w = tid < W ? tid : W - 1;
+    const int ew = f.base_parts[w].ecc;
+    unsigned long long R[4][4];
+#pragma unroll
+    for (int e = 0; e < 4; e++)
+#pragma unroll
+        for (int l = 1; l <= 3; l++) R[e][l] = f.rows[((size_t)(l - 1) * N + ends[e]) * W + w];
+    // (3) added edge t = tid & 1: x's words at y's column
+    const int tx = (tid & 1) ? p2 : p1, ty = (tid & 1) ? q2 : q1;
+    const int ewd = f.base_parts[ty >> 6].ecc;
+    unsigned long long b3[3];
+#pragma unroll
+    for (int l = 1; l <= 3; l++) b3[l - 1] = f.rows[((size_t)(l - 1) * N + tx) * W + (ty >> 6)];
+    const __hip_bfloat16 *wq = wodd ? f.wq2 : f.wq;
+    // (4) the row of the second-layer entry
+    int4 ur[GG_FEAT_FAST_D / 4];
+    {
+        const int4 *r = reinterpret_cast<const int4 *>(f.adjR + (size_t)uz * d);
+#pragma unroll
+        for (int j = 0; j < GG_FEAT_FAST_D / 4; j++)
+            ur[j] = 4 * j < d ? r[j] : make_int4(-1, -1, -1, -1);
+    }
+    // counts of column tid (each at most 128)
+    int cnt[24];
+#pragma unroll
+    for (int i = 0; i < 24; i++) cnt[i] = 0;
+    int eccl[4] = {0, 0, 0, 0};
+    if (tid < W) {
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            R[e][0] = ball_word(f, 0, tid, ends[e], ew);
+            L.r1[e][tid] = R[e][1];
+        }
+        column_counts(f, tid, ew, ends, kind, R, cnt, eccl);
+    }
+    {
+        unsigned pk[12];
+#pragma unroll
+        for (int j = 0; j < 12; j++) pk[j] = (unsigned)cnt[2 * j] | ((unsigned)cnt[2 * j + 1] << 16);
+        const unsigned tot = wave_sum12_scatter(pk, lane);
+        const int word = ((lane & 32) ? 6 : 0) + ((lane & 16) ? 3 : 0) + ((lane & 8) ? 2 : 0) +
+                         ((lane & 4) ? 1 : 0);
+        if ((lane & 3) == 0 && !((lane & 8) && (lane & 4))) L.cnt[wv][word] = tot;
+    }
+#pragma unroll
+    for (int e = 0; e < 4; e++) eccl[e] = wave_max32(eccl[e]);
+    if (lane == 0) {
+#pragma unroll
+        for (int e = 0; e < 4; e++) L.ecc[wv][e] = eccl[e];
+    }
+    if (tid < 2) L.dist[tid] = pair_dist(f, tx, ty, ewd, b3);
+    __syncthreads();
+    // U(u, v) of the four directions: entry z of v's row not in B1(u) with
+    // exactly one neighbour in B1(u); per-direction counts (at most d <= 32)
+    // as the bytes of one word
+    {
+        auto ball1 = [&](int e, int z) -> bool {
+            return (L.r1[e][z >> 6] >> (z & 63)) & 1ull;
+        };
+        int hit = 0;
+        if (tid < items && !(uz == pick4(a, b, c, dd, ue) || ball1(ue, uz))) {
+            int common = 0;
+#pragma unroll
+            for (int j = 0; j < GG_FEAT_FAST_D / 4; j++) {
+                if (4 * j >= d) break;
+                const int4 q = ur[j];
+                common += (ball1(ue, q.x) ? 1 : 0) + (ball1(ue, q.y) ? 1 : 0) +
+                          (ball1(ue, q.z) ? 1 : 0) + (ball1(ue, q.w) ? 1 : 0);
+            }
+            hit = common == 1 ? 1 : 0;
+        }
+        unsigned x = (unsigned)hit << (8 * ue);
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) x += (unsigned)__shfl_xor((int)x, o, 64);
+        if (lane == 0) L.uq[wv] = x;
+    }
+    __syncthreads();
+    if (wv == 0 && lane < GG_FEATS) {
+        // feature `lane`, with the expression of write_features
+        const int i = lane;
+        auto count = [&](int j) {
+            const int h = (j & 1) ? 16 : 0, wj = j >> 1;
+            return (int)(((L.cnt[0][wj] >> h) & 0xffffu) + ((L.cnt[1][wj] >> h) & 0xffffu) +
+                         ((L.cnt[2][wj] >> h) & 0xffffu) + ((L.cnt[3][wj] >> h) & 0xffffu));
+        };
+        const float d1 = (float)d, d2 = d1 * d1, n = (float)N, inv_diam = 1.f / (float)diam;
+        float x;
+        if (i < 16) {
+            const int t = i >> 3, r = i & 7;
+            if (r == 0) {
+                x = L.dist[t] * inv_diam;
+            } else {
+                const float den = r <= 2 ? n : r <= 5 ? d2 : d1;
+                x = (float)count(t * 7 + r - 1) / den;
+            }
+        } else if (i < 28) {
+            const int t = (i - 16) / 6, r = (i - 16) - 6 * t;
+            if (r < 2) {
+                const int e = 2 * t + r;
+                const unsigned u = L.uq[0] + L.uq[1] + L.uq[2] + L.uq[3];
+                x = (float)(int)((u >> (8 * e)) & 0xffu) / d1;
+            } else if (r < 5) {
+                x = (float)count(14 + t * 3 + r - 2) / d2;
+            } else {
+                int e0 = 0, e1 = 0;
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    e0 = L.ecc[q][2 * t] > e0 ? L.ecc[q][2 * t] : e0;
+                    e1 = L.ecc[q][2 * t + 1] > e1 ? L.ecc[q][2 * t + 1] : e1;
+                }
+                x = (float)(e0 + e1 - 2 * diam) * 0.5f;
+            }
+        } else {
+            x = (float)count(20 + i - 28) / n;
+        }
+        f.X[(size_t)p * GG_FEATS + i] = x;
+        L.x[i] = x;
+    }
+    if (!f.score) return;
+    // hidden unit tid of the score (its weights loaded while wave 0 writes
+    // the features)
+    uint4 w1[GG_FEATS / 8];
+    float b1 = 0.f, w2 = 0.f;
+    if (tid < GG_HIDDEN) {
+        const uint4 *r = reinterpret_cast<const uint4 *>(wq + OFF_W1 + tid * GG_FEATS);
+#pragma unroll
+        for (int i = 0; i < GG_FEATS / 8; i++) w1[i] = r[i];
+        b1 = bf(wq[OFF_B1 + tid]);
+        w2 = bf(wq[OFF_W2 + tid]);
+    }
+    __syncthreads();
+    // the MLP score with the arithmetic of k_mlp_forward (same products, fma
+    // order and sums; products rounded before they are summed, as
+    // k_mlp_forward's pass through LDS does): bit-identical
+    {
+#pragma clang fp contract(off)
+        if (tid < GG_HIDDEN) {
+            float acc = b1;
+#pragma unroll
+            for (int i = 0; i < GG_FEATS / 8; i++) {
+                const uint4 u = w1[i];
+                acc = fmaf(bf_lo(u.x), L.x[8 * i + 0], acc);
+                acc = fmaf(bf_hi(u.x), L.x[8 * i + 1], acc);
+                acc = fmaf(bf_lo(u.y), L.x[8 * i + 2], acc);
+                acc = fmaf(bf_hi(u.y), L.x[8 * i + 3], acc);
+                acc = fmaf(bf_lo(u.z), L.x[8 * i + 4], acc);
+                acc = fmaf(bf_hi(u.z), L.x[8 * i + 5], acc);
+                acc = fmaf(bf_lo(u.w), L.x[8 * i + 6], acc);
+                acc = fmaf(bf_hi(u.w), L.x[8 * i + 7], acc);
+            }
+            L.rh[tid] = w2 * (acc > 0.f ? acc : 0.f);
+        }
+        __syncthreads();
+        if (wv == 0) {
+            float o = 0.f;
+#pragma unroll
+            for (int q = 0; q < GG_HIDDEN / 64; q++) o = o + L.rh[lane + 64 * q];
+#pragma unroll
+            for (int x = 32; x > 0; x >>= 1) o = o + __shfl_xor(o, x, 64);
+            if (lane == 0) f.score[p] = o + bf(wq[OFF_B2]);
+        }
+    }
+}
+
+// One wave per proposal (workgroup bid: proposals 4 bid ..) where the
+// workgroup pass does not apply.  Counts are wave-reduced popcounts over the
+// W words.
 __device__ __forceinline__ void features_body(const FeatArgs &f, int bid) {
     __shared__ unsigned long long s_r1[4][4][GG_FEAT_LDS_W];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -180,6 +550,7 @@ __device__ __forceinline__ void features_body(const FeatArgs &f, int bid) {
     const bool stage = f.W <= GG_FEAT_LDS_W;
     const int *m = f.moves + (size_t)p * 5;
     const int a = m[0], b = m[1], c = m[2], dd = m[3], kind = m[4];
+    const bool wodd = f.score && f.wpar && (*f.wpar & 1);
     // added edges (p1,q1),(p2,q2); removed (a,b),(c,dd)
     const int p1 = a, q1 = kind ? dd : c, p2 = b, q2 = kind ? c : dd;
     const int ends[4] = {a, b, c, dd};
@@ -190,9 +561,14 @@ __device__ __forceinline__ void features_body(const FeatArgs &f, int bid) {
     // eccentricity of each endpoint x: the first level whose ball row R_l(x)
     // is full in every column (per lane over its columns, then the wave max)
     int eccl[4] = {0, 0, 0, 0};
+    int uq[4];
+    int dist[2];
+    float xf[GG_FEATS];
+#pragma unroll
+    for (int i = 0; i < GG_FEATS; i++) xf[i] = 0.f;
+    const int diam = feat_diam(f);
     for (int w = lane; w < f.W; w += 64) {
         const int ew = f.base_parts[w].ecc;
-        const unsigned long long valid = ~pad_bits(f, w);
         unsigned long long R[4][4];  // R[endpoint][level 0..3]
 #pragma unroll
         for (int e = 0; e < 4; e++)
@@ -202,39 +578,7 @@ __device__ __forceinline__ void features_body(const FeatArgs &f, int bid) {
 #pragma unroll
             for (int e = 0; e < 4; e++) s_r1[wv][e][w] = R[e][1];
         }
-#pragma unroll
-        for (int e = 0; e < 4; e++) {
-            int fl = 0;
-            for (int l = 0; l <= ew; l++) {
-                const unsigned long long x = l < 4 ? R[e][l] : ball_word(f, l, w, ends[e], ew);
-                if (x == ~0ull) { fl = l; break; }
-            }
-            eccl[e] = fl > eccl[e] ? fl : eccl[e];
-        }
-        auto L = [&](int e, int l) { return R[e][l] & ~R[e][l - 1]; };
-        // index of p1,q1,p2,q2 in ends: a=0,b=1,c=2,dd=3
-        const int ip1 = 0, iq1 = kind ? 3 : 2, ip2 = 1, iq2 = kind ? 2 : 3;
-        const int pairs[2][2] = {{ip1, iq1}, {ip2, iq2}};
-#pragma unroll
-        for (int t = 0; t < 2; t++) {
-            const int x = pairs[t][0], y = pairs[t][1];
-            cnt[t * 7 + 0] += __popcll(L(x, 3) & L(y, 3));
-            cnt[t * 7 + 1] += __popcll(~R[x][3] & ~R[y][3]);
-            cnt[t * 7 + 2] += __popcll(L(x, 2) & L(y, 2));
-            cnt[t * 7 + 3] += __popcll(L(y, 2) & ~R[x][3]);
-            cnt[t * 7 + 4] += __popcll(L(x, 2) & ~R[y][3]);
-            cnt[t * 7 + 5] += __popcll(L(y, 1) & ~R[x][3]) + __popcll(L(x, 1) & ~R[y][3]);
-            cnt[t * 7 + 6] += __popcll(L(y, 1) & ~R[x][2]) + __popcll(L(x, 1) & ~R[y][2]);
-        }
-#pragma unroll
-        for (int t = 0; t < 2; t++) {
-            const int x = 2 * t, y = 2 * t + 1;
-            cnt[14 + t * 3 + 0] += __popcll(L(x, 2) & L(y, 2));
-            cnt[14 + t * 3 + 1] += __popcll(R[x][2] & R[y][2] & valid);
-            cnt[14 + t * 3 + 2] += __popcll(L(x, 1) & L(y, 2)) + __popcll(L(x, 2) & L(y, 1));
-        }
-#pragma unroll
-        for (int e = 0; e < 4; e++) cnt[20 + e] += __popcll(~R[e][3]);
+        column_counts(f, w, ew, ends, kind, R, cnt, eccl);
     }
 #pragma unroll
     for (int i = 0; i < 24; i++) {
@@ -246,45 +590,33 @@ __device__ __forceinline__ void features_body(const FeatArgs &f, int bid) {
 #pragma unroll
     for (int e = 0; e < 4; e++) eccl[e] = wave_max32(eccl[e]);
     __syncthreads();
-    int uq[4];
     uniq_second_layer4(f, ends, stage ? s_r1[wv] : nullptr, lane, uq);
-    float xf[GG_FEATS];
-#pragma unroll
-    for (int i = 0; i < GG_FEATS; i++) xf[i] = 0.f;
-    if (lane == 0) write_features(f, p, a, b, c, dd, kind, cnt, eccl, uq, xf);
+    if (lane == 0) {
+        dist[0] = pair_dist(f, p1, q1);
+        dist[1] = pair_dist(f, p2, q2);
+        write_features(f, diam, dist, cnt, eccl, uq, xf);
+    }
     if (lane == 0 && active) {
         float *X = f.X + (size_t)p * GG_FEATS;
 #pragma unroll
         for (int i = 0; i < GG_FEATS; i++) X[i] = xf[i];
     }
     if (!f.score) return;
-    // the MLP score of the proposal, with the arithmetic of k_mlp_forward
-    // (same products, fma order and sums; products rounded before they are
-    // summed, as k_mlp_forward's pass through LDS does): bit-identical
-    {
-#pragma clang fp contract(off)
-        float xs[GG_FEATS];
+    float xs[GG_FEATS];
 #pragma unroll
-        for (int i = 0; i < GG_FEATS; i++) xs[i] = __shfl(xf[i], 0, 64);
-        const __hip_bfloat16 *wq = (f.wpar && (*f.wpar & 1)) ? f.wq2 : f.wq;
-        float o = 0.f;
-#pragma unroll
-        for (int q = 0; q < GG_HIDDEN / 64; q++) {
-            const int h = lane + 64 * q;
-            const __hip_bfloat16 *w1 = wq + OFF_W1 + h * GG_FEATS;
-            float acc = bf(wq[OFF_B1 + h]);
-#pragma unroll
-            for (int i = 0; i < GG_FEATS; i++) acc = fmaf(bf(w1[i]), xs[i], acc);
-            const float r = bf(wq[OFF_W2 + h]) * (acc > 0.f ? acc : 0.f);
-            o = o + r;
-        }
-#pragma unroll
-        for (int d = 32; d > 0; d >>= 1) o = o + __shfl_xor(o, d, 64);
-        if (lane == 0 && active) f.score[p] = o + bf(wq[OFF_B2]);
-    }
+    for (int i = 0; i < GG_FEATS; i++) xs[i] = __shfl(xf[i], 0, 64);
+    LaneWeights lw;
+    load_lane_weights(wodd ? f.wq2 : f.wq, lane, lw);
+    const float o = mlp_score(xs, lw.w1, lw.b1, lw.w2, lw.b2);
+    if (lane == 0 && active) f.score[p] = o;
 }
 
 __global__ void __launch_bounds__(256) k_features(FeatArgs f) { features_body(f, blockIdx.x); }
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+k_features_wg(FeatArgs f) {
+    __shared__ FeatWgLds L;
+    features_wg_body(f, blockIdx.x, L);
+}
 
 // ---- MLP ---------------------------------------------------------------------------------
 
@@ -335,20 +667,39 @@ __global__ void __launch_bounds__(256) k_mlp_forward(const float *X, int P,
 // Group g takes samples [16g, 16g+16) and writes its partial gradient (and
 // partial squared error at index GG_NPARAMS) to part[g][]; the reduction
 // sums the partials in group order, so the result is deterministic.
+struct GradLds {
+    float sx[GG_GRAD_SAMPLES][GG_FEATS];
+    float sa[GG_GRAD_SAMPLES][GG_HIDDEN];  // hidden pre-activations
+    float sg[GG_GRAD_SAMPLES];             // dLoss/dOutput (0 for padding)
+    float se[GG_GRAD_SAMPLES];             // squared error
+    int si[GG_GRAD_SAMPLES];               // sample rows (idx may be host memory)
+};
 __device__ __forceinline__ void grad_part_body(const float *X, const float *Y, const int *idx,
                                                int B, const __hip_bfloat16 *wq, float *part,
-                                               int g) {
-    __shared__ float sw[GG_NPARAMS];
-    __shared__ float sx[GG_GRAD_SAMPLES][GG_FEATS];
-    __shared__ float sa[GG_GRAD_SAMPLES][GG_HIDDEN];  // hidden pre-activations
-    __shared__ float sg[GG_GRAD_SAMPLES];             // dLoss/dOutput (0 for padding)
-    __shared__ float se[GG_GRAD_SAMPLES];             // squared error
-    __shared__ int si[GG_GRAD_SAMPLES];               // sample rows (idx may be host memory)
+                                               int g, GradLds &L) {
+    float(&sx)[GG_GRAD_SAMPLES][GG_FEATS] = L.sx;
+    float(&sa)[GG_GRAD_SAMPLES][GG_HIDDEN] = L.sa;
+    float(&sg)[GG_GRAD_SAMPLES] = L.sg;
+    float(&se)[GG_GRAD_SAMPLES] = L.se;
+    int(&si)[GG_GRAD_SAMPLES] = L.si;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int s0 = g * GG_GRAD_SAMPLES;
     const int ns = min(GG_GRAD_SAMPLES, B - s0);
+    // threads h and h+128 each take half of the samples for hidden unit h
+    const int h = tid & (GG_HIDDEN - 1), half = tid >> 7;
     if (tid < ns) si[tid] = idx ? idx[s0 + tid] : s0 + tid;
-    for (int i = tid; i < GG_NPARAMS; i += 256) sw[i] = bf(wq[i]);
+    // the weights this thread uses, read from the bf16 copy as they are
+    // needed (row h of W1, b1[h], w2[h]; the output weights of hidden units
+    // lane and lane + 64, b2)
+    uint4 w1q[GG_FEATS / 8];
+    {
+        const uint4 *r = reinterpret_cast<const uint4 *>(wq + OFF_W1 + h * GG_FEATS);
+#pragma unroll
+        for (int i = 0; i < GG_FEATS / 8; i++) w1q[i] = r[i];
+    }
+    const float b1 = bf(wq[OFF_B1 + h]), w2h = bf(wq[OFF_W2 + h]);
+    const float w2o[2] = {bf(wq[OFF_W2 + lane]), bf(wq[OFF_W2 + lane + 64])};
+    const float b2 = bf(wq[OFF_B2]);
     __syncthreads();
     for (int i = tid; i < GG_GRAD_SAMPLES * GG_FEATS; i += 256) {
         const int s = i / GG_FEATS, f = i - s * GG_FEATS;
@@ -357,13 +708,19 @@ __device__ __forceinline__ void grad_part_body(const float *X, const float *Y, c
         sx[s][f] = v;
     }
     __syncthreads();
-    // threads h and h+128 each take half of the samples for hidden unit h
-    const int h = tid & (GG_HIDDEN - 1), half = tid >> 7;
     {
         float w1[GG_FEATS];
 #pragma unroll
-        for (int i = 0; i < GG_FEATS; i++) w1[i] = sw[OFF_W1 + h * GG_FEATS + i];
-        const float b1 = sw[OFF_B1 + h];
+        for (int i = 0; i < GG_FEATS / 8; i++) {
+            w1[8 * i + 0] = bf_lo(w1q[i].x);
+            w1[8 * i + 1] = bf_hi(w1q[i].x);
+            w1[8 * i + 2] = bf_lo(w1q[i].y);
+            w1[8 * i + 3] = bf_hi(w1q[i].y);
+            w1[8 * i + 4] = bf_lo(w1q[i].z);
+            w1[8 * i + 5] = bf_hi(w1q[i].z);
+            w1[8 * i + 6] = bf_lo(w1q[i].w);
+            w1[8 * i + 7] = bf_hi(w1q[i].w);
+        }
         for (int q = 0; q < GG_GRAD_SAMPLES / 2; q++) {
             const int s = half * (GG_GRAD_SAMPLES / 2) + q;
             float acc = b1;
@@ -375,20 +732,21 @@ __device__ __forceinline__ void grad_part_body(const float *X, const float *Y, c
     __syncthreads();
     for (int s = wv; s < GG_GRAD_SAMPLES; s += 4) {
         float p = 0.f;
-        for (int k = lane; k < GG_HIDDEN; k += 64) {
-            const float a = sa[s][k];
-            p = fmaf(sw[OFF_W2 + k], a > 0.f ? a : 0.f, p);
+#pragma unroll
+        for (int q = 0; q < GG_HIDDEN / 64; q++) {
+            const float a = sa[s][lane + 64 * q];
+            p = fmaf(w2o[q], a > 0.f ? a : 0.f, p);
         }
         for (int o = 32; o > 0; o >>= 1) p += __shfl_xor(p, o, 64);
         if (lane == 0) {
             float e = 0.f;
-            if (s < ns) e = (p + sw[OFF_B2]) - Y[si[s]];
+            if (s < ns) e = (p + b2) - Y[si[s]];
             sg[s] = 2.f * e / (float)B;
             se[s] = e * e;
         }
     }
     __syncthreads();
-    const float w2 = sw[OFF_W2 + h];
+    const float w2 = w2h;
     float gw1[GG_FEATS / 2];
 #pragma unroll
     for (int i = 0; i < GG_FEATS / 2; i++) gw1[i] = 0.f;
@@ -427,7 +785,8 @@ __global__ void __launch_bounds__(256) k_mlp_grad_part(const float *X, const flo
                                                        const __hip_bfloat16 *wq2,
                                                        const int *par) {
     if (gate && !gate->on) return;
-    grad_part_body(X, Y, idx, B, (par && (*par & 1)) ? wq2 : wq, part, blockIdx.x);
+    __shared__ GradLds L;
+    grad_part_body(X, Y, idx, B, (par && (*par & 1)) ? wq2 : wq, part, blockIdx.x, L);
 }
 
 // Features and scores of the P proposals (workgroups < nfeat) beside the
@@ -439,8 +798,26 @@ __global__ void __launch_bounds__(256) k_features_grad(FeatArgs f, GradArgs g, i
         return;
     }
     if (!g.gate->on) return;
+    __shared__ GradLds L;
     grad_part_body(g.X, g.Y, g.idx, g.B, (*g.par & 1) ? g.wq2 : g.wq, g.part,
-                   (int)blockIdx.x - nfeat);
+                   (int)blockIdx.x - nfeat, L);
+}
+
+// The same with one workgroup per proposal (features_fast_ok), the two
+// bodies sharing one LDS block.  The ngrad gradient workgroups come first, so
+// they are dispatched with the first proposals rather than after the last.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+k_features_wg_grad(FeatArgs f, GradArgs g, int ngrad) {
+    __shared__ union {
+        FeatWgLds f;
+        GradLds g;
+    } L;
+    if ((int)blockIdx.x >= ngrad) {
+        features_wg_body(f, (int)blockIdx.x - ngrad, L.f);
+        return;
+    }
+    if (!g.gate->on) return;
+    grad_part_body(g.X, g.Y, g.idx, g.B, (*g.par & 1) ? g.wq2 : g.wq, g.part, blockIdx.x, L.g);
 }
 
 // Gradient reduction and Adam step in one launch, one thread per parameter
@@ -723,12 +1100,14 @@ int gg::features_score_dev(gg_batch *b, gg_mlp *m, const int *d_moves, int P, fl
     f.wq2 = m && d_score ? m->d_wq2 : nullptr;
     f.wpar = m && d_score ? wpar : nullptr;
     f.score = m ? d_score : nullptr;
-    const int nfeat = (P + 3) / 4;
+    const int nfeat = features_groups(f);
+    const bool wg = features_fast_ok(f);
     if (grad) {
         const int G = (grad->B + GG_GRAD_SAMPLES - 1) / GG_GRAD_SAMPLES;
-        hipLaunchKernelGGL(k_features_grad, dim3(nfeat + G), dim3(256), 0, st, f, *grad, nfeat);
+        hipLaunchKernelGGL(wg ? k_features_wg_grad : k_features_grad, dim3(nfeat + G), dim3(256),
+                           0, st, f, *grad, wg ? G : nfeat);
     } else {
-        hipLaunchKernelGGL(k_features, dim3(nfeat), dim3(256), 0, st, f);
+        hipLaunchKernelGGL(wg ? k_features_wg : k_features, dim3(nfeat), dim3(256), 0, st, f);
     }
     GG_HIP(hipGetLastError());
     return GG_OK;

@@ -74,8 +74,11 @@ def _moves(gg, adj, k, rng):
 
 
 # n = 1000: the last column holds padding sources; n = 17000: rows wider than
-# the LDS-staged radius-1 rows (the kernel reads the stored levels instead)
-@pytest.mark.parametrize("n,d", [(1000, 6), (1024, 6), (4096, 12), (17000, 8)])
+# the LDS-staged radius-1 rows (the kernel reads the stored levels instead);
+# d = 6: degrees off the 16-byte row loads of the all-loads-first pass; d = 32:
+# its largest degree (4d = 128 second-layer entries, two per lane)
+@pytest.mark.parametrize("n,d", [(1000, 6), (1024, 6), (4096, 12), (17000, 8), (1000, 8),
+                                 (2048, 24), (1024, 32)])
 def test_features_match_set_definition(gpu, n, d):
     gg = gpu
     rng = np.random.default_rng(n)
@@ -195,7 +198,7 @@ def test_learned_searches_back_to_back_are_independent(gpu):
     assert runs[0] == runs[2]
 
 
-@pytest.mark.parametrize("n,d", [(1000, 6), (4096, 12)])
+@pytest.mark.parametrize("n,d", [(1000, 6), (4096, 12), (2048, 24)])
 def test_fused_features_and_scores(gpu, n, d):
     """The learned search's fused launch (features, then the MLP score in the
     same wave) gives the set-defined features and exactly the scores of the
