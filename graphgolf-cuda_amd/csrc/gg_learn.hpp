@@ -64,35 +64,55 @@ struct AdamArgs {
     const int *par;
 };
 
+// One Adam step of a parameter from its moments m0, v0 and value w0.
+__device__ __forceinline__ void adam_math(float m0, float v0, float w0, float gi, float lr,
+                                          float b1t, float b2t, float &mi, float &vi, float &wi) {
+    const float beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
+    mi = beta1 * m0 + (1.f - beta1) * gi;
+    vi = beta2 * v0 + (1.f - beta2) * gi * gi;
+    const float mh = mi / (1.f - b1t), vh = vi / (1.f - b2t);
+    wi = w0 - lr * mh / (sqrtf(vh) + eps);
+}
+
 __device__ __forceinline__ void adam_update(float *w, float *m, float *v, __hip_bfloat16 *wq,
                                             int i, float gi, float lr, float b1t, float b2t) {
-    const float beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
-    const float mi = beta1 * m[i] + (1.f - beta1) * gi;
-    const float vi = beta2 * v[i] + (1.f - beta2) * gi * gi;
+    float mi, vi, wi;
+    adam_math(m[i], v[i], w[i], gi, lr, b1t, b2t, mi, vi, wi);
     m[i] = mi;
     v[i] = vi;
-    const float mh = mi / (1.f - b1t), vh = vi / (1.f - b2t);
-    const float wi = w[i] - lr * mh / (sqrtf(vh) + eps);
     w[i] = wi;
     wq[i] = __float2bfloat16(wi);
 }
 
-// Parameter i (i == GG_NPARAMS: the loss) of an Adam step.
+// Parameter i (i == GG_NPARAMS: the loss) of an Adam step.  Every load the
+// step needs (gate, moments, master value, the partial gradients) is issued
+// before any store.
 __device__ __forceinline__ void reduce_adam_body(const AdamArgs &a, int i) {
-    __hip_bfloat16 *wq = (a.par && ((*a.par - 1) & 1)) ? a.wq2 : a.wq;
-    if (!a.gate->on) {
-        if (a.par && i < GG_NPARAMS) wq[i] = __float2bfloat16(a.w[i]);
+    const int par = a.par ? *a.par : 0;
+    const int on = a.gate->on;
+    const float b1t = a.gate->b1t, b2t = a.gate->b2t;
+    const int ic = i < GG_NPARAMS ? i : 0;
+    const float m0 = a.m[ic], v0 = a.v[ic], w0 = a.w[ic];
+    __hip_bfloat16 *wq = (a.par && ((par - 1) & 1)) ? a.wq2 : a.wq;
+    if (!on) {
+        if (a.par && i < GG_NPARAMS) wq[i] = __float2bfloat16(w0);
         return;
     }
     if (i > GG_NPARAMS) return;
     float acc = 0.f;
+#pragma unroll 16
     for (int g = 0; g < a.G; g++) acc += a.part[(size_t)g * GG_PART_STRIDE + i];
     if (i == GG_NPARAMS) {
         *a.loss = acc / (float)a.B;
         return;
     }
+    float mi, vi, wi;
+    adam_math(m0, v0, w0, acc, a.lr, b1t, b2t, mi, vi, wi);
     a.grad[i] = acc;
-    adam_update(a.w, a.m, a.v, wq, i, acc, a.lr, a.gate->b1t, a.gate->b2t);
+    a.m[i] = mi;
+    a.v[i] = vi;
+    a.w[i] = wi;
+    wq[i] = __float2bfloat16(wi);
 }
 
 namespace gg {

@@ -72,9 +72,33 @@ __global__ void __launch_bounds__(64) k_propose(ProposeArgs pa) { propose_body(p
 // counts the predecessors of proposal 16g + (t mod 16) within the (t / 16)-th
 // sixteenth of the scores, and the sixteen partial counts are summed in LDS.
 #define GG_RANK_PER 16
-__device__ __forceinline__ void rank_body(const float *score, int P, int *rank, int bid) {
-    __shared__ float s_sc[GG_SELECT_MAX_P];
+// Where the K chosen proposals go (the rows k_select writes).
+struct SelArgs {
+    int K;
+    uint64_t seed;
+    const int *ctl;          // [0] step, [1] warm-up flag
+    const gg_swap *psw;      // [P] proposals
+    const int *pmoves;       // [P][5]
+    gg_swap *out_sw;         // [K]
+    int *out_moves;          // [K][5]
+    int *out_sel;            // [K]
+};
+
+__device__ __forceinline__ void sel_write(const SelArgs &s, int slot, int p) {
+    s.out_sw[slot] = s.psw[p];
+#pragma unroll
+    for (int j = 0; j < 5; j++) s.out_moves[slot * 5 + j] = s.pmoves[(size_t)p * 5 + j];
+    s.out_sel[slot] = p;
+}
+
+// Ranks of proposals [16 bid, 16 bid + 16): rank[p] = #{q : s_q < s_p or
+// (s_q = s_p and q < p)} with NaN scores last.  With sel, a proposal the
+// selection takes writes its own row: ranks below G = K - K/4 go to slot
+// rank (greedy part), and during the warm-up proposal p < K to slot p.
+__device__ __forceinline__ void rank_body(const float *score, int P, int *rank, int bid,
+                                          float *s_sc, const SelArgs *sel = nullptr) {
     __shared__ int s_part[16][GG_RANK_PER];
+    const bool warm = sel && sel->ctl[1] != 0;  // loaded with the scores
     for (int q = threadIdx.x; q < P; q += 256) {
         const float x = score[q];
         s_sc[q] = x == x ? x : INFINITY;
@@ -84,10 +108,11 @@ __device__ __forceinline__ void rank_body(const float *score, int P, int *rank, 
     const int p = bid * GG_RANK_PER + j;
     int r = 0;
     if (p < P) {
+        // segment seg counts the scores q = seg (mod 16): the four segments
+        // of a wave read four banks
         const float x = s_sc[p];
-        const int q0 = (P * seg) >> 4, q1 = (P * (seg + 1)) >> 4;
 #pragma unroll 8
-        for (int q = q0; q < q1; q++) {
+        for (int q = seg; q < P; q += 16) {
             const float y = s_sc[q];
             r += (y < x || (y == x && q < p)) ? 1 : 0;
         }
@@ -99,22 +124,120 @@ __device__ __forceinline__ void rank_body(const float *score, int P, int *rank, 
 #pragma unroll
         for (int i = 0; i < 16; i++) t += s_part[i][j];
         rank[p] = t;
+        if (sel) {
+            const int G = sel->K - sel->K / 4;
+            if (warm ? p < sel->K : t < G) sel_write(*sel, warm ? p : t, p);
+        }
+    }
+}
+
+// The exploration part of a selection after the warm-up, by one workgroup:
+// slots G..K-1 take the first draws of stream (seed, step, GG_EXPLORE_LANE)
+// that fall on a proposal of rank >= G not drawn before (k_select's rule).
+// Draws are made 32 at a time: draw i of a batch from stream state s0 is
+// mix(s0 + (i+1) * 0x9E3779B97F4A7C15) whenever no draw of the batch needs
+// Lemire's rejection step (x * P mod 2^64 >= P for every one; otherwise the
+// batch is redrawn one at a time).  Draw i is taken iff its rank is >= G and
+// no earlier draw fell on the same proposal (an earlier draw of that
+// proposal was taken or had rank < G); picks fill the slots in draw order.
+#define GG_EXPLORE_BATCH 32
+#define GG_EXPLORE_MAX_K 64  // the resident loop's K bound
+__device__ void explore_body(const float *score, int P, const SelArgs &s, float *s_sc) {
+    __shared__ int s_x[GG_EXPLORE_BATCH], s_r[GG_EXPLORE_BATCH], s_pick[GG_EXPLORE_MAX_K];
+    __shared__ int s_n;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int K = s.K, G = K - K / 4;
+    // the header and the scores loaded together (the scores are not needed
+    // during the warm-up)
+    const int step = s.ctl[0], warm = s.ctl[1];
+    for (int q = tid; q < P; q += 256) {
+        const float x = score[q];
+        s_sc[q] = x == x ? x : INFINITY;
+    }
+    if (warm != 0 || G >= K) return;
+    // wave 0 draws; every thread keeps the stream state
+    uint64_t st = sm64_stream(s.seed, (uint64_t)step, GG_EXPLORE_LANE);
+    int n = G;
+    for (;;) {
+        if (tid < 64) {
+            const bool in = lane < GG_EXPLORE_BATCH;
+            const uint64_t x = sm64_mix(st + (uint64_t)(lane + 1) * 0x9E3779B97F4A7C15ull);
+            const bool dirty = in && x * (uint64_t)P < (uint64_t)P;
+            if (__ballot(dirty) == 0ull) {
+                if (in) s_x[lane] = (int)mul_hi64(x, (uint64_t)P);
+            } else if (lane == 0) {
+                uint64_t t = st;
+                for (int i = 0; i < GG_EXPLORE_BATCH; i++) s_x[i] = (int)sm64_below(t, (uint64_t)P);
+            }
+        }
+        __syncthreads();
+        // rank of draw tid / 8 over the scores q = tid % 8 (mod 8)
+        {
+            const int i = tid >> 3, part = tid & 7;
+            const int xi = s_x[i];
+            const float xv = s_sc[xi];
+            int r = 0;
+#pragma unroll 8
+            for (int q = part; q < P; q += 8) {
+                const float y = s_sc[q];
+                r += (y < xv || (y == xv && q < xi)) ? 1 : 0;
+            }
+            r += __shfl_xor(r, 1, 64);
+            r += __shfl_xor(r, 2, 64);
+            r += __shfl_xor(r, 4, 64);
+            if (part == 0) s_r[i] = r;
+        }
+        __syncthreads();
+        if (tid < 64) {
+            const bool in = lane < GG_EXPLORE_BATCH;
+            const int x = in ? s_x[lane] : -1;
+            bool first = true;
+#pragma unroll
+            for (int j = 0; j < GG_EXPLORE_BATCH; j++) {
+                const int xj = __shfl(x, j, 64);
+                first = first && !(j < lane && xj == x);
+            }
+            for (int j = G; j < n; j++) first = first && s_pick[j] != x;  // earlier batches
+            const bool take = in && first && s_r[in ? lane : 0] >= G;
+            const unsigned long long m = __ballot(take);
+            const int slot = n + __popcll(m & ((1ull << lane) - 1ull));
+            if (take && slot < K) {
+                s_pick[slot] = x;
+                sel_write(s, slot, x);
+            }
+            if (lane == 0) s_n = min(K, n + __popcll(m));
+        }
+        __syncthreads();
+        n = s_n;
+        if (n >= K) break;
+        // a batch held too few picks: the stream continues after its draws
+        // (every thread advances its copy of the state the same way)
+        for (int i = 0; i < GG_EXPLORE_BATCH; i++) (void)sm64_below(st, (uint64_t)P);
     }
 }
 
 __global__ void __launch_bounds__(256) k_rank(const float *score, int P, int *rank) {
-    rank_body(score, P, rank, blockIdx.x);
+    __shared__ float s_sc[GG_SELECT_MAX_P];
+    rank_body(score, P, rank, blockIdx.x, s_sc);
 }
 
-// The ranks (workgroups < nrank) beside the reduction and Adam update of a
-// gated training step whose gradient the last feature launch carried.
+// The ranks and the selection (workgroup 0 the exploration draws, workgroups
+// 1..nrank the ranks, each writing its greedy picks), beside the reduction
+// and Adam update of a gated training step whose gradient the last feature
+// launch carried.  Writes what k_rank followed by k_select writes.
 __global__ void __launch_bounds__(256) k_rank_adam(const float *score, int P, int *rank,
-                                                   int nrank, AdamArgs a) {
-    if ((int)blockIdx.x < nrank) {
-        rank_body(score, P, rank, blockIdx.x);
+                                                   int nrank, AdamArgs a, SelArgs sel) {
+    __shared__ float s_sc[GG_SELECT_MAX_P];
+    const int b = blockIdx.x;
+    if (b == 0) {
+        explore_body(score, P, sel, s_sc);
         return;
     }
-    reduce_adam_body(a, ((int)blockIdx.x - nrank) * 256 + threadIdx.x);
+    if (b <= nrank) {
+        rank_body(score, P, rank, b - 1, s_sc, &sel);
+        return;
+    }
+    reduce_adam_body(a, (b - 1 - nrank) * 256 + threadIdx.x);
 }
 
 __global__ void __launch_bounds__(1024) k_select(const int *rank, int P, int K, uint64_t seed,
@@ -716,10 +839,11 @@ static int loop_enqueue(gg_search *s, bool first, bool next) {
                                  L->d_ctl + 2, par, &ga)) != GG_OK)
         return st;
     const int nrank = (P + GG_RANK_PER - 1) / GG_RANK_PER;
-    hipLaunchKernelGGL(k_rank_adam, dim3(nrank + GG_ADAM_GROUPS), dim3(256), 0, L->st, L->d_score,
-                       P, L->d_rank, nrank, mlp_adam_args(L->mlp, a.B, lr, L->d_gate, par));
-    hipLaunchKernelGGL(k_select, dim3(1), dim3(1024), 0, L->st, L->d_rank, P, K, s->cfg.seed,
-                       L->d_ctl, L->d_psw, L->d_moves, b_swaps(b), L->d_msel, L->d_sel);
+    const SelArgs sel{K, s->cfg.seed, L->d_ctl, L->d_psw, L->d_moves, b_swaps(b), L->d_msel,
+                      L->d_sel};
+    hipLaunchKernelGGL(k_rank_adam, dim3(1 + nrank + GG_ADAM_GROUPS), dim3(256), 0, L->st,
+                       L->d_score, P, L->d_rank, nrank,
+                       mlp_adam_args(L->mlp, a.B, lr, L->d_gate, par), sel);
     GG_HIP(hipGetLastError());
     SparseArgs fin{};
     int nvalid = 0;

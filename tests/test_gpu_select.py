@@ -22,15 +22,18 @@ def _check_step(s, seed, adj_before, K, P, check_moves=True):
     return step, warm, sc
 
 
-def test_selection_matches_restated_rule(gpu):
+# K = 32 of P = 40: the 8 exploration picks come from the 16 proposals ranked
+# 24 and below, so a step often needs more than one batch of 32 draws
+@pytest.mark.parametrize("K,P,warm_steps,greedy", [(16, 256, 32, 16), (32, 40, 8, 32)])
+def test_selection_matches_restated_rule(gpu, K, P, warm_steps, greedy):
     gg = gpu
-    n, d, K, P, seed = 2048, 8, 16, 256, 9
+    n, d, seed = 2048, 8, 9
     adj = gg.random_regular(n, d, 6)
     cfg = gg.search_cfg(mode=gg.MODE_LEARNED, K=K, P=P, total_steps=200, T0=1e-4, T1=1e-6,
-                        seed=seed, warmup_pairs=512, train_batch=256, lr=2e-3)
+                        seed=seed, warmup_pairs=warm_steps * K, train_batch=256, lr=2e-3)
     s = gg.Search(adj, cfg)
     warm_seen, greedy_seen = 0, 0
-    for t in range(48):  # 32 warm-up steps (512 pairs / 16), then selection by score
+    for t in range(warm_steps + greedy):  # the warm-up steps, then selection by score
         before = s.graph()
         s.run(1)
         step, warm, sc = _check_step(s, seed, before, K, P, check_moves=(t % 4 == 0))
@@ -38,7 +41,7 @@ def test_selection_matches_restated_rule(gpu):
         warm_seen += warm
         greedy_seen += 1 - warm
         assert np.isfinite(sc).all()
-    assert warm_seen == 32 and greedy_seen == 16
+    assert warm_seen == warm_steps and greedy_seen == greedy
     s.close()
 
 
