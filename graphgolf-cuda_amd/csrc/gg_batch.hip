@@ -748,9 +748,13 @@ __global__ void __launch_bounds__(1024) k_commit_levels(CommitArgs c) {
     }
     const int w = blockIdx.x;
     const size_t t = (size_t)p.gk * c.W + w;
+    // the column's task words and its log length, loaded together
+    const size_t lt = ((size_t)p.k * c.W + w) * c.S + (l - 1);
     const int ecc_new = commit_col_ecc(c, p, w);
+    const int state = c.task_state[t], end = c.task_end[t], n = c.lg_n[lt];
+    const int becc = c.base_parts[w].ecc;
     unsigned long long *__restrict__ dst = c.snap + ((size_t)(l - 1) * c.W + w) * N;
-    if (c.task_state[t] == GG_TASK_DENSE) {
+    if (state == GG_TASK_DENSE) {
         if (l >= ecc_new) return;
         const int slot = commit_ep_slot(c, p, w);
         const unsigned long long *__restrict__ src =
@@ -766,25 +770,25 @@ __global__ void __launch_bounds__(1024) k_commit_levels(CommitArgs c) {
         return;
     }
     if (endpoint_mask(*p.sw, w)) return;  // the endpoint chunks rewrite this column
-    const int end = c.task_end[t];
-    const size_t lt = ((size_t)p.k * c.W + w) * c.S + (l - 1);
     const int *key = c.lg_key + lt * c.cap;
     const unsigned long long *word = c.lg_word + lt * c.cap;
-    // the row-major copy stores full words explicitly: the Delta entries are
-    // all that change (no fill)
-    if (c.rows && l <= c.RL && l <= end) {
-        unsigned long long *rdst = c.rows + (size_t)(l - 1) * N * c.W + w;
-        const int n = c.lg_n[lt];
-        for (int i = tid; i < n; i += 1024) rdst[(size_t)key[i] * c.W] = word[i];
-    }
-    if (l >= ecc_new) return;
-    if (l >= c.base_parts[w].ecc) {
+    const bool rw = c.rows && l <= c.RL && l <= end;
+    const bool store = l < ecc_new;
+    if (!rw && !store) return;
+    if (store && l >= becc) {
         // the base column was full at this level, so nothing was stored
         for (int v = tid; v < N; v += 1024) dst[v] = ~0ull;
         __syncthreads();
     }
-    const int n = c.lg_n[lt];
-    for (int i = tid; i < n; i += 1024) dst[key[i]] = word[i];
+    // the row-major copy stores full words explicitly: the Delta entries are
+    // all that change (no fill)
+    unsigned long long *rdst = c.rows ? c.rows + (size_t)(l - 1) * N * c.W + w : nullptr;
+    for (int i = tid; i < n; i += 1024) {
+        const int x = key[i];
+        const unsigned long long y = word[i];
+        if (rw) rdst[(size_t)x * c.W] = y;
+        if (store) dst[x] = y;
+    }
 }
 
 // One wave: column w's new part and per-level counts.  Runs after
@@ -802,59 +806,103 @@ __device__ __forceinline__ void commit_parts_body(const CommitArgs &c, int w) {
         if (lane == 0) c.base_parts[w] = dp;
         return;
     }
-    if (lane != 0) return;
+    // level l = lane of the column, computed by lane l (prefix sums over the
+    // wave); lane 0 holds the base's level-0 count.  Endpoint sources (masked
+    // columns) add their single-source counts in place of the base's, as the
+    // finalize does.  Every load precedes the stores that rewrite the row.
     const int ecc = commit_col_ecc(c, p, w);
-    // one pass in place: level l's old count is read before it is replaced;
-    // endpoint sources (masked columns) add their single-source counts in
-    // place of the base's, as the finalize does
-    const int blev = c.base_parts[w].levels;
+    const int blev = c.base_parts[w].levels, becc = c.base_parts[w].ecc;
     const long long *dd = c.dcnt + t * c.L1;
     const int end = c.task_end[t];
-    const unsigned long long em = endpoint_mask(*p.sw, w);
+    const gg_swap &sk = *p.sw;
+    const int l = lane;
+    const long long b0 = l < c.L1 ? br[l] : 0, d0 = l < c.L1 ? dd[l] : 0;
+    const unsigned long long em = endpoint_mask(sk, w);
     const long long nm = __popcll(em);
-    const int becc = c.base_parts[w].ecc;
-    long long cb = br[0], cprev = cb, tot = 0, cx = nm;
-    for (int l = 1; l <= ecc; l++) {
-        if (l <= blev) cb += br[l];
-        long long corr = 0;
-        if (em) {
-            for (int j = 0; j < 4; j++) {
-                if (!endpoint_first(*p.sw, j) || (p.sw->v[j] >> 6) != w) continue;
-                const size_t t4 = (size_t)4 * p.gk + j;
-                if (l <= c.ss_ecc[t4]) cx += c.ss_cnt[t4 * c.L1 + l];
-            }
-            const long long bx = l < becc ? c.mb_cnt[t * (c.S + 1) + l] : (long long)c.N * nm;
-            corr = cx - bx;
+    long long xc = 0, bx = 0;
+    if (em) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            if (!(endpoint_first(sk, j) && (sk.v[j] >> 6) == w)) continue;
+            const size_t t4 = (size_t)4 * p.gk + j;
+            const int sse = c.ss_ecc[t4];
+            if (l >= 1 && l <= sse && l < c.L1) xc += c.ss_cnt[t4 * c.L1 + l];
         }
-        const long long cur = cb + dd[l <= end ? l : end] + corr;
-        br[l] = cur - cprev;
-        tot += (long long)l * (cur - cprev);
-        cprev = cur;
+        bx = l < becc ? (l <= c.S ? (long long)c.mb_cnt[t * (c.S + 1) + l] : 0)
+                      : (long long)c.N * nm;
     }
-    if (ecc + 1 < c.L1) br[ecc + 1] = 0;
-    ColPart np;
-    np.total = tot;
-    np.reached = cprev;
-    np.ecc = ecc;
-    np.levels = ecc + 1;
-    np.flags = 0;
-    np.pad = 0;
-    c.base_parts[w] = np;
+    if (ecc >= 64 || end >= 64) {
+        // deeper than one wave of levels: lane 0 alone, level by level
+        if (lane != 0) return;
+        long long cb = br[0], cprev = cb, tot = 0, cx = nm;
+        for (int q = 1; q <= ecc; q++) {
+            if (q <= blev) cb += br[q];
+            long long corr = 0;
+            if (em) {
+                for (int j = 0; j < 4; j++) {
+                    if (!endpoint_first(sk, j) || (sk.v[j] >> 6) != w) continue;
+                    const size_t t4 = (size_t)4 * p.gk + j;
+                    if (q <= c.ss_ecc[t4]) cx += c.ss_cnt[t4 * c.L1 + q];
+                }
+                const long long bq = q < becc ? c.mb_cnt[t * (c.S + 1) + q] : (long long)c.N * nm;
+                corr = cx - bq;
+            }
+            const long long cur = cb + dd[q <= end ? q : end] + corr;
+            br[q] = cur - cprev;
+            tot += (long long)q * (cur - cprev);
+            cprev = cur;
+        }
+        if (ecc + 1 < c.L1) br[ecc + 1] = 0;
+        ColPart np;
+        np.total = tot; np.reached = cprev; np.ecc = ecc; np.levels = ecc + 1;
+        np.flags = 0; np.pad = 0;
+        c.base_parts[w] = np;
+        return;
+    }
+    // inclusive prefix sums over the levels
+    long long cb = (l == 0 || l <= blev) ? b0 : 0, cx = xc;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const long long u = __shfl_up(cb, o, 64), v = __shfl_up(cx, o, 64);
+        if (lane >= o) { cb += u; cx += v; }
+    }
+    const long long dend = __shfl(d0, end, 64);
+    long long cur = cb + (l <= end ? d0 : dend) + (em ? nm + cx - bx : 0);
+    if (l == 0) cur = b0;
+    const long long inc = cur - __shfl_up(cur, 1, 64);
+    const bool lv = l >= 1 && l <= ecc;
+    long long tot = lv ? (long long)l * inc : 0;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o, 64);
+    const long long reached = __shfl(cur, ecc, 64);
+    if (lv) br[l] = inc;
+    if (l == ecc + 1 && l < c.L1) br[l] = 0;
+    if (lane == 0) {
+        ColPart np;
+        np.total = tot;
+        np.reached = reached;
+        np.ecc = ecc;
+        np.levels = ecc + 1;
+        np.flags = 0;
+        np.pad = 0;
+        c.base_parts[w] = np;
+    }
 }
 
 __global__ void __launch_bounds__(64) k_commit_parts(CommitArgs c) {
     commit_parts_body(c, blockIdx.x);
 }
 
-// The parts (workgroups < W) beside the next step's proposal draw (the
+// The next step's proposal draw (workgroups < nprop) beside the parts (the W
 // workgroups after them): the draw reads the resident rows k_commit_levels
 // has already rewritten.
-__global__ void __launch_bounds__(64) k_commit_parts_propose(CommitArgs c, ProposeArgs pa) {
-    if ((int)blockIdx.x < c.W) {
-        commit_parts_body(c, blockIdx.x);
+__global__ void __launch_bounds__(64) k_commit_parts_propose(CommitArgs c, ProposeArgs pa,
+                                                             int nprop) {
+    if ((int)blockIdx.x < nprop) {
+        propose_body(pa, blockIdx.x);  // the longer chain: dispatched first
         return;
     }
-    propose_body(pa, (int)blockIdx.x - c.W);
+    commit_parts_body(c, (int)blockIdx.x - nprop);
 }
 
 // Applies the swap to the resident tables (host mirror and device).
@@ -937,7 +985,7 @@ static hipError_t launch_commit(gg_batch *b, const CommitArgs &c,
                        b->stream, c);
     if (next)
         hipLaunchKernelGGL(k_commit_parts_propose, dim3(b->W + (next->P + 63) / 64), dim3(64), 0,
-                           b->stream, c, *next);
+                           b->stream, c, *next, (next->P + 63) / 64);
     else
         hipLaunchKernelGGL(k_commit_parts, dim3(b->W), dim3(64), 0, b->stream, c);
     return hipGetLastError();

@@ -351,20 +351,26 @@ __device__ __forceinline__ void loop_accept_body(const AcceptArgs &a) {
     const int cur_diam = c->cur_diam, head = c->head, count0 = c->count, adam_t = c->adam_t,
               nrec = c->nrec, trained = c->trained;
     ResultRec r{};
-    if (lane < K) r = a.res[lane];
-    // feature rows of the chosen proposals: lane = (row, half), 16 floats
-    float4 fx[2][4];
-    int frow[2];
+    int mvk[5] = {0, 0, 0, 0, 0}, swk[12];  // lane k: candidate k's move and swap words
 #pragma unroll
-    for (int u = 0; u < 2; u++) {
-        const int hr = lane + 64 * u;
-        frow[u] = hr < 2 * K ? hr : -1;
-        if (frow[u] < 0) continue;
-        const float4 *src = reinterpret_cast<const float4 *>(
-            a.x + (size_t)a.psel[hr >> 1] * GG_FEATS + (hr & 1) * (GG_FEATS / 2));
+    for (int j = 0; j < 12; j++) swk[j] = 0;
+    if (lane < K) {
+        r = a.res[lane];
 #pragma unroll
-        for (int q = 0; q < 4; q++) fx[u][q] = src[q];
+        for (int j = 0; j < 5; j++) mvk[j] = a.msel[lane * 5 + j];
+        const int *sp = reinterpret_cast<const int *>(a.swaps + lane);
+#pragma unroll
+        for (int j = 0; j < 12; j++) swk[j] = sp[j];
     }
+    // feature rows of the chosen proposals: lane = (row, half), 16 floats
+    // (row index clamped: every lane loads, lanes past 2K store nothing)
+    const int hr0 = lane < 2 * K ? lane : 0, hr1 = lane + 64 < 2 * K ? lane + 64 : 0;
+    const float4 *src0 = reinterpret_cast<const float4 *>(
+        a.x + (size_t)a.psel[hr0 >> 1] * GG_FEATS + (hr0 & 1) * (GG_FEATS / 2));
+    const float4 *src1 = reinterpret_cast<const float4 *>(
+        a.x + (size_t)a.psel[hr1 >> 1] * GG_FEATS + (hr1 & 1) * (GG_FEATS / 2));
+    const float4 f00 = src0[0], f01 = src0[1], f02 = src0[2], f03 = src0[3];
+    const float4 f10 = src1[0], f11 = src1[1], f12 = src1[2], f13 = src1[3];
     if (halted || err) {
         for (int i = lane; i < a.nqctl; i += 64) a.qctl[i] = 0;
         if (lane == 0) {
@@ -374,6 +380,19 @@ __device__ __forceinline__ void loop_accept_body(const AcceptArgs &a) {
             a.dctl[3] = 0;
         }
         return;
+    }
+    // training rows: the feature rows go to the ring now (their slots follow
+    // from the head alone), so they are not held through the decision
+    if (lane < 2 * K) {
+        float4 *dst = reinterpret_cast<float4 *>(
+            a.bx + (size_t)((head + (lane >> 1)) % a.cap) * GG_FEATS + (lane & 1) * (GG_FEATS / 2));
+        dst[0] = f00; dst[1] = f01; dst[2] = f02; dst[3] = f03;
+    }
+    if (lane + 64 < 2 * K) {
+        const int hr = lane + 64;
+        float4 *dst = reinterpret_cast<float4 *>(
+            a.bx + (size_t)((head + (hr >> 1)) % a.cap) * GG_FEATS + (hr & 1) * (GG_FEATS / 2));
+        dst[0] = f10; dst[1] = f11; dst[2] = f12; dst[3] = f13;
     }
     const long long pairs = (long long)a.N * a.N;
     // the best candidate by (connected, diameter, total, index)
@@ -392,8 +411,17 @@ __device__ __forceinline__ void loop_accept_body(const AcceptArgs &a) {
         if (better) { key_bad = ob; key_d = od; key_t = ot; key_k = okk; }
     }
     const int bi = key_k;
-    const int mv = lane < 5 ? a.msel[bi * 5 + lane] : 0;
-    const int swv = lane < 12 ? reinterpret_cast<const int *>(a.swaps + bi)[lane] : 0;
+    int mv = 0, swv = 0;  // lane j: word j of the best candidate's move / swap
+#pragma unroll
+    for (int j = 0; j < 5; j++) {
+        const int x = __shfl(mvk[j], bi, 64);
+        if (lane == j) mv = x;
+    }
+#pragma unroll
+    for (int j = 0; j < 12; j++) {
+        const int x = __shfl(swk[j], bi, 64);
+        if (lane == j) swv = x;
+    }
     ResultRec y;
     y.total = __shfl(r.total, bi, 64);
     y.reached = __shfl(r.reached, bi, 64);
@@ -429,16 +457,7 @@ __device__ __forceinline__ void loop_accept_body(const AcceptArgs &a) {
     }
     const bool inplace = accept && ok && y.diameter - 1 <= a.S &&
                          !(y.flags & (GG_FLAG_QUEUED | GG_FLAG_LIMIT));
-    // ---- stores: training pairs (feature rows and normalised targets)
-#pragma unroll
-    for (int u = 0; u < 2; u++) {
-        if (frow[u] < 0) continue;
-        const int hr = frow[u];
-        float4 *dst = reinterpret_cast<float4 *>(
-            a.bx + (size_t)((head + (hr >> 1)) % a.cap) * GG_FEATS + (hr & 1) * (GG_FEATS / 2));
-#pragma unroll
-        for (int q = 0; q < 4; q++) dst[q] = fx[u][q];
-    }
+    // ---- stores: normalised targets of the training pairs
     if (lane < K) {
         const double raw = rok ? (double)(r.total - cur_total) +
                                      (double)(r.diameter - cur_diam) * (double)a.N * a.N

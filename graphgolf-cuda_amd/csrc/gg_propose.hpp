@@ -69,29 +69,33 @@ struct ProposeArgs {
 
 // The proposals of workgroup bid (64 threads, one proposal each).
 __device__ __forceinline__ void propose_body(const ProposeArgs &pa, int bid) {
-    __shared__ int s_step;
     const int *adjR = pa.adjR;
     const int N = pa.N, d = pa.d;
-    if (threadIdx.x == 0) {
-        s_step = pa.ctl[0];
-        if (bid == 0) { pa.dctl[0] = pa.ctl[0]; pa.dctl[1] = pa.ctl[1]; pa.dctl[2] = pa.ctl[2]; }
+    // the header, read by every thread (no barrier); workgroup 0 copies it
+    const int step = pa.ctl[0];
+    if (bid == 0 && threadIdx.x == 0) {
+        const int h1 = pa.ctl[1], h2 = pa.ctl[2];
+        pa.dctl[0] = step;
+        pa.dctl[1] = h1;
+        pa.dctl[2] = h2;
     }
-    __syncthreads();
     const int p = bid * 64 + threadIdx.x;
     if (p >= pa.P) return;
-    uint64_t st = sm64_stream(pa.seed, (uint64_t)s_step, (uint64_t)p);
+    uint64_t st = sm64_stream(pa.seed, (uint64_t)step, (uint64_t)p);
     const bool reg = d <= GG_PROPOSE_ROW;
     for (int att = 0; att < GG_PROPOSE_MAX_ATTEMPTS; att++) {
         const int a = (int)sm64_below(st, N), ia = (int)sm64_below(st, d);
         const int c = (int)sm64_below(st, N), ic = (int)sm64_below(st, d);
         const int kind = (int)sm64_below(st, 2);
         const int b = adjR[(size_t)a * d + ia], dd = adjR[(size_t)c * d + ic];
+        // a's row is loaded with b and d (it depends on the draw alone)
+        PRow<GG_PROPOSE_ROW> ra;
+        if (reg) ra.load(adjR, a, d);
         if (a == b || a == c || a == dd || b == c || b == dd || c == dd) continue;
         const int xa = kind ? dd : c, xb = kind ? c : dd, xc = kind ? b : a, xd = kind ? a : b;
         int ib, id;
         if (reg) {
-            PRow<GG_PROPOSE_ROW> ra, rb, rd;
-            ra.load(adjR, a, d);
+            PRow<GG_PROPOSE_ROW> rb, rd;
             rb.load(adjR, b, d);
             rd.load(adjR, dd, d);
             if (ra.slot_of(d, xa) >= 0 || rb.slot_of(d, xb) >= 0) continue;

@@ -108,8 +108,11 @@ struct SparseArgs {
 
 // The endpoint sources of swap `sw` among column w's 64 sources (bits of the
 // column word); rows may repeat in a swap, vertices count once.
-__host__ __device__ inline unsigned long long endpoint_mask(const gg_swap &sw, int w) {
+// (Inlined with its loops unrolled, so a swap held in registers stays there:
+// an out-of-line call would force the caller's copy into memory.)
+__host__ __device__ __forceinline__ unsigned long long endpoint_mask(const gg_swap &sw, int w) {
     unsigned long long m = 0;
+#pragma unroll
     for (int j = 0; j < 4; j++)
         if (sw.v[j] >= 0 && (sw.v[j] >> 6) == w) m |= 1ull << (sw.v[j] & 63);
     return m;
@@ -117,8 +120,9 @@ __host__ __device__ inline unsigned long long endpoint_mask(const gg_swap &sw, i
 
 // Task index of endpoint j of a swap, or -1 when it is unused or repeats an
 // earlier endpoint's vertex.
-__host__ __device__ inline bool endpoint_first(const gg_swap &sw, int j) {
+__host__ __device__ __forceinline__ bool endpoint_first(const gg_swap &sw, int j) {
     if (sw.v[j] < 0) return false;
+#pragma unroll
     for (int jj = 0; jj < j; jj++)
         if (sw.v[jj] == sw.v[j]) return false;
     return true;
@@ -860,10 +864,13 @@ __device__ __forceinline__ void sparse_finalize_body(const SparseArgs &a, int nv
         const ColPart bp = a.base_parts[w];
         const int tflags = a.task_flags[t];
         long long pbr[kLU], pdd[kLU];
+        int pmb[kLU];  // base pairs of the masked bits (used by endpoint columns)
+        const int *mb = a.mb_cnt + t * (a.S + 1);
 #pragma unroll
         for (int i = 0; i < kLU; i++) {
             pbr[i] = br[i + 1];
             pdd[i] = dd[i + 1];
+            pmb[i] = mb[i + 1 <= a.S ? i + 1 : a.S];
         }
         if (ts == GG_TASK_DENSE) {
             const ColPart &p = a.dense_parts[t];
@@ -903,7 +910,7 @@ __device__ __forceinline__ void sparse_finalize_body(const SparseArgs &a, int nv
 #pragma unroll
                 for (int j = 0; j < 4; j++)
                     if (mine[j] && l <= sse[j]) pcx[i] += ssc[j][i];
-                pbx[i] = l < bp.ecc ? a.mb_cnt[t * (a.S + 1) + l] : (long long)a.N * nm;
+                pbx[i] = l < bp.ecc ? (long long)pmb[i] : (long long)a.N * nm;
             }
             if (l == end) dend = pdd[i];
         }
