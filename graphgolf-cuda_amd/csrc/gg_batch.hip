@@ -60,7 +60,10 @@ static int batch_alloc(gg_batch *b) {
     if (hipMalloc(&b->d_tab, tb) != hipSuccess) return GG_OOM;
     if (hipHostMalloc(&b->h_tab, tb, hipHostMallocDefault) != hipSuccess) return GG_OOM;
     b->tab_bytes = tb;
-    if (hipMalloc((void **)&b->d_adjR, sizeof(int) * (size_t)N * D) != hipSuccess) return GG_OOM;
+    if (hipMalloc((void **)&b->d_adjR, sizeof(int) * ((size_t)N * D + GG_ADJ_ROW_PAD)) !=
+        hipSuccess)
+        return GG_OOM;
+    GG_HIP(hipMemset(b->d_adjR, 0, sizeof(int) * ((size_t)N * D + GG_ADJ_ROW_PAD)));
     if (hipMalloc((void **)&b->d_src, sizeof(int) * N) != hipSuccess) return GG_OOM;
     if (hipMalloc((void **)&b->d_parts, sizeof(ColPart) * (size_t)W * (K + 1)) != hipSuccess)
         return GG_OOM;

@@ -31,6 +31,11 @@ struct ResultRec {
     int pad;
 };
 
+// Ints of zero padding after the last row of a device adjacency array
+// (gg_batch's row-major neighbours): a row read as 32 ints in 16-byte loads
+// stays in bounds for every degree up to 32.
+#define GG_ADJ_ROW_PAD 32
+
 #define GG_FLAG_LIMIT 1
 #define GG_FLAG_QUEUED 2  // a column ran in the dense fallback queue (no stored levels)
 

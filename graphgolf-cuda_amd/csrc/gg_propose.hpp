@@ -37,6 +37,17 @@ struct PRow {
             for (int i = 0; i < DM; i++) u[i] = i < d ? r[i] : -1;
         }
     }
+    // the row as DM ints in 16-byte loads with no branch (d a multiple of 4,
+    // DM <= 32: the array's GG_ADJ_ROW_PAD ints of padding keep the last
+    // row's read in bounds; entries past d are ignored by slot_of)
+    __device__ __forceinline__ void load_all(const int *adjR, int v, int d) {
+        const int4 *r4 = reinterpret_cast<const int4 *>(adjR + (size_t)v * d);
+#pragma unroll
+        for (int q = 0; q < DM / 4; q++) {
+            const int4 x = r4[q];
+            u[4 * q] = x.x; u[4 * q + 1] = x.y; u[4 * q + 2] = x.z; u[4 * q + 3] = x.w;
+        }
+    }
     __device__ __forceinline__ int slot_of(int d, int x) const {
         int s = -1;
 #pragma unroll
@@ -83,21 +94,30 @@ __device__ __forceinline__ void propose_body(const ProposeArgs &pa, int bid) {
     if (p >= pa.P) return;
     uint64_t st = sm64_stream(pa.seed, (uint64_t)step, (uint64_t)p);
     const bool reg = d <= GG_PROPOSE_ROW;
+    // every row of an attempt in two rounds of branch-free loads: b, d and
+    // a's row, then the rows of b and d
+    const bool vec = reg && (d & 3) == 0;
     for (int att = 0; att < GG_PROPOSE_MAX_ATTEMPTS; att++) {
         const int a = (int)sm64_below(st, N), ia = (int)sm64_below(st, d);
         const int c = (int)sm64_below(st, N), ic = (int)sm64_below(st, d);
         const int kind = (int)sm64_below(st, 2);
         const int b = adjR[(size_t)a * d + ia], dd = adjR[(size_t)c * d + ic];
-        // a's row is loaded with b and d (it depends on the draw alone)
-        PRow<GG_PROPOSE_ROW> ra;
-        if (reg) ra.load(adjR, a, d);
+        PRow<GG_PROPOSE_ROW> ra, rb, rd;
+        if (vec) {
+            ra.load_all(adjR, a, d);
+            rb.load_all(adjR, b, d);
+            rd.load_all(adjR, dd, d);
+        } else if (reg) {
+            ra.load(adjR, a, d);
+        }
         if (a == b || a == c || a == dd || b == c || b == dd || c == dd) continue;
         const int xa = kind ? dd : c, xb = kind ? c : dd, xc = kind ? b : a, xd = kind ? a : b;
         int ib, id;
         if (reg) {
-            PRow<GG_PROPOSE_ROW> rb, rd;
-            rb.load(adjR, b, d);
-            rd.load(adjR, dd, d);
+            if (!vec) {
+                rb.load(adjR, b, d);
+                rd.load(adjR, dd, d);
+            }
             if (ra.slot_of(d, xa) >= 0 || rb.slot_of(d, xb) >= 0) continue;
             ib = rb.slot_of(d, a);
             id = rd.slot_of(d, c);
