@@ -664,10 +664,14 @@ __device__ __forceinline__ int commit_col_ecc(const CommitArgs &c, const CommitP
 // holds them, the stored word elsewhere) and written to the stored level and
 // its row-major copy (full words at every level) where it changes.  One
 // workgroup per chunk: the chunks are disjoint, so no word is written twice.
+// Threads of a commit workgroup; a masked chunk of 1024 vertices is four per
+// thread.
+#define GG_COMMIT_BS 256
+#define GG_COMMIT_CHUNK 1024
 __device__ void commit_masked_chunk(const CommitArgs &c, const CommitPick &p, int w, int l,
                                     int v0, unsigned long long m) {
-    __shared__ unsigned long long s_dw[1024];
-    __shared__ unsigned s_dm[32];
+    __shared__ unsigned long long s_dw[GG_COMMIT_CHUNK];
+    __shared__ unsigned s_dm[GG_COMMIT_CHUNK / 32];
     const int tid = threadIdx.x, N = c.N;
     const size_t t = (size_t)p.gk * c.W + w;
     const int ecc_new = commit_col_ecc(c, p, w), end = c.task_end[t];
@@ -681,27 +685,29 @@ __device__ void commit_masked_chunk(const CommitArgs &c, const CommitPick &p, in
         const int *key = c.lg_key + lt * c.cap;
         const unsigned long long *word = c.lg_word + lt * c.cap;
         const int n = c.lg_n[lt];
-        for (int i = tid; i < n; i += 1024) {
+        for (int i = tid; i < n; i += GG_COMMIT_BS) {
             const int x = key[i] - v0;
-            if (x < 0 || x >= 1024) continue;
+            if (x < 0 || x >= GG_COMMIT_CHUNK) continue;
             s_dw[x] = word[i];
             atomicOr(&s_dm[x >> 5], 1u << (x & 31));
         }
     }
     __syncthreads();
-    const int v = v0 + tid;
-    if (v >= N) return;
     unsigned long long *dst = c.snap + ((size_t)(l - 1) * c.W + w) * N;
-    // the word as stored (levels at or above the base column's eccentricity
-    // are not stored: all ones)
-    const unsigned long long o = full ? ~0ull : dst[v];
-    unsigned long long x = ~0ull;
-    if (store) {
-        const bool logged = (s_dm[tid >> 5] >> (tid & 31)) & 1u;
-        x = commit_unmask(c, p, w, l, m, v, logged ? s_dw[tid] : o);
-        if (full || x != o) dst[v] = x;
+    for (int j = 0; j < GG_COMMIT_CHUNK / GG_COMMIT_BS; j++) {
+        const int vl = tid + GG_COMMIT_BS * j, v = v0 + vl;
+        if (v >= N) break;
+        // the word as stored (levels at or above the base column's
+        // eccentricity are not stored: all ones)
+        const unsigned long long o = full ? ~0ull : dst[v];
+        unsigned long long x = ~0ull;
+        if (store) {
+            const bool logged = (s_dm[vl >> 5] >> (vl & 31)) & 1u;
+            x = commit_unmask(c, p, w, l, m, v, logged ? s_dw[vl] : o);
+            if (full || x != o) dst[v] = x;
+        }
+        if (rw && x != o) c.rows[((size_t)(l - 1) * N + v) * c.W + w] = x;
     }
-    if (rw && x != o) c.rows[((size_t)(l - 1) * N + v) * c.W + w] = x;
 }
 
 // Grid (W + 4 * chunks, S) with chunks = ceil(N / 1024): workgroup (w, l-1),
@@ -710,7 +716,7 @@ __device__ void commit_masked_chunk(const CommitArgs &c, const CommitPick &p, in
 // holding endpoint sources, 1024 vertices each (endpoint j, vertex chunk):
 // the masked columns' words, and the row-major copy of columns that ran
 // densely.
-__global__ void __launch_bounds__(1024) k_commit_levels(CommitArgs c) {
+__global__ void __launch_bounds__(GG_COMMIT_BS) k_commit_levels(CommitArgs c) {
     CommitPick p;
     if (!commit_pick(c, p)) return;
     const int l = blockIdx.y + 1, tid = threadIdx.x;
@@ -729,7 +735,7 @@ __global__ void __launch_bounds__(1024) k_commit_levels(CommitArgs c) {
     }
     const int N = c.N;
     if ((int)blockIdx.x >= c.W) {
-        const int chunks = (N + 1023) >> 10;
+        const int chunks = (N + GG_COMMIT_CHUNK - 1) / GG_COMMIT_CHUNK;
         const int e = ((int)blockIdx.x - c.W) / chunks, ch = ((int)blockIdx.x - c.W) - e * chunks;
         const int v0 = p.sw->v[e];
         if (v0 < 0) return;
@@ -738,15 +744,17 @@ __global__ void __launch_bounds__(1024) k_commit_levels(CommitArgs c) {
         for (int j = 0; j < e; j++)
             if (p.sw->v[j] >= 0 && (p.sw->v[j] >> 6) == w) return;
         if (c.task_state[(size_t)p.gk * c.W + w] != GG_TASK_DENSE) {
-            commit_masked_chunk(c, p, w, l, ch * 1024, endpoint_mask(*p.sw, w));
+            commit_masked_chunk(c, p, w, l, ch * GG_COMMIT_CHUNK, endpoint_mask(*p.sw, w));
             return;
         }
         // a column that ran densely (no single-source tail): its copy
         if (!c.rows || l > c.RL) return;
         const int ecc_new = commit_col_ecc(c, p, w);
         const unsigned long long *src = c.ep_snap + ((size_t)(4 * p.k + e) * c.S + (l - 1)) * N;
-        const int v = ch * 1024 + tid;
-        if (v < N) c.rows[((size_t)(l - 1) * N + v) * c.W + w] = l < ecc_new ? src[v] : ~0ull;
+        for (int j = 0; j < GG_COMMIT_CHUNK / GG_COMMIT_BS; j++) {
+            const int v = ch * GG_COMMIT_CHUNK + tid + GG_COMMIT_BS * j;
+            if (v < N) c.rows[((size_t)(l - 1) * N + v) * c.W + w] = l < ecc_new ? src[v] : ~0ull;
+        }
         return;
     }
     const int w = blockIdx.x;
@@ -766,9 +774,9 @@ __global__ void __launch_bounds__(1024) k_commit_levels(CommitArgs c) {
             const uint4 *__restrict__ s4 = reinterpret_cast<const uint4 *>(src);
             uint4 *__restrict__ d4 = reinterpret_cast<uint4 *>(dst);
 #pragma unroll 8
-            for (int i = tid; i < (N >> 1); i += 1024) d4[i] = s4[i];
+            for (int i = tid; i < (N >> 1); i += GG_COMMIT_BS) d4[i] = s4[i];
         } else {
-            for (int v = tid; v < N; v += 1024) dst[v] = src[v];
+            for (int v = tid; v < N; v += GG_COMMIT_BS) dst[v] = src[v];
         }
         return;
     }
@@ -780,13 +788,13 @@ __global__ void __launch_bounds__(1024) k_commit_levels(CommitArgs c) {
     if (!rw && !store) return;
     if (store && l >= becc) {
         // the base column was full at this level, so nothing was stored
-        for (int v = tid; v < N; v += 1024) dst[v] = ~0ull;
+        for (int v = tid; v < N; v += GG_COMMIT_BS) dst[v] = ~0ull;
         __syncthreads();
     }
     // the row-major copy stores full words explicitly: the Delta entries are
     // all that change (no fill)
     unsigned long long *rdst = c.rows ? c.rows + (size_t)(l - 1) * N * c.W + w : nullptr;
-    for (int i = tid; i < n; i += 1024) {
+    for (int i = tid; i < n; i += GG_COMMIT_BS) {
         const int x = key[i];
         const unsigned long long y = word[i];
         if (rw) rdst[(size_t)x * c.W] = y;
@@ -983,9 +991,9 @@ static CommitArgs commit_args(gg_batch *b, int kc, const CommitSel *sel) {
 
 static hipError_t launch_commit(gg_batch *b, const CommitArgs &c,
                                 const ProposeArgs *next = nullptr) {
-    const int chunks = (b->N + 1023) / 1024;
-    hipLaunchKernelGGL(k_commit_levels, dim3(b->W + 4 * chunks, b->snap_cap), dim3(1024), 0,
-                       b->stream, c);
+    const int chunks = (b->N + GG_COMMIT_CHUNK - 1) / GG_COMMIT_CHUNK;
+    hipLaunchKernelGGL(k_commit_levels, dim3(b->W + 4 * chunks, b->snap_cap), dim3(GG_COMMIT_BS),
+                       0, b->stream, c);
     if (next)
         hipLaunchKernelGGL(k_commit_parts_propose, dim3(b->W + (next->P + 63) / 64), dim3(64), 0,
                            b->stream, c, *next, (next->P + 63) / 64);
