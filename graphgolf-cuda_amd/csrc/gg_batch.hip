@@ -130,8 +130,10 @@ static int batch_alloc(gg_batch *b) {
         if (hipMalloc((void **)&b->d_ss_cnt, sizeof(int) * 4 * (size_t)K * L) != hipSuccess ||
             hipMalloc((void **)&b->d_ss_ecc, sizeof(int) * 4 * (size_t)K) != hipSuccess ||
             hipMalloc((void **)&b->d_ss_flags, sizeof(int) * 4 * (size_t)K) != hipSuccess ||
-            hipMalloc((void **)&b->d_mb_cnt, sizeof(int) * T * (cap + 1)) != hipSuccess)
+            hipMalloc((void **)&b->d_mb_cnt, sizeof(int) * T * (cap + 1)) != hipSuccess ||
+            hipMalloc((void **)&b->d_ccnt, sizeof(int) * (size_t)W) != hipSuccess)
             return GG_OOM;
+        GG_HIP(hipMemset(b->d_ccnt, 0, sizeof(int) * (size_t)W));
         // commit logs: optional (without them every commit runs a new BFS)
         const size_t lg = TC * cap * (size_t)b->cap;
         if (hipMalloc((void **)&b->d_lg_key, 4 * lg) != hipSuccess ||
@@ -191,7 +193,7 @@ extern "C" void gg_batch_destroy(gg_batch *b) {
                     b->d_task_state, b->d_task_end, b->d_task_flags, b->d_dcnt, b->d_queue,
                     b->d_qctl, b->d_dense_parts, b->d_lg_key, b->d_lg_word, b->d_lg_n,
                     b->d_ep_snap, b->d_ep_reach, b->d_rows, b->d_task_ecc, b->d_ss_cnt,
-                    b->d_ss_ecc, b->d_ss_flags, b->d_ss_vis, b->d_mb_cnt};
+                    b->d_ss_ecc, b->d_ss_flags, b->d_ss_vis, b->d_mb_cnt, b->d_ccnt};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     if (b->h_tab) (void)hipHostFree(b->h_tab);
@@ -605,6 +607,7 @@ struct CommitArgs {
     int *adjR;
     const CommitSel *sel;            // non-null: the candidate is read from here at launch
                                      // (k and gk above ignored; k < 0 commits nothing)
+    int *ccnt;                       // [W] workgroups done per column (zero between launches)
 };
 
 // The candidate being committed: the launch's own (k, gk), or the one a
@@ -716,13 +719,12 @@ __device__ void commit_masked_chunk(const CommitArgs &c, const CommitPick &p, in
 // holding endpoint sources, 1024 vertices each (endpoint j, vertex chunk):
 // the masked columns' words, and the row-major copy of columns that ran
 // densely.
-__global__ void __launch_bounds__(GG_COMMIT_BS) k_commit_levels(CommitArgs c) {
-    CommitPick p;
-    if (!commit_pick(c, p)) return;
-    const int l = blockIdx.y + 1, tid = threadIdx.x;
-    // the swap's slots in the resident tables (nothing in this launch or the
-    // parts launch reads them)
-    if (blockIdx.x == 0 && blockIdx.y == 0 && tid < 4) {
+__device__ __forceinline__ int commit_levels_body(const CommitArgs &c, const CommitPick &p,
+                                                  int bx, int l) {
+    const int tid = threadIdx.x;
+    // the swap's slots in the resident tables (no other workgroup of the
+    // launch reads them; a draw beside it applies the swap itself)
+    if (bx == 0 && l == 1 && tid < 4) {
         const int v = p.sw->v[tid];
         if (v >= 0) {
             const int slot = p.sw->slot[tid], u = p.sw->nbr[tid];
@@ -734,30 +736,30 @@ __global__ void __launch_bounds__(GG_COMMIT_BS) k_commit_levels(CommitArgs c) {
         }
     }
     const int N = c.N;
-    if ((int)blockIdx.x >= c.W) {
+    if (bx >= c.W) {
         const int chunks = (N + GG_COMMIT_CHUNK - 1) / GG_COMMIT_CHUNK;
-        const int e = ((int)blockIdx.x - c.W) / chunks, ch = ((int)blockIdx.x - c.W) - e * chunks;
+        const int e = (bx - c.W) / chunks, ch = (bx - c.W) - e * chunks;
         const int v0 = p.sw->v[e];
-        if (v0 < 0) return;
+        if (v0 < 0) return -1;
         const int w = v0 >> 6;
         // each column once: by its first endpoint
         for (int j = 0; j < e; j++)
-            if (p.sw->v[j] >= 0 && (p.sw->v[j] >> 6) == w) return;
+            if (p.sw->v[j] >= 0 && (p.sw->v[j] >> 6) == w) return -1;
         if (c.task_state[(size_t)p.gk * c.W + w] != GG_TASK_DENSE) {
             commit_masked_chunk(c, p, w, l, ch * GG_COMMIT_CHUNK, endpoint_mask(*p.sw, w));
-            return;
+            return w;
         }
         // a column that ran densely (no single-source tail): its copy
-        if (!c.rows || l > c.RL) return;
+        if (!c.rows || l > c.RL) return w;
         const int ecc_new = commit_col_ecc(c, p, w);
         const unsigned long long *src = c.ep_snap + ((size_t)(4 * p.k + e) * c.S + (l - 1)) * N;
         for (int j = 0; j < GG_COMMIT_CHUNK / GG_COMMIT_BS; j++) {
             const int v = ch * GG_COMMIT_CHUNK + tid + GG_COMMIT_BS * j;
             if (v < N) c.rows[((size_t)(l - 1) * N + v) * c.W + w] = l < ecc_new ? src[v] : ~0ull;
         }
-        return;
+        return w;
     }
-    const int w = blockIdx.x;
+    const int w = bx;
     const size_t t = (size_t)p.gk * c.W + w;
     // the column's task words and its log length, loaded together
     const size_t lt = ((size_t)p.k * c.W + w) * c.S + (l - 1);
@@ -766,7 +768,7 @@ __global__ void __launch_bounds__(GG_COMMIT_BS) k_commit_levels(CommitArgs c) {
     const int becc = c.base_parts[w].ecc;
     unsigned long long *__restrict__ dst = c.snap + ((size_t)(l - 1) * c.W + w) * N;
     if (state == GG_TASK_DENSE) {
-        if (l >= ecc_new) return;
+        if (l >= ecc_new) return w;
         const int slot = commit_ep_slot(c, p, w);
         const unsigned long long *__restrict__ src =
             c.ep_snap + ((size_t)slot * c.S + (l - 1)) * N;
@@ -778,14 +780,14 @@ __global__ void __launch_bounds__(GG_COMMIT_BS) k_commit_levels(CommitArgs c) {
         } else {
             for (int v = tid; v < N; v += GG_COMMIT_BS) dst[v] = src[v];
         }
-        return;
+        return w;
     }
-    if (endpoint_mask(*p.sw, w)) return;  // the endpoint chunks rewrite this column
+    if (endpoint_mask(*p.sw, w)) return w;  // the endpoint chunks rewrite this column
     const int *key = c.lg_key + lt * c.cap;
     const unsigned long long *word = c.lg_word + lt * c.cap;
     const bool rw = c.rows && l <= c.RL && l <= end;
     const bool store = l < ecc_new;
-    if (!rw && !store) return;
+    if (!rw && !store) return w;
     if (store && l >= becc) {
         // the base column was full at this level, so nothing was stored
         for (int v = tid; v < N; v += GG_COMMIT_BS) dst[v] = ~0ull;
@@ -800,10 +802,11 @@ __global__ void __launch_bounds__(GG_COMMIT_BS) k_commit_levels(CommitArgs c) {
         if (rw) rdst[(size_t)x * c.W] = y;
         if (store) dst[x] = y;
     }
+    return w;
 }
 
-// One wave: column w's new part and per-level counts.  Runs after
-// k_commit_levels, which reads the old parts.
+// One wave: column w's new part and per-level counts.  Runs once every
+// workgroup of the commit launch that reads the column's old part is done.
 __device__ __forceinline__ void commit_parts_body(const CommitArgs &c, int w) {
     CommitPick p;
     if (!commit_pick(c, p)) return;
@@ -900,20 +903,57 @@ __device__ __forceinline__ void commit_parts_body(const CommitArgs &c, int w) {
     }
 }
 
-__global__ void __launch_bounds__(64) k_commit_parts(CommitArgs c) {
-    commit_parts_body(c, blockIdx.x);
+// Workgroups of the commit launch that handle column w: its S level
+// workgroups, and the S chunk workgroups per vertex chunk of the endpoint that
+// first lies in it.
+__device__ __forceinline__ int commit_col_wgs(const CommitArgs &c, const CommitPick &p, int w) {
+    const int chunks = (c.N + GG_COMMIT_CHUNK - 1) / GG_COMMIT_CHUNK;
+    bool ep = false;
+#pragma unroll
+    for (int j = 0; j < 4; j++) ep = ep || (p.sw->v[j] >= 0 && (p.sw->v[j] >> 6) == w);
+    return c.S * (1 + (ep ? chunks : 0));
 }
 
-// The next step's proposal draw (workgroups < nprop) beside the parts (the W
-// workgroups after them): the draw reads the resident rows k_commit_levels
-// has already rewritten.
-__global__ void __launch_bounds__(64) k_commit_parts_propose(CommitArgs c, ProposeArgs pa,
-                                                             int nprop) {
-    if ((int)blockIdx.x < nprop) {
-        propose_body(pa, blockIdx.x);  // the longer chain: dispatched first
-        return;
+// The in-place commit in one launch.  Grid (nprop + W + 4 * chunks, S): with
+// PROP, workgroups x < nprop (y = 0) draw the next step's proposals on the
+// adjacency with the committed swap applied (four lanes per proposal); the
+// others rewrite the stored levels (commit_levels_body), and the last
+// workgroup to finish a column, counted in c.ccnt[w], rewrites its part and
+// per-level counts (commit_parts_body, one wave) once every workgroup that
+// reads the column's old part is done.
+template <bool PROP>
+__global__ void __launch_bounds__(GG_COMMIT_BS) k_commit(CommitArgs c, ProposeArgs pa, int nprop) {
+    __shared__ int s_last;
+    int bx = blockIdx.x;
+    if (PROP) {
+        if (bx < nprop) {
+            if (blockIdx.y != 0) return;
+            CommitPick pk;
+            propose_split_body(pa, commit_pick(c, pk) ? pk.sw : nullptr, bx);
+            return;
+        }
+        bx -= nprop;
     }
-    commit_parts_body(c, (int)blockIdx.x - nprop);
+    CommitPick p;
+    if (!commit_pick(c, p)) return;
+    const int w = commit_levels_body(c, p, bx, (int)blockIdx.y + 1);
+    if (w < 0) return;
+    __syncthreads();  // every thread's reads of the column's old part are done
+    if (threadIdx.x == 0) {
+        const int old = atomicAdd(&c.ccnt[w], 1);
+        const bool last = old == commit_col_wgs(c, p, w) - 1;
+        if (last) c.ccnt[w] = 0;
+        s_last = last;
+    }
+    __syncthreads();
+    if (!s_last || threadIdx.x >= 64) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    commit_parts_body(c, w);
+}
+
+// The next step's draw on the committed graph (one lane per proposal).
+__global__ void __launch_bounds__(64) k_propose_next(ProposeArgs pa) {
+    propose_body(pa, blockIdx.x);
 }
 
 // Applies the swap to the resident tables (host mirror and device).
@@ -986,19 +1026,25 @@ static CommitArgs commit_args(gg_batch *b, int kc, const CommitSel *sel) {
     c.tab16 = b->narrow ? (uint16_t *)b->d_tab : nullptr;
     c.tab32 = b->narrow ? nullptr : (uint32_t *)b->d_tab;
     c.adjR = b->d_adjR;
+    c.ccnt = b->d_ccnt;
     return c;
 }
 
 static hipError_t launch_commit(gg_batch *b, const CommitArgs &c,
                                 const ProposeArgs *next = nullptr) {
     const int chunks = (b->N + GG_COMMIT_CHUNK - 1) / GG_COMMIT_CHUNK;
-    hipLaunchKernelGGL(k_commit_levels, dim3(b->W + 4 * chunks, b->snap_cap), dim3(GG_COMMIT_BS),
-                       0, b->stream, c);
-    if (next)
-        hipLaunchKernelGGL(k_commit_parts_propose, dim3(b->W + (next->P + 63) / 64), dim3(64), 0,
-                           b->stream, c, *next, (next->P + 63) / 64);
-    else
-        hipLaunchKernelGGL(k_commit_parts, dim3(b->W), dim3(64), 0, b->stream, c);
+    const int ncol = b->W + 4 * chunks;
+    if (next && propose_split_ok(next->d)) {
+        const int nprop = (4 * next->P + GG_COMMIT_BS - 1) / GG_COMMIT_BS;
+        hipLaunchKernelGGL(k_commit<true>, dim3(nprop + ncol, b->snap_cap), dim3(GG_COMMIT_BS), 0,
+                           b->stream, c, *next, nprop);
+    } else {
+        hipLaunchKernelGGL(k_commit<false>, dim3(ncol, b->snap_cap), dim3(GG_COMMIT_BS), 0,
+                           b->stream, c, ProposeArgs{}, 0);
+        if (next)  // degrees the four-lane draw does not take: drawn after the commit
+            hipLaunchKernelGGL(k_propose_next, dim3((next->P + 63) / 64), dim3(64), 0, b->stream,
+                               *next);
+    }
     return hipGetLastError();
 }
 
@@ -1072,7 +1118,7 @@ extern "C" int gg_batch_commit_known(gg_batch *b, const gg_swap *sw, const gg_re
     const CommitArgs c = commit_args(b, kc, nullptr);
     hipError_t e = launch_commit(b, c);
     if (e != hipSuccess) {
-        gg_set_last_hip_error(e, "k_commit_levels", __FILE__, __LINE__);
+        gg_set_last_hip_error(e, "k_commit", __FILE__, __LINE__);
         return GG_HIP_ERROR;
     }
     note_inplace_commit(b, sw, known);
@@ -1088,7 +1134,7 @@ int batch_enqueue_commit_dev(gg_batch *b, const CommitSel *sel, const ProposeArg
     const CommitArgs c = commit_args(b, 0, sel);
     hipError_t e = launch_commit(b, c, next);
     if (e != hipSuccess) {
-        gg_set_last_hip_error(e, "k_commit_levels", __FILE__, __LINE__);
+        gg_set_last_hip_error(e, "k_commit", __FILE__, __LINE__);
         return GG_HIP_ERROR;
     }
     return GG_OK;

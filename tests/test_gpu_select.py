@@ -133,3 +133,52 @@ def test_resident_loop_acceptance_matches_restated_rule(gpu, hill):
     o = oracle_eval.evaluate(s.graph())
     assert (o[0], o[1]) == (st.total_cur, st.diam_cur)
     s.close()
+
+
+def _apply_move(G, m):
+    """The 2-opt rewiring of move (a, b, c, d, kind) applied in place, each
+    removed neighbour's slot taking the new one (the rows as the swap
+    rewrites them)."""
+    a, b, c, d, kind = (int(x) for x in m)
+    xa, xb, xc, xd = (d, c, b, a) if kind else (c, d, a, b)
+    for v, old, new in ((a, b, xa), (b, a, xb), (c, d, xc), (d, c, xd)):
+        G[v][list(G[v]).index(old)] = new
+
+
+def test_resident_loop_draws_on_the_committed_graph(gpu):
+    """In the resident loop every step but a chunk's first draws its
+    proposals in the commit launch of the step before, on the adjacency with
+    that step's swap applied while the commit writes it.  Each chunk's last
+    step's draw equals the restated draw on the graph after the step before
+    it, and the driver's graph after 100 such steps is the initial graph with
+    the logged accepted moves applied: symmetric, every swap written at the
+    right slots; its oracle total is the driver's."""
+    gg = gpu
+    n, d, K, P, seed = 2048, 8, 16, 256, 22
+    adj0 = np.array(gg.random_regular(n, d, 7))
+    cfg = gg.search_cfg(mode=gg.MODE_LEARNED, rule=gg.RULE_HILL, K=K, P=P, total_steps=600,
+                        T0=1e-4, T1=1e-6, seed=seed, warmup_pairs=1024, train_batch=256,
+                        lr=3e-3)
+    s = gg.Search(adj0, cfg)
+    s.set_log(1000)
+    s.run(70)  # 64 warm-up steps, then the loop
+    for _ in range(20):
+        G = np.array(s.graph())
+        s.run(5)
+        for row in s.log()[-5:-1]:
+            if int(row[9]):
+                _apply_move(G, row[1:6])
+        step, warm, sc, mv, sel = s.last_selection()
+        ref = [devrng_ref.propose(G, seed, step, p) for p in range(P)]
+        assert [tuple(int(x) for x in r) for r in mv] == ref, step
+    st = s.run(0)
+    G = adj0.copy()
+    for row in s.log():
+        if int(row[9]):
+            _apply_move(G, row[1:6])
+    H = np.array(s.graph())
+    assert np.array_equal(H, G)
+    assert all(v in H[u] for v in range(n) for u in H[v])
+    o = oracle_eval.evaluate(H)
+    assert (o[0], o[1]) == (st.total_cur, st.diam_cur)
+    s.close()
