@@ -131,9 +131,8 @@ static int batch_alloc(gg_batch *b) {
             hipMalloc((void **)&b->d_ss_ecc, sizeof(int) * 4 * (size_t)K) != hipSuccess ||
             hipMalloc((void **)&b->d_ss_flags, sizeof(int) * 4 * (size_t)K) != hipSuccess ||
             hipMalloc((void **)&b->d_mb_cnt, sizeof(int) * T * (cap + 1)) != hipSuccess ||
-            hipMalloc((void **)&b->d_ccnt, sizeof(int) * (size_t)W) != hipSuccess)
+            hipMalloc((void **)&b->d_oldecc, sizeof(int) * (size_t)W) != hipSuccess)
             return GG_OOM;
-        GG_HIP(hipMemset(b->d_ccnt, 0, sizeof(int) * (size_t)W));
         // commit logs: optional (without them every commit runs a new BFS)
         const size_t lg = TC * cap * (size_t)b->cap;
         if (hipMalloc((void **)&b->d_lg_key, 4 * lg) != hipSuccess ||
@@ -193,7 +192,7 @@ extern "C" void gg_batch_destroy(gg_batch *b) {
                     b->d_task_state, b->d_task_end, b->d_task_flags, b->d_dcnt, b->d_queue,
                     b->d_qctl, b->d_dense_parts, b->d_lg_key, b->d_lg_word, b->d_lg_n,
                     b->d_ep_snap, b->d_ep_reach, b->d_rows, b->d_task_ecc, b->d_ss_cnt,
-                    b->d_ss_ecc, b->d_ss_flags, b->d_ss_vis, b->d_mb_cnt, b->d_ccnt};
+                    b->d_ss_ecc, b->d_ss_flags, b->d_ss_vis, b->d_mb_cnt, b->d_oldecc};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     if (b->h_tab) (void)hipHostFree(b->h_tab);
@@ -383,6 +382,7 @@ static int enqueue_sparse(gg_batch *b, int K, bool reset = true) {
     s.ss_flags = b->d_ss_flags;
     s.ss_vis = b->d_ss_vis;
     s.mb_cnt = b->d_mb_cnt;
+    s.old_ecc = b->d_oldecc;
     s.undirected = b->dc.push_ok ? 1 : 0;
     const size_t lds = sparse_lds_bytes(N, b->skcap);
     // single-source tail of the sparse kernel (endpoint sources): the
@@ -607,7 +607,8 @@ struct CommitArgs {
     int *adjR;
     const CommitSel *sel;            // non-null: the candidate is read from here at launch
                                      // (k and gk above ignored; k < 0 commits nothing)
-    int *ccnt;                       // [W] workgroups done per column (zero between launches)
+    const int *old_ecc;              // [W] base column eccentricities as the evaluation read
+                                     // them (the parts are rewritten beside the levels)
 };
 
 // The candidate being committed: the launch's own (k, gk), or the one a
@@ -678,7 +679,7 @@ __device__ void commit_masked_chunk(const CommitArgs &c, const CommitPick &p, in
     const int tid = threadIdx.x, N = c.N;
     const size_t t = (size_t)p.gk * c.W + w;
     const int ecc_new = commit_col_ecc(c, p, w), end = c.task_end[t];
-    const bool store = l < ecc_new, full = l >= c.base_parts[w].ecc;
+    const bool store = l < ecc_new, full = l >= c.old_ecc[w];
     const bool rw = c.rows && l <= c.RL;
     if (!store && !rw) return;  // nothing kept at this level
     if (tid < 32) s_dm[tid] = 0u;
@@ -765,7 +766,7 @@ __device__ __forceinline__ int commit_levels_body(const CommitArgs &c, const Com
     const size_t lt = ((size_t)p.k * c.W + w) * c.S + (l - 1);
     const int ecc_new = commit_col_ecc(c, p, w);
     const int state = c.task_state[t], end = c.task_end[t], n = c.lg_n[lt];
-    const int becc = c.base_parts[w].ecc;
+    const int becc = c.old_ecc[w];
     unsigned long long *__restrict__ dst = c.snap + ((size_t)(l - 1) * c.W + w) * N;
     if (state == GG_TASK_DENSE) {
         if (l >= ecc_new) return w;
@@ -805,8 +806,8 @@ __device__ __forceinline__ int commit_levels_body(const CommitArgs &c, const Com
     return w;
 }
 
-// One wave: column w's new part and per-level counts.  Runs once every
-// workgroup of the commit launch that reads the column's old part is done.
+// One wave: column w's new part and per-level counts (no other workgroup of
+// the commit launch reads the part).
 __device__ __forceinline__ void commit_parts_body(const CommitArgs &c, int w) {
     CommitPick p;
     if (!commit_pick(c, p)) return;
@@ -903,27 +904,15 @@ __device__ __forceinline__ void commit_parts_body(const CommitArgs &c, int w) {
     }
 }
 
-// Workgroups of the commit launch that handle column w: its S level
-// workgroups, and the S chunk workgroups per vertex chunk of the endpoint that
-// first lies in it.
-__device__ __forceinline__ int commit_col_wgs(const CommitArgs &c, const CommitPick &p, int w) {
-    const int chunks = (c.N + GG_COMMIT_CHUNK - 1) / GG_COMMIT_CHUNK;
-    bool ep = false;
-#pragma unroll
-    for (int j = 0; j < 4; j++) ep = ep || (p.sw->v[j] >= 0 && (p.sw->v[j] >> 6) == w);
-    return c.S * (1 + (ep ? chunks : 0));
-}
-
-// The in-place commit in one launch.  Grid (nprop + W + 4 * chunks, S): with
-// PROP, workgroups x < nprop (y = 0) draw the next step's proposals on the
-// adjacency with the committed swap applied (four lanes per proposal); the
-// others rewrite the stored levels (commit_levels_body), and the last
-// workgroup to finish a column, counted in c.ccnt[w], rewrites its part and
-// per-level counts (commit_parts_body, one wave) once every workgroup that
-// reads the column's old part is done.
+// The in-place commit in one launch.  Grid (nprop + 2W + 4 * chunks, S), y =
+// level - 1: with PROP, workgroups x < nprop (y = 0) draw the next step's
+// proposals on the adjacency with the committed swap applied (four lanes per
+// proposal); the next W (y = 0) rewrite one column's part and per-level
+// counts each (commit_parts_body, one wave); the rest rewrite the stored
+// levels (commit_levels_body).  No level workgroup reads the parts: the
+// columns' old eccentricities come from the evaluation launch (old_ecc).
 template <bool PROP>
 __global__ void __launch_bounds__(GG_COMMIT_BS) k_commit(CommitArgs c, ProposeArgs pa, int nprop) {
-    __shared__ int s_last;
     int bx = blockIdx.x;
     if (PROP) {
         if (bx < nprop) {
@@ -934,21 +923,15 @@ __global__ void __launch_bounds__(GG_COMMIT_BS) k_commit(CommitArgs c, ProposeAr
         }
         bx -= nprop;
     }
+    if (bx < c.W) {
+        if (blockIdx.y != 0 || threadIdx.x >= 64) return;
+        commit_parts_body(c, bx);
+        return;
+    }
+    bx -= c.W;
     CommitPick p;
     if (!commit_pick(c, p)) return;
-    const int w = commit_levels_body(c, p, bx, (int)blockIdx.y + 1);
-    if (w < 0) return;
-    __syncthreads();  // every thread's reads of the column's old part are done
-    if (threadIdx.x == 0) {
-        const int old = atomicAdd(&c.ccnt[w], 1);
-        const bool last = old == commit_col_wgs(c, p, w) - 1;
-        if (last) c.ccnt[w] = 0;
-        s_last = last;
-    }
-    __syncthreads();
-    if (!s_last || threadIdx.x >= 64) return;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    commit_parts_body(c, w);
+    (void)commit_levels_body(c, p, bx, (int)blockIdx.y + 1);
 }
 
 // The next step's draw on the committed graph (one lane per proposal).
@@ -1026,14 +1009,14 @@ static CommitArgs commit_args(gg_batch *b, int kc, const CommitSel *sel) {
     c.tab16 = b->narrow ? (uint16_t *)b->d_tab : nullptr;
     c.tab32 = b->narrow ? nullptr : (uint32_t *)b->d_tab;
     c.adjR = b->d_adjR;
-    c.ccnt = b->d_ccnt;
+    c.old_ecc = b->d_oldecc;
     return c;
 }
 
 static hipError_t launch_commit(gg_batch *b, const CommitArgs &c,
                                 const ProposeArgs *next = nullptr) {
     const int chunks = (b->N + GG_COMMIT_CHUNK - 1) / GG_COMMIT_CHUNK;
-    const int ncol = b->W + 4 * chunks;
+    const int ncol = 2 * b->W + 4 * chunks;
     if (next && propose_split_ok(next->d)) {
         const int nprop = (4 * next->P + GG_COMMIT_BS - 1) / GG_COMMIT_BS;
         hipLaunchKernelGGL(k_commit<true>, dim3(nprop + ncol, b->snap_cap), dim3(GG_COMMIT_BS), 0,

@@ -55,7 +55,7 @@ struct gg_batch {
     int *d_ss_ecc = nullptr;                  // [4*max_k]
     int *d_mb_cnt = nullptr;                  // [max_k][W][snap_cap+1]
     int *d_ss_flags = nullptr;                // [4*max_k]
-    int *d_ccnt = nullptr;                    // [W] commit workgroups done per column
+    int *d_oldecc = nullptr;                  // [W] base column eccentricities read by k_sparse
     unsigned *d_ss_vis = nullptr;             // [4*chunk_max][snap_cap][ceil(N/32)] (commit)
     int log_valid = 0;                        // the last eval was sparse with logs kept
     int log_k0 = 0, log_K = 0;                // candidates the logs cover: [log_k0, log_K)

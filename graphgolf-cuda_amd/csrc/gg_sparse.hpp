@@ -102,6 +102,9 @@ struct SparseArgs {
     // per task (g, w) with masked endpoint sources: their base reached pairs
     // after levels 0..S (popcount of the masked bits of B_l)
     int *mb_cnt;                     // [(k0+K)][W][S+1]
+    int *old_ecc;                    // [W] the base columns' eccentricities as this launch
+                                     // read them (the commit reads them there, so the
+                                     // new parts can be written beside it)
     unsigned *ss_vis;                // [4K][S][ceil(N/32)] candidate visited after level l
     int undirected;                  // every candidate graph is undirected (top-down allowed)
 };
@@ -445,6 +448,7 @@ __global__ void __launch_bounds__(BS) k_sparse(SparseArgs a) {
     const int kc = a.k0 + kb;                                     // global
     const int NB = (N + 31) >> 5;
     const int ecc_w = a.base_parts[w].ecc;
+    if (has_col && gi == 0 && threadIdx.x == 0 && a.old_ecc) a.old_ecc[w] = ecc_w;
     const unsigned long long pad = pad_mask(w, N);
     const int NBE = (NB + 1) & ~1;
     unsigned *s_ubits = reinterpret_cast<unsigned *>(Bprev + N);  // [NBE]
