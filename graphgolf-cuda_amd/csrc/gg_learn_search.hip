@@ -31,7 +31,7 @@ namespace gg {
 // back and reads one StepRec per step when a chunk of steps ends.
 #define GG_LOOP_HOST_COMMIT 1  // the accepted candidate has no kept logs: host commit
 #define GG_LOOP_NO_SWAP 2      // a proposal found no simple swap
-#define GG_LOOP_REC_CAP 512    // steps per chunk (records in pinned memory)
+#define GG_LOOP_REC_CAP 512    // steps per chunk (records in device memory, read after it)
 #define GG_LOOP_INFLIGHT 4     // graph launches enqueued ahead of the GPU
 #define GG_LOOP_UNROLL 8       // steps per graph launch
 #define GG_ACCEPT_LANE 0xFFFFFFFEull
@@ -321,7 +321,7 @@ struct AcceptArgs {
     TrainGate *gate;
     float *bx, *by;          // replay ring: features, targets
     int *idx;                // minibatch rows
-    StepRec *rec;            // pinned, GG_LOOP_REC_CAP entries
+    StepRec *rec;            // GG_LOOP_REC_CAP entries in device memory
     int *qctl;               // the incremental launch's queue counters
     unsigned *done;          // finalize workgroups done (zero between launches)
 };
@@ -585,7 +585,8 @@ struct LearnState {
     TrainGate *d_gate = nullptr;                   // the step's Adam step
     int *d_bidx = nullptr;                         // minibatch rows drawn on the GPU
     unsigned *d_done = nullptr;                    // finalize workgroups done
-    StepRec *h_rec = nullptr;                      // one record per step (pinned)
+    StepRec *h_rec = nullptr;                      // one record per step (pinned host copy)
+    StepRec *d_rec = nullptr;                      // the device writes them here
     hipGraph_t lgraph = nullptr;
     hipGraphExec_t lexec = nullptr;
     int l_push = -1;
@@ -633,7 +634,8 @@ static int alloc_learn(LearnState *L) {
     if (hipMalloc((void **)&L->d_done, 4) != hipSuccess) return GG_OOM;
     if (hipMemset(L->d_done, 0, 4) != hipSuccess) return GG_HIP_ERROR;
     if (hipHostMalloc((void **)&L->h_rec, sizeof(StepRec) * GG_LOOP_REC_CAP,
-                      hipHostMallocDefault) != hipSuccess)
+                      hipHostMallocDefault) != hipSuccess ||
+        hipMalloc((void **)&L->d_rec, sizeof(StepRec) * GG_LOOP_REC_CAP) != hipSuccess)
         return GG_OOM;
     for (hipEvent_t &e : L->lev)
         if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return GG_HIP_ERROR;
@@ -672,7 +674,7 @@ void learn_destroy(gg_search *s) {
     gg_mlp_destroy(L->mlp);
     void *dev[] = {L->d_moves, L->d_x, L->d_score, L->d_sel, L->d_bx,   L->d_by,   L->d_idx,
                    L->d_psw,   L->d_msel, L->d_ctl, L->d_rank, L->d_loop, L->d_csel, L->d_gate,
-                   L->d_bidx,  L->d_done};
+                   L->d_bidx,  L->d_done, L->d_rec};
     for (void *p : dev)
         if (p) (void)hipFree(p);
     void *host[] = {L->h_moves, L->h_score, L->h_sel,  L->h_idx,
@@ -838,7 +840,7 @@ static int loop_enqueue(gg_search *s, bool first, bool next) {
     a.T0 = s->cfg.T0; a.T1 = s->cfg.T1; a.mu = L->mu; a.sigma = L->sigma; a.seed = s->cfg.seed;
     a.res = batch_dres(b); a.swaps = b_swaps(b); a.msel = L->d_msel; a.x = L->d_x;
     a.psel = L->d_sel; a.ctl = L->d_loop; a.dctl = L->d_ctl; a.sel = L->d_csel;
-    a.gate = L->d_gate; a.bx = L->d_bx; a.by = L->d_by; a.idx = L->d_bidx; a.rec = L->h_rec;
+    a.gate = L->d_gate; a.bx = L->d_bx; a.by = L->d_by; a.idx = L->d_bidx; a.rec = L->d_rec;
     a.qctl = qctl;
     a.done = L->d_done;
     const float lr = (float)(s->cfg.lr > 0 ? s->cfg.lr : 1e-3);
@@ -968,6 +970,8 @@ int learn_loop_run(gg_search *s, int steps, int *done) {
     rst = mlp_refresh_dev(L->mlp, L->st);
     if (rst != GG_OK) return rst;
     GG_HIP(hipMemcpyAsync(L->h_loop, L->d_loop, sizeof(LoopCtl), hipMemcpyDeviceToHost, L->st));
+    GG_HIP(hipMemcpyAsync(L->h_rec, L->d_rec, sizeof(StepRec) * (size_t)n, hipMemcpyDeviceToHost,
+                          L->st));
     GG_HIP(hipStreamSynchronize(L->st));
     const int P = L->P, K = L->K;
     for (int i = 0; i < h.nrec; i++) {
