@@ -737,11 +737,15 @@ template <class Topo, int BS, int VPT, int DT>
 __global__ void __launch_bounds__(BS) k_dense_lds_queue(Topo topo, DenseArgs a) {
     extern __shared__ unsigned long long vis[];
     __shared__ int s_task;
+    // the queue is complete when this launch starts: an empty one (the usual
+    // case) costs one load, no claim
+    const int qn = *a.queue_count;
+    if (qn == 0) return;
     for (;;) {
         if (threadIdx.x == 0) s_task = atomicAdd(a.queue_next, 1);
         __syncthreads();
         const int t = s_task;
-        if (t >= *a.queue_count) break;
+        if (t >= qn) break;
         const int2 kw = a.queue[t];
         ColOut o = col_out(a, kw.y);
         if (a.ep_snap && a.swaps && kw.x >= a.ep_k0) {
