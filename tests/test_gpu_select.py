@@ -145,7 +145,10 @@ def _apply_move(G, m):
         G[v][list(G[v]).index(old)] = new
 
 
-def test_resident_loop_draws_on_the_committed_graph(gpu):
+# d = 8 and 12: the four-lane draw beside the commit (two and three row
+# quads); d = 6: degrees it does not take, drawn after the commit instead
+@pytest.mark.parametrize("n,d", [(2048, 8), (1536, 12), (1024, 6)])
+def test_resident_loop_draws_on_the_committed_graph(gpu, n, d):
     """In the resident loop every step but a chunk's first draws its
     proposals in the commit launch of the step before, on the adjacency with
     that step's swap applied while the commit writes it.  Each chunk's last
@@ -154,7 +157,7 @@ def test_resident_loop_draws_on_the_committed_graph(gpu):
     the logged accepted moves applied: symmetric, every swap written at the
     right slots; its oracle total is the driver's."""
     gg = gpu
-    n, d, K, P, seed = 2048, 8, 16, 256, 22
+    K, P, seed = 16, 256, 22
     adj0 = np.array(gg.random_regular(n, d, 7))
     cfg = gg.search_cfg(mode=gg.MODE_LEARNED, rule=gg.RULE_HILL, K=K, P=P, total_steps=600,
                         T0=1e-4, T1=1e-6, seed=seed, warmup_pairs=1024, train_batch=256,
