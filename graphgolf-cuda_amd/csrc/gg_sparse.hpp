@@ -134,6 +134,12 @@ __host__ __device__ __forceinline__ bool endpoint_first(const gg_swap &sw, int j
 // Row of candidate vertex v with the swap's replacements applied: DT > 0
 // loads the whole row as 16-byte vectors (one memory latency), DT = 0 loads
 // slot by slot.
+// Whether v is one of the swap's endpoints (rows other than those keep the
+// base neighbours).
+__device__ __forceinline__ bool swap_touches(const SwapRegs &sw, int v) {
+    return v == sw.v[0] || v == sw.v[1] || v == sw.v[2] || v == sw.v[3];
+}
+
 template <int DT>
 struct CandRow {
     int u[DT > 0 ? DT : 1];
@@ -144,6 +150,8 @@ struct CandRow {
             int4 x = r[q];
             u[4 * q] = x.x; u[4 * q + 1] = x.y; u[4 * q + 2] = x.z; u[4 * q + 3] = x.w;
         }
+        // (unconditional: a branch around these compare-selects costs the
+        // column code more in spilled scalar registers than it saves)
 #pragma unroll
         for (int i = 0; i < DT; i++)
 #pragma unroll
@@ -256,11 +264,13 @@ __device__ int ss_bfs(const int *adjR, int N, int D, int lmax, int S, int x,
                             const int v = lin[e];
                             const int4 x = reinterpret_cast<const int4 *>(adjR + (size_t)v * D)[q];
                             y4[h][0] = x.x; y4[h][1] = x.y; y4[h][2] = x.z; y4[h][3] = x.w;
+                            if (swap_touches(sw, v)) {
 #pragma unroll
-                            for (int i = 0; i < 4; i++)
+                                for (int i = 0; i < 4; i++)
 #pragma unroll
-                                for (int j = 0; j < 4; j++)
-                                    if (sw.v[j] == v && sw.s[j] == 4 * q + i) y4[h][i] = sw.n[j];
+                                    for (int j = 0; j < 4; j++)
+                                        if (sw.v[j] == v && sw.s[j] == 4 * q + i) y4[h][i] = sw.n[j];
+                            }
                         }
                     }
 #pragma unroll
@@ -309,14 +319,16 @@ __device__ int ss_bfs(const int *adjR, int N, int D, int lmax, int S, int x,
                 // nearly always meets it there), the rest only where not
                 auto quad_hit = [&](int v, int q, int4 x) {
                     int y[4] = {x.x, x.y, x.z, x.w};
+                    if (swap_touches(sw, v)) {
+#pragma unroll
+                        for (int i = 0; i < 4; i++)
+#pragma unroll
+                            for (int j = 0; j < 4; j++)
+                                if (sw.v[j] == v && sw.s[j] == 4 * q + i) y[i] = sw.n[j];
+                    }
                     bool hit = false;
 #pragma unroll
-                    for (int i = 0; i < 4; i++) {
-#pragma unroll
-                        for (int j = 0; j < 4; j++)
-                            if (sw.v[j] == v && sw.s[j] == 4 * q + i) y[i] = sw.n[j];
-                        hit = hit || ((front[y[i] >> 5] >> (y[i] & 31)) & 1u);
-                    }
+                    for (int i = 0; i < 4; i++) hit = hit || ((front[y[i] >> 5] >> (y[i] & 31)) & 1u);
                     return hit;
                 };
                 constexpr int Q = DT / 4;
