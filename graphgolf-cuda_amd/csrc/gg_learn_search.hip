@@ -587,12 +587,14 @@ struct LearnState {
     unsigned *d_done = nullptr;                    // finalize workgroups done
     StepRec *h_rec = nullptr;                      // one record per step (pinned host copy)
     StepRec *d_rec = nullptr;                      // the device writes them here
-    hipGraph_t lgraph = nullptr;
-    hipGraphExec_t lexec = nullptr;
+    // the resident loop's captured launches, [unrolled][first][draws]: one or
+    // GG_LOOP_UNROLL steps; the first step draws its own proposals (a chunk's
+    // first launch) or the step before drew them; the last step's commit
+    // draws the next step's (every launch but a chunk's last)
+    hipGraph_t lgraph[2][2][2] = {};
+    hipGraphExec_t lexec[2][2][2] = {};
     int l_push = -1;
     hipEvent_t lev[GG_LOOP_INFLIGHT] = {};          // steps in flight: one event each
-    hipGraph_t lgraph_u = nullptr;                 // GG_LOOP_UNROLL steps in one graph
-    hipGraphExec_t lexec_u = nullptr;
 };
 
 static int alloc_learn(LearnState *L) {
@@ -665,12 +667,14 @@ void learn_destroy(gg_search *s) {
     if (L->st) (void)hipStreamSynchronize(L->st);
     if (L->gexec) (void)hipGraphExecDestroy(L->gexec);
     if (L->graph) (void)hipGraphDestroy(L->graph);
-    if (L->lexec) (void)hipGraphExecDestroy(L->lexec);
-    if (L->lgraph) (void)hipGraphDestroy(L->lgraph);
+    for (int i = 0; i < 8; i++) {
+        hipGraphExec_t &x = L->lexec[i >> 2][(i >> 1) & 1][i & 1];
+        hipGraph_t &g = L->lgraph[i >> 2][(i >> 1) & 1][i & 1];
+        if (x) (void)hipGraphExecDestroy(x);
+        if (g) (void)hipGraphDestroy(g);
+    }
     for (hipEvent_t e : L->lev)
         if (e) (void)hipEventDestroy(e);
-    if (L->lexec_u) (void)hipGraphExecDestroy(L->lexec_u);
-    if (L->lgraph_u) (void)hipGraphDestroy(L->lgraph_u);
     gg_mlp_destroy(L->mlp);
     void *dev[] = {L->d_moves, L->d_x, L->d_score, L->d_sel, L->d_bx,   L->d_by,   L->d_idx,
                    L->d_psw,   L->d_msel, L->d_ctl, L->d_rank, L->d_loop, L->d_csel, L->d_gate,
@@ -875,14 +879,18 @@ static int loop_enqueue(gg_search *s, bool first, bool next) {
     return batch_enqueue_commit_dev(b, L->d_csel, next ? &pn : nullptr);
 }
 
-// Captures `steps` consecutive steps into one executable graph.
-static int loop_capture_n(gg_search *s, int steps, hipGraph_t *gr, hipGraphExec_t *ex) {
+// Captures `steps` consecutive steps into one executable graph (first: the
+// first step draws its own proposals; draws: the last step's commit draws
+// the next step's).
+static int loop_capture_n(gg_search *s, int steps, bool first, bool draws, hipGraph_t *gr,
+                          hipGraphExec_t *ex) {
     LearnState *L = s->learn;
     if (*ex) { (void)hipGraphExecDestroy(*ex); *ex = nullptr; }
     if (*gr) { (void)hipGraphDestroy(*gr); *gr = nullptr; }
     GG_HIP(hipStreamBeginCapture(L->st, hipStreamCaptureModeThreadLocal));
     int st = GG_OK;
-    for (int i = 0; i < steps && st == GG_OK; i++) st = loop_enqueue(s, i == 0, i + 1 < steps);
+    for (int i = 0; i < steps && st == GG_OK; i++)
+        st = loop_enqueue(s, first && i == 0, i + 1 < steps || draws);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(L->st, &g);
     if (st != GG_OK) {
@@ -895,12 +903,27 @@ static int loop_capture_n(gg_search *s, int steps, hipGraph_t *gr, hipGraphExec_
     return GG_OK;
 }
 
-static int loop_capture(gg_search *s) {
+// The launch variant [unrolled][first][draws], captured the first time it is
+// needed after the batch evaluator's state last changed (which drops them all).
+static int loop_graph(gg_search *s, int u, bool first, bool draws, hipGraphExec_t *out) {
     LearnState *L = s->learn;
-    int st = loop_capture_n(s, 1, &L->lgraph, &L->lexec);
-    if (st == GG_OK) st = loop_capture_n(s, GG_LOOP_UNROLL, &L->lgraph_u, &L->lexec_u);
-    if (st != GG_OK) return st;
-    L->l_push = batch_push_state(L->batch);
+    const int push = batch_push_state(L->batch);
+    if (L->l_push != push) {
+        for (int i = 0; i < 8; i++) {
+            hipGraphExec_t &x = L->lexec[i >> 2][(i >> 1) & 1][i & 1];
+            hipGraph_t &g = L->lgraph[i >> 2][(i >> 1) & 1][i & 1];
+            if (x) { (void)hipGraphExecDestroy(x); x = nullptr; }
+            if (g) { (void)hipGraphDestroy(g); g = nullptr; }
+        }
+        L->l_push = push;
+    }
+    const int ui = u > 1 ? 1 : 0;
+    hipGraphExec_t &x = L->lexec[ui][first][draws];
+    if (!x) {
+        int st = loop_capture_n(s, u, first, draws, &L->lgraph[ui][first][draws], &x);
+        if (st != GG_OK) return st;
+    }
+    *out = x;
     return GG_OK;
 }
 
@@ -919,10 +942,17 @@ int learn_loop_run(gg_search *s, int steps, int *done) {
     // graph replay (for tools that do not follow graph launches)
     const char *lg = getenv("GG_LOOP_GRAPH");
     const bool use_graph = !(lg && lg[0] == '0');
-    if (use_graph && (!L->lexec || L->l_push != batch_push_state(b))) {
-        int st = loop_capture(s);
-        if (st != GG_OK) return st;
-    }
+    const int n = std::min(steps, GG_LOOP_REC_CAP);
+    // the chunk's launch variants, captured (on the idle stream) before any
+    // work of the chunk is queued
+    if (use_graph)
+        for (int i = 0; i < n;) {
+            const int u = n - i >= GG_LOOP_UNROLL ? GG_LOOP_UNROLL : 1;
+            hipGraphExec_t x = nullptr;
+            int st = loop_graph(s, u, i == 0, i + u < n, &x);
+            if (st != GG_OK) return st;
+            i += u;
+        }
     LoopCtl &h = *L->h_loop;
     memset(&h, 0, sizeof(h));
     h.step = s->step;
@@ -937,7 +967,6 @@ int learn_loop_run(gg_search *s, int steps, int *done) {
     h.head = L->head;
     h.count = L->count;
     h.adam_t = L->mlp->t;
-    const int n = std::min(steps, GG_LOOP_REC_CAP);
     GG_HIP(hipMemcpyAsync(L->d_loop, L->h_loop, sizeof(LoopCtl), hipMemcpyHostToDevice, L->st));
     // both bf16 copies of the weights from the masters (the loop alternates
     // them), and no training step pending for the first step to carry
@@ -948,12 +977,18 @@ int learn_loop_run(gg_search *s, int steps, int *done) {
     for (int i = 0, j = 0; i < n; j++) {
         hipEvent_t &e = L->lev[j % GG_LOOP_INFLIGHT];
         if (j >= GG_LOOP_INFLIGHT) GG_HIP(hipEventSynchronize(e));
+        // every step's commit draws the next step's proposals, but the
+        // chunk's last (its draws, scores and choice stay readable)
         const int u = n - i >= GG_LOOP_UNROLL ? GG_LOOP_UNROLL : 1;
+        const bool first = i == 0, draws = i + u < n;
         if (use_graph) {
-            GG_HIP(hipGraphLaunch(u > 1 ? L->lexec_u : L->lexec, L->st));
+            hipGraphExec_t x = nullptr;
+            int st = loop_graph(s, u, first, draws, &x);
+            if (st != GG_OK) return st;
+            GG_HIP(hipGraphLaunch(x, L->st));
         } else {
             for (int q = 0; q < u; q++) {
-                int st = loop_enqueue(s, q == 0, q + 1 < u);
+                int st = loop_enqueue(s, first && q == 0, q + 1 < u || draws);
                 if (st != GG_OK) return st;
             }
         }
